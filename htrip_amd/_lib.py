@@ -1,0 +1,293 @@
+"""ctypes binding of libhtrip_gpu.so (include/htrip_gpu.h).
+
+The library is loaded from this package directory only; there is no CPU
+fallback: if it is missing or the device is absent, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhtrip_gpu.so")
+INF = 60000
+
+HTG_OK, HTG_E_INVALID, HTG_E_NOMEM, HTG_E_HIP, HTG_E_STATE, HTG_E_RANGE = 0, -1, -2, -3, -4, -5
+
+# Every symbol include/htrip_gpu.h declares (checked by tests/test_abi.py).
+EXPORTS = [
+    "htg_ctx_create", "htg_ctx_destroy", "htg_last_error", "htg_ctx_mem_info",
+    "htg_map_create", "htg_map_destroy", "htg_map_clone", "htg_map_footprint", "htg_map_info_get",
+    "htg_build", "htg_open_cells", "htg_open_cell", "htg_query_all_pairs", "htg_query_batch",
+    "htg_last_batch_stats", "htg_sync",
+    "htg_read_leaf_D", "htg_read_leaf_S", "htg_node_dims", "htg_read_node_DU", "htg_read_node_SU",
+    "htg_get_tuning", "htg_set_tuning",
+]
+
+
+class HtgError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"htg error {code}: {msg}")
+        self.code = code
+
+
+class Config(C.Structure):
+    _fields_ = [("tile_size_b", C.c_int32), ("group_w", C.c_int32), ("group_h", C.c_int32),
+                ("hybrid_k_threshold", C.c_int32)]
+
+
+class MapInfo(C.Structure):
+    _fields_ = [("rows", C.c_int32), ("cols", C.c_int32), ("S", C.c_int32), ("H", C.c_int32),
+                ("num_leaf_tiles_x", C.c_int32), ("num_leaves", C.c_int32), ("built", C.c_int32),
+                ("reserved", C.c_int32), ("total_memory_bytes", C.c_uint64), ("device_bytes", C.c_uint64)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("n_touched_leaves", C.c_int32), ("n_dirty_ancestors", C.c_int32), ("n_pivots", C.c_int32)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("pivot_block", C.c_int32), ("zq_tile", C.c_int32), ("use_graph", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load the library (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: run `python -m htrip_amd.build` (or __graft_entry__.build())")
+    lib = C.CDLL(path)
+    vp, i32, u64 = C.c_void_p, C.c_int32, C.c_uint64
+    pi32, pu16, pu8 = C.POINTER(C.c_int32), C.POINTER(C.c_uint16), C.POINTER(C.c_uint8)
+    sig = {
+        "htg_ctx_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
+        "htg_ctx_destroy": (C.c_int, [vp]),
+        "htg_last_error": (C.c_char_p, []),
+        "htg_ctx_mem_info": (C.c_int, [vp, C.POINTER(u64), C.POINTER(u64)]),
+        "htg_map_create": (C.c_int, [vp, C.POINTER(Config), i32, i32, pu8, u64, C.POINTER(vp)]),
+        "htg_map_destroy": (C.c_int, [vp]),
+        "htg_map_clone": (C.c_int, [vp, u64, C.POINTER(vp)]),
+        "htg_map_footprint": (C.c_int, [C.POINTER(Config), i32, i32, C.POINTER(u64)]),
+        "htg_map_info_get": (C.c_int, [vp, C.POINTER(MapInfo)]),
+        "htg_build": (C.c_int, [vp]),
+        "htg_open_cells": (C.c_int, [vp, pi32, i32]),
+        "htg_open_cell": (C.c_int, [vp, i32, i32]),
+        "htg_query_all_pairs": (C.c_int, [vp, pi32, i32, pu16]),
+        "htg_query_batch": (C.c_int, [vp, pi32, pi32, i32, pu16]),
+        "htg_last_batch_stats": (C.c_int, [vp, C.POINTER(BatchStats)]),
+        "htg_sync": (C.c_int, [vp]),
+        "htg_read_leaf_D": (C.c_int, [vp, i32, pu16]),
+        "htg_read_leaf_S": (C.c_int, [vp, i32, pu16]),
+        "htg_node_dims": (C.c_int, [vp, i32, pi32, pi32, pi32]),
+        "htg_read_node_DU": (C.c_int, [vp, i32, i32, pu16]),
+        "htg_read_node_SU": (C.c_int, [vp, i32, i32, pu16]),
+        "htg_get_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
+        "htg_set_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int) -> None:
+    if rc != HTG_OK:
+        msg = _lib.htg_last_error().decode() if _lib is not None else ""
+        raise HtgError(rc, msg)
+
+
+def ptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+class Context:
+    """One device (htg_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        self.h = C.c_void_p()
+        check(self.lib.htg_ctx_create(device, C.byref(self.h)))
+
+    def mem_info(self):
+        f, t = C.c_uint64(), C.c_uint64()
+        check(self.lib.htg_ctx_mem_info(self.h, C.byref(f), C.byref(t)))
+        return f.value, t.value
+
+    def close(self):
+        if self.h:
+            self.lib.htg_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+
+_default_ctx = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+def footprint(b: int, g: int, rows: int, cols: int) -> int:
+    lib = load()
+    cfg = Config(b, g, g, 0)
+    out = C.c_uint64()
+    check(lib.htg_map_footprint(C.byref(cfg), rows, cols, C.byref(out)))
+    return out.value
+
+
+class GpuMap:
+    """One resident hierarchy (htg_map): the GPU MinPlusOracle over a cell array.
+
+    Method names mirror oracle/oracle.py so parity tests read the same against
+    either side.
+    """
+
+    def __init__(self, cells: np.ndarray, b: int, g: int = 2, ctx: Context | None = None, budget: int = 0,
+                 _handle=None):
+        self.ctx = ctx or default_context()
+        self.lib = self.ctx.lib
+        self.b, self.g = b, g
+        self.cells = np.ascontiguousarray(cells, dtype=np.uint8).copy()
+        if _handle is not None:
+            self.h = _handle
+            return
+        self.h = C.c_void_p()
+        cfg = Config(b, g, g, 0)
+        check(self.lib.htg_map_create(self.ctx.h, C.byref(cfg), self.cells.shape[0], self.cells.shape[1],
+                                      ptr(self.cells, C.c_uint8), budget, C.byref(self.h)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.htg_map_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def clone(self, budget: int = 0) -> "GpuMap":
+        h = C.c_void_p()
+        check(self.lib.htg_map_clone(self.h, budget, C.byref(h)))
+        return GpuMap(self.cells, self.b, self.g, self.ctx, _handle=h)
+
+    def build(self) -> bool:
+        rc = self.lib.htg_build(self.h)
+        if rc == HTG_E_INVALID:
+            return False
+        check(rc)
+        return True
+
+    def open_cells(self, rc) -> None:
+        rc = np.ascontiguousarray(rc, dtype=np.int32).reshape(-1, 2)
+        for r, c in rc:
+            if 0 <= r < self.cells.shape[0] and 0 <= c < self.cells.shape[1]:
+                self.cells[r, c] = 1
+        check(self.lib.htg_open_cells(self.h, ptr(rc, C.c_int32), len(rc)))
+
+    def open_cell(self, r: int, c: int) -> None:
+        if 0 <= r < self.cells.shape[0] and 0 <= c < self.cells.shape[1]:
+            self.cells[r, c] = 1
+        check(self.lib.htg_open_cell(self.h, r, c))
+
+    def query(self, vps) -> np.ndarray:
+        vps = np.ascontiguousarray(vps, dtype=np.int32).reshape(-1, 2)
+        k = len(vps)
+        out = np.empty((k, k), dtype=np.uint16)
+        check(self.lib.htg_query_all_pairs(self.h, ptr(vps, C.c_int32), k, ptr(out, C.c_uint16)))
+        return out
+
+    def query_batch(self, vps_list) -> list:
+        ks = np.array([len(v) for v in vps_list], dtype=np.int32)
+        allv = np.ascontiguousarray(np.concatenate([np.asarray(v, np.int32).reshape(-1, 2) for v in vps_list])
+                                    if len(vps_list) else np.zeros((0, 2), np.int32), dtype=np.int32)
+        out = np.empty(int((ks.astype(np.int64) ** 2).sum()), dtype=np.uint16)
+        check(self.lib.htg_query_batch(self.h, ptr(allv, C.c_int32), ptr(ks, C.c_int32), len(ks),
+                                       ptr(out, C.c_uint16)))
+        res, o = [], 0
+        for k in ks:
+            res.append(out[o:o + k * k].reshape(k, k))
+            o += int(k) * int(k)
+        return res
+
+    def info(self) -> MapInfo:
+        i = MapInfo()
+        check(self.lib.htg_map_info_get(self.h, C.byref(i)))
+        return i
+
+    @property
+    def S(self) -> int:
+        return self.info().S
+
+    @property
+    def H(self) -> int:
+        return self.info().H
+
+    def num_leaves(self) -> int:
+        return self.info().num_leaves
+
+    def node_dims(self, lvl: int):
+        n, v, e = C.c_int32(), C.c_int32(), C.c_int32()
+        check(self.lib.htg_node_dims(self.h, lvl, C.byref(n), C.byref(v), C.byref(e)))
+        return n.value, v.value, e.value
+
+    def num_nodes(self, lvl: int) -> int:
+        return self.node_dims(lvl)[0]
+
+    def leaf_D(self, idx: int) -> np.ndarray:
+        n = self.b * self.b
+        out = np.empty((n, n), dtype=np.uint16)
+        check(self.lib.htg_read_leaf_D(self.h, idx, ptr(out, C.c_uint16)))
+        return out
+
+    def leaf_S(self, idx: int) -> np.ndarray:
+        p = 4 * self.b - 4
+        out = np.empty((p, p), dtype=np.uint16)
+        check(self.lib.htg_read_leaf_S(self.h, idx, ptr(out, C.c_uint16)))
+        return out
+
+    def node_DU(self, lvl: int, idx: int) -> np.ndarray:
+        _, nv, _ = self.node_dims(lvl)
+        out = np.empty((nv, nv), dtype=np.uint16)
+        check(self.lib.htg_read_node_DU(self.h, lvl, idx, ptr(out, C.c_uint16)))
+        return out
+
+    def node_SU(self, lvl: int, idx: int) -> np.ndarray:
+        _, _, ne = self.node_dims(lvl)
+        out = np.empty((ne, ne), dtype=np.uint16)
+        check(self.lib.htg_read_node_SU(self.h, lvl, idx, ptr(out, C.c_uint16)))
+        return out
+
+    def last_batch_stats(self):
+        s = BatchStats()
+        check(self.lib.htg_last_batch_stats(self.h, C.byref(s)))
+        return (s.n_touched_leaves, s.n_dirty_ancestors, s.n_pivots)
+
+    def total_memory_bytes(self) -> int:
+        return int(self.info().total_memory_bytes)
+
+    def tuning(self) -> Tuning:
+        t = Tuning()
+        check(self.lib.htg_get_tuning(self.h, C.byref(t)))
+        return t
+
+    def set_tuning(self, **kw) -> None:
+        t = self.tuning()
+        for k, v in kw.items():
+            setattr(t, k, v)
+        check(self.lib.htg_set_tuning(self.h, C.byref(t)))
+
+    def sync(self) -> None:
+        check(self.lib.htg_sync(self.h))

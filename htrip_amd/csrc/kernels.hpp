@@ -1,0 +1,94 @@
+// Launch wrappers for the gfx950 kernels (kernels_build.hip, kernels_query.hip).
+// Each wrapper validates nothing: the runtime checks shapes before calling.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace htg {
+
+// ---- build / update ------------------------------------------------------
+// Tile-restricted BFS from every cell of each listed leaf; writes the leaf's
+// D (b^2 x b^2) and port summary S ((4b-4)^2).  When changed != nullptr,
+// changed[i] is set to 1 if listed leaf i's S decreased anywhere.
+void launch_leaf_apsp(const uint8_t* pass, int S, int b, int nl, const int* leaf_list, int n_list, uint16_t* leafD,
+                      uint16_t* leafS, int* changed, hipStream_t st);
+
+// D_U of each listed node := block-diagonal child summaries, INF elsewhere.
+// Child summaries come from leafS (lvl 1, stride (4b-4)^2) or the previous
+// level's S_U.
+void launch_node_assemble(int lvl, int g, int pc, int nV, int ld, int nt, const uint16_t* child_sum,
+                          size_t child_stride, const int* node_list, int n_list, uint16_t* DU, size_t node_stride,
+                          hipStream_t st);
+
+// Seam edges: for each (u, v) pair of the level's seam template (relative
+// cell coordinates in vu_rc) whose cells are both passable, D_U[u][v] and
+// D_U[v][u] := min(., 1).  flags (per node, nV bytes) marks rows that
+// decreased (nullable).
+void launch_seam_relax(int nV, int ld, int s, int nt, const int2* seams, int n_seams, const short2* vu_rc,
+                       const uint8_t* pass, int S, const int* node_list, int n_list, uint16_t* DU, size_t node_stride,
+                       uint8_t* flags, hipStream_t st);
+
+// Explicit seam edges (absolute V_U pairs per node, from the update path).
+void launch_edge_relax(int nV, int ld, const int4* edges /* node slot, u, v, w */, int n_edges,
+                       const int* node_list, uint16_t* DU, size_t node_stride, uint8_t* flags, hipStream_t st);
+
+// D_U child blocks := min(D_U block, child summary) for children whose mask bit
+// is set; flags marks rows that decreased.
+void launch_child_relax(int lvl, int g, int pc, int nV, int ld, int nt, const uint16_t* child_sum,
+                        size_t child_stride, const int* node_list, const uint32_t* child_mask, int n_list,
+                        uint16_t* DU, size_t node_stride, uint8_t* flags, hipStream_t st);
+
+// Flags -> pivot list per node (ascending V_U order); counts[i] = |K_i|;
+// flags are cleared.
+void launch_compact_flags(int nV, const int* node_list, int n_list, uint8_t* flags, int* pivots, int piv_stride,
+                          int* counts, hipStream_t st);
+
+// One pivot block of the closure: for every listed node with more than t*P
+// pivots, R := the node's block pivot rows after Floyd-Warshall over the
+// block's pivots (P x nV, into Rbuf slot i).
+void launch_closure_pivots(int P, int nV, int ld, const int* node_list, int n_list, const int* pivots, int piv_stride,
+                           const int* counts, int t, const uint16_t* DU, size_t node_stride, uint16_t* Rbuf,
+                           hipStream_t st);
+// D_U := min(D_U, R^T (x) R) over the block (symmetric rank-P min-plus update).
+void launch_closure_update(int P, int nV, int ld, const int* node_list, int n_list, const int* counts, int t,
+                           uint16_t* DU, size_t node_stride, const uint16_t* Rbuf, hipStream_t st);
+
+// S_U := D_U[ext][ext]; changed[i] = 1 if listed node i's S_U decreased.
+void launch_project_su(int nV, int ld, int ne, const int* ext_idx, const int* node_list, int n_list,
+                       const uint16_t* DU, size_t node_stride, uint16_t* SU, int* changed, hipStream_t st);
+
+// pass[r*S + c] = 1 for the listed cells.
+void launch_set_pass(uint8_t* pass, int S, const int2* cells, int n, hipStream_t st);
+
+// ---- query -----------------------------------------------------------------
+struct QuerySlots {           // one entry per valid viewpoint, sorted by tile key
+    const int2* rc;           // absolute (r, c)
+    int n;
+};
+
+// out matrices: INF, 0 on the diagonal of valid viewpoints, leaf D for
+// same-leaf pairs.  vp_* arrays are per viewpoint (all queries packed).
+void launch_q_init(int nq, const int* q_k, const long long* q_out, const int* q_vp, const int2* vp_rc,
+                   const uint8_t* vp_valid, int b, int nl, const uint16_t* leafD, uint16_t* out, int kmax,
+                   hipStream_t st);
+// X0[slot][p] = leaf D row of the slot's cell at port p.
+void launch_q_leaf(int n, const int2* slot_rc, int b, int nl, const uint16_t* leafD, uint16_t* X0, hipStream_t st);
+// Z[slot][x] = min_{a in seam(q)} X[slot][a] (+) D_W[q*pc + a][col(x)] for work
+// items (node W, quad q, first slot, count).
+void launch_q_z(int M, int nV, int ld, int pc, const uint16_t* DU, size_t node_stride, const int* seam_list,
+                const int* seam_off, const int* cols, int ncols, int ldz, bool gather, const int4* items,
+                int n_items, const uint16_t* X, uint16_t* Z, hipStream_t st);
+// Meet: for row items (group, row position) update out[i][j] for all j in the
+// group: min_{s in seam(q_j)} Z[i][zpos_q_j[s]] (+) X[j][seam_q_j[s]].
+void launch_q_meet(int pc, int cs, int g, const int* seam_list, const int* seam_off, const int* zpos, int ldz,
+                   const int2* row_items, int n_rows, const int4* groups, const int* glist, const int2* slot_rc,
+                   const int* slot_q, const int* slot_i, const int* q_k, const long long* q_out, const uint16_t* X,
+                   const uint16_t* Z, uint16_t* out, hipStream_t st);
+// Lift: X_l[slot][e] = min(Z[slot][colpos(ext_idx[e])], X_{l-1}[slot][own port of e]).
+void launch_q_lift(int n, int ne, const int* ext_zcol, const int* ext_child, const int* ext_cport, int pc, int cs,
+                   int g, int ldz, const int2* slot_rc, const uint16_t* Xprev, const uint16_t* Z, uint16_t* Xnext,
+                   hipStream_t st);
+
+}  // namespace htg

@@ -1,0 +1,497 @@
+// gfx950 kernels for index construction and incremental update.
+//
+// Arithmetic is the reference's saturating u16 tropical semiring
+// (include/htrip/types.hpp:66-70), evaluated two lanes at a time with
+// v_pk_add_u16 (clamp) + v_pk_min_u16.  Every stored value is <= INF = 60000,
+// so min(x, sat16(a + b)) == min(x, add_dist(a, b)) exactly.
+#include <hip/hip_runtime.h>
+
+#include "geometry.hpp"
+#include "kernels.hpp"
+
+namespace htg {
+
+typedef unsigned short us2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t relax2(uint32_t acc, uint32_t a, uint32_t b) {
+    us2 x = __builtin_bit_cast(us2, a), y = __builtin_bit_cast(us2, b), z = __builtin_bit_cast(us2, acc);
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_min(z, __builtin_elementwise_add_sat(x, y)));
+}
+__device__ __forceinline__ uint32_t dup16(uint32_t v) { return (v & 0xffffu) | (v << 16); }
+__device__ __forceinline__ uint16_t add1(uint16_t a, uint16_t b) {
+    uint32_t s = uint32_t(a) + b;
+    return s >= kInf ? kInf : uint16_t(s);
+}
+
+// ---------------------------------------------------------------------------
+// Leaf all-pairs BFS (src/leaf_tile.cpp:56-153 + projection :198-207).
+// One wavefront runs 64/LANES BFS sources at once; lane r of a source group
+// owns tile row r as a bitmask and a BFS step is a shift/or of neighbour rows
+// (bit-parallel frontier expansion).  Distances land in LDS and are streamed
+// out as whole D rows (coalesced); port sources also write their S row.
+template <int LANES, typename RowT>
+__global__ __launch_bounds__(256) void k_leaf_apsp(const uint8_t* __restrict__ pass, int S, int b, int nl,
+                                                   const int* __restrict__ leaf_list, int chunks,
+                                                   uint16_t* __restrict__ leafD, uint16_t* __restrict__ leafS,
+                                                   int* __restrict__ changed) {
+    constexpr int SPW = 64 / LANES;
+    constexpr int SWG = 4 * SPW;
+    extern __shared__ uint16_t dist[];  // [SWG][b*b]
+    const int li = blockIdx.x / chunks, chunk = blockIdx.x % chunks;
+    const int leaf = leaf_list ? leaf_list[li] : li;
+    const int n = b * b;
+    const int R0 = (leaf / nl) * b, C0 = (leaf % nl) * b;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int grp = lane / LANES, r = lane % LANES;
+    const int slot = wave * SPW + grp;
+    const int src = chunk * SWG + slot;
+
+    uint32_t* d32 = reinterpret_cast<uint32_t*>(dist);
+    for (int i = tid; i < SWG * n / 2; i += 256) d32[i] = kInf2;
+    RowT prow = 0;
+    if (r < b) {
+        const uint8_t* p = pass + size_t(R0 + r) * S + C0;
+        for (int c = 0; c < b; ++c) prow |= RowT(p[c] != 0) << c;
+    }
+    __syncthreads();
+
+    const bool valid = src < n;
+    const int sr = valid ? src / b : 0, sc = valid ? src % b : 0;
+    const RowT srow = __shfl(prow, sr, LANES);
+    RowT front = (valid && r == sr && ((srow >> sc) & 1)) ? (RowT(1) << sc) : RowT(0);
+    RowT vis = front;
+    if (valid && r == 0) dist[slot * n + src] = 0;  // D[i][i] = 0 for every cell
+    uint16_t d = 0;
+    while (__ballot(front != 0)) {
+        ++d;
+        RowT up = __shfl_up(front, 1, LANES);
+        RowT dn = __shfl_down(front, 1, LANES);
+        if (r == 0) up = 0;
+        if (r == LANES - 1) dn = 0;
+        RowT nb = ((front << 1) | (front >> 1) | up | dn) & prow & ~vis;
+        vis |= nb;
+        front = nb;
+        uint16_t* drow = dist + slot * n + r * b;
+        while (nb) {
+            int c = sizeof(RowT) == 8 ? __builtin_ctzll(uint64_t(nb)) : __builtin_ctz(uint32_t(nb));
+            drow[c] = d;
+            nb &= nb - 1;
+        }
+    }
+    __syncthreads();
+
+    const int first = chunk * SWG;
+    const int nvalid = min(SWG, n - first);
+    uint16_t* gD = leafD + size_t(leaf) * n * n + size_t(first) * n;
+    if ((n & 7) == 0) {
+        const uint4* s4 = reinterpret_cast<const uint4*>(dist);
+        uint4* g4 = reinterpret_cast<uint4*>(gD);
+        for (int i = tid; i < nvalid * n / 8; i += 256) g4[i] = s4[i];
+    } else {
+        for (int i = tid; i < nvalid * n; i += 256) gD[i] = dist[i];
+    }
+    const int pl = 4 * b - 4;
+    uint16_t* gS = leafS + size_t(leaf) * pl * pl;
+    for (int s = 0; s < nvalid; ++s) {
+        const int cell = first + s;
+        const int pi = port_index(cell / b, cell % b, b);
+        if (pi < 0) continue;
+        for (int pj = tid; pj < pl; pj += 256) {
+            int pr, pc;
+            port_rc(pj, b, pr, pc);
+            const uint16_t v = dist[s * n + pr * b + pc];
+            uint16_t* dst = gS + size_t(pi) * pl + pj;
+            if (changed && v < *dst) changed[li] = 1;
+            *dst = v;
+        }
+    }
+}
+
+void launch_leaf_apsp(const uint8_t* pass, int S, int b, int nl, const int* leaf_list, int n_list, uint16_t* leafD,
+                      uint16_t* leafS, int* changed, hipStream_t st) {
+    const int n = b * b;
+    auto go = [&](auto kern, int lanes) {
+        const int swg = 4 * (64 / lanes);
+        const int chunks = (n + swg - 1) / swg;
+        const size_t lds = size_t(swg) * n * sizeof(uint16_t);
+        hipLaunchKernelGGL(kern, dim3(n_list * chunks), dim3(256), lds, st, pass, S, b, nl, leaf_list, chunks, leafD,
+                           leafS, changed);
+    };
+    if (b <= 2) go(k_leaf_apsp<2, uint32_t>, 2);
+    else if (b <= 4) go(k_leaf_apsp<4, uint32_t>, 4);
+    else if (b <= 8) go(k_leaf_apsp<8, uint32_t>, 8);
+    else if (b <= 16) go(k_leaf_apsp<16, uint32_t>, 16);
+    else if (b <= 32) go(k_leaf_apsp<32, uint32_t>, 32);
+    else go(k_leaf_apsp<64, uint64_t>, 64);
+}
+
+// ---------------------------------------------------------------------------
+// D_U assembly (src/internal_node.cpp:98-145): identity + child summaries on
+// the diagonal blocks.  A child summary with no finite off-diagonal entry is
+// the identity, so copying every child unconditionally gives the same D_U.
+__global__ __launch_bounds__(256) void k_node_assemble(int g, int pc, int nV, int ld, int nt, int child_nt,
+                                                       const uint16_t* __restrict__ child_sum, size_t child_stride,
+                                                       const int* __restrict__ node_list,
+                                                       uint16_t* __restrict__ DU, size_t node_stride) {
+    const int groups_per_row = ld / 8;
+    const long long gid = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (gid >= (long long)nV * groups_per_row) return;
+    const int v = int(gid / groups_per_row), u0 = int(gid % groups_per_row) * 8;
+    const int node = node_list[blockIdx.y];
+    const int tx = node % nt, ty = node / nt;
+    const int qv = v / pc, pv = v % pc;
+    const int cidx = (ty * g + qv / g) * child_nt + (tx * g + qv % g);
+    const uint16_t* src = child_sum + size_t(cidx) * child_stride + size_t(pv) * pc;
+    uint16_t o[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int u = u0 + e;
+        o[e] = (u < nV && u / pc == qv) ? src[u % pc] : kInf;
+    }
+    uint4 w;
+    w.x = o[0] | (uint32_t(o[1]) << 16);
+    w.y = o[2] | (uint32_t(o[3]) << 16);
+    w.z = o[4] | (uint32_t(o[5]) << 16);
+    w.w = o[6] | (uint32_t(o[7]) << 16);
+    *reinterpret_cast<uint4*>(DU + size_t(node) * node_stride + size_t(v) * ld + u0) = w;
+}
+
+void launch_node_assemble(int lvl, int g, int pc, int nV, int ld, int nt, const uint16_t* child_sum,
+                          size_t child_stride, const int* node_list, int n_list, uint16_t* DU, size_t node_stride,
+                          hipStream_t st) {
+    (void)lvl;
+    const long long total = (long long)nV * (ld / 8);
+    dim3 grid(unsigned((total + 255) / 256), unsigned(n_list));
+    hipLaunchKernelGGL(k_node_assemble, grid, dim3(256), 0, st, g, pc, nV, ld, nt, nt * g, child_sum, child_stride,
+                       node_list, DU, node_stride);
+}
+
+// ---------------------------------------------------------------------------
+// Seam edges between sibling children (src/internal_node.cpp:147-173).
+__global__ __launch_bounds__(256) void k_seam_relax(int nV, int ld, int s, int nt, const int2* __restrict__ seams,
+                                                    int n_seams, const short2* __restrict__ vu_rc,
+                                                    const uint8_t* __restrict__ pass, int S,
+                                                    const int* __restrict__ node_list, uint16_t* __restrict__ DU,
+                                                    size_t node_stride, uint8_t* __restrict__ flags) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n_seams) return;
+    const int node = node_list[blockIdx.y];
+    const int R0 = (node / nt) * s, C0 = (node % nt) * s;
+    const int2 e = seams[i];
+    const short2 a = vu_rc[e.x], c = vu_rc[e.y];
+    if (!pass[size_t(R0 + a.x) * S + C0 + a.y] || !pass[size_t(R0 + c.x) * S + C0 + c.y]) return;
+    uint16_t* D = DU + size_t(node) * node_stride;
+    uint16_t& uv = D[size_t(e.x) * ld + e.y];
+    uint16_t& vu = D[size_t(e.y) * ld + e.x];
+    if (uv > 1 || vu > 1) {
+        uv = uv > 1 ? 1 : uv;
+        vu = vu > 1 ? 1 : vu;
+        if (flags) {
+            flags[size_t(blockIdx.y) * nV + e.x] = 1;
+            flags[size_t(blockIdx.y) * nV + e.y] = 1;
+        }
+    }
+}
+
+void launch_seam_relax(int nV, int ld, int s, int nt, const int2* seams, int n_seams, const short2* vu_rc,
+                       const uint8_t* pass, int S, const int* node_list, int n_list, uint16_t* DU, size_t node_stride,
+                       uint8_t* flags, hipStream_t st) {
+    if (n_seams == 0 || n_list == 0) return;
+    dim3 grid(unsigned((n_seams + 255) / 256), unsigned(n_list));
+    hipLaunchKernelGGL(k_seam_relax, grid, dim3(256), 0, st, nV, ld, s, nt, seams, n_seams, vu_rc, pass, S, node_list,
+                       DU, node_stride, flags);
+}
+
+// Explicit edges (node slot, u, v, w): D_U[u][v] = D_U[v][u] = min(., w)
+// (src/internal_node.cpp:356-365).  The update path lists each pair once.
+__global__ __launch_bounds__(256) void k_edge_relax(int nV, int ld, const int4* __restrict__ edges, int n_edges,
+                                                    const int* __restrict__ node_list, uint16_t* __restrict__ DU,
+                                                    size_t node_stride, uint8_t* __restrict__ flags) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n_edges) return;
+    const int4 e = edges[i];
+    uint16_t* D = DU + size_t(node_list[e.x]) * node_stride;
+    const uint16_t w = uint16_t(e.w);
+    uint16_t& uv = D[size_t(e.y) * ld + e.z];
+    if (w < uv) {
+        uv = w;
+        D[size_t(e.z) * ld + e.y] = w;
+        flags[size_t(e.x) * nV + e.y] = 1;
+        flags[size_t(e.x) * nV + e.z] = 1;
+    }
+}
+
+void launch_edge_relax(int nV, int ld, const int4* edges, int n_edges, const int* node_list, uint16_t* DU,
+                       size_t node_stride, uint8_t* flags, hipStream_t st) {
+    if (n_edges == 0) return;
+    hipLaunchKernelGGL(k_edge_relax, dim3(unsigned((n_edges + 255) / 256)), dim3(256), 0, st, nV, ld, edges, n_edges,
+                       node_list, DU, node_stride, flags);
+}
+
+// Child decreases (src/internal_node.cpp:343-354) expressed as an elementwise
+// min of the child's new summary into its diagonal block: entries that did not
+// decrease are already <= the summary, so only decreased entries move.
+__global__ __launch_bounds__(256) void k_child_relax(int g, int pc, int nV, int ld, int nt, int child_nt,
+                                                     const uint16_t* __restrict__ child_sum, size_t child_stride,
+                                                     const int* __restrict__ node_list,
+                                                     const uint32_t* __restrict__ child_mask,
+                                                     uint16_t* __restrict__ DU, size_t node_stride,
+                                                     uint8_t* __restrict__ flags) {
+    const long long per_child = (long long)pc * pc;
+    const long long gid = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (gid >= per_child * g * g) return;
+    const int q = int(gid / per_child);
+    if (!((child_mask[blockIdx.y] >> q) & 1u)) return;
+    const int a = int((gid % per_child) / pc), c = int(gid % pc);
+    const int node = node_list[blockIdx.y];
+    const int tx = node % nt, ty = node / nt;
+    const int cidx = (ty * g + q / g) * child_nt + (tx * g + q % g);
+    const uint16_t w = child_sum[size_t(cidx) * child_stride + size_t(a) * pc + c];
+    uint16_t& dst = DU[size_t(node) * node_stride + size_t(q * pc + a) * ld + q * pc + c];
+    if (w < dst) {
+        dst = w;
+        flags[size_t(blockIdx.y) * nV + q * pc + a] = 1;
+    }
+}
+
+void launch_child_relax(int lvl, int g, int pc, int nV, int ld, int nt, const uint16_t* child_sum,
+                        size_t child_stride, const int* node_list, const uint32_t* child_mask, int n_list,
+                        uint16_t* DU, size_t node_stride, uint8_t* flags, hipStream_t st) {
+    (void)lvl;
+    if (n_list == 0) return;
+    const long long total = (long long)pc * pc * g * g;
+    dim3 grid(unsigned((total + 255) / 256), unsigned(n_list));
+    hipLaunchKernelGGL(k_child_relax, grid, dim3(256), 0, st, g, pc, nV, ld, nt, nt * g, child_sum, child_stride,
+                       node_list, child_mask, DU, node_stride, flags);
+}
+
+// ---------------------------------------------------------------------------
+// Row flags -> ascending pivot list (one workgroup per node).
+__global__ __launch_bounds__(256) void k_compact_flags(int nV, uint8_t* __restrict__ flags, int* __restrict__ pivots,
+                                                       int piv_stride, int* __restrict__ counts) {
+    __shared__ int wsum[4];
+    uint8_t* f = flags + size_t(blockIdx.x) * nV;
+    int* out = pivots + size_t(blockIdx.x) * piv_stride;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int base = 0;
+    for (int c0 = 0; c0 < nV; c0 += 256) {
+        const int i = c0 + tid;
+        const bool on = i < nV && f[i] != 0;
+        const unsigned long long m = __ballot(on);
+        const int before = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) wsum[wave] = __popcll(m);
+        __syncthreads();
+        int off = base;
+        for (int w = 0; w < wave; ++w) off += wsum[w];
+        if (on) {
+            out[off + before] = i;
+            f[i] = 0;
+        }
+        base += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        __syncthreads();
+    }
+    if (tid == 0) counts[blockIdx.x] = base;
+}
+
+void launch_compact_flags(int nV, const int* node_list, int n_list, uint8_t* flags, int* pivots, int piv_stride,
+                          int* counts, hipStream_t st) {
+    (void)node_list;
+    if (n_list == 0) return;
+    hipLaunchKernelGGL(k_compact_flags, dim3(unsigned(n_list)), dim3(256), 0, st, nV, flags, pivots, piv_stride,
+                       counts);
+}
+
+// ---------------------------------------------------------------------------
+// Blocked Floyd-Warshall over a pivot subset, phase 1: the block's pivot rows.
+// With C* the closure of the block's P x P pivot submatrix,
+//   R[k][j] = min_{k'} C*[k][k'] (+) D[piv_k'][j]
+// are the pivot rows after FW over the block (src/internal_node.cpp:199-231
+// runs the same pivots one at a time).
+template <int P>
+__global__ __launch_bounds__(256) void k_closure_pivots(int nV, int ld, const int* __restrict__ node_list,
+                                                        const int* __restrict__ pivots, int piv_stride,
+                                                        const int* __restrict__ counts, int t,
+                                                        const uint16_t* __restrict__ DU, size_t node_stride,
+                                                        uint16_t* __restrict__ Rbuf) {
+    const int slot = blockIdx.y;
+    const int p = min(P, counts[slot] - t * P);
+    if (p <= 0) return;
+    __shared__ int kr[P];
+    __shared__ uint16_t C[P][P + 1];
+    __shared__ uint32_t CdT[P][P];  // CdT[k2][k] = dup(C*[k][k2])
+    const int tid = threadIdx.x;
+    const uint16_t* D = DU + size_t(node_list[slot]) * node_stride;
+    const int* K = pivots + size_t(slot) * piv_stride + t * P;
+    if (tid < P) kr[tid] = tid < p ? K[tid] : 0;
+    __syncthreads();
+    for (int i = tid; i < P * P; i += 256) {
+        const int a = i / P, c = i % P;
+        C[a][c] = (a < p && c < p) ? D[size_t(kr[a]) * ld + kr[c]] : (a == c ? 0 : kInf);
+    }
+    __syncthreads();
+    for (int k = 0; k < p; ++k) {
+        for (int i = tid; i < p * p; i += 256) {
+            const int a = i / p, c = i % p;
+            const uint16_t v = add1(C[a][k], C[k][c]);
+            if (v < C[a][c]) C[a][c] = v;
+        }
+        __syncthreads();
+    }
+    for (int i = tid; i < P * P; i += 256) {
+        const int k2 = i / P, k = i % P;
+        CdT[k2][k] = dup16(C[k][k2]);
+    }
+    __syncthreads();
+    const int j0 = (blockIdx.x * 256 + tid) * 2;
+    if (j0 >= nV) return;
+    uint32_t acc[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k) acc[k] = kInf2;
+    for (int k2 = 0; k2 < p; ++k2) {
+        const uint32_t dv = *reinterpret_cast<const uint32_t*>(D + size_t(kr[k2]) * ld + j0);
+        const uint4* c4 = reinterpret_cast<const uint4*>(&CdT[k2][0]);
+#pragma unroll
+        for (int k = 0; k < P; k += 4) {
+            const uint4 cv = c4[k / 4];
+            acc[k + 0] = relax2(acc[k + 0], cv.x, dv);
+            acc[k + 1] = relax2(acc[k + 1], cv.y, dv);
+            acc[k + 2] = relax2(acc[k + 2], cv.z, dv);
+            acc[k + 3] = relax2(acc[k + 3], cv.w, dv);
+        }
+    }
+    uint16_t* R = Rbuf + size_t(slot) * P * ld;
+#pragma unroll
+    for (int k = 0; k < P; ++k)
+        if (k < p) *reinterpret_cast<uint32_t*>(R + size_t(k) * ld + j0) = acc[k];
+}
+
+// Phase 2: D := min(D, R^T (x) R).  A 256-thread workgroup owns a 128 x 128
+// tile, 8 x 8 elements per thread held as 32 packed registers; R's row slices
+// for the tile sit in LDS (rows pre-duplicated into both u16 halves).
+template <int P>
+__global__ __launch_bounds__(256) void k_closure_update(int nV, int ld, const int* __restrict__ node_list,
+                                                        const int* __restrict__ counts, int t, int tiles,
+                                                        uint16_t* __restrict__ DU, size_t node_stride,
+                                                        const uint16_t* __restrict__ Rbuf) {
+    const int slot = blockIdx.y;
+    const int p = min(P, counts[slot] - t * P);
+    if (p <= 0) return;
+    __shared__ uint4 Rr4[P][32];  // dup'd R[k][i0 .. i0+127]
+    __shared__ uint4 Rc4[P][16];  // R[k][j0 .. j0+127]
+    const int I = blockIdx.x / tiles, J = blockIdx.x % tiles;
+    const int i0 = I * 128, j0 = J * 128;
+    const int tid = threadIdx.x;
+    const uint16_t* R = Rbuf + size_t(slot) * P * ld;
+    uint32_t* Rr = reinterpret_cast<uint32_t*>(&Rr4[0][0]);
+    uint16_t* Rc = reinterpret_cast<uint16_t*>(&Rc4[0][0]);
+    for (int i = tid; i < P * 128; i += 256) {
+        const int k = i / 128, x = i % 128;
+        const bool kin = k < p;
+        Rr[i] = (kin && i0 + x < nV) ? dup16(R[size_t(k) * ld + i0 + x]) : kInf2;
+        Rc[i] = (kin && j0 + x < nV) ? R[size_t(k) * ld + j0 + x] : kInf;
+    }
+    __syncthreads();
+    const int tx = tid & 15, ty = tid >> 4;
+    const int col = j0 + tx * 8;
+    uint16_t* D = DU + size_t(node_list[slot]) * node_stride;
+    uint32_t acc[8][4];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const int i = i0 + ty * 8 + r;
+        uint4 v = make_uint4(kInf2, kInf2, kInf2, kInf2);
+        if (i < nV && col < ld) v = *reinterpret_cast<const uint4*>(D + size_t(i) * ld + col);
+        acc[r][0] = v.x;
+        acc[r][1] = v.y;
+        acc[r][2] = v.z;
+        acc[r][3] = v.w;
+    }
+#pragma unroll 2
+    for (int k = 0; k < p; ++k) {
+        const uint4 a0 = Rr4[k][ty * 2], a1 = Rr4[k][ty * 2 + 1];
+        const uint4 c = Rc4[k][tx];
+        const uint32_t av[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            acc[r][0] = relax2(acc[r][0], av[r], c.x);
+            acc[r][1] = relax2(acc[r][1], av[r], c.y);
+            acc[r][2] = relax2(acc[r][2], av[r], c.z);
+            acc[r][3] = relax2(acc[r][3], av[r], c.w);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const int i = i0 + ty * 8 + r;
+        if (i < nV && col < ld)
+            *reinterpret_cast<uint4*>(D + size_t(i) * ld + col) = make_uint4(acc[r][0], acc[r][1], acc[r][2], acc[r][3]);
+    }
+}
+
+void launch_closure_pivots(int P, int nV, int ld, const int* node_list, int n_list, const int* pivots, int piv_stride,
+                           const int* counts, int t, const uint16_t* DU, size_t node_stride, uint16_t* Rbuf,
+                           hipStream_t st) {
+    if (n_list == 0) return;
+    dim3 grid(unsigned((nV + 511) / 512), unsigned(n_list));
+    if (P == 16)
+        hipLaunchKernelGGL(k_closure_pivots<16>, grid, dim3(256), 0, st, nV, ld, node_list, pivots, piv_stride, counts,
+                           t, DU, node_stride, Rbuf);
+    else if (P == 32)
+        hipLaunchKernelGGL(k_closure_pivots<32>, grid, dim3(256), 0, st, nV, ld, node_list, pivots, piv_stride, counts,
+                           t, DU, node_stride, Rbuf);
+    else
+        hipLaunchKernelGGL(k_closure_pivots<64>, grid, dim3(256), 0, st, nV, ld, node_list, pivots, piv_stride, counts,
+                           t, DU, node_stride, Rbuf);
+}
+
+void launch_closure_update(int P, int nV, int ld, const int* node_list, int n_list, const int* counts, int t,
+                           uint16_t* DU, size_t node_stride, const uint16_t* Rbuf, hipStream_t st) {
+    if (n_list == 0) return;
+    const int tiles = (nV + 127) / 128;
+    dim3 grid(unsigned(tiles * tiles), unsigned(n_list));
+    if (P == 16)
+        hipLaunchKernelGGL(k_closure_update<16>, grid, dim3(256), 0, st, nV, ld, node_list, counts, t, tiles, DU,
+                           node_stride, Rbuf);
+    else if (P == 32)
+        hipLaunchKernelGGL(k_closure_update<32>, grid, dim3(256), 0, st, nV, ld, node_list, counts, t, tiles, DU,
+                           node_stride, Rbuf);
+    else
+        hipLaunchKernelGGL(k_closure_update<64>, grid, dim3(256), 0, st, nV, ld, node_list, counts, t, tiles, DU,
+                           node_stride, Rbuf);
+}
+
+// ---------------------------------------------------------------------------
+// S_U := D_U[ext][ext] (src/internal_node.cpp:233-242, :415-433).
+__global__ __launch_bounds__(256) void k_project_su(int ld, int ne, const int* __restrict__ ext_idx,
+                                                    const int* __restrict__ node_list,
+                                                    const uint16_t* __restrict__ DU, size_t node_stride,
+                                                    uint16_t* __restrict__ SU, int* __restrict__ changed) {
+    const long long gid = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (gid >= (long long)ne * ne) return;
+    const int e1 = int(gid / ne), e2 = int(gid % ne);
+    const int node = node_list[blockIdx.y];
+    const uint16_t v = DU[size_t(node) * node_stride + size_t(ext_idx[e1]) * ld + ext_idx[e2]];
+    uint16_t* dst = SU + size_t(node) * ne * ne + gid;
+    if (changed && v < *dst) changed[blockIdx.y] = 1;
+    *dst = v;
+}
+
+void launch_project_su(int nV, int ld, int ne, const int* ext_idx, const int* node_list, int n_list,
+                       const uint16_t* DU, size_t node_stride, uint16_t* SU, int* changed, hipStream_t st) {
+    (void)nV;
+    if (n_list == 0) return;
+    const long long total = (long long)ne * ne;
+    dim3 grid(unsigned((total + 255) / 256), unsigned(n_list));
+    hipLaunchKernelGGL(k_project_su, grid, dim3(256), 0, st, ld, ne, ext_idx, node_list, DU, node_stride, SU,
+                       changed);
+}
+
+__global__ void k_set_pass(uint8_t* pass, int S, const int2* cells, int n) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) pass[size_t(cells[i].x) * S + cells[i].y] = 1;
+}
+
+void launch_set_pass(uint8_t* pass, int S, const int2* cells, int n, hipStream_t st) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_set_pass, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, pass, S, cells, n);
+}
+
+}  // namespace htg

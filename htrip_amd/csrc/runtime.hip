@@ -1,0 +1,1012 @@
+// Host runtime of libhtrip_gpu.so: the C ABI of include/htrip_gpu.h.
+//
+// A map (htg_map) holds one hierarchy resident in HBM:
+//   pass   S x S u8           passable cells, padded to the root side
+//   leafD  L x b^2 x b^2 u16  tile-restricted all-pairs distances
+//   leafS  L x (4b-4)^2 u16   leaf port summaries
+//   DU[l]  N_l x nV x ld u16  node interface closures (rows padded to 128 B)
+//   SU[l]  N_l x ne x ne u16  node exterior summaries
+// Geometry (port order, V_U order, seam pairs) is identical to the reference
+// so every table can be compared element for element.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/htrip_gpu.h"
+#include "geometry.hpp"
+#include "kernels.hpp"
+#include "runtime.hpp"
+
+using namespace htg;
+
+namespace htg {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    return fail(HTG_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int DevBuf::ensure(size_t n) {
+    if (n <= bytes) return HTG_OK;
+    release();
+    if (n == 0) return HTG_OK;
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+        p = nullptr;
+        (void)hipGetLastError();
+        return fail(HTG_E_NOMEM, "hipMalloc(" + std::to_string(n) + "): " + hipGetErrorString(e));
+    }
+    bytes = n;
+    return HTG_OK;
+}
+void DevBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+int HostBuf::ensure(size_t n) {
+    if (n <= bytes) return HTG_OK;
+    release();
+    if (n == 0) return HTG_OK;
+    hipError_t e = hipHostMalloc(&p, n, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        p = nullptr;
+        (void)hipGetLastError();
+        return fail(HTG_E_NOMEM, "hipHostMalloc(" + std::to_string(n) + "): " + hipGetErrorString(e));
+    }
+    bytes = n;
+    return HTG_OK;
+}
+void HostBuf::release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Level geometry templates (relative to the node origin), shared by every
+// node of a level.  Follows src/internal_node.cpp:13-92.
+static void make_level(LevelGeo& L, int lvl, int b, int g, int S, int H) {
+    L.d = level_dims(lvl, b, g, S);
+    const LevelDims& d = L.d;
+    L.nn = d.nt * d.nt;
+    L.node_stride = size_t(d.nV) * d.ld;
+    L.vu_rc.resize(d.nV);
+    for (int v = 0; v < d.nV; ++v) {
+        const int q = v / d.pc, p = v % d.pc;
+        int r, c;
+        port_rc(p, d.cs, r, c);
+        L.vu_rc[v] = make_short2(short((q / g) * d.cs + r), short((q % g) * d.cs + c));
+    }
+    auto vertex = [&](int r, int c) -> int {  // src/internal_node.cpp:306-319
+        if (r < 0 || c < 0 || r >= d.s || c >= d.s) return -1;
+        const int q = (r / d.cs) * g + c / d.cs;
+        const int p = port_index(r % d.cs, c % d.cs, d.cs);
+        return p < 0 ? -1 : q * d.pc + p;
+    };
+    L.ext_idx.resize(d.ne);
+    L.ext_child.resize(d.ne);
+    L.ext_cport.resize(d.ne);
+    for (int e = 0; e < d.ne; ++e) {
+        int r, c;
+        port_rc(e, d.s, r, c);
+        const int v = vertex(r, c);
+        L.ext_idx[e] = v;
+        L.ext_child[e] = v / d.pc;
+        L.ext_cport[e] = v % d.pc;
+    }
+    // Seam candidates: cross-child ports at Manhattan distance 1, listed once (u < v).
+    L.seams.clear();
+    L.seam_off.assign(g * g + 1, 0);
+    L.seam_list.clear();
+    const int dr[4] = {-1, 1, 0, 0}, dc[4] = {0, 0, -1, 1};
+    for (int q = 0; q < g * g; ++q) {
+        L.seam_off[q] = int(L.seam_list.size());
+        for (int p = 0; p < d.pc; ++p) {
+            const int u = q * d.pc + p;
+            const short2 a = L.vu_rc[u];
+            bool seam_port = false;
+            for (int k = 0; k < 4; ++k) {
+                const int r = a.x + dr[k], c = a.y + dc[k];
+                if (r < 0 || c < 0 || r >= d.s || c >= d.s) continue;
+                if ((r / d.cs) * g + c / d.cs == q) continue;
+                seam_port = true;
+                const int v = vertex(r, c);
+                if (v > u) L.seams.push_back(make_int2(u, v));
+            }
+            if (seam_port) L.seam_list.push_back(p);
+        }
+    }
+    L.seam_off[g * g] = int(L.seam_list.size());
+    // Query projection columns: every V_U port below the root (seam ports
+    // for the meet, exterior ports for the lift); only seam ports at the root.
+    L.gather = (lvl == H);
+    L.zcols.clear();
+    if (!L.gather) {
+        L.zcols.resize(d.nV);
+        std::iota(L.zcols.begin(), L.zcols.end(), 0);
+    } else {
+        for (int q = 0; q < g * g; ++q)
+            for (int i = L.seam_off[q]; i < L.seam_off[q + 1]; ++i) L.zcols.push_back(q * d.pc + L.seam_list[i]);
+        std::sort(L.zcols.begin(), L.zcols.end());
+    }
+    L.ncols = int(L.zcols.size());
+    L.ldz = round_up(std::max(L.ncols, 8), 8);
+    std::vector<int> colpos(d.nV, -1);
+    for (int i = 0; i < L.ncols; ++i) colpos[L.zcols[i]] = i;
+    L.zpos.resize(L.seam_list.size());
+    for (int q = 0; q < g * g; ++q)
+        for (int i = L.seam_off[q]; i < L.seam_off[q + 1]; ++i) L.zpos[i] = colpos[q * d.pc + L.seam_list[i]];
+    L.ext_zcol.resize(d.ne);
+    for (int e = 0; e < d.ne; ++e) L.ext_zcol[e] = colpos[L.ext_idx[e]] < 0 ? 0 : colpos[L.ext_idx[e]];
+}
+
+template <typename T>
+static int upload(DevBuf& dst, const std::vector<T>& src) {
+    int rc = dst.ensure(std::max<size_t>(src.size() * sizeof(T), 16));
+    if (rc) return rc;
+    if (!src.empty()) {
+        hipError_t e = hipMemcpy(dst.p, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_fail(e, "hipMemcpy geometry");
+    }
+    return HTG_OK;
+}
+
+static uint64_t tables_bytes(int b, int g, int S, int H) {
+    const uint64_t nl = uint64_t(S / b), n = uint64_t(b) * b, pl = 4ull * b - 4;
+    uint64_t t = uint64_t(S) * S + nl * nl * (n * n + pl * pl) * 2;
+    for (int l = 1; l <= H; ++l) {
+        LevelDims d = level_dims(l, b, g, S);
+        t += uint64_t(d.nt) * d.nt * (uint64_t(d.nV) * d.ld + uint64_t(d.ne) * d.ne) * 2;
+    }
+    return t;
+}
+
+// Largest per-level build workspace: flags, pivot lists, counts, pivot rows.
+static uint64_t build_ws_bytes(int b, int g, int S, int H, int P) {
+    uint64_t m = 0;
+    for (int l = 1; l <= H; ++l) {
+        LevelDims d = level_dims(l, b, g, S);
+        const uint64_t nn = uint64_t(d.nt) * d.nt;
+        const uint64_t w = nn * (uint64_t(d.nV) * (1 + 4) + 16 + uint64_t(P) * d.ld * 2) + nn * 4 * 4;
+        m = std::max(m, w);
+    }
+    return m;
+}
+
+static void geometry(int b, int g, int rows, int cols, int& S, int& H) {
+    S = b;
+    while (S < std::max(rows, cols)) S *= g;
+    H = 0;
+    for (int t = S / b; t > 1; t /= g) ++H;
+}
+
+int Map::ensure_build_ws(int lvl) {
+    const LevelGeo& L = lv[lvl - 1];
+    const size_t nn = size_t(L.nn);
+    int rc;
+    if ((rc = ws_nodes.ensure(nn * 4))) return rc;
+    if ((rc = ws_flags.ensure(nn * L.d.nV))) return rc;
+    if ((rc = ws_piv.ensure(nn * L.d.nV * 4))) return rc;
+    if ((rc = ws_counts.ensure(nn * 4 + 16))) return rc;
+    if ((rc = ws_R.ensure(nn * size_t(tune.pivot_block) * L.d.ld * 2))) return rc;
+    if ((rc = ws_changed.ensure(nn * 4 + 16))) return rc;
+    if ((rc = ws_mask.ensure(nn * 4 + 16))) return rc;
+    return HTG_OK;
+}
+
+// Closure over each listed node's pivot list (already compacted into ws_piv
+// with counts in ws_counts; host copy of counts in hcounts).
+int Map::run_closure(int lvl, int n_list, const std::vector<int>& hcounts) {
+    LevelGeo& L = lv[lvl - 1];
+    int maxc = 0;
+    for (int i = 0; i < n_list; ++i) maxc = std::max(maxc, hcounts[i]);
+    const int P = tune.pivot_block;
+    const int blocks = (maxc + P - 1) / P;
+    const int* nodes = static_cast<const int*>(ws_nodes.p);
+    for (int t = 0; t < blocks; ++t) {
+        launch_closure_pivots(P, L.d.nV, L.d.ld, nodes, n_list, static_cast<const int*>(ws_piv.p), L.d.nV,
+                              static_cast<const int*>(ws_counts.p), t, static_cast<const uint16_t*>(L.DU.p),
+                              L.node_stride, static_cast<uint16_t*>(ws_R.p), stream);
+        launch_closure_update(P, L.d.nV, L.d.ld, nodes, n_list, static_cast<const int*>(ws_counts.p), t,
+                              static_cast<uint16_t*>(L.DU.p), L.node_stride, static_cast<const uint16_t*>(ws_R.p),
+                              stream);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "closure launch");
+    return HTG_OK;
+}
+
+const uint16_t* Map::child_summaries(int lvl, size_t& stride) const {
+    if (lvl == 1) {
+        stride = size_t(4 * b - 4) * (4 * b - 4);
+        return static_cast<const uint16_t*>(leafS.p);
+    }
+    const LevelGeo& C = lv[lvl - 2];
+    stride = size_t(C.d.ne) * C.d.ne;
+    return static_cast<const uint16_t*>(C.SU.p);
+}
+
+int Map::build() {
+    hipError_t e;
+    launch_leaf_apsp(static_cast<const uint8_t*>(d_pass.p), S, b, nl, nullptr, nl * nl, static_cast<uint16_t*>(leafD.p),
+                     static_cast<uint16_t*>(leafS.p), nullptr, stream);
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "leaf apsp launch");
+    for (int lvl = 1; lvl <= H; ++lvl) {
+        LevelGeo& L = lv[lvl - 1];
+        int rc = ensure_build_ws(lvl);
+        if (rc) return rc;
+        std::vector<int> all(L.nn);
+        std::iota(all.begin(), all.end(), 0);
+        if ((e = hipMemcpyAsync(ws_nodes.p, all.data(), all.size() * 4, hipMemcpyHostToDevice, stream)) != hipSuccess)
+            return hip_fail(e, "upload node list");
+        if ((e = hipMemsetAsync(ws_flags.p, 0, size_t(L.nn) * L.d.nV, stream)) != hipSuccess)
+            return hip_fail(e, "clear flags");
+        size_t cstride;
+        const uint16_t* csum = child_summaries(lvl, cstride);
+        launch_node_assemble(lvl, g, L.d.pc, L.d.nV, L.d.ld, L.d.nt, csum, cstride, static_cast<const int*>(ws_nodes.p),
+                             L.nn, static_cast<uint16_t*>(L.DU.p), L.node_stride, stream);
+        launch_seam_relax(L.d.nV, L.d.ld, L.d.s, L.d.nt, static_cast<const int2*>(L.d_seams.p), int(L.seams.size()),
+                          static_cast<const short2*>(L.d_vu_rc.p), static_cast<const uint8_t*>(d_pass.p), S,
+                          static_cast<const int*>(ws_nodes.p), L.nn, static_cast<uint16_t*>(L.DU.p), L.node_stride,
+                          static_cast<uint8_t*>(ws_flags.p), stream);
+        launch_compact_flags(L.d.nV, static_cast<const int*>(ws_nodes.p), L.nn, static_cast<uint8_t*>(ws_flags.p),
+                             static_cast<int*>(ws_piv.p), L.d.nV, static_cast<int*>(ws_counts.p), stream);
+        std::vector<int> hc(L.nn);
+        if ((e = hipMemcpyAsync(hc.data(), ws_counts.p, size_t(L.nn) * 4, hipMemcpyDeviceToHost, stream)) != hipSuccess)
+            return hip_fail(e, "download counts");
+        if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "build sync");
+        if ((rc = run_closure(lvl, L.nn, hc))) return rc;
+        launch_project_su(L.d.nV, L.d.ld, L.d.ne, static_cast<const int*>(L.d_ext_idx.p),
+                          static_cast<const int*>(ws_nodes.p), L.nn, static_cast<const uint16_t*>(L.DU.p),
+                          L.node_stride, static_cast<uint16_t*>(L.SU.p), nullptr, stream);
+        if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "level launch");
+    }
+    if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "build sync");
+    built = true;
+    epoch++;
+    return HTG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Incremental reveal (src/minplus_oracle.cpp:530-729).  Dirty leaves are
+// recomputed by the leaf BFS kernel (the reference's pivot relaxation
+// computes the same tile closure, src/leaf_tile.cpp:297-374); each dirty
+// ancestor takes min(D_U block, new child summary) plus new seam edges and
+// closes over the rows that decreased -- exactly the reference's pivot set K
+// (src/internal_node.cpp:332-413).
+int Map::open_cells(const int32_t* rcs, int n) {
+    stats = htg_batch_stats{0, 0, 0};
+    if (n <= 0) return HTG_OK;
+    hipError_t e;
+    int rc;
+    const int dr[4] = {-1, 1, 0, 0}, dc[4] = {0, 0, -1, 1};
+    auto passable = [&](int r, int c) { return r >= 0 && c >= 0 && r < rows && c < cols && pass[size_t(r) * S + c]; };
+    // 1. group by leaf (first touch order); remember pre-batch passability
+    std::vector<int> dirty;
+    std::map<int, std::vector<int2>> by_leaf;
+    std::vector<int2> newly;  // cells that became passable
+    std::vector<int2> cells;
+    cells.reserve(n);
+    for (int i = 0; i < n; ++i) {
+        const int r = rcs[2 * i], c = rcs[2 * i + 1];
+        if (r < 0 || c < 0 || r >= S || c >= S) continue;
+        cells.push_back(make_int2(r, c));
+        const int leaf = (r / b) * nl + c / b;
+        auto& v = by_leaf[leaf];
+        if (v.empty()) dirty.push_back(leaf);
+        v.push_back(make_int2(r, c));
+    }
+    if (dirty.empty()) return HTG_OK;
+    std::vector<uint8_t> before;  // passability of touched cells before the batch
+    std::map<long long, uint8_t> prev;
+    auto was = [&](int r, int c) -> bool {
+        if (r < 0 || c < 0 || r >= rows || c >= cols) return false;
+        auto it = prev.find((long long)r * S + c);
+        return it != prev.end() ? it->second : pass[size_t(r) * S + c];
+    };
+    for (const int2& p : cells) {
+        if (p.x >= rows || p.y >= cols) continue;  // padding cells stay impassable
+        const long long key = (long long)p.x * S + p.y;
+        if (!prev.count(key)) prev[key] = pass[size_t(key)];
+        if (!pass[size_t(key)]) {
+            pass[size_t(key)] = 1;
+            cells_code[size_t(p.x) * cols + p.y] = 1;
+            newly.push_back(p);
+        }
+    }
+    // 2. leaf pivot counts, as the reference's leaf update would collect them
+    int piv = 0;
+    for (int leaf : dirty) {
+        const int R0 = (leaf / nl) * b, C0 = (leaf % nl) * b;
+        std::vector<int> K;
+        for (const int2& p : by_leaf[leaf]) {
+            if (!passable(p.x, p.y)) continue;
+            for (int k = 0; k < 4; ++k) {
+                const int vr = p.x + dr[k], vc = p.y + dc[k];
+                if (vr < R0 || vr >= R0 + b || vc < C0 || vc >= C0 + b || !passable(vr, vc)) continue;
+                if (was(p.x, p.y) && was(vr, vc)) continue;
+                K.push_back((p.x - R0) * b + (p.y - C0));
+                K.push_back((vr - R0) * b + (vc - C0));
+            }
+        }
+        std::sort(K.begin(), K.end());
+        piv += int(std::unique(K.begin(), K.end()) - K.begin());
+    }
+    // 3. seam edges by LCA level and node (src/minplus_oracle.cpp:563-598)
+    std::vector<std::map<int, std::vector<std::pair<int, int>>>> seams(H + 1);
+    for (const int2& p : cells) {
+        if (!passable(p.x, p.y)) continue;
+        for (int k = 0; k < 4; ++k) {
+            const int nr = p.x + dr[k], nc = p.y + dc[k];
+            if (!passable(nr, nc)) continue;
+            int ax = p.y / b, ay = p.x / b, bx = nc / b, by = nr / b;
+            if (ax == bx && ay == by) continue;
+            int lca = 0;
+            for (int l = 1; l <= H; ++l) {
+                ax /= g; ay /= g; bx /= g; by /= g;
+                if (ax == bx && ay == by) { lca = l; break; }
+            }
+            if (lca < 1) continue;
+            const LevelGeo& L = lv[lca - 1];
+            const int node = ay * L.d.nt + ax;
+            const int R0 = ay * L.d.s, C0 = ax * L.d.s;
+            auto vert = [&](int r, int c) {
+                r -= R0;
+                c -= C0;
+                const int q = (r / L.d.cs) * g + c / L.d.cs;
+                return q * L.d.pc + port_index(r % L.d.cs, c % L.d.cs, L.d.cs);
+            };
+            int u = vert(p.x, p.y), v = vert(nr, nc);
+            if (u > v) std::swap(u, v);
+            seams[lca][node].push_back({u, v});
+        }
+    }
+    // 4. device: passability, dirty leaves
+    if (!newly.empty()) {
+        if ((rc = ws_cells.ensure(newly.size() * sizeof(int2)))) return rc;
+        if ((e = hipMemcpyAsync(ws_cells.p, newly.data(), newly.size() * sizeof(int2), hipMemcpyHostToDevice, stream)) !=
+            hipSuccess)
+            return hip_fail(e, "upload cells");
+        launch_set_pass(static_cast<uint8_t*>(d_pass.p), S, static_cast<const int2*>(ws_cells.p), int(newly.size()),
+                        stream);
+    }
+    const int nd = int(dirty.size());
+    if ((rc = ws_leaves.ensure(size_t(nd) * 4))) return rc;
+    if ((rc = ws_leaf_changed.ensure(size_t(nd) * 4))) return rc;
+    if ((e = hipMemcpyAsync(ws_leaves.p, dirty.data(), size_t(nd) * 4, hipMemcpyHostToDevice, stream)) != hipSuccess)
+        return hip_fail(e, "upload leaves");
+    if ((e = hipMemsetAsync(ws_leaf_changed.p, 0, size_t(nd) * 4, stream)) != hipSuccess)
+        return hip_fail(e, "clear leaf flags");
+    launch_leaf_apsp(static_cast<const uint8_t*>(d_pass.p), S, b, nl, static_cast<const int*>(ws_leaves.p), nd,
+                     static_cast<uint16_t*>(leafD.p), static_cast<uint16_t*>(leafS.p),
+                     static_cast<int*>(ws_leaf_changed.p), stream);
+    stats.n_touched_leaves = nd;
+    stats.n_pivots = piv;
+    if (H == 0) {
+        if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "update sync");
+        return HTG_OK;  // the reference returns before bumping epoch here
+    }
+    std::vector<int> lchanged(nd);
+    if ((e = hipMemcpyAsync(lchanged.data(), ws_leaf_changed.p, size_t(nd) * 4, hipMemcpyDeviceToHost, stream)) !=
+        hipSuccess)
+        return hip_fail(e, "download leaf flags");
+    if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "update sync");
+    // child masks of level-1 parents, in first-touch order
+    std::vector<int> active;
+    std::map<int, uint32_t> mask;
+    for (int i = 0; i < nd; ++i) {
+        if (!lchanged[i]) continue;
+        const int tx = dirty[i] % nl, ty = dirty[i] / nl;
+        const int parent = (ty / g) * lv[0].d.nt + tx / g;
+        const int quad = (ty % g) * g + tx % g;
+        if (!mask.count(parent)) active.push_back(parent);
+        mask[parent] |= 1u << quad;
+    }
+    int anc = 0;
+    for (int lvl = 1; lvl <= H; ++lvl) {
+        LevelGeo& L = lv[lvl - 1];
+        for (auto& kv : seams[lvl])
+            if (!mask.count(kv.first)) {
+                active.push_back(kv.first);
+                mask[kv.first] = 0;
+            }
+        if (active.empty()) {
+            bool higher = false;
+            for (int hl = lvl + 1; hl <= H && !higher; ++hl) higher = !seams[hl].empty();
+            if (!higher) break;
+            continue;
+        }
+        anc += int(active.size());
+        const int na = int(active.size());
+        if ((rc = ensure_build_ws(lvl))) return rc;
+        std::vector<uint32_t> hmask(na);
+        std::vector<int4> edges;
+        for (int i = 0; i < na; ++i) {
+            hmask[i] = mask[active[i]];
+            auto it = seams[lvl].find(active[i]);
+            if (it == seams[lvl].end()) continue;
+            auto& lst = it->second;
+            std::sort(lst.begin(), lst.end());
+            lst.erase(std::unique(lst.begin(), lst.end()), lst.end());
+            for (auto& uv : lst) edges.push_back(make_int4(i, uv.first, uv.second, 1));
+        }
+        if ((e = hipMemcpyAsync(ws_nodes.p, active.data(), size_t(na) * 4, hipMemcpyHostToDevice, stream)) != hipSuccess)
+            return hip_fail(e, "upload nodes");
+        if ((e = hipMemcpyAsync(ws_mask.p, hmask.data(), size_t(na) * 4, hipMemcpyHostToDevice, stream)) != hipSuccess)
+            return hip_fail(e, "upload masks");
+        if ((e = hipMemsetAsync(ws_flags.p, 0, size_t(na) * L.d.nV, stream)) != hipSuccess)
+            return hip_fail(e, "clear flags");
+        size_t cstride;
+        const uint16_t* csum = child_summaries(lvl, cstride);
+        launch_child_relax(lvl, g, L.d.pc, L.d.nV, L.d.ld, L.d.nt, csum, cstride, static_cast<const int*>(ws_nodes.p),
+                           static_cast<const uint32_t*>(ws_mask.p), na, static_cast<uint16_t*>(L.DU.p), L.node_stride,
+                           static_cast<uint8_t*>(ws_flags.p), stream);
+        if (!edges.empty()) {
+            if ((rc = ws_edges.ensure(edges.size() * sizeof(int4)))) return rc;
+            if ((e = hipMemcpyAsync(ws_edges.p, edges.data(), edges.size() * sizeof(int4), hipMemcpyHostToDevice,
+                                    stream)) != hipSuccess)
+                return hip_fail(e, "upload edges");
+            launch_edge_relax(L.d.nV, L.d.ld, static_cast<const int4*>(ws_edges.p), int(edges.size()),
+                              static_cast<const int*>(ws_nodes.p), static_cast<uint16_t*>(L.DU.p), L.node_stride,
+                              static_cast<uint8_t*>(ws_flags.p), stream);
+        }
+        launch_compact_flags(L.d.nV, static_cast<const int*>(ws_nodes.p), na, static_cast<uint8_t*>(ws_flags.p),
+                             static_cast<int*>(ws_piv.p), L.d.nV, static_cast<int*>(ws_counts.p), stream);
+        std::vector<int> hc(na);
+        if ((e = hipMemcpyAsync(hc.data(), ws_counts.p, size_t(na) * 4, hipMemcpyDeviceToHost, stream)) != hipSuccess)
+            return hip_fail(e, "download counts");
+        if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "update sync");
+        if ((rc = run_closure(lvl, na, hc))) return rc;
+        // S_U refresh for nodes that closed over something (the reference
+        // skips nodes with an empty pivot set, src/internal_node.cpp:368-370)
+        if ((e = hipMemsetAsync(ws_changed.p, 0, size_t(na) * 4, stream)) != hipSuccess)
+            return hip_fail(e, "clear changed");
+        launch_project_su(L.d.nV, L.d.ld, L.d.ne, static_cast<const int*>(L.d_ext_idx.p),
+                          static_cast<const int*>(ws_nodes.p), na, static_cast<const uint16_t*>(L.DU.p), L.node_stride,
+                          static_cast<uint16_t*>(L.SU.p), static_cast<int*>(ws_changed.p), stream);
+        std::vector<int> nchanged(na);
+        if ((e = hipMemcpyAsync(nchanged.data(), ws_changed.p, size_t(na) * 4, hipMemcpyDeviceToHost, stream)) !=
+            hipSuccess)
+            return hip_fail(e, "download changed");
+        if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "update sync");
+        std::vector<int> next_active;
+        std::map<int, uint32_t> next_mask;
+        if (lvl < H) {
+            const int ntp = lv[lvl].d.nt;
+            for (int i = 0; i < na; ++i) {
+                if (!nchanged[i]) continue;
+                const int tx = active[i] % L.d.nt, ty = active[i] / L.d.nt;
+                const int parent = (ty / g) * ntp + tx / g;
+                const int quad = (ty % g) * g + tx % g;
+                if (!next_mask.count(parent)) next_active.push_back(parent);
+                next_mask[parent] |= 1u << quad;
+            }
+        }
+        active = std::move(next_active);
+        mask = std::move(next_mask);
+    }
+    stats.n_dirty_ancestors = anc;
+    epoch++;
+    return HTG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batched all-pairs query.  Host planning: viewpoints are sorted by their
+// tile key (base g^2 digits from the root down), so at every level the
+// viewpoints that share (W, q) and those that share W are contiguous runs.
+int Map::query(const int32_t* rcs, const int32_t* ks, int nq, uint16_t* out_host, QueryWS& ws) {
+    hipError_t e;
+    int rc;
+    long long total_vp = 0, total_out = 0;
+    int kmax = 0;
+    for (int q = 0; q < nq; ++q) {
+        if (ks[q] < 0) return fail(HTG_E_INVALID, "negative k");
+        total_vp += ks[q];
+        total_out += (long long)ks[q] * ks[q];
+        kmax = std::max(kmax, ks[q]);
+    }
+    if (total_out == 0) return HTG_OK;
+    // ---- plan
+    QueryPlan& P = ws.plan;
+    P.clear();
+    P.q_k.assign(ks, ks + nq);
+    P.q_out.resize(nq);
+    P.q_vp.resize(nq);
+    long long oo = 0, vv = 0;
+    for (int q = 0; q < nq; ++q) {
+        P.q_out[q] = oo;
+        P.q_vp[q] = int(vv);
+        oo += (long long)ks[q] * ks[q];
+        vv += ks[q];
+    }
+    P.vp_rc.resize(total_vp);
+    P.vp_valid.resize(total_vp);
+    struct SlotKey {
+        unsigned long long key;
+        int q, i, vp;
+    };
+    std::vector<SlotKey> sk;
+    sk.reserve(total_vp);
+    for (int q = 0; q < nq; ++q)
+        for (int i = 0; i < ks[q]; ++i) {
+            const int v = P.q_vp[q] + i;
+            const int r = rcs[2 * v], c = rcs[2 * v + 1];
+            P.vp_rc[v] = make_int2(r, c);
+            const bool ok = r >= 0 && c >= 0 && r < rows && c < cols && pass[size_t(r) * S + c];
+            P.vp_valid[v] = ok;
+            if (!ok) continue;
+            unsigned long long key = 0;
+            for (int l = H; l >= 1; --l) {
+                const int cs = lv[l - 1].d.cs;
+                key = key * unsigned(g * g) + unsigned(((r / cs) % g) * g + (c / cs) % g);
+            }
+            sk.push_back({key, q, i, v});
+        }
+    std::sort(sk.begin(), sk.end(), [](const SlotKey& a, const SlotKey& c) {
+        if (a.key != c.key) return a.key < c.key;
+        if (a.q != c.q) return a.q < c.q;
+        return a.i < c.i;
+    });
+    const int ns = int(sk.size());
+    P.slot_rc.resize(ns);
+    P.slot_q.resize(ns);
+    P.slot_i.resize(ns);
+    for (int s = 0; s < ns; ++s) {
+        P.slot_rc[s] = P.vp_rc[sk[s].vp];
+        P.slot_q[s] = sk[s].q;
+        P.slot_i[s] = sk[s].i;
+    }
+    P.levels.resize(H);
+    unsigned long long div = 1;
+    for (int l = 1; l <= H; ++l) {
+        const LevelGeo& L = lv[l - 1];
+        LevelPlan& LP = P.levels[l - 1];
+        // (W, q) runs -> projection items
+        int maxrun = 0;
+        for (int s = 0; s < ns;) {
+            int t = s + 1;
+            while (t < ns && sk[t].key / div == sk[s].key / div) ++t;
+            maxrun = std::max(maxrun, t - s);
+            s = t;
+        }
+        int M = 1;
+        while (M < maxrun && M < tune.zq_tile) M *= 2;
+        LP.M = M;
+        for (int s = 0; s < ns;) {
+            int t = s + 1;
+            while (t < ns && sk[t].key / div == sk[s].key / div) ++t;
+            const int2 rc0 = P.slot_rc[s];
+            const int W = (rc0.x / L.d.s) * L.d.nt + rc0.y / L.d.s;
+            const int q = int((sk[s].key / div) % unsigned(g * g));
+            for (int a = s; a < t; a += M) LP.items.push_back(make_int4(W, q, a, std::min(M, t - a)));
+            s = t;
+        }
+        // (query, W) groups -> meet rows
+        const unsigned long long divW = div * unsigned(g * g);
+        for (int s = 0; s < ns;) {
+            int t = s + 1;
+            while (t < ns && sk[t].key / divW == sk[s].key / divW) ++t;
+            std::map<int, std::vector<int>> byq;
+            for (int a = s; a < t; ++a) byq[sk[a].q].push_back(a);
+            for (auto& kv : byq) {
+                if (kv.second.size() < 2) continue;
+                const int gi = int(LP.groups.size());
+                LP.groups.push_back(make_int4(kv.first, 0, int(LP.glist.size()), int(kv.second.size())));
+                for (size_t a = 0; a < kv.second.size(); ++a) LP.rows.push_back(make_int2(gi, int(a)));
+                LP.glist.insert(LP.glist.end(), kv.second.begin(), kv.second.end());
+            }
+            s = t;
+        }
+        div = divW;
+    }
+    // ---- upload (one packed staging buffer)
+    Packer pk;
+    const size_t o_qk = pk.add(P.q_k), o_qout = pk.add(P.q_out), o_qvp = pk.add(P.q_vp), o_vprc = pk.add(P.vp_rc),
+                 o_vpvalid = pk.add(P.vp_valid), o_src = pk.add(P.slot_rc), o_sq = pk.add(P.slot_q),
+                 o_si = pk.add(P.slot_i);
+    std::vector<size_t> o_items(H), o_groups(H), o_glist(H), o_rows(H);
+    for (int l = 0; l < H; ++l) {
+        o_items[l] = pk.add(P.levels[l].items);
+        o_groups[l] = pk.add(P.levels[l].groups);
+        o_glist[l] = pk.add(P.levels[l].glist);
+        o_rows[l] = pk.add(P.levels[l].rows);
+    }
+    if ((rc = ws.h_plan.ensure(pk.size))) return rc;
+    pk.write(static_cast<char*>(ws.h_plan.p));
+    if ((rc = ws.d_plan.ensure(pk.size))) return rc;
+    size_t maxpc = size_t(4 * b - 4), maxldz = 8;
+    for (int l = 1; l <= H; ++l) {
+        maxpc = std::max(maxpc, size_t(lv[l - 1].d.ne));
+        maxldz = std::max(maxldz, size_t(lv[l - 1].ldz));
+    }
+    const size_t nsl = size_t(std::max(ns, 1));
+    if ((rc = ws.X[0].ensure(nsl * maxpc * 2))) return rc;
+    if ((rc = ws.X[1].ensure(nsl * maxpc * 2))) return rc;
+    if ((rc = ws.Z.ensure(nsl * maxldz * 2))) return rc;
+    if ((rc = ws.d_out.ensure(size_t(total_out) * 2))) return rc;
+    if ((rc = ws.h_out.ensure(size_t(total_out) * 2))) return rc;
+    hipStream_t st = ws.stream;
+    if ((e = hipMemcpyAsync(ws.d_plan.p, ws.h_plan.p, pk.size, hipMemcpyHostToDevice, st)) != hipSuccess)
+        return hip_fail(e, "upload plan");
+    char* dp = static_cast<char*>(ws.d_plan.p);
+    auto I = [&](size_t o) { return reinterpret_cast<const int*>(dp + o); };
+    uint16_t* dout = static_cast<uint16_t*>(ws.d_out.p);
+    const uint16_t* dleafD = static_cast<const uint16_t*>(leafD.p);
+    launch_q_init(nq, I(o_qk), reinterpret_cast<const long long*>(dp + o_qout), I(o_qvp),
+                  reinterpret_cast<const int2*>(dp + o_vprc), reinterpret_cast<const uint8_t*>(dp + o_vpvalid), b, nl,
+                  dleafD, dout, kmax, st);
+    const int2* srcs = reinterpret_cast<const int2*>(dp + o_src);
+    uint16_t* Xa = static_cast<uint16_t*>(ws.X[0].p);
+    uint16_t* Xb = static_cast<uint16_t*>(ws.X[1].p);
+    uint16_t* Z = static_cast<uint16_t*>(ws.Z.p);
+    if (H > 0 && ns > 0) launch_q_leaf(ns, srcs, b, nl, dleafD, Xa, st);
+    for (int l = 1; l <= H && ns > 0; ++l) {
+        const LevelGeo& L = lv[l - 1];
+        const LevelPlan& LP = P.levels[l - 1];
+        launch_q_z(LP.M, L.d.nV, L.d.ld, L.d.pc, static_cast<const uint16_t*>(L.DU.p), L.node_stride,
+                   static_cast<const int*>(L.d_seam_list.p), static_cast<const int*>(L.d_seam_off.p),
+                   static_cast<const int*>(L.d_zcols.p), L.ncols, L.ldz, L.gather,
+                   reinterpret_cast<const int4*>(dp + o_items[l - 1]), int(LP.items.size()), Xa, Z, st);
+        launch_q_meet(L.d.pc, L.d.cs, g, static_cast<const int*>(L.d_seam_list.p),
+                      static_cast<const int*>(L.d_seam_off.p), static_cast<const int*>(L.d_zpos.p), L.ldz,
+                      reinterpret_cast<const int2*>(dp + o_rows[l - 1]), int(LP.rows.size()),
+                      reinterpret_cast<const int4*>(dp + o_groups[l - 1]), I(o_glist[l - 1]), srcs, I(o_sq), I(o_si),
+                      I(o_qk), reinterpret_cast<const long long*>(dp + o_qout), Xa, Z, dout, st);
+        if (l < H) {
+            launch_q_lift(ns, L.d.ne, static_cast<const int*>(L.d_ext_zcol.p), static_cast<const int*>(L.d_ext_child.p),
+                          static_cast<const int*>(L.d_ext_cport.p), L.d.pc, L.d.cs, g, L.ldz, srcs, Xa, Z, Xb, st);
+            std::swap(Xa, Xb);
+        }
+    }
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "query launch");
+    if (out_host) {
+        if ((e = hipMemcpyAsync(ws.h_out.p, ws.d_out.p, size_t(total_out) * 2, hipMemcpyDeviceToHost, st)) !=
+            hipSuccess)
+            return hip_fail(e, "download result");
+        if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(e, "query sync");
+        std::memcpy(out_host, ws.h_out.p, size_t(total_out) * 2);
+    }
+    return HTG_OK;
+}
+
+int QueryWS::init() {
+    if (stream) return HTG_OK;
+    hipError_t e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
+    return HTG_OK;
+}
+QueryWS::~QueryWS() {
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+}  // namespace htg
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+struct htg_ctx {
+    int device = 0;
+};
+
+#define HTG_GUARD_BEGIN try {
+#define HTG_GUARD_END                                                   \
+    }                                                                   \
+    catch (const std::bad_alloc&) {                                     \
+        return htg::fail(HTG_E_NOMEM, "host allocation failed");        \
+    }                                                                   \
+    catch (...) {                                                       \
+        return htg::fail(HTG_E_INVALID, "unexpected exception");        \
+    }
+
+extern "C" {
+
+const char* htg_last_error(void) { return htg::g_err.c_str(); }
+
+int htg_ctx_create(int device, htg_ctx** out) {
+    HTG_GUARD_BEGIN
+    if (!out) return fail(HTG_E_INVALID, "null out");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDeviceCount");
+    if (device < 0 || device >= n) return fail(HTG_E_INVALID, "no such device");
+    if ((e = hipSetDevice(device)) != hipSuccess) return hip_fail(e, "hipSetDevice");
+    auto* c = new htg_ctx();
+    c->device = device;
+    *out = c;
+    return HTG_OK;
+    HTG_GUARD_END
+}
+
+int htg_ctx_destroy(htg_ctx* ctx) {
+    delete ctx;
+    return HTG_OK;
+}
+
+int htg_ctx_mem_info(htg_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes) {
+    if (!ctx) return fail(HTG_E_INVALID, "null ctx");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    size_t f = 0, t = 0;
+    if ((e = hipMemGetInfo(&f, &t)) != hipSuccess) return hip_fail(e, "hipMemGetInfo");
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    return HTG_OK;
+}
+
+int htg_map_footprint(const htg_config* cfg, int32_t rows, int32_t cols, uint64_t* bytes) {
+    if (!cfg || !bytes) return fail(HTG_E_INVALID, "null argument");
+    const int b = cfg->tile_size_b, g = cfg->group_w;
+    if (b < 2 || b > 64 || g < 2 || cfg->group_h != g || g * g > 32 || rows <= 0 || cols <= 0)
+        return fail(HTG_E_INVALID, "unsupported configuration");
+    int S, H;
+    geometry(b, g, rows, cols, S, H);
+    *bytes = tables_bytes(b, g, S, H) + build_ws_bytes(b, g, S, H, 64);
+    return HTG_OK;
+}
+
+int htg_map_create(htg_ctx* ctx, const htg_config* cfg, int32_t rows, int32_t cols, const uint8_t* cells,
+                   uint64_t hbm_budget, htg_map** out) {
+    HTG_GUARD_BEGIN
+    if (!ctx || !cfg || !cells || !out) return fail(HTG_E_INVALID, "null argument");
+    // include/htrip/types.hpp:221-224 plus this build's limits: 2 <= b <= 64
+    // (one tile row per lane) and g^2 <= 32 (child masks are 32-bit).
+    const int b = cfg->tile_size_b, g = cfg->group_w;
+    if (b <= 0 || g < 2 || cfg->group_h != g || g * g > 256) return fail(HTG_E_INVALID, "invalid HierarchyConfig");
+    if (b < 2 || b > 64 || g * g > 32) return fail(HTG_E_INVALID, "tile_size_b must be 2..64 and group <= 5");
+    if (rows <= 0 || cols <= 0) return fail(HTG_E_INVALID, "empty grid");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    uint64_t need = 0;
+    int rc = htg_map_footprint(cfg, rows, cols, &need);
+    if (rc) return rc;
+    if (hbm_budget && need > hbm_budget)
+        return fail(HTG_E_NOMEM, "map needs " + std::to_string(need) + " bytes, budget " + std::to_string(hbm_budget));
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) == hipSuccess && need > fr)
+        return fail(HTG_E_NOMEM, "map needs " + std::to_string(need) + " bytes, device has " + std::to_string(fr));
+    std::unique_ptr<htg_map> m(new htg_map());
+    m->ctx = ctx;
+    m->cfg = *cfg;
+    m->b = b;
+    m->g = g;
+    m->rows = rows;
+    m->cols = cols;
+    m->budget = hbm_budget;
+    geometry(b, g, rows, cols, m->S, m->H);
+    m->nl = m->S / b;
+    m->cells_code.assign(cells, cells + size_t(rows) * cols);
+    m->pass.assign(size_t(m->S) * m->S, 0);
+    for (int r = 0; r < rows; ++r)
+        for (int c = 0; c < cols; ++c) m->pass[size_t(r) * m->S + c] = cells[size_t(r) * cols + c] == 1;
+    if ((e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking)) != hipSuccess)
+        return hip_fail(e, "hipStreamCreate");
+    if ((rc = m->qws.init())) return rc;
+    const size_t n = size_t(b) * b, pl = size_t(4 * b - 4), nleaves = size_t(m->nl) * m->nl;
+    if ((rc = m->d_pass.ensure(m->pass.size()))) return rc;
+    if ((rc = m->leafD.ensure(nleaves * n * n * 2))) return rc;
+    if ((rc = m->leafS.ensure(nleaves * pl * pl * 2))) return rc;
+    if ((e = hipMemcpy(m->d_pass.p, m->pass.data(), m->pass.size(), hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(e, "upload grid");
+    m->lv.resize(m->H);
+    for (int l = 1; l <= m->H; ++l) {
+        LevelGeo& L = m->lv[l - 1];
+        make_level(L, l, b, g, m->S, m->H);
+        if ((rc = L.DU.ensure(size_t(L.nn) * L.node_stride * 2))) return rc;
+        if ((rc = L.SU.ensure(size_t(L.nn) * L.d.ne * L.d.ne * 2))) return rc;
+        if ((rc = upload(L.d_vu_rc, L.vu_rc)) || (rc = upload(L.d_ext_idx, L.ext_idx)) ||
+            (rc = upload(L.d_ext_child, L.ext_child)) || (rc = upload(L.d_ext_cport, L.ext_cport)) ||
+            (rc = upload(L.d_seams, L.seams)) || (rc = upload(L.d_seam_list, L.seam_list)) ||
+            (rc = upload(L.d_seam_off, L.seam_off)) || (rc = upload(L.d_zcols, L.zcols)) ||
+            (rc = upload(L.d_zpos, L.zpos)) || (rc = upload(L.d_ext_zcol, L.ext_zcol)))
+            return rc;
+    }
+    m->device_bytes = tables_bytes(b, g, m->S, m->H);
+    *out = m.release();
+    return HTG_OK;
+    HTG_GUARD_END
+}
+
+int htg_map_destroy(htg_map* map) {
+    if (!map) return HTG_OK;
+    (void)hipSetDevice(map->ctx->device);
+    (void)hipStreamSynchronize(map->stream);
+    (void)hipStreamSynchronize(map->qws.stream);
+    if (map->stream) (void)hipStreamDestroy(map->stream);
+    delete map;
+    return HTG_OK;
+}
+
+int htg_map_clone(const htg_map* src, uint64_t hbm_budget, htg_map** out) {
+    HTG_GUARD_BEGIN
+    if (!src || !out) return fail(HTG_E_INVALID, "null argument");
+    htg_map* m = nullptr;
+    int rc = htg_map_create(src->ctx, &src->cfg, src->rows, src->cols, src->cells_code.data(), hbm_budget, &m);
+    if (rc) return rc;
+    std::unique_ptr<htg_map> hold(m);
+    m->pass = src->pass;
+    m->tune = src->tune;
+    m->built = src->built;
+    m->stats = src->stats;
+    m->epoch = src->epoch;
+    hipError_t e;
+    (void)hipStreamSynchronize(src->stream);
+    if ((e = hipMemcpy(m->d_pass.p, src->d_pass.p, src->d_pass.bytes < m->d_pass.bytes ? src->d_pass.bytes : m->d_pass.bytes,
+                       hipMemcpyDeviceToDevice)) != hipSuccess ||
+        (e = hipMemcpy(m->leafD.p, src->leafD.p, m->leafD.bytes, hipMemcpyDeviceToDevice)) != hipSuccess ||
+        (e = hipMemcpy(m->leafS.p, src->leafS.p, m->leafS.bytes, hipMemcpyDeviceToDevice)) != hipSuccess)
+        return hip_fail(e, "clone copy");
+    for (int l = 0; l < m->H; ++l) {
+        if ((e = hipMemcpy(m->lv[l].DU.p, src->lv[l].DU.p, size_t(m->lv[l].nn) * m->lv[l].node_stride * 2,
+                           hipMemcpyDeviceToDevice)) != hipSuccess ||
+            (e = hipMemcpy(m->lv[l].SU.p, src->lv[l].SU.p, size_t(m->lv[l].nn) * m->lv[l].d.ne * m->lv[l].d.ne * 2,
+                           hipMemcpyDeviceToDevice)) != hipSuccess)
+            return hip_fail(e, "clone copy");
+    }
+    *out = hold.release();
+    return HTG_OK;
+    HTG_GUARD_END
+}
+
+int htg_map_info_get(const htg_map* map, htg_map_info* out) {
+    if (!map || !out) return fail(HTG_E_INVALID, "null argument");
+    out->rows = map->rows;
+    out->cols = map->cols;
+    out->S = map->S;
+    out->H = map->H;
+    out->num_leaf_tiles_x = map->nl;
+    out->num_leaves = map->nl * map->nl;
+    out->built = map->built ? 1 : 0;
+    out->reserved = 0;
+    // Reference accounting (src/minplus_oracle.cpp:2116-2134): leaf D + S,
+    // node D_U + S_U + child_to_ext (nV x nExt).
+    const uint64_t n = uint64_t(map->b) * map->b, pl = 4ull * map->b - 4;
+    uint64_t t = uint64_t(map->nl) * map->nl * (n * n + pl * pl) * 2;
+    for (const auto& L : map->lv)
+        t += uint64_t(L.nn) * (uint64_t(L.d.nV) * L.d.nV + uint64_t(L.d.ne) * L.d.ne + uint64_t(L.d.nV) * L.d.ne) * 2;
+    out->total_memory_bytes = t;
+    out->device_bytes = map->device_bytes;
+    return HTG_OK;
+}
+
+int htg_build(htg_map* map) {
+    HTG_GUARD_BEGIN
+    if (!map) return fail(HTG_E_INVALID, "null map");
+    hipError_t e = hipSetDevice(map->ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    return map->build();
+    HTG_GUARD_END
+}
+
+int htg_open_cells(htg_map* map, const int32_t* rc, int32_t n) {
+    HTG_GUARD_BEGIN
+    if (!map || (n > 0 && !rc)) return fail(HTG_E_INVALID, "null argument");
+    if (!map->built) return fail(HTG_E_STATE, "map not built");
+    hipError_t e = hipSetDevice(map->ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    return map->open_cells(rc, n);
+    HTG_GUARD_END
+}
+
+int htg_open_cell(htg_map* map, int32_t r, int32_t c) {
+    HTG_GUARD_BEGIN
+    if (!map) return fail(HTG_E_INVALID, "null map");
+    if (!map->built) return fail(HTG_E_STATE, "map not built");
+    hipError_t e = hipSetDevice(map->ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    // onCellOpened does not touch last_batch_stats (src/minplus_oracle.cpp:139-177)
+    const htg_batch_stats keep = map->stats;
+    const int32_t rcv[2] = {r, c};
+    int rcode = map->open_cells(rcv, 1);
+    map->stats = keep;
+    return rcode;
+    HTG_GUARD_END
+}
+
+int htg_query_batch(htg_map* map, const int32_t* rc, const int32_t* ks, int32_t nq, uint16_t* out) {
+    HTG_GUARD_BEGIN
+    if (!map || (nq > 0 && (!ks || !out))) return fail(HTG_E_INVALID, "null argument");
+    if (!map->built) return fail(HTG_E_STATE, "map not built");
+    hipError_t e = hipSetDevice(map->ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    // queries read what the map's update stream wrote
+    if ((e = hipStreamSynchronize(map->stream)) != hipSuccess) return hip_fail(e, "update sync");
+    return map->query(rc, ks, nq, out, map->qws);
+    HTG_GUARD_END
+}
+
+int htg_query_all_pairs(htg_map* map, const int32_t* rc, int32_t k, uint16_t* out) {
+    return htg_query_batch(map, rc, &k, 1, out);
+}
+
+int htg_last_batch_stats(const htg_map* map, htg_batch_stats* out) {
+    if (!map || !out) return fail(HTG_E_INVALID, "null argument");
+    *out = map->stats;
+    return HTG_OK;
+}
+
+int htg_sync(htg_map* map) {
+    if (!map) return fail(HTG_E_INVALID, "null map");
+    hipError_t e = hipStreamSynchronize(map->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(map->qws.stream);
+    if (e != hipSuccess) return hip_fail(e, "sync");
+    return HTG_OK;
+}
+
+static int read_rows(htg_map* map, const void* src, size_t rows, size_t cols, size_t ld, uint16_t* out) {
+    hipError_t e = hipSetDevice(map->ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    if ((e = hipStreamSynchronize(map->stream)) != hipSuccess) return hip_fail(e, "sync");
+    if ((e = hipMemcpy2D(out, cols * 2, src, ld * 2, cols * 2, rows, hipMemcpyDeviceToHost)) != hipSuccess)
+        return hip_fail(e, "read table");
+    return HTG_OK;
+}
+
+int htg_read_leaf_D(htg_map* map, int32_t leaf, uint16_t* out) {
+    if (!map || !out) return fail(HTG_E_INVALID, "null argument");
+    if (leaf < 0 || leaf >= map->nl * map->nl) return fail(HTG_E_RANGE, "leaf index");
+    const size_t n = size_t(map->b) * map->b;
+    return read_rows(map, static_cast<const uint16_t*>(map->leafD.p) + size_t(leaf) * n * n, n, n, n, out);
+}
+
+int htg_read_leaf_S(htg_map* map, int32_t leaf, uint16_t* out) {
+    if (!map || !out) return fail(HTG_E_INVALID, "null argument");
+    if (leaf < 0 || leaf >= map->nl * map->nl) return fail(HTG_E_RANGE, "leaf index");
+    const size_t p = size_t(4 * map->b - 4);
+    return read_rows(map, static_cast<const uint16_t*>(map->leafS.p) + size_t(leaf) * p * p, p, p, p, out);
+}
+
+int htg_node_dims(const htg_map* map, int32_t lvl, int32_t* num_nodes, int32_t* nV, int32_t* nExt) {
+    if (!map) return fail(HTG_E_INVALID, "null map");
+    if (lvl < 1 || lvl > map->H) return fail(HTG_E_RANGE, "level");
+    const LevelGeo& L = map->lv[lvl - 1];
+    if (num_nodes) *num_nodes = L.nn;
+    if (nV) *nV = L.d.nV;
+    if (nExt) *nExt = L.d.ne;
+    return HTG_OK;
+}
+
+int htg_read_node_DU(htg_map* map, int32_t lvl, int32_t idx, uint16_t* out) {
+    if (!map || !out) return fail(HTG_E_INVALID, "null argument");
+    if (lvl < 1 || lvl > map->H || idx < 0 || idx >= map->lv[lvl - 1].nn) return fail(HTG_E_RANGE, "node index");
+    const LevelGeo& L = map->lv[lvl - 1];
+    return read_rows(map, static_cast<const uint16_t*>(L.DU.p) + size_t(idx) * L.node_stride, L.d.nV, L.d.nV, L.d.ld,
+                     out);
+}
+
+int htg_read_node_SU(htg_map* map, int32_t lvl, int32_t idx, uint16_t* out) {
+    if (!map || !out) return fail(HTG_E_INVALID, "null argument");
+    if (lvl < 1 || lvl > map->H || idx < 0 || idx >= map->lv[lvl - 1].nn) return fail(HTG_E_RANGE, "node index");
+    const LevelGeo& L = map->lv[lvl - 1];
+    const size_t ne = size_t(L.d.ne);
+    return read_rows(map, static_cast<const uint16_t*>(L.SU.p) + size_t(idx) * ne * ne, ne, ne, ne, out);
+}
+
+int htg_get_tuning(const htg_map* map, htg_tuning* out) {
+    if (!map || !out) return fail(HTG_E_INVALID, "null argument");
+    *out = map->tune;
+    return HTG_OK;
+}
+
+int htg_set_tuning(htg_map* map, const htg_tuning* t) {
+    if (!map || !t) return fail(HTG_E_INVALID, "null argument");
+    if (t->pivot_block != 16 && t->pivot_block != 32 && t->pivot_block != 64)
+        return fail(HTG_E_INVALID, "pivot_block must be 16, 32 or 64");
+    if (t->zq_tile < 1 || t->zq_tile > 16) return fail(HTG_E_INVALID, "zq_tile must be 1..16");
+    map->tune = *t;
+    return HTG_OK;
+}
+
+}  // extern "C"

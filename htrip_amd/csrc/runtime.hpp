@@ -1,0 +1,160 @@
+// Internal types of the host runtime (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/htrip_gpu.h"
+#include "geometry.hpp"
+
+namespace htg {
+
+extern thread_local std::string g_err;
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+// Grow-only device buffer.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) {
+        o.p = nullptr;
+        o.bytes = 0;
+    }
+    ~DevBuf() { release(); }
+    int ensure(size_t n);
+    void release();
+};
+
+// Grow-only pinned host buffer.
+struct HostBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    HostBuf() = default;
+    HostBuf(const HostBuf&) = delete;
+    HostBuf& operator=(const HostBuf&) = delete;
+    HostBuf(HostBuf&& o) noexcept : p(o.p), bytes(o.bytes) {
+        o.p = nullptr;
+        o.bytes = 0;
+    }
+    ~HostBuf() { release(); }
+    int ensure(size_t n);
+    void release();
+};
+
+// Packs several host vectors into one upload, each 16-byte aligned.
+struct Packer {
+    struct Part {
+        const void* src;
+        size_t off, bytes;
+    };
+    std::vector<Part> parts;
+    size_t size = 0;
+    template <typename T>
+    size_t add(const std::vector<T>& v) {
+        const size_t off = size;
+        parts.push_back({v.data(), off, v.size() * sizeof(T)});
+        size = (off + v.size() * sizeof(T) + 15) / 16 * 16 + (v.empty() ? 16 : 0);
+        return off;
+    }
+    void write(char* dst) const {
+        for (const Part& p : parts)
+            if (p.bytes) std::memcpy(dst + p.off, p.src, p.bytes);
+    }
+};
+
+struct LevelGeo {
+    LevelDims d;
+    int nn = 0;              // nodes at this level
+    size_t node_stride = 0;  // elements per node D_U (nV * ld)
+    std::vector<short2> vu_rc;
+    std::vector<int> ext_idx, ext_child, ext_cport;
+    std::vector<int2> seams;
+    std::vector<int> seam_list, seam_off;  // per-child seam ports
+    std::vector<int> zcols, zpos, ext_zcol;
+    int ncols = 0, ldz = 0;
+    bool gather = false;
+    DevBuf d_vu_rc, d_ext_idx, d_ext_child, d_ext_cport, d_seams, d_seam_list, d_seam_off, d_zcols, d_zpos, d_ext_zcol;
+    DevBuf DU, SU;
+};
+
+struct LevelPlan {
+    int M = 1;
+    std::vector<int4> items;   // (W, q, first slot, count)
+    std::vector<int4> groups;  // (query, 0, glist offset, count)
+    std::vector<int> glist;    // slot indices
+    std::vector<int2> rows;    // (group, row position)
+};
+
+struct QueryPlan {
+    std::vector<int> q_k, q_vp;
+    std::vector<long long> q_out;
+    std::vector<int2> vp_rc;
+    std::vector<uint8_t> vp_valid;
+    std::vector<int2> slot_rc;
+    std::vector<int> slot_q, slot_i;
+    std::vector<LevelPlan> levels;
+    void clear() {
+        q_k.clear();
+        q_vp.clear();
+        q_out.clear();
+        vp_rc.clear();
+        vp_valid.clear();
+        slot_rc.clear();
+        slot_q.clear();
+        slot_i.clear();
+        for (auto& l : levels) {
+            l.items.clear();
+            l.groups.clear();
+            l.glist.clear();
+            l.rows.clear();
+        }
+    }
+};
+
+// Per-stream query workspace (grow-only, like HTripQueryScratch,
+// include/htrip/minplus_oracle.hpp:189-238).
+struct QueryWS {
+    hipStream_t stream = nullptr;
+    QueryPlan plan;
+    HostBuf h_plan, h_out;
+    DevBuf d_plan, X[2], Z, d_out;
+    int init();
+    ~QueryWS();
+};
+
+struct Map {
+    htg_ctx* ctx = nullptr;
+    htg_config cfg{};
+    int b = 16, g = 2, rows = 0, cols = 0, S = 0, H = 0, nl = 0;
+    std::vector<uint8_t> cells_code;  // rows*cols cell codes as last known
+    std::vector<uint8_t> pass;        // S*S passable flags (host mirror of d_pass)
+    DevBuf d_pass, leafD, leafS;
+    std::vector<LevelGeo> lv;  // lv[l-1] is level l
+    hipStream_t stream = nullptr;
+    bool built = false;
+    uint64_t epoch = 0;
+    htg_batch_stats stats{0, 0, 0};
+    uint64_t budget = 0, device_bytes = 0;
+    htg_tuning tune{64, 16, 0, 0};
+    QueryWS qws;
+    DevBuf ws_nodes, ws_flags, ws_piv, ws_counts, ws_R, ws_changed, ws_mask, ws_edges, ws_cells, ws_leaves,
+        ws_leaf_changed;
+
+    int build();
+    int open_cells(const int32_t* rc, int n);
+    int query(const int32_t* rc, const int32_t* ks, int nq, uint16_t* out_host, QueryWS& ws);
+    int ensure_build_ws(int lvl);
+    int run_closure(int lvl, int n_list, const std::vector<int>& hcounts);
+    const uint16_t* child_summaries(int lvl, size_t& stride) const;
+};
+
+}  // namespace htg
+
+struct htg_map : htg::Map {};

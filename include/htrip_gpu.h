@@ -1,0 +1,142 @@
+/*
+ * htrip_gpu.h -- C ABI of the MI355X (gfx950) min-plus distance oracle.
+ *
+ * This is the drop-in boundary beneath HTRIP's MinPlusOracle
+ * (include/htrip/minplus_oracle.hpp in the reference).  Plain pointers and
+ * sizes only; no C++ types, no exceptions.  Every call returns an int status:
+ * HTG_OK (0) or a negative HTG_E_* code; htg_last_error() gives the message
+ * of the last failure on the calling thread.
+ *
+ * Arithmetic is the reference's tropical semiring on uint16 distances:
+ * INF = 60000, saturating add, min (include/htrip/types.hpp:34,45,66-70).
+ * Cell codes follow include/htrip/types.hpp:101-105: 0 Unknown, 1 Free,
+ * 2 Obstacle; only Free cells are passable (include/htrip/grid_map.hpp:142-145).
+ *
+ * Threading: a map handle is used by one host thread at a time (like the
+ * reference, whose const queries share mutable scratch,
+ * include/htrip/minplus_oracle.hpp:240-242).  Different handles may be used
+ * concurrently from different threads.  The server API owns its maps'
+ * ordering itself.
+ */
+#ifndef HTRIP_GPU_H
+#define HTRIP_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HTG_INF 60000
+
+enum {
+    HTG_OK = 0,
+    HTG_E_INVALID = -1, /* bad argument or configuration */
+    HTG_E_NOMEM = -2,   /* admission refused: HBM budget or device memory */
+    HTG_E_HIP = -3,     /* HIP runtime error */
+    HTG_E_STATE = -4,   /* call not valid in the handle's state (e.g. query before build) */
+    HTG_E_RANGE = -5    /* index out of range */
+};
+
+typedef struct htg_ctx htg_ctx;
+typedef struct htg_map htg_map;
+typedef struct htg_server htg_server;
+
+/* include/htrip/types.hpp:199-225 (HierarchyConfig): tile_size_b, group_w,
+ * group_h, hybrid_k_threshold.  group_w must equal group_h. */
+typedef struct htg_config {
+    int32_t tile_size_b;
+    int32_t group_w;
+    int32_t group_h;
+    int32_t hybrid_k_threshold;
+} htg_config;
+
+/* Geometry and footprint of a built map (include/htrip/minplus_oracle.hpp:41-47). */
+typedef struct htg_map_info {
+    int32_t rows, cols;
+    int32_t S, H;              /* root side b*g^H >= max(rows, cols); hierarchy depth */
+    int32_t num_leaf_tiles_x;  /* = num_leaf_tiles_y = S / b */
+    int32_t num_leaves;
+    int32_t built;
+    int32_t reserved;
+    uint64_t total_memory_bytes; /* reference accounting, src/minplus_oracle.cpp:2116-2134 */
+    uint64_t device_bytes;       /* HBM actually held by this map */
+} htg_map_info;
+
+/* Counters of the last update call (include/htrip/minplus_oracle.hpp:251-256). */
+typedef struct htg_batch_stats {
+    int32_t n_touched_leaves;
+    int32_t n_dirty_ancestors;
+    int32_t n_pivots;
+} htg_batch_stats;
+
+/* ---- context --------------------------------------------------------- */
+int htg_ctx_create(int device, htg_ctx** out);
+int htg_ctx_destroy(htg_ctx* ctx);
+const char* htg_last_error(void);
+/* Free / total HBM of the context's device, in bytes. */
+int htg_ctx_mem_info(htg_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
+
+/* ---- map lifecycle ---------------------------------------------------- */
+/* Replaces MinPlusOracle(HierarchyConfig) (minplus_oracle.hpp:59) plus the
+ * GridMap it will be built from.  cells: rows*cols bytes (0/1/2 codes).
+ * hbm_budget: bytes the map may hold on the device, 0 = no limit beyond the
+ * device's free memory; a map that would exceed it is refused with
+ * HTG_E_NOMEM (admission by memory). */
+int htg_map_create(htg_ctx* ctx, const htg_config* cfg, int32_t rows, int32_t cols, const uint8_t* cells,
+                   uint64_t hbm_budget, htg_map** out);
+int htg_map_destroy(htg_map* map);
+/* Deep copy of device state: the reference oracle is copied by value by its
+ * callers (apps/bench_update_latency.cpp:178, src/robot_simulator.cpp:160). */
+int htg_map_clone(const htg_map* src, uint64_t hbm_budget, htg_map** out);
+/* Bytes a map of this shape would hold on the device (for admission planning). */
+int htg_map_footprint(const htg_config* cfg, int32_t rows, int32_t cols, uint64_t* bytes);
+int htg_map_info_get(const htg_map* map, htg_map_info* out);
+
+/* ---- hot path ----------------------------------------------------------- */
+/* MinPlusOracle::build (src/minplus_oracle.cpp:48-137).  Returns
+ * HTG_E_INVALID where the reference returns false (invalid config, empty grid). */
+int htg_build(htg_map* map);
+/* MinPlusOracle::onBatchCellsOpened (src/minplus_oracle.cpp:530-729).  rc holds
+ * n (row, col) pairs.  The listed cells become Free (the reference's caller sets
+ * them Free before the call, README.md:71-91); out-of-range cells are ignored
+ * (src/minplus_oracle.cpp:546-547). */
+int htg_open_cells(htg_map* map, const int32_t* rc, int32_t n);
+/* MinPlusOracle::onCellOpened (src/minplus_oracle.cpp:139-177). */
+int htg_open_cell(htg_map* map, int32_t r, int32_t c);
+/* MinPlusOracle::queryAllPairs (src/minplus_oracle.cpp:934-1426).  rc holds k
+ * (row, col) viewpoints; out receives k*k distances, row-major, host memory.
+ * Impassable or out-of-range viewpoints give an INF row and column, INF
+ * diagonal (src/minplus_oracle.cpp:961-965, :993-997). */
+int htg_query_all_pairs(htg_map* map, const int32_t* rc, int32_t k, uint16_t* out);
+/* Same, for nq queries in one call: query q has ks[q] viewpoints (packed back to
+ * back in rc) and its k*k result at out + sum_{p<q} ks[p]^2.  One coalesced set
+ * of launches serves all of them. */
+int htg_query_batch(htg_map* map, const int32_t* rc, const int32_t* ks, int32_t nq, uint16_t* out);
+int htg_last_batch_stats(const htg_map* map, htg_batch_stats* out);
+int htg_sync(htg_map* map);
+
+/* ---- table access (the reference keeps these as public members:
+ * LeafTile::D/S, InternalNode::D_U/S_U, include/htrip/leaf_tile.hpp,
+ * include/htrip/internal_node.hpp).  Copies to host memory. --------------- */
+int htg_read_leaf_D(htg_map* map, int32_t leaf, uint16_t* out);   /* b^2 x b^2 */
+int htg_read_leaf_S(htg_map* map, int32_t leaf, uint16_t* out);   /* (4b-4)^2 */
+int htg_node_dims(const htg_map* map, int32_t lvl, int32_t* num_nodes, int32_t* nV, int32_t* nExt);
+int htg_read_node_DU(htg_map* map, int32_t lvl, int32_t idx, uint16_t* out); /* nV x nV */
+int htg_read_node_SU(htg_map* map, int32_t lvl, int32_t idx, uint16_t* out); /* nExt x nExt */
+
+/* ---- build tuning (the project's autotuner picks these) --------------- */
+typedef struct htg_tuning {
+    int32_t pivot_block;   /* pivots per closure block: 16, 32 or 64 */
+    int32_t zq_tile;       /* viewpoints per query-projection work item: 1..16 */
+    int32_t use_graph;     /* capture per-level build launches in a hipGraph */
+    int32_t reserved;
+} htg_tuning;
+int htg_get_tuning(const htg_map* map, htg_tuning* out);
+int htg_set_tuning(htg_map* map, const htg_tuning* t);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HTRIP_GPU_H */
