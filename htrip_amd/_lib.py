@@ -23,8 +23,15 @@ EXPORTS = [
     "htg_build", "htg_open_cells", "htg_open_cell", "htg_query_all_pairs", "htg_query_batch",
     "htg_last_batch_stats", "htg_sync",
     "htg_read_leaf_D", "htg_read_leaf_S", "htg_node_dims", "htg_read_node_DU", "htg_read_node_SU",
+    "htg_profile_enable", "htg_profile_get",
     "htg_get_tuning", "htg_set_tuning",
 ]
+
+K_LEAF_APSP, K_CLOSURE_PIVOTS, K_CLOSURE_UPDATE, K_Q_PROJECT, K_Q_MEET, K_Q_LIFT = range(6)
+KERNEL_NAMES = {
+    K_LEAF_APSP: "k_leaf_apsp", K_CLOSURE_PIVOTS: "k_closure_pivots", K_CLOSURE_UPDATE: "k_closure_update",
+    K_Q_PROJECT: "k_q_z", K_Q_MEET: "k_q_meet", K_Q_LIFT: "k_q_lift",
+}
 
 
 class HtgError(RuntimeError):
@@ -46,6 +53,11 @@ class MapInfo(C.Structure):
 
 class BatchStats(C.Structure):
     _fields_ = [("n_touched_leaves", C.c_int32), ("n_dirty_ancestors", C.c_int32), ("n_pivots", C.c_int32)]
+
+
+class KernelStats(C.Structure):
+    _fields_ = [("launches", C.c_uint64), ("total_ms", C.c_double), ("bytes", C.c_double),
+                ("updates", C.c_double)]
 
 
 class Tuning(C.Structure):
@@ -88,6 +100,8 @@ def load(path: str = LIB_PATH):
         "htg_node_dims": (C.c_int, [vp, i32, pi32, pi32, pi32]),
         "htg_read_node_DU": (C.c_int, [vp, i32, i32, pu16]),
         "htg_read_node_SU": (C.c_int, [vp, i32, i32, pu16]),
+        "htg_profile_enable": (C.c_int, [vp, C.c_int]),
+        "htg_profile_get": (C.c_int, [vp, C.c_int, C.POINTER(KernelStats)]),
         "htg_get_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
         "htg_set_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
     }
@@ -291,3 +305,11 @@ class GpuMap:
 
     def sync(self) -> None:
         check(self.lib.htg_sync(self.h))
+
+    def profile(self, on: bool = True) -> None:
+        check(self.lib.htg_profile_enable(self.h, 1 if on else 0))
+
+    def kernel_stats(self, cls: int) -> dict:
+        s = KernelStats()
+        check(self.lib.htg_profile_get(self.h, cls, C.byref(s)))
+        return {"launches": int(s.launches), "total_ms": s.total_ms, "bytes": s.bytes, "updates": s.updates}

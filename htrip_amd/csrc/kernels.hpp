@@ -63,11 +63,6 @@ void launch_project_su(int nV, int ld, int ne, const int* ext_idx, const int* no
 void launch_set_pass(uint8_t* pass, int S, const int2* cells, int n, hipStream_t st);
 
 // ---- query -----------------------------------------------------------------
-struct QuerySlots {           // one entry per valid viewpoint, sorted by tile key
-    const int2* rc;           // absolute (r, c)
-    int n;
-};
-
 // out matrices: INF, 0 on the diagonal of valid viewpoints, leaf D for
 // same-leaf pairs.  vp_* arrays are per viewpoint (all queries packed).
 void launch_q_init(int nq, const int* q_k, const long long* q_out, const int* q_vp, const int2* vp_rc,
@@ -75,19 +70,22 @@ void launch_q_init(int nq, const int* q_k, const long long* q_out, const int* q_
                    hipStream_t st);
 // X0[slot][p] = leaf D row of the slot's cell at port p.
 void launch_q_leaf(int n, const int2* slot_rc, int b, int nl, const uint16_t* leafD, uint16_t* X0, hipStream_t st);
-// Z[slot][x] = min_{a in seam(q)} X[slot][a] (+) D_W[q*pc + a][col(x)] for work
-// items (node W, quad q, first slot, count).
-void launch_q_z(int M, int nV, int ld, int pc, const uint16_t* DU, size_t node_stride, const int* seam_list,
-                const int* seam_off, const int* cols, int ncols, int ldz, bool gather, const int4* items,
-                int n_items, const uint16_t* X, uint16_t* Z, hipStream_t st);
-// Meet: for row items (group, row position) update out[i][j] for all j in the
-// group: min_{s in seam(q_j)} Z[i][zpos_q_j[s]] (+) X[j][seam_q_j[s]].
-void launch_q_meet(int pc, int cs, int g, const int* seam_list, const int* seam_off, const int* zpos, int ldz,
-                   const int2* row_items, int n_rows, const int4* groups, const int* glist, const int2* slot_rc,
-                   const int* slot_q, const int* slot_i, const int* q_k, const long long* q_out, const uint16_t* X,
+// Projection partials: Zp[chunk][slot][col] = min over the chunk's seam rows a
+// of X[slot][a] (+) D_W[q*pc + a][cols(col)] for work items (W, q, first
+// slot, count); ksplit row chunks per item.
+void launch_q_z(int M, int ld, int pc, const uint16_t* DU, size_t node_stride, const int* seam_list,
+                const int* seam_off, const int* cols, int ncols, int ldp, const int4* items, int n_items, int ksplit,
+                const uint16_t* X, uint16_t* Zp, size_t part_stride, hipStream_t st);
+// Z[slot][pos] = min_c Zp[c][slot][src[pos]] (INF where src[pos] < 0).
+void launch_q_zred(int n, int nz, int ldz, const int* src, int ldp, int ksplit, const uint16_t* Zp,
+                   size_t part_stride, uint16_t* Z, hipStream_t st);
+// Meet tiles (8 ints each: group, q2, i0, j0, j_end): out[i][j] = min(out,
+// min over the q2 seam block of Z[i] (+) Z[j]).
+void launch_q_meet(const int* tiles, int n_tiles, const int4* groups, const int* glist, const int* zoff,
+                   const int* zlen, int ldz, const int* slot_i, const int* q_k, const long long* q_out,
                    const uint16_t* Z, uint16_t* out, hipStream_t st);
-// Lift: X_l[slot][e] = min(Z[slot][colpos(ext_idx[e])], X_{l-1}[slot][own port of e]).
-void launch_q_lift(int n, int ne, const int* ext_zcol, const int* ext_child, const int* ext_cport, int pc, int cs,
+// Lift: X_l[slot][e] = min(Z[slot][ext_zpos[e]], X_{l-1}[slot][own port of e]).
+void launch_q_lift(int n, int ne, const int* ext_zpos, const int* ext_child, const int* ext_cport, int pc, int cs,
                    int g, int ldz, const int2* slot_rc, const uint16_t* Xprev, const uint16_t* Z, uint16_t* Xnext,
                    hipStream_t st);
 

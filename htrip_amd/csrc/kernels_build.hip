@@ -347,16 +347,24 @@ __global__ __launch_bounds__(256) void k_closure_pivots(int nV, int ld, const in
     uint32_t acc[P];
 #pragma unroll
     for (int k = 0; k < P; ++k) acc[k] = kInf2;
-    for (int k2 = 0; k2 < p; ++k2) {
-        const uint32_t dv = *reinterpret_cast<const uint32_t*>(D + size_t(kr[k2]) * ld + j0);
-        const uint4* c4 = reinterpret_cast<const uint4*>(&CdT[k2][0]);
+    constexpr int PF = 8;  // pivot-row loads in flight
+    for (int k0 = 0; k0 < p; k0 += PF) {
+        uint32_t dv[PF];
 #pragma unroll
-        for (int k = 0; k < P; k += 4) {
-            const uint4 cv = c4[k / 4];
-            acc[k + 0] = relax2(acc[k + 0], cv.x, dv);
-            acc[k + 1] = relax2(acc[k + 1], cv.y, dv);
-            acc[k + 2] = relax2(acc[k + 2], cv.z, dv);
-            acc[k + 3] = relax2(acc[k + 3], cv.w, dv);
+        for (int u = 0; u < PF; ++u)
+            dv[u] = *reinterpret_cast<const uint32_t*>(D + size_t(kr[k0 + u]) * ld + j0);  // kr[k >= p] = row 0
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+            // CdT rows k2 >= p hold INF off the diagonal; acc[k >= p] is never stored
+            const uint4* c4 = reinterpret_cast<const uint4*>(&CdT[k0 + u][0]);
+#pragma unroll
+            for (int k = 0; k < P; k += 4) {
+                const uint4 cv = c4[k / 4];
+                acc[k + 0] = relax2(acc[k + 0], cv.x, dv[u]);
+                acc[k + 1] = relax2(acc[k + 1], cv.y, dv[u]);
+                acc[k + 2] = relax2(acc[k + 2], cv.z, dv[u]);
+                acc[k + 3] = relax2(acc[k + 3], cv.w, dv[u]);
+            }
         }
     }
     uint16_t* R = Rbuf + size_t(slot) * P * ld;

@@ -75,6 +75,59 @@ void HostBuf::release() {
     bytes = 0;
 }
 
+hipEvent_t Profiler::get() {
+    if (!pool.empty()) {
+        hipEvent_t e = pool.back();
+        pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return e;
+}
+int Profiler::begin(int cls, hipStream_t st, double bytes, double updates) {
+    if (!on) return -1;
+    Pending p{cls, get(), get(), bytes, updates};
+    if (!p.a || !p.b) return -1;
+    (void)hipEventRecord(p.a, st);
+    pending.push_back(p);
+    return int(pending.size()) - 1;
+}
+void Profiler::end(int idx, hipStream_t st) {
+    if (idx < 0) return;
+    (void)hipEventRecord(pending[idx].b, st);
+}
+void Profiler::harvest() {
+    for (auto& p : pending) {
+        float ms = 0.f;
+        if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            htg_kernel_stats& a = acc[p.cls];
+            a.launches++;
+            a.total_ms += ms;
+            a.bytes += p.bytes;
+            a.updates += p.updates;
+        }
+        (void)hipGetLastError();
+        pool.push_back(p.a);
+        pool.push_back(p.b);
+    }
+    pending.clear();
+}
+void Profiler::reset() {
+    harvest();
+    for (auto& a : acc) a = htg_kernel_stats{0, 0.0, 0.0, 0.0};
+}
+Profiler::~Profiler() {
+    for (auto& p : pending) {
+        pool.push_back(p.a);
+        pool.push_back(p.b);
+    }
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+}
+
 // ---------------------------------------------------------------------------
 // Level geometry templates (relative to the node origin), shared by every
 // node of a level.  Follows src/internal_node.cpp:13-92.
@@ -138,19 +191,48 @@ static void make_level(LevelGeo& L, int lvl, int b, int g, int S, int H) {
         L.zcols.resize(d.nV);
         std::iota(L.zcols.begin(), L.zcols.end(), 0);
     } else {
+        // seam ports, widened to aligned groups of 4 consecutive V_U indices
+        std::vector<int> grp;
         for (int q = 0; q < g * g; ++q)
-            for (int i = L.seam_off[q]; i < L.seam_off[q + 1]; ++i) L.zcols.push_back(q * d.pc + L.seam_list[i]);
-        std::sort(L.zcols.begin(), L.zcols.end());
+            for (int i = L.seam_off[q]; i < L.seam_off[q + 1]; ++i) grp.push_back((q * d.pc + L.seam_list[i]) / 4);
+        std::sort(grp.begin(), grp.end());
+        grp.erase(std::unique(grp.begin(), grp.end()), grp.end());
+        for (int g4 : grp)
+            for (int e = 0; e < 4; ++e) L.zcols.push_back(4 * g4 + e);
     }
     L.ncols = int(L.zcols.size());
-    L.ldz = round_up(std::max(L.ncols, 8), 8);
-    std::vector<int> colpos(d.nV, -1);
-    for (int i = 0; i < L.ncols; ++i) colpos[L.zcols[i]] = i;
-    L.zpos.resize(L.seam_list.size());
-    for (int q = 0; q < g * g; ++q)
-        for (int i = L.seam_off[q]; i < L.seam_off[q + 1]; ++i) L.zpos[i] = colpos[q * d.pc + L.seam_list[i]];
-    L.ext_zcol.resize(d.ne);
-    for (int e = 0; e < d.ne; ++e) L.ext_zcol[e] = colpos[L.ext_idx[e]] < 0 ? 0 : colpos[L.ext_idx[e]];
+    L.ldp = round_up(std::max(L.ncols, 8), 8);
+    std::vector<int> natcol(d.nV, -1), zpos_of(d.nV, -1);
+    for (int i = 0; i < L.ncols; ++i) natcol[L.zcols[i]] = i;
+    L.zsrc.clear();
+    L.zoff.assign(g * g, 0);
+    L.zlen.assign(g * g, 0);
+    L.nsmax = 0;
+    for (int q = 0; q < g * g; ++q) {
+        L.zoff[q] = int(L.zsrc.size());
+        const int ns = L.seam_off[q + 1] - L.seam_off[q];
+        L.nsmax = std::max(L.nsmax, ns);
+        for (int i = L.seam_off[q]; i < L.seam_off[q + 1]; ++i) {
+            const int v = q * d.pc + L.seam_list[i];
+            zpos_of[v] = int(L.zsrc.size());
+            L.zsrc.push_back(natcol[v]);
+        }
+        if (ns & 1) L.zsrc.push_back(-1);
+        L.zlen[q] = int(L.zsrc.size()) - L.zoff[q];
+    }
+    L.ext_zpos.assign(d.ne, 0);
+    if (!L.gather) {
+        for (int e = 0; e < d.ne; ++e) {
+            const int v = L.ext_idx[e];
+            if (zpos_of[v] < 0) {
+                zpos_of[v] = int(L.zsrc.size());
+                L.zsrc.push_back(natcol[v]);
+            }
+            L.ext_zpos[e] = zpos_of[v];
+        }
+    }
+    L.nz = int(L.zsrc.size());
+    L.ldz = round_up(std::max(L.nz, 8), 8);
 }
 
 template <typename T>
@@ -217,12 +299,28 @@ int Map::run_closure(int lvl, int n_list, const std::vector<int>& hcounts) {
     const int blocks = (maxc + P - 1) / P;
     const int* nodes = static_cast<const int*>(ws_nodes.p);
     for (int t = 0; t < blocks; ++t) {
+        double upd_p = 0, upd_u = 0, by_p = 0, by_u = 0;
+        if (prof.on) {
+            const double nV = L.d.nV;
+            for (int i = 0; i < n_list; ++i) {
+                const double p = std::max(0, std::min(P, hcounts[i] - t * P));
+                if (p <= 0) continue;
+                upd_p += p * p * nV + p * p * p;
+                by_p += (p * nV + p * nV) * 2;  // pivot rows in, R out
+                upd_u += p * nV * nV;
+                by_u += (2 * nV * nV + 2 * p * nV) * 2;  // D_U tile in + out, R in
+            }
+        }
+        int pi = prof.begin(HTG_K_CLOSURE_PIVOTS, stream, by_p, upd_p);
         launch_closure_pivots(P, L.d.nV, L.d.ld, nodes, n_list, static_cast<const int*>(ws_piv.p), L.d.nV,
                               static_cast<const int*>(ws_counts.p), t, static_cast<const uint16_t*>(L.DU.p),
                               L.node_stride, static_cast<uint16_t*>(ws_R.p), stream);
+        prof.end(pi, stream);
+        pi = prof.begin(HTG_K_CLOSURE_UPDATE, stream, by_u, upd_u);
         launch_closure_update(P, L.d.nV, L.d.ld, nodes, n_list, static_cast<const int*>(ws_counts.p), t,
                               static_cast<uint16_t*>(L.DU.p), L.node_stride, static_cast<const uint16_t*>(ws_R.p),
                               stream);
+        prof.end(pi, stream);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "closure launch");
@@ -241,8 +339,11 @@ const uint16_t* Map::child_summaries(int lvl, size_t& stride) const {
 
 int Map::build() {
     hipError_t e;
+    const double leaf_bytes = (double(b) * b * b * b + double(4 * b - 4) * (4 * b - 4)) * 2 + double(b) * b;
+    int pi = prof.begin(HTG_K_LEAF_APSP, stream, leaf_bytes * nl * nl, 0);
     launch_leaf_apsp(static_cast<const uint8_t*>(d_pass.p), S, b, nl, nullptr, nl * nl, static_cast<uint16_t*>(leafD.p),
                      static_cast<uint16_t*>(leafS.p), nullptr, stream);
+    prof.end(pi, stream);
     if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "leaf apsp launch");
     for (int lvl = 1; lvl <= H; ++lvl) {
         LevelGeo& L = lv[lvl - 1];
@@ -275,6 +376,7 @@ int Map::build() {
         if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "level launch");
     }
     if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "build sync");
+    prof.harvest();
     built = true;
     epoch++;
     return HTG_OK;
@@ -390,9 +492,12 @@ int Map::open_cells(const int32_t* rcs, int n) {
         return hip_fail(e, "upload leaves");
     if ((e = hipMemsetAsync(ws_leaf_changed.p, 0, size_t(nd) * 4, stream)) != hipSuccess)
         return hip_fail(e, "clear leaf flags");
+    const double leaf_bytes = (double(b) * b * b * b + double(4 * b - 4) * (4 * b - 4)) * 2 + double(b) * b;
+    int lpi = prof.begin(HTG_K_LEAF_APSP, stream, leaf_bytes * nd, 0);
     launch_leaf_apsp(static_cast<const uint8_t*>(d_pass.p), S, b, nl, static_cast<const int*>(ws_leaves.p), nd,
                      static_cast<uint16_t*>(leafD.p), static_cast<uint16_t*>(leafS.p),
                      static_cast<int*>(ws_leaf_changed.p), stream);
+    prof.end(lpi, stream);
     stats.n_touched_leaves = nd;
     stats.n_pivots = piv;
     if (H == 0) {
@@ -499,6 +604,8 @@ int Map::open_cells(const int32_t* rcs, int n) {
         mask = std::move(next_mask);
     }
     stats.n_dirty_ancestors = anc;
+    if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "update sync");
+    prof.harvest();
     epoch++;
     return HTG_OK;
 }
@@ -592,9 +699,18 @@ int Map::query(const int32_t* rcs, const int32_t* ks, int nq, uint16_t* out_host
             const int W = (rc0.x / L.d.s) * L.d.nt + rc0.y / L.d.s;
             const int q = int((sk[s].key / div) % unsigned(g * g));
             for (int a = s; a < t; a += M) LP.items.push_back(make_int4(W, q, a, std::min(M, t - a)));
+            const double nsq = L.seam_off[q + 1] - L.seam_off[q];
+            LP.z_bytes += (nsq * L.ncols + double(t - s) * (nsq + L.ncols)) * 2;
+            LP.z_updates += double(t - s) * nsq * L.ncols;
             s = t;
         }
-        // (query, W) groups -> meet rows
+        {  // split the seam rows until the launch has enough wavefronts
+            const long long base = (long long)LP.items.size() * ((L.ncols + 255) / 256);
+            int ks = base > 0 ? int((2048 + base - 1) / base) : 1;
+            ks = std::min(ks, std::max(1, L.nsmax / 32));
+            LP.ksplit = std::max(1, ks);
+        }
+        // (query, W) groups -> meet tiles per target child q2
         const unsigned long long divW = div * unsigned(g * g);
         for (int s = 0; s < ns;) {
             int t = s + 1;
@@ -602,11 +718,25 @@ int Map::query(const int32_t* rcs, const int32_t* ks, int nq, uint16_t* out_host
             std::map<int, std::vector<int>> byq;
             for (int a = s; a < t; ++a) byq[sk[a].q].push_back(a);
             for (auto& kv : byq) {
-                if (kv.second.size() < 2) continue;
+                const int n = int(kv.second.size());
+                if (n < 2) continue;
                 const int gi = int(LP.groups.size());
-                LP.groups.push_back(make_int4(kv.first, 0, int(LP.glist.size()), int(kv.second.size())));
-                for (size_t a = 0; a < kv.second.size(); ++a) LP.rows.push_back(make_int2(gi, int(a)));
+                const int goff = int(LP.glist.size());
+                LP.groups.push_back(make_int4(kv.first, 0, goff, n));
                 LP.glist.insert(LP.glist.end(), kv.second.begin(), kv.second.end());
+                for (int j0 = 0; j0 < n;) {
+                    const int q2 = int((sk[kv.second[j0]].key / div) % unsigned(g * g));
+                    int je = j0 + 1;
+                    while (je < n && int((sk[kv.second[je]].key / div) % unsigned(g * g)) == q2) ++je;
+                    for (int i0 = 0; i0 < n; i0 += 16)
+                        for (int jj = j0; jj < je; jj += 16) {
+                            const int tt[8] = {gi, q2, i0, jj, je, 0, 0, 0};
+                            LP.mtiles.insert(LP.mtiles.end(), tt, tt + 8);
+                            LP.m_updates += double(std::min(16, n - i0)) * std::min(16, je - jj) * L.zlen[q2];
+                            LP.m_bytes += double(std::min(16, n - i0) + std::min(16, je - jj)) * L.zlen[q2] * 2;
+                        }
+                    j0 = je;
+                }
             }
             s = t;
         }
@@ -615,14 +745,16 @@ int Map::query(const int32_t* rcs, const int32_t* ks, int nq, uint16_t* out_host
     // ---- upload (one packed staging buffer)
     Packer pk;
     const size_t o_qk = pk.add(P.q_k), o_qout = pk.add(P.q_out), o_qvp = pk.add(P.q_vp), o_vprc = pk.add(P.vp_rc),
-                 o_vpvalid = pk.add(P.vp_valid), o_src = pk.add(P.slot_rc), o_sq = pk.add(P.slot_q),
+                 o_vpvalid = pk.add(P.vp_valid), o_src = pk.add(P.slot_rc),
                  o_si = pk.add(P.slot_i);
-    std::vector<size_t> o_items(H), o_groups(H), o_glist(H), o_rows(H);
+    std::vector<size_t> o_items(H), o_groups(H), o_glist(H), o_tiles(H);
+    size_t maxpart = 0;
     for (int l = 0; l < H; ++l) {
         o_items[l] = pk.add(P.levels[l].items);
         o_groups[l] = pk.add(P.levels[l].groups);
         o_glist[l] = pk.add(P.levels[l].glist);
-        o_rows[l] = pk.add(P.levels[l].rows);
+        o_tiles[l] = pk.add(P.levels[l].mtiles);
+        maxpart = std::max(maxpart, size_t(P.levels[l].ksplit) * lv[l].ldp);
     }
     if ((rc = ws.h_plan.ensure(pk.size))) return rc;
     pk.write(static_cast<char*>(ws.h_plan.p));
@@ -636,6 +768,7 @@ int Map::query(const int32_t* rcs, const int32_t* ks, int nq, uint16_t* out_host
     if ((rc = ws.X[0].ensure(nsl * maxpc * 2))) return rc;
     if ((rc = ws.X[1].ensure(nsl * maxpc * 2))) return rc;
     if ((rc = ws.Z.ensure(nsl * maxldz * 2))) return rc;
+    if ((rc = ws.Zp.ensure(nsl * std::max<size_t>(maxpart, 8) * 2))) return rc;
     if ((rc = ws.d_out.ensure(size_t(total_out) * 2))) return rc;
     if ((rc = ws.h_out.ensure(size_t(total_out) * 2))) return rc;
     hipStream_t st = ws.stream;
@@ -656,18 +789,26 @@ int Map::query(const int32_t* rcs, const int32_t* ks, int nq, uint16_t* out_host
     for (int l = 1; l <= H && ns > 0; ++l) {
         const LevelGeo& L = lv[l - 1];
         const LevelPlan& LP = P.levels[l - 1];
-        launch_q_z(LP.M, L.d.nV, L.d.ld, L.d.pc, static_cast<const uint16_t*>(L.DU.p), L.node_stride,
+        int pi = prof.begin(HTG_K_Q_PROJECT, st, LP.z_bytes, LP.z_updates);
+        const size_t part_stride = nsl * L.ldp;
+        launch_q_z(LP.M, L.d.ld, L.d.pc, static_cast<const uint16_t*>(L.DU.p), L.node_stride,
                    static_cast<const int*>(L.d_seam_list.p), static_cast<const int*>(L.d_seam_off.p),
-                   static_cast<const int*>(L.d_zcols.p), L.ncols, L.ldz, L.gather,
-                   reinterpret_cast<const int4*>(dp + o_items[l - 1]), int(LP.items.size()), Xa, Z, st);
-        launch_q_meet(L.d.pc, L.d.cs, g, static_cast<const int*>(L.d_seam_list.p),
-                      static_cast<const int*>(L.d_seam_off.p), static_cast<const int*>(L.d_zpos.p), L.ldz,
-                      reinterpret_cast<const int2*>(dp + o_rows[l - 1]), int(LP.rows.size()),
-                      reinterpret_cast<const int4*>(dp + o_groups[l - 1]), I(o_glist[l - 1]), srcs, I(o_sq), I(o_si),
-                      I(o_qk), reinterpret_cast<const long long*>(dp + o_qout), Xa, Z, dout, st);
+                   static_cast<const int*>(L.d_zcols.p), L.ncols, L.ldp,
+                   reinterpret_cast<const int4*>(dp + o_items[l - 1]), int(LP.items.size()), LP.ksplit, Xa,
+                   static_cast<uint16_t*>(ws.Zp.p), part_stride, st);
+        launch_q_zred(ns, L.nz, L.ldz, static_cast<const int*>(L.d_zsrc.p), L.ldp, LP.ksplit,
+                      static_cast<const uint16_t*>(ws.Zp.p), part_stride, Z, st);
+        prof.end(pi, st);
+        pi = prof.begin(HTG_K_Q_MEET, st, LP.m_bytes, LP.m_updates);
+        launch_q_meet(I(o_tiles[l - 1]), int(LP.mtiles.size() / 8), reinterpret_cast<const int4*>(dp + o_groups[l - 1]),
+                      I(o_glist[l - 1]), static_cast<const int*>(L.d_zoff.p), static_cast<const int*>(L.d_zlen.p),
+                      L.ldz, I(o_si), I(o_qk), reinterpret_cast<const long long*>(dp + o_qout), Z, dout, st);
+        prof.end(pi, st);
         if (l < H) {
-            launch_q_lift(ns, L.d.ne, static_cast<const int*>(L.d_ext_zcol.p), static_cast<const int*>(L.d_ext_child.p),
+            pi = prof.begin(HTG_K_Q_LIFT, st, double(ns) * (2.0 * L.d.ne + L.d.pc) * 2, 0);
+            launch_q_lift(ns, L.d.ne, static_cast<const int*>(L.d_ext_zpos.p), static_cast<const int*>(L.d_ext_child.p),
                           static_cast<const int*>(L.d_ext_cport.p), L.d.pc, L.d.cs, g, L.ldz, srcs, Xa, Z, Xb, st);
+            prof.end(pi, st);
             std::swap(Xa, Xb);
         }
     }
@@ -678,6 +819,7 @@ int Map::query(const int32_t* rcs, const int32_t* ks, int nq, uint16_t* out_host
             return hip_fail(e, "download result");
         if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(e, "query sync");
         std::memcpy(out_host, ws.h_out.p, size_t(total_out) * 2);
+        prof.harvest();
     }
     return HTG_OK;
 }
@@ -810,7 +952,8 @@ int htg_map_create(htg_ctx* ctx, const htg_config* cfg, int32_t rows, int32_t co
             (rc = upload(L.d_ext_child, L.ext_child)) || (rc = upload(L.d_ext_cport, L.ext_cport)) ||
             (rc = upload(L.d_seams, L.seams)) || (rc = upload(L.d_seam_list, L.seam_list)) ||
             (rc = upload(L.d_seam_off, L.seam_off)) || (rc = upload(L.d_zcols, L.zcols)) ||
-            (rc = upload(L.d_zpos, L.zpos)) || (rc = upload(L.d_ext_zcol, L.ext_zcol)))
+            (rc = upload(L.d_zsrc, L.zsrc)) || (rc = upload(L.d_zoff, L.zoff)) || (rc = upload(L.d_zlen, L.zlen)) ||
+            (rc = upload(L.d_ext_zpos, L.ext_zpos)))
             return rc;
     }
     m->device_bytes = tables_bytes(b, g, m->S, m->H);
@@ -992,6 +1135,21 @@ int htg_read_node_SU(htg_map* map, int32_t lvl, int32_t idx, uint16_t* out) {
     const LevelGeo& L = map->lv[lvl - 1];
     const size_t ne = size_t(L.d.ne);
     return read_rows(map, static_cast<const uint16_t*>(L.SU.p) + size_t(idx) * ne * ne, ne, ne, ne, out);
+}
+
+int htg_profile_enable(htg_map* map, int on) {
+    if (!map) return fail(HTG_E_INVALID, "null map");
+    map->prof.reset();
+    map->prof.on = on != 0;
+    return HTG_OK;
+}
+
+int htg_profile_get(htg_map* map, int kernel_class, htg_kernel_stats* out) {
+    if (!map || !out) return fail(HTG_E_INVALID, "null argument");
+    if (kernel_class < 0 || kernel_class >= HTG_K_COUNT) return fail(HTG_E_RANGE, "kernel class");
+    map->prof.harvest();
+    *out = map->prof.acc[kernel_class];
+    return HTG_OK;
 }
 
 int htg_get_tuning(const htg_map* map, htg_tuning* out) {

@@ -77,19 +77,24 @@ struct LevelGeo {
     std::vector<int> ext_idx, ext_child, ext_cport;
     std::vector<int2> seams;
     std::vector<int> seam_list, seam_off;  // per-child seam ports
-    std::vector<int> zcols, zpos, ext_zcol;
-    int ncols = 0, ldz = 0;
+    // Query projection: natural columns (V_U indices) the projection computes,
+    // and the Z layout: per-child seam blocks (even-padded) then the other
+    // exterior ports.  zsrc[pos] = natural column of Z position pos or -1.
+    std::vector<int> zcols, zsrc, zoff, zlen, ext_zpos;
+    int ncols = 0, ldp = 0, nz = 0, ldz = 0, nsmax = 0;
     bool gather = false;
-    DevBuf d_vu_rc, d_ext_idx, d_ext_child, d_ext_cport, d_seams, d_seam_list, d_seam_off, d_zcols, d_zpos, d_ext_zcol;
+    DevBuf d_vu_rc, d_ext_idx, d_ext_child, d_ext_cport, d_seams, d_seam_list, d_seam_off, d_zcols, d_zsrc, d_zoff,
+        d_zlen, d_ext_zpos;
     DevBuf DU, SU;
 };
 
 struct LevelPlan {
-    int M = 1;
+    int M = 1, ksplit = 1;
+    std::vector<int> mtiles;   // meet tiles, 8 ints each
+    double z_bytes = 0, z_updates = 0, m_bytes = 0, m_updates = 0;
     std::vector<int4> items;   // (W, q, first slot, count)
     std::vector<int4> groups;  // (query, 0, glist offset, count)
     std::vector<int> glist;    // slot indices
-    std::vector<int2> rows;    // (group, row position)
 };
 
 struct QueryPlan {
@@ -110,10 +115,11 @@ struct QueryPlan {
         slot_q.clear();
         slot_i.clear();
         for (auto& l : levels) {
+            l.z_bytes = l.z_updates = l.m_bytes = l.m_updates = 0;
             l.items.clear();
+            l.mtiles.clear();
             l.groups.clear();
             l.glist.clear();
-            l.rows.clear();
         }
     }
 };
@@ -124,9 +130,30 @@ struct QueryWS {
     hipStream_t stream = nullptr;
     QueryPlan plan;
     HostBuf h_plan, h_out;
-    DevBuf d_plan, X[2], Z, d_out;
+    DevBuf d_plan, X[2], Z, Zp, d_out;
     int init();
     ~QueryWS();
+};
+
+// Event-bracketed launch timing (see htg_profile_enable).
+struct Profiler {
+    bool on = false;
+    struct Pending {
+        int cls;
+        hipEvent_t a, b;
+        double bytes, updates;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> pool;
+    htg_kernel_stats acc[HTG_K_COUNT]{};
+    hipEvent_t get();
+    // Records the start event; returns an index to pass to end().
+    int begin(int cls, hipStream_t st, double bytes, double updates);
+    void end(int idx, hipStream_t st);
+    // Accumulate every completed pair (call after the streams synchronized).
+    void harvest();
+    void reset();
+    ~Profiler();
 };
 
 struct Map {
@@ -144,6 +171,7 @@ struct Map {
     uint64_t budget = 0, device_bytes = 0;
     htg_tuning tune{64, 16, 0, 0};
     QueryWS qws;
+    Profiler prof;
     DevBuf ws_nodes, ws_flags, ws_piv, ws_counts, ws_R, ws_changed, ws_mask, ws_edges, ws_cells, ws_leaves,
         ws_leaf_changed;
 

@@ -1,0 +1,257 @@
+#pragma once
+// Drop-in replacement of the reference's include/htrip/minplus_oracle.hpp:
+// htrip::MinPlusOracle (alias HTripOracle) with the same hot-path signatures
+// (build, onCellOpened, onBatchCellsOpened, queryAllPairs with
+// HTripQueryScratch, queryDistance) and the public metadata callers read
+// (config, S, H, num_leaf_tiles_x/y, leaves.size(), depth(), findLcaLevel,
+// totalMemoryBytes, last_batch_stats).  The index itself lives in HBM behind
+// the C ABI of include/htrip_gpu.h; link with -lhtrip_gpu.
+//
+// Differences from the reference, all outside the results:
+//  * leaves / levels hold tile metadata only; tables are read with
+//    readLeafD / readNodeDU (copies from the device).
+//  * GPU failures throw std::runtime_error (the reference has no error channel
+//    besides build()'s bool).
+//  * hybrid_k_threshold is accepted and ignored: the GPU query is exact for
+//    every k, as the reference's BFS fallback is (minplus_oracle.cpp:951-958).
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "grid_map.hpp"
+#include "htrip_gpu.h"
+#include "types.hpp"
+
+namespace htrip {
+
+namespace detail {
+inline htg_ctx* device_context() {
+    static htg_ctx* ctx = [] {
+        htg_ctx* c = nullptr;
+        if (htg_ctx_create(0, &c) != HTG_OK) throw std::runtime_error(std::string("htrip: ") + htg_last_error());
+        return c;
+    }();
+    return ctx;
+}
+inline void check(int rc) {
+    if (rc != HTG_OK) throw std::runtime_error(std::string("htrip: ") + htg_last_error());
+}
+}  // namespace detail
+
+// Tile metadata (include/htrip/leaf_tile.hpp:41-52, internal_node.hpp:35-46).
+struct LeafTile {
+    int tx = 0, ty = 0, R0 = 0, C0 = 0, b = 16;
+};
+struct InternalNode {
+    int level = 1, tx = 0, ty = 0, R0 = 0, C0 = 0, s = 32;
+};
+
+class MinPlusOracle {
+public:
+    HierarchyConfig config;
+    int S = 0;
+    int H = 0;
+    int num_leaf_tiles_x = 0;
+    int num_leaf_tiles_y = 0;
+    std::vector<LeafTile> leaves;
+    std::vector<std::vector<InternalNode>> levels;
+    uint32_t update_counter = 0;
+    uint64_t epoch = 0;
+
+    struct LastBatchStats {  // minplus_oracle.hpp:251-256
+        int n_touched_leaves = 0;
+        int n_dirty_ancestors = 0;
+        int n_pivots = 0;
+    };
+    LastBatchStats last_batch_stats{};
+
+    // Caller-owned query workspace (minplus_oracle.hpp:189-238): the device
+    // workspace belongs to the map; this keeps the host staging of a call.
+    struct HTripQueryScratch {
+        std::vector<int32_t> rc;
+        void ensureCapacity(size_t k, int /*H*/, size_t /*num_leaves*/) { rc.reserve(2 * k); }
+    };
+
+    MinPlusOracle() = default;
+    explicit MinPlusOracle(HierarchyConfig cfg) : config(cfg) {}
+    MinPlusOracle(const MinPlusOracle& o) { *this = o; }
+    MinPlusOracle& operator=(const MinPlusOracle& o) {
+        if (this == &o) return *this;
+        copyMeta(o);
+        map_.reset();
+        if (o.map_) {
+            htg_map* m = nullptr;
+            detail::check(htg_map_clone(o.map_.get(), 0, &m));
+            map_.reset(m);
+        }
+        return *this;
+    }
+    MinPlusOracle(MinPlusOracle&&) noexcept = default;
+    MinPlusOracle& operator=(MinPlusOracle&&) noexcept = default;
+
+    // src/minplus_oracle.cpp:48-137
+    bool build(const GridMap& grid) {
+        if (!config.isValid() || grid.rows <= 0 || grid.cols <= 0) return false;
+        std::vector<uint8_t> cells(grid.observed.size());
+        for (size_t i = 0; i < cells.size(); ++i) cells[i] = uint8_t(grid.observed[i]);
+        htg_config c{config.tile_size_b, config.group_w, config.group_h, config.hybrid_k_threshold};
+        htg_map* m = nullptr;
+        const int rc = htg_map_create(detail::device_context(), &c, grid.rows, grid.cols, cells.data(), 0, &m);
+        if (rc == HTG_E_INVALID) return false;
+        detail::check(rc);
+        map_.reset(m);
+        detail::check(htg_build(m));
+        htg_map_info info{};
+        detail::check(htg_map_info_get(m, &info));
+        S = info.S;
+        H = info.H;
+        num_leaf_tiles_x = num_leaf_tiles_y = info.num_leaf_tiles_x;
+        leaves.clear();
+        const int b = config.tile_size_b;
+        for (int ty = 0; ty < num_leaf_tiles_y; ++ty)
+            for (int tx = 0; tx < num_leaf_tiles_x; ++tx) leaves.push_back({tx, ty, ty * b, tx * b, b});
+        levels.assign(size_t(H), {});
+        for (int lvl = 1; lvl <= H; ++lvl) {
+            int side = b;
+            for (int l = 0; l < lvl; ++l) side *= config.group_w;
+            const int nt = S / side > 0 ? S / side : 1;
+            for (int ty = 0; ty < nt; ++ty)
+                for (int tx = 0; tx < nt; ++tx) levels[size_t(lvl - 1)].push_back({lvl, tx, ty, ty * side, tx * side, side});
+        }
+        epoch++;
+        return true;
+    }
+
+    [[nodiscard]] int depth() const noexcept { return H; }
+    [[nodiscard]] int leafIndex(int tx, int ty) const noexcept { return ty * num_leaf_tiles_x + tx; }
+    // src/minplus_oracle.cpp:15-22
+    [[nodiscard]] int tileIndexAtLevel(int lvl, int tx, int ty) const noexcept {
+        int side = config.tile_size_b;
+        for (int l = 0; l < lvl; ++l) side *= config.group_w;
+        int nx = S / side;
+        if (nx <= 0) nx = 1;
+        return ty * nx + tx;
+    }
+    // src/minplus_oracle.cpp:24-46
+    [[nodiscard]] int findLcaLevel(Point p1, Point p2) const noexcept {
+        const int b = config.tile_size_b;
+        int ax = p1.c / b, ay = p1.r / b, bx = p2.c / b, by = p2.r / b;
+        if (ax == bx && ay == by) return 0;
+        for (int lvl = 1; lvl <= H; ++lvl) {
+            ax /= config.group_w;
+            ay /= config.group_h;
+            bx /= config.group_w;
+            by /= config.group_h;
+            if (ax == bx && ay == by) return lvl;
+        }
+        return H + 1;
+    }
+
+    // src/minplus_oracle.cpp:139-177.  The cell must already be Free in grid.
+    void onCellOpened(Point p, const GridMap& grid) {
+        if (!map_ || grid.getObserved(p) == CellState::Obstacle) return;
+        detail::check(htg_open_cell(map_.get(), p.r, p.c));
+        ++update_counter;
+    }
+
+    // src/minplus_oracle.cpp:530-729.  Cells must already be Free in grid
+    // (README.md:71-91); cells the grid marks Obstacle carry no edges in the
+    // reference (edgeWeight is INF) and are skipped here.
+    void onBatchCellsOpened(const std::vector<Point>& newly_opened, const GridMap& grid) {
+        last_batch_stats = {};
+        if (!map_ || newly_opened.empty()) return;
+        std::vector<int32_t> rc;
+        rc.reserve(2 * newly_opened.size());
+        for (const Point& p : newly_opened) {
+            if (grid.getObserved(p) == CellState::Obstacle) continue;
+            rc.push_back(p.r);
+            rc.push_back(p.c);
+        }
+        detail::check(htg_open_cells(map_.get(), rc.data(), int32_t(rc.size() / 2)));
+        htg_batch_stats s{};
+        detail::check(htg_last_batch_stats(map_.get(), &s));
+        last_batch_stats = {s.n_touched_leaves, s.n_dirty_ancestors, s.n_pivots};
+        ++update_counter;
+        epoch++;
+    }
+
+    // src/minplus_oracle.cpp:934-1419
+    void queryAllPairs(const std::vector<Point>& viewpoints, const GridMap& /*grid*/, std::vector<dist_t>& out,
+                       HTripQueryScratch& scratch) const {
+        const size_t k = viewpoints.size();
+        out.resize(k * k);
+        if (k == 0) return;
+        if (!map_) {
+            std::fill(out.begin(), out.end(), INF);
+            return;
+        }
+        scratch.rc.resize(2 * k);
+        for (size_t i = 0; i < k; ++i) {
+            scratch.rc[2 * i] = viewpoints[i].r;
+            scratch.rc[2 * i + 1] = viewpoints[i].c;
+        }
+        detail::check(htg_query_all_pairs(map_.get(), scratch.rc.data(), int32_t(k), out.data()));
+    }
+    // src/minplus_oracle.cpp:1421-1426
+    [[nodiscard]] std::vector<dist_t> queryAllPairs(const std::vector<Point>& viewpoints, const GridMap& grid) const {
+        std::vector<dist_t> m;
+        queryAllPairs(viewpoints, grid, m, query_scratch);
+        return m;
+    }
+    // src/minplus_oracle.cpp:829-928: the matrix query on the pair.
+    [[nodiscard]] dist_t queryDistance(Point s, Point t, const GridMap& grid) const {
+        auto m = queryAllPairs({s, t}, grid);
+        return m[1];
+    }
+
+    // src/minplus_oracle.cpp:2116-2134
+    [[nodiscard]] size_t totalMemoryBytes() const {
+        if (!map_) return 0;
+        htg_map_info info{};
+        detail::check(htg_map_info_get(map_.get(), &info));
+        return size_t(info.total_memory_bytes);
+    }
+
+    // Table copies (the reference exposes LeafTile::D and InternalNode::D_U).
+    [[nodiscard]] std::vector<dist_t> readLeafD(int leaf) const {
+        const size_t n = size_t(config.tile_size_b) * config.tile_size_b;
+        std::vector<dist_t> d(n * n);
+        detail::check(htg_read_leaf_D(map_.get(), leaf, d.data()));
+        return d;
+    }
+    [[nodiscard]] std::vector<dist_t> readNodeDU(int lvl, int idx) const {
+        int32_t nn = 0, nv = 0, ne = 0;
+        detail::check(htg_node_dims(map_.get(), lvl, &nn, &nv, &ne));
+        std::vector<dist_t> d(size_t(nv) * nv);
+        detail::check(htg_read_node_DU(map_.get(), lvl, idx, d.data()));
+        return d;
+    }
+    [[nodiscard]] htg_map* handle() const noexcept { return map_.get(); }
+
+    mutable HTripQueryScratch query_scratch;
+
+private:
+    struct MapDeleter {
+        void operator()(htg_map* m) const noexcept { htg_map_destroy(m); }
+    };
+    std::unique_ptr<htg_map, MapDeleter> map_;
+
+    void copyMeta(const MinPlusOracle& o) {
+        config = o.config;
+        S = o.S;
+        H = o.H;
+        num_leaf_tiles_x = o.num_leaf_tiles_x;
+        num_leaf_tiles_y = o.num_leaf_tiles_y;
+        leaves = o.leaves;
+        levels = o.levels;
+        update_counter = o.update_counter;
+        epoch = o.epoch;
+        last_batch_stats = o.last_batch_stats;
+    }
+};
+
+using HTripOracle = MinPlusOracle;  // minplus_oracle.hpp:411
+
+}  // namespace htrip

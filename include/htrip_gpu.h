@@ -126,6 +126,29 @@ int htg_node_dims(const htg_map* map, int32_t lvl, int32_t* num_nodes, int32_t* 
 int htg_read_node_DU(htg_map* map, int32_t lvl, int32_t idx, uint16_t* out); /* nV x nV */
 int htg_read_node_SU(htg_map* map, int32_t lvl, int32_t idx, uint16_t* out); /* nExt x nExt */
 
+/* ---- kernel timing ----------------------------------------------------------
+ * When enabled, the runtime brackets every launch of the kernel classes below
+ * with hipEvents recorded on the stream the kernel runs on, and accumulates
+ * launch counts, elapsed time and the algorithmic bytes / min-plus updates of
+ * each launch.  Enabling resets the counters. */
+enum {
+    HTG_K_LEAF_APSP = 0,
+    HTG_K_CLOSURE_PIVOTS = 1,
+    HTG_K_CLOSURE_UPDATE = 2,
+    HTG_K_Q_PROJECT = 3,
+    HTG_K_Q_MEET = 4,
+    HTG_K_Q_LIFT = 5,
+    HTG_K_COUNT = 6
+};
+typedef struct htg_kernel_stats {
+    uint64_t launches;
+    double total_ms;
+    double bytes;   /* algorithmic bytes moved (unique reads + writes) */
+    double updates; /* min-plus updates (one saturating add + one min each) */
+} htg_kernel_stats;
+int htg_profile_enable(htg_map* map, int on);
+int htg_profile_get(htg_map* map, int kernel_class, htg_kernel_stats* out);
+
 /* ---- build tuning (the project's autotuner picks these) --------------- */
 typedef struct htg_tuning {
     int32_t pivot_block;   /* pivots per closure block: 16, 32 or 64 */

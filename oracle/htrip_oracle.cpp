@@ -691,9 +691,15 @@ void orc_open_cells(void* h, const int32_t* rc, int n) {
         if (o->grid.in(p.r, p.c)) o->grid.obs[size_t(p.r) * o->grid.cols + p.c] = 1;
     o->open_cells(pts);
 }
+// onCellOpened leaves last_batch_stats untouched (src/minplus_oracle.cpp:139-177).
 void orc_open_cell(void* h, int r, int c) {
+    auto* o = static_cast<Oracle*>(h);
+    const int t = o->last_touched, a = o->last_anc, p = o->last_pivots;
     int32_t rc[2] = {r, c};
     orc_open_cells(h, rc, 1);
+    o->last_touched = t;
+    o->last_anc = a;
+    o->last_pivots = p;
 }
 void orc_query(void* h, const int32_t* rc, int k, uint16_t* out) {
     static_cast<Oracle*>(h)->query(to_pts(rc, k), out);

@@ -1,0 +1,150 @@
+// C++20 drop-in test: code written against the reference's MinPlusOracle API,
+// compiled against htrip_amd/include/htrip/*.hpp and linked with
+// libhtrip_gpu.so.  Mirrors suites of the reference's apps/test_exactness.cpp
+// (known answers 1 and 37, blocked viewpoint INF, batch openings, copy
+// semantics) and checks every matrix against a BFS written here.
+// Exit status 0 = all passed.
+#include <cstdio>
+#include <queue>
+#include <random>
+#include <vector>
+
+#include "htrip/grid_map.hpp"
+#include "htrip/minplus_oracle.hpp"
+
+using namespace htrip;
+
+static std::vector<dist_t> bfs_all_pairs(const GridMap& g, const std::vector<Point>& vps) {
+    const size_t k = vps.size();
+    std::vector<dist_t> m(k * k, INF);
+    std::vector<int> dist(size_t(g.rows) * g.cols);
+    for (size_t i = 0; i < k; ++i) {
+        if (!g.isPassable(vps[i])) continue;
+        std::fill(dist.begin(), dist.end(), -1);
+        std::queue<Point> q;
+        q.push(vps[i]);
+        dist[size_t(g.index(vps[i]))] = 0;
+        while (!q.empty()) {
+            Point u = q.front();
+            q.pop();
+            const Point nb[4] = {{u.r - 1, u.c}, {u.r + 1, u.c}, {u.r, u.c - 1}, {u.r, u.c + 1}};
+            for (const Point& v : nb) {
+                if (!g.isPassable(v) || dist[size_t(g.index(v))] >= 0) continue;
+                dist[size_t(g.index(v))] = dist[size_t(g.index(u))] + 1;
+                q.push(v);
+            }
+        }
+        for (size_t j = 0; j < k; ++j)
+            if (g.isPassable(vps[j]) && dist[size_t(g.index(vps[j]))] >= 0)
+                m[i * k + j] = dist_t(std::min(dist[size_t(g.index(vps[j]))], int(INF)));
+    }
+    return m;
+}
+
+static int failures = 0;
+static void expect(bool ok, const char* what) {
+    std::printf("  %-58s %s\n", what, ok ? "PASSED" : "FAILED");
+    if (!ok) ++failures;
+}
+
+static bool matches_bfs(const MinPlusOracle& o, const GridMap& g, const std::vector<Point>& vps) {
+    return o.queryAllPairs(vps, g) == bfs_all_pairs(g, vps);
+}
+
+int main() {
+    {  // Suite 1: seam crossing between leaves, d = 1 (test_exactness.cpp:97-146)
+        GridMap g(32, 32, CellState::Obstacle);
+        g.openDoorway({4, 16});
+        HierarchyConfig cfg;
+        cfg.tile_size_b = 8;
+        MinPlusOracle o(cfg);
+        expect(o.build(g), "suite 1 build");
+        g.openDoorway({4, 15});
+        o.onCellOpened({4, 15}, g);
+        expect(o.queryDistance({4, 15}, {4, 16}, g) == 1, "suite 1 seam (4,15)-(4,16) = 1");
+        for (int c = 10; c <= 20; ++c) {
+            g.openDoorway({4, c});
+            o.onCellOpened({4, c}, g);
+        }
+        expect(matches_bfs(o, g, {{4, 10}, {4, 13}, {4, 15}, {4, 16}, {4, 18}, {4, 20}}), "suite 1 corridor == BFS");
+    }
+    {  // Suite 8: dense-meet tail, d = 37 (:594-667)
+        GridMap g(32, 32, CellState::Obstacle);
+        for (int c = 16; c <= 31; ++c) g.openDoorway({2, c});
+        for (int r = 2; r <= 15; ++r) g.openDoorway({r, 24});
+        for (int r = 16; r <= 20; ++r) g.openDoorway({r, 24});
+        for (int c = 16; c <= 31; ++c) g.openDoorway({20, c});
+        for (int r = 16; r <= 31; ++r)
+            for (int c = 0; c <= 15; ++c) g.openDoorway({r, c});
+        HierarchyConfig cfg;
+        cfg.tile_size_b = 16;
+        HTripOracle o(cfg);
+        o.build(g);
+        expect(o.queryDistance({2, 24}, {25, 10}, g) == 37, "suite 8 d((2,24),(25,10)) = 37");
+        expect(matches_bfs(o, g, {{2, 24}, {25, 10}, {20, 20}, {8, 24}}), "suite 8 matrix == BFS");
+    }
+    {  // Suite 12: blocked viewpoint INF, door opens a finite path (:842-893)
+        GridMap g(32, 32, CellState::Free);
+        for (int r = 0; r < 32; ++r) g.blockCorridor({r, 16});
+        HierarchyConfig cfg;
+        cfg.tile_size_b = 16;
+        MinPlusOracle o(cfg);
+        o.build(g);
+        std::vector<Point> vps = {{4, 4}, {8, 12}, {4, 20}, {8, 28}, {20, 16}};
+        auto m = o.queryAllPairs(vps, g);
+        expect(m[4 * 5 + 4] == INF && m[0 * 5 + 2] == INF, "suite 12 blocked viewpoint and split halves INF");
+        g.openDoorway({8, 16});
+        o.onCellOpened({8, 16}, g);
+        expect(matches_bfs(o, g, vps) && o.queryAllPairs(vps, g)[2] < INF, "suite 12 door opened == BFS");
+    }
+    {  // Suite 16.3: one 64-cell batch, open grid -> Manhattan (:1132-1163)
+        GridMap g(64, 64, CellState::Free);
+        for (int r = 0; r < 64; ++r) g.setObserved(r, 31, CellState::Obstacle);
+        HierarchyConfig cfg;
+        cfg.tile_size_b = 16;
+        MinPlusOracle o(cfg);
+        o.build(g);
+        std::vector<Point> batch;
+        for (int r = 0; r < 64; ++r) {
+            g.setObserved(r, 31, CellState::Free);
+            batch.push_back({r, 31});
+        }
+        o.onBatchCellsOpened(batch, g);
+        expect(o.queryDistance({32, 10}, {32, 45}, g) == 35, "suite 16.3 batch column, d = 35");
+        expect(o.last_batch_stats.n_touched_leaves == 4, "suite 16.3 last_batch_stats touched leaves");
+    }
+    {  // Random exploration with batches; copy semantics (bench_update_latency.cpp:178)
+        std::mt19937_64 rng(7);
+        GridMap g(96, 96, CellState::Unknown);
+        std::vector<Point> hidden;
+        for (int r = 0; r < 96; ++r)
+            for (int c = 0; c < 96; ++c) {
+                const bool wall = (r % 12 == 0 && c % 5 != 0) || (c % 17 == 0 && r % 7 != 0);
+                if (wall) g.setObserved(r, c, CellState::Obstacle);
+                else if (r < 40 && c < 40) g.setObserved(r, c, CellState::Free);
+                else hidden.push_back({r, c});
+            }
+        std::shuffle(hidden.begin(), hidden.end(), rng);
+        HierarchyConfig cfg;
+        cfg.tile_size_b = 8;
+        MinPlusOracle o(cfg);
+        o.build(g);
+        MinPlusOracle snapshot = o;  // deep copy of the device index
+        GridMap g0 = g;
+        bool ok = true;
+        for (size_t s = 0; s < 10; ++s) {
+            std::vector<Point> batch(hidden.begin() + long(s * 300), hidden.begin() + long((s + 1) * 300));
+            for (const Point& p : batch) g.openDoorway(p);
+            o.onBatchCellsOpened(batch, g);
+            std::vector<Point> vps;
+            for (int i = 0; i < 20; ++i) vps.push_back({int(rng() % 96), int(rng() % 96)});
+            ok = ok && matches_bfs(o, g, vps);
+        }
+        expect(ok, "random batch reveals == BFS after every batch");
+        std::vector<Point> vps = {{1, 1}, {30, 30}, {39, 2}, {60, 60}};
+        expect(matches_bfs(snapshot, g0, vps), "copied oracle keeps the pre-update state");
+        expect(o.totalMemoryBytes() > 0 && o.depth() == o.H, "totalMemoryBytes / depth");
+    }
+    std::printf("%s (%d failure%s)\n", failures ? "FAILED" : "ALL PASSED", failures, failures == 1 ? "" : "s");
+    return failures ? 1 : 0;
+}

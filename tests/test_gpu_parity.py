@@ -56,3 +56,109 @@ def test_build_tables_and_query(gpu_ctx, kind, n, b, g, seed):
     np.testing.assert_array_equal(O.query(vps), expect)
     np.testing.assert_array_equal(G.query(vps), expect)
     assert G.total_memory_bytes() == O.total_memory_bytes()
+
+
+REVEAL = [
+    # (kind, n, b, g, seed, radius, batch, steps)
+    ("street", 96, 8, 2, 21, 20, 37, 12),
+    ("street", 128, 16, 2, 22, 24, 64, 10),
+    ("maze", 64, 8, 2, 23, 12, 29, 10),
+    ("street", 64, 8, 4, 24, 14, 33, 8),
+    ("street", 128, 32, 2, 25, 30, 101, 6),
+]
+
+
+@pytest.mark.parametrize("kind,n,b,g,seed,radius,batch,steps", REVEAL)
+def test_incremental_reveal(gpu_ctx, kind, n, b, g, seed, radius, batch, steps):
+    """onBatchCellsOpened: tables and stats equal the oracle after every step;
+    tables equal a fresh build (monotone exactness); queries equal BFS."""
+    print(f"reveal {kind} n={n} b={b} g={g} seed={seed}", flush=True)
+    truth = make(kind, n, seed)
+    start = grids.sample_free(truth, 1, seed)[0]
+    obs = grids.disk_observed(truth, [tuple(start)], radius)
+    order = grids.frontier_order(truth, obs)
+    O = Oracle("port", obs, b, g)
+    O.build()
+    G = GpuMap(obs, b, g)
+    G.build()
+    for st in range(steps):
+        cells = order[st * batch:(st + 1) * batch]
+        if len(cells) == 0:
+            break
+        O.open_cells(cells)
+        G.open_cells(cells)
+        assert G.last_batch_stats() == O.last_batch_stats(), st
+        vps = grids.sample_free(O.cells, 24, seed + st)
+        np.testing.assert_array_equal(G.query(vps), grid_bfs(O.cells, vps), err_msg=f"step {st}")
+    assert_tables_equal(G, O)
+    F = GpuMap(O.cells, b, g)
+    F.build()
+    assert_tables_equal(F, O)
+
+
+def test_single_cell_openings(gpu_ctx):
+    """onCellOpened one cell at a time (Suite 5 style) including seam cells."""
+    truth = grids.street_grid(64, 31)
+    obs = grids.disk_observed(truth, [(32, 32)], 12)
+    order = grids.frontier_order(truth, obs)[:60]
+    O = Oracle("port", obs, 8, 2)
+    O.build()
+    G = GpuMap(obs, 8, 2)
+    G.build()
+    before = G.last_batch_stats()
+    for r, c in order:
+        O.open_cell(int(r), int(c))
+        G.open_cell(int(r), int(c))
+    assert G.last_batch_stats() == before
+    assert_tables_equal(G, O)
+
+
+def test_query_batch_and_edge_cases(gpu_ctx):
+    """Coalesced queries of mixed k equal the same queries run alone."""
+    cells = grids.street_grid(128, 41)
+    G = GpuMap(cells, 16, 2)
+    G.build()
+    qs = [grids.sample_free(cells, k, 100 + k) for k in (1, 2, 7, 33, 64, 5)]
+    qs.append(np.zeros((0, 2), np.int32))
+    qs.append(np.array([[0, 0], [0, 0], [127, 127], [-3, -3]], np.int32))
+    alone = [G.query(q) for q in qs]
+    mixed = G.query_batch(qs)
+    for a, m, q in zip(alone, mixed, qs):
+        np.testing.assert_array_equal(a, m)
+        np.testing.assert_array_equal(m, grid_bfs(cells, q))
+
+
+def test_clone_is_independent(gpu_ctx):
+    truth = grids.street_grid(64, 51)
+    obs = grids.disk_observed(truth, [(20, 20)], 16)
+    order = grids.frontier_order(truth, obs)
+    G = GpuMap(obs, 8, 2)
+    G.build()
+    C = G.clone()
+    G.open_cells(order[:80])
+    vps = grids.sample_free(obs, 20, 7)
+    np.testing.assert_array_equal(C.query(vps), grid_bfs(obs, vps))
+    np.testing.assert_array_equal(G.query(vps), grid_bfs(G.cells, vps))
+
+
+def test_saturation_maze(gpu_ctx):
+    """Long maze distances: sums of two near-diameter distances must clamp, never wrap."""
+    cells = grids.maze_grid(256, 61)
+    G = GpuMap(cells, 16, 2)
+    G.build()
+    vps = grids.sample_free(cells, 48, 62)
+    expect = grid_bfs(cells, vps)
+    assert expect[expect < 60000].max() > 4000
+    np.testing.assert_array_equal(G.query(vps), expect)
+
+
+def test_cpp_dropin_program(gpu_ctx):
+    """The reference-API C++ program (tests/cpp/test_dropin.cpp) against the GPU library."""
+    import os
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(__file__), "cpp", "test_dropin")
+    assert os.path.exists(exe), "built by __graft_entry__.build()"
+    res = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(res.stdout)
+    assert res.returncode == 0, res.stdout + res.stderr

@@ -1,0 +1,221 @@
+"""CPU tests that pin the oracle (oracle/htrip_oracle.cpp) before it is trusted.
+
+Checks, each chosen so that a plausible mistake fails at least one:
+  * golden fixtures written by the compiled reference (tests/golden/golden.json):
+    every table digest, query matrix and last_batch_stats value;
+  * the known answers the reference's suites state (apps/test_exactness.cpp);
+  * the compiled reference itself on fresh seeds, when oracle/_ref is built;
+  * closed forms: leaf D = tile-restricted BFS, D_U = node-restricted BFS
+    between port cells, query = global BFS, open grid = Manhattan distance;
+  * sensitivity: a wrong port order or a dropped same-child shortcut is caught.
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+from htrip_amd import grids
+from oracle import oracle as orc
+from oracle.oracle import Oracle, grid_bfs
+from tests.golden import cases
+
+GOLDEN = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "golden.json")))
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def digests(o):
+    hl, hn = hashlib.sha256(), hashlib.sha256()
+    for i in range(o.num_leaves()):
+        hl.update(o.leaf_D(i).tobytes())
+        hl.update(o.leaf_S(i).tobytes())
+    for lvl in range(1, o.H + 1):
+        for i in range(o.num_nodes(lvl)):
+            hn.update(o.node_DU(lvl, i).tobytes())
+            hn.update(o.node_SU(lvl, i).tobytes())
+    return {"leaves": hl.hexdigest(), "nodes": hn.hexdigest()}
+
+
+def run_golden(make_oracle, name):
+    g = GOLDEN["cases"][name]
+    c = g["spec"]
+    cells = cases.make_cells(c)
+    assert sha(cells) == g["grid_sha256"], "generator drift"
+    o = make_oracle(cells, c["b"], c["g"])
+    assert o.build()
+    assert (o.S, o.H, o.total_memory_bytes()) == (g["S"], g["H"], g["total_memory_bytes"])
+    assert digests(o) == g["tables"]
+    assert o.query(cases.viewpoints(c, cells)).tobytes().hex() == g["query_hex"]
+    for s, (batch, exp) in enumerate(zip(cases.reveal_batches(c, cells), g["steps"])):
+        if batch["single"]:
+            for r, cc in batch["cells"]:
+                o.open_cell(int(r), int(cc))
+        else:
+            o.open_cells(np.array(batch["cells"], dtype=np.int32).reshape(-1, 2))
+        assert list(o.last_batch_stats()) == exp["stats"], f"stats step {s}"
+        vq = cases.viewpoints(c, o.cells, salt=s + 1)
+        assert o.query(vq).tobytes().hex() == exp["query_hex"], f"query step {s}"
+    if g["steps"]:
+        assert digests(o) == g["tables_after"]
+    return o
+
+
+@pytest.mark.parametrize("name", sorted(GOLDEN["cases"]))
+def test_port_matches_golden(name):
+    run_golden(lambda cells, b, g: Oracle("port", cells, b, g), name)
+
+
+@pytest.mark.parametrize("case,step,i,j,d", cases.KNOWN)
+def test_known_answers(case, step, i, j, d):
+    c = next(x for x in cases.CASES if x["name"] == case)
+    cells = cases.make_cells(c)
+    o = Oracle("port", cells, c["b"], c["g"])
+    o.build()
+    for batch in cases.reveal_batches(c, cells)[:step]:
+        if batch["single"]:
+            for r, cc in batch["cells"]:
+                o.open_cell(int(r), int(cc))
+        else:
+            o.open_cells(np.array(batch["cells"], dtype=np.int32).reshape(-1, 2))
+    vps = cases.viewpoints(c, o.cells)
+    m = o.query(vps)
+    assert m[i, j] == d and m[j, i] == d
+    assert grid_bfs(o.cells, vps)[i, j] == d
+
+
+@pytest.mark.skipif(not orc.available("reference"), reason="oracle/_ref not built (reference sources absent)")
+@pytest.mark.parametrize("seed", [11, 12, 13])
+def test_port_matches_compiled_reference(seed):
+    truth = grids.street_grid(80, seed)
+    obs = grids.disk_observed(truth, [tuple(grids.sample_free(truth, 1, seed)[0])], 25)
+    order = grids.frontier_order(truth, obs)
+    for b, g in ((8, 2), (16, 2), (4, 4)):
+        P, R = Oracle("port", obs, b, g), Oracle("reference", obs, b, g)
+        P.build()
+        R.build()
+        assert digests(P) == digests(R)
+        for st in range(4):
+            batch = order[st * 45:(st + 1) * 45]
+            P.open_cells(batch)
+            R.open_cells(batch)
+            assert P.last_batch_stats() == R.last_batch_stats()
+        assert digests(P) == digests(R)
+        vps = grids.sample_free(P.cells, 20, seed)
+        np.testing.assert_array_equal(P.query(vps), R.query(vps))
+        np.testing.assert_array_equal(P.bfs(vps), R.bfs(vps))
+
+
+def port_cells(R0, C0, s):
+    """Canonical port order of a side-s tile (src/leaf_tile.cpp:11-30)."""
+    out = [(R0, C0 + c) for c in range(s)] + [(R0 + s - 1, C0 + c) for c in range(s)]
+    out += [(R0 + r, C0) for r in range(1, s - 1)] + [(R0 + r, C0 + s - 1) for r in range(1, s - 1)]
+    return out
+
+
+def node_ports(o, lvl, tx, ty):
+    cs = o.b * o.g ** (lvl - 1)
+    s = cs * o.g
+    vu = []
+    for q in range(o.g * o.g):
+        vu += port_cells(ty * s + (q // o.g) * cs, tx * s + (q % o.g) * cs, cs)
+    return vu, s
+
+
+@pytest.mark.parametrize("kind,b,g", [("street", 8, 2), ("maze", 8, 2), ("street", 4, 4)])
+def test_closed_forms(kind, b, g):
+    cells = grids.street_grid(64, 5) if kind == "street" else grids.maze_grid(64, 5)
+    o = Oracle("port", cells, b, g)
+    o.build()
+    nl = o.S // b
+    # leaf D rows = tile-restricted BFS (src/leaf_tile.cpp:56-153)
+    for leaf in (0, nl + 1, nl * nl - 1):
+        D = o.leaf_D(leaf)
+        R0, C0 = (leaf // nl) * b, (leaf % nl) * b
+        for u in range(0, b * b, 3):
+            exp = o.region_bfs(R0, C0, b, b, R0 + u // b, C0 + u % b).ravel()
+            np.testing.assert_array_equal(D[u], exp)
+    # D_U entries = node-restricted BFS between port cells (src/internal_node.cpp:94-245)
+    for lvl in range(1, o.H + 1):
+        nt = max(1, o.S // (b * g ** lvl))
+        for idx in (0, o.num_nodes(lvl) - 1):
+            tx, ty = idx % nt, idx // nt
+            vu, s = node_ports(o, lvl, tx, ty)
+            DU = o.node_DU(lvl, idx)
+            for a in range(0, len(vu), max(1, len(vu) // 9)):
+                reg = o.region_bfs(ty * s, tx * s, s, s, *vu[a])
+                exp = np.array([reg[r - ty * s, c - tx * s] for r, c in vu], dtype=np.uint16)
+                np.testing.assert_array_equal(DU[a], exp, err_msg=f"lvl {lvl} node {idx} row {a}")
+    vps = grids.sample_free(cells, 30, 9)
+    np.testing.assert_array_equal(o.query(vps), grid_bfs(cells, vps))
+
+
+def test_open_grid_is_manhattan():
+    cells = np.ones((48, 48), np.uint8)
+    o = Oracle("port", cells, 8, 2)
+    o.build()
+    vps = grids.sample_free(cells, 25, 3)
+    man = np.abs(vps[:, None, 0] - vps[None, :, 0]) + np.abs(vps[:, None, 1] - vps[None, :, 1])
+    np.testing.assert_array_equal(o.query(vps), man.astype(np.uint16))
+
+
+def test_single_leaf_is_bfs():
+    cells = grids.street_grid(16, 4)
+    o = Oracle("port", cells, 16, 2)
+    o.build()
+    assert o.H == 0
+    vps = grids.sample_free(cells, 20, 4)
+    np.testing.assert_array_equal(o.query(vps), grid_bfs(cells, vps))
+
+
+def test_checks_catch_a_wrong_port_order():
+    """The D_U closed form pins the port order: a transposed side order fails it."""
+    cells = grids.street_grid(32, 8)
+    o = Oracle("port", cells, 8, 2)
+    o.build()
+    vu, s = node_ports(o, 1, 0, 0)
+    DU = o.node_DU(1, 0)
+    pc = 4 * 8 - 4
+    wrong = list(vu)
+    wrong[16:22], wrong[22:28] = vu[22:28], vu[16:22]  # left and right columns swapped in child 0
+    mism = 0
+    for a in range(pc):
+        reg = o.region_bfs(0, 0, s, s, *wrong[a])
+        exp = np.array([reg[r, c] for r, c in wrong], dtype=np.uint16)
+        mism += int((DU[a] != exp).any())
+    assert mism > 0
+
+
+def test_checks_catch_a_dropped_same_child_shortcut():
+    """Two viewpoints in one leaf whose shortest path leaves the leaf: the
+    tile-restricted distance is 19, the true distance 9.  A query that kept the
+    leaf distance for same-leaf pairs (no same-child meet above the leaf) would
+    return 19; the known answer 9 pins the shortcut."""
+    cells = np.full((16, 16), 2, np.uint8)
+    cells[8, 0:8] = 1
+    cells[8, 4] = 2          # wall inside the row
+    cells[8:15, 0] = 1       # in-leaf detour: down, across row 14, up
+    cells[14, 0:8] = 1
+    cells[8:15, 7] = 1
+    cells[7, 0:8] = 1        # detour through the leaf above
+    vps = np.array([[8, 0], [8, 7]], np.int32)
+    o = Oracle("port", cells, 8, 2)
+    o.build()
+    tile = o.region_bfs(8, 0, 8, 8, 8, 0)
+    assert tile[0, 7] == 19
+    assert grid_bfs(cells, vps)[0, 1] == 9
+    assert o.query(vps)[0, 1] == 9
+
+
+def test_grid_generators_are_deterministic():
+    a, b_ = grids.street_grid(256, 3), grids.street_grid(256, 3)
+    np.testing.assert_array_equal(a, b_)
+    assert 0.62 < (a == 1).mean() < 0.72
+    m = grids.maze_grid(512, 1)
+    assert int((m == 1).sum()) == 131071  # (n/2)^2 lattice cells + (n/2)^2 - 1 carved walls
+    v1, v2 = grids.sample_free(a, 50, 9), grids.sample_free(a, 50, 9)
+    np.testing.assert_array_equal(v1, v2)
+    assert len({tuple(p) for p in v1}) == 50
