@@ -615,6 +615,20 @@ int Map::open_cells(const int32_t* rcs, int n) {
 // tile key (base g^2 digits from the root down), so at every level the
 // viewpoints that share (W, q) and those that share W are contiguous runs.
 int Map::query(const int32_t* rcs, const int32_t* ks, int nq, uint16_t* out_host, QueryWS& ws) {
+    int rc = query_launch(rcs, ks, nq, ws, out_host != nullptr);
+    if (rc) return rc;
+    return query_finish(ws, out_host);
+}
+
+int Map::query_finish(QueryWS& ws, uint16_t* out_host) {
+    hipError_t e = hipEventSynchronize(ws.done);
+    if (e != hipSuccess) return hip_fail(e, "query sync");
+    if (out_host && ws.total_out) std::memcpy(out_host, ws.h_out.p, size_t(ws.total_out) * 2);
+    prof.harvest();
+    return HTG_OK;
+}
+
+int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws, bool to_host) {
     hipError_t e;
     int rc;
     long long total_vp = 0, total_out = 0;
@@ -625,7 +639,11 @@ int Map::query(const int32_t* rcs, const int32_t* ks, int nq, uint16_t* out_host
         total_out += (long long)ks[q] * ks[q];
         kmax = std::max(kmax, ks[q]);
     }
-    if (total_out == 0) return HTG_OK;
+    ws.total_out = total_out;
+    if (total_out == 0) {
+        hipError_t e0 = hipEventRecord(ws.done, ws.stream);
+        return e0 == hipSuccess ? HTG_OK : hip_fail(e0, "record");
+    }
     // ---- plan
     QueryPlan& P = ws.plan;
     P.clear();
@@ -813,14 +831,12 @@ int Map::query(const int32_t* rcs, const int32_t* ks, int nq, uint16_t* out_host
         }
     }
     if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "query launch");
-    if (out_host) {
+    if (to_host) {
         if ((e = hipMemcpyAsync(ws.h_out.p, ws.d_out.p, size_t(total_out) * 2, hipMemcpyDeviceToHost, st)) !=
             hipSuccess)
             return hip_fail(e, "download result");
-        if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(e, "query sync");
-        std::memcpy(out_host, ws.h_out.p, size_t(total_out) * 2);
-        prof.harvest();
     }
+    if ((e = hipEventRecord(ws.done, st)) != hipSuccess) return hip_fail(e, "record");
     return HTG_OK;
 }
 
@@ -828,9 +844,12 @@ int QueryWS::init() {
     if (stream) return HTG_OK;
     hipError_t e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
     if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
+    if ((e = hipEventCreateWithFlags(&done, hipEventDisableTiming)) != hipSuccess) return hip_fail(e, "hipEventCreate");
     return HTG_OK;
 }
 QueryWS::~QueryWS() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (done) (void)hipEventDestroy(done);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -839,10 +858,6 @@ QueryWS::~QueryWS() {
 // ===========================================================================
 // C ABI
 // ===========================================================================
-struct htg_ctx {
-    int device = 0;
-};
-
 #define HTG_GUARD_BEGIN try {
 #define HTG_GUARD_END                                                   \
     }                                                                   \

@@ -128,6 +128,8 @@ struct QueryPlan {
 // include/htrip/minplus_oracle.hpp:189-238).
 struct QueryWS {
     hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;  // recorded after the batch's last operation
+    long long total_out = 0;
     QueryPlan plan;
     HostBuf h_plan, h_out;
     DevBuf d_plan, X[2], Z, Zp, d_out;
@@ -178,6 +180,10 @@ struct Map {
     int build();
     int open_cells(const int32_t* rc, int n);
     int query(const int32_t* rc, const int32_t* ks, int nq, uint16_t* out_host, QueryWS& ws);
+    // Asynchronous halves of query(): plan + launch on ws.stream (results
+    // copied to ws.h_out when to_host), then wait for ws.done.
+    int query_launch(const int32_t* rc, const int32_t* ks, int nq, QueryWS& ws, bool to_host);
+    int query_finish(QueryWS& ws, uint16_t* out_host);
     int ensure_build_ws(int lvl);
     int run_closure(int lvl, int n_list, const std::vector<int>& hcounts);
     const uint16_t* child_summaries(int lvl, size_t& stride) const;
@@ -186,3 +192,7 @@ struct Map {
 }  // namespace htg
 
 struct htg_map : htg::Map {};
+
+struct htg_ctx {
+    int device = 0;
+};

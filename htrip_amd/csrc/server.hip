@@ -1,0 +1,424 @@
+// Serving loop: resident sessions, admission by HBM budget, continuous
+// batching of update batches and mixed-k all-pairs queries.
+//
+// The reference's only serving-like caller is its robot simulator, which
+// double-buffers the oracle and runs updates and queries on worker threads
+// (src/robot_simulator.cpp:897-1058, :1483-1580).  Here one worker thread
+// owns every resident map:
+//  * requests of a session are served in submission order: a query observes
+//    exactly the updates submitted before it;
+//  * consecutive queries of a session are coalesced into one batched launch
+//    (htg_query_batch semantics), consecutive updates into one reveal batch
+//    (the union of monotone openings equals applying them in turn);
+//  * up to n_slots batches are in flight on their own streams, so batches of
+//    different sessions overlap on the device, and requests that arrive while
+//    batches run are picked up as soon as a slot frees (no tick barrier);
+//  * an update waits until its session's in-flight query batches completed
+//    (they read the tables it rewrites).
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/htrip_gpu.h"
+#include "runtime.hpp"
+
+using namespace htg;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+    uint64_t ticket = 0;
+    int session = 0;
+    int type = 0;  // 0 query, 1 update
+    std::vector<int32_t> rc;
+    int32_t count = 0;
+    uint16_t* out = nullptr;
+    Clock::time_point t_submit, t_done;
+    int status = HTG_OK;
+    bool done = false;
+};
+
+struct Batch {
+    QueryWS ws;
+    int session = -1;
+    std::vector<Request*> reqs;
+    bool busy = false;
+};
+
+struct Session {
+    htg_map* map = nullptr;
+    uint64_t bytes = 0;
+    std::deque<Request*> pending;
+    int inflight = 0;  // query batches on the device
+    int updating = 0;  // an update running on the worker
+};
+
+}  // namespace
+
+struct htg_server {
+    htg_ctx* ctx = nullptr;
+    htg_server_config cfg{};
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::map<int, Session> sessions;
+    int next_session = 1;
+    uint64_t used = 0;
+    std::vector<std::unique_ptr<Batch>> pool;
+    std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
+    uint64_t next_ticket = 1;
+    std::thread worker;
+    bool stop = false;
+    htg_server_stats stats{};
+    std::string worker_error;
+
+    void run();
+    bool retire();
+    bool dispatch();
+};
+
+namespace {
+
+// Copy a finished batch's matrices to the callers and mark them done.
+void finish_batch(htg_server* s, Batch& b) {
+    Session& ss = s->sessions.at(b.session);
+    int rc = ss.map->query_finish(b.ws, nullptr);
+    const uint16_t* res = static_cast<const uint16_t*>(b.ws.h_out.p);
+    long long off = 0;
+    const auto now = Clock::now();
+    for (Request* r : b.reqs) {
+        const size_t n = size_t(r->count) * r->count;
+        if (rc == HTG_OK && r->out && n) std::memcpy(r->out, res + off, n * 2);
+        off += (long long)n;
+        r->status = rc;
+        r->t_done = now;
+        r->done = true;
+    }
+    ss.inflight--;
+    b.busy = false;
+    b.reqs.clear();
+}
+
+}  // namespace
+
+bool htg_server::retire() {
+    bool any = false;
+    for (auto& bp : pool) {
+        Batch& b = *bp;
+        if (!b.busy) continue;
+        if (hipEventQuery(b.ws.done) != hipSuccess) {
+            (void)hipGetLastError();
+            continue;
+        }
+        std::lock_guard<std::mutex> lk(mu);
+        finish_batch(this, b);
+        any = true;
+    }
+    if (any) cv_done.notify_all();
+    return any;
+}
+
+bool htg_server::dispatch() {
+    bool any = false;
+    std::vector<int> ids;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        for (auto& kv : sessions) ids.push_back(kv.first);
+    }
+    for (int id : ids) {
+        for (;;) {
+            std::unique_lock<std::mutex> lk(mu);
+            auto it = sessions.find(id);
+            if (it == sessions.end()) break;
+            Session& ss = it->second;
+            if (ss.pending.empty()) break;
+            std::vector<Request*> run;
+            if (ss.pending.front()->type == 0) {
+                Batch* fb = nullptr;
+                for (auto& bp : pool)
+                    if (!bp->busy) {
+                        fb = bp.get();
+                        break;
+                    }
+                if (!fb) return any;  // every slot busy: retire first
+                long long vps = 0;
+                while (!ss.pending.empty() && ss.pending.front()->type == 0 &&
+                       int(run.size()) < cfg.max_batch_queries &&
+                       (run.empty() || vps + ss.pending.front()->count <= cfg.max_batch_viewpoints)) {
+                    vps += ss.pending.front()->count;
+                    run.push_back(ss.pending.front());
+                    ss.pending.pop_front();
+                }
+                fb->busy = true;
+                fb->session = id;
+                fb->reqs = run;
+                ss.inflight++;
+                std::vector<int32_t> rc, ks;
+                for (Request* r : run) {
+                    rc.insert(rc.end(), r->rc.begin(), r->rc.end());
+                    ks.push_back(r->count);
+                }
+                htg_map* m = ss.map;
+                lk.unlock();
+                const int st = m->query_launch(rc.data(), ks.data(), int(ks.size()), fb->ws, true);
+                lk.lock();
+                stats.query_batches++;
+                stats.queries += run.size();
+                if (st != HTG_OK) {  // fail this batch's requests, keep serving the others
+                    const auto now = Clock::now();
+                    for (Request* r : run) {
+                        r->status = st;
+                        r->t_done = now;
+                        r->done = true;
+                    }
+                    ss.inflight--;
+                    fb->busy = false;
+                    fb->reqs.clear();
+                    cv_done.notify_all();
+                }
+            } else {
+                if (ss.inflight > 0) break;  // its query batches still read the tables
+                while (!ss.pending.empty() && ss.pending.front()->type == 1) {
+                    run.push_back(ss.pending.front());
+                    ss.pending.pop_front();
+                }
+                ss.updating++;
+                std::vector<int32_t> cells;
+                for (Request* r : run) cells.insert(cells.end(), r->rc.begin(), r->rc.end());
+                htg_map* m = ss.map;
+                lk.unlock();
+                const int st = m->open_cells(cells.data(), int(cells.size() / 2));
+                lk.lock();
+                ss.updating--;
+                stats.update_batches++;
+                stats.updates += run.size();
+                const auto now = Clock::now();
+                for (Request* r : run) {
+                    r->status = st;
+                    r->t_done = now;
+                    r->done = true;
+                }
+                cv_done.notify_all();
+            }
+            any = true;
+        }
+    }
+    return any;
+}
+
+void htg_server::run() {
+    if (hipSetDevice(ctx->device) != hipSuccess) {
+        worker_error = "hipSetDevice failed in the server worker";
+        return;
+    }
+    for (;;) {
+        bool inflight = false;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            for (auto& bp : pool) inflight |= bp->busy;
+            bool pending = false;
+            for (auto& kv : sessions) pending |= !kv.second.pending.empty();
+            if (stop && !pending && !inflight) return;
+            if (!pending && !inflight) {
+                cv_work.wait(lk);
+                continue;
+            }
+        }
+        const bool r = retire();
+        const bool d = dispatch();
+        if (!r && !d) std::this_thread::yield();
+    }
+}
+
+// ===========================================================================
+extern "C" {
+
+int htg_server_create(htg_ctx* ctx, const htg_server_config* cfg, htg_server** out) {
+    try {
+        if (!ctx || !cfg || !out) return fail(HTG_E_INVALID, "null argument");
+        if (cfg->n_slots < 1 || cfg->n_slots > 64 || cfg->max_batch_queries < 1 || cfg->max_batch_viewpoints < 1)
+            return fail(HTG_E_INVALID, "bad server configuration");
+        hipError_t e = hipSetDevice(ctx->device);
+        if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+        std::unique_ptr<htg_server> s(new htg_server());
+        s->ctx = ctx;
+        s->cfg = *cfg;
+        for (int i = 0; i < cfg->n_slots; ++i) {
+            s->pool.emplace_back(new Batch());
+            int rc = s->pool.back()->ws.init();
+            if (rc) return rc;
+        }
+        htg_server* raw = s.get();
+        s->worker = std::thread([raw] { raw->run(); });
+        *out = s.release();
+        return HTG_OK;
+    } catch (...) {
+        return fail(HTG_E_INVALID, "server create failed");
+    }
+}
+
+int htg_server_destroy(htg_server* s) {
+    if (!s) return HTG_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    for (auto& kv : s->sessions) htg_map_destroy(kv.second.map);
+    delete s;
+    return HTG_OK;
+}
+
+int htg_server_add_map(htg_server* s, const htg_config* cfg, int32_t rows, int32_t cols, const uint8_t* cells,
+                       int32_t* session) {
+    if (!s || !cfg || !cells || !session) return fail(HTG_E_INVALID, "null argument");
+    uint64_t need = 0;
+    int rc = htg_map_footprint(cfg, rows, cols, &need);
+    if (rc) return rc;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->cfg.hbm_budget && s->used + need > s->cfg.hbm_budget)
+            return fail(HTG_E_NOMEM, "admission refused: session needs " + std::to_string(need) + " bytes, " +
+                                         std::to_string(s->cfg.hbm_budget - s->used) + " left in the budget");
+        s->used += need;  // reserve before building
+    }
+    htg_map* m = nullptr;
+    rc = htg_map_create(s->ctx, cfg, rows, cols, cells, 0, &m);
+    if (rc == HTG_OK) rc = htg_build(m);
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (rc != HTG_OK) {
+        if (m) htg_map_destroy(m);
+        s->used -= need;
+        return rc;
+    }
+    const int id = s->next_session++;
+    s->sessions[id].map = m;
+    s->sessions[id].bytes = need;
+    *session = id;
+    return HTG_OK;
+}
+
+int htg_server_remove_map(htg_server* s, int32_t session) {
+    if (!s) return fail(HTG_E_INVALID, "null server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->sessions.find(session);
+    if (it == s->sessions.end()) return fail(HTG_E_RANGE, "no such session");
+    // drain this session's work first
+    s->cv_done.wait(lk, [&] {
+        return it->second.pending.empty() && it->second.inflight == 0 && it->second.updating == 0;
+    });
+    htg_map* m = it->second.map;
+    s->used -= it->second.bytes;
+    s->sessions.erase(it);
+    lk.unlock();
+    htg_map_destroy(m);
+    return HTG_OK;
+}
+
+static int submit(htg_server* s, int32_t session, int type, const int32_t* rc, int32_t count, uint16_t* out,
+                  uint64_t* ticket) {
+    if (!s || !ticket || count < 0 || (count > 0 && !rc)) return fail(HTG_E_INVALID, "bad argument");
+    std::unique_ptr<Request> r(new Request());
+    r->type = type;
+    r->session = session;
+    r->rc.assign(rc, rc + 2 * size_t(count));
+    r->count = count;
+    r->out = out;
+    r->t_submit = Clock::now();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->sessions.find(session);
+        if (it == s->sessions.end()) return fail(HTG_E_RANGE, "no such session");
+        r->ticket = s->next_ticket++;
+        *ticket = r->ticket;
+        it->second.pending.push_back(r.get());
+        s->reqs[r->ticket] = std::move(r);
+    }
+    s->cv_work.notify_one();
+    return HTG_OK;
+}
+
+int htg_server_submit_query(htg_server* s, int32_t session, const int32_t* rc, int32_t k, uint16_t* out,
+                            uint64_t* ticket) {
+    return submit(s, session, 0, rc, k, out, ticket);
+}
+
+int htg_server_submit_update(htg_server* s, int32_t session, const int32_t* rc, int32_t n, uint64_t* ticket) {
+    return submit(s, session, 1, rc, n, nullptr, ticket);
+}
+
+int htg_server_wait(htg_server* s, uint64_t ticket, double* latency_s) {
+    if (!s) return fail(HTG_E_INVALID, "null server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return fail(HTG_E_RANGE, "unknown ticket");
+    Request* r = it->second.get();
+    s->cv_done.wait(lk, [&] { return r->done; });
+    const int st = r->status;
+    if (latency_s) *latency_s = std::chrono::duration<double>(r->t_done - r->t_submit).count();
+    s->reqs.erase(it);
+    if (st != HTG_OK) return fail(st, "request failed");
+    return HTG_OK;
+}
+
+int htg_server_stats_get(htg_server* s, htg_server_stats* out) {
+    if (!s || !out) return fail(HTG_E_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    *out = s->stats;
+    out->hbm_used = s->used;
+    out->sessions = int32_t(s->sessions.size());
+    return HTG_OK;
+}
+
+int htg_server_map(htg_server* s, int32_t session, htg_map** out) {
+    if (!s || !out) return fail(HTG_E_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->sessions.find(session);
+    if (it == s->sessions.end()) return fail(HTG_E_RANGE, "no such session");
+    *out = it->second.map;
+    return HTG_OK;
+}
+
+int htg_server_replay(htg_server* s, int32_t n, const double* t_arrival, const int32_t* session, const int32_t* type,
+                      const int32_t* count, const int64_t* rc_off, const int32_t* rc, const int64_t* out_off,
+                      uint16_t* out, double* t_done) {
+    if (!s || n < 0 || (n > 0 && (!t_arrival || !session || !type || !count || !rc_off || !rc || !t_done)))
+        return fail(HTG_E_INVALID, "null argument");
+    std::vector<uint64_t> tickets(size_t(n), 0);
+    const auto t0 = Clock::now();
+    int first_err = HTG_OK;
+    for (int i = 0; i < n; ++i) {
+        const auto due = t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(t_arrival[i]));
+        while (Clock::now() < due) {
+        }
+        uint16_t* o = (type[i] == 0 && out && out_off && out_off[i] >= 0) ? out + out_off[i] : nullptr;
+        int rcode = submit(s, session[i], type[i], rc + 2 * rc_off[i], count[i], o, &tickets[i]);
+        if (rcode != HTG_OK && first_err == HTG_OK) first_err = rcode;
+    }
+    for (int i = 0; i < n; ++i) {
+        if (!tickets[i]) {
+            t_done[i] = -1;
+            continue;
+        }
+        std::unique_lock<std::mutex> lk(s->mu);
+        auto it = s->reqs.find(tickets[i]);
+        Request* r = it->second.get();
+        s->cv_done.wait(lk, [&] { return r->done; });
+        t_done[i] = std::chrono::duration<double>(r->t_done - t0).count();
+        if (r->status != HTG_OK && first_err == HTG_OK) first_err = r->status;
+        s->reqs.erase(it);
+    }
+    if (first_err != HTG_OK) return fail(first_err, "replay: a request failed");
+    return HTG_OK;
+}
+
+}  // extern "C"

@@ -135,8 +135,9 @@ def disk_observed(truth: np.ndarray, centers, radius: int) -> np.ndarray:
     return np.where(inside, truth, UNKNOWN).astype(np.uint8)
 
 
-def frontier_order(truth: np.ndarray, observed: np.ndarray) -> np.ndarray:
-    """Free-but-unobserved cells in BFS order from the observed free set (int32 (n, 2))."""
+def frontier_order(truth: np.ndarray, observed: np.ndarray, limit: int | None = None) -> np.ndarray:
+    """Free-but-unobserved cells in BFS order from the observed free set (int32 (n, 2)),
+    at most ``limit`` of them."""
     from collections import deque
 
     rows, cols = truth.shape
@@ -152,6 +153,8 @@ def frontier_order(truth: np.ndarray, observed: np.ndarray) -> np.ndarray:
                 seen[v] = True
                 order.append(v)
                 q.append(v)
+        if limit is not None and len(order) >= limit:
+            break
     o = np.array(order, dtype=np.int64)
     return np.stack([o // cols, o % cols], axis=1).astype(np.int32) if len(o) else np.zeros((0, 2), np.int32)
 

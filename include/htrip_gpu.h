@@ -149,6 +149,52 @@ typedef struct htg_kernel_stats {
 int htg_profile_enable(htg_map* map, int on);
 int htg_profile_get(htg_map* map, int kernel_class, htg_kernel_stats* out);
 
+/* ---- serving ------------------------------------------------------------
+ * A server keeps every admitted map resident and serves requests from a
+ * worker thread.  Requests of one session are served in submission order (a
+ * query sees exactly the updates submitted before it); consecutive queries of
+ * a session are coalesced into one batched launch, consecutive updates into
+ * one reveal batch; up to n_slots query batches are in flight at once on
+ * their own streams, and requests that arrive meanwhile join the next batch
+ * as soon as a slot frees.  No counterpart in the reference (its simulator
+ * double-buffers whole oracles instead, src/robot_simulator.cpp:897-1058). */
+typedef struct htg_server_config {
+    uint64_t hbm_budget;          /* admission limit for all sessions' maps; 0 = no limit */
+    int32_t n_slots;              /* query batches in flight (1..64) */
+    int32_t max_batch_queries;    /* queries coalesced per batch */
+    int32_t max_batch_viewpoints; /* viewpoints per batch (a larger query still runs alone) */
+    int32_t reserved;
+} htg_server_config;
+typedef struct htg_server_stats {
+    uint64_t query_batches, queries, update_batches, updates, hbm_used;
+    int32_t sessions, reserved;
+} htg_server_stats;
+int htg_server_create(htg_ctx* ctx, const htg_server_config* cfg, htg_server** out);
+int htg_server_destroy(htg_server* s);
+/* Admits and builds a session; HTG_E_NOMEM when it does not fit the budget. */
+int htg_server_add_map(htg_server* s, const htg_config* cfg, int32_t rows, int32_t cols, const uint8_t* cells,
+                       int32_t* session);
+/* Waits for the session's requests, then frees its memory for new sessions. */
+int htg_server_remove_map(htg_server* s, int32_t session);
+/* out (k*k, host) must stay valid until the request completes. */
+int htg_server_submit_query(htg_server* s, int32_t session, const int32_t* rc, int32_t k, uint16_t* out,
+                            uint64_t* ticket);
+int htg_server_submit_update(htg_server* s, int32_t session, const int32_t* rc, int32_t n, uint64_t* ticket);
+/* Blocks until the request completed; latency_s = completion - submission. */
+int htg_server_wait(htg_server* s, uint64_t ticket, double* latency_s);
+int htg_server_stats_get(htg_server* s, htg_server_stats* out);
+/* The session's map, for reading tables; do not use it while requests run. */
+int htg_server_map(htg_server* s, int32_t session, htg_map** out);
+/* Trace replay: submits request i at t_arrival[i] seconds after the call
+ * (busy-wait pacing on the calling thread), waits for all, and writes each
+ * completion time (seconds after the call) to t_done.  Request i is a query
+ * (type 0, count = k) or an update (type 1, count = cells) on session[i]
+ * with its (row, col) pairs at rc + 2*rc_off[i]; a query's matrix goes to
+ * out + out_off[i] when out_off[i] >= 0. */
+int htg_server_replay(htg_server* s, int32_t n, const double* t_arrival, const int32_t* session, const int32_t* type,
+                      const int32_t* count, const int64_t* rc_off, const int32_t* rc, const int64_t* out_off,
+                      uint16_t* out, double* t_done);
+
 /* ---- build tuning (the project's autotuner picks these) --------------- */
 typedef struct htg_tuning {
     int32_t pivot_block;   /* pivots per closure block: 16, 32 or 64 */

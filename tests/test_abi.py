@@ -26,7 +26,9 @@ def test_library_built_and_exports_every_declared_symbol():
     assert len(names) >= 20
     for n in names:
         assert hasattr(lib, n), n
-    assert sorted(_lib.EXPORTS) == names
+    from htrip_amd import serving
+
+    assert sorted(_lib.EXPORTS + serving.SERVER_EXPORTS) == names
 
 
 def test_footprint_is_host_only_and_matches_geometry():
