@@ -88,7 +88,7 @@ def cpu_baseline(gmap, cells, queries, budget_s, expect, served_checks=()):
         if not np.array_equal(orc.grid_bfs(state, vps), got):
             raise SystemExit("served result differs from BFS on its map state")
     if served_checks:
-        log(f"exactness: {len(served_checks)} served matrices equal BFS on their map states")
+        log(f"exactness: {len(served_checks)} served matrices equal BFS on their snapshot states")
     if not orc.available("reference"):
         return None
     R = orc.Oracle("reference", cells, B, G)
@@ -118,6 +118,27 @@ def cpu_baseline(gmap, cells, queries, budget_s, expect, served_checks=()):
     log(f"cpu baseline: {done} queries in {spent:.2f}s, all equal to the GPU results")
     return {"value": done / spent, "unit": "matrices/s", "cores": 1, "kind": "reference",
             "sample": f"{done} queryAllPairs k={len(queries[0])} (1 thread, reference AVX2 sources) on the same map"}
+
+
+def served_checks(tr, state0, base, n=4, kmax=96):
+    """(state, viewpoints, result) for the first n small queries of a trace,
+    with the state rebuilt from the update prefix each result reports (the
+    session had applied `base` updates when the trace started)."""
+    if state0 is None:
+        return []
+    upd = [i for i in range(len(tr.t)) if tr.type[i] == 1]
+    out = []
+    for i in range(len(tr.t)):
+        if tr.type[i] != 0 or tr.count[i] > kmax:
+            continue
+        st = state0.copy()
+        for j in upd[:int(tr.version[i]) - base]:
+            c = tr.cells_of(j)
+            st[c[:, 0], c[:, 1]] = 1
+        out.append((st, tr.cells_of(i), tr.query_result(i)))
+        if len(out) == n:
+            break
+    return out
 
 
 def run_plain(args):
@@ -154,8 +175,9 @@ def run_served(args):
     init = rv.observed.copy()
     log(f"grid {N}x{N}: {int((truth == 1).sum())} free, {int((init == 1).sum())} observed at start "
         f"({time.time() - t0:.1f}s)")
+    mode = serving.SNAPSHOT if args.consistency == "snapshot" else serving.STRICT
     srv = serving.Server(n_slots=args.slots, max_batch_queries=args.batch_queries,
-                         max_batch_viewpoints=args.batch_viewpoints)
+                         max_batch_viewpoints=args.batch_viewpoints, consistency=mode)
     t0 = time.time()
     sid = srv.add_map(init, B, G)
     log(f"session admitted and built in {1e3 * (time.time() - t0):.1f} ms")
@@ -172,6 +194,7 @@ def run_served(args):
     srv.replay(warm)
     burst = serving.make_trace(sid, rv, args.window * 8, 1e7, 12)
     tb = srv.replay(burst)
+    n_upd_before = int((warm.type == 1).sum() + (burst.type == 1).sum())
     cap = serving.latency_summary(burst, tb)["matrices_per_s"] / 0.9
     log(f"burst capacity ~{cap:.0f} requests/s")
 
@@ -180,14 +203,16 @@ def run_served(args):
     chosen = None
     for frac in args.rate_fracs:
         rate = cap * frac
-        tr = serving.make_trace(sid, rv, args.window * args.steps, rate, 100 + len(trials),
-                                record_states=0 if (args.no_cpu or trials) else 4)
+        first = not trials
+        state0 = rv.observed.copy() if first and not args.no_cpu else None
+        tr = serving.make_trace(sid, rv, args.window * args.steps, rate, 100 + len(trials))
         smap.profile(True)
         td = srv.replay(tr)
         s = serving.latency_summary(tr, td)
         s["rate"] = rate
         s["roofline"], s["valu"], s["kernels"] = kernel_roofline(smap)
-        trials.append((tr, td, s))
+        trials.append((tr, td, s, state0, n_upd_before))
+        n_upd_before += int((tr.type == 1).sum())
         log(f"rate {rate:.0f}/s: {s['matrices_per_s']:.0f} matrices/s, p50 {s['p50_ms']:.3f} ms, "
             f"p99 {s['p99_ms']:.3f} ms, updates p50 {s['update_p50_ms']} ms")
         if s["p99_ms"] < 1.0:
@@ -195,14 +220,11 @@ def run_served(args):
             break
     if chosen is None:
         chosen = len(trials) - 1
-    tr, td, s = trials[chosen]
+    tr, td, s = trials[chosen][:3]
 
     cpu = None
     if not args.no_cpu:
-        tr0 = trials[0][0]
-        states = dict(tr0.meta["states"])
-        idx = list(states)[:4]
-        checks = [(states[i], tr0.cells_of(i), tr0.query_result(i)) for i in idx]
+        checks = served_checks(trials[0][0], trials[0][3], trials[0][4])
         cpu = cpu_baseline(plain, init, pqs, args.cpu_seconds, pres, checks)
 
     value = s["matrices_per_s"]
@@ -214,7 +236,7 @@ def run_served(args):
         "dtype": "u16", "data": "synthetic",
         "config": {"workload": "1024x1024 street-like grid seed 3, b=32, g=2x2, 16 robots; Poisson stream of "
                                "queryAllPairs k~logU[8,256] + 10% update batches of 16..512 cells",
-                   "mode": "served", "window": args.window, "slots": args.slots,
+                   "mode": "served", "consistency": args.consistency, "window": args.window, "slots": args.slots,
                    "max_batch_queries": args.batch_queries},
         "p50_ms": s["p50_ms"], "p99_ms": s["p99_ms"], "max_ms": s["max_ms"],
         "update_p50_ms": s["update_p50_ms"], "update_p99_ms": s["update_p99_ms"],
@@ -227,18 +249,57 @@ def run_served(args):
     }
 
 
+def run_update(args):
+    """Incremental reveal latency on the serving map: robot batches of 16..512 cells."""
+    from htrip_amd import _lib, grids, serving
+
+    truth = grids.street_grid(N, SEED)
+    rv = serving.RobotReveal(truth, ROBOTS, RADIUS, seed=5, limit=40000)
+    gmap = _lib.GpuMap(rv.observed, B, G)
+    gmap.build()
+    rng = grids.SplitMix64(77)
+    lat, sizes, stats = [], [], []
+    for i in range(args.warmup + args.steps):
+        m = serving.loguniform_k(rng, 16, 512)
+        cells = rv.next_batch(rng.below(ROBOTS), m)
+        if i == args.warmup:
+            gmap.profile(True)
+        t0 = time.perf_counter()
+        gmap.open_cells(cells)
+        dt = time.perf_counter() - t0
+        if i >= args.warmup:
+            lat.append(dt)
+            sizes.append(len(cells))
+            stats.append(gmap.last_batch_stats())
+    lat = np.array(lat)
+    _, _, kernels = kernel_roofline(gmap)
+    log(f"update: p50 {1e3 * np.median(lat):.3f} ms, p99 {1e3 * np.percentile(lat, 99):.3f} ms, "
+        f"mean cells {np.mean(sizes):.0f}")
+    for k, v in kernels.items():
+        log(f"  {k:18s} launches {v['launches']:6d} total {v['total_ms']:9.2f} ms")
+    return {"metric": "onBatchCellsOpened latency (1024^2 street, b=32, robot batches 16..512 cells)",
+            "value": float(np.median(lat) * 1e3), "unit": "ms", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": float(lat.mean() * 1e3), "higher_is_better": False,
+            "scaling": "weak", "vs_baseline": None, "dtype": "u16", "data": "synthetic",
+            "config": {"workload": "1024x1024 street seed 3, b=32, 16 robots, reveal batches logU[16,512]",
+                       "mode": "update"},
+            "p99_ms": float(np.percentile(lat, 99) * 1e3), "mean_cells": float(np.mean(sizes)),
+            "mean_stats": [float(x) for x in np.mean(np.array(stats), axis=0)], "kernels": kernels}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--mode", default="served", choices=["plain", "served"])
+    ap.add_argument("--mode", default="served", choices=["plain", "served", "update"])
     ap.add_argument("--k", type=int, default=64)
     ap.add_argument("--window", type=int, default=64, help="requests per served step")
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--batch-queries", type=int, default=64)
     ap.add_argument("--batch-viewpoints", type=int, default=8192)
     ap.add_argument("--plain-steps", type=int, default=200)
+    ap.add_argument("--consistency", default="snapshot", choices=["snapshot", "strict"])
     ap.add_argument("--rate-fracs", type=float, nargs="+", default=[0.8, 0.6, 0.45, 0.3, 0.2])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -246,7 +307,7 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU (one MI355X per box); --gpus must be 1"}))
         sys.exit(2)
-    line = run_plain(args) if args.mode == "plain" else run_served(args)
+    line = {"plain": run_plain, "served": run_served, "update": run_update}[args.mode](args)
     print(json.dumps(line))
 
 

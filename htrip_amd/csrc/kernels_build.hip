@@ -313,12 +313,15 @@ __global__ __launch_bounds__(256) void k_closure_pivots(int nV, int ld, const in
                                                         const int* __restrict__ counts, int t,
                                                         const uint16_t* __restrict__ DU, size_t node_stride,
                                                         uint16_t* __restrict__ Rbuf) {
+    // Workgroup = 128 columns of one node; wave w computes output pivot rows
+    // [w*P/4, (w+1)*P/4) for the lane's 2 columns.
+    constexpr int KW = P / 4;
     const int slot = blockIdx.y;
     const int p = min(P, counts[slot] - t * P);
     if (p <= 0) return;
     __shared__ int kr[P];
-    __shared__ uint16_t C[P][P + 1];
-    __shared__ uint32_t CdT[P][P];  // CdT[k2][k] = dup(C*[k][k2])
+    __shared__ __align__(16) uint16_t C[P][P];
+    __shared__ __align__(16) uint32_t CdT[P][P];  // CdT[k2][k] = dup(C*[k][k2])
     const int tid = threadIdx.x;
     const uint16_t* D = DU + size_t(node_list[slot]) * node_stride;
     const int* K = pivots + size_t(slot) * piv_stride + t * P;
@@ -329,12 +332,28 @@ __global__ __launch_bounds__(256) void k_closure_pivots(int nV, int ld, const in
         C[a][c] = (a < p && c < p) ? D[size_t(kr[a]) * ld + kr[c]] : (a == c ? 0 : kInf);
     }
     __syncthreads();
+    // Floyd-Warshall over the block's pivots.  Thread t keeps EPT entries of
+    // row fa in registers; each step reads pivot row k and its own column-k
+    // entry from LDS, relaxes, and publishes its entries for the next step.
+    // Rows/columns >= p hold the identity and never change.
+    constexpr int EPT = P * P / 256, TPR = P / EPT;
+    const int fa = tid / TPR, fc0 = (tid % TPR) * EPT;
+    uint16_t cr[EPT];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) cr[e] = C[fa][fc0 + e];
     for (int k = 0; k < p; ++k) {
-        for (int i = tid; i < p * p; i += 256) {
-            const int a = i / p, c = i % p;
-            const uint16_t v = add1(C[a][k], C[k][c]);
-            if (v < C[a][c]) C[a][c] = v;
+        const uint16_t aik = C[fa][k];
+        uint16_t rk[EPT];
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) rk[e] = C[k][fc0 + e];
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) {
+            const uint16_t v = add1(aik, rk[e]);
+            cr[e] = v < cr[e] ? v : cr[e];
         }
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) C[fa][fc0 + e] = cr[e];
         __syncthreads();
     }
     for (int i = tid; i < P * P; i += 256) {
@@ -342,11 +361,13 @@ __global__ __launch_bounds__(256) void k_closure_pivots(int nV, int ld, const in
         CdT[k2][k] = dup16(C[k][k2]);
     }
     __syncthreads();
-    const int j0 = (blockIdx.x * 256 + tid) * 2;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int j0 = blockIdx.x * 128 + lane * 2;
     if (j0 >= nV) return;
-    uint32_t acc[P];
+    const int kbase = wave * KW;
+    uint32_t acc[KW];
 #pragma unroll
-    for (int k = 0; k < P; ++k) acc[k] = kInf2;
+    for (int k = 0; k < KW; ++k) acc[k] = kInf2;
     constexpr int PF = 8;  // pivot-row loads in flight
     for (int k0 = 0; k0 < p; k0 += PF) {
         uint32_t dv[PF];
@@ -356,9 +377,9 @@ __global__ __launch_bounds__(256) void k_closure_pivots(int nV, int ld, const in
 #pragma unroll
         for (int u = 0; u < PF; ++u) {
             // CdT rows k2 >= p hold INF off the diagonal; acc[k >= p] is never stored
-            const uint4* c4 = reinterpret_cast<const uint4*>(&CdT[k0 + u][0]);
+            const uint4* c4 = reinterpret_cast<const uint4*>(&CdT[k0 + u][kbase]);
 #pragma unroll
-            for (int k = 0; k < P; k += 4) {
+            for (int k = 0; k < KW; k += 4) {
                 const uint4 cv = c4[k / 4];
                 acc[k + 0] = relax2(acc[k + 0], cv.x, dv[u]);
                 acc[k + 1] = relax2(acc[k + 1], cv.y, dv[u]);
@@ -369,8 +390,8 @@ __global__ __launch_bounds__(256) void k_closure_pivots(int nV, int ld, const in
     }
     uint16_t* R = Rbuf + size_t(slot) * P * ld;
 #pragma unroll
-    for (int k = 0; k < P; ++k)
-        if (k < p) *reinterpret_cast<uint32_t*>(R + size_t(k) * ld + j0) = acc[k];
+    for (int k = 0; k < KW; ++k)
+        if (kbase + k < p) *reinterpret_cast<uint32_t*>(R + size_t(kbase + k) * ld + j0) = acc[k];
 }
 
 // Phase 2: D := min(D, R^T (x) R).  A 256-thread workgroup owns a 128 x 128
@@ -438,7 +459,7 @@ void launch_closure_pivots(int P, int nV, int ld, const int* node_list, int n_li
                            const int* counts, int t, const uint16_t* DU, size_t node_stride, uint16_t* Rbuf,
                            hipStream_t st) {
     if (n_list == 0) return;
-    dim3 grid(unsigned((nV + 511) / 512), unsigned(n_list));
+    dim3 grid(unsigned((nV + 127) / 128), unsigned(n_list));
     if (P == 16)
         hipLaunchKernelGGL(k_closure_pivots<16>, grid, dim3(256), 0, st, nV, ld, node_list, pivots, piv_stride, counts,
                            t, DU, node_stride, Rbuf);

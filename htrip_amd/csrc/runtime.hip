@@ -10,6 +10,7 @@
 // so every table can be compared element for element.
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -90,20 +91,35 @@ hipEvent_t Profiler::get() {
 }
 int Profiler::begin(int cls, hipStream_t st, double bytes, double updates) {
     if (!on) return -1;
+    std::lock_guard<std::mutex> lk(mu);
     Pending p{cls, get(), get(), bytes, updates};
     if (!p.a || !p.b) return -1;
     (void)hipEventRecord(p.a, st);
-    pending.push_back(p);
-    return int(pending.size()) - 1;
+    const int token = next_token++ & 0x3fffffff;
+    open[token] = p;
+    return token;
 }
-void Profiler::end(int idx, hipStream_t st) {
-    if (idx < 0) return;
-    (void)hipEventRecord(pending[idx].b, st);
+void Profiler::end(int token, hipStream_t st) {
+    if (token < 0) return;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = open.find(token);
+    if (it == open.end()) return;
+    (void)hipEventRecord(it->second.b, st);
+    pending.push_back(it->second);
+    open.erase(it);
 }
 void Profiler::harvest() {
+    std::lock_guard<std::mutex> lk(mu);
+    std::vector<Pending> keep;
     for (auto& p : pending) {
         float ms = 0.f;
-        if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+        hipError_t q = hipEventQuery(p.b);
+        if (q == hipErrorNotReady) {
+            (void)hipGetLastError();
+            keep.push_back(p);
+            continue;
+        }
+        if (q == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
             htg_kernel_stats& a = acc[p.cls];
             a.launches++;
             a.total_ms += ms;
@@ -114,13 +130,15 @@ void Profiler::harvest() {
         pool.push_back(p.a);
         pool.push_back(p.b);
     }
-    pending.clear();
+    pending.swap(keep);
 }
 void Profiler::reset() {
     harvest();
+    std::lock_guard<std::mutex> lk(mu);
     for (auto& a : acc) a = htg_kernel_stats{0, 0.0, 0.0, 0.0};
 }
 Profiler::~Profiler() {
+    for (auto& kv : open) pending.push_back(kv.second);
     for (auto& p : pending) {
         pool.push_back(p.a);
         pool.push_back(p.b);
@@ -300,7 +318,7 @@ int Map::run_closure(int lvl, int n_list, const std::vector<int>& hcounts) {
     const int* nodes = static_cast<const int*>(ws_nodes.p);
     for (int t = 0; t < blocks; ++t) {
         double upd_p = 0, upd_u = 0, by_p = 0, by_u = 0;
-        if (prof.on) {
+        if (prof_ref().on) {
             const double nV = L.d.nV;
             for (int i = 0; i < n_list; ++i) {
                 const double p = std::max(0, std::min(P, hcounts[i] - t * P));
@@ -311,16 +329,16 @@ int Map::run_closure(int lvl, int n_list, const std::vector<int>& hcounts) {
                 by_u += (2 * nV * nV + 2 * p * nV) * 2;  // D_U tile in + out, R in
             }
         }
-        int pi = prof.begin(HTG_K_CLOSURE_PIVOTS, stream, by_p, upd_p);
+        int pi = prof_ref().begin(HTG_K_CLOSURE_PIVOTS, stream, by_p, upd_p);
         launch_closure_pivots(P, L.d.nV, L.d.ld, nodes, n_list, static_cast<const int*>(ws_piv.p), L.d.nV,
                               static_cast<const int*>(ws_counts.p), t, static_cast<const uint16_t*>(L.DU.p),
                               L.node_stride, static_cast<uint16_t*>(ws_R.p), stream);
-        prof.end(pi, stream);
-        pi = prof.begin(HTG_K_CLOSURE_UPDATE, stream, by_u, upd_u);
+        prof_ref().end(pi, stream);
+        pi = prof_ref().begin(HTG_K_CLOSURE_UPDATE, stream, by_u, upd_u);
         launch_closure_update(P, L.d.nV, L.d.ld, nodes, n_list, static_cast<const int*>(ws_counts.p), t,
                               static_cast<uint16_t*>(L.DU.p), L.node_stride, static_cast<const uint16_t*>(ws_R.p),
                               stream);
-        prof.end(pi, stream);
+        prof_ref().end(pi, stream);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "closure launch");
@@ -340,10 +358,10 @@ const uint16_t* Map::child_summaries(int lvl, size_t& stride) const {
 int Map::build() {
     hipError_t e;
     const double leaf_bytes = (double(b) * b * b * b + double(4 * b - 4) * (4 * b - 4)) * 2 + double(b) * b;
-    int pi = prof.begin(HTG_K_LEAF_APSP, stream, leaf_bytes * nl * nl, 0);
+    int pi = prof_ref().begin(HTG_K_LEAF_APSP, stream, leaf_bytes * nl * nl, 0);
     launch_leaf_apsp(static_cast<const uint8_t*>(d_pass.p), S, b, nl, nullptr, nl * nl, static_cast<uint16_t*>(leafD.p),
                      static_cast<uint16_t*>(leafS.p), nullptr, stream);
-    prof.end(pi, stream);
+    prof_ref().end(pi, stream);
     if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "leaf apsp launch");
     for (int lvl = 1; lvl <= H; ++lvl) {
         LevelGeo& L = lv[lvl - 1];
@@ -376,10 +394,18 @@ int Map::build() {
         if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "level launch");
     }
     if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "build sync");
-    prof.harvest();
+    prof_ref().harvest();
     built = true;
     epoch++;
     return HTG_OK;
+}
+
+static int debug_level() {
+    static const int lvl = [] {
+        const char* e = std::getenv("HTG_DEBUG");
+        return e ? std::atoi(e) : 0;
+    }();
+    return lvl;
 }
 
 // ---------------------------------------------------------------------------
@@ -391,6 +417,8 @@ int Map::build() {
 // (src/internal_node.cpp:332-413).
 int Map::open_cells(const int32_t* rcs, int n) {
     stats = htg_batch_stats{0, 0, 0};
+    dirty.clear();
+    dirty.nodes.resize(H);
     if (n <= 0) return HTG_OK;
     hipError_t e;
     int rc;
@@ -486,6 +514,8 @@ int Map::open_cells(const int32_t* rcs, int n) {
                         stream);
     }
     const int nd = int(dirty.size());
+    this->dirty.leaves = dirty;
+    this->dirty.cells = newly;
     if ((rc = ws_leaves.ensure(size_t(nd) * 4))) return rc;
     if ((rc = ws_leaf_changed.ensure(size_t(nd) * 4))) return rc;
     if ((e = hipMemcpyAsync(ws_leaves.p, dirty.data(), size_t(nd) * 4, hipMemcpyHostToDevice, stream)) != hipSuccess)
@@ -493,11 +523,11 @@ int Map::open_cells(const int32_t* rcs, int n) {
     if ((e = hipMemsetAsync(ws_leaf_changed.p, 0, size_t(nd) * 4, stream)) != hipSuccess)
         return hip_fail(e, "clear leaf flags");
     const double leaf_bytes = (double(b) * b * b * b + double(4 * b - 4) * (4 * b - 4)) * 2 + double(b) * b;
-    int lpi = prof.begin(HTG_K_LEAF_APSP, stream, leaf_bytes * nd, 0);
+    int lpi = prof_ref().begin(HTG_K_LEAF_APSP, stream, leaf_bytes * nd, 0);
     launch_leaf_apsp(static_cast<const uint8_t*>(d_pass.p), S, b, nl, static_cast<const int*>(ws_leaves.p), nd,
                      static_cast<uint16_t*>(leafD.p), static_cast<uint16_t*>(leafS.p),
                      static_cast<int*>(ws_leaf_changed.p), stream);
-    prof.end(lpi, stream);
+    prof_ref().end(lpi, stream);
     stats.n_touched_leaves = nd;
     stats.n_pivots = piv;
     if (H == 0) {
@@ -536,6 +566,7 @@ int Map::open_cells(const int32_t* rcs, int n) {
         }
         anc += int(active.size());
         const int na = int(active.size());
+        this->dirty.nodes[lvl - 1] = active;
         if ((rc = ensure_build_ws(lvl))) return rc;
         std::vector<uint32_t> hmask(na);
         std::vector<int4> edges;
@@ -574,6 +605,16 @@ int Map::open_cells(const int32_t* rcs, int n) {
         if ((e = hipMemcpyAsync(hc.data(), ws_counts.p, size_t(na) * 4, hipMemcpyDeviceToHost, stream)) != hipSuccess)
             return hip_fail(e, "download counts");
         if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "update sync");
+        if (debug_level()) {
+            long long sum = 0;
+            int mx = 0;
+            for (int v : hc) {
+                sum += v;
+                mx = std::max(mx, v);
+            }
+            std::fprintf(stderr, "[htg] update lvl %d: %d nodes (nV %d), pivots sum %lld max %d\n", lvl, na, L.d.nV,
+                         sum, mx);
+        }
         if ((rc = run_closure(lvl, na, hc))) return rc;
         // S_U refresh for nodes that closed over something (the reference
         // skips nodes with an empty pivot set, src/internal_node.cpp:368-370)
@@ -605,8 +646,47 @@ int Map::open_cells(const int32_t* rcs, int n) {
     }
     stats.n_dirty_ancestors = anc;
     if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "update sync");
-    prof.harvest();
+    prof_ref().harvest();
     epoch++;
+    return HTG_OK;
+}
+
+int Map::catch_up_from(const Map& src, const DirtySet& ds) {
+    hipError_t e;
+    const size_t n = size_t(b) * b, pl = size_t(4 * b - 4);
+    for (const int2& c : ds.cells) {
+        pass[size_t(c.x) * S + c.y] = 1;
+        cells_code[size_t(c.x) * cols + c.y] = 1;
+    }
+    if (!ds.cells.empty() &&
+        (e = hipMemcpyAsync(d_pass.p, src.d_pass.p, size_t(S) * S, hipMemcpyDeviceToDevice, stream)) != hipSuccess)
+        return hip_fail(e, "catch-up pass");
+    for (int leaf : ds.leaves) {
+        if ((e = hipMemcpyAsync(static_cast<uint16_t*>(leafD.p) + size_t(leaf) * n * n,
+                                static_cast<const uint16_t*>(src.leafD.p) + size_t(leaf) * n * n, n * n * 2,
+                                hipMemcpyDeviceToDevice, stream)) != hipSuccess ||
+            (e = hipMemcpyAsync(static_cast<uint16_t*>(leafS.p) + size_t(leaf) * pl * pl,
+                                static_cast<const uint16_t*>(src.leafS.p) + size_t(leaf) * pl * pl, pl * pl * 2,
+                                hipMemcpyDeviceToDevice, stream)) != hipSuccess)
+            return hip_fail(e, "catch-up leaf");
+    }
+    for (size_t l = 0; l < ds.nodes.size() && l < lv.size(); ++l) {
+        LevelGeo& L = lv[l];
+        const LevelGeo& Ls = src.lv[l];
+        const size_t ne2 = size_t(L.d.ne) * L.d.ne;
+        for (int node : ds.nodes[l]) {
+            if ((e = hipMemcpyAsync(static_cast<uint16_t*>(L.DU.p) + size_t(node) * L.node_stride,
+                                    static_cast<const uint16_t*>(Ls.DU.p) + size_t(node) * L.node_stride,
+                                    L.node_stride * 2, hipMemcpyDeviceToDevice, stream)) != hipSuccess ||
+                (e = hipMemcpyAsync(static_cast<uint16_t*>(L.SU.p) + size_t(node) * ne2,
+                                    static_cast<const uint16_t*>(Ls.SU.p) + size_t(node) * ne2, ne2 * 2,
+                                    hipMemcpyDeviceToDevice, stream)) != hipSuccess)
+                return hip_fail(e, "catch-up node");
+        }
+    }
+    if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "catch-up sync");
+    stats = src.stats;
+    epoch = src.epoch;
     return HTG_OK;
 }
 
@@ -624,7 +704,7 @@ int Map::query_finish(QueryWS& ws, uint16_t* out_host) {
     hipError_t e = hipEventSynchronize(ws.done);
     if (e != hipSuccess) return hip_fail(e, "query sync");
     if (out_host && ws.total_out) std::memcpy(out_host, ws.h_out.p, size_t(ws.total_out) * 2);
-    prof.harvest();
+    prof_ref().harvest();
     return HTG_OK;
 }
 
@@ -807,7 +887,7 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
     for (int l = 1; l <= H && ns > 0; ++l) {
         const LevelGeo& L = lv[l - 1];
         const LevelPlan& LP = P.levels[l - 1];
-        int pi = prof.begin(HTG_K_Q_PROJECT, st, LP.z_bytes, LP.z_updates);
+        int pi = prof_ref().begin(HTG_K_Q_PROJECT, st, LP.z_bytes, LP.z_updates);
         const size_t part_stride = nsl * L.ldp;
         launch_q_z(LP.M, L.d.ld, L.d.pc, static_cast<const uint16_t*>(L.DU.p), L.node_stride,
                    static_cast<const int*>(L.d_seam_list.p), static_cast<const int*>(L.d_seam_off.p),
@@ -816,17 +896,17 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
                    static_cast<uint16_t*>(ws.Zp.p), part_stride, st);
         launch_q_zred(ns, L.nz, L.ldz, static_cast<const int*>(L.d_zsrc.p), L.ldp, LP.ksplit,
                       static_cast<const uint16_t*>(ws.Zp.p), part_stride, Z, st);
-        prof.end(pi, st);
-        pi = prof.begin(HTG_K_Q_MEET, st, LP.m_bytes, LP.m_updates);
+        prof_ref().end(pi, st);
+        pi = prof_ref().begin(HTG_K_Q_MEET, st, LP.m_bytes, LP.m_updates);
         launch_q_meet(I(o_tiles[l - 1]), int(LP.mtiles.size() / 8), reinterpret_cast<const int4*>(dp + o_groups[l - 1]),
                       I(o_glist[l - 1]), static_cast<const int*>(L.d_zoff.p), static_cast<const int*>(L.d_zlen.p),
                       L.ldz, I(o_si), I(o_qk), reinterpret_cast<const long long*>(dp + o_qout), Z, dout, st);
-        prof.end(pi, st);
+        prof_ref().end(pi, st);
         if (l < H) {
-            pi = prof.begin(HTG_K_Q_LIFT, st, double(ns) * (2.0 * L.d.ne + L.d.pc) * 2, 0);
+            pi = prof_ref().begin(HTG_K_Q_LIFT, st, double(ns) * (2.0 * L.d.ne + L.d.pc) * 2, 0);
             launch_q_lift(ns, L.d.ne, static_cast<const int*>(L.d_ext_zpos.p), static_cast<const int*>(L.d_ext_child.p),
                           static_cast<const int*>(L.d_ext_cport.p), L.d.pc, L.d.cs, g, L.ldz, srcs, Xa, Z, Xb, st);
-            prof.end(pi, st);
+            prof_ref().end(pi, st);
             std::swap(Xa, Xb);
         }
     }
@@ -1154,16 +1234,16 @@ int htg_read_node_SU(htg_map* map, int32_t lvl, int32_t idx, uint16_t* out) {
 
 int htg_profile_enable(htg_map* map, int on) {
     if (!map) return fail(HTG_E_INVALID, "null map");
-    map->prof.reset();
-    map->prof.on = on != 0;
+    map->prof_ref().reset();
+    map->prof_ref().on = on != 0;
     return HTG_OK;
 }
 
 int htg_profile_get(htg_map* map, int kernel_class, htg_kernel_stats* out) {
     if (!map || !out) return fail(HTG_E_INVALID, "null argument");
     if (kernel_class < 0 || kernel_class >= HTG_K_COUNT) return fail(HTG_E_RANGE, "kernel class");
-    map->prof.harvest();
-    *out = map->prof.acc[kernel_class];
+    map->prof_ref().harvest();
+    *out = map->prof_ref().acc[kernel_class];
     return HTG_OK;
 }
 

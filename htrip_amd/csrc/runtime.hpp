@@ -4,7 +4,10 @@
 
 #include <cstdint>
 #include <cstring>
+#include <memory>
+#include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/htrip_gpu.h"
@@ -137,25 +140,41 @@ struct QueryWS {
     ~QueryWS();
 };
 
-// Event-bracketed launch timing (see htg_profile_enable).
+// Event-bracketed launch timing (see htg_profile_enable).  Thread-safe; a
+// session's two snapshot maps share one instance.
 struct Profiler {
+    std::mutex mu;
     bool on = false;
     struct Pending {
         int cls;
         hipEvent_t a, b;
         double bytes, updates;
     };
-    std::vector<Pending> pending;
+    std::vector<Pending> pending;          // ended, not yet harvested
+    std::unordered_map<int, Pending> open;  // begun, not yet ended
+    int next_token = 0;
     std::vector<hipEvent_t> pool;
     htg_kernel_stats acc[HTG_K_COUNT]{};
     hipEvent_t get();
-    // Records the start event; returns an index to pass to end().
+    // Records the start event; returns a token to pass to end().
     int begin(int cls, hipStream_t st, double bytes, double updates);
     void end(int idx, hipStream_t st);
-    // Accumulate every completed pair (call after the streams synchronized).
+    // Accumulate every completed pair; pairs still running stay pending.
     void harvest();
     void reset();
     ~Profiler();
+};
+
+// Tables an update rewrote: recomputed leaves and, per level, touched nodes.
+struct DirtySet {
+    std::vector<int> leaves;
+    std::vector<std::vector<int>> nodes;  // nodes[l-1]
+    std::vector<int2> cells;              // cells that became passable
+    void clear() {
+        leaves.clear();
+        nodes.clear();
+        cells.clear();
+    }
 };
 
 struct Map {
@@ -173,7 +192,9 @@ struct Map {
     uint64_t budget = 0, device_bytes = 0;
     htg_tuning tune{64, 16, 0, 0};
     QueryWS qws;
-    Profiler prof;
+    std::shared_ptr<Profiler> profp = std::make_shared<Profiler>();
+    Profiler& prof_ref() { return *profp; }
+    DirtySet dirty;  // filled by open_cells
     DevBuf ws_nodes, ws_flags, ws_piv, ws_counts, ws_R, ws_changed, ws_mask, ws_edges, ws_cells, ws_leaves,
         ws_leaf_changed;
 
@@ -187,6 +208,9 @@ struct Map {
     int ensure_build_ws(int lvl);
     int run_closure(int lvl, int n_list, const std::vector<int>& hcounts);
     const uint16_t* child_summaries(int lvl, size_t& stride) const;
+    // Make this map equal to src after src applied one update with dirty set
+    // ds (this map equal to src before it): copies only the rewritten tables.
+    int catch_up_from(const Map& src, const DirtySet& ds);
 };
 
 }  // namespace htg

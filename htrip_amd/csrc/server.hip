@@ -3,18 +3,25 @@
 //
 // The reference's only serving-like caller is its robot simulator, which
 // double-buffers the oracle and runs updates and queries on worker threads
-// (src/robot_simulator.cpp:897-1058, :1483-1580).  Here one worker thread
-// owns every resident map:
-//  * requests of a session are served in submission order: a query observes
-//    exactly the updates submitted before it;
-//  * consecutive queries of a session are coalesced into one batched launch
-//    (htg_query_batch semantics), consecutive updates into one reveal batch
-//    (the union of monotone openings equals applying them in turn);
-//  * up to n_slots batches are in flight on their own streams, so batches of
-//    different sessions overlap on the device, and requests that arrive while
-//    batches run are picked up as soon as a slot frees (no tick barrier);
-//  * an update waits until its session's in-flight query batches completed
-//    (they read the tables it rewrites).
+// (src/robot_simulator.cpp:897-1058, :1483-1580).  Two consistency modes:
+//
+//  strict   -- requests of a session are served in submission order: a query
+//              observes exactly the updates submitted before it.  Consecutive
+//              queries are coalesced into one batched launch, consecutive
+//              updates into one reveal batch (the union of monotone openings
+//              equals applying them in turn); an update waits until the
+//              session's in-flight query batches completed.
+//  snapshot -- every session holds two copies of its hierarchy.  Queries run
+//              on the front copy as soon as a slot frees and never wait for
+//              updates; an update thread merges all pending updates, applies
+//              them to the back copy (after bringing it level by copying the
+//              tables the previous update rewrote) and swaps front and back.
+//              A query observes a prefix of its session's updates; the
+//              prefix length is reported with the result.
+//
+// In both modes up to n_slots query batches are in flight on their own
+// streams, batches of different sessions overlap on the device, and requests
+// that arrive while batches run join the next batch as soon as a slot frees.
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -43,6 +50,7 @@ struct Request {
     int32_t count = 0;
     uint16_t* out = nullptr;
     Clock::time_point t_submit, t_done;
+    int64_t version = 0;  // query: updates in its snapshot; update: its index in the session
     int status = HTG_OK;
     bool done = false;
 };
@@ -50,17 +58,30 @@ struct Request {
 struct Batch {
     QueryWS ws;
     int session = -1;
+    int map_idx = 0;
     std::vector<Request*> reqs;
     bool busy = false;
 };
 
 struct Session {
-    htg_map* map = nullptr;
+    htg_map* maps[2] = {nullptr, nullptr};
+    int front = 0;
     uint64_t bytes = 0;
-    std::deque<Request*> pending;
-    int inflight = 0;  // query batches on the device
-    int updating = 0;  // an update running on the worker
+    std::deque<Request*> pending;      // strict: everything; snapshot: queries
+    std::deque<Request*> upd_pending;  // snapshot: updates
+    int inflight[2] = {0, 0};          // query batches on the device, per copy
+    int updating = 0;
+    int64_t submitted_updates = 0;
+    int64_t applied = 0;  // update requests applied to the front copy
+    bool back_stale = false;
+    DirtySet last_dirty;
 };
+
+void mark_done(Request* r, int st, Clock::time_point now) {
+    r->status = st;
+    r->t_done = now;
+    r->done = true;
+}
 
 }  // namespace
 
@@ -68,46 +89,24 @@ struct htg_server {
     htg_ctx* ctx = nullptr;
     htg_server_config cfg{};
     std::mutex mu;
-    std::condition_variable cv_work, cv_done;
+    std::condition_variable cv_work, cv_done, cv_upd;
     std::map<int, Session> sessions;
     int next_session = 1;
     uint64_t used = 0;
     std::vector<std::unique_ptr<Batch>> pool;
     std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
     uint64_t next_ticket = 1;
-    std::thread worker;
+    std::thread worker, updater;
     bool stop = false;
     htg_server_stats stats{};
-    std::string worker_error;
 
+    bool snapshot() const { return cfg.consistency == HTG_CONSISTENCY_SNAPSHOT; }
     void run();
+    void run_updates();
     bool retire();
     bool dispatch();
+    bool update_one(std::unique_lock<std::mutex>& lk);
 };
-
-namespace {
-
-// Copy a finished batch's matrices to the callers and mark them done.
-void finish_batch(htg_server* s, Batch& b) {
-    Session& ss = s->sessions.at(b.session);
-    int rc = ss.map->query_finish(b.ws, nullptr);
-    const uint16_t* res = static_cast<const uint16_t*>(b.ws.h_out.p);
-    long long off = 0;
-    const auto now = Clock::now();
-    for (Request* r : b.reqs) {
-        const size_t n = size_t(r->count) * r->count;
-        if (rc == HTG_OK && r->out && n) std::memcpy(r->out, res + off, n * 2);
-        off += (long long)n;
-        r->status = rc;
-        r->t_done = now;
-        r->done = true;
-    }
-    ss.inflight--;
-    b.busy = false;
-    b.reqs.clear();
-}
-
-}  // namespace
 
 bool htg_server::retire() {
     bool any = false;
@@ -119,10 +118,26 @@ bool htg_server::retire() {
             continue;
         }
         std::lock_guard<std::mutex> lk(mu);
-        finish_batch(this, b);
+        Session& ss = sessions.at(b.session);
+        const int rc = ss.maps[b.map_idx]->query_finish(b.ws, nullptr);
+        const uint16_t* res = static_cast<const uint16_t*>(b.ws.h_out.p);
+        long long off = 0;
+        const auto now = Clock::now();
+        for (Request* r : b.reqs) {
+            const size_t n = size_t(r->count) * r->count;
+            if (rc == HTG_OK && r->out && n) std::memcpy(r->out, res + off, n * 2);
+            off += (long long)n;
+            mark_done(r, rc, now);
+        }
+        ss.inflight[b.map_idx]--;
+        b.busy = false;
+        b.reqs.clear();
         any = true;
     }
-    if (any) cv_done.notify_all();
+    if (any) {
+        cv_done.notify_all();
+        cv_upd.notify_all();
+    }
     return any;
 }
 
@@ -157,16 +172,19 @@ bool htg_server::dispatch() {
                     run.push_back(ss.pending.front());
                     ss.pending.pop_front();
                 }
+                const int mi = snapshot() ? ss.front : 0;
                 fb->busy = true;
                 fb->session = id;
+                fb->map_idx = mi;
                 fb->reqs = run;
-                ss.inflight++;
+                ss.inflight[mi]++;
                 std::vector<int32_t> rc, ks;
                 for (Request* r : run) {
                     rc.insert(rc.end(), r->rc.begin(), r->rc.end());
                     ks.push_back(r->count);
+                    r->version = ss.applied;
                 }
-                htg_map* m = ss.map;
+                htg_map* m = ss.maps[mi];
                 lk.unlock();
                 const int st = m->query_launch(rc.data(), ks.data(), int(ks.size()), fb->ws, true);
                 lk.lock();
@@ -174,18 +192,14 @@ bool htg_server::dispatch() {
                 stats.queries += run.size();
                 if (st != HTG_OK) {  // fail this batch's requests, keep serving the others
                     const auto now = Clock::now();
-                    for (Request* r : run) {
-                        r->status = st;
-                        r->t_done = now;
-                        r->done = true;
-                    }
-                    ss.inflight--;
+                    for (Request* r : run) mark_done(r, st, now);
+                    ss.inflight[mi]--;
                     fb->busy = false;
                     fb->reqs.clear();
                     cv_done.notify_all();
                 }
-            } else {
-                if (ss.inflight > 0) break;  // its query batches still read the tables
+            } else {  // strict mode only: updates are queued with the queries
+                if (ss.inflight[0] > 0) break;  // its query batches still read the tables
                 while (!ss.pending.empty() && ss.pending.front()->type == 1) {
                     run.push_back(ss.pending.front());
                     ss.pending.pop_front();
@@ -193,19 +207,16 @@ bool htg_server::dispatch() {
                 ss.updating++;
                 std::vector<int32_t> cells;
                 for (Request* r : run) cells.insert(cells.end(), r->rc.begin(), r->rc.end());
-                htg_map* m = ss.map;
+                htg_map* m = ss.maps[0];
                 lk.unlock();
                 const int st = m->open_cells(cells.data(), int(cells.size() / 2));
                 lk.lock();
                 ss.updating--;
+                ss.applied += int64_t(run.size());
                 stats.update_batches++;
                 stats.updates += run.size();
                 const auto now = Clock::now();
-                for (Request* r : run) {
-                    r->status = st;
-                    r->t_done = now;
-                    r->done = true;
-                }
+                for (Request* r : run) mark_done(r, st, now);
                 cv_done.notify_all();
             }
             any = true;
@@ -215,16 +226,12 @@ bool htg_server::dispatch() {
 }
 
 void htg_server::run() {
-    if (hipSetDevice(ctx->device) != hipSuccess) {
-        worker_error = "hipSetDevice failed in the server worker";
-        return;
-    }
+    if (hipSetDevice(ctx->device) != hipSuccess) return;
     for (;;) {
-        bool inflight = false;
         {
             std::unique_lock<std::mutex> lk(mu);
+            bool inflight = false, pending = false;
             for (auto& bp : pool) inflight |= bp->busy;
-            bool pending = false;
             for (auto& kv : sessions) pending |= !kv.second.pending.empty();
             if (stop && !pending && !inflight) return;
             if (!pending && !inflight) {
@@ -238,13 +245,65 @@ void htg_server::run() {
     }
 }
 
+// Snapshot mode: apply one session's pending updates to its back copy and
+// swap.  Called with lk held; returns false when no session had work.
+bool htg_server::update_one(std::unique_lock<std::mutex>& lk) {
+    for (auto& kv : sessions) {
+        Session& ss = kv.second;
+        if (ss.upd_pending.empty() || ss.updating) continue;
+        const int back = 1 - ss.front;
+        if (ss.inflight[back] > 0) continue;  // queries still read the back copy
+        std::vector<Request*> run(ss.upd_pending.begin(), ss.upd_pending.end());
+        ss.upd_pending.clear();
+        ss.updating++;
+        std::vector<int32_t> cells;
+        for (Request* r : run) cells.insert(cells.end(), r->rc.begin(), r->rc.end());
+        htg_map* fm = ss.maps[ss.front];
+        htg_map* bm = ss.maps[back];
+        const bool stale = ss.back_stale;
+        DirtySet prev = ss.last_dirty;
+        lk.unlock();
+        int st = stale ? bm->catch_up_from(*fm, prev) : HTG_OK;
+        if (st == HTG_OK) st = bm->open_cells(cells.data(), int(cells.size() / 2));
+        lk.lock();
+        ss.updating--;
+        if (st == HTG_OK) {
+            ss.front = back;
+            ss.applied += int64_t(run.size());
+            ss.last_dirty = bm->dirty;
+            ss.back_stale = true;
+        }
+        stats.update_batches++;
+        stats.updates += run.size();
+        const auto now = Clock::now();
+        for (Request* r : run) mark_done(r, st, now);
+        cv_done.notify_all();
+        return true;
+    }
+    return false;
+}
+
+void htg_server::run_updates() {
+    if (hipSetDevice(ctx->device) != hipSuccess) return;
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+        if (update_one(lk)) continue;
+        bool pending = false;
+        for (auto& kv : sessions) pending |= !kv.second.upd_pending.empty();
+        if (stop && !pending) return;
+        // woken by submissions and by retired batches (the back copy frees up)
+        cv_upd.wait_for(lk, std::chrono::microseconds(200));
+    }
+}
+
 // ===========================================================================
 extern "C" {
 
 int htg_server_create(htg_ctx* ctx, const htg_server_config* cfg, htg_server** out) {
     try {
         if (!ctx || !cfg || !out) return fail(HTG_E_INVALID, "null argument");
-        if (cfg->n_slots < 1 || cfg->n_slots > 64 || cfg->max_batch_queries < 1 || cfg->max_batch_viewpoints < 1)
+        if (cfg->n_slots < 1 || cfg->n_slots > 64 || cfg->max_batch_queries < 1 || cfg->max_batch_viewpoints < 1 ||
+            (cfg->consistency != HTG_CONSISTENCY_STRICT && cfg->consistency != HTG_CONSISTENCY_SNAPSHOT))
             return fail(HTG_E_INVALID, "bad server configuration");
         hipError_t e = hipSetDevice(ctx->device);
         if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
@@ -258,6 +317,7 @@ int htg_server_create(htg_ctx* ctx, const htg_server_config* cfg, htg_server** o
         }
         htg_server* raw = s.get();
         s->worker = std::thread([raw] { raw->run(); });
+        if (raw->snapshot()) s->updater = std::thread([raw] { raw->run_updates(); });
         *out = s.release();
         return HTG_OK;
     } catch (...) {
@@ -272,8 +332,12 @@ int htg_server_destroy(htg_server* s) {
         s->stop = true;
     }
     s->cv_work.notify_all();
+    s->cv_upd.notify_all();
     if (s->worker.joinable()) s->worker.join();
-    for (auto& kv : s->sessions) htg_map_destroy(kv.second.map);
+    if (s->updater.joinable()) s->updater.join();
+    for (auto& kv : s->sessions)
+        for (htg_map* m : kv.second.maps)
+            if (m) htg_map_destroy(m);
     delete s;
     return HTG_OK;
 }
@@ -284,6 +348,8 @@ int htg_server_add_map(htg_server* s, const htg_config* cfg, int32_t rows, int32
     uint64_t need = 0;
     int rc = htg_map_footprint(cfg, rows, cols, &need);
     if (rc) return rc;
+    const int copies = s->snapshot() ? 2 : 1;
+    need *= uint64_t(copies);
     {
         std::lock_guard<std::mutex> lk(s->mu);
         if (s->cfg.hbm_budget && s->used + need > s->cfg.hbm_budget)
@@ -291,18 +357,25 @@ int htg_server_add_map(htg_server* s, const htg_config* cfg, int32_t rows, int32
                                          std::to_string(s->cfg.hbm_budget - s->used) + " left in the budget");
         s->used += need;  // reserve before building
     }
-    htg_map* m = nullptr;
-    rc = htg_map_create(s->ctx, cfg, rows, cols, cells, 0, &m);
-    if (rc == HTG_OK) rc = htg_build(m);
+    htg_map* m[2] = {nullptr, nullptr};
+    rc = htg_map_create(s->ctx, cfg, rows, cols, cells, 0, &m[0]);
+    if (rc == HTG_OK) rc = htg_build(m[0]);
+    if (rc == HTG_OK && copies == 2) {
+        rc = htg_map_clone(m[0], 0, &m[1]);
+        if (rc == HTG_OK) m[1]->profp = m[0]->profp;  // one profiler per session
+    }
     std::lock_guard<std::mutex> lk(s->mu);
     if (rc != HTG_OK) {
-        if (m) htg_map_destroy(m);
+        for (htg_map* x : m)
+            if (x) htg_map_destroy(x);
         s->used -= need;
         return rc;
     }
     const int id = s->next_session++;
-    s->sessions[id].map = m;
-    s->sessions[id].bytes = need;
+    Session& ss = s->sessions[id];
+    ss.maps[0] = m[0];
+    ss.maps[1] = m[1];
+    ss.bytes = need;
     *session = id;
     return HTG_OK;
 }
@@ -312,15 +385,17 @@ int htg_server_remove_map(htg_server* s, int32_t session) {
     std::unique_lock<std::mutex> lk(s->mu);
     auto it = s->sessions.find(session);
     if (it == s->sessions.end()) return fail(HTG_E_RANGE, "no such session");
-    // drain this session's work first
+    Session& ss = it->second;
     s->cv_done.wait(lk, [&] {
-        return it->second.pending.empty() && it->second.inflight == 0 && it->second.updating == 0;
+        return ss.pending.empty() && ss.upd_pending.empty() && ss.inflight[0] == 0 && ss.inflight[1] == 0 &&
+               ss.updating == 0;
     });
-    htg_map* m = it->second.map;
-    s->used -= it->second.bytes;
+    htg_map* m[2] = {ss.maps[0], ss.maps[1]};
+    s->used -= ss.bytes;
     s->sessions.erase(it);
     lk.unlock();
-    htg_map_destroy(m);
+    for (htg_map* x : m)
+        if (x) htg_map_destroy(x);
     return HTG_OK;
 }
 
@@ -338,12 +413,20 @@ static int submit(htg_server* s, int32_t session, int type, const int32_t* rc, i
         std::lock_guard<std::mutex> lk(s->mu);
         auto it = s->sessions.find(session);
         if (it == s->sessions.end()) return fail(HTG_E_RANGE, "no such session");
+        Session& ss = it->second;
         r->ticket = s->next_ticket++;
         *ticket = r->ticket;
-        it->second.pending.push_back(r.get());
+        if (type == 1) r->version = ss.submitted_updates++;
+        if (type == 1 && s->snapshot())
+            ss.upd_pending.push_back(r.get());
+        else
+            ss.pending.push_back(r.get());
         s->reqs[r->ticket] = std::move(r);
     }
-    s->cv_work.notify_one();
+    if (type == 1 && s->snapshot())
+        s->cv_upd.notify_one();
+    else
+        s->cv_work.notify_one();
     return HTG_OK;
 }
 
@@ -356,7 +439,7 @@ int htg_server_submit_update(htg_server* s, int32_t session, const int32_t* rc, 
     return submit(s, session, 1, rc, n, nullptr, ticket);
 }
 
-int htg_server_wait(htg_server* s, uint64_t ticket, double* latency_s) {
+int htg_server_wait_ex(htg_server* s, uint64_t ticket, double* latency_s, int64_t* version) {
     if (!s) return fail(HTG_E_INVALID, "null server");
     std::unique_lock<std::mutex> lk(s->mu);
     auto it = s->reqs.find(ticket);
@@ -365,9 +448,14 @@ int htg_server_wait(htg_server* s, uint64_t ticket, double* latency_s) {
     s->cv_done.wait(lk, [&] { return r->done; });
     const int st = r->status;
     if (latency_s) *latency_s = std::chrono::duration<double>(r->t_done - r->t_submit).count();
+    if (version) *version = r->version;
     s->reqs.erase(it);
     if (st != HTG_OK) return fail(st, "request failed");
     return HTG_OK;
+}
+
+int htg_server_wait(htg_server* s, uint64_t ticket, double* latency_s) {
+    return htg_server_wait_ex(s, ticket, latency_s, nullptr);
 }
 
 int htg_server_stats_get(htg_server* s, htg_server_stats* out) {
@@ -384,13 +472,13 @@ int htg_server_map(htg_server* s, int32_t session, htg_map** out) {
     std::lock_guard<std::mutex> lk(s->mu);
     auto it = s->sessions.find(session);
     if (it == s->sessions.end()) return fail(HTG_E_RANGE, "no such session");
-    *out = it->second.map;
+    *out = it->second.maps[it->second.front];
     return HTG_OK;
 }
 
 int htg_server_replay(htg_server* s, int32_t n, const double* t_arrival, const int32_t* session, const int32_t* type,
                       const int32_t* count, const int64_t* rc_off, const int32_t* rc, const int64_t* out_off,
-                      uint16_t* out, double* t_done) {
+                      uint16_t* out, double* t_done, int64_t* version) {
     if (!s || n < 0 || (n > 0 && (!t_arrival || !session || !type || !count || !rc_off || !rc || !t_done)))
         return fail(HTG_E_INVALID, "null argument");
     std::vector<uint64_t> tickets(size_t(n), 0);
@@ -414,6 +502,7 @@ int htg_server_replay(htg_server* s, int32_t n, const double* t_arrival, const i
         Request* r = it->second.get();
         s->cv_done.wait(lk, [&] { return r->done; });
         t_done[i] = std::chrono::duration<double>(r->t_done - t0).count();
+        if (version) version[i] = r->version;
         if (r->status != HTG_OK && first_err == HTG_OK) first_err = r->status;
         s->reqs.erase(it);
     }

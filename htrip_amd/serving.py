@@ -20,9 +20,12 @@ from . import _lib, grids
 from ._lib import Config, check, ptr
 
 
+STRICT, SNAPSHOT = 0, 1
+
+
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("n_slots", C.c_int32), ("max_batch_queries", C.c_int32),
-                ("max_batch_viewpoints", C.c_int32), ("reserved", C.c_int32)]
+                ("max_batch_viewpoints", C.c_int32), ("consistency", C.c_int32)]
 
 
 class ServerStats(C.Structure):
@@ -44,9 +47,10 @@ def _bind(lib):
         "htg_server_submit_query": (C.c_int, [vp, i32, pi32, i32, pu16, C.POINTER(u64)]),
         "htg_server_submit_update": (C.c_int, [vp, i32, pi32, i32, C.POINTER(u64)]),
         "htg_server_wait": (C.c_int, [vp, u64, pf64]),
+        "htg_server_wait_ex": (C.c_int, [vp, u64, pf64, pi64]),
         "htg_server_stats_get": (C.c_int, [vp, C.POINTER(ServerStats)]),
         "htg_server_map": (C.c_int, [vp, i32, C.POINTER(vp)]),
-        "htg_server_replay": (C.c_int, [vp, i32, pf64, pi32, pi32, pi32, pi64, pi32, pi64, pu16, pf64]),
+        "htg_server_replay": (C.c_int, [vp, i32, pf64, pi32, pi32, pi32, pi64, pi32, pi64, pu16, pf64, pi64]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -56,20 +60,20 @@ def _bind(lib):
 
 
 SERVER_EXPORTS = ["htg_server_create", "htg_server_destroy", "htg_server_add_map", "htg_server_remove_map",
-                  "htg_server_submit_query", "htg_server_submit_update", "htg_server_wait", "htg_server_stats_get",
-                  "htg_server_map", "htg_server_replay"]
+                  "htg_server_submit_query", "htg_server_submit_update", "htg_server_wait", "htg_server_wait_ex",
+                  "htg_server_stats_get", "htg_server_map", "htg_server_replay"]
 
 
 class Server:
     """Resident sessions + continuous batching (htg_server)."""
 
     def __init__(self, hbm_budget: int = 0, n_slots: int = 4, max_batch_queries: int = 64,
-                 max_batch_viewpoints: int = 8192, ctx=None):
+                 max_batch_viewpoints: int = 8192, consistency: int = STRICT, ctx=None):
         self.ctx = ctx or _lib.default_context()
         self.lib = self.ctx.lib
         _bind(self.lib)
         self.h = C.c_void_p()
-        cfg = ServerConfig(hbm_budget, n_slots, max_batch_queries, max_batch_viewpoints, 0)
+        cfg = ServerConfig(hbm_budget, n_slots, max_batch_queries, max_batch_viewpoints, consistency)
         check(self.lib.htg_server_create(self.ctx.h, C.byref(cfg), C.byref(self.h)))
         self._keep = {}
 
@@ -110,12 +114,13 @@ class Server:
         check(self.lib.htg_server_submit_update(self.h, sid, ptr(cells, C.c_int32), len(cells), C.byref(t)))
         return t.value
 
-    def wait(self, ticket: int) -> float:
+    def wait(self, ticket: int, with_version: bool = False):
         lat = C.c_double()
-        rc = self.lib.htg_server_wait(self.h, ticket, C.byref(lat))
+        ver = C.c_int64()
+        rc = self.lib.htg_server_wait_ex(self.h, ticket, C.byref(lat), C.byref(ver))
         self._keep.pop(ticket, None)
         check(rc)
-        return lat.value
+        return (lat.value, ver.value) if with_version else lat.value
 
     def stats(self) -> dict:
         s = ServerStats()
@@ -131,13 +136,16 @@ class Server:
         return m
 
     def replay(self, trace: "Trace") -> np.ndarray:
-        """Replay a trace with native arrival pacing; returns completion times (s)."""
+        """Replay a trace with native arrival pacing; returns completion times (s)
+        and stores each request's version in trace.version."""
         n = len(trace.t)
         t_done = np.zeros(n, dtype=np.float64)
+        trace.version = np.zeros(n, dtype=np.int64)
         check(self.lib.htg_server_replay(
             self.h, n, ptr(trace.t, C.c_double), ptr(trace.session, C.c_int32), ptr(trace.type, C.c_int32),
             ptr(trace.count, C.c_int32), ptr(trace.rc_off, C.c_int64), ptr(trace.rc, C.c_int32),
-            ptr(trace.out_off, C.c_int64), ptr(trace.out, C.c_uint16), ptr(t_done, C.c_double)))
+            ptr(trace.out_off, C.c_int64), ptr(trace.out, C.c_uint16), ptr(t_done, C.c_double),
+            ptr(trace.version, C.c_int64)))
         return t_done
 
 
@@ -152,6 +160,7 @@ class Trace:
     out_off: np.ndarray      # int64 offsets into out (-1: none)
     out: np.ndarray          # uint16 results
     meta: dict = field(default_factory=dict)
+    version: np.ndarray | None = None  # filled by Server.replay
 
     def query_result(self, i: int) -> np.ndarray:
         k = int(self.count[i])

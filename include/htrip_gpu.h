@@ -150,20 +150,26 @@ int htg_profile_enable(htg_map* map, int on);
 int htg_profile_get(htg_map* map, int kernel_class, htg_kernel_stats* out);
 
 /* ---- serving ------------------------------------------------------------
- * A server keeps every admitted map resident and serves requests from a
- * worker thread.  Requests of one session are served in submission order (a
- * query sees exactly the updates submitted before it); consecutive queries of
- * a session are coalesced into one batched launch, consecutive updates into
- * one reveal batch; up to n_slots query batches are in flight at once on
- * their own streams, and requests that arrive meanwhile join the next batch
- * as soon as a slot frees.  No counterpart in the reference (its simulator
- * double-buffers whole oracles instead, src/robot_simulator.cpp:897-1058). */
+ * A server keeps every admitted map resident and serves requests from worker
+ * threads.  Strict mode: requests of one session are served in submission
+ * order (a query sees exactly the updates submitted before it); consecutive
+ * queries are coalesced into one batched launch, consecutive updates into one
+ * reveal batch.  Snapshot mode: each session is double-buffered like the
+ * reference simulator's oracle pair (src/robot_simulator.cpp:897-1058):
+ * queries never wait for updates and see the newest applied prefix of them.
+ * Up to n_slots query batches are in flight at once on their own streams,
+ * and requests that arrive meanwhile join the next batch as soon as a slot
+ * frees.  No counterpart in the reference's C++ API. */
+enum {
+    HTG_CONSISTENCY_STRICT = 0,  /* a query sees exactly the updates submitted before it */
+    HTG_CONSISTENCY_SNAPSHOT = 1 /* double-buffered: a query sees the newest applied prefix of updates */
+};
 typedef struct htg_server_config {
     uint64_t hbm_budget;          /* admission limit for all sessions' maps; 0 = no limit */
     int32_t n_slots;              /* query batches in flight (1..64) */
     int32_t max_batch_queries;    /* queries coalesced per batch */
     int32_t max_batch_viewpoints; /* viewpoints per batch (a larger query still runs alone) */
-    int32_t reserved;
+    int32_t consistency;          /* HTG_CONSISTENCY_* */
 } htg_server_config;
 typedef struct htg_server_stats {
     uint64_t query_batches, queries, update_batches, updates, hbm_used;
@@ -182,6 +188,9 @@ int htg_server_submit_query(htg_server* s, int32_t session, const int32_t* rc, i
 int htg_server_submit_update(htg_server* s, int32_t session, const int32_t* rc, int32_t n, uint64_t* ticket);
 /* Blocks until the request completed; latency_s = completion - submission. */
 int htg_server_wait(htg_server* s, uint64_t ticket, double* latency_s);
+/* Same; version = for a query, how many of its session's updates (in
+ * submission order) its result includes; for an update, its index. */
+int htg_server_wait_ex(htg_server* s, uint64_t ticket, double* latency_s, int64_t* version);
 int htg_server_stats_get(htg_server* s, htg_server_stats* out);
 /* The session's map, for reading tables; do not use it while requests run. */
 int htg_server_map(htg_server* s, int32_t session, htg_map** out);
@@ -190,10 +199,11 @@ int htg_server_map(htg_server* s, int32_t session, htg_map** out);
  * completion time (seconds after the call) to t_done.  Request i is a query
  * (type 0, count = k) or an update (type 1, count = cells) on session[i]
  * with its (row, col) pairs at rc + 2*rc_off[i]; a query's matrix goes to
- * out + out_off[i] when out_off[i] >= 0. */
+ * out + out_off[i] when out_off[i] >= 0.  version (nullable) receives each
+ * request's version as in htg_server_wait_ex. */
 int htg_server_replay(htg_server* s, int32_t n, const double* t_arrival, const int32_t* session, const int32_t* type,
                       const int32_t* count, const int64_t* rc_off, const int32_t* rc, const int64_t* out_off,
-                      uint16_t* out, double* t_done);
+                      uint16_t* out, double* t_done, int64_t* version);
 
 /* ---- build tuning (the project's autotuner picks these) --------------- */
 typedef struct htg_tuning {

@@ -26,6 +26,18 @@ def states_after(trace, init_obs):
     return out
 
 
+def states_by_version(trace, init_obs):
+    """Observed grid after the first v update requests, for every v."""
+    obs = init_obs.copy()
+    out = [obs.copy()]
+    for i in range(len(trace.t)):
+        if trace.type[i] == 1:
+            c = trace.cells_of(i)
+            obs[c[:, 0], c[:, 1]] = 1
+            out.append(obs.copy())
+    return out
+
+
 @pytest.mark.parametrize("rate,slots,batch_q", [(5e4, 4, 16), (2e3, 2, 4), (1e9, 3, 64)])
 def test_mixed_stream_alone_equals_in_the_mix(gpu_ctx, rate, slots, batch_q):
     print(f"server stream rate={rate} slots={slots} batch={batch_q}", flush=True)
@@ -52,6 +64,41 @@ def test_mixed_stream_alone_equals_in_the_mix(gpu_ctx, rate, slots, batch_q):
             alone.close()
     s = srv.stats()
     assert s["queries"] == int((tr.type == 0).sum()) and s["updates"] == int((tr.type == 1).sum())
+    srv.close()
+
+
+@pytest.mark.parametrize("rate", [3e4, 1e9])
+def test_snapshot_stream_each_result_matches_its_snapshot(gpu_ctx, rate):
+    """Snapshot (double-buffered) mode: queries never wait for updates; each
+    result equals BFS on the state with exactly the update prefix it reports,
+    which equals running it alone on that state."""
+    print(f"snapshot stream rate={rate}", flush=True)
+    truth = grids.street_grid(192, 75)
+    rv = serving.RobotReveal(truth, 4, 20, 76, limit=4000)
+    init = rv.observed.copy()
+    srv = serving.Server(n_slots=3, max_batch_queries=16, max_batch_viewpoints=2048,
+                         consistency=serving.SNAPSHOT)
+    sid = srv.add_map(init, 16)
+    tr = serving.make_trace(sid, rv, 200, rate, 77, update_frac=0.3, k_lo=2, k_hi=48, upd_lo=4, upd_hi=80)
+    srv.replay(tr)
+    sv = states_by_version(tr, init)
+    n_upd = int((tr.type == 1).sum())
+    seen = []
+    prev_max = 0
+    for i in range(len(tr.t)):
+        if tr.type[i] != 0:
+            continue
+        v = int(tr.version[i])
+        assert 0 <= v <= n_upd
+        np.testing.assert_array_equal(tr.query_result(i), grid_bfs(sv[v], tr.cells_of(i)), err_msg=f"req {i} v {v}")
+        seen.append(v)
+    assert max(seen) > 0  # updates landed while queries ran
+    # after draining, a fresh query sees every update
+    vps = grids.sample_free(sv[-1], 30, 5)
+    t, out = srv.submit_query(sid, vps)
+    _, v = srv.wait(t, with_version=True)
+    assert v == n_upd
+    np.testing.assert_array_equal(out, grid_bfs(sv[-1], vps))
     srv.close()
 
 
