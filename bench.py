@@ -177,7 +177,8 @@ def run_served(args):
         f"({time.time() - t0:.1f}s)")
     mode = serving.SNAPSHOT if args.consistency == "snapshot" else serving.STRICT
     srv = serving.Server(n_slots=args.slots, max_batch_queries=args.batch_queries,
-                         max_batch_viewpoints=args.batch_viewpoints, consistency=mode)
+                         max_batch_viewpoints=args.batch_viewpoints, consistency=mode,
+                         update_cu_percent=args.update_cu)
     t0 = time.time()
     sid = srv.add_map(init, B, G)
     log(f"session admitted and built in {1e3 * (time.time() - t0):.1f} ms")
@@ -190,9 +191,10 @@ def run_served(args):
     log(f"plain per-call k=64: {prate:.0f} matrices/s ({1e3 * pdt / args.plain_steps:.3f} ms/query)")
 
     # warm-up and capacity estimate: a burst (all arrivals within ~1 ms)
-    warm = serving.make_trace(sid, rv, 200, 1e6, 11)
+    uf = args.update_frac
+    warm = serving.make_trace(sid, rv, 200, 1e6, 11, update_frac=uf)
     srv.replay(warm)
-    burst = serving.make_trace(sid, rv, args.window * 8, 1e7, 12)
+    burst = serving.make_trace(sid, rv, args.window * 8, 1e7, 12, update_frac=uf)
     tb = srv.replay(burst)
     n_upd_before = int((warm.type == 1).sum() + (burst.type == 1).sum())
     cap = serving.latency_summary(burst, tb)["matrices_per_s"] / 0.9
@@ -205,7 +207,7 @@ def run_served(args):
         rate = cap * frac
         first = not trials
         state0 = rv.observed.copy() if first and not args.no_cpu else None
-        tr = serving.make_trace(sid, rv, args.window * args.steps, rate, 100 + len(trials))
+        tr = serving.make_trace(sid, rv, args.window * args.steps, rate, 100 + len(trials), update_frac=uf)
         smap.profile(True)
         td = srv.replay(tr)
         s = serving.latency_summary(tr, td)
@@ -237,7 +239,8 @@ def run_served(args):
         "config": {"workload": "1024x1024 street-like grid seed 3, b=32, g=2x2, 16 robots; Poisson stream of "
                                "queryAllPairs k~logU[8,256] + 10% update batches of 16..512 cells",
                    "mode": "served", "consistency": args.consistency, "window": args.window, "slots": args.slots,
-                   "max_batch_queries": args.batch_queries},
+                   "max_batch_queries": args.batch_queries, "update_cu_percent": args.update_cu,
+                   "update_frac": uf},
         "p50_ms": s["p50_ms"], "p99_ms": s["p99_ms"], "max_ms": s["max_ms"],
         "update_p50_ms": s["update_p50_ms"], "update_p99_ms": s["update_p99_ms"],
         "arrival_rate_per_s": s["rate"],
@@ -295,7 +298,9 @@ def main():
     ap.add_argument("--mode", default="served", choices=["plain", "served", "update"])
     ap.add_argument("--k", type=int, default=64)
     ap.add_argument("--window", type=int, default=64, help="requests per served step")
-    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--slots", type=int, default=3, help="query streams (the update stream makes one more)")
+    ap.add_argument("--update-cu", type=int, default=0, help="percent of CUs for update kernels (0 = all)")
+    ap.add_argument("--update-frac", type=float, default=0.1, help="share of update requests in the stream")
     ap.add_argument("--batch-queries", type=int, default=64)
     ap.add_argument("--batch-viewpoints", type=int, default=8192)
     ap.add_argument("--plain-steps", type=int, default=200)

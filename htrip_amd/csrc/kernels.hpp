@@ -42,18 +42,34 @@ void launch_child_relax(int lvl, int g, int pc, int nV, int ld, int nt, const ui
 
 // Flags -> pivot list per node (ascending V_U order); counts[i] = |K_i|;
 // flags are cleared.
+// Update path, over each listed node's live octets (max_live8 = the largest
+// live octet count in the list): child relax for the children whose flag in
+// the level below is set (child_slot[i*g^2 + q] = the child's index there,
+// -1 when not listed), pivot compaction, and S_U refresh of the nodes with
+// pivots (vu_ext[v] = exterior index of V_U port v or -1).
+void launch_child_relax_live(int g, int pc, int nV, int ld, int nt, const uint16_t* child_sum, size_t child_stride,
+                             const int* node_list, const int* child_slot, const int* child_changed, int n_list,
+                             const int* live8_cnt, const int* live8_list, int ng8, int max_live8, uint16_t* DU,
+                             size_t node_stride, uint8_t* flags, hipStream_t st);
+void launch_compact_flags_live(int nV, const int* node_list, int n_list, const int* live8_cnt, const int* live8_list,
+                               int ng8, uint8_t* flags, int* pivots, int piv_stride, int* counts, hipStream_t st);
+void launch_project_su_live(int nV, int ld, int ne, const int* vu_ext, const int* node_list, const int* counts,
+                            int n_list, const int* live8_cnt, const int* live8_list, int ng8, int max_live8,
+                            const uint16_t* DU, size_t node_stride, uint16_t* SU, int* changed, hipStream_t st);
 void launch_compact_flags(int nV, const int* node_list, int n_list, uint8_t* flags, int* pivots, int piv_stride,
                           int* counts, hipStream_t st);
 
 // One pivot block of the closure: for every listed node with more than t*P
 // pivots, R := the node's block pivot rows after Floyd-Warshall over the
-// block's pivots (P x nV, into Rbuf slot i).
+// block's pivots (P x nV, into Rbuf slot i), on the node's live octets
+// (live8_list, live8_cnt; max_live8 = the largest count in the list).
 void launch_closure_pivots(int P, int nV, int ld, const int* node_list, int n_list, const int* pivots, int piv_stride,
-                           const int* counts, int t, const uint16_t* DU, size_t node_stride, uint16_t* Rbuf,
-                           hipStream_t st);
-// D_U := min(D_U, R^T (x) R) over the block (symmetric rank-P min-plus update).
+                           const int* counts, int t, const uint16_t* DU, size_t node_stride, const int* live8_cnt,
+                           const int* live8_list, int ng8, int max_live8, uint16_t* Rbuf, hipStream_t st);
+// Phase 2 of the same block, D := min(D, R^T (x) R), over live octets.
 void launch_closure_update(int P, int nV, int ld, const int* node_list, int n_list, const int* counts, int t,
-                           uint16_t* DU, size_t node_stride, const uint16_t* Rbuf, hipStream_t st);
+                           uint16_t* DU, size_t node_stride, const int* live8_cnt, const int* live8_list, int ng8,
+                           int max_live8, const uint16_t* Rbuf, hipStream_t st);
 
 // S_U := D_U[ext][ext]; changed[i] = 1 if listed node i's S_U decreased.
 void launch_project_su(int nV, int ld, int ne, const int* ext_idx, const int* node_list, int n_list,
@@ -63,30 +79,39 @@ void launch_project_su(int nV, int ld, int ne, const int* ext_idx, const int* no
 void launch_set_pass(uint8_t* pass, int S, const int2* cells, int n, hipStream_t st);
 
 // ---- query -----------------------------------------------------------------
-// out matrices: INF, 0 on the diagonal of valid viewpoints, leaf D for
-// same-leaf pairs.  vp_* arrays are per viewpoint (all queries packed).
-void launch_q_init(int nq, const int* q_k, const long long* q_out, const int* q_vp, const int2* vp_rc,
-                   const uint8_t* vp_valid, int b, int nl, const uint16_t* leafD, uint16_t* out, int kmax,
-                   hipStream_t st);
-// X0[slot][p] = leaf D row of the slot's cell at port p.
-void launch_q_leaf(int n, const int2* slot_rc, int b, int nl, const uint16_t* leafD, uint16_t* X0, hipStream_t st);
+// One launch: out matrices (INF, 0 on the diagonal of valid viewpoints, leaf
+// D for same-leaf pairs; vp_* arrays are per viewpoint, all queries packed)
+// and X0[slot][p] = leaf D row of the slot's cell at leaf port p; clears
+// wstat (nullable).
+// INF fills: n_fill ranges (start lo, start hi, n8, first block) of n8 units
+// of 8 u16 from fill_base, fill_blocks blocks in all.
+void launch_q_front(int nq, const int* q_k, const long long* q_out, const int* q_vp, const int2* vp_rc,
+                    const uint8_t* vp_valid, int kmax, int n_slots, const int2* slot_rc, int b, int nl,
+                    const uint16_t* leafD, uint16_t* out, uint16_t* X0, int n_fill, const int4* fills,
+                    int fill_blocks, uint16_t* fill_base, unsigned long long* wstat, hipStream_t st);
 // Projection partials: Zp[chunk][slot][col] = min over the chunk's seam rows a
-// of X[slot][a] (+) D_W[q*pc + a][cols(col)] for work items (W, q, first
-// slot, count); ksplit row chunks per item.
+// of X[slot][a] (+) D_W[q*pc + a][cols(col)] for work items (W, q | first-of-
+// run flag << 16, first slot, count); ksplit row chunks per item; only the
+// live column groups of W (live_list, live_cnt), col_tiles = max over the
+// items of ceil(live_cnt / 64).  With ksplit == 1 the results go straight to
+// Z (position colpos[col]) and the lifted Xn (exterior port colext[col]).
 void launch_q_z(int M, int ld, int pc, const uint16_t* DU, size_t node_stride, const int* seam_list,
-                const int* seam_off, const int* cols, int ncols, int ldp, const int4* items, int n_items, int ksplit,
-                const uint16_t* X, uint16_t* Zp, size_t part_stride, hipStream_t st);
-// Z[slot][pos] = min_c Zp[c][slot][src[pos]] (INF where src[pos] < 0).
-void launch_q_zred(int n, int nz, int ldz, const int* src, int ldp, int ksplit, const uint16_t* Zp,
-                   size_t part_stride, uint16_t* Z, hipStream_t st);
-// Meet tiles (8 ints each: group, q2, i0, j0, j_end): out[i][j] = min(out,
-// min over the q2 seam block of Z[i] (+) Z[j]).
-void launch_q_meet(const int* tiles, int n_tiles, const int4* groups, const int* glist, const int* zoff,
-                   const int* zlen, int ldz, const int* slot_i, const int* q_k, const long long* q_out,
-                   const uint16_t* Z, uint16_t* out, hipStream_t st);
-// Lift: X_l[slot][e] = min(Z[slot][ext_zpos[e]], X_{l-1}[slot][own port of e]).
-void launch_q_lift(int n, int ne, const int* ext_zpos, const int* ext_child, const int* ext_cport, int pc, int cs,
-                   int g, int ldz, const int2* slot_rc, const uint16_t* Xprev, const uint16_t* Z, uint16_t* Xnext,
+                const int* seam_off, const int* cols, int ngrp, const int* live_cnt, const int* live_list,
+                int col_tiles, int ldp, const int4* items, int n_items, int ksplit, const uint16_t* X, uint16_t* Zp,
+                size_t part_stride, const int* colpos, const int* colext, const int* ext_child,
+                const int* ext_cport, uint16_t* Z, int ldz, uint16_t* Xn, int ne, unsigned long long* wstat,
+                hipStream_t st);
+// Fold: Z[slot][pos] = min_c Zp[c][slot][src[pos]] (INF where src[pos] < 0 or
+// the column's group is not live in the slot's node) and, for ne > 0, the
+// lift Xnext[slot][e] = min(folded column ext_col[e], Xprev[slot][own port of e]).
+void launch_q_fold(int n, int nz, int ldz, const int* src, int ldp, int ksplit, const uint16_t* Zp,
+                   size_t part_stride, const int2* slot_rc, int side, int nt, int ngrp, const uint8_t* live_map,
+                   uint16_t* Z, int ne, const int* ext_col, const int* ext_child, const int* ext_cport, int pc, int cs,
+                   int g, const uint16_t* Xprev, uint16_t* Xnext, hipStream_t st);
+// Meet tiles of every level (8 ints each: group, q2, i0, j0, j_end, level):
+// out[i][j] = min(out, min over the q2 seam block of Z_level[i] (+) Z_level[j]).
+void launch_q_meet(const int* tiles, int n_tiles, const int4* groups, const int* glist, const int* zgeo, int g2,
+                   const int* slot_i, const int* q_k, const long long* q_out, const uint16_t* Z, uint16_t* out,
                    hipStream_t st);
 
 }  // namespace htg

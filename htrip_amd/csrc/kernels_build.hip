@@ -265,6 +265,107 @@ void launch_child_relax(int lvl, int g, int pc, int nV, int ld, int nt, const ui
                        node_list, child_mask, DU, node_stride, flags);
 }
 
+// Child relax of an update over live octets: for V_U ports v1, v2 of the
+// same child q whose summary changed (its flag in the level below is set),
+// D_U[v1][v2] = min(D_U[v1][v2], S_q[v1 - q*pc][v2 - q*pc]); rows that
+// decreased are flagged as pivots (src/internal_node.cpp:343-370).  Dead
+// ports have INF rows and columns in S_q and D_U alike, so only live octet
+// pairs are visited: thread = (row octet, column octet, row in the octet).
+__global__ __launch_bounds__(256) void k_child_relax_live(int g, int pc, int nV, int ld, int nt, int child_nt,
+                                                          const uint16_t* __restrict__ child_sum,
+                                                          size_t child_stride, const int* __restrict__ node_list,
+                                                          const int* __restrict__ child_slot,
+                                                          const int* __restrict__ child_changed,
+                                                          const int* __restrict__ live8_cnt,
+                                                          const int* __restrict__ live8_list, int ng8,
+                                                          uint16_t* __restrict__ DU, size_t node_stride,
+                                                          uint8_t* __restrict__ flags) {
+    const int i = blockIdx.y;
+    const int node = node_list[i];
+    const int n8 = live8_cnt[node];
+    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (long long)n8 * n8 * 8) return;
+    const int* l8 = live8_list + size_t(node) * ng8;
+    const long long pr = t >> 3;
+    const int v1 = 8 * l8[pr / n8] + int(t & 7);
+    if (v1 >= nV) return;
+    const int q = v1 / pc;
+    const int cs = child_slot[i * g * g + q];
+    if (cs < 0 || !child_changed[cs]) return;
+    const int c0 = 8 * l8[pr % n8];
+    const int lo = q * pc, hi = lo + pc;
+    if (c0 + 8 <= lo || c0 >= hi) return;  // column octet outside child q
+    const int tx = node % nt, ty = node / nt;
+    const int cidx = (ty * g + q / g) * child_nt + (tx * g + q % g);
+    const uint16_t* Srow = child_sum + size_t(cidx) * child_stride + size_t(v1 - lo) * pc;
+    uint16_t* Drow = DU + size_t(node) * node_stride + size_t(v1) * ld;
+    bool dec = false;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int v2 = c0 + k;
+        if (v2 < lo || v2 >= hi) continue;
+        const uint16_t w = Srow[v2 - lo];
+        if (w < Drow[v2]) {
+            Drow[v2] = w;
+            dec = true;
+        }
+    }
+    if (dec) flags[size_t(i) * nV + v1] = 1;
+}
+
+void launch_child_relax_live(int g, int pc, int nV, int ld, int nt, const uint16_t* child_sum, size_t child_stride,
+                             const int* node_list, const int* child_slot, const int* child_changed, int n_list,
+                             const int* live8_cnt, const int* live8_list, int ng8, int max_live8, uint16_t* DU,
+                             size_t node_stride, uint8_t* flags, hipStream_t st) {
+    if (n_list == 0 || max_live8 == 0) return;
+    const long long total = (long long)max_live8 * max_live8 * 8;
+    const dim3 grid{unsigned((total + 255) / 256), unsigned(n_list), 1u};
+    hipLaunchKernelGGL(k_child_relax_live, grid, dim3(256), 0, st, g, pc, nV, ld, nt, nt * g, child_sum, child_stride,
+                       node_list, child_slot, child_changed, live8_cnt, live8_list, ng8, DU, node_stride, flags);
+}
+
+// Row flags -> ascending pivot list over the live octets (flags are only
+// ever set on live rows); clears the flags it reads.  One workgroup per node.
+__global__ __launch_bounds__(256) void k_compact_flags_live(int nV, const int* __restrict__ node_list,
+                                                            const int* __restrict__ live8_cnt,
+                                                            const int* __restrict__ live8_list, int ng8,
+                                                            uint8_t* __restrict__ flags, int* __restrict__ pivots,
+                                                            int piv_stride, int* __restrict__ counts) {
+    __shared__ int wsum[4];
+    const int node = node_list[blockIdx.x];
+    const int n = 8 * live8_cnt[node];
+    const int* l8 = live8_list + size_t(node) * ng8;
+    uint8_t* f = flags + size_t(blockIdx.x) * nV;
+    int* out = pivots + size_t(blockIdx.x) * piv_stride;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int base = 0;
+    for (int c0 = 0; c0 < n; c0 += 256) {
+        const int e = c0 + tid;
+        const int v = e < n ? 8 * l8[e >> 3] + (e & 7) : nV;
+        const bool on = v < nV && f[v] != 0;
+        const unsigned long long m = __ballot(on);
+        const int before = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) wsum[wave] = __popcll(m);
+        __syncthreads();
+        int off = base;
+        for (int w = 0; w < wave; ++w) off += wsum[w];
+        if (on) {
+            out[off + before] = v;
+            f[v] = 0;
+        }
+        base += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        __syncthreads();
+    }
+    if (tid == 0) counts[blockIdx.x] = base;
+}
+
+void launch_compact_flags_live(int nV, const int* node_list, int n_list, const int* live8_cnt, const int* live8_list,
+                               int ng8, uint8_t* flags, int* pivots, int piv_stride, int* counts, hipStream_t st) {
+    if (n_list == 0) return;
+    hipLaunchKernelGGL(k_compact_flags_live, dim3(unsigned(n_list)), dim3(256), 0, st, nV, node_list, live8_cnt,
+                       live8_list, ng8, flags, pivots, piv_stride, counts);
+}
+
 // ---------------------------------------------------------------------------
 // Row flags -> ascending pivot list (one workgroup per node).
 __global__ __launch_bounds__(256) void k_compact_flags(int nV, uint8_t* __restrict__ flags, int* __restrict__ pivots,
@@ -312,21 +413,39 @@ __global__ __launch_bounds__(256) void k_closure_pivots(int nV, int ld, const in
                                                         const int* __restrict__ pivots, int piv_stride,
                                                         const int* __restrict__ counts, int t,
                                                         const uint16_t* __restrict__ DU, size_t node_stride,
+                                                        const int* __restrict__ live8_cnt,
+                                                        const int* __restrict__ live8_list, int ng8,
                                                         uint16_t* __restrict__ Rbuf) {
-    // Workgroup = 128 columns of one node; wave w computes output pivot rows
-    // [w*P/4, (w+1)*P/4) for the lane's 2 columns.
+    // Workgroup = 16 live octets (128 columns) of one node; wave w computes
+    // output pivot rows [w*P/4, (w+1)*P/4) for the lane's 2 columns.  Columns
+    // of dead octets (no passable port) stay INF in every pivot row and are
+    // skipped.
     constexpr int KW = P / 4;
     const int slot = blockIdx.y;
     const int p = min(P, counts[slot] - t * P);
     if (p <= 0) return;
+    const int node = node_list[slot];
+    const int nl8 = live8_cnt[node];
+    if (int(blockIdx.x) * 16 >= nl8) return;
     __shared__ int kr[P];
     __shared__ __align__(16) uint16_t C[P][P];
     __shared__ __align__(16) uint32_t CdT[P][P];  // CdT[k2][k] = dup(C*[k][k2])
     const int tid = threadIdx.x;
-    const uint16_t* D = DU + size_t(node_list[slot]) * node_stride;
+    const uint16_t* D = DU + size_t(node) * node_stride;
     const int* K = pivots + size_t(slot) * piv_stride + t * P;
+    __shared__ uint32_t Dt[P][64];  // pivot rows at this block's 128 columns (2 per lane)
     if (tid < P) kr[tid] = tid < p ? K[tid] : 0;
     __syncthreads();
+    const int wave = tid >> 6, lane = tid & 63;
+    const int oct = blockIdx.x * 16 + (lane >> 2);
+    const bool col_ok = oct < nl8;
+    const int j0 = col_ok ? 8 * live8_list[size_t(node) * ng8 + oct] + (lane & 3) * 2 : 0;
+    // Issue the pivot-row loads first (wave w takes rows w, w + 4, ...); they
+    // land while the pivot block is closed below.
+    constexpr int RPW = P / 4;
+    uint32_t pre[RPW];
+#pragma unroll
+    for (int i = 0; i < RPW; ++i) pre[i] = *reinterpret_cast<const uint32_t*>(D + size_t(kr[wave + 4 * i]) * ld + j0);
     for (int i = tid; i < P * P; i += 256) {
         const int a = i / P, c = i % P;
         C[a][c] = (a < p && c < p) ? D[size_t(kr[a]) * ld + kr[c]] : (a == c ? 0 : kInf);
@@ -360,32 +479,25 @@ __global__ __launch_bounds__(256) void k_closure_pivots(int nV, int ld, const in
         const int k2 = i / P, k = i % P;
         CdT[k2][k] = dup16(C[k][k2]);
     }
+#pragma unroll
+    for (int i = 0; i < RPW; ++i) Dt[wave + 4 * i][lane] = pre[i];
     __syncthreads();
-    const int wave = tid >> 6, lane = tid & 63;
-    const int j0 = blockIdx.x * 128 + lane * 2;
-    if (j0 >= nV) return;
+    if (!col_ok) return;
     const int kbase = wave * KW;
     uint32_t acc[KW];
 #pragma unroll
     for (int k = 0; k < KW; ++k) acc[k] = kInf2;
-    constexpr int PF = 8;  // pivot-row loads in flight
-    for (int k0 = 0; k0 < p; k0 += PF) {
-        uint32_t dv[PF];
+    for (int k2 = 0; k2 < p; ++k2) {
+        const uint32_t dv = Dt[k2][lane];
+        // CdT rows k2 >= p hold INF off the diagonal; acc[k >= p] is never stored
+        const uint4* c4 = reinterpret_cast<const uint4*>(&CdT[k2][kbase]);
 #pragma unroll
-        for (int u = 0; u < PF; ++u)
-            dv[u] = *reinterpret_cast<const uint32_t*>(D + size_t(kr[k0 + u]) * ld + j0);  // kr[k >= p] = row 0
-#pragma unroll
-        for (int u = 0; u < PF; ++u) {
-            // CdT rows k2 >= p hold INF off the diagonal; acc[k >= p] is never stored
-            const uint4* c4 = reinterpret_cast<const uint4*>(&CdT[k0 + u][kbase]);
-#pragma unroll
-            for (int k = 0; k < KW; k += 4) {
-                const uint4 cv = c4[k / 4];
-                acc[k + 0] = relax2(acc[k + 0], cv.x, dv[u]);
-                acc[k + 1] = relax2(acc[k + 1], cv.y, dv[u]);
-                acc[k + 2] = relax2(acc[k + 2], cv.z, dv[u]);
-                acc[k + 3] = relax2(acc[k + 3], cv.w, dv[u]);
-            }
+        for (int k = 0; k < KW; k += 4) {
+            const uint4 cv = c4[k / 4];
+            acc[k + 0] = relax2(acc[k + 0], cv.x, dv);
+            acc[k + 1] = relax2(acc[k + 1], cv.y, dv);
+            acc[k + 2] = relax2(acc[k + 2], cv.z, dv);
+            acc[k + 3] = relax2(acc[k + 3], cv.w, dv);
         }
     }
     uint16_t* R = Rbuf + size_t(slot) * P * ld;
@@ -394,41 +506,54 @@ __global__ __launch_bounds__(256) void k_closure_pivots(int nV, int ld, const in
         if (kbase + k < p) *reinterpret_cast<uint32_t*>(R + size_t(kbase + k) * ld + j0) = acc[k];
 }
 
-// Phase 2: D := min(D, R^T (x) R).  A 256-thread workgroup owns a 128 x 128
-// tile, 8 x 8 elements per thread held as 32 packed registers; R's row slices
-// for the tile sit in LDS (rows pre-duplicated into both u16 halves).
+// Phase 2: D := min(D, R^T (x) R) over the live octets only (rows and
+// columns of dead ports are INF off the diagonal and no pivot changes them).
+// A 256-thread workgroup owns a tile of 16 x 16 live octets (128 x 128
+// entries), 8 x 8 elements per thread held as 32 packed registers; R's row
+// slices for the tile sit in LDS (rows pre-duplicated into both u16 halves).
 template <int P>
 __global__ __launch_bounds__(256) void k_closure_update(int nV, int ld, const int* __restrict__ node_list,
                                                         const int* __restrict__ counts, int t, int tiles,
                                                         uint16_t* __restrict__ DU, size_t node_stride,
+                                                        const int* __restrict__ live8_cnt,
+                                                        const int* __restrict__ live8_list, int ng8,
                                                         const uint16_t* __restrict__ Rbuf) {
     const int slot = blockIdx.y;
     const int p = min(P, counts[slot] - t * P);
     if (p <= 0) return;
-    __shared__ uint4 Rr4[P][32];  // dup'd R[k][i0 .. i0+127]
-    __shared__ uint4 Rc4[P][16];  // R[k][j0 .. j0+127]
+    const int node = node_list[slot];
+    const int nl8 = live8_cnt[node];
     const int I = blockIdx.x / tiles, J = blockIdx.x % tiles;
-    const int i0 = I * 128, j0 = J * 128;
+    if (I * 16 >= nl8 || J * 16 >= nl8) return;  // tile past this node's live octets
+    __shared__ uint4 Rr4[P][32];  // dup'd R[k][rows of the tile's 16 row octets]
+    __shared__ uint4 Rc4[P][16];  // R[k][columns of the tile's 16 column octets]
+    __shared__ int ro[16], co[16];  // first port of each row / column octet, -1 past the live ones
     const int tid = threadIdx.x;
+    const int* l8 = live8_list + size_t(node) * ng8;
+    if (tid < 16) ro[tid] = I * 16 + tid < nl8 ? 8 * l8[I * 16 + tid] : -1;
+    else if (tid < 32) co[tid - 16] = J * 16 + tid - 16 < nl8 ? 8 * l8[J * 16 + tid - 16] : -1;
+    __syncthreads();
     const uint16_t* R = Rbuf + size_t(slot) * P * ld;
     uint32_t* Rr = reinterpret_cast<uint32_t*>(&Rr4[0][0]);
     uint16_t* Rc = reinterpret_cast<uint16_t*>(&Rc4[0][0]);
     for (int i = tid; i < P * 128; i += 256) {
         const int k = i / 128, x = i % 128;
         const bool kin = k < p;
-        Rr[i] = (kin && i0 + x < nV) ? dup16(R[size_t(k) * ld + i0 + x]) : kInf2;
-        Rc[i] = (kin && j0 + x < nV) ? R[size_t(k) * ld + j0 + x] : kInf;
+        const int ri = ro[x >> 3], ci = co[x >> 3];
+        Rr[i] = (kin && ri >= 0 && ri + (x & 7) < nV) ? dup16(R[size_t(k) * ld + ri + (x & 7)]) : kInf2;
+        Rc[i] = (kin && ci >= 0 && ci + (x & 7) < nV) ? R[size_t(k) * ld + ci + (x & 7)] : kInf;
     }
     __syncthreads();
     const int tx = tid & 15, ty = tid >> 4;
-    const int col = j0 + tx * 8;
-    uint16_t* D = DU + size_t(node_list[slot]) * node_stride;
+    const int col = co[tx], row0 = ro[ty];
+    const bool ok = col >= 0 && row0 >= 0;
+    uint16_t* D = DU + size_t(node) * node_stride;
     uint32_t acc[8][4];
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
-        const int i = i0 + ty * 8 + r;
+        const int i = row0 + r;
         uint4 v = make_uint4(kInf2, kInf2, kInf2, kInf2);
-        if (i < nV && col < ld) v = *reinterpret_cast<const uint4*>(D + size_t(i) * ld + col);
+        if (ok && i < nV) v = *reinterpret_cast<const uint4*>(D + size_t(i) * ld + col);
         acc[r][0] = v.x;
         acc[r][1] = v.y;
         acc[r][2] = v.z;
@@ -449,42 +574,39 @@ __global__ __launch_bounds__(256) void k_closure_update(int nV, int ld, const in
     }
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
-        const int i = i0 + ty * 8 + r;
-        if (i < nV && col < ld)
+        const int i = row0 + r;
+        if (ok && i < nV)
             *reinterpret_cast<uint4*>(D + size_t(i) * ld + col) = make_uint4(acc[r][0], acc[r][1], acc[r][2], acc[r][3]);
     }
 }
 
 void launch_closure_pivots(int P, int nV, int ld, const int* node_list, int n_list, const int* pivots, int piv_stride,
-                           const int* counts, int t, const uint16_t* DU, size_t node_stride, uint16_t* Rbuf,
-                           hipStream_t st) {
-    if (n_list == 0) return;
-    dim3 grid(unsigned((nV + 127) / 128), unsigned(n_list));
-    if (P == 16)
-        hipLaunchKernelGGL(k_closure_pivots<16>, grid, dim3(256), 0, st, nV, ld, node_list, pivots, piv_stride, counts,
-                           t, DU, node_stride, Rbuf);
-    else if (P == 32)
-        hipLaunchKernelGGL(k_closure_pivots<32>, grid, dim3(256), 0, st, nV, ld, node_list, pivots, piv_stride, counts,
-                           t, DU, node_stride, Rbuf);
-    else
-        hipLaunchKernelGGL(k_closure_pivots<64>, grid, dim3(256), 0, st, nV, ld, node_list, pivots, piv_stride, counts,
-                           t, DU, node_stride, Rbuf);
+                           const int* counts, int t, const uint16_t* DU, size_t node_stride, const int* live8_cnt,
+                           const int* live8_list, int ng8, int max_live8, uint16_t* Rbuf, hipStream_t st) {
+    if (n_list == 0 || max_live8 == 0) return;
+    const dim3 grid{unsigned((max_live8 + 15) / 16), unsigned(n_list), 1u};
+#define HTG_CP(PP)                                                                                                 \
+    hipLaunchKernelGGL(k_closure_pivots<PP>, grid, dim3(256), 0, st, nV, ld, node_list, pivots, piv_stride, counts, \
+                       t, DU, node_stride, live8_cnt, live8_list, ng8, Rbuf)
+    if (P == 16) HTG_CP(16);
+    else if (P == 32) HTG_CP(32);
+    else HTG_CP(64);
+#undef HTG_CP
 }
 
 void launch_closure_update(int P, int nV, int ld, const int* node_list, int n_list, const int* counts, int t,
-                           uint16_t* DU, size_t node_stride, const uint16_t* Rbuf, hipStream_t st) {
-    if (n_list == 0) return;
-    const int tiles = (nV + 127) / 128;
-    dim3 grid(unsigned(tiles * tiles), unsigned(n_list));
-    if (P == 16)
-        hipLaunchKernelGGL(k_closure_update<16>, grid, dim3(256), 0, st, nV, ld, node_list, counts, t, tiles, DU,
-                           node_stride, Rbuf);
-    else if (P == 32)
-        hipLaunchKernelGGL(k_closure_update<32>, grid, dim3(256), 0, st, nV, ld, node_list, counts, t, tiles, DU,
-                           node_stride, Rbuf);
-    else
-        hipLaunchKernelGGL(k_closure_update<64>, grid, dim3(256), 0, st, nV, ld, node_list, counts, t, tiles, DU,
-                           node_stride, Rbuf);
+                           uint16_t* DU, size_t node_stride, const int* live8_cnt, const int* live8_list, int ng8,
+                           int max_live8, const uint16_t* Rbuf, hipStream_t st) {
+    if (n_list == 0 || max_live8 == 0) return;
+    const int tiles = (max_live8 + 15) / 16;
+    const dim3 grid{unsigned(tiles * tiles), unsigned(n_list), 1u};
+#define HTG_CU(PP)                                                                                             \
+    hipLaunchKernelGGL(k_closure_update<PP>, grid, dim3(256), 0, st, nV, ld, node_list, counts, t, tiles, DU, \
+                       node_stride, live8_cnt, live8_list, ng8, Rbuf)
+    if (P == 16) HTG_CU(16);
+    else if (P == 32) HTG_CU(32);
+    else HTG_CU(64);
+#undef HTG_CU
 }
 
 // ---------------------------------------------------------------------------
@@ -511,6 +633,56 @@ void launch_project_su(int nV, int ld, int ne, const int* ext_idx, const int* no
     dim3 grid(unsigned((total + 255) / 256), unsigned(n_list));
     hipLaunchKernelGGL(k_project_su, grid, dim3(256), 0, st, ld, ne, ext_idx, node_list, DU, node_stride, SU,
                        changed);
+}
+
+// S_U refresh of an update over live octet pairs: S_U[e1][e2] = D_U[v1][v2]
+// for exterior ports e = vu_ext[v] of live octets (dead exterior entries
+// never change); skips nodes without pivots (their D_U did not change);
+// changed[i] = 1 when an entry decreased.
+__global__ __launch_bounds__(256) void k_project_su_live(int nV, int ld, int ne, const int* __restrict__ vu_ext,
+                                                         const int* __restrict__ node_list,
+                                                         const int* __restrict__ counts,
+                                                         const int* __restrict__ live8_cnt,
+                                                         const int* __restrict__ live8_list, int ng8,
+                                                         const uint16_t* __restrict__ DU, size_t node_stride,
+                                                         uint16_t* __restrict__ SU, int* __restrict__ changed) {
+    const int i = blockIdx.y;
+    if (counts[i] == 0) return;
+    const int node = node_list[i];
+    const int n8 = live8_cnt[node];
+    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (long long)n8 * n8 * 8) return;
+    const int* l8 = live8_list + size_t(node) * ng8;
+    const long long pr = t >> 3;
+    const int v1 = 8 * l8[pr / n8] + int(t & 7);
+    if (v1 >= nV) return;
+    const int e1 = vu_ext[v1];
+    if (e1 < 0) return;
+    const int c0 = 8 * l8[pr % n8];
+    const uint16_t* Drow = DU + size_t(node) * node_stride + size_t(v1) * ld;
+    uint16_t* Srow = SU + size_t(node) * ne * ne + size_t(e1) * ne;
+    bool dec = false;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int v2 = c0 + k;
+        if (v2 >= nV) break;
+        const int e2 = vu_ext[v2];
+        if (e2 < 0) continue;
+        const uint16_t v = Drow[v2];
+        dec |= v < Srow[e2];
+        Srow[e2] = v;
+    }
+    if (dec) changed[i] = 1;
+}
+
+void launch_project_su_live(int nV, int ld, int ne, const int* vu_ext, const int* node_list, const int* counts,
+                            int n_list, const int* live8_cnt, const int* live8_list, int ng8, int max_live8,
+                            const uint16_t* DU, size_t node_stride, uint16_t* SU, int* changed, hipStream_t st) {
+    if (n_list == 0 || max_live8 == 0) return;
+    const long long total = (long long)max_live8 * max_live8 * 8;
+    const dim3 grid{unsigned((total + 255) / 256), unsigned(n_list), 1u};
+    hipLaunchKernelGGL(k_project_su_live, grid, dim3(256), 0, st, nV, ld, ne, vu_ext, node_list, counts, live8_cnt,
+                       live8_list, ng8, DU, node_stride, SU, changed);
 }
 
 __global__ void k_set_pass(uint8_t* pass, int S, const int2* cells, int n) {

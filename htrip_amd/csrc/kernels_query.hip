@@ -36,116 +36,177 @@ __device__ __forceinline__ uint32_t qmin2(uint32_t a, uint32_t b) {
                                                                   __builtin_bit_cast(us2q, b)));
 }
 
-// Output init (src/minplus_oracle.cpp:940-965, same-leaf pairs :1027-1048).
-__global__ __launch_bounds__(256) void k_q_init(const int* __restrict__ q_k, const long long* __restrict__ q_out,
-                                                const int* __restrict__ q_vp, const int2* __restrict__ vp_rc,
-                                                const uint8_t* __restrict__ vp_valid, int b, int nl,
-                                                const uint16_t* __restrict__ leafD, uint16_t* __restrict__ out) {
-    const int Q = blockIdx.y;
-    const int k = q_k[Q];
-    const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (idx >= (long long)k * k) return;
-    const int i = int(idx / k), j = int(idx % k);
-    const int vi = q_vp[Q] + i, vj = q_vp[Q] + j;
-    uint16_t val = kInf;
-    if (vp_valid[vi] && vp_valid[vj]) {
-        if (i == j) {
-            val = 0;
-        } else {
-            const int2 a = vp_rc[vi], c = vp_rc[vj];
-            if (a.x / b == c.x / b && a.y / b == c.y / b) {
-                const int n = b * b;
-                const size_t leaf = size_t(a.x / b) * nl + a.y / b;
-                val = leafD[leaf * n * n + size_t((a.x % b) * b + a.y % b) * n + (c.x % b) * b + c.y % b];
+// Front of the query, one 1-D launch with three block ranges:
+//  [0, nq * ib): output init (src/minplus_oracle.cpp:940-965, same-leaf pairs
+//     :1027-1048) of query blk / ib: INF, 0 on the diagonal of valid
+//     viewpoints, leaf D for same-leaf pairs;
+//  next leaf_blocks: level-0 lift, X0[slot][p] = leaf D row of the slot's cell
+//     at leaf port p (src/minplus_oracle.cpp:1000-1019);
+//  then INF fills of the buffers whose live-only writers leave gaps: fill
+//     range r = (start lo, start hi, n8, first block), n8 units of 8 u16.
+// Block 0 also clears the k_q_z work counters.
+__global__ __launch_bounds__(256) void k_q_front(int nq, int ib, int leaf_blocks, const int* __restrict__ q_k,
+                                                 const long long* __restrict__ q_out, const int* __restrict__ q_vp,
+                                                 const int2* __restrict__ vp_rc, const uint8_t* __restrict__ vp_valid,
+                                                 int n_slots, const int2* __restrict__ slot_rc, int b, int nl,
+                                                 const uint16_t* __restrict__ leafD, uint16_t* __restrict__ out,
+                                                 uint16_t* __restrict__ X0, int n_fill, const int4* __restrict__ fills,
+                                                 uint16_t* __restrict__ fill_base,
+                                                 unsigned long long* __restrict__ wstat) {
+    const int n = b * b;
+    const int blk = int(blockIdx.x);
+    if (wstat && blk == 0 && threadIdx.x < 2) wstat[threadIdx.x] = 0;
+    if (blk < nq * ib) {
+        const int Q = blk / ib;
+        const int k = q_k[Q];
+        const long long idx = (long long)(blk % ib) * 256 + threadIdx.x;
+        if (idx >= (long long)k * k) return;
+        const int i = int(idx / k), j = int(idx % k);
+        const int vi = q_vp[Q] + i, vj = q_vp[Q] + j;
+        uint16_t val = kInf;
+        if (vp_valid[vi] && vp_valid[vj]) {
+            if (i == j) {
+                val = 0;
+            } else {
+                const int2 a = vp_rc[vi], c = vp_rc[vj];
+                if (a.x / b == c.x / b && a.y / b == c.y / b) {
+                    const size_t leaf = size_t(a.x / b) * nl + a.y / b;
+                    val = leafD[leaf * n * n + size_t((a.x % b) * b + a.y % b) * n + (c.x % b) * b + c.y % b];
+                }
             }
         }
+        out[q_out[Q] + idx] = val;
+        return;
     }
-    out[q_out[Q] + idx] = val;
+    const int lb = blk - nq * ib;
+    if (lb < leaf_blocks) {
+        const int pl = 4 * b - 4;
+        const long long gid = (long long)lb * 256 + threadIdx.x;
+        if (gid >= (long long)n_slots * pl) return;
+        const int s = int(gid / pl), p = int(gid % pl);
+        const int2 rc = slot_rc[s];
+        int pr, pcc;
+        port_rc(p, b, pr, pcc);
+        const size_t leaf = size_t(rc.x / b) * nl + rc.y / b;
+        X0[gid] = leafD[leaf * n * n + size_t((rc.x % b) * b + rc.y % b) * n + pr * b + pcc];
+        return;
+    }
+    const int fb = lb - leaf_blocks;
+    int r = 0;
+    while (r + 1 < n_fill && fills[r + 1].w <= fb) ++r;
+    const int4 f = fills[r];
+    const long long u = (long long)(fb - f.w) * 256 + threadIdx.x;
+    if (u >= f.z) return;
+    const size_t start = (size_t(unsigned(f.y)) << 32) | unsigned(f.x);
+    reinterpret_cast<uint4*>(fill_base + start)[u] = make_uint4(kInf2, kInf2, kInf2, kInf2);
 }
 
-void launch_q_init(int nq, const int* q_k, const long long* q_out, const int* q_vp, const int2* vp_rc,
-                   const uint8_t* vp_valid, int b, int nl, const uint16_t* leafD, uint16_t* out, int kmax,
-                   hipStream_t st) {
-    if (nq == 0 || kmax == 0) return;
-    const long long total = (long long)kmax * kmax;
-    dim3 grid(unsigned((total + 255) / 256), unsigned(nq));
-    hipLaunchKernelGGL(k_q_init, grid, dim3(256), 0, st, q_k, q_out, q_vp, vp_rc, vp_valid, b, nl, leafD, out);
-}
-
-// Level-0 lift: leaf D row of the viewpoint at the leaf's ports
-// (src/minplus_oracle.cpp:1000-1019).
-__global__ __launch_bounds__(256) void k_q_leaf(int n_slots, const int2* __restrict__ slot_rc, int b, int nl,
-                                                const uint16_t* __restrict__ leafD, uint16_t* __restrict__ X0) {
-    const int pl = 4 * b - 4;
-    const long long gid = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (gid >= (long long)n_slots * pl) return;
-    const int s = int(gid / pl), p = int(gid % pl);
-    const int2 rc = slot_rc[s];
-    const int n = b * b;
-    int pr, pcc;
-    port_rc(p, b, pr, pcc);
-    const size_t leaf = size_t(rc.x / b) * nl + rc.y / b;
-    X0[gid] = leafD[leaf * n * n + size_t((rc.x % b) * b + rc.y % b) * n + pr * b + pcc];
-}
-
-void launch_q_leaf(int n, const int2* slot_rc, int b, int nl, const uint16_t* leafD, uint16_t* X0, hipStream_t st) {
-    if (n == 0) return;
-    const long long total = (long long)n * (4 * b - 4);
-    hipLaunchKernelGGL(k_q_leaf, dim3(unsigned((total + 255) / 256)), dim3(256), 0, st, n, slot_rc, b, nl, leafD, X0);
+void launch_q_front(int nq, const int* q_k, const long long* q_out, const int* q_vp, const int2* vp_rc,
+                    const uint8_t* vp_valid, int kmax, int n_slots, const int2* slot_rc, int b, int nl,
+                    const uint16_t* leafD, uint16_t* out, uint16_t* X0, int n_fill, const int4* fills,
+                    int fill_blocks, uint16_t* fill_base, unsigned long long* wstat, hipStream_t st) {
+    if (nq == 0) return;
+    const int ib = int(((long long)kmax * kmax + 255) / 256);
+    const int leaf_blocks = int(((long long)n_slots * (4 * b - 4) + 255) / 256);
+    const long long blocks = (long long)nq * ib + leaf_blocks + fill_blocks;
+    if (blocks == 0) return;
+    hipLaunchKernelGGL(k_q_front, dim3(unsigned(blocks)), dim3(256), 0, st, nq, ib, leaf_blocks, q_k, q_out, q_vp,
+                       vp_rc, vp_valid, n_slots, slot_rc, b, nl, leafD, out, X0, n_fill, fills, fill_base, wstat);
 }
 
 // Projection Z (see file comment), split over the seam rows: work item
-// (W, q, first slot, count), column tile (256 columns, 4 per lane) and row
-// chunk blockIdx.z.  Each wavefront keeps M x 4 packed accumulators and reads
-// one 8-byte slice of a D_W row per seam row; the M viewpoints' X values for
-// a chunk of rows sit in LDS duplicated into both u16 halves.  Partial minima
-// go to Zp[chunk]; k_q_zred folds the chunks.
+// (W, q, first slot, count), column tile blockIdx.y and row chunk blockIdx.z;
+// four wavefronts share the columns and take interleaved rounds of rows.
+// Only live column groups are computed: live_list[W] holds the groups of 4
+// projection columns with at least one passable port (the others are INF and
+// k_q_zred writes them so), 64 groups = 256 columns per workgroup.  Each
+// wavefront keeps M x 4 packed accumulators and reads one 8-byte slice of a
+// D_W row per seam row.  Seam rows whose X is INF for all M viewpoints add
+// nothing and are dropped: each round stages 64 rows, compacts the finite
+// ones (ballot + prefix count) into LDS with their X values duplicated into
+// both u16 halves, and pads the count to PF rows with INF.  Partial minima go
+// to Zp[chunk] and k_q_fold folds the chunks; with one chunk (ksplit == 1)
+// the kernel folds in registers instead: column c goes to Z position
+// colpos[c] and, below the root, lifts to exterior port colext[c] of W,
+// Xn[slot][e] = min(Z, X[slot][own port of e] when e is a port of q)
+// (src/minplus_oracle.cpp:1113-1154).  wstat (nullable) accumulates the
+// work done: [0] bytes (D slices of distinct (W, q) items, X reads, Zp
+// writes, each counted once), [1] min-plus updates on finite rows.
 template <int M>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 4))) void k_q_z(int ld, int pc, const uint16_t* __restrict__ DU, size_t node_stride,
-                                            const int* __restrict__ seam_list, const int* __restrict__ seam_off,
-                                            const int* __restrict__ cols, int ncols, int ldp,
-                                            const int4* __restrict__ items, int ksplit,
-                                            const uint16_t* __restrict__ X, uint16_t* __restrict__ Zp,
-                                            size_t part_stride) {
-    constexpr int ACH = 64;  // seam rows staged per round
+__global__ __launch_bounds__(256) void k_q_z(
+    int ld, int pc, const uint16_t* __restrict__ DU, size_t node_stride, const int* __restrict__ seam_list,
+    const int* __restrict__ seam_off, const int* __restrict__ cols, int ngrp, const int* __restrict__ live_cnt,
+    const int* __restrict__ live_list, int ldp, const int4* __restrict__ items, int ksplit,
+    const uint16_t* __restrict__ X, uint16_t* __restrict__ Zp, size_t part_stride,
+    const int* __restrict__ colpos, const int* __restrict__ colext, const int* __restrict__ ext_child,
+    const int* __restrict__ ext_cport, uint16_t* __restrict__ Z, int ldz, uint16_t* __restrict__ Xn, int ne,
+    unsigned long long* __restrict__ wstat) {
+    constexpr int ACH = 64;  // seam rows staged per wavefront and round
     constexpr int PF = 8;    // D rows in flight per lane
-    // Rows past the item's count (v >= m) and past the chunk's end carry INF
-    // in xs: min(acc, INF (+) d) leaves acc unchanged, so the loop runs
-    // branch-free over full tiles.
-    __shared__ __align__(16) uint32_t xs[ACH][M];  // row-major by seam row: one row's M values are contiguous
-    __shared__ unsigned roff[ACH];
+    constexpr int NW = 4;    // wavefronts per workgroup: same columns, interleaved row rounds
+    // Padding rows carry INF in xs: min(acc, INF (+) d) leaves acc unchanged,
+    // so the loop runs branch-free over whole PF blocks.
+    __shared__ __align__(16) uint32_t xs[NW][ACH][M];  // per wavefront, row-major: one row's M values contiguous
+    __shared__ unsigned roff[NW][ACH];
+    __shared__ uint2 red[NW - 1][M][64];  // accumulators of wavefronts 1..NW-1 for the final minimum
     const int4 it = items[blockIdx.x];
-    const int W = it.x, q = it.y, s0 = it.z, m = it.w;
-    const int tid = threadIdx.x;
-    const int c0 = (blockIdx.y * 64 + tid) * 4;
-    const bool active = c0 < ncols;
+    const int W = it.x, q = it.y & 0xffff, s0 = it.z, m = it.w;
+    const bool first_item = (it.y >> 16) & 1;  // first item of its (W, q) run: counts the D bytes once
+    const int tid = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int nlive = live_cnt[W];
+    const int gi = blockIdx.y * 64 + tid;
+    if (blockIdx.y * 64 >= nlive) return;  // whole workgroup past W's live groups
+    const bool active = gi < nlive;
+    const int c0 = active ? live_list[size_t(W) * ngrp + gi] * 4 : 0;
     // Projection columns come in aligned groups of 4 consecutive V_U ports,
     // so one 8-byte load per seam row feeds the lane's 4 columns.
-    const int base = active ? cols[c0] : 0;
+    const int base = cols[c0];
     const uint16_t* D = DU + size_t(W) * node_stride + base;
     const int* seam = seam_list + seam_off[q];
     const int ns = seam_off[q + 1] - seam_off[q];
     const int a_lo = int((long long)ns * blockIdx.z / ksplit), a_hi = int((long long)ns * (blockIdx.z + 1) / ksplit);
+    const unsigned pad_off = unsigned(q * pc + seam[0]) * unsigned(ld);
+    const int rounds = (a_hi - a_lo + NW * ACH - 1) / (NW * ACH);  // the same for every wavefront
+    int rows_done = 0;
     uint32_t acc[M][2];
 #pragma unroll
     for (int v = 0; v < M; ++v) acc[v][0] = acc[v][1] = kInf2;
-    for (int a0 = a_lo; a0 < a_hi; a0 += ACH) {
-        const int na = min(ACH, a_hi - a0);
-        __syncthreads();
+    for (int r = 0; r < rounds; ++r) {
+        const int a0 = a_lo + (r * NW + w) * ACH;
+        const int na = max(0, min(ACH, a_hi - a0));
         const int sr = tid < na ? seam[a0 + tid] : seam[0];
-        roff[tid] = unsigned(q * pc + sr) * unsigned(ld);
+        uint32_t xv[M];
+        bool fin = false;
 #pragma unroll
-        for (int v = 0; v < M; ++v)
-            xs[tid][v] = (v < m && tid < na) ? qdup(X[size_t(s0 + v) * pc + sr]) : kInf2;
+        for (int v = 0; v < M; ++v) {
+            xv[v] = (v < m && tid < na) ? uint32_t(X[size_t(s0 + v) * pc + sr]) : uint32_t(kInf);
+            fin |= xv[v] < kInf;
+        }
+        const unsigned long long bal = __ballot(fin);
+        const int nk = __popcll(bal);
+        const int nkp = (nk + PF - 1) / PF * PF;
+        const int pos = __popcll(bal & ((1ull << tid) - 1ull));
+        __syncthreads();  // previous round's readers are done with xs / roff
+        if (fin) {
+            roff[w][pos] = unsigned(q * pc + sr) * unsigned(ld);
+#pragma unroll
+            for (int v = 0; v < M; ++v) xs[w][pos][v] = qdup(xv[v]);
+        }
+        if (tid >= nk && tid < nkp) {
+            roff[w][tid] = pad_off;
+#pragma unroll
+            for (int v = 0; v < M; ++v) xs[w][tid][v] = kInf2;
+        }
         __syncthreads();
+        rows_done += nk;
         if (!active) continue;
-        for (int a = 0; a < na; a += PF) {
+        for (int a = 0; a < nkp; a += PF) {
             uint2 dv[PF];
 #pragma unroll
-            for (int u = 0; u < PF; ++u) dv[u] = *reinterpret_cast<const uint2*>(D + roff[a + u]);
+            for (int u = 0; u < PF; ++u) dv[u] = *reinterpret_cast<const uint2*>(D + roff[w][a + u]);
 #pragma unroll
             for (int u = 0; u < PF; ++u) {
-                const uint32_t* xr = xs[a + u];
+                const uint32_t* xr = xs[w][a + u];
                 // all saturating adds first, then the mins: independent
                 // instructions back to back instead of add->min chains
                 uint32_t t0[M], t1[M];
@@ -162,21 +223,79 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 4))) void
             }
         }
     }
-    if (!active) return;
-    uint16_t* P = Zp + size_t(blockIdx.z) * part_stride;
+    if (wstat && tid == 0) {
+        const int nact = min(64, nlive - blockIdx.y * 64);
+        const unsigned long long cols4 = 4ull * nact;
+        unsigned long long bytes = first_item ? (unsigned long long)rows_done * cols4 * 2 : 0ull;
+        if (w == 0) {
+            if (blockIdx.y == 0) bytes += (unsigned long long)m * (a_hi - a_lo) * 2;
+            bytes += (unsigned long long)m * cols4 * 2;
+        }
+        atomicAdd(&wstat[0], bytes);
+        atomicAdd(&wstat[1], (unsigned long long)rows_done * cols4 * m);
+    }
+    // minimum over the wavefronts, in wavefront 0
+    if (w > 0) {
 #pragma unroll
-    for (int v = 0; v < M; ++v)
-        if (v < m) *reinterpret_cast<uint2*>(P + size_t(s0 + v) * ldp + c0) = make_uint2(acc[v][0], acc[v][1]);
+        for (int v = 0; v < M; ++v) red[w - 1][v][tid] = make_uint2(acc[v][0], acc[v][1]);
+    }
+    __syncthreads();
+    if (w > 0) return;
+#pragma unroll
+    for (int j = 0; j < NW - 1; ++j)
+#pragma unroll
+        for (int v = 0; v < M; ++v) {
+            const uint2 o = red[j][v][tid];
+            acc[v][0] = qmin2(acc[v][0], o.x);
+            acc[v][1] = qmin2(acc[v][1], o.y);
+        }
+    if (!active) return;
+    if (ksplit > 1) {
+        uint16_t* P = Zp + size_t(blockIdx.z) * part_stride;
+#pragma unroll
+        for (int v = 0; v < M; ++v)
+            if (v < m) *reinterpret_cast<uint2*>(P + size_t(s0 + v) * ldp + c0) = make_uint2(acc[v][0], acc[v][1]);
+        return;
+    }
+    const int4 cp = *reinterpret_cast<const int4*>(colpos + c0);
+    const int4 ce = *reinterpret_cast<const int4*>(colext + c0);
+    const int pos[4] = {cp.x, cp.y, cp.z, cp.w}, ext[4] = {ce.x, ce.y, ce.z, ce.w};
+    int own[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) own[k] = (ext[k] >= 0 && ext_child[ext[k]] == q) ? ext_cport[ext[k]] : -1;
+#pragma unroll
+    for (int v = 0; v < M; ++v) {
+        if (v >= m) continue;
+        const size_t sl = size_t(s0 + v);
+        const uint16_t val[4] = {uint16_t(acc[v][0] & 0xffffu), uint16_t(acc[v][0] >> 16),
+                                 uint16_t(acc[v][1] & 0xffffu), uint16_t(acc[v][1] >> 16)};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (pos[k] >= 0) Z[sl * ldz + pos[k]] = val[k];
+            if (ext[k] >= 0) {
+                uint16_t x = val[k];
+                if (own[k] >= 0) {
+                    const uint16_t d = X[sl * pc + own[k]];
+                    x = d < x ? d : x;
+                }
+                Xn[sl * ne + ext[k]] = x;
+            }
+        }
+    }
 }
 
 void launch_q_z(int M, int ld, int pc, const uint16_t* DU, size_t node_stride, const int* seam_list,
-                const int* seam_off, const int* cols, int ncols, int ldp, const int4* items, int n_items, int ksplit,
-                const uint16_t* X, uint16_t* Zp, size_t part_stride, hipStream_t st) {
-    if (n_items == 0) return;
-    dim3 grid(unsigned(n_items), unsigned((ncols + 255) / 256), unsigned(ksplit));
+                const int* seam_off, const int* cols, int ngrp, const int* live_cnt, const int* live_list,
+                int col_tiles, int ldp, const int4* items, int n_items, int ksplit, const uint16_t* X, uint16_t* Zp,
+                size_t part_stride, const int* colpos, const int* colext, const int* ext_child,
+                const int* ext_cport, uint16_t* Z, int ldz, uint16_t* Xn, int ne, unsigned long long* wstat,
+                hipStream_t st) {
+    if (n_items == 0 || col_tiles == 0) return;
+    const dim3 grid{unsigned(n_items), unsigned(col_tiles), unsigned(ksplit)};
 #define HTG_QZ(MM)                                                                                                  \
-    hipLaunchKernelGGL((k_q_z<MM>), grid, dim3(64), 0, st, ld, pc, DU, node_stride, seam_list, seam_off, cols, ncols, \
-                       ldp, items, ksplit, X, Zp, part_stride)
+    hipLaunchKernelGGL((k_q_z<MM>), grid, dim3(256), 0, st, ld, pc, DU, node_stride, seam_list, seam_off, cols, ngrp, \
+                       live_cnt, live_list, ldp, items, ksplit, X, Zp, part_stride, colpos, colext, ext_child, \
+                       ext_cport, Z, ldz, Xn, ne, wstat)
     if (M <= 1) HTG_QZ(1);
     else if (M <= 2) HTG_QZ(2);
     else if (M <= 4) HTG_QZ(4);
@@ -185,121 +304,157 @@ void launch_q_z(int M, int ld, int pc, const uint16_t* DU, size_t node_stride, c
 #undef HTG_QZ
 }
 
-// Fold the row chunks and lay Z out as contiguous per-child seam blocks
-// (then the remaining exterior ports): Z[slot][pos] = min_c Zp[c][slot][src[pos]],
-// INF where src[pos] < 0 (block padding).
-__global__ __launch_bounds__(256) void k_q_zred(int n, int nz, int ldz, const int* __restrict__ src, int ldp,
-                                                int ksplit, const uint16_t* __restrict__ Zp, size_t part_stride,
-                                                uint16_t* __restrict__ Z) {
-    const long long gid = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (gid >= (long long)n * nz) return;
-    const int s = int(gid / nz), pos = int(gid % nz);
-    const int col = src[pos];
+// Fold of the row chunks, one launch with two index ranges per slot:
+//  pos < nz: Z[slot][pos] = min_c Zp[c][slot][src[pos]], the layout of
+//     contiguous per-child seam blocks then the other exterior ports; INF
+//     where src[pos] < 0 (block padding) or where the column's group is not
+//     live in the slot's node (no passable port: k_q_z skipped it);
+//  pos = nz + e (below the root): lift to W's exterior port e
+//     (src/minplus_oracle.cpp:1113-1154), Xnext[slot][e] = min(folded column
+//     ext_col[e], X[slot][own port of e] when e is a port of the slot's child).
+__device__ __forceinline__ uint16_t fold_col(int col, int s, int W, int ngrp, const uint8_t* __restrict__ live_map,
+                                             const uint16_t* __restrict__ Zp, int ldp, int ksplit,
+                                             size_t part_stride) {
     uint16_t v = kInf;
-    if (col >= 0) {
+    if (col >= 0 && live_map[size_t(W) * ngrp + col / 4]) {
         const uint16_t* p = Zp + size_t(s) * ldp + col;
         for (int c = 0; c < ksplit; ++c) {
             const uint16_t x = p[size_t(c) * part_stride];
             v = x < v ? x : v;
         }
     }
-    Z[size_t(s) * ldz + pos] = v;
+    return v;
 }
 
-void launch_q_zred(int n, int nz, int ldz, const int* src, int ldp, int ksplit, const uint16_t* Zp,
-                   size_t part_stride, uint16_t* Z, hipStream_t st) {
+__global__ __launch_bounds__(256) void k_q_fold(int n, int nz, int ldz, const int* __restrict__ src, int ldp,
+                                                int ksplit, const uint16_t* __restrict__ Zp, size_t part_stride,
+                                                const int2* __restrict__ slot_rc, int side, int nt, int ngrp,
+                                                const uint8_t* __restrict__ live_map, uint16_t* __restrict__ Z, int ne,
+                                                const int* __restrict__ ext_col, const int* __restrict__ ext_child,
+                                                const int* __restrict__ ext_cport, int pc, int cs, int g,
+                                                const uint16_t* __restrict__ Xprev, uint16_t* __restrict__ Xnext) {
+    const int per = nz + ne;
+    const long long gid = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (gid >= (long long)n * per) return;
+    const int s = int(gid / per), pos = int(gid % per);
+    const int2 rc = slot_rc[s];
+    const int W = (rc.x / side) * nt + rc.y / side;
+    if (pos < nz) {
+        Z[size_t(s) * ldz + pos] = fold_col(src[pos], s, W, ngrp, live_map, Zp, ldp, ksplit, part_stride);
+        return;
+    }
+    const int e = pos - nz;
+    uint16_t v = fold_col(ext_col[e], s, W, ngrp, live_map, Zp, ldp, ksplit, part_stride);
+    const int q = ((rc.x / cs) % g) * g + (rc.y / cs) % g;
+    if (ext_child[e] == q) {
+        const uint16_t d = Xprev[size_t(s) * pc + ext_cport[e]];
+        v = d < v ? d : v;
+    }
+    Xnext[size_t(s) * ne + e] = v;
+}
+
+void launch_q_fold(int n, int nz, int ldz, const int* src, int ldp, int ksplit, const uint16_t* Zp,
+                   size_t part_stride, const int2* slot_rc, int side, int nt, int ngrp, const uint8_t* live_map,
+                   uint16_t* Z, int ne, const int* ext_col, const int* ext_child, const int* ext_cport, int pc, int cs,
+                   int g, const uint16_t* Xprev, uint16_t* Xnext, hipStream_t st) {
     if (n == 0) return;
-    const long long total = (long long)n * nz;
-    hipLaunchKernelGGL(k_q_zred, dim3(unsigned((total + 255) / 256)), dim3(256), 0, st, n, nz, ldz, src, ldp, ksplit,
-                       Zp, part_stride, Z);
+    const long long total = (long long)n * (nz + ne);
+    hipLaunchKernelGGL(k_q_fold, dim3(unsigned((total + 255) / 256)), dim3(256), 0, st, n, nz, ldz, src, ldp, ksplit,
+                       Zp, part_stride, slot_rc, side, nt, ngrp, live_map, Z, ne, ext_col, ext_child, ext_cport, pc,
+                       cs, g, Xprev, Xnext);
 }
 
 // Meet: m_ij = min_{b in seam(q_j)} Z_i[b] (+) Z_j[b].  Every candidate is the
 // length of a real walk (i -> b -> j inside W), and Z_j[b] <= X_j[b] on q_j's
 // seam ports (D_W[b][b] = 0), so the candidate is never above the
 // reference's X_i-projection (+) X_j meet: the minimum over levels is exact.
-// One 256-thread workgroup computes a 16 x 16 tile of (i, j) pairs of one
-// (query, W, q2) item, streaming the seam block through LDS as packed pairs.
+// One launch serves every level: Z of each level stays in its own region.
+// One 256-thread workgroup computes an 8 x 8 tile of (i, j) pairs of one
+// (query, W, q2) group, 4 lanes per pair on interleaved columns, streaming
+// the seam block through LDS as packed pairs.
+// Tiles are 8 ints (group, q2, i0, j0, j_end, level); zgeo holds per level
+// (Z offset lo, hi, ldz, 0, zoff[g2], zlen[g2]).  A pair meets at several
+// levels, so the result is merged with a 16-bit atomic minimum.
+__device__ __forceinline__ void atomic_min_u16(uint16_t* o, uint16_t v) {
+    const uintptr_t a = reinterpret_cast<uintptr_t>(o);
+    unsigned* w = reinterpret_cast<unsigned*>(a & ~uintptr_t(3));
+    const int sh = (a & 2) ? 16 : 0;
+    unsigned old = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (;;) {
+        if (((old >> sh) & 0xffffu) <= v) return;
+        const unsigned nw = (old & ~(0xffffu << sh)) | (unsigned(v) << sh);
+        const unsigned prev = atomicCAS(w, old, nw);
+        if (prev == old) return;
+        old = prev;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_q_meet(const int* __restrict__ tiles, const int4* __restrict__ groups,
-                                                const int* __restrict__ glist, const int* __restrict__ zoff,
-                                                const int* __restrict__ zlen, int ldz,
+                                                const int* __restrict__ glist, const int* __restrict__ zgeo, int g2,
                                                 const int* __restrict__ slot_i, const int* __restrict__ q_k,
-                                                const long long* __restrict__ q_out, const uint16_t* __restrict__ Z,
+                                                const long long* __restrict__ q_out, const uint16_t* __restrict__ Zall,
                                                 uint16_t* __restrict__ out) {
-    constexpr int CH = 256;  // packed pairs per round (512 columns)
-    __shared__ uint32_t A[16][CH + 1];
-    __shared__ uint32_t Bm[16][CH + 1];
+    constexpr int CH = 256;      // packed pairs per round (512 columns)
+    constexpr int LDR = CH + 4;  // row stride: the 8 B rows a wavefront reads fall in distinct banks
+    __shared__ uint32_t A[8][LDR];
+    __shared__ uint32_t Bm[8][LDR];
     const int* t = tiles + size_t(blockIdx.x) * 8;
-    const int g = t[0], q2 = t[1], i0 = t[2], j0 = t[3], jend = t[4];
-    const int4 grp = groups[g];
+    const int gi = t[0], q2 = t[1], i0 = t[2], j0 = t[3], jend = t[4], lvl = t[5];
+    const int* zg = zgeo + size_t(lvl) * (4 + 2 * g2);
+    const uint16_t* Z = Zall + ((size_t(unsigned(zg[1])) << 32) | unsigned(zg[0]));
+    const int ldz = zg[2];
+    const int4 grp = groups[gi];
     const int* lst = glist + grp.z;
     const int n = grp.w;
-    const int tid = threadIdx.x, ti = tid >> 4, tj = tid & 15;
-    const int off = zoff[q2], len2 = zlen[q2] / 2;  // block length is even
-    const int ni = min(16, n - i0), nj = min(16, jend - j0);
-    // row pointers of this tile (packed-pair view of Z)
-    const int r = tid >> 4, lane16 = tid & 15;
-    const uint32_t* za = r < ni ? reinterpret_cast<const uint32_t*>(Z + size_t(lst[i0 + r]) * ldz + off) : nullptr;
-    const uint32_t* zb = r < nj ? reinterpret_cast<const uint32_t*>(Z + size_t(lst[j0 + r]) * ldz + off) : nullptr;
-    uint32_t acc = kInf2;
+    const int tid = threadIdx.x;
+    const int sub = tid & 3, pair = tid >> 2, pi = pair >> 3, pj = pair & 7;
+    const int off = zg[4 + q2], len2 = zg[4 + g2 + q2] / 2;  // block length is even
+    const int ni = min(8, n - i0), nj = min(8, jend - j0);
+    // staging: thread (row, l16) loads columns l16, l16 + 16, ... of one of
+    // the 16 rows (8 of A, then 8 of B)
+    const int row = tid >> 4, l16 = tid & 15;
+    const bool is_a = row < 8;
+    const int rr = row & 7;
+    const int nrow = is_a ? ni : nj;
+    const uint32_t* zr =
+        rr < nrow ? reinterpret_cast<const uint32_t*>(Z + size_t(lst[(is_a ? i0 : j0) + rr]) * ldz + off) : nullptr;
+    uint32_t* dst = is_a ? A[rr] : Bm[rr];
+    uint32_t acc0 = kInf2, acc1 = kInf2;
     for (int c0 = 0; c0 < len2; c0 += CH) {
         const int cn = min(CH, len2 - c0);
         __syncthreads();
-        // 16 rows x CH pairs for each side; thread (r, lane16) loads its row's
-        // columns lane16, lane16 + 16, ... (CH / 16 independent loads).
 #pragma unroll
         for (int u = 0; u < CH / 16; ++u) {
-            const int cc = lane16 + 16 * u;
-            A[r][cc] = (za && cc < cn) ? za[c0 + cc] : kInf2;
-            Bm[r][cc] = (zb && cc < cn) ? zb[c0 + cc] : kInf2;
+            const int cc = l16 + 16 * u;
+            dst[cc] = (zr && cc < cn) ? zr[c0 + cc] : kInf2;
         }
         __syncthreads();
-#pragma unroll 8
-        for (int cc = 0; cc < cn; ++cc) acc = qrelax2(acc, A[ti][cc], Bm[tj][cc]);
+        // 4 lanes per pair take interleaved columns; two accumulators
+        int cc = sub;
+        for (; cc + 4 < cn; cc += 8) {
+            acc0 = qrelax2(acc0, A[pi][cc], Bm[pj][cc]);
+            acc1 = qrelax2(acc1, A[pi][cc + 4], Bm[pj][cc + 4]);
+        }
+        if (cc < cn) acc0 = qrelax2(acc0, A[pi][cc], Bm[pj][cc]);
     }
-    if (ti >= ni || tj >= nj) return;
-    const int si = lst[i0 + ti], sj = lst[j0 + tj];
+    uint32_t acc = qmin2(acc0, acc1);
+    acc = qmin2(acc, uint32_t(__shfl_xor(int(acc), 1)));
+    acc = qmin2(acc, uint32_t(__shfl_xor(int(acc), 2)));
+    if (sub != 0 || pi >= ni || pj >= nj) return;
+    const int si = lst[i0 + pi], sj = lst[j0 + pj];
     if (si == sj) return;
     const uint16_t best = min(uint16_t(acc & 0xffffu), uint16_t(acc >> 16));
+    if (best >= kInf) return;
     const int Q = grp.x;
-    uint16_t* o = out + q_out[Q] + size_t(slot_i[si]) * q_k[Q] + slot_i[sj];
-    if (best < *o) *o = best;
+    atomic_min_u16(out + q_out[Q] + size_t(slot_i[si]) * q_k[Q] + slot_i[sj], best);
 }
 
-void launch_q_meet(const int* tiles, int n_tiles, const int4* groups, const int* glist, const int* zoff,
-                   const int* zlen, int ldz, const int* slot_i, const int* q_k, const long long* q_out,
-                   const uint16_t* Z, uint16_t* out, hipStream_t st) {
-    if (n_tiles == 0) return;
-    hipLaunchKernelGGL(k_q_meet, dim3(unsigned(n_tiles)), dim3(256), 0, st, tiles, groups, glist, zoff, zlen, ldz,
-                       slot_i, q_k, q_out, Z, out);
-}
-
-// Lift to W's exterior ports (src/minplus_oracle.cpp:1113-1154).
-__global__ __launch_bounds__(256) void k_q_lift(int n, int ne, const int* __restrict__ ext_zpos,
-                                                const int* __restrict__ ext_child, const int* __restrict__ ext_cport,
-                                                int pc, int cs, int g, int ldz, const int2* __restrict__ slot_rc,
-                                                const uint16_t* __restrict__ Xprev, const uint16_t* __restrict__ Z,
-                                                uint16_t* __restrict__ Xnext) {
-    const long long gid = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (gid >= (long long)n * ne) return;
-    const int s = int(gid / ne), e = int(gid % ne);
-    const int2 rc = slot_rc[s];
-    const int q = ((rc.x / cs) % g) * g + (rc.y / cs) % g;
-    uint16_t v = Z[size_t(s) * ldz + ext_zpos[e]];
-    if (ext_child[e] == q) {
-        const uint16_t d = Xprev[size_t(s) * pc + ext_cport[e]];
-        v = d < v ? d : v;
-    }
-    Xnext[gid] = v;
-}
-
-void launch_q_lift(int n, int ne, const int* ext_zpos, const int* ext_child, const int* ext_cport, int pc, int cs,
-                   int g, int ldz, const int2* slot_rc, const uint16_t* Xprev, const uint16_t* Z, uint16_t* Xnext,
+void launch_q_meet(const int* tiles, int n_tiles, const int4* groups, const int* glist, const int* zgeo, int g2,
+                   const int* slot_i, const int* q_k, const long long* q_out, const uint16_t* Z, uint16_t* out,
                    hipStream_t st) {
-    if (n == 0) return;
-    const long long total = (long long)n * ne;
-    hipLaunchKernelGGL(k_q_lift, dim3(unsigned((total + 255) / 256)), dim3(256), 0, st, n, ne, ext_zpos, ext_child,
-                       ext_cport, pc, cs, g, ldz, slot_rc, Xprev, Z, Xnext);
+    if (n_tiles == 0) return;
+    hipLaunchKernelGGL(k_q_meet, dim3(unsigned(n_tiles)), dim3(256), 0, st, tiles, groups, glist, zgeo, g2, slot_i,
+                       q_k, q_out, Z, out);
 }
 
 }  // namespace htg

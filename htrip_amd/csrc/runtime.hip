@@ -132,6 +132,11 @@ void Profiler::harvest() {
     }
     pending.swap(keep);
 }
+void Profiler::add_work(int cls, double bytes, double updates) {
+    std::lock_guard<std::mutex> lk(mu);
+    acc[cls].bytes += bytes;
+    acc[cls].updates += updates;
+}
 void Profiler::reset() {
     harvest();
     std::lock_guard<std::mutex> lk(mu);
@@ -251,14 +256,144 @@ static void make_level(LevelGeo& L, int lvl, int b, int g, int S, int H) {
     }
     L.nz = int(L.zsrc.size());
     L.ldz = round_up(std::max(L.nz, 8), 8);
+    L.ext_col.assign(d.ne, -1);
+    if (!L.gather)
+        for (int e = 0; e < d.ne; ++e) L.ext_col[e] = L.zsrc[size_t(L.ext_zpos[e])];
+    L.vu_ext.assign(d.nV, -1);
+    for (int e = 0; e < d.ne; ++e) L.vu_ext[size_t(L.ext_idx[e])] = e;
+    L.colpos.assign(L.ncols, -1);
+    L.colext.assign(L.ncols, -1);
+    for (int i = 0; i < L.ncols; ++i) L.colpos[i] = zpos_of[L.zcols[i]];
+    if (!L.gather)
+        for (int e = 0; e < d.ne; ++e) L.colext[size_t(L.ext_col[e])] = e;
+    L.seam_live.assign(size_t(L.nn) * g * g, 0);
+    L.ng8 = d.ld / 8;
+    L.ngrp = L.ncols / 4;
 }
 
+int Map::init_live() {
+    // per level: cnt (nn), list (nn * ngrp), cnt8 (nn), list8 (nn * ng8) ints, then map (nn * ngrp) bytes
+    auto a16 = [](size_t x) { return (x + 15) / 16 * 16; };
+    std::vector<size_t> off(lv.size() * 5);
+    size_t total = 0;
+    for (size_t l = 0; l < lv.size(); ++l) {
+        const LevelGeo& L = lv[l];
+        const size_t nn = size_t(L.nn);
+        const size_t sz[5] = {nn * 4, nn * size_t(L.ngrp) * 4, nn * 4, nn * size_t(L.ng8) * 4, nn * size_t(L.ngrp)};
+        for (int k = 0; k < 5; ++k) {
+            off[l * 5 + k] = total;
+            total += a16(std::max<size_t>(sz[k], 16));
+        }
+    }
+    int rc;
+    if ((rc = live_h.ensure(total)) || (rc = live_d.ensure(total))) return rc;
+    std::memset(live_h.p, 0, total);
+    char* h = static_cast<char*>(live_h.p);
+    char* d = static_cast<char*>(live_d.p);
+    for (size_t l = 0; l < lv.size(); ++l) {
+        LevelGeo& L = lv[l];
+        L.h_live_cnt = reinterpret_cast<int*>(h + off[l * 5 + 0]);
+        L.h_live_list = reinterpret_cast<int*>(h + off[l * 5 + 1]);
+        L.h_live8_cnt = reinterpret_cast<int*>(h + off[l * 5 + 2]);
+        L.h_live8_list = reinterpret_cast<int*>(h + off[l * 5 + 3]);
+        L.h_live_map = reinterpret_cast<uint8_t*>(h + off[l * 5 + 4]);
+        L.d_live_cnt = reinterpret_cast<const int*>(d + off[l * 5 + 0]);
+        L.d_live_list = reinterpret_cast<const int*>(d + off[l * 5 + 1]);
+        L.d_live8_cnt = reinterpret_cast<const int*>(d + off[l * 5 + 2]);
+        L.d_live8_list = reinterpret_cast<const int*>(d + off[l * 5 + 3]);
+        L.d_live_map = reinterpret_cast<const uint8_t*>(d + off[l * 5 + 4]);
+    }
+    hipError_t e;
+    if (!live_evt && (e = hipEventCreateWithFlags(&live_evt, hipEventDisableTiming)) != hipSuccess)
+        return hip_fail(e, "hipEventCreate");
+    return HTG_OK;
+}
+
+Map::~Map() {
+    if (live_evt) {
+        (void)hipEventSynchronize(live_evt);
+        (void)hipEventDestroy(live_evt);
+    }
+}
+
+int Map::refresh_live(const std::vector<int2>* cells) {
+    hipError_t e;
+    if (live_pending) {  // the previous upload still reads the pinned block
+        if ((e = hipEventSynchronize(live_evt)) != hipSuccess) return hip_fail(e, "live upload wait");
+        live_pending = false;
+    }
+    const char* hbase = static_cast<const char*>(live_h.p);
+    size_t lo = ~size_t(0), hi = 0;  // changed byte range of the block
+    auto touch = [&](const void* p, size_t bytes) {
+        const size_t o = size_t(static_cast<const char*>(p) - hbase);
+        lo = std::min(lo, o);
+        hi = std::max(hi, o + bytes);
+    };
+    for (int l = 1; l <= H; ++l) {
+        LevelGeo& L = lv[l - 1];
+        const int s = L.d.s, nt = L.d.nt, ng = L.ngrp;
+        std::vector<int> nodes;
+        if (!cells) {
+            nodes.resize(size_t(L.nn));
+            std::iota(nodes.begin(), nodes.end(), 0);
+        } else {
+            for (const int2& c : *cells) nodes.push_back((c.x / s) * nt + c.y / s);
+            std::sort(nodes.begin(), nodes.end());
+            nodes.erase(std::unique(nodes.begin(), nodes.end()), nodes.end());
+        }
+        if (nodes.empty()) continue;
+        for (int W : nodes) {
+            const int R0 = (W / nt) * s, C0 = (W % nt) * s;
+            auto live_port = [&](int v) {
+                const short2 p = L.vu_rc[size_t(v)];
+                return pass[size_t(R0 + p.x) * S + size_t(C0 + p.y)] != 0;
+            };
+            int cnt = 0;
+            for (int gi = 0; gi < ng; ++gi) {
+                bool live = false;
+                for (int k = 0; k < 4 && !live; ++k) live = live_port(L.zcols[size_t(4 * gi + k)]);
+                L.h_live_map[size_t(W) * ng + gi] = live ? 1 : 0;
+                if (live) L.h_live_list[size_t(W) * ng + cnt++] = gi;
+            }
+            L.h_live_cnt[W] = cnt;
+            for (int q = 0; q < g * g; ++q) {
+                int ns_live = 0;
+                for (int i = L.seam_off[q]; i < L.seam_off[q + 1]; ++i) ns_live += live_port(q * L.d.pc + L.seam_list[i]);
+                L.seam_live[size_t(W) * g * g + q] = ns_live;
+            }
+            int c8 = 0;
+            for (int o = 0; o < L.ng8; ++o) {
+                bool live = false;
+                for (int v = 8 * o; v < 8 * o + 8 && v < L.d.nV && !live; ++v) live = live_port(v);
+                if (live) L.h_live8_list[size_t(W) * L.ng8 + c8++] = o;
+            }
+            L.h_live8_cnt[W] = c8;
+        }
+        const size_t a = size_t(nodes.front()), z = size_t(nodes.back()) + 1;
+        touch(L.h_live_cnt + a, (z - a) * 4);
+        touch(L.h_live_list + a * ng, (z - a) * ng * 4);
+        touch(L.h_live8_cnt + a, (z - a) * 4);
+        touch(L.h_live8_list + a * L.ng8, (z - a) * L.ng8 * 4);
+        touch(L.h_live_map + a * ng, (z - a) * ng);
+    }
+    if (hi <= lo) return HTG_OK;
+    if ((e = hipMemcpyAsync(static_cast<char*>(live_d.p) + lo, hbase + lo, hi - lo, hipMemcpyHostToDevice, stream)) !=
+            hipSuccess ||
+        (e = hipEventRecord(live_evt, stream)) != hipSuccess)
+        return hip_fail(e, "upload live groups");
+    live_pending = true;
+    return HTG_OK;
+}
+
+// Synchronous copies go through the map's stream: the null stream would take
+// a hardware queue of its own.
 template <typename T>
-static int upload(DevBuf& dst, const std::vector<T>& src) {
+static int upload(DevBuf& dst, const std::vector<T>& src, hipStream_t st) {
     int rc = dst.ensure(std::max<size_t>(src.size() * sizeof(T), 16));
     if (rc) return rc;
     if (!src.empty()) {
-        hipError_t e = hipMemcpy(dst.p, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice);
+        hipError_t e = hipMemcpyAsync(dst.p, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
         if (e != hipSuccess) return hip_fail(e, "hipMemcpy geometry");
     }
     return HTG_OK;
@@ -309,20 +444,23 @@ int Map::ensure_build_ws(int lvl) {
 
 // Closure over each listed node's pivot list (already compacted into ws_piv
 // with counts in ws_counts; host copy of counts in hcounts).
-int Map::run_closure(int lvl, int n_list, const std::vector<int>& hcounts) {
+int Map::run_closure(int lvl, const int* nodes, const std::vector<int>& hnodes, const std::vector<int>& hcounts) {
     LevelGeo& L = lv[lvl - 1];
-    int maxc = 0;
-    for (int i = 0; i < n_list; ++i) maxc = std::max(maxc, hcounts[i]);
+    const int n_list = int(hnodes.size());
+    int maxc = 0, max8 = 0;
+    for (int i = 0; i < n_list; ++i) {
+        maxc = std::max(maxc, hcounts[i]);
+        if (hcounts[i] > 0) max8 = std::max(max8, L.h_live8_cnt[hnodes[i]]);
+    }
     const int P = tune.pivot_block;
     const int blocks = (maxc + P - 1) / P;
-    const int* nodes = static_cast<const int*>(ws_nodes.p);
     for (int t = 0; t < blocks; ++t) {
         double upd_p = 0, upd_u = 0, by_p = 0, by_u = 0;
         if (prof_ref().on) {
-            const double nV = L.d.nV;
             for (int i = 0; i < n_list; ++i) {
                 const double p = std::max(0, std::min(P, hcounts[i] - t * P));
                 if (p <= 0) continue;
+                const double nV = 8.0 * L.h_live8_cnt[hnodes[i]];  // live rows / columns
                 upd_p += p * p * nV + p * p * p;
                 by_p += (p * nV + p * nV) * 2;  // pivot rows in, R out
                 upd_u += p * nV * nV;
@@ -332,12 +470,15 @@ int Map::run_closure(int lvl, int n_list, const std::vector<int>& hcounts) {
         int pi = prof_ref().begin(HTG_K_CLOSURE_PIVOTS, stream, by_p, upd_p);
         launch_closure_pivots(P, L.d.nV, L.d.ld, nodes, n_list, static_cast<const int*>(ws_piv.p), L.d.nV,
                               static_cast<const int*>(ws_counts.p), t, static_cast<const uint16_t*>(L.DU.p),
-                              L.node_stride, static_cast<uint16_t*>(ws_R.p), stream);
+                              L.node_stride, L.d_live8_cnt,
+                              L.d_live8_list, L.ng8, max8, static_cast<uint16_t*>(ws_R.p),
+                              stream);
         prof_ref().end(pi, stream);
         pi = prof_ref().begin(HTG_K_CLOSURE_UPDATE, stream, by_u, upd_u);
         launch_closure_update(P, L.d.nV, L.d.ld, nodes, n_list, static_cast<const int*>(ws_counts.p), t,
-                              static_cast<uint16_t*>(L.DU.p), L.node_stride, static_cast<const uint16_t*>(ws_R.p),
-                              stream);
+                              static_cast<uint16_t*>(L.DU.p), L.node_stride, L.d_live8_cnt,
+                              L.d_live8_list, L.ng8, max8,
+                              static_cast<const uint16_t*>(ws_R.p), stream);
         prof_ref().end(pi, stream);
     }
     hipError_t e = hipGetLastError();
@@ -387,7 +528,7 @@ int Map::build() {
         if ((e = hipMemcpyAsync(hc.data(), ws_counts.p, size_t(L.nn) * 4, hipMemcpyDeviceToHost, stream)) != hipSuccess)
             return hip_fail(e, "download counts");
         if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "build sync");
-        if ((rc = run_closure(lvl, L.nn, hc))) return rc;
+        if ((rc = run_closure(lvl, static_cast<const int*>(ws_nodes.p), all, hc))) return rc;
         launch_project_su(L.d.nV, L.d.ld, L.d.ne, static_cast<const int*>(L.d_ext_idx.p),
                           static_cast<const int*>(ws_nodes.p), L.nn, static_cast<const uint16_t*>(L.DU.p),
                           L.node_stride, static_cast<uint16_t*>(L.SU.p), nullptr, stream);
@@ -457,6 +598,7 @@ int Map::open_cells(const int32_t* rcs, int n) {
             newly.push_back(p);
         }
     }
+    if (!newly.empty() && (rc = refresh_live(&newly))) return rc;
     // 2. leaf pivot counts, as the reference's leaf update would collect them
     int piv = 0;
     for (int leaf : dirty) {
@@ -504,29 +646,76 @@ int Map::open_cells(const int32_t* rcs, int n) {
             seams[lca][node].push_back({u, v});
         }
     }
-    // 4. device: passability, dirty leaves
-    if (!newly.empty()) {
-        if ((rc = ws_cells.ensure(newly.size() * sizeof(int2)))) return rc;
-        if ((e = hipMemcpyAsync(ws_cells.p, newly.data(), newly.size() * sizeof(int2), hipMemcpyHostToDevice, stream)) !=
-            hipSuccess)
-            return hip_fail(e, "upload cells");
-        launch_set_pass(static_cast<uint8_t*>(d_pass.p), S, static_cast<const int2*>(ws_cells.p), int(newly.size()),
-                        stream);
-    }
+    // 4. candidate nodes per level: every ancestor of a dirty leaf (plus, to
+    // be safe, every node with new seam edges).  The child masks of the
+    // reference's routing (a parent is dirty when a child summary changed,
+    // src/minplus_oracle.cpp:650-722) are resolved on the device from the
+    // changed flags of the level below, so a level needs one host sync.
     const int nd = int(dirty.size());
     this->dirty.leaves = dirty;
     this->dirty.cells = newly;
-    if ((rc = ws_leaves.ensure(size_t(nd) * 4))) return rc;
-    if ((rc = ws_leaf_changed.ensure(size_t(nd) * 4))) return rc;
-    if ((e = hipMemcpyAsync(ws_leaves.p, dirty.data(), size_t(nd) * 4, hipMemcpyHostToDevice, stream)) != hipSuccess)
-        return hip_fail(e, "upload leaves");
-    if ((e = hipMemsetAsync(ws_leaf_changed.p, 0, size_t(nd) * 4, stream)) != hipSuccess)
-        return hip_fail(e, "clear leaf flags");
+    const int g2 = g * g;
+    std::vector<std::vector<int>> cand(size_t(H) + 1), cslot(size_t(H) + 1);
+    std::vector<std::vector<int4>> edges(size_t(H) + 1);
+    {
+        std::vector<int> below = dirty;  // level 0: dirty leaves in upload order
+        int below_nt = nl;
+        for (int lvl = 1; lvl <= H; ++lvl) {
+            const LevelGeo& L = lv[lvl - 1];
+            std::vector<int>& c = cand[size_t(lvl)];
+            for (int ch : below) c.push_back(((ch / below_nt) / g) * L.d.nt + (ch % below_nt) / g);
+            for (auto& kv : seams[size_t(lvl)]) c.push_back(kv.first);
+            std::sort(c.begin(), c.end());
+            c.erase(std::unique(c.begin(), c.end()), c.end());
+            std::unordered_map<int, int> pos_below;
+            for (size_t k = 0; k < below.size(); ++k) pos_below[below[k]] = int(k);
+            std::vector<int>& cs = cslot[size_t(lvl)];
+            cs.assign(c.size() * size_t(g2), -1);
+            for (size_t i = 0; i < c.size(); ++i) {
+                const int tx = c[i] % L.d.nt, ty = c[i] / L.d.nt;
+                for (int q = 0; q < g2; ++q) {
+                    auto it = pos_below.find((ty * g + q / g) * below_nt + tx * g + q % g);
+                    if (it != pos_below.end()) cs[i * size_t(g2) + size_t(q)] = it->second;
+                }
+                auto sit = seams[size_t(lvl)].find(c[i]);
+                if (sit == seams[size_t(lvl)].end()) continue;
+                auto& lst = sit->second;
+                std::sort(lst.begin(), lst.end());
+                lst.erase(std::unique(lst.begin(), lst.end()), lst.end());
+                for (auto& uv : lst) edges[size_t(lvl)].push_back(make_int4(int(i), uv.first, uv.second, 1));
+            }
+            below = c;
+            below_nt = L.d.nt;
+        }
+    }
+    // one packed upload; changed flags: leaves, then each level's candidates
+    Packer pk;
+    const size_t o_cells = pk.add(newly), o_leaves = pk.add(dirty);
+    std::vector<size_t> o_cand(size_t(H) + 1), o_cslot(size_t(H) + 1), o_edges(size_t(H) + 1), o_chg(size_t(H) + 1);
+    size_t nchg = size_t(nd);
+    for (int lvl = 1; lvl <= H; ++lvl) {
+        o_cand[size_t(lvl)] = pk.add(cand[size_t(lvl)]);
+        o_cslot[size_t(lvl)] = pk.add(cslot[size_t(lvl)]);
+        o_edges[size_t(lvl)] = pk.add(edges[size_t(lvl)]);
+        o_chg[size_t(lvl)] = nchg;
+        nchg += cand[size_t(lvl)].size();
+    }
+    if ((rc = ws_upd_h.ensure(pk.size)) || (rc = ws_upd.ensure(pk.size)) || (rc = ws_chg.ensure(nchg * 4 + 16)) ||
+        (rc = ws_chg_h.ensure(nchg * 4 + 16)))
+        return rc;
+    pk.write(static_cast<char*>(ws_upd_h.p));
+    if ((e = hipMemcpyAsync(ws_upd.p, ws_upd_h.p, pk.size, hipMemcpyHostToDevice, stream)) != hipSuccess ||
+        (e = hipMemsetAsync(ws_chg.p, 0, nchg * 4, stream)) != hipSuccess)
+        return hip_fail(e, "upload update lists");
+    const char* up = static_cast<const char*>(ws_upd.p);
+    int* chg = static_cast<int*>(ws_chg.p);
+    if (!newly.empty())
+        launch_set_pass(static_cast<uint8_t*>(d_pass.p), S, reinterpret_cast<const int2*>(up + o_cells),
+                        int(newly.size()), stream);
     const double leaf_bytes = (double(b) * b * b * b + double(4 * b - 4) * (4 * b - 4)) * 2 + double(b) * b;
     int lpi = prof_ref().begin(HTG_K_LEAF_APSP, stream, leaf_bytes * nd, 0);
-    launch_leaf_apsp(static_cast<const uint8_t*>(d_pass.p), S, b, nl, static_cast<const int*>(ws_leaves.p), nd,
-                     static_cast<uint16_t*>(leafD.p), static_cast<uint16_t*>(leafS.p),
-                     static_cast<int*>(ws_leaf_changed.p), stream);
+    launch_leaf_apsp(static_cast<const uint8_t*>(d_pass.p), S, b, nl, reinterpret_cast<const int*>(up + o_leaves), nd,
+                     static_cast<uint16_t*>(leafD.p), static_cast<uint16_t*>(leafS.p), chg, stream);
     prof_ref().end(lpi, stream);
     stats.n_touched_leaves = nd;
     stats.n_pivots = piv;
@@ -534,115 +723,88 @@ int Map::open_cells(const int32_t* rcs, int n) {
         if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "update sync");
         return HTG_OK;  // the reference returns before bumping epoch here
     }
-    std::vector<int> lchanged(nd);
-    if ((e = hipMemcpyAsync(lchanged.data(), ws_leaf_changed.p, size_t(nd) * 4, hipMemcpyDeviceToHost, stream)) !=
-        hipSuccess)
-        return hip_fail(e, "download leaf flags");
-    if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "update sync");
-    // child masks of level-1 parents, in first-touch order
-    std::vector<int> active;
-    std::map<int, uint32_t> mask;
-    for (int i = 0; i < nd; ++i) {
-        if (!lchanged[i]) continue;
-        const int tx = dirty[i] % nl, ty = dirty[i] / nl;
-        const int parent = (ty / g) * lv[0].d.nt + tx / g;
-        const int quad = (ty % g) * g + tx % g;
-        if (!mask.count(parent)) active.push_back(parent);
-        mask[parent] |= 1u << quad;
-    }
-    int anc = 0;
+    bool below_any = true;  // the level below may have changed flags set
+    const int* chg_below = chg;
+    int top = 0;  // highest level processed
     for (int lvl = 1; lvl <= H; ++lvl) {
         LevelGeo& L = lv[lvl - 1];
-        for (auto& kv : seams[lvl])
-            if (!mask.count(kv.first)) {
-                active.push_back(kv.first);
-                mask[kv.first] = 0;
-            }
-        if (active.empty()) {
+        const std::vector<int>& cl = cand[size_t(lvl)];
+        const int na = int(cl.size());
+        int* chg_here = chg + o_chg[size_t(lvl)];
+        if (!below_any && edges[size_t(lvl)].empty()) {  // nothing can change at this level
             bool higher = false;
-            for (int hl = lvl + 1; hl <= H && !higher; ++hl) higher = !seams[hl].empty();
+            for (int hl = lvl + 1; hl <= H && !higher; ++hl) higher = !edges[size_t(hl)].empty();
             if (!higher) break;
+            chg_below = chg_here;  // all clear
             continue;
         }
-        anc += int(active.size());
-        const int na = int(active.size());
-        this->dirty.nodes[lvl - 1] = active;
+        top = lvl;
         if ((rc = ensure_build_ws(lvl))) return rc;
-        std::vector<uint32_t> hmask(na);
-        std::vector<int4> edges;
-        for (int i = 0; i < na; ++i) {
-            hmask[i] = mask[active[i]];
-            auto it = seams[lvl].find(active[i]);
-            if (it == seams[lvl].end()) continue;
-            auto& lst = it->second;
-            std::sort(lst.begin(), lst.end());
-            lst.erase(std::unique(lst.begin(), lst.end()), lst.end());
-            for (auto& uv : lst) edges.push_back(make_int4(i, uv.first, uv.second, 1));
-        }
-        if ((e = hipMemcpyAsync(ws_nodes.p, active.data(), size_t(na) * 4, hipMemcpyHostToDevice, stream)) != hipSuccess)
-            return hip_fail(e, "upload nodes");
-        if ((e = hipMemcpyAsync(ws_mask.p, hmask.data(), size_t(na) * 4, hipMemcpyHostToDevice, stream)) != hipSuccess)
-            return hip_fail(e, "upload masks");
+        const int* dnodes = reinterpret_cast<const int*>(up + o_cand[size_t(lvl)]);
+        int max8 = 0;
+        for (int node : cl) max8 = std::max(max8, L.h_live8_cnt[node]);
         if ((e = hipMemsetAsync(ws_flags.p, 0, size_t(na) * L.d.nV, stream)) != hipSuccess)
             return hip_fail(e, "clear flags");
         size_t cstride;
         const uint16_t* csum = child_summaries(lvl, cstride);
-        launch_child_relax(lvl, g, L.d.pc, L.d.nV, L.d.ld, L.d.nt, csum, cstride, static_cast<const int*>(ws_nodes.p),
-                           static_cast<const uint32_t*>(ws_mask.p), na, static_cast<uint16_t*>(L.DU.p), L.node_stride,
-                           static_cast<uint8_t*>(ws_flags.p), stream);
-        if (!edges.empty()) {
-            if ((rc = ws_edges.ensure(edges.size() * sizeof(int4)))) return rc;
-            if ((e = hipMemcpyAsync(ws_edges.p, edges.data(), edges.size() * sizeof(int4), hipMemcpyHostToDevice,
-                                    stream)) != hipSuccess)
-                return hip_fail(e, "upload edges");
-            launch_edge_relax(L.d.nV, L.d.ld, static_cast<const int4*>(ws_edges.p), int(edges.size()),
-                              static_cast<const int*>(ws_nodes.p), static_cast<uint16_t*>(L.DU.p), L.node_stride,
+        launch_child_relax_live(g, L.d.pc, L.d.nV, L.d.ld, L.d.nt, csum, cstride, dnodes,
+                                reinterpret_cast<const int*>(up + o_cslot[size_t(lvl)]), chg_below, na, L.d_live8_cnt,
+                                L.d_live8_list, L.ng8, max8, static_cast<uint16_t*>(L.DU.p), L.node_stride,
+                                static_cast<uint8_t*>(ws_flags.p), stream);
+        if (!edges[size_t(lvl)].empty())
+            launch_edge_relax(L.d.nV, L.d.ld, reinterpret_cast<const int4*>(up + o_edges[size_t(lvl)]),
+                              int(edges[size_t(lvl)].size()), dnodes, static_cast<uint16_t*>(L.DU.p), L.node_stride,
                               static_cast<uint8_t*>(ws_flags.p), stream);
-        }
-        launch_compact_flags(L.d.nV, static_cast<const int*>(ws_nodes.p), na, static_cast<uint8_t*>(ws_flags.p),
-                             static_cast<int*>(ws_piv.p), L.d.nV, static_cast<int*>(ws_counts.p), stream);
-        std::vector<int> hc(na);
+        launch_compact_flags_live(L.d.nV, dnodes, na, L.d_live8_cnt, L.d_live8_list, L.ng8,
+                                  static_cast<uint8_t*>(ws_flags.p), static_cast<int*>(ws_piv.p), L.d.nV,
+                                  static_cast<int*>(ws_counts.p), stream);
+        std::vector<int> hc(static_cast<size_t>(na));
         if ((e = hipMemcpyAsync(hc.data(), ws_counts.p, size_t(na) * 4, hipMemcpyDeviceToHost, stream)) != hipSuccess)
             return hip_fail(e, "download counts");
         if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "update sync");
-        if (debug_level()) {
-            long long sum = 0;
-            int mx = 0;
-            for (int v : hc) {
-                sum += v;
-                mx = std::max(mx, v);
-            }
+        long long sum = 0;
+        int mx = 0;
+        for (int i = 0; i < na; ++i) {
+            sum += hc[size_t(i)];
+            mx = std::max(mx, hc[size_t(i)]);
+            if (hc[size_t(i)] > 0) this->dirty.nodes[size_t(lvl) - 1].push_back(cl[size_t(i)]);
+        }
+        if (debug_level())
             std::fprintf(stderr, "[htg] update lvl %d: %d nodes (nV %d), pivots sum %lld max %d\n", lvl, na, L.d.nV,
                          sum, mx);
+        if (sum > 0) {
+            if ((rc = run_closure(lvl, dnodes, cl, hc))) return rc;
+            // S_U refresh for nodes that closed over something (the reference
+            // skips nodes with an empty pivot set, src/internal_node.cpp:368-370)
+            launch_project_su_live(L.d.nV, L.d.ld, L.d.ne, static_cast<const int*>(L.d_vu_ext.p), dnodes,
+                                   static_cast<const int*>(ws_counts.p), na, L.d_live8_cnt, L.d_live8_list, L.ng8,
+                                   max8, static_cast<const uint16_t*>(L.DU.p), L.node_stride,
+                                   static_cast<uint16_t*>(L.SU.p), chg_here, stream);
+            if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "update launch");
         }
-        if ((rc = run_closure(lvl, na, hc))) return rc;
-        // S_U refresh for nodes that closed over something (the reference
-        // skips nodes with an empty pivot set, src/internal_node.cpp:368-370)
-        if ((e = hipMemsetAsync(ws_changed.p, 0, size_t(na) * 4, stream)) != hipSuccess)
-            return hip_fail(e, "clear changed");
-        launch_project_su(L.d.nV, L.d.ld, L.d.ne, static_cast<const int*>(L.d_ext_idx.p),
-                          static_cast<const int*>(ws_nodes.p), na, static_cast<const uint16_t*>(L.DU.p), L.node_stride,
-                          static_cast<uint16_t*>(L.SU.p), static_cast<int*>(ws_changed.p), stream);
-        std::vector<int> nchanged(na);
-        if ((e = hipMemcpyAsync(nchanged.data(), ws_changed.p, size_t(na) * 4, hipMemcpyDeviceToHost, stream)) !=
-            hipSuccess)
-            return hip_fail(e, "download changed");
-        if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "update sync");
-        std::vector<int> next_active;
-        std::map<int, uint32_t> next_mask;
-        if (lvl < H) {
-            const int ntp = lv[lvl].d.nt;
-            for (int i = 0; i < na; ++i) {
-                if (!nchanged[i]) continue;
-                const int tx = active[i] % L.d.nt, ty = active[i] / L.d.nt;
-                const int parent = (ty / g) * ntp + tx / g;
-                const int quad = (ty % g) * g + tx % g;
-                if (!next_mask.count(parent)) next_active.push_back(parent);
-                next_mask[parent] |= 1u << quad;
+        below_any = sum > 0;
+        chg_below = chg_here;
+    }
+    // stats: a node is a dirty ancestor when a child summary changed or it
+    // got new seam edges (the reference's routing)
+    if ((e = hipMemcpyAsync(ws_chg_h.p, ws_chg.p, nchg * 4, hipMemcpyDeviceToHost, stream)) != hipSuccess)
+        return hip_fail(e, "download changed");
+    if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "update sync");
+    const int* hchg = static_cast<const int*>(ws_chg_h.p);
+    int anc = 0;
+    for (int lvl = 1; lvl <= top; ++lvl) {
+        const std::vector<int>& cl = cand[size_t(lvl)];
+        const size_t below_off = lvl == 1 ? 0 : o_chg[size_t(lvl) - 1];
+        std::vector<uint8_t> has_edge(cl.size(), 0);
+        for (const int4& ed : edges[size_t(lvl)]) has_edge[size_t(ed.x)] = 1;
+        for (size_t i = 0; i < cl.size(); ++i) {
+            bool act = has_edge[i] != 0;
+            for (int q = 0; q < g2 && !act; ++q) {
+                const int sl = cslot[size_t(lvl)][i * size_t(g2) + size_t(q)];
+                act = sl >= 0 && hchg[below_off + size_t(sl)] != 0;
             }
+            anc += act;
         }
-        active = std::move(next_active);
-        mask = std::move(next_mask);
     }
     stats.n_dirty_ancestors = anc;
     if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "update sync");
@@ -661,6 +823,10 @@ int Map::catch_up_from(const Map& src, const DirtySet& ds) {
     if (!ds.cells.empty() &&
         (e = hipMemcpyAsync(d_pass.p, src.d_pass.p, size_t(S) * S, hipMemcpyDeviceToDevice, stream)) != hipSuccess)
         return hip_fail(e, "catch-up pass");
+    if (!ds.cells.empty()) {
+        const int rc = refresh_live(&ds.cells);
+        if (rc) return rc;
+    }
     for (int leaf : ds.leaves) {
         if ((e = hipMemcpyAsync(static_cast<uint16_t*>(leafD.p) + size_t(leaf) * n * n,
                                 static_cast<const uint16_t*>(src.leafD.p) + size_t(leaf) * n * n, n * n * 2,
@@ -704,6 +870,11 @@ int Map::query_finish(QueryWS& ws, uint16_t* out_host) {
     hipError_t e = hipEventSynchronize(ws.done);
     if (e != hipSuccess) return hip_fail(e, "query sync");
     if (out_host && ws.total_out) std::memcpy(out_host, ws.h_out.p, size_t(ws.total_out) * 2);
+    if (ws.counted) {
+        const unsigned long long* w = static_cast<const unsigned long long*>(ws.h_wstat.p);
+        prof_ref().add_work(HTG_K_Q_PROJECT, double(w[0]), double(w[1]));
+        ws.counted = false;
+    }
     prof_ref().harvest();
     return HTG_OK;
 }
@@ -775,6 +946,28 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
         P.slot_i[s] = sk[s].i;
     }
     P.levels.resize(H);
+    const int g2 = g * g;
+    const size_t nsl = size_t(std::max(ns, 1));
+    // Q layout: X_0 .. X_{H-1}, then Z_1 .. Z_H
+    std::vector<size_t> xoff(size_t(H) + 1, 0), zoffq(size_t(H) + 1, 0);
+    size_t qsz = 0;
+    auto a8 = [](size_t x) { return (x + 7) / 8 * 8; };
+    for (int l = 0; l < H; ++l) {
+        xoff[size_t(l)] = qsz;
+        qsz += a8(nsl * size_t(l == 0 ? 4 * b - 4 : lv[size_t(l) - 1].d.ne));
+    }
+    for (int l = 1; l <= H; ++l) {
+        zoffq[size_t(l)] = qsz;
+        qsz += a8(nsl * size_t(lv[size_t(l) - 1].ldz));
+    }
+    size_t zbase = zoffq[1 <= H ? 1 : 0];
+    int fill_blocks = 0;
+    auto add_fill = [&](size_t start, size_t count) {  // start and count are multiples of 8
+        const long long n8 = (long long)(count / 8);
+        if (n8 == 0) return;
+        P.fills.push_back(make_int4(int(start & 0xffffffffu), int(start >> 32), int(n8), fill_blocks));
+        fill_blocks += int((n8 + 255) / 256);
+    };
     unsigned long long div = 1;
     for (int l = 1; l <= H; ++l) {
         const LevelGeo& L = lv[l - 1];
@@ -790,26 +983,46 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
         int M = 1;
         while (M < maxrun && M < tune.zq_tile) M *= 2;
         LP.M = M;
+        long long waves = 0;  // wavefronts per row chunk
+        int rows_max = 0;     // passable seam rows of the largest item
         for (int s = 0; s < ns;) {
             int t = s + 1;
             while (t < ns && sk[t].key / div == sk[s].key / div) ++t;
             const int2 rc0 = P.slot_rc[s];
             const int W = (rc0.x / L.d.s) * L.d.nt + rc0.y / L.d.s;
-            const int q = int((sk[s].key / div) % unsigned(g * g));
-            for (int a = s; a < t; a += M) LP.items.push_back(make_int4(W, q, a, std::min(M, t - a)));
-            const double nsq = L.seam_off[q + 1] - L.seam_off[q];
-            LP.z_bytes += (nsq * L.ncols + double(t - s) * (nsq + L.ncols)) * 2;
-            LP.z_updates += double(t - s) * nsq * L.ncols;
+            const int q = int((sk[s].key / div) % unsigned(g2));
+            const int tiles = (L.h_live_cnt[W] + 63) / 64;
+            LP.col_tiles = std::max(LP.col_tiles, tiles);
+            rows_max = std::max(rows_max, L.seam_live[size_t(W) * g2 + q]);
+            for (int a = s; a < t; a += M) {
+                LP.items.push_back(make_int4(W, q | (a == s ? 1 << 16 : 0), a, std::min(M, t - a)));
+                waves += tiles;
+            }
             s = t;
         }
-        {  // split the seam rows until the launch has enough wavefronts
-            const long long base = (long long)LP.items.size() * ((L.ncols + 255) / 256);
-            int ks = base > 0 ? int((2048 + base - 1) / base) : 1;
+        {  // Row chunks: one (fold in registers, no fold launch) while an item
+           // has at most kRowsPerChunk passable seam rows; beyond that, enough
+           // chunks for that bound and to give the launch ~2048 workgroups.
+            constexpr int kRowsPerChunk = 512;  // 4 wavefronts x 128 rows
+            int ks = (rows_max + kRowsPerChunk - 1) / kRowsPerChunk;
+            if (ks > 1 && waves > 0) ks = std::max(ks, int((2048 + waves - 1) / waves));
             ks = std::min(ks, std::max(1, L.nsmax / 32));
             LP.ksplit = std::max(1, ks);
         }
+        // Z region of this level and its layout for the meet
+        zbase = zoffq[size_t(l)];
+        P.zgeo.push_back(int(zbase & 0xffffffffu));
+        P.zgeo.push_back(int(zbase >> 32));
+        P.zgeo.push_back(L.ldz);
+        P.zgeo.push_back(0);
+        P.zgeo.insert(P.zgeo.end(), L.zoff.begin(), L.zoff.end());
+        P.zgeo.insert(P.zgeo.end(), L.zlen.begin(), L.zlen.end());
+        if (LP.ksplit == 1 && ns > 0) {  // the projection writes live columns only
+            add_fill(zoffq[size_t(l)], a8(nsl * size_t(L.ldz)));
+            if (l < H) add_fill(xoff[size_t(l)], a8(nsl * size_t(L.d.ne)));
+        }
         // (query, W) groups -> meet tiles per target child q2
-        const unsigned long long divW = div * unsigned(g * g);
+        const unsigned long long divW = div * unsigned(g2);
         for (int s = 0; s < ns;) {
             int t = s + 1;
             while (t < ns && sk[t].key / divW == sk[s].key / divW) ++t;
@@ -818,20 +1031,20 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
             for (auto& kv : byq) {
                 const int n = int(kv.second.size());
                 if (n < 2) continue;
-                const int gi = int(LP.groups.size());
-                const int goff = int(LP.glist.size());
-                LP.groups.push_back(make_int4(kv.first, 0, goff, n));
-                LP.glist.insert(LP.glist.end(), kv.second.begin(), kv.second.end());
+                const int gi = int(P.groups.size());
+                const int goff = int(P.glist.size());
+                P.groups.push_back(make_int4(kv.first, 0, goff, n));
+                P.glist.insert(P.glist.end(), kv.second.begin(), kv.second.end());
                 for (int j0 = 0; j0 < n;) {
-                    const int q2 = int((sk[kv.second[j0]].key / div) % unsigned(g * g));
+                    const int q2 = int((sk[kv.second[j0]].key / div) % unsigned(g2));
                     int je = j0 + 1;
-                    while (je < n && int((sk[kv.second[je]].key / div) % unsigned(g * g)) == q2) ++je;
-                    for (int i0 = 0; i0 < n; i0 += 16)
-                        for (int jj = j0; jj < je; jj += 16) {
-                            const int tt[8] = {gi, q2, i0, jj, je, 0, 0, 0};
-                            LP.mtiles.insert(LP.mtiles.end(), tt, tt + 8);
-                            LP.m_updates += double(std::min(16, n - i0)) * std::min(16, je - jj) * L.zlen[q2];
-                            LP.m_bytes += double(std::min(16, n - i0) + std::min(16, je - jj)) * L.zlen[q2] * 2;
+                    while (je < n && int((sk[kv.second[je]].key / div) % unsigned(g2)) == q2) ++je;
+                    for (int i0 = 0; i0 < n; i0 += 8)
+                        for (int jj = j0; jj < je; jj += 8) {
+                            const int tt[8] = {gi, q2, i0, jj, je, l - 1, 0, 0};
+                            P.mtiles.insert(P.mtiles.end(), tt, tt + 8);
+                            P.m_updates += double(std::min(8, n - i0)) * std::min(8, je - jj) * L.zlen[q2];
+                            P.m_bytes += double(std::min(8, n - i0) + std::min(8, je - jj)) * L.zlen[q2] * 2;
                         }
                     j0 = je;
                 }
@@ -843,31 +1056,22 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
     // ---- upload (one packed staging buffer)
     Packer pk;
     const size_t o_qk = pk.add(P.q_k), o_qout = pk.add(P.q_out), o_qvp = pk.add(P.q_vp), o_vprc = pk.add(P.vp_rc),
-                 o_vpvalid = pk.add(P.vp_valid), o_src = pk.add(P.slot_rc),
-                 o_si = pk.add(P.slot_i);
-    std::vector<size_t> o_items(H), o_groups(H), o_glist(H), o_tiles(H);
+                 o_vpvalid = pk.add(P.vp_valid), o_src = pk.add(P.slot_rc), o_si = pk.add(P.slot_i),
+                 o_tiles = pk.add(P.mtiles), o_groups = pk.add(P.groups), o_glist = pk.add(P.glist),
+                 o_zgeo = pk.add(P.zgeo), o_fills = pk.add(P.fills);
+    std::vector<size_t> o_items(H);
     size_t maxpart = 0;
     for (int l = 0; l < H; ++l) {
         o_items[l] = pk.add(P.levels[l].items);
-        o_groups[l] = pk.add(P.levels[l].groups);
-        o_glist[l] = pk.add(P.levels[l].glist);
-        o_tiles[l] = pk.add(P.levels[l].mtiles);
         maxpart = std::max(maxpart, size_t(P.levels[l].ksplit) * lv[l].ldp);
     }
     if ((rc = ws.h_plan.ensure(pk.size))) return rc;
     pk.write(static_cast<char*>(ws.h_plan.p));
     if ((rc = ws.d_plan.ensure(pk.size))) return rc;
-    size_t maxpc = size_t(4 * b - 4), maxldz = 8;
-    for (int l = 1; l <= H; ++l) {
-        maxpc = std::max(maxpc, size_t(lv[l - 1].d.ne));
-        maxldz = std::max(maxldz, size_t(lv[l - 1].ldz));
-    }
-    const size_t nsl = size_t(std::max(ns, 1));
-    if ((rc = ws.X[0].ensure(nsl * maxpc * 2))) return rc;
-    if ((rc = ws.X[1].ensure(nsl * maxpc * 2))) return rc;
-    if ((rc = ws.Z.ensure(nsl * maxldz * 2))) return rc;
+    if ((rc = ws.Q.ensure(std::max<size_t>(qsz, 8) * 2))) return rc;
     if ((rc = ws.Zp.ensure(nsl * std::max<size_t>(maxpart, 8) * 2))) return rc;
-    if ((rc = ws.d_out.ensure(size_t(total_out) * 2))) return rc;
+    // the meet's 16-bit atomics address whole 32-bit words
+    if ((rc = ws.d_out.ensure(size_t(total_out) * 2 + 4))) return rc;
     if ((rc = ws.h_out.ensure(size_t(total_out) * 2))) return rc;
     hipStream_t st = ws.stream;
     if ((e = hipMemcpyAsync(ws.d_plan.p, ws.h_plan.p, pk.size, hipMemcpyHostToDevice, st)) != hipSuccess)
@@ -876,39 +1080,53 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
     auto I = [&](size_t o) { return reinterpret_cast<const int*>(dp + o); };
     uint16_t* dout = static_cast<uint16_t*>(ws.d_out.p);
     const uint16_t* dleafD = static_cast<const uint16_t*>(leafD.p);
-    launch_q_init(nq, I(o_qk), reinterpret_cast<const long long*>(dp + o_qout), I(o_qvp),
-                  reinterpret_cast<const int2*>(dp + o_vprc), reinterpret_cast<const uint8_t*>(dp + o_vpvalid), b, nl,
-                  dleafD, dout, kmax, st);
+    ws.counted = prof_ref().on;
+    unsigned long long* wstat = nullptr;
+    if (ws.counted) {
+        if ((rc = ws.d_wstat.ensure(16)) || (rc = ws.h_wstat.ensure(16))) return rc;
+        wstat = static_cast<unsigned long long*>(ws.d_wstat.p);
+    }
     const int2* srcs = reinterpret_cast<const int2*>(dp + o_src);
-    uint16_t* Xa = static_cast<uint16_t*>(ws.X[0].p);
-    uint16_t* Xb = static_cast<uint16_t*>(ws.X[1].p);
-    uint16_t* Z = static_cast<uint16_t*>(ws.Z.p);
-    if (H > 0 && ns > 0) launch_q_leaf(ns, srcs, b, nl, dleafD, Xa, st);
+    uint16_t* Qb = static_cast<uint16_t*>(ws.Q.p);
+    launch_q_front(nq, I(o_qk), reinterpret_cast<const long long*>(dp + o_qout), I(o_qvp),
+                   reinterpret_cast<const int2*>(dp + o_vprc), reinterpret_cast<const uint8_t*>(dp + o_vpvalid), kmax,
+                   H > 0 ? ns : 0, srcs, b, nl, dleafD, dout, Qb + xoff[0], int(P.fills.size()),
+                   reinterpret_cast<const int4*>(dp + o_fills), fill_blocks, Qb, wstat, st);
     for (int l = 1; l <= H && ns > 0; ++l) {
         const LevelGeo& L = lv[l - 1];
         const LevelPlan& LP = P.levels[l - 1];
-        int pi = prof_ref().begin(HTG_K_Q_PROJECT, st, LP.z_bytes, LP.z_updates);
+        const uint16_t* Xl = Qb + xoff[size_t(l) - 1];            // X of this level (stride pc)
+        uint16_t* Xn = l < H ? Qb + xoff[size_t(l)] : nullptr;  // lifted X (stride ne)
+        uint16_t* Zl = Qb + zoffq[size_t(l)];
+        const int ne = l < H ? L.d.ne : 0;  // no lift above the root
+        // bytes and updates of the projection are counted on the device
+        const int pi = prof_ref().begin(HTG_K_Q_PROJECT, st, 0.0, 0.0);
         const size_t part_stride = nsl * L.ldp;
         launch_q_z(LP.M, L.d.ld, L.d.pc, static_cast<const uint16_t*>(L.DU.p), L.node_stride,
                    static_cast<const int*>(L.d_seam_list.p), static_cast<const int*>(L.d_seam_off.p),
-                   static_cast<const int*>(L.d_zcols.p), L.ncols, L.ldp,
-                   reinterpret_cast<const int4*>(dp + o_items[l - 1]), int(LP.items.size()), LP.ksplit, Xa,
-                   static_cast<uint16_t*>(ws.Zp.p), part_stride, st);
-        launch_q_zred(ns, L.nz, L.ldz, static_cast<const int*>(L.d_zsrc.p), L.ldp, LP.ksplit,
-                      static_cast<const uint16_t*>(ws.Zp.p), part_stride, Z, st);
+                   static_cast<const int*>(L.d_zcols.p), L.ngrp, L.d_live_cnt,
+                   L.d_live_list, LP.col_tiles, L.ldp,
+                   reinterpret_cast<const int4*>(dp + o_items[l - 1]), int(LP.items.size()), LP.ksplit, Xl,
+                   static_cast<uint16_t*>(ws.Zp.p), part_stride, static_cast<const int*>(L.d_colpos.p),
+                   static_cast<const int*>(L.d_colext.p), static_cast<const int*>(L.d_ext_child.p),
+                   static_cast<const int*>(L.d_ext_cport.p), Zl, L.ldz, Xn, ne, wstat, st);
         prof_ref().end(pi, st);
-        pi = prof_ref().begin(HTG_K_Q_MEET, st, LP.m_bytes, LP.m_updates);
-        launch_q_meet(I(o_tiles[l - 1]), int(LP.mtiles.size() / 8), reinterpret_cast<const int4*>(dp + o_groups[l - 1]),
-                      I(o_glist[l - 1]), static_cast<const int*>(L.d_zoff.p), static_cast<const int*>(L.d_zlen.p),
-                      L.ldz, I(o_si), I(o_qk), reinterpret_cast<const long long*>(dp + o_qout), Z, dout, st);
-        prof_ref().end(pi, st);
-        if (l < H) {
-            pi = prof_ref().begin(HTG_K_Q_LIFT, st, double(ns) * (2.0 * L.d.ne + L.d.pc) * 2, 0);
-            launch_q_lift(ns, L.d.ne, static_cast<const int*>(L.d_ext_zpos.p), static_cast<const int*>(L.d_ext_child.p),
-                          static_cast<const int*>(L.d_ext_cport.p), L.d.pc, L.d.cs, g, L.ldz, srcs, Xa, Z, Xb, st);
-            prof_ref().end(pi, st);
-            std::swap(Xa, Xb);
+        if (LP.ksplit > 1) {
+            const int pf = prof_ref().begin(HTG_K_Q_LIFT, st, double(ns) * (L.nz + 2.0 * ne + L.d.pc) * 2, 0);
+            launch_q_fold(ns, L.nz, L.ldz, static_cast<const int*>(L.d_zsrc.p), L.ldp, LP.ksplit,
+                          static_cast<const uint16_t*>(ws.Zp.p), part_stride, srcs, L.d.s, L.d.nt, L.ngrp,
+                          L.d_live_map, Zl, ne, static_cast<const int*>(L.d_ext_col.p),
+                          static_cast<const int*>(L.d_ext_child.p), static_cast<const int*>(L.d_ext_cport.p), L.d.pc,
+                          L.d.cs, g, Xl, Xn, st);
+            prof_ref().end(pf, st);
         }
+    }
+    if (ns > 0 && !P.mtiles.empty()) {
+        const int pm = prof_ref().begin(HTG_K_Q_MEET, st, P.m_bytes, P.m_updates);
+        launch_q_meet(I(o_tiles), int(P.mtiles.size() / 8), reinterpret_cast<const int4*>(dp + o_groups), I(o_glist),
+                      I(o_zgeo), g2, I(o_si), I(o_qk), reinterpret_cast<const long long*>(dp + o_qout), Qb, dout,
+                      st);
+        prof_ref().end(pm, st);
     }
     if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "query launch");
     if (to_host) {
@@ -916,21 +1134,28 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
             hipSuccess)
             return hip_fail(e, "download result");
     }
+    if (wstat && (e = hipMemcpyAsync(ws.h_wstat.p, wstat, 16, hipMemcpyDeviceToHost, st)) != hipSuccess)
+        return hip_fail(e, "download counters");
     if ((e = hipEventRecord(ws.done, st)) != hipSuccess) return hip_fail(e, "record");
     return HTG_OK;
 }
 
-int QueryWS::init() {
+int QueryWS::init(hipStream_t ext) {
     if (stream) return HTG_OK;
-    hipError_t e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
-    if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
+    hipError_t e;
+    if (ext) {
+        stream = ext;
+        own_stream = false;
+    } else if ((e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)) != hipSuccess) {
+        return hip_fail(e, "hipStreamCreate");
+    }
     if ((e = hipEventCreateWithFlags(&done, hipEventDisableTiming)) != hipSuccess) return hip_fail(e, "hipEventCreate");
     return HTG_OK;
 }
 QueryWS::~QueryWS() {
     if (stream) (void)hipStreamSynchronize(stream);
     if (done) (void)hipEventDestroy(done);
-    if (stream) (void)hipStreamDestroy(stream);
+    if (stream && own_stream) (void)hipStreamDestroy(stream);
 }
 
 }  // namespace htg
@@ -994,8 +1219,11 @@ int htg_map_footprint(const htg_config* cfg, int32_t rows, int32_t cols, uint64_
     return HTG_OK;
 }
 
-int htg_map_create(htg_ctx* ctx, const htg_config* cfg, int32_t rows, int32_t cols, const uint8_t* cells,
-                   uint64_t hbm_budget, htg_map** out) {
+}  // extern "C"
+
+namespace htg {
+int map_create_on(htg_ctx* ctx, const htg_config* cfg, int32_t rows, int32_t cols, const uint8_t* cells,
+                  uint64_t hbm_budget, hipStream_t shared, htg_map** out) {
     HTG_GUARD_BEGIN
     if (!ctx || !cfg || !cells || !out) return fail(HTG_E_INVALID, "null argument");
     // include/htrip/types.hpp:221-224 plus this build's limits: 2 <= b <= 64
@@ -1028,14 +1256,20 @@ int htg_map_create(htg_ctx* ctx, const htg_config* cfg, int32_t rows, int32_t co
     m->pass.assign(size_t(m->S) * m->S, 0);
     for (int r = 0; r < rows; ++r)
         for (int c = 0; c < cols; ++c) m->pass[size_t(r) * m->S + c] = cells[size_t(r) * cols + c] == 1;
-    if ((e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking)) != hipSuccess)
+    if (shared) {
+        m->stream = shared;
+        m->own_stream = false;
+    } else if ((e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking)) != hipSuccess) {
         return hip_fail(e, "hipStreamCreate");
-    if ((rc = m->qws.init())) return rc;
+    }
+    hipStream_t st = m->stream;
+    if ((rc = m->qws.init(st))) return rc;  // plain queries run in order with updates
     const size_t n = size_t(b) * b, pl = size_t(4 * b - 4), nleaves = size_t(m->nl) * m->nl;
     if ((rc = m->d_pass.ensure(m->pass.size()))) return rc;
     if ((rc = m->leafD.ensure(nleaves * n * n * 2))) return rc;
     if ((rc = m->leafS.ensure(nleaves * pl * pl * 2))) return rc;
-    if ((e = hipMemcpy(m->d_pass.p, m->pass.data(), m->pass.size(), hipMemcpyHostToDevice)) != hipSuccess)
+    if ((e = hipMemcpyAsync(m->d_pass.p, m->pass.data(), m->pass.size(), hipMemcpyHostToDevice, st)) != hipSuccess ||
+        (e = hipStreamSynchronize(st)) != hipSuccess)
         return hip_fail(e, "upload grid");
     m->lv.resize(m->H);
     for (int l = 1; l <= m->H; ++l) {
@@ -1043,35 +1277,29 @@ int htg_map_create(htg_ctx* ctx, const htg_config* cfg, int32_t rows, int32_t co
         make_level(L, l, b, g, m->S, m->H);
         if ((rc = L.DU.ensure(size_t(L.nn) * L.node_stride * 2))) return rc;
         if ((rc = L.SU.ensure(size_t(L.nn) * L.d.ne * L.d.ne * 2))) return rc;
-        if ((rc = upload(L.d_vu_rc, L.vu_rc)) || (rc = upload(L.d_ext_idx, L.ext_idx)) ||
-            (rc = upload(L.d_ext_child, L.ext_child)) || (rc = upload(L.d_ext_cport, L.ext_cport)) ||
-            (rc = upload(L.d_seams, L.seams)) || (rc = upload(L.d_seam_list, L.seam_list)) ||
-            (rc = upload(L.d_seam_off, L.seam_off)) || (rc = upload(L.d_zcols, L.zcols)) ||
-            (rc = upload(L.d_zsrc, L.zsrc)) || (rc = upload(L.d_zoff, L.zoff)) || (rc = upload(L.d_zlen, L.zlen)) ||
-            (rc = upload(L.d_ext_zpos, L.ext_zpos)))
+        if ((rc = upload(L.d_vu_rc, L.vu_rc, st)) || (rc = upload(L.d_ext_idx, L.ext_idx, st)) ||
+            (rc = upload(L.d_ext_child, L.ext_child, st)) || (rc = upload(L.d_ext_cport, L.ext_cport, st)) ||
+            (rc = upload(L.d_seams, L.seams, st)) || (rc = upload(L.d_seam_list, L.seam_list, st)) ||
+            (rc = upload(L.d_seam_off, L.seam_off, st)) || (rc = upload(L.d_zcols, L.zcols, st)) ||
+            (rc = upload(L.d_zsrc, L.zsrc, st)) || (rc = upload(L.d_zoff, L.zoff, st)) || (rc = upload(L.d_zlen, L.zlen, st)) ||
+            (rc = upload(L.d_ext_zpos, L.ext_zpos, st)) || (rc = upload(L.d_ext_col, L.ext_col, st)) ||
+            (rc = upload(L.d_colpos, L.colpos, st)) || (rc = upload(L.d_colext, L.colext, st)) ||
+            (rc = upload(L.d_vu_ext, L.vu_ext, st)))
             return rc;
     }
+    if ((rc = m->init_live()) || (rc = m->refresh_live(nullptr))) return rc;
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(e, "upload live groups");
     m->device_bytes = tables_bytes(b, g, m->S, m->H);
     *out = m.release();
     return HTG_OK;
     HTG_GUARD_END
 }
 
-int htg_map_destroy(htg_map* map) {
-    if (!map) return HTG_OK;
-    (void)hipSetDevice(map->ctx->device);
-    (void)hipStreamSynchronize(map->stream);
-    (void)hipStreamSynchronize(map->qws.stream);
-    if (map->stream) (void)hipStreamDestroy(map->stream);
-    delete map;
-    return HTG_OK;
-}
-
-int htg_map_clone(const htg_map* src, uint64_t hbm_budget, htg_map** out) {
+int map_clone_on(const htg_map* src, uint64_t hbm_budget, hipStream_t shared, htg_map** out) {
     HTG_GUARD_BEGIN
     if (!src || !out) return fail(HTG_E_INVALID, "null argument");
     htg_map* m = nullptr;
-    int rc = htg_map_create(src->ctx, &src->cfg, src->rows, src->cols, src->cells_code.data(), hbm_budget, &m);
+    int rc = map_create_on(src->ctx, &src->cfg, src->rows, src->cols, src->cells_code.data(), hbm_budget, shared, &m);
     if (rc) return rc;
     std::unique_ptr<htg_map> hold(m);
     m->pass = src->pass;
@@ -1080,22 +1308,48 @@ int htg_map_clone(const htg_map* src, uint64_t hbm_budget, htg_map** out) {
     m->stats = src->stats;
     m->epoch = src->epoch;
     hipError_t e;
+    const hipStream_t st = m->stream;
     (void)hipStreamSynchronize(src->stream);
-    if ((e = hipMemcpy(m->d_pass.p, src->d_pass.p, src->d_pass.bytes < m->d_pass.bytes ? src->d_pass.bytes : m->d_pass.bytes,
-                       hipMemcpyDeviceToDevice)) != hipSuccess ||
-        (e = hipMemcpy(m->leafD.p, src->leafD.p, m->leafD.bytes, hipMemcpyDeviceToDevice)) != hipSuccess ||
-        (e = hipMemcpy(m->leafS.p, src->leafS.p, m->leafS.bytes, hipMemcpyDeviceToDevice)) != hipSuccess)
+    const auto d2d = hipMemcpyDeviceToDevice;
+    if ((e = hipMemcpyAsync(m->d_pass.p, src->d_pass.p,
+                            src->d_pass.bytes < m->d_pass.bytes ? src->d_pass.bytes : m->d_pass.bytes, d2d, st)) !=
+            hipSuccess ||
+        (e = hipMemcpyAsync(m->leafD.p, src->leafD.p, m->leafD.bytes, d2d, st)) != hipSuccess ||
+        (e = hipMemcpyAsync(m->leafS.p, src->leafS.p, m->leafS.bytes, d2d, st)) != hipSuccess)
         return hip_fail(e, "clone copy");
     for (int l = 0; l < m->H; ++l) {
-        if ((e = hipMemcpy(m->lv[l].DU.p, src->lv[l].DU.p, size_t(m->lv[l].nn) * m->lv[l].node_stride * 2,
-                           hipMemcpyDeviceToDevice)) != hipSuccess ||
-            (e = hipMemcpy(m->lv[l].SU.p, src->lv[l].SU.p, size_t(m->lv[l].nn) * m->lv[l].d.ne * m->lv[l].d.ne * 2,
-                           hipMemcpyDeviceToDevice)) != hipSuccess)
+        if ((e = hipMemcpyAsync(m->lv[l].DU.p, src->lv[l].DU.p, size_t(m->lv[l].nn) * m->lv[l].node_stride * 2, d2d,
+                                st)) != hipSuccess ||
+            (e = hipMemcpyAsync(m->lv[l].SU.p, src->lv[l].SU.p,
+                                size_t(m->lv[l].nn) * m->lv[l].d.ne * m->lv[l].d.ne * 2, d2d, st)) != hipSuccess)
             return hip_fail(e, "clone copy");
     }
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(e, "clone copy");
     *out = hold.release();
     return HTG_OK;
     HTG_GUARD_END
+}
+}  // namespace htg
+
+extern "C" {
+
+int htg_map_create(htg_ctx* ctx, const htg_config* cfg, int32_t rows, int32_t cols, const uint8_t* cells,
+                   uint64_t hbm_budget, htg_map** out) {
+    return htg::map_create_on(ctx, cfg, rows, cols, cells, hbm_budget, nullptr, out);
+}
+
+int htg_map_destroy(htg_map* map) {
+    if (!map) return HTG_OK;
+    (void)hipSetDevice(map->ctx->device);
+    (void)hipStreamSynchronize(map->stream);
+    (void)hipStreamSynchronize(map->qws.stream);
+    if (map->stream && map->own_stream) (void)hipStreamDestroy(map->stream);
+    delete map;
+    return HTG_OK;
+}
+
+int htg_map_clone(const htg_map* src, uint64_t hbm_budget, htg_map** out) {
+    return htg::map_clone_on(src, hbm_budget, nullptr, out);
 }
 
 int htg_map_info_get(const htg_map* map, htg_map_info* out) {
@@ -1187,7 +1441,9 @@ static int read_rows(htg_map* map, const void* src, size_t rows, size_t cols, si
     hipError_t e = hipSetDevice(map->ctx->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     if ((e = hipStreamSynchronize(map->stream)) != hipSuccess) return hip_fail(e, "sync");
-    if ((e = hipMemcpy2D(out, cols * 2, src, ld * 2, cols * 2, rows, hipMemcpyDeviceToHost)) != hipSuccess)
+    if ((e = hipMemcpy2DAsync(out, cols * 2, src, ld * 2, cols * 2, rows, hipMemcpyDeviceToHost, map->stream)) !=
+            hipSuccess ||
+        (e = hipStreamSynchronize(map->stream)) != hipSuccess)
         return hip_fail(e, "read table");
     return HTG_OK;
 }

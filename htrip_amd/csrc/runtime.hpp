@@ -83,21 +83,35 @@ struct LevelGeo {
     // Query projection: natural columns (V_U indices) the projection computes,
     // and the Z layout: per-child seam blocks (even-padded) then the other
     // exterior ports.  zsrc[pos] = natural column of Z position pos or -1.
-    std::vector<int> zcols, zsrc, zoff, zlen, ext_zpos;
+    std::vector<int> zcols, zsrc, zoff, zlen, ext_zpos, ext_col;  // ext_col[e]: projection column of port e
+    // per projection column: Z position (or -1) and exterior port index below
+    // the root (or -1), for projections that fold in registers
+    std::vector<int> colpos, colext;
+    std::vector<int> vu_ext;  // [v]: exterior port index of V_U port v, or -1
     int ncols = 0, ldp = 0, nz = 0, ldz = 0, nsmax = 0;
     bool gather = false;
+    // Live column groups: group gi = projection columns 4gi..4gi+3.  A group
+    // is live in node W when one of its ports is passable; the others give
+    // INF and the projection skips them.  live_list[W*ngrp ...] holds the
+    // first live_cnt[W] live groups of W, live_map[W*ngrp + gi] the flags.
+    // Live octets for the closure: octet o = V_U ports 8o..8o+7, live when
+    // one of them is passable (only live rows and columns of D_U can change).
+    // These arrays live in the map's pinned live block (h_*) and its device
+    // copy (d_*), uploaded with one copy after each change.
+    int ngrp = 0, ng8 = 0;
+    int *h_live_cnt = nullptr, *h_live_list = nullptr, *h_live8_cnt = nullptr, *h_live8_list = nullptr;
+    uint8_t* h_live_map = nullptr;
+    const int *d_live_cnt = nullptr, *d_live_list = nullptr, *d_live8_cnt = nullptr, *d_live8_list = nullptr;
+    const uint8_t* d_live_map = nullptr;
+    std::vector<int> seam_live;  // [W * g^2 + q]: passable seam ports of child q (rows a projection may use)
     DevBuf d_vu_rc, d_ext_idx, d_ext_child, d_ext_cport, d_seams, d_seam_list, d_seam_off, d_zcols, d_zsrc, d_zoff,
-        d_zlen, d_ext_zpos;
+        d_zlen, d_ext_zpos, d_ext_col, d_colpos, d_colext, d_vu_ext;
     DevBuf DU, SU;
 };
 
 struct LevelPlan {
-    int M = 1, ksplit = 1;
-    std::vector<int> mtiles;   // meet tiles, 8 ints each
-    double z_bytes = 0, z_updates = 0, m_bytes = 0, m_updates = 0;
-    std::vector<int4> items;   // (W, q, first slot, count)
-    std::vector<int4> groups;  // (query, 0, glist offset, count)
-    std::vector<int> glist;    // slot indices
+    int M = 1, ksplit = 1, col_tiles = 0;
+    std::vector<int4> items;  // (W, q | first-of-run << 16, first slot, count)
 };
 
 struct QueryPlan {
@@ -108,6 +122,13 @@ struct QueryPlan {
     std::vector<int2> slot_rc;
     std::vector<int> slot_q, slot_i;
     std::vector<LevelPlan> levels;
+    // meet, every level in one launch
+    std::vector<int> mtiles;   // 8 ints each: group, q2, i0, j0, j_end, level
+    std::vector<int4> groups;  // (query, 0, glist offset, count)
+    std::vector<int> glist;    // slot indices
+    std::vector<int> zgeo;     // per level: Z offset lo, hi, ldz, 0, zoff[g2], zlen[g2]
+    std::vector<int4> fills;   // INF fills of Q: (start lo, start hi, n8, first block)
+    double m_bytes = 0, m_updates = 0;
     void clear() {
         q_k.clear();
         q_vp.clear();
@@ -118,12 +139,15 @@ struct QueryPlan {
         slot_q.clear();
         slot_i.clear();
         for (auto& l : levels) {
-            l.z_bytes = l.z_updates = l.m_bytes = l.m_updates = 0;
+            l.col_tiles = 0;
             l.items.clear();
-            l.mtiles.clear();
-            l.groups.clear();
-            l.glist.clear();
         }
+        mtiles.clear();
+        groups.clear();
+        glist.clear();
+        zgeo.clear();
+        fills.clear();
+        m_bytes = m_updates = 0;
     }
 };
 
@@ -131,12 +155,16 @@ struct QueryPlan {
 // include/htrip/minplus_oracle.hpp:189-238).
 struct QueryWS {
     hipStream_t stream = nullptr;
+    bool own_stream = true;
     hipEvent_t done = nullptr;  // recorded after the batch's last operation
     long long total_out = 0;
     QueryPlan plan;
-    HostBuf h_plan, h_out;
-    DevBuf d_plan, X[2], Z, Zp, d_out;
-    int init();
+    HostBuf h_plan, h_out, h_wstat;
+    // Q: X of levels 0..H-1 then Z of levels 1..H, each region 8-aligned
+    DevBuf d_plan, Q, Zp, d_out, d_wstat;  // d_wstat: k_q_z work counters (bytes, updates)
+    bool counted = false;                        // this batch's counters go to the profiler
+    // Runs on `ext` when given (not owned), else on a stream of its own.
+    int init(hipStream_t ext = nullptr);
     ~QueryWS();
 };
 
@@ -161,6 +189,8 @@ struct Profiler {
     void end(int idx, hipStream_t st);
     // Accumulate every completed pair; pairs still running stay pending.
     void harvest();
+    // Work counted on the device (bytes, updates), added without a launch.
+    void add_work(int cls, double bytes, double updates);
     void reset();
     ~Profiler();
 };
@@ -185,7 +215,11 @@ struct Map {
     std::vector<uint8_t> pass;        // S*S passable flags (host mirror of d_pass)
     DevBuf d_pass, leafD, leafS;
     std::vector<LevelGeo> lv;  // lv[l-1] is level l
+    // Build/update stream; also the plain query stream.  A server's maps
+    // share the server's update stream (own_stream false), so that a process
+    // needs few hardware queues.
     hipStream_t stream = nullptr;
+    bool own_stream = true;
     bool built = false;
     uint64_t epoch = 0;
     htg_batch_stats stats{0, 0, 0};
@@ -195,8 +229,15 @@ struct Map {
     std::shared_ptr<Profiler> profp = std::make_shared<Profiler>();
     Profiler& prof_ref() { return *profp; }
     DirtySet dirty;  // filled by open_cells
+    HostBuf live_h;  // live arrays of every level (pinned)
+    DevBuf live_d;
+    hipEvent_t live_evt = nullptr;  // recorded after the last upload of live_h
+    bool live_pending = false;
+    ~Map();
     DevBuf ws_nodes, ws_flags, ws_piv, ws_counts, ws_R, ws_changed, ws_mask, ws_edges, ws_cells, ws_leaves,
         ws_leaf_changed;
+    DevBuf ws_upd, ws_chg;  // update: packed per-level lists; changed flags (leaves, then every level)
+    HostBuf ws_upd_h, ws_chg_h;
 
     int build();
     int open_cells(const int32_t* rc, int n);
@@ -206,7 +247,14 @@ struct Map {
     int query_launch(const int32_t* rc, const int32_t* ks, int nq, QueryWS& ws, bool to_host);
     int query_finish(QueryWS& ws, uint16_t* out_host);
     int ensure_build_ws(int lvl);
-    int run_closure(int lvl, int n_list, const std::vector<int>& hcounts);
+    // Lay out the live block of every level (once, at creation).
+    int init_live();
+    // Recompute the live column groups and octets of the nodes that contain
+    // one of `cells` (every node when cells is null) and upload the changed
+    // byte range with one asynchronous copy on the map stream.
+    int refresh_live(const std::vector<int2>* cells);
+    // nodes: the listed nodes (host copy of ws_nodes)
+    int run_closure(int lvl, const int* dnodes, const std::vector<int>& nodes, const std::vector<int>& hcounts);
     const uint16_t* child_summaries(int lvl, size_t& stride) const;
     // Make this map equal to src after src applied one update with dirty set
     // ds (this map equal to src before it): copies only the rewritten tables.
@@ -220,3 +268,10 @@ struct htg_map : htg::Map {};
 struct htg_ctx {
     int device = 0;
 };
+
+namespace htg {
+// htg_map_create / htg_map_clone on a given stream (nullptr: a new one).
+int map_create_on(htg_ctx* ctx, const htg_config* cfg, int32_t rows, int32_t cols, const uint8_t* cells,
+                  uint64_t hbm_budget, hipStream_t shared, htg_map** out);
+int map_clone_on(const htg_map* src, uint64_t hbm_budget, hipStream_t shared, htg_map** out);
+}  // namespace htg

@@ -77,6 +77,23 @@ struct Session {
     DirtySet last_dirty;
 };
 
+// Update stream, restricted to about pct % of the CUs when 0 < pct < 100.
+// The mask takes whole blocks of 8 consecutive CU bits, spread evenly over
+// the index range, so every XCD gets its share whether the CU numbering
+// interleaves the XCDs or not.
+hipError_t make_update_stream(int device, int pct, hipStream_t* out) {
+    int ncu = 0;
+    hipError_t e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device);
+    if (e != hipSuccess) return e;
+    if (pct <= 0 || pct >= 100 || ncu < 16) return hipStreamCreateWithFlags(out, hipStreamNonBlocking);
+    std::vector<uint32_t> mask(size_t(ncu + 31) / 32, 0u);
+    const int blocks = ncu / 8;
+    for (int t = 0; t < blocks; ++t)
+        if ((t + 1) * pct / 100 > t * pct / 100)
+            for (int i = t * 8; i < t * 8 + 8; ++i) mask[size_t(i) / 32] |= 1u << (i % 32);
+    return hipExtStreamCreateWithCUMask(out, uint32_t(mask.size()), mask.data());
+}
+
 void mark_done(Request* r, int st, Clock::time_point now) {
     r->status = st;
     r->t_done = now;
@@ -94,6 +111,7 @@ struct htg_server {
     int next_session = 1;
     uint64_t used = 0;
     std::vector<std::unique_ptr<Batch>> pool;
+    hipStream_t ustream = nullptr;  // builds and updates of every session
     std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
     uint64_t next_ticket = 1;
     std::thread worker, updater;
@@ -303,6 +321,7 @@ int htg_server_create(htg_ctx* ctx, const htg_server_config* cfg, htg_server** o
     try {
         if (!ctx || !cfg || !out) return fail(HTG_E_INVALID, "null argument");
         if (cfg->n_slots < 1 || cfg->n_slots > 64 || cfg->max_batch_queries < 1 || cfg->max_batch_viewpoints < 1 ||
+            cfg->update_cu_percent < 0 || cfg->update_cu_percent > 100 ||
             (cfg->consistency != HTG_CONSISTENCY_STRICT && cfg->consistency != HTG_CONSISTENCY_SNAPSHOT))
             return fail(HTG_E_INVALID, "bad server configuration");
         hipError_t e = hipSetDevice(ctx->device);
@@ -310,6 +329,13 @@ int htg_server_create(htg_ctx* ctx, const htg_server_config* cfg, htg_server** o
         std::unique_ptr<htg_server> s(new htg_server());
         s->ctx = ctx;
         s->cfg = *cfg;
+        // One update stream and n_slots query streams: a process has few
+        // hardware queues (GPU_MAX_HW_QUEUES, 4 by default), and streams
+        // beyond that share queues, so that a query would wait behind update
+        // kernels.  With update_cu_percent the update stream runs on that
+        // share of the CUs only, and queries always find free CUs.
+        if ((e = make_update_stream(ctx->device, cfg->update_cu_percent, &s->ustream)) != hipSuccess)
+            return hip_fail(e, "update stream");
         for (int i = 0; i < cfg->n_slots; ++i) {
             s->pool.emplace_back(new Batch());
             int rc = s->pool.back()->ws.init();
@@ -338,6 +364,8 @@ int htg_server_destroy(htg_server* s) {
     for (auto& kv : s->sessions)
         for (htg_map* m : kv.second.maps)
             if (m) htg_map_destroy(m);
+    s->pool.clear();
+    if (s->ustream) (void)hipStreamDestroy(s->ustream);
     delete s;
     return HTG_OK;
 }
@@ -358,10 +386,10 @@ int htg_server_add_map(htg_server* s, const htg_config* cfg, int32_t rows, int32
         s->used += need;  // reserve before building
     }
     htg_map* m[2] = {nullptr, nullptr};
-    rc = htg_map_create(s->ctx, cfg, rows, cols, cells, 0, &m[0]);
+    rc = map_create_on(s->ctx, cfg, rows, cols, cells, 0, s->ustream, &m[0]);
     if (rc == HTG_OK) rc = htg_build(m[0]);
     if (rc == HTG_OK && copies == 2) {
-        rc = htg_map_clone(m[0], 0, &m[1]);
+        rc = map_clone_on(m[0], 0, s->ustream, &m[1]);
         if (rc == HTG_OK) m[1]->profp = m[0]->profp;  // one profiler per session
     }
     std::lock_guard<std::mutex> lk(s->mu);

@@ -170,6 +170,8 @@ typedef struct htg_server_config {
     int32_t max_batch_queries;    /* queries coalesced per batch */
     int32_t max_batch_viewpoints; /* viewpoints per batch (a larger query still runs alone) */
     int32_t consistency;          /* HTG_CONSISTENCY_* */
+    int32_t update_cu_percent;    /* builds/updates run on this share of the CUs (1..99); 0 or 100 = all */
+    int32_t reserved;
 } htg_server_config;
 typedef struct htg_server_stats {
     uint64_t query_batches, queries, update_batches, updates, hbm_used;
