@@ -30,7 +30,7 @@ EXPORTS = [
 K_LEAF_APSP, K_CLOSURE_PIVOTS, K_CLOSURE_UPDATE, K_Q_PROJECT, K_Q_MEET, K_Q_LIFT = range(6)
 KERNEL_NAMES = {
     K_LEAF_APSP: "k_leaf_apsp", K_CLOSURE_PIVOTS: "k_closure_pivots", K_CLOSURE_UPDATE: "k_closure_update",
-    K_Q_PROJECT: "k_q_z", K_Q_MEET: "k_q_meet", K_Q_LIFT: "k_q_lift",
+    K_Q_PROJECT: "k_q_z", K_Q_MEET: "k_q_meet", K_Q_LIFT: "k_q_fold",
 }
 
 

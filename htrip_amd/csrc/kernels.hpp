@@ -8,12 +8,20 @@
 
 namespace htg {
 
+// Start/stop events attached to one kernel dispatch (hipExtLaunchKernelGGL),
+// so the measured interval is the kernel's own execution.  Null events: no
+// timing.  Launchers that take one return whether they launched.
+struct KTimer {
+    hipEvent_t a = nullptr, b = nullptr;
+};
+
+
 // ---- build / update ------------------------------------------------------
 // Tile-restricted BFS from every cell of each listed leaf; writes the leaf's
 // D (b^2 x b^2) and port summary S ((4b-4)^2).  When changed != nullptr,
 // changed[i] is set to 1 if listed leaf i's S decreased anywhere.
-void launch_leaf_apsp(const uint8_t* pass, int S, int b, int nl, const int* leaf_list, int n_list, uint16_t* leafD,
-                      uint16_t* leafS, int* changed, hipStream_t st);
+bool launch_leaf_apsp(const uint8_t* pass, int S, int b, int nl, const int* leaf_list, int n_list, uint16_t* leafD,
+                      uint16_t* leafS, int* changed, hipStream_t st, KTimer kt = {});
 
 // D_U of each listed node := block-diagonal child summaries, INF elsewhere.
 // Child summaries come from leafS (lvl 1, stride (4b-4)^2) or the previous
@@ -63,13 +71,13 @@ void launch_compact_flags(int nV, const int* node_list, int n_list, uint8_t* fla
 // pivots, R := the node's block pivot rows after Floyd-Warshall over the
 // block's pivots (P x nV, into Rbuf slot i), on the node's live octets
 // (live8_list, live8_cnt; max_live8 = the largest count in the list).
-void launch_closure_pivots(int P, int nV, int ld, const int* node_list, int n_list, const int* pivots, int piv_stride,
+bool launch_closure_pivots(int P, int nV, int ld, const int* node_list, int n_list, const int* pivots, int piv_stride,
                            const int* counts, int t, const uint16_t* DU, size_t node_stride, const int* live8_cnt,
-                           const int* live8_list, int ng8, int max_live8, uint16_t* Rbuf, hipStream_t st);
+                           const int* live8_list, int ng8, int max_live8, uint16_t* Rbuf, hipStream_t st, KTimer kt = {});
 // Phase 2 of the same block, D := min(D, R^T (x) R), over live octets.
-void launch_closure_update(int P, int nV, int ld, const int* node_list, int n_list, const int* counts, int t,
+bool launch_closure_update(int P, int nV, int ld, const int* node_list, int n_list, const int* counts, int t,
                            uint16_t* DU, size_t node_stride, const int* live8_cnt, const int* live8_list, int ng8,
-                           int max_live8, const uint16_t* Rbuf, hipStream_t st);
+                           int max_live8, const uint16_t* Rbuf, hipStream_t st, KTimer kt = {});
 
 // S_U := D_U[ext][ext]; changed[i] = 1 if listed node i's S_U decreased.
 void launch_project_su(int nV, int ld, int ne, const int* ext_idx, const int* node_list, int n_list,
@@ -95,23 +103,23 @@ void launch_q_front(int nq, const int* q_k, const long long* q_out, const int* q
 // live column groups of W (live_list, live_cnt), col_tiles = max over the
 // items of ceil(live_cnt / 64).  With ksplit == 1 the results go straight to
 // Z (position colpos[col]) and the lifted Xn (exterior port colext[col]).
-void launch_q_z(int M, int ld, int pc, const uint16_t* DU, size_t node_stride, const int* seam_list,
+bool launch_q_z(int M, int ld, int pc, const uint16_t* DU, size_t node_stride, const int* seam_list,
                 const int* seam_off, const int* cols, int ngrp, const int* live_cnt, const int* live_list,
                 int col_tiles, int ldp, const int4* items, int n_items, int ksplit, const uint16_t* X, uint16_t* Zp,
                 size_t part_stride, const int* colpos, const int* colext, const int* ext_child,
                 const int* ext_cport, uint16_t* Z, int ldz, uint16_t* Xn, int ne, unsigned long long* wstat,
-                hipStream_t st);
+                hipStream_t st, KTimer kt = {});
 // Fold: Z[slot][pos] = min_c Zp[c][slot][src[pos]] (INF where src[pos] < 0 or
 // the column's group is not live in the slot's node) and, for ne > 0, the
 // lift Xnext[slot][e] = min(folded column ext_col[e], Xprev[slot][own port of e]).
-void launch_q_fold(int n, int nz, int ldz, const int* src, int ldp, int ksplit, const uint16_t* Zp,
+bool launch_q_fold(int n, int nz, int ldz, const int* src, int ldp, int ksplit, const uint16_t* Zp,
                    size_t part_stride, const int2* slot_rc, int side, int nt, int ngrp, const uint8_t* live_map,
                    uint16_t* Z, int ne, const int* ext_col, const int* ext_child, const int* ext_cport, int pc, int cs,
-                   int g, const uint16_t* Xprev, uint16_t* Xnext, hipStream_t st);
+                   int g, const uint16_t* Xprev, uint16_t* Xnext, hipStream_t st, KTimer kt = {});
 // Meet tiles of every level (8 ints each: group, q2, i0, j0, j_end, level):
 // out[i][j] = min(out, min over the q2 seam block of Z_level[i] (+) Z_level[j]).
-void launch_q_meet(const int* tiles, int n_tiles, const int4* groups, const int* glist, const int* zgeo, int g2,
+bool launch_q_meet(const int* tiles, int n_tiles, const int4* groups, const int* glist, const int* zgeo, int g2,
                    const int* slot_i, const int* q_k, const long long* q_out, const uint16_t* Z, uint16_t* out,
-                   hipStream_t st);
+                   hipStream_t st, KTimer kt = {});
 
 }  // namespace htg

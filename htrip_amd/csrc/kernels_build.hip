@@ -5,6 +5,7 @@
 // v_pk_add_u16 (clamp) + v_pk_min_u16.  Every stored value is <= INF = 60000,
 // so min(x, sat16(a + b)) == min(x, add_dist(a, b)) exactly.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include "geometry.hpp"
 #include "kernels.hpp"
@@ -107,15 +108,16 @@ __global__ __launch_bounds__(256) void k_leaf_apsp(const uint8_t* __restrict__ p
     }
 }
 
-void launch_leaf_apsp(const uint8_t* pass, int S, int b, int nl, const int* leaf_list, int n_list, uint16_t* leafD,
-                      uint16_t* leafS, int* changed, hipStream_t st) {
+bool launch_leaf_apsp(const uint8_t* pass, int S, int b, int nl, const int* leaf_list, int n_list, uint16_t* leafD,
+                      uint16_t* leafS, int* changed, hipStream_t st, KTimer kt) {
+    if (n_list == 0) return false;
     const int n = b * b;
     auto go = [&](auto kern, int lanes) {
         const int swg = 4 * (64 / lanes);
         const int chunks = (n + swg - 1) / swg;
-        const size_t lds = size_t(swg) * n * sizeof(uint16_t);
-        hipLaunchKernelGGL(kern, dim3(n_list * chunks), dim3(256), lds, st, pass, S, b, nl, leaf_list, chunks, leafD,
-                           leafS, changed);
+        const uint32_t lds = uint32_t(size_t(swg) * n * sizeof(uint16_t));
+        hipExtLaunchKernelGGL(kern, dim3(n_list * chunks), dim3(256), lds, st, kt.a, kt.b, 0, pass, S, b, nl, leaf_list,
+                              chunks, leafD, leafS, changed);
     };
     if (b <= 2) go(k_leaf_apsp<2, uint32_t>, 2);
     else if (b <= 4) go(k_leaf_apsp<4, uint32_t>, 4);
@@ -123,6 +125,7 @@ void launch_leaf_apsp(const uint8_t* pass, int S, int b, int nl, const int* leaf
     else if (b <= 16) go(k_leaf_apsp<16, uint32_t>, 16);
     else if (b <= 32) go(k_leaf_apsp<32, uint32_t>, 32);
     else go(k_leaf_apsp<64, uint64_t>, 64);
+    return true;
 }
 
 // ---------------------------------------------------------------------------
@@ -580,33 +583,35 @@ __global__ __launch_bounds__(256) void k_closure_update(int nV, int ld, const in
     }
 }
 
-void launch_closure_pivots(int P, int nV, int ld, const int* node_list, int n_list, const int* pivots, int piv_stride,
+bool launch_closure_pivots(int P, int nV, int ld, const int* node_list, int n_list, const int* pivots, int piv_stride,
                            const int* counts, int t, const uint16_t* DU, size_t node_stride, const int* live8_cnt,
-                           const int* live8_list, int ng8, int max_live8, uint16_t* Rbuf, hipStream_t st) {
-    if (n_list == 0 || max_live8 == 0) return;
+                           const int* live8_list, int ng8, int max_live8, uint16_t* Rbuf, hipStream_t st, KTimer kt) {
+    if (n_list == 0 || max_live8 == 0) return false;
     const dim3 grid{unsigned((max_live8 + 15) / 16), unsigned(n_list), 1u};
 #define HTG_CP(PP)                                                                                                 \
-    hipLaunchKernelGGL(k_closure_pivots<PP>, grid, dim3(256), 0, st, nV, ld, node_list, pivots, piv_stride, counts, \
-                       t, DU, node_stride, live8_cnt, live8_list, ng8, Rbuf)
+    hipExtLaunchKernelGGL(k_closure_pivots<PP>, grid, dim3(256), 0, st, kt.a, kt.b, 0, nV, ld, node_list, pivots,  \
+                          piv_stride, counts, t, DU, node_stride, live8_cnt, live8_list, ng8, Rbuf)
     if (P == 16) HTG_CP(16);
     else if (P == 32) HTG_CP(32);
     else HTG_CP(64);
 #undef HTG_CP
+    return true;
 }
 
-void launch_closure_update(int P, int nV, int ld, const int* node_list, int n_list, const int* counts, int t,
+bool launch_closure_update(int P, int nV, int ld, const int* node_list, int n_list, const int* counts, int t,
                            uint16_t* DU, size_t node_stride, const int* live8_cnt, const int* live8_list, int ng8,
-                           int max_live8, const uint16_t* Rbuf, hipStream_t st) {
-    if (n_list == 0 || max_live8 == 0) return;
+                           int max_live8, const uint16_t* Rbuf, hipStream_t st, KTimer kt) {
+    if (n_list == 0 || max_live8 == 0) return false;
     const int tiles = (max_live8 + 15) / 16;
     const dim3 grid{unsigned(tiles * tiles), unsigned(n_list), 1u};
 #define HTG_CU(PP)                                                                                             \
-    hipLaunchKernelGGL(k_closure_update<PP>, grid, dim3(256), 0, st, nV, ld, node_list, counts, t, tiles, DU, \
-                       node_stride, live8_cnt, live8_list, ng8, Rbuf)
+    hipExtLaunchKernelGGL(k_closure_update<PP>, grid, dim3(256), 0, st, kt.a, kt.b, 0, nV, ld, node_list, counts, t, \
+                          tiles, DU, node_stride, live8_cnt, live8_list, ng8, Rbuf)
     if (P == 16) HTG_CU(16);
     else if (P == 32) HTG_CU(32);
     else HTG_CU(64);
 #undef HTG_CU
+    return true;
 }
 
 // ---------------------------------------------------------------------------

@@ -14,6 +14,7 @@
 // viewpoints that share (W, q), so every D_W row read feeds up to M
 // viewpoints from registers.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include "geometry.hpp"
 #include "kernels.hpp"
@@ -284,16 +285,17 @@ __global__ __launch_bounds__(256) void k_q_z(
     }
 }
 
-void launch_q_z(int M, int ld, int pc, const uint16_t* DU, size_t node_stride, const int* seam_list,
+bool launch_q_z(int M, int ld, int pc, const uint16_t* DU, size_t node_stride, const int* seam_list,
                 const int* seam_off, const int* cols, int ngrp, const int* live_cnt, const int* live_list,
                 int col_tiles, int ldp, const int4* items, int n_items, int ksplit, const uint16_t* X, uint16_t* Zp,
                 size_t part_stride, const int* colpos, const int* colext, const int* ext_child,
                 const int* ext_cport, uint16_t* Z, int ldz, uint16_t* Xn, int ne, unsigned long long* wstat,
-                hipStream_t st) {
-    if (n_items == 0 || col_tiles == 0) return;
+                hipStream_t st, KTimer kt) {
+    if (n_items == 0 || col_tiles == 0) return false;
     const dim3 grid{unsigned(n_items), unsigned(col_tiles), unsigned(ksplit)};
 #define HTG_QZ(MM)                                                                                                  \
-    hipLaunchKernelGGL((k_q_z<MM>), grid, dim3(256), 0, st, ld, pc, DU, node_stride, seam_list, seam_off, cols, ngrp, \
+    hipExtLaunchKernelGGL((k_q_z<MM>), grid, dim3(256), 0, st, kt.a, kt.b, 0, ld, pc, DU, node_stride, seam_list,   \
+                          seam_off, cols, ngrp,                                                                     \
                        live_cnt, live_list, ldp, items, ksplit, X, Zp, part_stride, colpos, colext, ext_child, \
                        ext_cport, Z, ldz, Xn, ne, wstat)
     if (M <= 1) HTG_QZ(1);
@@ -302,6 +304,7 @@ void launch_q_z(int M, int ld, int pc, const uint16_t* DU, size_t node_stride, c
     else if (M <= 8) HTG_QZ(8);
     else HTG_QZ(16);
 #undef HTG_QZ
+    return true;
 }
 
 // Fold of the row chunks, one launch with two index ranges per slot:
@@ -353,15 +356,16 @@ __global__ __launch_bounds__(256) void k_q_fold(int n, int nz, int ldz, const in
     Xnext[size_t(s) * ne + e] = v;
 }
 
-void launch_q_fold(int n, int nz, int ldz, const int* src, int ldp, int ksplit, const uint16_t* Zp,
+bool launch_q_fold(int n, int nz, int ldz, const int* src, int ldp, int ksplit, const uint16_t* Zp,
                    size_t part_stride, const int2* slot_rc, int side, int nt, int ngrp, const uint8_t* live_map,
                    uint16_t* Z, int ne, const int* ext_col, const int* ext_child, const int* ext_cport, int pc, int cs,
-                   int g, const uint16_t* Xprev, uint16_t* Xnext, hipStream_t st) {
-    if (n == 0) return;
+                   int g, const uint16_t* Xprev, uint16_t* Xnext, hipStream_t st, KTimer kt) {
+    if (n == 0) return false;
     const long long total = (long long)n * (nz + ne);
-    hipLaunchKernelGGL(k_q_fold, dim3(unsigned((total + 255) / 256)), dim3(256), 0, st, n, nz, ldz, src, ldp, ksplit,
-                       Zp, part_stride, slot_rc, side, nt, ngrp, live_map, Z, ne, ext_col, ext_child, ext_cport, pc,
-                       cs, g, Xprev, Xnext);
+    hipExtLaunchKernelGGL(k_q_fold, dim3(unsigned((total + 255) / 256)), dim3(256), 0, st, kt.a, kt.b, 0, n, nz, ldz,
+                          src, ldp, ksplit, Zp, part_stride, slot_rc, side, nt, ngrp, live_map, Z, ne, ext_col,
+                          ext_child, ext_cport, pc, cs, g, Xprev, Xnext);
+    return true;
 }
 
 // Meet: m_ij = min_{b in seam(q_j)} Z_i[b] (+) Z_j[b].  Every candidate is the
@@ -449,12 +453,13 @@ __global__ __launch_bounds__(256) void k_q_meet(const int* __restrict__ tiles, c
     atomic_min_u16(out + q_out[Q] + size_t(slot_i[si]) * q_k[Q] + slot_i[sj], best);
 }
 
-void launch_q_meet(const int* tiles, int n_tiles, const int4* groups, const int* glist, const int* zgeo, int g2,
+bool launch_q_meet(const int* tiles, int n_tiles, const int4* groups, const int* glist, const int* zgeo, int g2,
                    const int* slot_i, const int* q_k, const long long* q_out, const uint16_t* Z, uint16_t* out,
-                   hipStream_t st) {
-    if (n_tiles == 0) return;
-    hipLaunchKernelGGL(k_q_meet, dim3(unsigned(n_tiles)), dim3(256), 0, st, tiles, groups, glist, zgeo, g2, slot_i,
-                       q_k, q_out, Z, out);
+                   hipStream_t st, KTimer kt) {
+    if (n_tiles == 0) return false;
+    hipExtLaunchKernelGGL(k_q_meet, dim3(unsigned(n_tiles)), dim3(256), 0, st, kt.a, kt.b, 0, tiles, groups, glist,
+                          zgeo, g2, slot_i, q_k, q_out, Z, out);
+    return true;
 }
 
 }  // namespace htg

@@ -89,24 +89,34 @@ hipEvent_t Profiler::get() {
     }
     return e;
 }
-int Profiler::begin(int cls, hipStream_t st, double bytes, double updates) {
+int Profiler::open(int cls, double bytes, double updates, KTimer& kt) {
+    kt = KTimer{};
     if (!on) return -1;
     std::lock_guard<std::mutex> lk(mu);
     Pending p{cls, get(), get(), bytes, updates};
-    if (!p.a || !p.b) return -1;
-    (void)hipEventRecord(p.a, st);
+    if (!p.a || !p.b) {
+        if (p.a) pool.push_back(p.a);
+        if (p.b) pool.push_back(p.b);
+        return -1;
+    }
+    kt.a = p.a;
+    kt.b = p.b;
     const int token = next_token++ & 0x3fffffff;
-    open[token] = p;
+    this->open_[token] = p;
     return token;
 }
-void Profiler::end(int token, hipStream_t st) {
+void Profiler::close(int token, bool launched) {
     if (token < 0) return;
     std::lock_guard<std::mutex> lk(mu);
-    auto it = open.find(token);
-    if (it == open.end()) return;
-    (void)hipEventRecord(it->second.b, st);
-    pending.push_back(it->second);
-    open.erase(it);
+    auto it = open_.find(token);
+    if (it == open_.end()) return;
+    if (launched) {
+        pending.push_back(it->second);
+    } else {
+        pool.push_back(it->second.a);
+        pool.push_back(it->second.b);
+    }
+    open_.erase(it);
 }
 void Profiler::harvest() {
     std::lock_guard<std::mutex> lk(mu);
@@ -143,7 +153,7 @@ void Profiler::reset() {
     for (auto& a : acc) a = htg_kernel_stats{0, 0.0, 0.0, 0.0};
 }
 Profiler::~Profiler() {
-    for (auto& kv : open) pending.push_back(kv.second);
+    for (auto& kv : open_) pending.push_back(kv.second);
     for (auto& p : pending) {
         pool.push_back(p.a);
         pool.push_back(p.b);
@@ -467,19 +477,18 @@ int Map::run_closure(int lvl, const int* nodes, const std::vector<int>& hnodes, 
                 by_u += (2 * nV * nV + 2 * p * nV) * 2;  // D_U tile in + out, R in
             }
         }
-        int pi = prof_ref().begin(HTG_K_CLOSURE_PIVOTS, stream, by_p, upd_p);
-        launch_closure_pivots(P, L.d.nV, L.d.ld, nodes, n_list, static_cast<const int*>(ws_piv.p), L.d.nV,
-                              static_cast<const int*>(ws_counts.p), t, static_cast<const uint16_t*>(L.DU.p),
-                              L.node_stride, L.d_live8_cnt,
-                              L.d_live8_list, L.ng8, max8, static_cast<uint16_t*>(ws_R.p),
-                              stream);
-        prof_ref().end(pi, stream);
-        pi = prof_ref().begin(HTG_K_CLOSURE_UPDATE, stream, by_u, upd_u);
-        launch_closure_update(P, L.d.nV, L.d.ld, nodes, n_list, static_cast<const int*>(ws_counts.p), t,
-                              static_cast<uint16_t*>(L.DU.p), L.node_stride, L.d_live8_cnt,
-                              L.d_live8_list, L.ng8, max8,
-                              static_cast<const uint16_t*>(ws_R.p), stream);
-        prof_ref().end(pi, stream);
+        KTimer kt;
+        int pi = prof_ref().open(HTG_K_CLOSURE_PIVOTS, by_p, upd_p, kt);
+        bool la = launch_closure_pivots(P, L.d.nV, L.d.ld, nodes, n_list, static_cast<const int*>(ws_piv.p), L.d.nV,
+                                        static_cast<const int*>(ws_counts.p), t, static_cast<const uint16_t*>(L.DU.p),
+                                        L.node_stride, L.d_live8_cnt, L.d_live8_list, L.ng8, max8,
+                                        static_cast<uint16_t*>(ws_R.p), stream, kt);
+        prof_ref().close(pi, la);
+        pi = prof_ref().open(HTG_K_CLOSURE_UPDATE, by_u, upd_u, kt);
+        la = launch_closure_update(P, L.d.nV, L.d.ld, nodes, n_list, static_cast<const int*>(ws_counts.p), t,
+                                   static_cast<uint16_t*>(L.DU.p), L.node_stride, L.d_live8_cnt, L.d_live8_list,
+                                   L.ng8, max8, static_cast<const uint16_t*>(ws_R.p), stream, kt);
+        prof_ref().close(pi, la);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "closure launch");
@@ -499,10 +508,12 @@ const uint16_t* Map::child_summaries(int lvl, size_t& stride) const {
 int Map::build() {
     hipError_t e;
     const double leaf_bytes = (double(b) * b * b * b + double(4 * b - 4) * (4 * b - 4)) * 2 + double(b) * b;
-    int pi = prof_ref().begin(HTG_K_LEAF_APSP, stream, leaf_bytes * nl * nl, 0);
-    launch_leaf_apsp(static_cast<const uint8_t*>(d_pass.p), S, b, nl, nullptr, nl * nl, static_cast<uint16_t*>(leafD.p),
-                     static_cast<uint16_t*>(leafS.p), nullptr, stream);
-    prof_ref().end(pi, stream);
+    KTimer kt;
+    int pi = prof_ref().open(HTG_K_LEAF_APSP, leaf_bytes * nl * nl, 0, kt);
+    const bool la = launch_leaf_apsp(static_cast<const uint8_t*>(d_pass.p), S, b, nl, nullptr, nl * nl,
+                                     static_cast<uint16_t*>(leafD.p), static_cast<uint16_t*>(leafS.p), nullptr, stream,
+                                     kt);
+    prof_ref().close(pi, la);
     if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "leaf apsp launch");
     for (int lvl = 1; lvl <= H; ++lvl) {
         LevelGeo& L = lv[lvl - 1];
@@ -713,10 +724,12 @@ int Map::open_cells(const int32_t* rcs, int n) {
         launch_set_pass(static_cast<uint8_t*>(d_pass.p), S, reinterpret_cast<const int2*>(up + o_cells),
                         int(newly.size()), stream);
     const double leaf_bytes = (double(b) * b * b * b + double(4 * b - 4) * (4 * b - 4)) * 2 + double(b) * b;
-    int lpi = prof_ref().begin(HTG_K_LEAF_APSP, stream, leaf_bytes * nd, 0);
-    launch_leaf_apsp(static_cast<const uint8_t*>(d_pass.p), S, b, nl, reinterpret_cast<const int*>(up + o_leaves), nd,
-                     static_cast<uint16_t*>(leafD.p), static_cast<uint16_t*>(leafS.p), chg, stream);
-    prof_ref().end(lpi, stream);
+    KTimer lkt;
+    const int lpi = prof_ref().open(HTG_K_LEAF_APSP, leaf_bytes * nd, 0, lkt);
+    const bool lla = launch_leaf_apsp(static_cast<const uint8_t*>(d_pass.p), S, b, nl,
+                                      reinterpret_cast<const int*>(up + o_leaves), nd, static_cast<uint16_t*>(leafD.p),
+                                      static_cast<uint16_t*>(leafS.p), chg, stream, lkt);
+    prof_ref().close(lpi, lla);
     stats.n_touched_leaves = nd;
     stats.n_pivots = piv;
     if (H == 0) {
@@ -1100,33 +1113,35 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
         uint16_t* Zl = Qb + zoffq[size_t(l)];
         const int ne = l < H ? L.d.ne : 0;  // no lift above the root
         // bytes and updates of the projection are counted on the device
-        const int pi = prof_ref().begin(HTG_K_Q_PROJECT, st, 0.0, 0.0);
+        KTimer kt;
+        const int pi = prof_ref().open(HTG_K_Q_PROJECT, 0.0, 0.0, kt);
         const size_t part_stride = nsl * L.ldp;
-        launch_q_z(LP.M, L.d.ld, L.d.pc, static_cast<const uint16_t*>(L.DU.p), L.node_stride,
+        const bool la = launch_q_z(LP.M, L.d.ld, L.d.pc, static_cast<const uint16_t*>(L.DU.p), L.node_stride,
                    static_cast<const int*>(L.d_seam_list.p), static_cast<const int*>(L.d_seam_off.p),
                    static_cast<const int*>(L.d_zcols.p), L.ngrp, L.d_live_cnt,
                    L.d_live_list, LP.col_tiles, L.ldp,
                    reinterpret_cast<const int4*>(dp + o_items[l - 1]), int(LP.items.size()), LP.ksplit, Xl,
                    static_cast<uint16_t*>(ws.Zp.p), part_stride, static_cast<const int*>(L.d_colpos.p),
                    static_cast<const int*>(L.d_colext.p), static_cast<const int*>(L.d_ext_child.p),
-                   static_cast<const int*>(L.d_ext_cport.p), Zl, L.ldz, Xn, ne, wstat, st);
-        prof_ref().end(pi, st);
+                   static_cast<const int*>(L.d_ext_cport.p), Zl, L.ldz, Xn, ne, wstat, st, kt);
+        prof_ref().close(pi, la);
         if (LP.ksplit > 1) {
-            const int pf = prof_ref().begin(HTG_K_Q_LIFT, st, double(ns) * (L.nz + 2.0 * ne + L.d.pc) * 2, 0);
-            launch_q_fold(ns, L.nz, L.ldz, static_cast<const int*>(L.d_zsrc.p), L.ldp, LP.ksplit,
+            const int pf = prof_ref().open(HTG_K_Q_LIFT, double(ns) * (L.nz + 2.0 * ne + L.d.pc) * 2, 0, kt);
+            const bool lf = launch_q_fold(ns, L.nz, L.ldz, static_cast<const int*>(L.d_zsrc.p), L.ldp, LP.ksplit,
                           static_cast<const uint16_t*>(ws.Zp.p), part_stride, srcs, L.d.s, L.d.nt, L.ngrp,
                           L.d_live_map, Zl, ne, static_cast<const int*>(L.d_ext_col.p),
                           static_cast<const int*>(L.d_ext_child.p), static_cast<const int*>(L.d_ext_cport.p), L.d.pc,
-                          L.d.cs, g, Xl, Xn, st);
-            prof_ref().end(pf, st);
+                          L.d.cs, g, Xl, Xn, st, kt);
+            prof_ref().close(pf, lf);
         }
     }
     if (ns > 0 && !P.mtiles.empty()) {
-        const int pm = prof_ref().begin(HTG_K_Q_MEET, st, P.m_bytes, P.m_updates);
-        launch_q_meet(I(o_tiles), int(P.mtiles.size() / 8), reinterpret_cast<const int4*>(dp + o_groups), I(o_glist),
-                      I(o_zgeo), g2, I(o_si), I(o_qk), reinterpret_cast<const long long*>(dp + o_qout), Qb, dout,
-                      st);
-        prof_ref().end(pm, st);
+        KTimer kt;
+        const int pm = prof_ref().open(HTG_K_Q_MEET, P.m_bytes, P.m_updates, kt);
+        const bool lm = launch_q_meet(I(o_tiles), int(P.mtiles.size() / 8), reinterpret_cast<const int4*>(dp + o_groups),
+                                      I(o_glist), I(o_zgeo), g2, I(o_si), I(o_qk),
+                                      reinterpret_cast<const long long*>(dp + o_qout), Qb, dout, st, kt);
+        prof_ref().close(pm, lm);
     }
     if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "query launch");
     if (to_host) {

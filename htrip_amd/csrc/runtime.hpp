@@ -12,6 +12,7 @@
 
 #include "../../include/htrip_gpu.h"
 #include "geometry.hpp"
+#include "kernels.hpp"
 
 namespace htg {
 
@@ -168,8 +169,9 @@ struct QueryWS {
     ~QueryWS();
 };
 
-// Event-bracketed launch timing (see htg_profile_enable).  Thread-safe; a
-// session's two snapshot maps share one instance.
+// Kernel timing (see htg_profile_enable): start/stop events attached to each
+// tracked dispatch.  Thread-safe; a session's two snapshot maps share one
+// instance.
 struct Profiler {
     std::mutex mu;
     bool on = false;
@@ -179,14 +181,17 @@ struct Profiler {
         double bytes, updates;
     };
     std::vector<Pending> pending;          // ended, not yet harvested
-    std::unordered_map<int, Pending> open;  // begun, not yet ended
+    std::unordered_map<int, Pending> open_;  // reserved, not yet closed
     int next_token = 0;
     std::vector<hipEvent_t> pool;
     htg_kernel_stats acc[HTG_K_COUNT]{};
     hipEvent_t get();
-    // Records the start event; returns a token to pass to end().
-    int begin(int cls, hipStream_t st, double bytes, double updates);
-    void end(int idx, hipStream_t st);
+    // Reserves the start/stop events of one kernel dispatch (kt, for the
+    // launcher); returns a token for close(), or -1 with null events when
+    // timing is off.
+    int open(int cls, double bytes, double updates, KTimer& kt);
+    // launched: the dispatch carried the events; otherwise they are recycled.
+    void close(int token, bool launched);
     // Accumulate every completed pair; pairs still running stay pending.
     void harvest();
     // Work counted on the device (bytes, updates), added without a launch.

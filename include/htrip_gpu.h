@@ -127,8 +127,9 @@ int htg_read_node_DU(htg_map* map, int32_t lvl, int32_t idx, uint16_t* out); /* 
 int htg_read_node_SU(htg_map* map, int32_t lvl, int32_t idx, uint16_t* out); /* nExt x nExt */
 
 /* ---- kernel timing ----------------------------------------------------------
- * When enabled, the runtime brackets every launch of the kernel classes below
- * with hipEvents recorded on the stream the kernel runs on, and accumulates
+ * When enabled, the runtime attaches start/stop hipEvents to every dispatch of
+ * the kernel classes below (hipExtLaunchKernelGGL, so the interval is the
+ * kernel's own execution), and accumulates
  * launch counts, elapsed time and the algorithmic bytes / min-plus updates of
  * each launch.  Enabling resets the counters. */
 enum {
@@ -137,7 +138,7 @@ enum {
     HTG_K_CLOSURE_UPDATE = 2,
     HTG_K_Q_PROJECT = 3,
     HTG_K_Q_MEET = 4,
-    HTG_K_Q_LIFT = 5,
+    HTG_K_Q_LIFT = 5, /* fold of row-chunk partials + lift (only when a level splits its rows) */
     HTG_K_COUNT = 6
 };
 typedef struct htg_kernel_stats {
