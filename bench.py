@@ -209,13 +209,20 @@ def run_served(args):
         first = not trials
         state0 = rv.observed.copy() if first and not args.no_cpu else None
         tr = serving.make_trace(sid, rv, args.window * args.steps, rate, 100 + len(trials), update_frac=uf)
+        st0 = srv.stats()
         td = srv.replay(tr)
+        st1 = srv.stats()
         s = serving.latency_summary(tr, td)
         s["rate"] = rate
+        nb = st1["query_batches"] - st0["query_batches"]
+        s["mean_batch_queries"] = (st1["queries"] - st0["queries"]) / max(nb, 1)
+        s["worker_plan_busy"] = (st1["plan_seconds"] - st0["plan_seconds"]) / max(s["span_s"], 1e-9)
+        s["update_busy"] = (st1["update_seconds"] - st0["update_seconds"]) / max(s["span_s"], 1e-9)
         trials.append((tr, td, s, state0, n_upd_before))
         n_upd_before += int((tr.type == 1).sum())
         log(f"rate {rate:.0f}/s: {s['matrices_per_s']:.0f} matrices/s, p50 {s['p50_ms']:.3f} ms, "
-            f"p99 {s['p99_ms']:.3f} ms, updates p50 {s['update_p50_ms']} ms")
+            f"p99 {s['p99_ms']:.3f} ms, updates p50 {s['update_p50_ms']} ms, batch {s['mean_batch_queries']:.2f}, "
+            f"worker planning {100 * s['worker_plan_busy']:.0f}%, updating {100 * s['update_busy']:.0f}%")
         if s["p99_ms"] < 1.0:
             chosen = len(trials) - 1
             break

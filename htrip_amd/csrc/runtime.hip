@@ -9,6 +9,7 @@
 // Geometry (port order, V_U order, seam pairs) is identical to the reference
 // so every table can be compared element for element.
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -895,6 +896,7 @@ int Map::query_finish(QueryWS& ws, uint16_t* out_host) {
 int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws, bool to_host) {
     hipError_t e;
     int rc;
+    const auto th0 = std::chrono::steady_clock::now();
     long long total_vp = 0, total_out = 0;
     int kmax = 0;
     for (int q = 0; q < nq; ++q) {
@@ -923,12 +925,12 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
     }
     P.vp_rc.resize(total_vp);
     P.vp_valid.resize(total_vp);
-    struct SlotKey {
-        unsigned long long key;
-        int q, i, vp;
-    };
-    std::vector<SlotKey> sk;
-    sk.reserve(total_vp);
+    // Slot order: viewpoints sorted by (tile key, viewpoint index); the key's
+    // base-g^2 digits run from the root down, so the viewpoints sharing
+    // (W, q) at any level form contiguous runs.
+    std::vector<SlotKey>& sk = P.sk;
+    sk.clear();
+    sk.reserve(size_t(total_vp));
     for (int q = 0; q < nq; ++q)
         for (int i = 0; i < ks[q]; ++i) {
             const int v = P.q_vp[q] + i;
@@ -945,9 +947,7 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
             sk.push_back({key, q, i, v});
         }
     std::sort(sk.begin(), sk.end(), [](const SlotKey& a, const SlotKey& c) {
-        if (a.key != c.key) return a.key < c.key;
-        if (a.q != c.q) return a.q < c.q;
-        return a.i < c.i;
+        return a.key != c.key ? a.key < c.key : a.vp < c.vp;  // vp orders (q, i)
     });
     const int ns = int(sk.size());
     P.slot_rc.resize(ns);
@@ -957,6 +957,18 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
         P.slot_rc[s] = P.vp_rc[sk[s].vp];
         P.slot_q[s] = sk[s].q;
         P.slot_i[s] = sk[s].i;
+    }
+    // Meet order: slots grouped by query, in slot order within a query
+    // (counting sort); the (query, W) groups of any level are then
+    // contiguous runs, and within a group the target children q2 are too.
+    std::vector<int>& by_q = P.by_q;
+    by_q.assign(size_t(ns), 0);
+    {
+        std::vector<int>& qstart = P.qstart;
+        qstart.assign(size_t(nq) + 1, 0);
+        for (int s2 = 0; s2 < ns; ++s2) qstart[size_t(sk[s2].q) + 1]++;
+        for (int q = 0; q < nq; ++q) qstart[size_t(q) + 1] += qstart[size_t(q)];
+        for (int s2 = 0; s2 < ns; ++s2) by_q[size_t(qstart[size_t(sk[s2].q)]++)] = s2;
     }
     P.levels.resize(H);
     const int g2 = g * g;
@@ -1036,36 +1048,36 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
         }
         // (query, W) groups -> meet tiles per target child q2
         const unsigned long long divW = div * unsigned(g2);
-        for (int s = 0; s < ns;) {
-            int t = s + 1;
-            while (t < ns && sk[t].key / divW == sk[s].key / divW) ++t;
-            std::map<int, std::vector<int>> byq;
-            for (int a = s; a < t; ++a) byq[sk[a].q].push_back(a);
-            for (auto& kv : byq) {
-                const int n = int(kv.second.size());
-                if (n < 2) continue;
+        for (int a0 = 0; a0 < ns;) {
+            const int q = sk[size_t(by_q[size_t(a0)])].q;
+            const unsigned long long w = sk[size_t(by_q[size_t(a0)])].key / divW;
+            int a1 = a0 + 1;
+            while (a1 < ns && sk[size_t(by_q[size_t(a1)])].q == q && sk[size_t(by_q[size_t(a1)])].key / divW == w) ++a1;
+            const int n = a1 - a0;
+            if (n >= 2) {
                 const int gi = int(P.groups.size());
-                const int goff = int(P.glist.size());
-                P.groups.push_back(make_int4(kv.first, 0, goff, n));
-                P.glist.insert(P.glist.end(), kv.second.begin(), kv.second.end());
+                P.groups.push_back(make_int4(q, 0, int(P.glist.size()), n));
+                P.glist.insert(P.glist.end(), by_q.begin() + a0, by_q.begin() + a1);
                 for (int j0 = 0; j0 < n;) {
-                    const int q2 = int((sk[kv.second[j0]].key / div) % unsigned(g2));
+                    const int q2 = int((sk[size_t(by_q[size_t(a0 + j0)])].key / div) % unsigned(g2));
                     int je = j0 + 1;
-                    while (je < n && int((sk[kv.second[je]].key / div) % unsigned(g2)) == q2) ++je;
+                    while (je < n && int((sk[size_t(by_q[size_t(a0 + je)])].key / div) % unsigned(g2)) == q2) ++je;
                     for (int i0 = 0; i0 < n; i0 += 8)
                         for (int jj = j0; jj < je; jj += 8) {
                             const int tt[8] = {gi, q2, i0, jj, je, l - 1, 0, 0};
                             P.mtiles.insert(P.mtiles.end(), tt, tt + 8);
-                            P.m_updates += double(std::min(8, n - i0)) * std::min(8, je - jj) * L.zlen[q2];
-                            P.m_bytes += double(std::min(8, n - i0) + std::min(8, je - jj)) * L.zlen[q2] * 2;
+                            P.m_updates += double(std::min(8, n - i0)) * std::min(8, je - jj) * L.zlen[size_t(q2)];
+                            P.m_bytes +=
+                                double(std::min(8, n - i0) + std::min(8, je - jj)) * L.zlen[size_t(q2)] * 2;
                         }
                     j0 = je;
                 }
             }
-            s = t;
+            a0 = a1;
         }
         div = divW;
     }
+    const auto th1 = std::chrono::steady_clock::now();
     // ---- upload (one packed staging buffer)
     Packer pk;
     const size_t o_qk = pk.add(P.q_k), o_qout = pk.add(P.q_out), o_qvp = pk.add(P.q_vp), o_vprc = pk.add(P.vp_rc),
@@ -1093,6 +1105,7 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
     auto I = [&](size_t o) { return reinterpret_cast<const int*>(dp + o); };
     uint16_t* dout = static_cast<uint16_t*>(ws.d_out.p);
     const uint16_t* dleafD = static_cast<const uint16_t*>(leafD.p);
+    const auto th2 = std::chrono::steady_clock::now();
     ws.counted = prof_ref().on;
     unsigned long long* wstat = nullptr;
     if (ws.counted) {
@@ -1152,6 +1165,31 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
     if (wstat && (e = hipMemcpyAsync(ws.h_wstat.p, wstat, 16, hipMemcpyDeviceToHost, st)) != hipSuccess)
         return hip_fail(e, "download counters");
     if ((e = hipEventRecord(ws.done, st)) != hipSuccess) return hip_fail(e, "record");
+    const auto th3 = std::chrono::steady_clock::now();
+    ws.host_s[0] += std::chrono::duration<double>(th1 - th0).count();
+    ws.host_s[1] += std::chrono::duration<double>(th2 - th1).count();
+    ws.host_s[2] += std::chrono::duration<double>(th3 - th2).count();
+    ws.host_batches++;
+    return HTG_OK;
+}
+
+int Map::reserve_query_ws(QueryWS& ws, int max_vps, long long max_out) const {
+    const size_t nsl = size_t(std::max(max_vps, 1));
+    auto a8 = [](size_t x) { return (x + 7) / 8 * 8; };
+    size_t qsz = a8(nsl * size_t(4 * b - 4)), zp = 8;
+    for (int l = 1; l <= H; ++l) {
+        const LevelGeo& L = lv[size_t(l) - 1];
+        if (l < H) qsz += a8(nsl * size_t(L.d.ne));
+        qsz += a8(nsl * size_t(L.ldz));
+        zp = std::max(zp, size_t(4) * L.ldp);  // up to four row chunks without growing
+    }
+    const size_t plan = nsl * 64 + (size_t(1) << 20);
+    int rc;
+    if ((rc = ws.Q.ensure(qsz * 2)) || (rc = ws.Zp.ensure(nsl * zp * 2)) ||
+        (rc = ws.d_out.ensure(size_t(max_out) * 2 + 4)) || (rc = ws.h_out.ensure(size_t(max_out) * 2)) ||
+        (rc = ws.d_plan.ensure(plan)) || (rc = ws.h_plan.ensure(plan)) || (rc = ws.d_wstat.ensure(16)) ||
+        (rc = ws.h_wstat.ensure(16)))
+        return rc;
     return HTG_OK;
 }
 
@@ -1168,6 +1206,10 @@ int QueryWS::init(hipStream_t ext) {
     return HTG_OK;
 }
 QueryWS::~QueryWS() {
+    if (debug_level() >= 2 && host_batches > 0)
+        std::fprintf(stderr, "[htg] query host time per batch (%lld batches): plan %.1f us, pack+upload %.1f us, "
+                     "launch %.1f us\n", host_batches, 1e6 * host_s[0] / host_batches, 1e6 * host_s[1] / host_batches,
+                     1e6 * host_s[2] / host_batches);
     if (stream) (void)hipStreamSynchronize(stream);
     if (done) (void)hipEventDestroy(done);
     if (stream && own_stream) (void)hipStreamDestroy(stream);

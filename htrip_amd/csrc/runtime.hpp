@@ -115,7 +115,14 @@ struct LevelPlan {
     std::vector<int4> items;  // (W, q | first-of-run << 16, first slot, count)
 };
 
+struct SlotKey {
+    unsigned long long key;  // tile path, base-g^2 digits from the root down
+    int q, i, vp;            // query, index in the query, viewpoint index
+};
+
 struct QueryPlan {
+    std::vector<SlotKey> sk;          // slot order
+    std::vector<int> by_q, qstart;    // meet order: slots grouped by query
     std::vector<int> q_k, q_vp;
     std::vector<long long> q_out;
     std::vector<int2> vp_rc;
@@ -164,6 +171,8 @@ struct QueryWS {
     // Q: X of levels 0..H-1 then Z of levels 1..H, each region 8-aligned
     DevBuf d_plan, Q, Zp, d_out, d_wstat;  // d_wstat: k_q_z work counters (bytes, updates)
     bool counted = false;                        // this batch's counters go to the profiler
+    double host_s[3] = {0, 0, 0};                // host time: planning, packing + upload, launches
+    long long host_batches = 0;
     // Runs on `ext` when given (not owned), else on a stream of its own.
     int init(hipStream_t ext = nullptr);
     ~QueryWS();
@@ -251,6 +260,10 @@ struct Map {
     // copied to ws.h_out when to_host), then wait for ws.done.
     int query_launch(const int32_t* rc, const int32_t* ks, int nq, QueryWS& ws, bool to_host);
     int query_finish(QueryWS& ws, uint16_t* out_host);
+    // Grow ws ahead of time for batches of up to max_vps viewpoints and
+    // max_out result entries, so that serving does not allocate (a device
+    // free synchronises the whole device).
+    int reserve_query_ws(QueryWS& ws, int max_vps, long long max_out) const;
     int ensure_build_ws(int lvl);
     // Lay out the live block of every level (once, at creation).
     int init_live();

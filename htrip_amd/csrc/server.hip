@@ -55,12 +55,14 @@ struct Request {
     bool done = false;
 };
 
+// One query slot: a stream workspace and the thread that serves it.
 struct Batch {
     QueryWS ws;
+    std::mutex ws_mu;  // held by the slot thread while a batch uses ws
     int session = -1;
     int map_idx = 0;
     std::vector<Request*> reqs;
-    bool busy = false;
+    std::thread th;
 };
 
 struct Session {
@@ -114,74 +116,46 @@ struct htg_server {
     hipStream_t ustream = nullptr;  // builds and updates of every session
     std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
     uint64_t next_ticket = 1;
-    std::thread worker, updater;
+    std::thread updater;
+    int rr = 0;  // session where the next slot starts looking (round robin)
     bool stop = false;
     htg_server_stats stats{};
 
     bool snapshot() const { return cfg.consistency == HTG_CONSISTENCY_SNAPSHOT; }
-    void run();
+    void slot_loop(Batch* b);
     void run_updates();
-    bool retire();
-    bool dispatch();
+    bool pending_work() const;
     bool update_one(std::unique_lock<std::mutex>& lk);
 };
 
-bool htg_server::retire() {
-    bool any = false;
-    for (auto& bp : pool) {
-        Batch& b = *bp;
-        if (!b.busy) continue;
-        if (hipEventQuery(b.ws.done) != hipSuccess) {
-            (void)hipGetLastError();
-            continue;
-        }
-        std::lock_guard<std::mutex> lk(mu);
-        Session& ss = sessions.at(b.session);
-        const int rc = ss.maps[b.map_idx]->query_finish(b.ws, nullptr);
-        const uint16_t* res = static_cast<const uint16_t*>(b.ws.h_out.p);
-        long long off = 0;
-        const auto now = Clock::now();
-        for (Request* r : b.reqs) {
-            const size_t n = size_t(r->count) * r->count;
-            if (rc == HTG_OK && r->out && n) std::memcpy(r->out, res + off, n * 2);
-            off += (long long)n;
-            mark_done(r, rc, now);
-        }
-        ss.inflight[b.map_idx]--;
-        b.busy = false;
-        b.reqs.clear();
-        any = true;
-    }
-    if (any) {
-        cv_done.notify_all();
-        cv_upd.notify_all();
-    }
-    return any;
+bool htg_server::pending_work() const {
+    for (auto& kv : sessions)
+        if (!kv.second.pending.empty()) return true;
+    return false;
 }
 
-bool htg_server::dispatch() {
-    bool any = false;
-    std::vector<int> ids;
-    {
-        std::lock_guard<std::mutex> lk(mu);
-        for (auto& kv : sessions) ids.push_back(kv.first);
-    }
-    for (int id : ids) {
-        for (;;) {
-            std::unique_lock<std::mutex> lk(mu);
-            auto it = sessions.find(id);
-            if (it == sessions.end()) break;
-            Session& ss = it->second;
-            if (ss.pending.empty()) break;
-            std::vector<Request*> run;
+// A slot thread takes the next dispatchable work of any session (round
+// robin), outside the lock plans and launches it on its own stream, waits
+// for it, and completes its requests.  Query batches of several slots (and
+// sessions) overlap on the device, and their host-side planning runs on
+// separate threads.  In strict mode a slot also applies the update at the
+// head of a session once the session's query batches have completed.
+void htg_server::slot_loop(Batch* b) {
+    if (hipSetDevice(ctx->device) != hipSuccess) return;
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+        int sid = -1;
+        bool is_update = false;
+        std::vector<Request*> run;
+        std::vector<int> ids;  // sessions, starting at the round-robin point
+        for (auto it = sessions.lower_bound(rr); it != sessions.end(); ++it) ids.push_back(it->first);
+        for (auto it = sessions.begin(); it != sessions.end() && it->first < rr; ++it) ids.push_back(it->first);
+        for (int id : ids) {
+            Session& ss = sessions.at(id);
+            // strict: nothing passes an update being applied; snapshot:
+            // updates go to the back copy and never hold queries back
+            if (ss.pending.empty() || (!snapshot() && ss.updating)) continue;
             if (ss.pending.front()->type == 0) {
-                Batch* fb = nullptr;
-                for (auto& bp : pool)
-                    if (!bp->busy) {
-                        fb = bp.get();
-                        break;
-                    }
-                if (!fb) return any;  // every slot busy: retire first
                 long long vps = 0;
                 while (!ss.pending.empty() && ss.pending.front()->type == 0 &&
                        int(run.size()) < cfg.max_batch_queries &&
@@ -190,76 +164,87 @@ bool htg_server::dispatch() {
                     run.push_back(ss.pending.front());
                     ss.pending.pop_front();
                 }
-                const int mi = snapshot() ? ss.front : 0;
-                fb->busy = true;
-                fb->session = id;
-                fb->map_idx = mi;
-                fb->reqs = run;
-                ss.inflight[mi]++;
-                std::vector<int32_t> rc, ks;
-                for (Request* r : run) {
-                    rc.insert(rc.end(), r->rc.begin(), r->rc.end());
-                    ks.push_back(r->count);
-                    r->version = ss.applied;
-                }
-                htg_map* m = ss.maps[mi];
-                lk.unlock();
-                const int st = m->query_launch(rc.data(), ks.data(), int(ks.size()), fb->ws, true);
-                lk.lock();
-                stats.query_batches++;
-                stats.queries += run.size();
-                if (st != HTG_OK) {  // fail this batch's requests, keep serving the others
-                    const auto now = Clock::now();
-                    for (Request* r : run) mark_done(r, st, now);
-                    ss.inflight[mi]--;
-                    fb->busy = false;
-                    fb->reqs.clear();
-                    cv_done.notify_all();
-                }
-            } else {  // strict mode only: updates are queued with the queries
-                if (ss.inflight[0] > 0) break;  // its query batches still read the tables
+            } else if (ss.inflight[0] == 0) {  // strict: the queries before the update have completed
                 while (!ss.pending.empty() && ss.pending.front()->type == 1) {
                     run.push_back(ss.pending.front());
                     ss.pending.pop_front();
                 }
-                ss.updating++;
-                std::vector<int32_t> cells;
-                for (Request* r : run) cells.insert(cells.end(), r->rc.begin(), r->rc.end());
-                htg_map* m = ss.maps[0];
-                lk.unlock();
-                const int st = m->open_cells(cells.data(), int(cells.size() / 2));
-                lk.lock();
-                ss.updating--;
-                ss.applied += int64_t(run.size());
-                stats.update_batches++;
-                stats.updates += run.size();
-                const auto now = Clock::now();
-                for (Request* r : run) mark_done(r, st, now);
-                cv_done.notify_all();
-            }
-            any = true;
-        }
-    }
-    return any;
-}
-
-void htg_server::run() {
-    if (hipSetDevice(ctx->device) != hipSuccess) return;
-    for (;;) {
-        {
-            std::unique_lock<std::mutex> lk(mu);
-            bool inflight = false, pending = false;
-            for (auto& bp : pool) inflight |= bp->busy;
-            for (auto& kv : sessions) pending |= !kv.second.pending.empty();
-            if (stop && !pending && !inflight) return;
-            if (!pending && !inflight) {
-                cv_work.wait(lk);
+                is_update = true;
+            } else {
                 continue;
             }
+            sid = id;
+            rr = id + 1;
+            break;
         }
-        const bool r = retire();
-        const bool d = dispatch();
-        if (!r && !d) std::this_thread::yield();
+        if (sid < 0) {
+            if (stop && !pending_work()) return;
+            cv_work.wait(lk);
+            continue;
+        }
+        Session& ss = sessions.at(sid);
+        if (is_update) {
+            ss.updating++;
+            std::vector<int32_t> cells;
+            for (Request* r : run) cells.insert(cells.end(), r->rc.begin(), r->rc.end());
+            htg_map* m = ss.maps[0];
+            lk.unlock();
+            const auto tu0 = Clock::now();
+            const int st = m->open_cells(cells.data(), int(cells.size() / 2));
+            const double tu = std::chrono::duration<double>(Clock::now() - tu0).count();
+            lk.lock();
+            stats.update_seconds += tu;
+            ss.updating--;
+            ss.applied += int64_t(run.size());
+            stats.update_batches++;
+            stats.updates += run.size();
+            const auto now = Clock::now();
+            for (Request* r : run) mark_done(r, st, now);
+            cv_done.notify_all();
+            cv_work.notify_all();  // queries queued behind the update may go now
+            continue;
+        }
+        const int mi = snapshot() ? ss.front : 0;
+        ss.inflight[mi]++;
+        std::vector<int32_t> rc, ks;
+        for (Request* r : run) {
+            rc.insert(rc.end(), r->rc.begin(), r->rc.end());
+            ks.push_back(r->count);
+            r->version = ss.applied;
+        }
+        htg_map* m = ss.maps[mi];
+        stats.query_batches++;
+        stats.queries += run.size();
+        if (pending_work()) cv_work.notify_one();  // more work queued: let another idle slot take it
+        lk.unlock();
+        std::unique_lock<std::mutex> wl(b->ws_mu);
+        const auto tp0 = Clock::now();
+        int st = m->query_launch(rc.data(), ks.data(), int(ks.size()), b->ws, true);
+        const double tp = std::chrono::duration<double>(Clock::now() - tp0).count();
+        if (st == HTG_OK) {
+            hipError_t q;
+            while ((q = hipEventQuery(b->ws.done)) == hipErrorNotReady) std::this_thread::yield();
+            (void)hipGetLastError();
+            st = m->query_finish(b->ws, nullptr);
+        }
+        if (st == HTG_OK) {
+            const uint16_t* res = static_cast<const uint16_t*>(b->ws.h_out.p);
+            long long off = 0;
+            for (Request* r : run) {
+                const size_t n = size_t(r->count) * r->count;
+                if (r->out && n) std::memcpy(r->out, res + off, n * 2);
+                off += (long long)n;
+            }
+        }
+        wl.unlock();
+        lk.lock();
+        stats.plan_seconds += tp;
+        const auto now = Clock::now();
+        for (Request* r : run) mark_done(r, st, now);
+        ss.inflight[mi]--;
+        cv_done.notify_all();
+        cv_upd.notify_all();
+        if (!snapshot()) cv_work.notify_all();  // a strict update may wait for this batch
     }
 }
 
@@ -281,9 +266,12 @@ bool htg_server::update_one(std::unique_lock<std::mutex>& lk) {
         const bool stale = ss.back_stale;
         DirtySet prev = ss.last_dirty;
         lk.unlock();
+        const auto tu0 = Clock::now();
         int st = stale ? bm->catch_up_from(*fm, prev) : HTG_OK;
         if (st == HTG_OK) st = bm->open_cells(cells.data(), int(cells.size() / 2));
+        const double tu = std::chrono::duration<double>(Clock::now() - tu0).count();
         lk.lock();
+        stats.update_seconds += tu;
         ss.updating--;
         if (st == HTG_OK) {
             ss.front = back;
@@ -296,6 +284,7 @@ bool htg_server::update_one(std::unique_lock<std::mutex>& lk) {
         const auto now = Clock::now();
         for (Request* r : run) mark_done(r, st, now);
         cv_done.notify_all();
+        cv_work.notify_all();
         return true;
     }
     return false;
@@ -342,7 +331,10 @@ int htg_server_create(htg_ctx* ctx, const htg_server_config* cfg, htg_server** o
             if (rc) return rc;
         }
         htg_server* raw = s.get();
-        s->worker = std::thread([raw] { raw->run(); });
+        for (auto& bp : s->pool) {
+            Batch* bb = bp.get();
+            bb->th = std::thread([raw, bb] { raw->slot_loop(bb); });
+        }
         if (raw->snapshot()) s->updater = std::thread([raw] { raw->run_updates(); });
         *out = s.release();
         return HTG_OK;
@@ -359,7 +351,8 @@ int htg_server_destroy(htg_server* s) {
     }
     s->cv_work.notify_all();
     s->cv_upd.notify_all();
-    if (s->worker.joinable()) s->worker.join();
+    for (auto& bp : s->pool)
+        if (bp->th.joinable()) bp->th.join();
     if (s->updater.joinable()) s->updater.join();
     for (auto& kv : s->sessions)
         for (htg_map* m : kv.second.maps)
@@ -391,6 +384,13 @@ int htg_server_add_map(htg_server* s, const htg_config* cfg, int32_t rows, int32
     if (rc == HTG_OK && copies == 2) {
         rc = map_clone_on(m[0], 0, s->ustream, &m[1]);
         if (rc == HTG_OK) m[1]->profp = m[0]->profp;  // one profiler per session
+    }
+    // size every slot's workspace for this map's largest batch up front
+    const long long max_out = std::min<long long>((long long)s->cfg.max_batch_viewpoints * s->cfg.max_batch_viewpoints,
+                                                  (long long)s->cfg.max_batch_queries * 256 * 256);
+    for (size_t i = 0; rc == HTG_OK && i < s->pool.size(); ++i) {
+        std::lock_guard<std::mutex> wl(s->pool[i]->ws_mu);
+        rc = m[0]->reserve_query_ws(s->pool[i]->ws, s->cfg.max_batch_viewpoints, max_out);
     }
     std::lock_guard<std::mutex> lk(s->mu);
     if (rc != HTG_OK) {

@@ -31,7 +31,8 @@ class ServerConfig(C.Structure):
 
 class ServerStats(C.Structure):
     _fields_ = [("query_batches", C.c_uint64), ("queries", C.c_uint64), ("update_batches", C.c_uint64),
-                ("updates", C.c_uint64), ("hbm_used", C.c_uint64), ("sessions", C.c_int32), ("reserved", C.c_int32)]
+                ("updates", C.c_uint64), ("hbm_used", C.c_uint64), ("sessions", C.c_int32), ("reserved", C.c_int32),
+                ("plan_seconds", C.c_double), ("update_seconds", C.c_double)]
 
 
 def _bind(lib):

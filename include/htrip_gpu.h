@@ -177,6 +177,8 @@ typedef struct htg_server_config {
 typedef struct htg_server_stats {
     uint64_t query_batches, queries, update_batches, updates, hbm_used;
     int32_t sessions, reserved;
+    double plan_seconds;   /* worker-thread time spent planning and launching query batches */
+    double update_seconds; /* time spent applying update batches (catch-up included) */
 } htg_server_stats;
 int htg_server_create(htg_ctx* ctx, const htg_server_config* cfg, htg_server** out);
 int htg_server_destroy(htg_server* s);

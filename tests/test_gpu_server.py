@@ -92,7 +92,8 @@ def test_snapshot_stream_each_result_matches_its_snapshot(gpu_ctx, rate):
         assert 0 <= v <= n_upd
         np.testing.assert_array_equal(tr.query_result(i), grid_bfs(sv[v], tr.cells_of(i)), err_msg=f"req {i} v {v}")
         seen.append(v)
-    assert max(seen) > 0  # updates landed while queries ran
+    if rate < 1e6:  # paced arrivals: updates land while queries run (a burst may finish first)
+        assert max(seen) > 0
     # after draining, a fresh query sees every update
     vps = grids.sample_free(sv[-1], 30, 5)
     t, out = srv.submit_query(sid, vps)
