@@ -312,7 +312,7 @@ def main():
     ap.add_argument("--batch-viewpoints", type=int, default=8192)
     ap.add_argument("--plain-steps", type=int, default=200)
     ap.add_argument("--consistency", default="snapshot", choices=["snapshot", "strict"])
-    ap.add_argument("--rate-fracs", type=float, nargs="+", default=[0.8, 0.6, 0.45, 0.3, 0.2])
+    ap.add_argument("--rate-fracs", type=float, nargs="+", default=[1.2, 1.05, 0.9, 0.75, 0.6, 0.45, 0.3, 0.2])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()

@@ -83,6 +83,16 @@ bool launch_closure_update(int P, int nV, int ld, const int* node_list, int n_li
 void launch_project_su(int nV, int ld, int ne, const int* ext_idx, const int* node_list, int n_list,
                        const uint16_t* DU, size_t node_stride, uint16_t* SU, int* changed, hipStream_t st);
 
+// Snapshot catch-up copies: whole items (item t = elems u16 at t*elems) and
+// the live rows of node tables (D_U rows of live octets, S_U rows of the
+// exterior ports among them).
+void launch_copy_items(const int* items, int n_items, size_t elems, const uint16_t* src, uint16_t* dst,
+                       hipStream_t st);
+void launch_copy_live_rows(const int* nodes, int n_nodes, int nV, int ld, size_t node_stride, const int* live8_cnt,
+                           const int* live8_list, int ng8, int max_live8, int ne, const int* vu_ext,
+                           const uint16_t* srcDU, uint16_t* dstDU, const uint16_t* srcSU, uint16_t* dstSU,
+                           hipStream_t st);
+
 // pass[r*S + c] = 1 for the listed cells.
 void launch_set_pass(uint8_t* pass, int S, const int2* cells, int n, hipStream_t st);
 

@@ -5,6 +5,8 @@
 // v_pk_add_u16 (clamp) + v_pk_min_u16.  Every stored value is <= INF = 60000,
 // so min(x, sat16(a + b)) == min(x, add_dist(a, b)) exactly.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 #include <hip/hip_ext.h>
 
 #include "geometry.hpp"
@@ -688,6 +690,74 @@ void launch_project_su_live(int nV, int ld, int ne, const int* vu_ext, const int
     const dim3 grid{unsigned((total + 255) / 256), unsigned(n_list), 1u};
     hipLaunchKernelGGL(k_project_su_live, grid, dim3(256), 0, st, nV, ld, ne, vu_ext, node_list, counts, live8_cnt,
                        live8_list, ng8, DU, node_stride, SU, changed);
+}
+
+// ---------------------------------------------------------------------------
+// Table copies for bringing a snapshot copy level with its twin.
+// Whole items (leaf tables): item t of the list is the contiguous block of
+// elems u16 at src + t*elems; vec: elems is a multiple of 8 (16-byte copies).
+template <bool VEC>
+__global__ __launch_bounds__(256) void k_copy_items(const int* __restrict__ items, size_t elems,
+                                                    const uint16_t* __restrict__ src, uint16_t* __restrict__ dst) {
+    const size_t base = size_t(items[blockIdx.y]) * elems;
+    const size_t units = VEC ? elems / 8 : elems;
+    for (size_t u = size_t(blockIdx.x) * 256 + threadIdx.x; u < units; u += size_t(gridDim.x) * 256) {
+        if (VEC)
+            reinterpret_cast<uint4*>(dst + base)[u] = reinterpret_cast<const uint4*>(src + base)[u];
+        else
+            dst[base + u] = src[base + u];
+    }
+}
+
+void launch_copy_items(const int* items, int n_items, size_t elems, const uint16_t* src, uint16_t* dst,
+                       hipStream_t st) {
+    if (n_items == 0 || elems == 0) return;
+    const bool vec = elems % 8 == 0;
+    const size_t units = vec ? elems / 8 : elems;
+    const unsigned bx = unsigned(std::min<size_t>((units + 255) / 256, 64));
+    const dim3 grid{bx, unsigned(n_items), 1u};
+    if (vec)
+        hipLaunchKernelGGL(k_copy_items<true>, grid, dim3(256), 0, st, items, elems, src, dst);
+    else
+        hipLaunchKernelGGL(k_copy_items<false>, grid, dim3(256), 0, st, items, elems, src, dst);
+}
+
+// Live rows of node tables: for each listed node, the D_U rows of its live
+// octets (ld u16 each) and the S_U rows of the exterior ports among them (ne
+// u16 each, ne a multiple of 4).  Rows of dead ports never change.  Block =
+// one row (blockIdx.x = live row slot, blockIdx.y = node).
+__global__ __launch_bounds__(256) void k_copy_live_rows(const int* __restrict__ nodes, int nV, int ld,
+                                                        size_t node_stride, const int* __restrict__ live8_cnt,
+                                                        const int* __restrict__ live8_list, int ng8, int ne,
+                                                        const int* __restrict__ vu_ext,
+                                                        const uint16_t* __restrict__ srcDU, uint16_t* __restrict__ dstDU,
+                                                        const uint16_t* __restrict__ srcSU,
+                                                        uint16_t* __restrict__ dstSU) {
+    const int node = nodes[blockIdx.y];
+    const int t = blockIdx.x;
+    if (t >= 8 * live8_cnt[node]) return;
+    const int v = 8 * live8_list[size_t(node) * ng8 + t / 8] + t % 8;
+    if (v >= nV) return;
+    const size_t roff = size_t(node) * node_stride + size_t(v) * ld;
+    const uint4* s4 = reinterpret_cast<const uint4*>(srcDU + roff);
+    uint4* d4 = reinterpret_cast<uint4*>(dstDU + roff);
+    for (int u = threadIdx.x; u < ld / 8; u += 256) d4[u] = s4[u];
+    const int e = vu_ext[v];
+    if (e < 0) return;
+    const size_t soff = (size_t(node) * ne + e) * ne;
+    const uint2* s2 = reinterpret_cast<const uint2*>(srcSU + soff);
+    uint2* d2 = reinterpret_cast<uint2*>(dstSU + soff);
+    for (int u = threadIdx.x; u < ne / 4; u += 256) d2[u] = s2[u];
+}
+
+void launch_copy_live_rows(const int* nodes, int n_nodes, int nV, int ld, size_t node_stride, const int* live8_cnt,
+                           const int* live8_list, int ng8, int max_live8, int ne, const int* vu_ext,
+                           const uint16_t* srcDU, uint16_t* dstDU, const uint16_t* srcSU, uint16_t* dstSU,
+                           hipStream_t st) {
+    if (n_nodes == 0 || max_live8 == 0) return;
+    const dim3 grid{unsigned(8 * max_live8), unsigned(n_nodes), 1u};
+    hipLaunchKernelGGL(k_copy_live_rows, grid, dim3(256), 0, st, nodes, nV, ld, node_stride, live8_cnt, live8_list, ng8,
+                       ne, vu_ext, srcDU, dstDU, srcSU, dstSU);
 }
 
 __global__ void k_set_pass(uint8_t* pass, int S, const int2* cells, int n) {

@@ -841,29 +841,36 @@ int Map::catch_up_from(const Map& src, const DirtySet& ds) {
         const int rc = refresh_live(&ds.cells);
         if (rc) return rc;
     }
-    for (int leaf : ds.leaves) {
-        if ((e = hipMemcpyAsync(static_cast<uint16_t*>(leafD.p) + size_t(leaf) * n * n,
-                                static_cast<const uint16_t*>(src.leafD.p) + size_t(leaf) * n * n, n * n * 2,
-                                hipMemcpyDeviceToDevice, stream)) != hipSuccess ||
-            (e = hipMemcpyAsync(static_cast<uint16_t*>(leafS.p) + size_t(leaf) * pl * pl,
-                                static_cast<const uint16_t*>(src.leafS.p) + size_t(leaf) * pl * pl, pl * pl * 2,
-                                hipMemcpyDeviceToDevice, stream)) != hipSuccess)
-            return hip_fail(e, "catch-up leaf");
-    }
+    // one packed upload of the leaf and node lists, then one copy launch per
+    // table kind and level
+    Packer pk;
+    const size_t o_leaves = pk.add(ds.leaves);
+    std::vector<size_t> o_nodes(ds.nodes.size());
+    for (size_t l = 0; l < ds.nodes.size(); ++l) o_nodes[l] = pk.add(ds.nodes[l]);
+    int rc;
+    if ((rc = ws_upd_h.ensure(pk.size)) || (rc = ws_upd.ensure(pk.size))) return rc;
+    pk.write(static_cast<char*>(ws_upd_h.p));
+    if ((e = hipMemcpyAsync(ws_upd.p, ws_upd_h.p, pk.size, hipMemcpyHostToDevice, stream)) != hipSuccess)
+        return hip_fail(e, "catch-up lists");
+    const char* up = static_cast<const char*>(ws_upd.p);
+    const int* dleaves = reinterpret_cast<const int*>(up + o_leaves);
+    const int nlv = int(ds.leaves.size());
+    launch_copy_items(dleaves, nlv, n * n, static_cast<const uint16_t*>(src.leafD.p), static_cast<uint16_t*>(leafD.p),
+                      stream);
+    launch_copy_items(dleaves, nlv, pl * pl, static_cast<const uint16_t*>(src.leafS.p),
+                      static_cast<uint16_t*>(leafS.p), stream);
     for (size_t l = 0; l < ds.nodes.size() && l < lv.size(); ++l) {
         LevelGeo& L = lv[l];
         const LevelGeo& Ls = src.lv[l];
-        const size_t ne2 = size_t(L.d.ne) * L.d.ne;
-        for (int node : ds.nodes[l]) {
-            if ((e = hipMemcpyAsync(static_cast<uint16_t*>(L.DU.p) + size_t(node) * L.node_stride,
-                                    static_cast<const uint16_t*>(Ls.DU.p) + size_t(node) * L.node_stride,
-                                    L.node_stride * 2, hipMemcpyDeviceToDevice, stream)) != hipSuccess ||
-                (e = hipMemcpyAsync(static_cast<uint16_t*>(L.SU.p) + size_t(node) * ne2,
-                                    static_cast<const uint16_t*>(Ls.SU.p) + size_t(node) * ne2, ne2 * 2,
-                                    hipMemcpyDeviceToDevice, stream)) != hipSuccess)
-                return hip_fail(e, "catch-up node");
-        }
+        int max8 = 0;
+        for (int node : ds.nodes[l]) max8 = std::max(max8, L.h_live8_cnt[node]);
+        launch_copy_live_rows(reinterpret_cast<const int*>(up + o_nodes[l]), int(ds.nodes[l].size()), L.d.nV, L.d.ld,
+                              L.node_stride, L.d_live8_cnt, L.d_live8_list, L.ng8, max8, L.d.ne,
+                              static_cast<const int*>(L.d_vu_ext.p), static_cast<const uint16_t*>(Ls.DU.p),
+                              static_cast<uint16_t*>(L.DU.p), static_cast<const uint16_t*>(Ls.SU.p),
+                              static_cast<uint16_t*>(L.SU.p), stream);
     }
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "catch-up copy");
     if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "catch-up sync");
     stats = src.stats;
     epoch = src.epoch;
