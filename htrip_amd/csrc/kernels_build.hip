@@ -457,28 +457,37 @@ __global__ __launch_bounds__(256) void k_closure_pivots(int nV, int ld, const in
     }
     __syncthreads();
     // Floyd-Warshall over the block's pivots.  Thread t keeps EPT entries of
-    // row fa in registers; each step reads pivot row k and its own column-k
-    // entry from LDS, relaxes, and publishes its entries for the next step.
+    // row fa in registers (packed pairs); each step reads pivot row k and its
+    // own column-k entry from LDS, relaxes, and publishes its entries.  Step
+    // k leaves row k and column k unchanged (C[k][k] = 0), so reads of step k
+    // never race with its writes and one barrier per step suffices.
     // Rows/columns >= p hold the identity and never change.
     constexpr int EPT = P * P / 256, TPR = P / EPT;
     const int fa = tid / TPR, fc0 = (tid % TPR) * EPT;
-    uint16_t cr[EPT];
+    if constexpr (EPT >= 2) {
+        constexpr int EP2 = EPT / 2;
+        uint32_t cr[EP2];
+        const uint32_t* Crow = reinterpret_cast<const uint32_t*>(&C[fa][fc0]);
 #pragma unroll
-    for (int e = 0; e < EPT; ++e) cr[e] = C[fa][fc0 + e];
-    for (int k = 0; k < p; ++k) {
-        const uint16_t aik = C[fa][k];
-        uint16_t rk[EPT];
+        for (int e = 0; e < EP2; ++e) cr[e] = Crow[e];
+        for (int k = 0; k < p; ++k) {
+            const uint32_t aik = dup16(C[fa][k]);
+            const uint32_t* rk = reinterpret_cast<const uint32_t*>(&C[k][fc0]);
 #pragma unroll
-        for (int e = 0; e < EPT; ++e) rk[e] = C[k][fc0 + e];
+            for (int e = 0; e < EP2; ++e) cr[e] = relax2(cr[e], aik, rk[e]);
+            uint32_t* Cw = reinterpret_cast<uint32_t*>(&C[fa][fc0]);
 #pragma unroll
-        for (int e = 0; e < EPT; ++e) {
-            const uint16_t v = add1(aik, rk[e]);
-            cr[e] = v < cr[e] ? v : cr[e];
+            for (int e = 0; e < EP2; ++e) Cw[e] = cr[e];
+            __syncthreads();
         }
-        __syncthreads();
-#pragma unroll
-        for (int e = 0; e < EPT; ++e) C[fa][fc0 + e] = cr[e];
-        __syncthreads();
+    } else {
+        uint16_t cr = C[fa][fc0];
+        for (int k = 0; k < p; ++k) {
+            const uint16_t v = add1(C[fa][k], C[k][fc0]);
+            cr = v < cr ? v : cr;
+            C[fa][fc0] = cr;
+            __syncthreads();
+        }
     }
     for (int i = tid; i < P * P; i += 256) {
         const int k2 = i / P, k = i % P;
@@ -539,14 +548,26 @@ __global__ __launch_bounds__(256) void k_closure_update(int nV, int ld, const in
     else if (tid < 32) co[tid - 16] = J * 16 + tid - 16 < nl8 ? 8 * l8[J * 16 + tid - 16] : -1;
     __syncthreads();
     const uint16_t* R = Rbuf + size_t(slot) * P * ld;
-    uint32_t* Rr = reinterpret_cast<uint32_t*>(&Rr4[0][0]);
-    uint16_t* Rc = reinterpret_cast<uint16_t*>(&Rc4[0][0]);
-    for (int i = tid; i < P * 128; i += 256) {
-        const int k = i / 128, x = i % 128;
-        const bool kin = k < p;
-        const int ri = ro[x >> 3], ci = co[x >> 3];
-        Rr[i] = (kin && ri >= 0 && ri + (x & 7) < nV) ? dup16(R[size_t(k) * ld + ri + (x & 7)]) : kInf2;
-        Rc[i] = (kin && ci >= 0 && ci + (x & 7) < nV) ? R[size_t(k) * ld + ci + (x & 7)] : kInf;
+    // stage with 16-byte loads: item i = (pivot k, octet slot o, side)
+    for (int i = tid; i < P * 32; i += 256) {
+        const int k = i >> 5, o = (i >> 1) & 15, col_side = i & 1;
+        const int base = col_side ? co[o] : ro[o];
+        uint4 v = make_uint4(kInf2, kInf2, kInf2, kInf2);
+        if (k < p && base >= 0) {
+            v = *reinterpret_cast<const uint4*>(R + size_t(k) * ld + base);
+            if (base + 8 > nV) {  // octet straddles nV: entries past it are padding
+                uint16_t* e = reinterpret_cast<uint16_t*>(&v);
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (base + j >= nV) e[j] = kInf;
+            }
+        }
+        if (col_side) {
+            Rc4[k][o] = v;
+        } else {
+            Rr4[k][2 * o] = make_uint4(dup16(v.x), dup16(v.x >> 16), dup16(v.y), dup16(v.y >> 16));
+            Rr4[k][2 * o + 1] = make_uint4(dup16(v.z), dup16(v.z >> 16), dup16(v.w), dup16(v.w >> 16));
+        }
     }
     __syncthreads();
     const int tx = tid & 15, ty = tid >> 4;
