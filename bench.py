@@ -198,7 +198,7 @@ def run_served(args):
     mode = serving.SNAPSHOT if args.consistency == "snapshot" else serving.STRICT
     srv = serving.Server(n_slots=args.slots, max_batch_queries=args.batch_queries,
                          max_batch_viewpoints=args.batch_viewpoints, consistency=mode,
-                         update_cu_percent=args.update_cu)
+                         update_cu_percent=args.update_cu, update_tick_us=args.update_tick_us)
     uf = args.update_frac
     kstats = {}  # kernel figures over every session of the served phase
 
@@ -252,10 +252,14 @@ def run_served(args):
         s["mean_batch_queries"] = (st1["queries"] - st0["queries"]) / max(nb, 1)
         s["worker_plan_busy"] = (st1["plan_seconds"] - st0["plan_seconds"]) / max(s["span_s"], 1e-9)
         s["update_busy"] = (st1["update_seconds"] - st0["update_seconds"]) / max(s["span_s"], 1e-9)
+        s["batch_ms"] = {k: 1e3 * (st1[f] - st0[f]) / max(nb, 1) for k, f in
+                         (("plan_launch", "plan_seconds"), ("gpu_wait", "gpu_wait_seconds"),
+                          ("finish", "finish_seconds"))}
         trials.append((tr, td, s, state0))
         log(f"rate {rate:.0f}/s: {s['matrices_per_s']:.0f} matrices/s, p50 {s['p50_ms']:.3f} ms, "
             f"p99 {s['p99_ms']:.3f} ms, updates p50 {s['update_p50_ms']} ms, batch {s['mean_batch_queries']:.2f}, "
-            f"worker planning {100 * s['worker_plan_busy']:.0f}%, updating {100 * s['update_busy']:.0f}%")
+            f"worker planning {100 * s['worker_plan_busy']:.0f}%, updating {100 * s['update_busy']:.0f}%, "
+            f"batch ms " + " ".join(f"{k} {v:.3f}" for k, v in s["batch_ms"].items()))
         if s["p99_ms"] < 1.0:
             chosen = len(trials) - 1
             break
@@ -281,7 +285,7 @@ def run_served(args):
                                "starts from the robots' start disks",
                    "mode": "served", "consistency": args.consistency, "window": args.window, "slots": args.slots,
                    "max_batch_queries": args.batch_queries, "update_cu_percent": args.update_cu,
-                   "update_frac": uf},
+                   "update_tick_us": args.update_tick_us, "update_frac": uf},
         "p50_ms": s["p50_ms"], "p99_ms": s["p99_ms"], "max_ms": s["max_ms"],
         "update_p50_ms": s["update_p50_ms"], "update_p99_ms": s["update_p99_ms"],
         "arrival_rate_per_s": s["rate"],
@@ -342,6 +346,7 @@ def main():
     ap.add_argument("--slots", type=int, default=3, help="query streams (the update stream makes one more)")
     ap.add_argument("--update-cu", type=int, default=0, help="percent of CUs for update kernels (0 = all)")
     ap.add_argument("--update-frac", type=float, default=0.1, help="share of update requests in the stream")
+    ap.add_argument("--update-tick-us", type=int, default=0, help="snapshot mode: minimum spacing of update batches")
     ap.add_argument("--batch-queries", type=int, default=64)
     ap.add_argument("--batch-viewpoints", type=int, default=8192)
     ap.add_argument("--plain-steps", type=int, default=200)

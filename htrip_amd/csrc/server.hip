@@ -77,6 +77,7 @@ struct Session {
     int64_t applied = 0;  // update requests applied to the front copy
     bool back_stale = false;
     DirtySet last_dirty;
+    Clock::time_point last_update{};  // start of the last update batch (snapshot mode)
 };
 
 // Update stream, restricted to about pct % of the CUs when 0 < pct < 100.
@@ -221,12 +222,14 @@ void htg_server::slot_loop(Batch* b) {
         const auto tp0 = Clock::now();
         int st = m->query_launch(rc.data(), ks.data(), int(ks.size()), b->ws, true);
         const double tp = std::chrono::duration<double>(Clock::now() - tp0).count();
+        const auto tg0 = Clock::now();
         if (st == HTG_OK) {
             hipError_t q;
             while ((q = hipEventQuery(b->ws.done)) == hipErrorNotReady) std::this_thread::yield();
             (void)hipGetLastError();
             st = m->query_finish(b->ws, nullptr);
         }
+        const auto tg1 = Clock::now();
         if (st == HTG_OK) {
             const uint16_t* res = static_cast<const uint16_t*>(b->ws.h_out.p);
             long long off = 0;
@@ -237,8 +240,11 @@ void htg_server::slot_loop(Batch* b) {
             }
         }
         wl.unlock();
+        const auto tg2 = Clock::now();
         lk.lock();
         stats.plan_seconds += tp;
+        stats.gpu_wait_seconds += std::chrono::duration<double>(tg1 - tg0).count();
+        stats.finish_seconds += std::chrono::duration<double>(tg2 - tg1).count();
         const auto now = Clock::now();
         for (Request* r : run) mark_done(r, st, now);
         ss.inflight[mi]--;
@@ -254,6 +260,9 @@ bool htg_server::update_one(std::unique_lock<std::mutex>& lk) {
     for (auto& kv : sessions) {
         Session& ss = kv.second;
         if (ss.upd_pending.empty() || ss.updating) continue;
+        if (!stop && cfg.update_tick_us > 0 &&
+            Clock::now() - ss.last_update < std::chrono::microseconds(cfg.update_tick_us))
+            continue;  // coalesce: the next tick takes everything pending then
         const int back = 1 - ss.front;
         if (ss.inflight[back] > 0) continue;  // queries still read the back copy
         std::vector<Request*> run(ss.upd_pending.begin(), ss.upd_pending.end());
@@ -267,6 +276,7 @@ bool htg_server::update_one(std::unique_lock<std::mutex>& lk) {
         DirtySet prev = ss.last_dirty;
         lk.unlock();
         const auto tu0 = Clock::now();
+        ss.last_update = tu0;
         int st = stale ? bm->catch_up_from(*fm, prev) : HTG_OK;
         if (st == HTG_OK) st = bm->open_cells(cells.data(), int(cells.size() / 2));
         const double tu = std::chrono::duration<double>(Clock::now() - tu0).count();
@@ -310,7 +320,7 @@ int htg_server_create(htg_ctx* ctx, const htg_server_config* cfg, htg_server** o
     try {
         if (!ctx || !cfg || !out) return fail(HTG_E_INVALID, "null argument");
         if (cfg->n_slots < 1 || cfg->n_slots > 64 || cfg->max_batch_queries < 1 || cfg->max_batch_viewpoints < 1 ||
-            cfg->update_cu_percent < 0 || cfg->update_cu_percent > 100 ||
+            cfg->update_cu_percent < 0 || cfg->update_cu_percent > 100 || cfg->update_tick_us < 0 ||
             (cfg->consistency != HTG_CONSISTENCY_STRICT && cfg->consistency != HTG_CONSISTENCY_SNAPSHOT))
             return fail(HTG_E_INVALID, "bad server configuration");
         hipError_t e = hipSetDevice(ctx->device);

@@ -172,13 +172,16 @@ typedef struct htg_server_config {
     int32_t max_batch_viewpoints; /* viewpoints per batch (a larger query still runs alone) */
     int32_t consistency;          /* HTG_CONSISTENCY_* */
     int32_t update_cu_percent;    /* builds/updates run on this share of the CUs (1..99); 0 or 100 = all */
-    int32_t reserved;
+    int32_t update_tick_us;       /* snapshot mode: a session's update batches start at least this far apart,
+                                     so more reveals merge into one closure (0 = as soon as possible) */
 } htg_server_config;
 typedef struct htg_server_stats {
     uint64_t query_batches, queries, update_batches, updates, hbm_used;
     int32_t sessions, reserved;
     double plan_seconds;   /* worker-thread time spent planning and launching query batches */
     double update_seconds; /* time spent applying update batches (catch-up included) */
+    double gpu_wait_seconds; /* slot time from the last launch until the batch's results were on the host */
+    double finish_seconds;   /* slot time copying results out and completing requests */
 } htg_server_stats;
 int htg_server_create(htg_ctx* ctx, const htg_server_config* cfg, htg_server** out);
 int htg_server_destroy(htg_server* s);
