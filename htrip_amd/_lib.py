@@ -24,7 +24,7 @@ EXPORTS = [
     "htg_last_batch_stats", "htg_sync",
     "htg_read_leaf_D", "htg_read_leaf_S", "htg_node_dims", "htg_read_node_DU", "htg_read_node_SU",
     "htg_profile_enable", "htg_profile_get",
-    "htg_get_tuning", "htg_set_tuning",
+    "htg_get_tuning", "htg_set_tuning", "htg_autotune",
 ]
 
 K_LEAF_APSP, K_CLOSURE_PIVOTS, K_CLOSURE_UPDATE, K_Q_PROJECT, K_Q_MEET, K_Q_LIFT = range(6)
@@ -104,6 +104,8 @@ def load(path: str = LIB_PATH):
         "htg_profile_get": (C.c_int, [vp, C.c_int, C.POINTER(KernelStats)]),
         "htg_get_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
         "htg_set_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
+        "htg_autotune": (C.c_int, [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int32, C.c_int32,
+                                   C.POINTER(Tuning)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -302,6 +304,15 @@ class GpuMap:
         for k, v in kw.items():
             setattr(t, k, v)
         check(self.lib.htg_set_tuning(self.h, C.byref(t)))
+
+    def autotune(self, sample=(), reps: int = 3) -> Tuning:
+        """htg_autotune: pick pivot block, graph capture and projection tile by timing."""
+        ks = np.array([len(v) for v in sample], dtype=np.int32)
+        allv = np.ascontiguousarray(np.concatenate([np.asarray(v, np.int32).reshape(-1, 2) for v in sample])
+                                    if len(sample) else np.zeros((0, 2), np.int32), dtype=np.int32)
+        t = Tuning()
+        check(self.lib.htg_autotune(self.h, ptr(allv, C.c_int32), ptr(ks, C.c_int32), len(ks), reps, C.byref(t)))
+        return t
 
     def sync(self) -> None:
         check(self.lib.htg_sync(self.h))

@@ -93,6 +93,9 @@ void launch_copy_live_rows(const int* nodes, int n_nodes, int nV, int ld, size_t
                            const uint16_t* srcDU, uint16_t* dstDU, const uint16_t* srcSU, uint16_t* dstSU,
                            hipStream_t st);
 
+// p[0..n) = v (graph-capturable).
+void launch_fill_u8(uint8_t* p, size_t n, uint8_t v, hipStream_t st);
+
 // pass[r*S + c] = 1 for the listed cells.
 void launch_set_pass(uint8_t* pass, int S, const int2* cells, int n, hipStream_t st);
 

@@ -781,6 +781,18 @@ void launch_copy_live_rows(const int* nodes, int n_nodes, int nV, int ld, size_t
                        ne, vu_ext, srcDU, dstDU, srcSU, dstSU);
 }
 
+// Byte fill (a kernel rather than hipMemsetAsync, so that it can sit in a
+// captured graph).
+__global__ __launch_bounds__(256) void k_fill_u8(uint8_t* __restrict__ p, size_t n, uint8_t v) {
+    for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) p[i] = v;
+}
+
+void launch_fill_u8(uint8_t* p, size_t n, uint8_t v, hipStream_t st) {
+    if (n == 0) return;
+    const unsigned blocks = unsigned(std::min<size_t>((n + 255) / 256, 4096));
+    hipLaunchKernelGGL(k_fill_u8, dim3(blocks), dim3(256), 0, st, p, n, v);
+}
+
 __global__ void k_set_pass(uint8_t* pass, int S, const int2* cells, int n) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i < n) pass[size_t(cells[i].x) * S + cells[i].y] = 1;

@@ -321,6 +321,7 @@ int Map::init_live() {
 }
 
 Map::~Map() {
+    if (build_graph) (void)hipGraphExecDestroy(build_graph);
     if (live_evt) {
         (void)hipEventSynchronize(live_evt);
         (void)hipEventDestroy(live_evt);
@@ -479,13 +480,13 @@ int Map::run_closure(int lvl, const int* nodes, const std::vector<int>& hnodes, 
             }
         }
         KTimer kt;
-        int pi = prof_ref().open(HTG_K_CLOSURE_PIVOTS, by_p, upd_p, kt);
+        int pi = prof_open(HTG_K_CLOSURE_PIVOTS, by_p, upd_p, kt);
         bool la = launch_closure_pivots(P, L.d.nV, L.d.ld, nodes, n_list, static_cast<const int*>(ws_piv.p), L.d.nV,
                                         static_cast<const int*>(ws_counts.p), t, static_cast<const uint16_t*>(L.DU.p),
                                         L.node_stride, L.d_live8_cnt, L.d_live8_list, L.ng8, max8,
                                         static_cast<uint16_t*>(ws_R.p), stream, kt);
         prof_ref().close(pi, la);
-        pi = prof_ref().open(HTG_K_CLOSURE_UPDATE, by_u, upd_u, kt);
+        pi = prof_open(HTG_K_CLOSURE_UPDATE, by_u, upd_u, kt);
         la = launch_closure_update(P, L.d.nV, L.d.ld, nodes, n_list, static_cast<const int*>(ws_counts.p), t,
                                    static_cast<uint16_t*>(L.DU.p), L.node_stride, L.d_live8_cnt, L.d_live8_list,
                                    L.ng8, max8, static_cast<const uint16_t*>(ws_R.p), stream, kt);
@@ -506,45 +507,118 @@ const uint16_t* Map::child_summaries(int lvl, size_t& stride) const {
     return static_cast<const uint16_t*>(C.SU.p);
 }
 
-int Map::build() {
+std::vector<int> Map::build_pivot_counts(int lvl) const {
+    const LevelGeo& L = lv[size_t(lvl) - 1];
+    std::vector<int> cnt(size_t(L.nn), 0);
+    std::vector<uint8_t> mark(size_t(L.d.nV));
+    for (int W = 0; W < L.nn; ++W) {
+        const int R0 = (W / L.d.nt) * L.d.s, C0 = (W % L.d.nt) * L.d.s;
+        std::fill(mark.begin(), mark.end(), 0);
+        int c = 0;
+        for (const int2& e : L.seams) {
+            const short2 a = L.vu_rc[size_t(e.x)], d = L.vu_rc[size_t(e.y)];
+            if (!pass[size_t(R0 + a.x) * S + size_t(C0 + a.y)] || !pass[size_t(R0 + d.x) * S + size_t(C0 + d.y)])
+                continue;
+            c += !mark[size_t(e.x)] + !mark[size_t(e.y)];
+            mark[size_t(e.x)] = mark[size_t(e.y)] = 1;
+        }
+        cnt[size_t(W)] = c;
+    }
+    return cnt;
+}
+
+// Build (src/minplus_oracle.cpp:48-137): leaf APSP, then per level assemble
+// -> seam edges -> pivot compaction -> blocked closure -> S_U, every node of
+// a level in the same launches.  The pivot counts come from the host, so the
+// sequence has no host sync and can run as one captured graph.
+int Map::enqueue_build(const std::vector<std::vector<int>>& hc) {
     hipError_t e;
     const double leaf_bytes = (double(b) * b * b * b + double(4 * b - 4) * (4 * b - 4)) * 2 + double(b) * b;
     KTimer kt;
-    int pi = prof_ref().open(HTG_K_LEAF_APSP, leaf_bytes * nl * nl, 0, kt);
+    int pi = prof_open(HTG_K_LEAF_APSP, leaf_bytes * nl * nl, 0, kt);
     const bool la = launch_leaf_apsp(static_cast<const uint8_t*>(d_pass.p), S, b, nl, nullptr, nl * nl,
                                      static_cast<uint16_t*>(leafD.p), static_cast<uint16_t*>(leafS.p), nullptr, stream,
                                      kt);
     prof_ref().close(pi, la);
-    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "leaf apsp launch");
     for (int lvl = 1; lvl <= H; ++lvl) {
-        LevelGeo& L = lv[lvl - 1];
-        int rc = ensure_build_ws(lvl);
-        if (rc) return rc;
-        std::vector<int> all(L.nn);
-        std::iota(all.begin(), all.end(), 0);
-        if ((e = hipMemcpyAsync(ws_nodes.p, all.data(), all.size() * 4, hipMemcpyHostToDevice, stream)) != hipSuccess)
-            return hip_fail(e, "upload node list");
-        if ((e = hipMemsetAsync(ws_flags.p, 0, size_t(L.nn) * L.d.nV, stream)) != hipSuccess)
-            return hip_fail(e, "clear flags");
+        LevelGeo& L = lv[size_t(lvl) - 1];
+        const int* nodes = static_cast<const int*>(L.d_all.p);
+        launch_fill_u8(static_cast<uint8_t*>(ws_flags.p), size_t(L.nn) * L.d.nV, 0, stream);
         size_t cstride;
         const uint16_t* csum = child_summaries(lvl, cstride);
-        launch_node_assemble(lvl, g, L.d.pc, L.d.nV, L.d.ld, L.d.nt, csum, cstride, static_cast<const int*>(ws_nodes.p),
-                             L.nn, static_cast<uint16_t*>(L.DU.p), L.node_stride, stream);
+        launch_node_assemble(lvl, g, L.d.pc, L.d.nV, L.d.ld, L.d.nt, csum, cstride, nodes, L.nn,
+                             static_cast<uint16_t*>(L.DU.p), L.node_stride, stream);
         launch_seam_relax(L.d.nV, L.d.ld, L.d.s, L.d.nt, static_cast<const int2*>(L.d_seams.p), int(L.seams.size()),
-                          static_cast<const short2*>(L.d_vu_rc.p), static_cast<const uint8_t*>(d_pass.p), S,
-                          static_cast<const int*>(ws_nodes.p), L.nn, static_cast<uint16_t*>(L.DU.p), L.node_stride,
-                          static_cast<uint8_t*>(ws_flags.p), stream);
-        launch_compact_flags(L.d.nV, static_cast<const int*>(ws_nodes.p), L.nn, static_cast<uint8_t*>(ws_flags.p),
-                             static_cast<int*>(ws_piv.p), L.d.nV, static_cast<int*>(ws_counts.p), stream);
-        std::vector<int> hc(L.nn);
-        if ((e = hipMemcpyAsync(hc.data(), ws_counts.p, size_t(L.nn) * 4, hipMemcpyDeviceToHost, stream)) != hipSuccess)
-            return hip_fail(e, "download counts");
-        if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "build sync");
-        if ((rc = run_closure(lvl, static_cast<const int*>(ws_nodes.p), all, hc))) return rc;
-        launch_project_su(L.d.nV, L.d.ld, L.d.ne, static_cast<const int*>(L.d_ext_idx.p),
-                          static_cast<const int*>(ws_nodes.p), L.nn, static_cast<const uint16_t*>(L.DU.p),
-                          L.node_stride, static_cast<uint16_t*>(L.SU.p), nullptr, stream);
-        if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "level launch");
+                          static_cast<const short2*>(L.d_vu_rc.p), static_cast<const uint8_t*>(d_pass.p), S, nodes,
+                          L.nn, static_cast<uint16_t*>(L.DU.p), L.node_stride, static_cast<uint8_t*>(ws_flags.p),
+                          stream);
+        launch_compact_flags(L.d.nV, nodes, L.nn, static_cast<uint8_t*>(ws_flags.p), static_cast<int*>(ws_piv.p),
+                             L.d.nV, static_cast<int*>(ws_counts.p), stream);
+        std::vector<int> all(size_t(L.nn));
+        std::iota(all.begin(), all.end(), 0);
+        int rc;
+        if ((rc = run_closure(lvl, nodes, all, hc[size_t(lvl)]))) return rc;
+        launch_project_su(L.d.nV, L.d.ld, L.d.ne, static_cast<const int*>(L.d_ext_idx.p), nodes, L.nn,
+                          static_cast<const uint16_t*>(L.DU.p), L.node_stride, static_cast<uint16_t*>(L.SU.p), nullptr,
+                          stream);
+    }
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "build launch");
+    return HTG_OK;
+}
+
+int Map::build() {
+    hipError_t e;
+    int rc;
+    std::vector<std::vector<int>> hc(size_t(H) + 1);
+    std::vector<int> sig;
+    int ws_lvl = 0;
+    size_t ws_need = 0;
+    for (int lvl = 1; lvl <= H; ++lvl) {
+        const LevelGeo& L = lv[size_t(lvl) - 1];
+        const size_t need = size_t(L.nn) * L.d.nV;
+        if (need > ws_need) ws_need = need, ws_lvl = lvl;
+        hc[size_t(lvl)] = build_pivot_counts(lvl);
+        int mx = 0, m8 = 0;
+        for (int W = 0; W < L.nn; ++W) {
+            mx = std::max(mx, hc[size_t(lvl)][size_t(W)]);
+            if (hc[size_t(lvl)][size_t(W)] > 0) m8 = std::max(m8, L.h_live8_cnt[W]);
+        }
+        sig.push_back(mx);
+        sig.push_back(m8);
+    }
+    sig.push_back(tune.pivot_block);
+    for (int lvl = 1; lvl <= H; ++lvl)  // every level's workspace before any launch (and any capture)
+        if ((rc = ensure_build_ws(lvl))) return rc;
+    (void)ws_lvl;
+    if (!tune.use_graph) {
+        if ((rc = enqueue_build(hc))) return rc;
+    } else {
+        if (build_graph && sig != build_sig) {
+            (void)hipGraphExecDestroy(build_graph);
+            build_graph = nullptr;
+        }
+        if (!build_graph) {
+            hipGraph_t graph = nullptr;
+            if ((e = hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed)) != hipSuccess)
+                return hip_fail(e, "begin capture");
+            capturing = true;
+            rc = enqueue_build(hc);
+            capturing = false;
+            e = hipStreamEndCapture(stream, &graph);
+            if (rc) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return rc;
+            }
+            if (e != hipSuccess) return hip_fail(e, "end capture");
+            e = hipGraphInstantiate(&build_graph, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) {
+                build_graph = nullptr;
+                return hip_fail(e, "graph instantiate");
+            }
+            build_sig = sig;
+        }
+        if ((e = hipGraphLaunch(build_graph, stream)) != hipSuccess) return hip_fail(e, "graph launch");
     }
     if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "build sync");
     prof_ref().harvest();
@@ -1350,6 +1424,10 @@ int map_create_on(htg_ctx* ctx, const htg_config* cfg, int32_t rows, int32_t col
             (rc = upload(L.d_colpos, L.colpos, st)) || (rc = upload(L.d_colext, L.colext, st)) ||
             (rc = upload(L.d_vu_ext, L.vu_ext, st)))
             return rc;
+        std::vector<int> all(size_t(L.nn));
+        std::iota(all.begin(), all.end(), 0);
+        if ((rc = upload(L.d_all, all, st)))
+            return rc;
     }
     if ((rc = m->init_live()) || (rc = m->refresh_live(nullptr))) return rc;
     if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(e, "upload live groups");
@@ -1571,6 +1649,70 @@ int htg_get_tuning(const htg_map* map, htg_tuning* out) {
     if (!map || !out) return fail(HTG_E_INVALID, "null argument");
     *out = map->tune;
     return HTG_OK;
+}
+
+// Autotuner: times a rebuild for every pivot block with and without graph
+// capture, and the sample query batch for every projection tile, and keeps
+// the fastest of each (ties keep the smaller choice).  The tables after a
+// rebuild are the same, so tuning never changes results.
+int htg_autotune(htg_map* map, const int32_t* rc, const int32_t* ks, int32_t nq, int32_t reps, htg_tuning* out) {
+    HTG_GUARD_BEGIN
+    if (!map) return fail(HTG_E_INVALID, "null map");
+    if (!map->built) return fail(HTG_E_STATE, "map not built");
+    if (nq > 0 && (!rc || !ks)) return fail(HTG_E_INVALID, "null sample");
+    hipError_t e = hipSetDevice(map->ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    if ((e = hipStreamSynchronize(map->stream)) != hipSuccess) return hip_fail(e, "sync");
+    reps = std::max(1, reps);
+    using clk = std::chrono::steady_clock;
+    htg_tuning best = map->tune;
+    double best_t = 1e30;
+    for (int graph = 0; graph <= 1; ++graph)
+        for (int P : {16, 32, 64}) {
+            map->tune.pivot_block = P;
+            map->tune.use_graph = graph;
+            int rcode = map->build();  // warm-up (and capture)
+            if (rcode) return rcode;
+            double t = 1e30;
+            for (int r = 0; r < reps; ++r) {
+                const auto t0 = clk::now();
+                if ((rcode = map->build())) return rcode;
+                t = std::min(t, std::chrono::duration<double>(clk::now() - t0).count());
+            }
+            if (t < best_t) {
+                best_t = t;
+                best.pivot_block = P;
+                best.use_graph = graph;
+            }
+        }
+    map->tune.pivot_block = best.pivot_block;
+    map->tune.use_graph = best.use_graph;
+    int rcode = map->build();  // leave the map built with the chosen settings
+    if (rcode) return rcode;
+    if (nq > 0) {
+        long long total = 0;
+        for (int q = 0; q < nq; ++q) total += (long long)ks[q] * ks[q];
+        std::vector<uint16_t> tmp(size_t(std::max<long long>(total, 1)));
+        double best_q = 1e30;
+        for (int Z : {4, 8, 16}) {
+            map->tune.zq_tile = Z;
+            if ((rcode = map->query(rc, ks, nq, tmp.data(), map->qws))) return rcode;
+            double t = 1e30;
+            for (int r = 0; r < reps; ++r) {
+                const auto t0 = clk::now();
+                if ((rcode = map->query(rc, ks, nq, tmp.data(), map->qws))) return rcode;
+                t = std::min(t, std::chrono::duration<double>(clk::now() - t0).count());
+            }
+            if (t < best_q) {
+                best_q = t;
+                best.zq_tile = Z;
+            }
+        }
+    }
+    map->tune = best;
+    if (out) *out = best;
+    return HTG_OK;
+    HTG_GUARD_END
 }
 
 int htg_set_tuning(htg_map* map, const htg_tuning* t) {

@@ -106,7 +106,7 @@ struct LevelGeo {
     const uint8_t* d_live_map = nullptr;
     std::vector<int> seam_live;  // [W * g^2 + q]: passable seam ports of child q (rows a projection may use)
     DevBuf d_vu_rc, d_ext_idx, d_ext_child, d_ext_cport, d_seams, d_seam_list, d_seam_off, d_zcols, d_zsrc, d_zoff,
-        d_zlen, d_ext_zpos, d_ext_col, d_colpos, d_colext, d_vu_ext;
+        d_zlen, d_ext_zpos, d_ext_col, d_colpos, d_colext, d_vu_ext, d_all;  // d_all: 0..nn-1
     DevBuf DU, SU;
 };
 
@@ -243,6 +243,19 @@ struct Map {
     std::shared_ptr<Profiler> profp = std::make_shared<Profiler>();
     Profiler& prof_ref() { return *profp; }
     DirtySet dirty;  // filled by open_cells
+    // Build captured as one hipGraph (tune.use_graph); replayed while the
+    // launch signature (per-level pivot blocks and grid sizes) is unchanged.
+    hipGraphExec_t build_graph = nullptr;
+    std::vector<int> build_sig;
+    bool capturing = false;  // no timing events inside a capture
+    int prof_open(int cls, double bytes, double updates, KTimer& kt) {
+        kt = KTimer{};
+        return capturing ? -1 : prof_ref().open(cls, bytes, updates, kt);
+    }
+    // Build pivots per node of a level (host): the endpoints of seam pairs
+    // whose cells are both passable.
+    std::vector<int> build_pivot_counts(int lvl) const;
+    int enqueue_build(const std::vector<std::vector<int>>& hc);
     HostBuf live_h;  // live arrays of every level (pinned)
     DevBuf live_d;
     hipEvent_t live_evt = nullptr;  // recorded after the last upload of live_h

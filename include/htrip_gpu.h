@@ -217,11 +217,18 @@ int htg_server_replay(htg_server* s, int32_t n, const double* t_arrival, const i
 typedef struct htg_tuning {
     int32_t pivot_block;   /* pivots per closure block: 16, 32 or 64 */
     int32_t zq_tile;       /* viewpoints per query-projection work item: 1..16 */
-    int32_t use_graph;     /* capture per-level build launches in a hipGraph */
+    int32_t use_graph;     /* run the build as one captured hipGraph (replayed while its launch shape holds) */
     int32_t reserved;
 } htg_tuning;
 int htg_get_tuning(const htg_map* map, htg_tuning* out);
 int htg_set_tuning(htg_map* map, const htg_tuning* t);
+/* Autotuner: rebuilds the map with every pivot block (16/32/64), with and
+ * without graph capture, and runs the sample batch (nq queries, as in
+ * htg_query_batch) with every projection tile (4/8/16), reps timed runs
+ * each; keeps the fastest of each on the map and returns them in out.  The
+ * tables and results do not depend on the tuning.  No counterpart in the
+ * reference (whose kernels have no tunable shape). */
+int htg_autotune(htg_map* map, const int32_t* rc, const int32_t* ks, int32_t nq, int32_t reps, htg_tuning* out);
 
 #ifdef __cplusplus
 }

@@ -162,3 +162,25 @@ def test_cpp_dropin_program(gpu_ctx):
     res = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     print(res.stdout)
     assert res.returncode == 0, res.stdout + res.stderr
+
+
+@pytest.mark.parametrize("kind,n,b,g,seed", [("maze", 128, 16, 2, 31), ("street", 96, 8, 2, 32),
+                                              ("street", 64, 8, 4, 33)])
+def test_graph_build_and_tuning_keep_tables(gpu_ctx, kind, n, b, g, seed):
+    """The build captured as a hipGraph (and replayed), every pivot block, and
+    the autotuner's choices give the oracle's tables and BFS distances."""
+    print(f"graph build {kind} n={n} b={b} g={g} seed={seed}", flush=True)
+    cells = make(kind, n, seed)
+    O = Oracle("port", cells, b, g)
+    assert O.build()
+    G = GpuMap(cells, b, g)
+    for P in (16, 32, 64):
+        G.set_tuning(pivot_block=P, use_graph=1)
+        assert G.build()
+        assert G.build()  # replays the captured graph
+        assert_tables_equal(G, O)
+    vps = grids.sample_free(cells, 48, seed + 3)
+    t = G.autotune([vps, vps[:17]], reps=1)
+    assert t.pivot_block in (16, 32, 64) and t.zq_tile in (4, 8, 16) and t.use_graph in (0, 1)
+    assert_tables_equal(G, O)
+    np.testing.assert_array_equal(G.query(vps), grid_bfs(cells, vps))
