@@ -16,6 +16,14 @@ Modes
       the same map state for the served/plain ratio.
   plain -- one step = one plain per-call queryAllPairs (k = 64) through the
       drop-in C ABI (host viewpoints in, host matrix out, synchronous).
+  update -- onBatchCellsOpened latency on the serving map (robot batches).
+  small -- 256^2 street grid, b=16: build + queryAllPairs k=64.
+  build -- 512^2 maze (width 1), b=16: batched build (graph-captured,
+      autotuned) + queryAllPairs k=128.
+  reveal -- 512^2 street grid, b=8: 500 steps of 256 reveals, each followed by
+      queryAllPairs k=32, checked against the reference step by step.
+  multi -- 8 resident sessions of mixed shapes, mixed traffic (k <= 512,
+      reveals <= 1024 cells), BFS spot checks.
 
 Timing: request latency = completion - arrival on the server's clock
 (steady_clock); kernel durations from hipEvents recorded on the stream each
@@ -137,8 +145,48 @@ def cpu_baseline(gmap, cells, queries, budget_s, expect, served_checks=()):
         if spent >= budget_s:
             break
     log(f"cpu baseline: {done} queries in {spent:.2f}s, all equal to the GPU results")
-    return {"value": done / spent, "unit": "matrices/s", "cores": 1, "kind": "reference",
-            "sample": f"{done} queryAllPairs k={len(queries[0])} (1 thread, reference AVX2 sources) on the same map"}
+    one = {"value": done / spent, "unit": "matrices/s", "cores": 1, "kind": "reference",
+           "sample": f"{done} queryAllPairs k={len(queries[0])} (1 thread, reference AVX2 sources) on the same map"}
+    one["all_cores"] = cpu_all_cores(R, queries[:max(done, 1) * 4], expect)
+    return one
+
+
+def host_cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))  # the GPU box's CPU share
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_all_cores(R, queries, expect):
+    """The reference's queryAllPairs on every host core (independent queries
+    over std::thread, each with its own scratch), same exactness gate."""
+    api = R.api
+    cores = host_cores()
+    ks = np.array([len(v) for v in queries], dtype=np.int32)
+    allv = np.ascontiguousarray(np.concatenate([np.asarray(v, np.int32) for v in queries]), dtype=np.int32)
+    out = np.empty(int((ks.astype(np.int64) ** 2).sum()), dtype=np.uint16)
+    spent = api["time_queries"](R.h, allv, ks, len(ks), cores, out.ctypes.data)
+    o = 0
+    for q, k in enumerate(ks):
+        if not np.array_equal(out[o:o + k * k].reshape(k, k), expect[q]):
+            raise SystemExit("reference CPU result (all cores) differs from the GPU result")
+        o += int(k) * int(k)
+    log(f"cpu baseline, {cores} threads: {len(ks)} queries in {spent:.2f}s")
+    return {"value": len(ks) / spent, "unit": "matrices/s", "cores": cores, "kind": "reference",
+            "cpu": cpu_model(), "nproc": os.cpu_count(),
+            "sample": f"{len(ks)} queryAllPairs ({cores} threads over independent queries)"}
 
 
 def served_checks(tr, state0, base, n=4, kmax=96):
@@ -335,12 +383,274 @@ def run_update(args):
             "mean_stats": [float(x) for x in np.mean(np.array(stats), axis=0)], "kernels": kernels}
 
 
+def ref_oracle(cells, b, g):
+    """The compiled reference (oracle/_ref) built on the CPU, one thread: (oracle, seconds)."""
+    from oracle import oracle as orc
+
+    if not orc.available("reference"):
+        return None, None
+    R = orc.Oracle("reference", cells, b, g)
+    t0 = time.perf_counter()
+    assert R.build()
+    return R, time.perf_counter() - t0
+
+
+def ref_query_rate(R, queries, expect, budget_s):
+    """Reference queryAllPairs, one thread, over a bounded sample; exactness gate against expect."""
+    api = R.api
+    done, spent = 0, 0.0
+    for q, vps in enumerate(queries):
+        vps = np.ascontiguousarray(vps, dtype=np.int32)
+        ks = np.array([len(vps)], dtype=np.int32)
+        out = np.empty(len(vps) * len(vps), dtype=np.uint16)
+        spent += api["time_queries"](R.h, vps, ks, 1, 1, out.ctypes.data)
+        if not np.array_equal(out.reshape(len(vps), -1), expect[q]):
+            raise SystemExit("reference CPU result differs from the GPU result")
+        done += 1
+        if spent >= budget_s:
+            break
+    return done, spent
+
+
+def build_times(gmap, reps):
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        gmap.build()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
+def run_small(args):
+    """256^2 street-like grid (seed 1), b=16: build, then queryAllPairs at k=64."""
+    from htrip_amd import _lib, grids
+
+    cells = grids.street_grid(256, 1)
+    gmap = _lib.GpuMap(cells, 16, 2)
+    gmap.build()
+    build_ms = 1e3 * build_times(gmap, 5)
+    rate, dt, qs, res = plain_rate(gmap, cells, 64, args.steps, args.warmup)
+    log(f"256^2 street b=16: build {build_ms:.2f} ms, query k=64 {rate:.0f} matrices/s")
+    cpu = None
+    if not args.no_cpu:
+        R, tb = ref_oracle(cells, 16, 2)
+        if R is not None:
+            done, spent = ref_query_rate(R, qs, res, args.cpu_seconds)
+            cpu = {"value": done / spent, "unit": "matrices/s", "cores": 1, "kind": "reference",
+                   "sample": f"{done} queryAllPairs k=64 (1 thread)", "build_ms": 1e3 * tb,
+                   "all_cores": cpu_all_cores(R, qs[:max(done, 1) * 4], res)}
+    return {"metric": "k x k query matrices/s (plain GPU path, k=64, 256^2 street grid, b=16)", "value": rate,
+            "unit": "matrices/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": dt * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "u16", "data": "synthetic",
+            "config": {"workload": "256x256 street-like grid seed 1, b=16, g=2x2; build + queryAllPairs k=64",
+                       "mode": "small"},
+            "build_ms": build_ms, "cpu_baseline": cpu}
+
+
+def run_build(args):
+    """512^2 DFS maze (seed 2, corridor and wall width 1), b=16: the batched
+    build (every node of a level per launch, the whole build as one captured
+    graph, autotuned), then queryAllPairs at k=128."""
+    from htrip_amd import _lib, grids
+
+    cells = grids.maze_grid(512, 2, 1)
+    gmap = _lib.GpuMap(cells, 16, 2)
+    gmap.build()
+    default_ms = 1e3 * build_times(gmap, 5)
+    sample = [grids.sample_free(cells, 128, 900 + i) for i in range(4)]
+    t = gmap.autotune(sample, reps=3)
+    tuned_ms = 1e3 * build_times(gmap, 5)
+    log(f"512^2 maze b=16: build {default_ms:.2f} ms (default), {tuned_ms:.2f} ms autotuned "
+        f"(pivot block {t.pivot_block}, graph {t.use_graph}, projection tile {t.zq_tile})")
+    rate, dt, qs, res = plain_rate(gmap, cells, 128, args.steps, args.warmup)
+    log(f"query k=128: {rate:.0f} matrices/s")
+    gmap.profile(True)
+    gmap.build()
+    _, _, kernels = kernel_roofline(gmap)
+    cpu = None
+    if not args.no_cpu:
+        R, tb = ref_oracle(cells, 16, 2)
+        if R is not None:
+            done, spent = ref_query_rate(R, qs, res, args.cpu_seconds)
+            cpu = {"value": 1.0 / tb, "unit": "builds/s", "cores": 1, "kind": "reference",
+                   "sample": "one full build (1 thread, reference AVX2 sources)", "build_ms": 1e3 * tb,
+                   "query_k128_matrices_per_s": done / spent,
+                   "query_all_cores": cpu_all_cores(R, qs[:max(done, 1) * 4], res)}
+            log(f"reference build {1e3 * tb:.0f} ms, query k=128 {done / spent:.1f} matrices/s (1 thread)")
+    return {"metric": "full build time, 512^2 maze b=16 (batched, graph-captured, autotuned)",
+            "value": tuned_ms, "unit": "ms", "n_gpus": 1, "steps": 5, "warmup": 1, "ms_per_step": tuned_ms,
+            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "u16", "data": "synthetic",
+            "config": {"workload": "512x512 DFS maze seed 2 width 1, b=16, g=2x2; build + queryAllPairs k=128",
+                       "mode": "build"},
+            "default_build_ms": default_ms,
+            "tuning": {"pivot_block": t.pivot_block, "use_graph": t.use_graph, "zq_tile": t.zq_tile},
+            "query_k128_matrices_per_s": rate, "build_kernels": kernels, "cpu_baseline": cpu}
+
+
+def run_reveal(args):
+    """512^2 street-like grid (seed 3), b=8: observed area starts as a disk of
+    radius 64; 500 steps of 256 frontier-order reveals, each followed by
+    queryAllPairs at k=32.  The reference replays the same steps for a bounded
+    sample and every compared matrix must be equal."""
+    from htrip_amd import _lib, grids
+
+    truth = grids.street_grid(512, 3)
+    start = grids.sample_free(truth, 1, 31)[0]
+    obs = grids.disk_observed(truth, [tuple(start)], 64)
+    order = grids.frontier_order(truth, obs)
+    steps = min(500, len(order) // 256)
+    gmap = _lib.GpuMap(obs, 8, 2)
+    gmap.build()
+    state = obs.copy()
+    batches, views, lat = [], [], []
+    gmap.profile(True)
+    t_all = time.perf_counter()
+    for st in range(steps):
+        cells = order[st * 256:(st + 1) * 256]
+        state[cells[:, 0], cells[:, 1]] = 1
+        vps = grids.sample_free(state, 32, 5000 + st)
+        t0 = time.perf_counter()
+        gmap.open_cells(cells)
+        m = gmap.query(vps)
+        lat.append(time.perf_counter() - t0)
+        batches.append(cells)
+        views.append((vps, m))
+    total = time.perf_counter() - t_all
+    lat = np.array(lat) * 1e3
+    log(f"512^2 street b=8 reveal: {steps} steps, step p50 {np.percentile(lat, 50):.3f} ms, "
+        f"p99 {np.percentile(lat, 99):.3f} ms")
+    _, _, kernels = kernel_roofline(gmap)
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+
+        if orc.available("reference"):
+            R = orc.Oracle("reference", obs, 8, 2)
+            assert R.build()
+            spent, n = 0.0, 0
+            for st in range(steps):
+                t0 = time.perf_counter()
+                R.open_cells(batches[st])
+                got = R.query(views[st][0])
+                spent += time.perf_counter() - t0
+                if not np.array_equal(got, views[st][1]):
+                    raise SystemExit(f"reveal step {st}: GPU matrix differs from the reference")
+                n += 1
+                if spent >= args.cpu_seconds:
+                    break
+            cpu = {"value": n / spent, "unit": "steps/s", "cores": 1, "kind": "reference",
+                   "sample": f"first {n} steps (onBatchCellsOpened 256 cells + queryAllPairs k=32), all equal"}
+            log(f"reference: {n} steps in {spent:.2f}s, every matrix equal")
+    return {"metric": "reveal steps/s (onBatchCellsOpened 256 cells + queryAllPairs k=32), 512^2 street b=8",
+            "value": steps / total, "unit": "steps/s", "n_gpus": 1, "steps": steps, "warmup": 0,
+            "ms_per_step": total * 1e3 / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "u16", "data": "synthetic",
+            "config": {"workload": "512x512 street-like grid seed 3, b=8, g=2x2, disk radius 64, 256 frontier "
+                                   "reveals per step + queryAllPairs k=32", "mode": "reveal"},
+            "step_p50_ms": float(np.percentile(lat, 50)), "step_p99_ms": float(np.percentile(lat, 99)),
+            "kernels": kernels, "cpu_baseline": cpu}
+
+
+def run_multi(args):
+    """Eight resident sessions admitted by HBM budget: 2x 256^2 street (b=16),
+    2x 512^2 DFS maze width 1 and 2x width 16 (b=16), 2x 1024^2 street (b=8,
+    H=7).  Mixed traffic: queryAllPairs k log-uniform in [8, 512] and reveal
+    batches of 16..1024 cells (10%), 4 robots per map, Poisson arrivals."""
+    from htrip_amd import _lib, grids, serving
+
+    maps = [("street", 256, 16, 101), ("street", 256, 16, 102), ("maze1", 512, 16, 103), ("maze1", 512, 16, 104),
+            ("maze16", 512, 16, 105), ("maze16", 512, 16, 106), ("street", 1024, 8, 107), ("street", 1024, 8, 108)]
+    truths = []
+    for kind, n, b, seed in maps:
+        if kind == "street":
+            truths.append(grids.street_grid(n, seed))
+        else:
+            truths.append(grids.maze_grid(n, seed, 1 if kind == "maze1" else 16))
+    need = sum(_lib.footprint(b, 2, n, n) for _, n, b, _ in maps) * 2
+    srv = serving.Server(hbm_budget=int(need * 1.05), n_slots=args.slots, max_batch_queries=args.batch_queries,
+                         max_batch_viewpoints=args.batch_viewpoints, consistency=serving.SNAPSHOT)
+    log(f"8 sessions, HBM budget {need * 1.05 / 2**30:.1f} GiB (two copies each)")
+
+    def fresh():
+        sess = []
+        t0 = time.time()
+        for (kind, n, b, seed), truth in zip(maps, truths):
+            rv = serving.RobotReveal(truth, 4, n // 16, seed=seed, limit=20000)
+            sid = srv.add_map(rv.observed, b, 2)
+            sess.append((sid, rv, rv.observed.copy()))
+        log(f"sessions admitted and built in {time.time() - t0:.2f}s")
+        return sess
+
+    def traffic(sess, n_each, rate, seed):
+        trs = [serving.make_trace(sid, rv, n_each, rate / len(sess), seed + i, update_frac=0.1, k_lo=8, k_hi=512,
+                                  upd_lo=16, upd_hi=1024) for i, (sid, rv, _) in enumerate(sess)]
+        return serving.merge_traces(trs)
+
+    sess = fresh()
+    burst = traffic(sess, 64, 1e8, 700)
+    tb = srv.replay(burst)
+    cap = serving.latency_summary(burst, tb)["matrices_per_s"] / 0.9
+    for sid, _, _ in sess:
+        srv.remove_map(sid)
+    log(f"burst capacity ~{cap:.0f} requests/s")
+    trials = []
+    for frac in args.rate_fracs[:5]:
+        sess = fresh()
+        tr = traffic(sess, args.steps * 8, cap * frac, 800 + len(trials))
+        td = srv.replay(tr)
+        s = serving.latency_summary(tr, td)
+        s["rate"] = cap * frac
+        checks = multi_checks(tr, sess) if not args.no_cpu else 0
+        for sid, _, _ in sess:
+            srv.remove_map(sid)
+        trials.append(s)
+        log(f"rate {s['rate']:.0f}/s: {s['matrices_per_s']:.0f} matrices/s, p50 {s['p50_ms']:.3f} ms, "
+            f"p99 {s['p99_ms']:.3f} ms, updates p50 {s['update_p50_ms']} ms; {checks} spot checks equal BFS")
+    best = max(trials, key=lambda x: x["matrices_per_s"] if x["p99_ms"] < 10.0 else 0.0)
+    return {"metric": "k x k query matrices/s over 8 resident sessions (mixed k <= 512 and reveals)",
+            "value": best["matrices_per_s"], "unit": "matrices/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": 0, "ms_per_step": best["span_s"] * 1e3 / args.steps, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "u16", "data": "synthetic",
+            "config": {"workload": "8 sessions: 2x256^2 street b16, 2x512^2 maze w1 b16, 2x512^2 maze w16 b16, "
+                                   "2x1024^2 street b8; k~logU[8,512], 10% reveals of 16..1024 cells",
+                       "mode": "multi"},
+            "p50_ms": best["p50_ms"], "p99_ms": best["p99_ms"], "rate_trials": trials}
+
+
+def multi_checks(tr, sess, per_session=1, kmax=64):
+    """Spot checks: small queries of each session against BFS on the state with
+    exactly the update prefix the result reports."""
+    from oracle import oracle as orc
+
+    n = 0
+    for sid, _, state0 in sess:
+        mine = [i for i in range(len(tr.t)) if tr.session[i] == sid]
+        upd = [i for i in mine if tr.type[i] == 1]
+        got = 0
+        for i in mine:
+            if tr.type[i] != 0 or tr.count[i] > kmax:
+                continue
+            st = state0.copy()
+            for j in upd[:int(tr.version[i])]:
+                c = tr.cells_of(j)
+                st[c[:, 0], c[:, 1]] = 1
+            if not np.array_equal(orc.grid_bfs(st, tr.cells_of(i)), tr.query_result(i)):
+                raise SystemExit(f"session {sid}: served result differs from BFS on its map state")
+            n += 1
+            got += 1
+            if got == per_session:
+                break
+    return n
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--mode", default="served", choices=["plain", "served", "update"])
+    ap.add_argument("--mode", default="served",
+                    choices=["plain", "served", "update", "small", "build", "reveal", "multi"])
     ap.add_argument("--k", type=int, default=64)
     ap.add_argument("--window", type=int, default=64, help="requests per served step")
     ap.add_argument("--slots", type=int, default=3, help="query streams (the update stream makes one more)")
@@ -358,7 +668,8 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU (one MI355X per box); --gpus must be 1"}))
         sys.exit(2)
-    line = {"plain": run_plain, "served": run_served, "update": run_update}[args.mode](args)
+    line = {"plain": run_plain, "served": run_served, "update": run_update, "small": run_small, "build": run_build,
+            "reveal": run_reveal, "multi": run_multi}[args.mode](args)
     print(json.dumps(line))
 
 

@@ -278,3 +278,24 @@ def latency_summary(trace: Trace, t_done: np.ndarray) -> dict:
         "update_p50_ms": float(np.percentile(lat[~q], 50) * 1e3) if (~q).any() else None,
         "update_p99_ms": float(np.percentile(lat[~q], 99) * 1e3) if (~q).any() else None,
     }
+
+
+def merge_traces(traces) -> Trace:
+    """Interleave several sessions' traces by arrival time (each session keeps
+    its own request order)."""
+    pair_base = out_base = 0
+    parts = []
+    for tr in traces:
+        parts.append((tr, pair_base, out_base))
+        pair_base += len(tr.rc) // 2
+        out_base += len(tr.out)
+    t = np.concatenate([tr.t for tr in traces])
+    order = np.argsort(t, kind="stable")
+    cat = lambda f: np.concatenate([getattr(tr, f) for tr in traces])  # noqa: E731
+    rc_off = np.concatenate([tr.rc_off + pb for tr, pb, _ in parts])
+    out_off = np.concatenate([np.where(tr.out_off >= 0, tr.out_off + ob, -1) for tr, _, ob in parts])
+    merged = Trace(t[order], cat("session")[order].astype(np.int32), cat("type")[order].astype(np.int32),
+                   cat("count")[order].astype(np.int32), rc_off[order].astype(np.int64),
+                   np.ascontiguousarray(np.concatenate([tr.rc for tr in traces]), dtype=np.int32),
+                   out_off[order].astype(np.int64), np.zeros(max(out_base, 1), np.uint16))
+    return merged

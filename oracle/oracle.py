@@ -123,9 +123,8 @@ class Oracle:
 
     def open_cells(self, rc) -> None:
         rc = np.ascontiguousarray(rc, dtype=np.int32).reshape(-1, 2)
-        for r, c in rc:
-            if 0 <= r < self.cells.shape[0] and 0 <= c < self.cells.shape[1]:
-                self.cells[r, c] = 1
+        ok = (rc[:, 0] >= 0) & (rc[:, 1] >= 0) & (rc[:, 0] < self.cells.shape[0]) & (rc[:, 1] < self.cells.shape[1])
+        self.cells[rc[ok, 0], rc[ok, 1]] = 1
         self.api["open_cells"](self.h, rc, len(rc))
 
     def open_cell(self, r: int, c: int) -> None:
