@@ -49,6 +49,9 @@ PEAK_HBM_GBS = 8000.0
 PEAK_UPDATES = 256 * 64 * 2 * 2.4e9 / 2
 
 N, B, G, SEED, ROBOTS, RADIUS = 1024, 32, 2, 3, 16, 64
+# Timing events cost host time per launch, so the served and update modes time
+# one operation in PROFILE_EVERY (counters and durations from the same ones).
+PROFILE_EVERY = 8
 
 
 def log(*a):
@@ -219,7 +222,7 @@ def run_plain(args):
     t0 = time.time()
     gmap.build()
     log(f"build {1e3 * (time.time() - t0):.1f} ms (S={gmap.S} H={gmap.H})")
-    gmap.profile(True)
+    gmap.profile(True, every=PROFILE_EVERY)
     rate, dt, qs, res = plain_rate(gmap, cells, args.k, args.steps, args.warmup)
     log(f"plain: {args.steps} queries k={args.k}: {rate:.0f} matrices/s")
     roof, valu, kernels = kernel_roofline(gmap)
@@ -259,7 +262,7 @@ def run_served(args):
         sid = srv.add_map(rv.observed, B, G)
         build_ms = 1e3 * (time.time() - t1)
         smap = srv.map_of(sid, rv.observed, B, G)
-        smap.profile(True)
+        smap.profile(True, every=PROFILE_EVERY)
         srv.replay(serving.make_trace(sid, rv, 100, 1e6, 11, update_frac=0.0))
         return rv, sid, smap, build_ms
 
@@ -359,7 +362,7 @@ def run_update(args):
         m = serving.loguniform_k(rng, 16, 512)
         cells = rv.next_batch(rng.below(ROBOTS), m)
         if i == args.warmup:
-            gmap.profile(True)
+            gmap.profile(True, every=4)
         t0 = time.perf_counter()
         gmap.open_cells(cells)
         dt = time.perf_counter() - t0
@@ -661,7 +664,8 @@ def main():
     ap.add_argument("--batch-viewpoints", type=int, default=8192)
     ap.add_argument("--plain-steps", type=int, default=200)
     ap.add_argument("--consistency", default="snapshot", choices=["snapshot", "strict"])
-    ap.add_argument("--rate-fracs", type=float, nargs="+", default=[1.2, 1.05, 0.9, 0.75, 0.6, 0.45, 0.3, 0.2])
+    ap.add_argument("--rate-fracs", type=float, nargs="+",
+                    default=[2.0, 1.75, 1.5, 1.3, 1.15, 1.0, 0.85, 0.7, 0.55, 0.4, 0.25])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()

@@ -316,8 +316,9 @@ class GpuMap:
     def sync(self) -> None:
         check(self.lib.htg_sync(self.h))
 
-    def profile(self, on: bool = True) -> None:
-        check(self.lib.htg_profile_enable(self.h, 1 if on else 0))
+    def profile(self, on: bool = True, every: int = 1) -> None:
+        """Kernel timing; every > 1 samples one operation (query batch, update, build) in `every`."""
+        check(self.lib.htg_profile_enable(self.h, max(1, every) if on else 0))
 
     def kernel_stats(self, cls: int) -> dict:
         s = KernelStats()

@@ -569,6 +569,7 @@ int Map::enqueue_build(const std::vector<std::vector<int>>& hc) {
 int Map::build() {
     hipError_t e;
     int rc;
+    timing_now = prof_ref().take();
     std::vector<std::vector<int>> hc(size_t(H) + 1);
     std::vector<int> sig;
     int ws_lvl = 0;
@@ -643,6 +644,7 @@ static int debug_level() {
 // closes over the rows that decreased -- exactly the reference's pivot set K
 // (src/internal_node.cpp:332-413).
 int Map::open_cells(const int32_t* rcs, int n) {
+    timing_now = prof_ref().take();
     stats = htg_batch_stats{0, 0, 0};
     dirty.clear();
     dirty.nodes.resize(H);
@@ -800,7 +802,7 @@ int Map::open_cells(const int32_t* rcs, int n) {
                         int(newly.size()), stream);
     const double leaf_bytes = (double(b) * b * b * b + double(4 * b - 4) * (4 * b - 4)) * 2 + double(b) * b;
     KTimer lkt;
-    const int lpi = prof_ref().open(HTG_K_LEAF_APSP, leaf_bytes * nd, 0, lkt);
+    const int lpi = prof_open(HTG_K_LEAF_APSP, leaf_bytes * nd, 0, lkt);
     const bool lla = launch_leaf_apsp(static_cast<const uint8_t*>(d_pass.p), S, b, nl,
                                       reinterpret_cast<const int*>(up + o_leaves), nd, static_cast<uint16_t*>(leafD.p),
                                       static_cast<uint16_t*>(leafS.p), chg, stream, lkt);
@@ -1187,7 +1189,7 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
     uint16_t* dout = static_cast<uint16_t*>(ws.d_out.p);
     const uint16_t* dleafD = static_cast<const uint16_t*>(leafD.p);
     const auto th2 = std::chrono::steady_clock::now();
-    ws.counted = prof_ref().on;
+    ws.counted = prof_ref().take();
     unsigned long long* wstat = nullptr;
     if (ws.counted) {
         if ((rc = ws.d_wstat.ensure(16)) || (rc = ws.h_wstat.ensure(16))) return rc;
@@ -1208,7 +1210,7 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
         const int ne = l < H ? L.d.ne : 0;  // no lift above the root
         // bytes and updates of the projection are counted on the device
         KTimer kt;
-        const int pi = prof_ref().open(HTG_K_Q_PROJECT, 0.0, 0.0, kt);
+        const int pi = ws.counted ? prof_ref().open(HTG_K_Q_PROJECT, 0.0, 0.0, kt) : -1;
         const size_t part_stride = nsl * L.ldp;
         const bool la = launch_q_z(LP.M, L.d.ld, L.d.pc, static_cast<const uint16_t*>(L.DU.p), L.node_stride,
                    static_cast<const int*>(L.d_seam_list.p), static_cast<const int*>(L.d_seam_off.p),
@@ -1220,7 +1222,8 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
                    static_cast<const int*>(L.d_ext_cport.p), Zl, L.ldz, Xn, ne, wstat, st, kt);
         prof_ref().close(pi, la);
         if (LP.ksplit > 1) {
-            const int pf = prof_ref().open(HTG_K_Q_LIFT, double(ns) * (L.nz + 2.0 * ne + L.d.pc) * 2, 0, kt);
+            const int pf =
+                ws.counted ? prof_ref().open(HTG_K_Q_LIFT, double(ns) * (L.nz + 2.0 * ne + L.d.pc) * 2, 0, kt) : -1;
             const bool lf = launch_q_fold(ns, L.nz, L.ldz, static_cast<const int*>(L.d_zsrc.p), L.ldp, LP.ksplit,
                           static_cast<const uint16_t*>(ws.Zp.p), part_stride, srcs, L.d.s, L.d.nt, L.ngrp,
                           L.d_live_map, Zl, ne, static_cast<const int*>(L.d_ext_col.p),
@@ -1231,7 +1234,7 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
     }
     if (ns > 0 && !P.mtiles.empty()) {
         KTimer kt;
-        const int pm = prof_ref().open(HTG_K_Q_MEET, P.m_bytes, P.m_updates, kt);
+        const int pm = ws.counted ? prof_ref().open(HTG_K_Q_MEET, P.m_bytes, P.m_updates, kt) : -1;
         const bool lm = launch_q_meet(I(o_tiles), int(P.mtiles.size() / 8), reinterpret_cast<const int4*>(dp + o_groups),
                                       I(o_glist), I(o_zgeo), g2, I(o_si), I(o_qk),
                                       reinterpret_cast<const long long*>(dp + o_qout), Qb, dout, st, kt);
@@ -1632,7 +1635,10 @@ int htg_read_node_SU(htg_map* map, int32_t lvl, int32_t idx, uint16_t* out) {
 
 int htg_profile_enable(htg_map* map, int on) {
     if (!map) return fail(HTG_E_INVALID, "null map");
+    if (on < 0) return fail(HTG_E_INVALID, "sampling period must be >= 0");
     map->prof_ref().reset();
+    map->prof_ref().every = on > 1 ? on : 1;
+    map->prof_ref().tick = 0;
     map->prof_ref().on = on != 0;
     return HTG_OK;
 }

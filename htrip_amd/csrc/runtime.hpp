@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <cstring>
 #include <memory>
@@ -184,6 +185,11 @@ struct QueryWS {
 struct Profiler {
     std::mutex mu;
     bool on = false;
+    int every = 1;                // time one operation (query batch, update, build) in `every`
+    std::atomic<long long> tick{0};
+    // Whether the next operation carries timing events (counters and
+    // durations then come from the same sampled operations).
+    bool take() { return on && (every <= 1 || tick.fetch_add(1) % every == 0); }
     struct Pending {
         int cls;
         hipEvent_t a, b;
@@ -247,10 +253,11 @@ struct Map {
     // launch signature (per-level pivot blocks and grid sizes) is unchanged.
     hipGraphExec_t build_graph = nullptr;
     std::vector<int> build_sig;
-    bool capturing = false;  // no timing events inside a capture
+    bool capturing = false;   // no timing events inside a capture
+    bool timing_now = false;  // the current build / update was sampled for timing
     int prof_open(int cls, double bytes, double updates, KTimer& kt) {
         kt = KTimer{};
-        return capturing ? -1 : prof_ref().open(cls, bytes, updates, kt);
+        return capturing || !timing_now ? -1 : prof_ref().open(cls, bytes, updates, kt);
     }
     // Build pivots per node of a level (host): the endpoints of seam pairs
     // whose cells are both passable.

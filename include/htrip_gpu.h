@@ -147,6 +147,9 @@ typedef struct htg_kernel_stats {
     double bytes;   /* algorithmic bytes moved (unique reads + writes) */
     double updates; /* min-plus updates (one saturating add + one min each) */
 } htg_kernel_stats;
+/* on: 0 off, 1 every operation, n > 1 one query batch / update / build in n
+ * (timing events cost host time per launch; counters and durations come from
+ * the same sampled operations). */
 int htg_profile_enable(htg_map* map, int on);
 int htg_profile_get(htg_map* map, int kernel_class, htg_kernel_stats* out);
 
