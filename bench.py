@@ -61,7 +61,7 @@ def log(*a):
 def kernel_stats_of(gmap):
     from htrip_amd import _lib
 
-    return {c: gmap.kernel_stats(c) for c in range(6)}
+    return {c: gmap.kernel_stats(c) for c in range(len(_lib.KERNEL_NAMES))}
 
 
 def roofline_of(stats):
@@ -79,7 +79,7 @@ def roofline_of(stats):
             "traffic": None, "kernel": "k_q_z", "avg_launch_ms": avg_ms, "launches": z["launches"]}
     valu = {"kernel": "k_q_z", "achieved_updates_per_s": ups, "peak_updates_per_s": PEAK_UPDATES,
             "frac": ups / PEAK_UPDATES}
-    kernels = {_lib.KERNEL_NAMES[c]: stats[c] for c in range(6)}
+    kernels = {_lib.KERNEL_NAMES[c]: stats[c] for c in range(len(_lib.KERNEL_NAMES))}
     return roof, valu, kernels
 
 
@@ -348,6 +348,9 @@ def run_served(args):
     }
 
 
+PIVOT_SETS = {"cover": 0, "all": 1}
+
+
 def run_update(args):
     """Incremental reveal latency on the serving map: robot batches of 16..512 cells."""
     from htrip_amd import _lib, grids, serving
@@ -355,12 +358,16 @@ def run_update(args):
     truth = grids.street_grid(N, SEED)
     rv = serving.RobotReveal(truth, ROBOTS, RADIUS, seed=5, limit=40000)
     gmap = _lib.GpuMap(rv.observed, B, G)
+    gmap.set_tuning(update_pivots=PIVOT_SETS[args.update_pivots])
     gmap.build()
     rng = grids.SplitMix64(77)
     lat, sizes, stats = [], [], []
     for i in range(args.warmup + args.steps):
-        m = serving.loguniform_k(rng, 16, 512)
-        cells = rv.next_batch(rng.below(ROBOTS), m)
+        parts = []
+        for _ in range(args.update_requests):  # the server coalesces queued requests like this
+            m = serving.loguniform_k(rng, 16, 512)
+            parts.append(rv.next_batch(rng.below(ROBOTS), m))
+        cells = np.vstack(parts)
         if i == args.warmup:
             gmap.profile(True, every=4)
         t0 = time.perf_counter()
@@ -376,12 +383,14 @@ def run_update(args):
         f"mean cells {np.mean(sizes):.0f}")
     for k, v in kernels.items():
         log(f"  {k:18s} launches {v['launches']:6d} total {v['total_ms']:9.2f} ms")
-    return {"metric": "onBatchCellsOpened latency (1024^2 street, b=32, robot batches 16..512 cells)",
+    per = f"{args.update_requests} robot batches" if args.update_requests > 1 else "a robot batch"
+    return {"metric": f"onBatchCellsOpened latency (1024^2 street, b=32, {per} of 16..512 cells)",
             "value": float(np.median(lat) * 1e3), "unit": "ms", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": float(lat.mean() * 1e3), "higher_is_better": False,
             "scaling": "weak", "vs_baseline": None, "dtype": "u16", "data": "synthetic",
             "config": {"workload": "1024x1024 street seed 3, b=32, 16 robots, reveal batches logU[16,512]",
-                       "mode": "update"},
+                       "mode": "update", "update_pivots": args.update_pivots,
+                       "requests_per_update": args.update_requests},
             "p99_ms": float(np.percentile(lat, 99) * 1e3), "mean_cells": float(np.mean(sizes)),
             "mean_stats": [float(x) for x in np.mean(np.array(stats), axis=0)], "kernels": kernels}
 
@@ -504,6 +513,7 @@ def run_reveal(args):
     order = grids.frontier_order(truth, obs)
     steps = min(500, len(order) // 256)
     gmap = _lib.GpuMap(obs, 8, 2)
+    gmap.set_tuning(update_pivots=PIVOT_SETS[args.update_pivots])
     gmap.build()
     state = obs.copy()
     batches, views, lat = [], [], []
@@ -550,7 +560,8 @@ def run_reveal(args):
             "ms_per_step": total * 1e3 / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "u16", "data": "synthetic",
             "config": {"workload": "512x512 street-like grid seed 3, b=8, g=2x2, disk radius 64, 256 frontier "
-                                   "reveals per step + queryAllPairs k=32", "mode": "reveal"},
+                                   "reveals per step + queryAllPairs k=32", "mode": "reveal",
+                       "update_pivots": args.update_pivots},
             "step_p50_ms": float(np.percentile(lat, 50)), "step_p99_ms": float(np.percentile(lat, 99)),
             "kernels": kernels, "cpu_baseline": cpu}
 
@@ -660,6 +671,11 @@ def main():
     ap.add_argument("--update-cu", type=int, default=0, help="percent of CUs for update kernels (0 = all)")
     ap.add_argument("--update-frac", type=float, default=0.1, help="share of update requests in the stream")
     ap.add_argument("--update-tick-us", type=int, default=0, help="snapshot mode: minimum spacing of update batches")
+    ap.add_argument("--update-pivots", default="cover", choices=sorted(PIVOT_SETS),
+                    help="update and reveal modes: close over a vertex cover of the decreased entries, or over "
+                         "every decreased row (the reference's pivot set)")
+    ap.add_argument("--update-requests", type=int, default=1,
+                    help="update mode: robot requests coalesced into each onBatchCellsOpened call")
     ap.add_argument("--batch-queries", type=int, default=64)
     ap.add_argument("--batch-viewpoints", type=int, default=8192)
     ap.add_argument("--plain-steps", type=int, default=200)

@@ -27,10 +27,11 @@ EXPORTS = [
     "htg_get_tuning", "htg_set_tuning", "htg_autotune",
 ]
 
-K_LEAF_APSP, K_CLOSURE_PIVOTS, K_CLOSURE_UPDATE, K_Q_PROJECT, K_Q_MEET, K_Q_LIFT = range(6)
+K_LEAF_APSP, K_CLOSURE_PIVOTS, K_CLOSURE_UPDATE, K_Q_PROJECT, K_Q_MEET, K_Q_LIFT, K_COVER_RELAX, K_COVER_PAIRS = range(8)
 KERNEL_NAMES = {
     K_LEAF_APSP: "k_leaf_apsp", K_CLOSURE_PIVOTS: "k_closure_pivots", K_CLOSURE_UPDATE: "k_closure_update",
-    K_Q_PROJECT: "k_q_z", K_Q_MEET: "k_q_meet", K_Q_LIFT: "k_q_fold",
+    K_Q_PROJECT: "k_q_z", K_Q_MEET: "k_q_meet", K_Q_LIFT: "k_q_fold", K_COVER_RELAX: "k_cover_relax",
+    K_COVER_PAIRS: "k_cover_pairs",
 }
 
 
@@ -62,7 +63,7 @@ class KernelStats(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("pivot_block", C.c_int32), ("zq_tile", C.c_int32), ("use_graph", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("update_pivots", C.c_int32)]
 
 
 _lib = None

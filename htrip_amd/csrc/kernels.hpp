@@ -39,8 +39,13 @@ void launch_seam_relax(int nV, int ld, int s, int nt, const int2* seams, int n_s
                        uint8_t* flags, hipStream_t st);
 
 // Explicit seam edges (absolute V_U pairs per node, from the update path).
+// flags: with plane > 0, three planes of n_list * nV bytes (C marks, B marks,
+// new-port mask) for the cover of an incremental update; plane == 0 marks
+// every decreased row as a pivot.
 void launch_edge_relax(int nV, int ld, const int4* edges /* node slot, u, v, w */, int n_edges,
-                       const int* node_list, uint16_t* DU, size_t node_stride, uint8_t* flags, hipStream_t st);
+                       const int* node_list, uint16_t* DU, size_t node_stride, uint8_t* flags, size_t plane,
+                       hipStream_t st);
+void launch_mark_u8(const int2* items /* slot, port */, int n, int nV, uint8_t* plane, hipStream_t st);
 
 // D_U child blocks := min(D_U block, child summary) for children whose mask bit
 // is set; flags marks rows that decreased.
@@ -58,9 +63,20 @@ void launch_child_relax(int lvl, int g, int pc, int nV, int ld, int nt, const ui
 void launch_child_relax_live(int g, int pc, int nV, int ld, int nt, const uint16_t* child_sum, size_t child_stride,
                              const int* node_list, const int* child_slot, const int* child_changed, int n_list,
                              const int* live8_cnt, const int* live8_list, int ng8, int max_live8, uint16_t* DU,
-                             size_t node_stride, uint8_t* flags, hipStream_t st);
+                             size_t node_stride, uint8_t* flags, size_t plane, hipStream_t st);
+// C marks -> pivots, counts[slot]; with plane > 0 also B-only marks -> bpiv,
+// counts[n_list + slot].
 void launch_compact_flags_live(int nV, const int* node_list, int n_list, const int* live8_cnt, const int* live8_list,
-                               int ng8, uint8_t* flags, int* pivots, int piv_stride, int* counts, hipStream_t st);
+                               int ng8, uint8_t* flags, size_t plane, int* pivots, int* bpiv, int piv_stride,
+                               int* counts, hipStream_t st);
+// Pre-relaxation of the cover rows C through the uncovered endpoints B
+// (counts as left by launch_compact_flags_live), then the C x C second hop.
+bool launch_cover_relax(int nV, int ld, const int* node_list, int n_list, const int* counts, const int* cpiv,
+                        const int* bpiv, int piv_stride, const int* live8_cnt, const int* live8_list, int ng8,
+                        int max_live8, int max_c, uint16_t* DU, size_t node_stride, hipStream_t st, KTimer kt = {});
+bool launch_cover_pairs(int nV, int ld, const int* node_list, int n_list, const int* counts, const int* cpiv,
+                        const int* bpiv, int piv_stride, int max_c, uint16_t* DU, size_t node_stride, hipStream_t st,
+                        KTimer kt = {});
 void launch_project_su_live(int nV, int ld, int ne, const int* vu_ext, const int* node_list, const int* counts,
                             int n_list, const int* live8_cnt, const int* live8_list, int ng8, int max_live8,
                             const uint16_t* DU, size_t node_stride, uint16_t* SU, int* changed, hipStream_t st);

@@ -209,9 +209,21 @@ void launch_seam_relax(int nV, int ld, int s, int nt, const int2* seams, int n_s
 
 // Explicit edges (node slot, u, v, w): D_U[u][v] = D_U[v][u] = min(., w)
 // (src/internal_node.cpp:356-365).  The update path lists each pair once.
+// Cover marks (incremental reveal).  With `plane` > 0 the flag buffer holds
+// three planes of n_list * nV bytes: C marks, B marks, and the new-port mask.
+// A decreased entry (v1, v2) marks row v1 into C when v1 is a new port or
+// neither endpoint is; an old row whose partner is new goes to B, the
+// uncovered endpoints.  Every decreased pair then has an endpoint in C.  With
+// `plane` == 0 every decreased row is a C mark (the reference's pivot set).
+__device__ __forceinline__ void cover_mark(uint8_t* flags, size_t plane, size_t row_off, bool self_new,
+                                           bool other_new) {
+    if (plane == 0 || self_new || !other_new) flags[row_off] = 1;
+    else flags[plane + row_off] = 1;
+}
+
 __global__ __launch_bounds__(256) void k_edge_relax(int nV, int ld, const int4* __restrict__ edges, int n_edges,
                                                     const int* __restrict__ node_list, uint16_t* __restrict__ DU,
-                                                    size_t node_stride, uint8_t* __restrict__ flags) {
+                                                    size_t node_stride, uint8_t* __restrict__ flags, size_t plane) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n_edges) return;
     const int4 e = edges[i];
@@ -221,16 +233,32 @@ __global__ __launch_bounds__(256) void k_edge_relax(int nV, int ld, const int4* 
     if (w < uv) {
         uv = w;
         D[size_t(e.z) * ld + e.y] = w;
-        flags[size_t(e.x) * nV + e.y] = 1;
-        flags[size_t(e.x) * nV + e.z] = 1;
+        const uint8_t* nw = flags + 2 * plane + size_t(e.x) * nV;
+        const bool ny = plane && nw[e.y], nz = plane && nw[e.z];
+        cover_mark(flags, plane, size_t(e.x) * nV + e.y, ny, nz);
+        cover_mark(flags, plane, size_t(e.x) * nV + e.z, nz, ny);
     }
 }
 
 void launch_edge_relax(int nV, int ld, const int4* edges, int n_edges, const int* node_list, uint16_t* DU,
-                       size_t node_stride, uint8_t* flags, hipStream_t st) {
+                       size_t node_stride, uint8_t* flags, size_t plane, hipStream_t st) {
     if (n_edges == 0) return;
     hipLaunchKernelGGL(k_edge_relax, dim3(unsigned((n_edges + 255) / 256)), dim3(256), 0, st, nV, ld, edges, n_edges,
-                       node_list, DU, node_stride, flags);
+                       node_list, DU, node_stride, flags, plane);
+}
+
+// items = (slot, port): plane[slot * nV + port] = 1.
+__global__ __launch_bounds__(256) void k_mark_u8(const int2* __restrict__ items, int n, int nV,
+                                                 uint8_t* __restrict__ plane) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int2 it = items[i];
+    plane[size_t(it.x) * nV + it.y] = 1;
+}
+
+void launch_mark_u8(const int2* items, int n, int nV, uint8_t* plane, hipStream_t st) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_mark_u8, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, items, n, nV, plane);
 }
 
 // Child decreases (src/internal_node.cpp:343-354) expressed as an elementwise
@@ -284,7 +312,7 @@ __global__ __launch_bounds__(256) void k_child_relax_live(int g, int pc, int nV,
                                                           const int* __restrict__ live8_cnt,
                                                           const int* __restrict__ live8_list, int ng8,
                                                           uint16_t* __restrict__ DU, size_t node_stride,
-                                                          uint8_t* __restrict__ flags) {
+                                                          uint8_t* __restrict__ flags, size_t plane) {
     const int i = blockIdx.y;
     const int node = node_list[i];
     const int n8 = live8_cnt[node];
@@ -304,7 +332,9 @@ __global__ __launch_bounds__(256) void k_child_relax_live(int g, int pc, int nV,
     const int cidx = (ty * g + q / g) * child_nt + (tx * g + q % g);
     const uint16_t* Srow = child_sum + size_t(cidx) * child_stride + size_t(v1 - lo) * pc;
     uint16_t* Drow = DU + size_t(node) * node_stride + size_t(v1) * ld;
-    bool dec = false;
+    const uint8_t* nw = flags + 2 * plane + size_t(i) * nV;
+    const bool n1 = plane && nw[v1];
+    bool dec_c = false, dec_b = false;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         const int v2 = c0 + k;
@@ -312,63 +342,233 @@ __global__ __launch_bounds__(256) void k_child_relax_live(int g, int pc, int nV,
         const uint16_t w = Srow[v2 - lo];
         if (w < Drow[v2]) {
             Drow[v2] = w;
-            dec = true;
+            if (!plane || n1 || !nw[v2]) dec_c = true;
+            else dec_b = true;
         }
     }
-    if (dec) flags[size_t(i) * nV + v1] = 1;
+    if (dec_c) flags[size_t(i) * nV + v1] = 1;
+    if (dec_b) flags[plane + size_t(i) * nV + v1] = 1;
 }
 
 void launch_child_relax_live(int g, int pc, int nV, int ld, int nt, const uint16_t* child_sum, size_t child_stride,
                              const int* node_list, const int* child_slot, const int* child_changed, int n_list,
                              const int* live8_cnt, const int* live8_list, int ng8, int max_live8, uint16_t* DU,
-                             size_t node_stride, uint8_t* flags, hipStream_t st) {
+                             size_t node_stride, uint8_t* flags, size_t plane, hipStream_t st) {
     if (n_list == 0 || max_live8 == 0) return;
     const long long total = (long long)max_live8 * max_live8 * 8;
     const dim3 grid{unsigned((total + 255) / 256), unsigned(n_list), 1u};
     hipLaunchKernelGGL(k_child_relax_live, grid, dim3(256), 0, st, g, pc, nV, ld, nt, nt * g, child_sum, child_stride,
-                       node_list, child_slot, child_changed, live8_cnt, live8_list, ng8, DU, node_stride, flags);
+                       node_list, child_slot, child_changed, live8_cnt, live8_list, ng8, DU, node_stride, flags,
+                       plane);
 }
 
 // Row flags -> ascending pivot list over the live octets (flags are only
 // ever set on live rows); clears the flags it reads.  One workgroup per node.
-__global__ __launch_bounds__(256) void k_compact_flags_live(int nV, const int* __restrict__ node_list,
+// With `plane` > 0 the B marks of rows outside C form a second list (bpiv,
+// counts[n_list + slot]).
+__global__ __launch_bounds__(256) void k_compact_flags_live(int nV, const int* __restrict__ node_list, int n_list,
                                                             const int* __restrict__ live8_cnt,
                                                             const int* __restrict__ live8_list, int ng8,
-                                                            uint8_t* __restrict__ flags, int* __restrict__ pivots,
+                                                            uint8_t* __restrict__ flags, size_t plane,
+                                                            int* __restrict__ pivots, int* __restrict__ bpiv,
                                                             int piv_stride, int* __restrict__ counts) {
-    __shared__ int wsum[4];
+    __shared__ int wsum[2][4];
     const int node = node_list[blockIdx.x];
     const int n = 8 * live8_cnt[node];
     const int* l8 = live8_list + size_t(node) * ng8;
     uint8_t* f = flags + size_t(blockIdx.x) * nV;
+    uint8_t* fb = plane ? f + plane : nullptr;
     int* out = pivots + size_t(blockIdx.x) * piv_stride;
+    int* outb = bpiv ? bpiv + size_t(blockIdx.x) * piv_stride : nullptr;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    int base = 0;
+    int base = 0, base_b = 0;
     for (int c0 = 0; c0 < n; c0 += 256) {
         const int e = c0 + tid;
         const int v = e < n ? 8 * l8[e >> 3] + (e & 7) : nV;
         const bool on = v < nV && f[v] != 0;
-        const unsigned long long m = __ballot(on);
-        const int before = __popcll(m & ((1ull << lane) - 1ull));
-        if (lane == 0) wsum[wave] = __popcll(m);
+        const bool on_b = fb && v < nV && !on && fb[v] != 0;
+        const unsigned long long m = __ballot(on), mb = __ballot(on_b);
+        const unsigned long long below = (1ull << lane) - 1ull;
+        if (lane == 0) {
+            wsum[0][wave] = __popcll(m);
+            wsum[1][wave] = __popcll(mb);
+        }
         __syncthreads();
-        int off = base;
-        for (int w = 0; w < wave; ++w) off += wsum[w];
+        int off = base, off_b = base_b;
+        for (int w = 0; w < wave; ++w) off += wsum[0][w], off_b += wsum[1][w];
         if (on) {
-            out[off + before] = v;
+            out[off + __popcll(m & below)] = v;
             f[v] = 0;
         }
-        base += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        if (on_b) outb[off_b + __popcll(mb & below)] = v;
+        if (fb && v < nV) fb[v] = 0;
+        base += wsum[0][0] + wsum[0][1] + wsum[0][2] + wsum[0][3];
+        base_b += wsum[1][0] + wsum[1][1] + wsum[1][2] + wsum[1][3];
         __syncthreads();
     }
-    if (tid == 0) counts[blockIdx.x] = base;
+    if (tid == 0) {
+        counts[blockIdx.x] = base;
+        if (fb) counts[n_list + blockIdx.x] = base_b;
+    }
 }
 
 void launch_compact_flags_live(int nV, const int* node_list, int n_list, const int* live8_cnt, const int* live8_list,
-                               int ng8, uint8_t* flags, int* pivots, int piv_stride, int* counts, hipStream_t st) {
+                               int ng8, uint8_t* flags, size_t plane, int* pivots, int* bpiv, int piv_stride,
+                               int* counts, hipStream_t st) {
     if (n_list == 0) return;
-    hipLaunchKernelGGL(k_compact_flags_live, dim3(unsigned(n_list)), dim3(256), 0, st, nV, node_list, live8_cnt,
-                       live8_list, ng8, flags, pivots, piv_stride, counts);
+    hipLaunchKernelGGL(k_compact_flags_live, dim3(unsigned(n_list)), dim3(256), 0, st, nV, node_list, n_list,
+                       live8_cnt, live8_list, ng8, flags, plane, pivots, plane ? bpiv : nullptr, piv_stride, counts);
+}
+
+// Pre-relaxation of the cover rows through the uncovered endpoints, before
+// the closure over C (incremental reveal):
+//   hop 1:  D[c][j]   = min(D[c][j],   min_{b in B} D[c][b] + D[b][j])   c in C, live j
+//   hop 2:  D[c1][c2] = min(D[c1][c2], min_{b in B} D[c1][b] + D[c2][b]) c1, c2 in C
+// Every shortest path of the updated node graph alternates closed old
+// distances and decreased entries; each decreased entry has an endpoint in
+// C, so between two consecutive C vertices (or a path end and its nearest C
+// vertex) lie at most two vertices of B.  Hop 1 folds one of them into the C
+// rows, hop 2 (on rows already relaxed, so D[c2][b] <= D[b][c2]) the second;
+// after both, the closure over C alone is exact for every pair.  Hop 1 only
+// reads B rows; a C row may be read by one workgroup while another lowers
+// it, which only ever substitutes a smaller valid path length.  Hop 2 reads
+// B columns and writes C columns.
+//
+// Both hops are one rectangular min-plus product, tiled like the closure
+// update: a workgroup owns 64 C rows x 16 column octets (thread = 4 rows x 8
+// columns, two u16 lanes per v_pk op) and streams the inner set B through LDS
+// in chunks of 32: dup'd A[b][row] = D[row][b] and X[b][octet] (hop 1: the B
+// row's live octet; hop 2: D[c2][b] for 8 consecutive cover entries c2).
+// Grid: x = row tile * column tiles + column tile, y = node slot.
+constexpr int kCoverRows = 64, kCoverOct = 16, kCoverK = 32;
+
+template <bool HOP2>
+__global__ __launch_bounds__(256) void k_cover_product(int nV, int ld, const int* __restrict__ node_list, int n_list,
+                                                       const int* __restrict__ counts, const int* __restrict__ cpiv,
+                                                       const int* __restrict__ bpiv, int piv_stride, int col_tiles,
+                                                       const int* __restrict__ live8_cnt,
+                                                       const int* __restrict__ live8_list, int ng8,
+                                                       uint16_t* __restrict__ DU, size_t node_stride) {
+    __shared__ uint4 A4[kCoverK][kCoverRows / 4];  // dup16(D[row][b]): 4 rows of a thread per uint4
+    __shared__ uint4 X4[kCoverK][kCoverOct];
+    __shared__ int rowv[kCoverRows], colv[kCoverOct * 8];
+    const int slot = blockIdx.y;
+    const int nc = counts[slot], nb = counts[n_list + slot];
+    const int I = blockIdx.x / col_tiles, J = blockIdx.x % col_tiles;
+    if (nb == 0 || I * kCoverRows >= nc) return;
+    const int node = node_list[slot];
+    const int n_cols = HOP2 ? nc : 8 * live8_cnt[node];  // columns this node has
+    if (J * kCoverOct * 8 >= n_cols) return;
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    const int* cl = cpiv + size_t(slot) * piv_stride;
+    const int* bl = bpiv + size_t(slot) * piv_stride;
+    uint16_t* D = DU + size_t(node) * node_stride;
+    if (tid < kCoverRows) rowv[tid] = I * kCoverRows + tid < nc ? cl[I * kCoverRows + tid] : -1;
+    if (tid < kCoverOct * 8) {
+        const int e = J * kCoverOct * 8 + tid;  // column entry of this tile
+        int v = -1;
+        if (HOP2) v = e < nc ? cl[e] : -1;
+        else if (e < n_cols) v = 8 * live8_list[size_t(node) * ng8 + (e >> 3)] + (e & 7);
+        colv[tid] = v < nV ? v : -1;
+    }
+    __syncthreads();
+    // accumulators: rows ty*4 .. +3, columns colv[tx*8 .. +7]
+    uint32_t acc[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int row = rowv[ty * 4 + r];
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            const int c0 = colv[tx * 8 + 2 * h], c1 = colv[tx * 8 + 2 * h + 1];
+            const uint32_t lo = row >= 0 && c0 >= 0 ? D[size_t(row) * ld + c0] : kInf;
+            const uint32_t hi = row >= 0 && c1 >= 0 ? D[size_t(row) * ld + c1] : kInf;
+            acc[r][h] = lo | (hi << 16);
+        }
+    }
+    for (int k0 = 0; k0 < nb; k0 += kCoverK) {
+        const int m = min(kCoverK, nb - k0);
+        __syncthreads();
+        // A: 32 x 64 gathered entries D[row][b]
+        for (int e = tid; e < kCoverK * kCoverRows; e += 256) {
+            const int k = e / kCoverRows, r = e % kCoverRows;
+            const int row = rowv[r];
+            const uint32_t v = k < m && row >= 0 ? dup16(D[size_t(row) * ld + bl[k0 + k]]) : kInf2;
+            reinterpret_cast<uint32_t*>(&A4[k][r >> 2])[r & 3] = v;
+        }
+        // X: 32 x 16 octets
+        for (int e = tid; e < kCoverK * kCoverOct; e += 256) {
+            const int k = e / kCoverOct, o = e % kCoverOct;
+            uint4 v = make_uint4(kInf2, kInf2, kInf2, kInf2);
+            if (k < m) {
+                const int b = bl[k0 + k];
+                uint16_t* ve = reinterpret_cast<uint16_t*>(&v);
+                if (HOP2) {
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) {
+                        const int c = colv[o * 8 + q];
+                        if (c >= 0) ve[q] = D[size_t(c) * ld + b];
+                    }
+                } else if (colv[o * 8] >= 0) {
+                    v = *reinterpret_cast<const uint4*>(D + size_t(b) * ld + colv[o * 8]);
+#pragma unroll
+                    for (int q = 0; q < 8; ++q)  // past nV: padding
+                        if (colv[o * 8 + q] < 0) ve[q] = kInf;
+                }
+            }
+            X4[k][o] = v;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int k = 0; k < m; ++k) {
+            const uint4 a = A4[k][ty];
+            const uint4 x = X4[k][tx];
+            const uint32_t av[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                acc[r][0] = relax2(acc[r][0], av[r], x.x);
+                acc[r][1] = relax2(acc[r][1], av[r], x.y);
+                acc[r][2] = relax2(acc[r][2], av[r], x.z);
+                acc[r][3] = relax2(acc[r][3], av[r], x.w);
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int row = rowv[ty * 4 + r];
+        if (row < 0) continue;
+        uint16_t* Dr = D + size_t(row) * ld;
+        if (!HOP2 && colv[tx * 8] >= 0 && colv[tx * 8 + 7] >= 0) {
+            *reinterpret_cast<uint4*>(Dr + colv[tx * 8]) = make_uint4(acc[r][0], acc[r][1], acc[r][2], acc[r][3]);
+            continue;
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int c = colv[tx * 8 + q];
+            if (c >= 0) Dr[c] = uint16_t(acc[r][q >> 1] >> (16 * (q & 1)));
+        }
+    }
+}
+
+bool launch_cover_relax(int nV, int ld, const int* node_list, int n_list, const int* counts, const int* cpiv,
+                        const int* bpiv, int piv_stride, const int* live8_cnt, const int* live8_list, int ng8,
+                        int max_live8, int max_c, uint16_t* DU, size_t node_stride, hipStream_t st, KTimer kt) {
+    if (n_list == 0 || max_live8 == 0 || max_c == 0) return false;
+    const int ct = (max_live8 + kCoverOct - 1) / kCoverOct, rt = (max_c + kCoverRows - 1) / kCoverRows;
+    const dim3 grid{unsigned(rt * ct), unsigned(n_list), 1u};
+    hipExtLaunchKernelGGL(k_cover_product<false>, grid, dim3(256), 0, st, kt.a, kt.b, 0, nV, ld, node_list, n_list,
+                          counts, cpiv, bpiv, piv_stride, ct, live8_cnt, live8_list, ng8, DU, node_stride);
+    return true;
+}
+
+bool launch_cover_pairs(int nV, int ld, const int* node_list, int n_list, const int* counts, const int* cpiv,
+                        const int* bpiv, int piv_stride, int max_c, uint16_t* DU, size_t node_stride, hipStream_t st,
+                        KTimer kt) {
+    if (n_list == 0 || max_c == 0) return false;
+    const int ct = (max_c + kCoverOct * 8 - 1) / (kCoverOct * 8), rt = (max_c + kCoverRows - 1) / kCoverRows;
+    const dim3 grid{unsigned(rt * ct), unsigned(n_list), 1u};
+    hipExtLaunchKernelGGL(k_cover_product<true>, grid, dim3(256), 0, st, kt.a, kt.b, 0, nV, ld, node_list, n_list,
+                          counts, cpiv, bpiv, piv_stride, ct, nullptr, nullptr, 0, DU, node_stride);
+    return true;
 }
 
 // ---------------------------------------------------------------------------

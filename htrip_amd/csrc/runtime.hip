@@ -421,13 +421,14 @@ static uint64_t tables_bytes(int b, int g, int S, int H) {
     return t;
 }
 
-// Largest per-level build workspace: flags, pivot lists, counts, pivot rows.
+// Largest per-level build workspace: flag planes, pivot and B lists, counts,
+// pivot rows.
 static uint64_t build_ws_bytes(int b, int g, int S, int H, int P) {
     uint64_t m = 0;
     for (int l = 1; l <= H; ++l) {
         LevelDims d = level_dims(l, b, g, S);
         const uint64_t nn = uint64_t(d.nt) * d.nt;
-        const uint64_t w = nn * (uint64_t(d.nV) * (1 + 4) + 16 + uint64_t(P) * d.ld * 2) + nn * 4 * 4;
+        const uint64_t w = nn * (uint64_t(d.nV) * (3 + 4 + 4) + 24 + uint64_t(P) * d.ld * 2) + nn * 4 * 4;
         m = std::max(m, w);
     }
     return m;
@@ -445,9 +446,10 @@ int Map::ensure_build_ws(int lvl) {
     const size_t nn = size_t(L.nn);
     int rc;
     if ((rc = ws_nodes.ensure(nn * 4))) return rc;
-    if ((rc = ws_flags.ensure(nn * L.d.nV))) return rc;
+    if ((rc = ws_flags.ensure(3 * nn * L.d.nV))) return rc;  // C marks, B marks, new ports
     if ((rc = ws_piv.ensure(nn * L.d.nV * 4))) return rc;
-    if ((rc = ws_counts.ensure(nn * 4 + 16))) return rc;
+    if ((rc = ws_bpiv.ensure(nn * L.d.nV * 4))) return rc;
+    if ((rc = ws_counts.ensure(2 * nn * 4 + 16))) return rc;
     if ((rc = ws_R.ensure(nn * size_t(tune.pivot_block) * L.d.ld * 2))) return rc;
     if ((rc = ws_changed.ensure(nn * 4 + 16))) return rc;
     if ((rc = ws_mask.ensure(nn * 4 + 16))) return rc;
@@ -641,8 +643,12 @@ static int debug_level() {
 // recomputed by the leaf BFS kernel (the reference's pivot relaxation
 // computes the same tile closure, src/leaf_tile.cpp:297-374); each dirty
 // ancestor takes min(D_U block, new child summary) plus new seam edges and
-// closes over the rows that decreased -- exactly the reference's pivot set K
-// (src/internal_node.cpp:332-413).
+// closes over the rows that decreased (src/internal_node.cpp:332-413).  By
+// default the closure runs over a vertex cover C of the decreased entries
+// (ports opened by this batch, plus both endpoints of decreases between old
+// ports) instead of every decreased row: the cover rows are first relaxed
+// through the uncovered endpoints B (k_cover_relax, k_cover_pairs), which
+// leaves the same closed table with far fewer pivots.
 int Map::open_cells(const int32_t* rcs, int n) {
     timing_now = prof_ref().take();
     stats = htg_batch_stats{0, 0, 0};
@@ -776,15 +782,37 @@ int Map::open_cells(const int32_t* rcs, int n) {
             below_nt = L.d.nt;
         }
     }
+    // new ports per level as (candidate slot, V_U index): the cover of the
+    // decreased entries is built around them
+    const bool cover = tune.update_pivots == 0;
+    std::vector<std::vector<int2>> newp(size_t(H) + 1);
+    if (cover) {
+        for (const int2& p : newly) {
+            for (int lvl = 1; lvl <= H; ++lvl) {
+                const LevelGeo& L = lv[lvl - 1];
+                const int ty = p.x / L.d.s, tx = p.y / L.d.s;
+                const std::vector<int>& c = cand[size_t(lvl)];
+                auto it = std::lower_bound(c.begin(), c.end(), ty * L.d.nt + tx);
+                if (it == c.end() || *it != ty * L.d.nt + tx) continue;
+                const int r = p.x - ty * L.d.s, cc = p.y - tx * L.d.s;
+                const int pi = port_index(r % L.d.cs, cc % L.d.cs, L.d.cs);
+                if (pi < 0) continue;
+                const int q = (r / L.d.cs) * g + cc / L.d.cs;
+                newp[size_t(lvl)].push_back(make_int2(int(it - c.begin()), q * L.d.pc + pi));
+            }
+        }
+    }
     // one packed upload; changed flags: leaves, then each level's candidates
     Packer pk;
     const size_t o_cells = pk.add(newly), o_leaves = pk.add(dirty);
-    std::vector<size_t> o_cand(size_t(H) + 1), o_cslot(size_t(H) + 1), o_edges(size_t(H) + 1), o_chg(size_t(H) + 1);
+    std::vector<size_t> o_cand(size_t(H) + 1), o_cslot(size_t(H) + 1), o_edges(size_t(H) + 1), o_chg(size_t(H) + 1),
+        o_newp(size_t(H) + 1);
     size_t nchg = size_t(nd);
     for (int lvl = 1; lvl <= H; ++lvl) {
         o_cand[size_t(lvl)] = pk.add(cand[size_t(lvl)]);
         o_cslot[size_t(lvl)] = pk.add(cslot[size_t(lvl)]);
         o_edges[size_t(lvl)] = pk.add(edges[size_t(lvl)]);
+        o_newp[size_t(lvl)] = pk.add(newp[size_t(lvl)]);
         o_chg[size_t(lvl)] = nchg;
         nchg += cand[size_t(lvl)].size();
     }
@@ -833,35 +861,70 @@ int Map::open_cells(const int32_t* rcs, int n) {
         const int* dnodes = reinterpret_cast<const int*>(up + o_cand[size_t(lvl)]);
         int max8 = 0;
         for (int node : cl) max8 = std::max(max8, L.h_live8_cnt[node]);
-        if ((e = hipMemsetAsync(ws_flags.p, 0, size_t(na) * L.d.nV, stream)) != hipSuccess)
+        const size_t plane = cover ? size_t(na) * L.d.nV : 0;
+        uint8_t* flags = static_cast<uint8_t*>(ws_flags.p);
+        if ((e = hipMemsetAsync(flags, 0, cover ? 3 * plane : size_t(na) * L.d.nV, stream)) != hipSuccess)
             return hip_fail(e, "clear flags");
+        if (cover)
+            launch_mark_u8(reinterpret_cast<const int2*>(up + o_newp[size_t(lvl)]), int(newp[size_t(lvl)].size()),
+                           L.d.nV, flags + 2 * plane, stream);
         size_t cstride;
         const uint16_t* csum = child_summaries(lvl, cstride);
         launch_child_relax_live(g, L.d.pc, L.d.nV, L.d.ld, L.d.nt, csum, cstride, dnodes,
                                 reinterpret_cast<const int*>(up + o_cslot[size_t(lvl)]), chg_below, na, L.d_live8_cnt,
-                                L.d_live8_list, L.ng8, max8, static_cast<uint16_t*>(L.DU.p), L.node_stride,
-                                static_cast<uint8_t*>(ws_flags.p), stream);
+                                L.d_live8_list, L.ng8, max8, static_cast<uint16_t*>(L.DU.p), L.node_stride, flags,
+                                plane, stream);
         if (!edges[size_t(lvl)].empty())
             launch_edge_relax(L.d.nV, L.d.ld, reinterpret_cast<const int4*>(up + o_edges[size_t(lvl)]),
                               int(edges[size_t(lvl)].size()), dnodes, static_cast<uint16_t*>(L.DU.p), L.node_stride,
-                              static_cast<uint8_t*>(ws_flags.p), stream);
-        launch_compact_flags_live(L.d.nV, dnodes, na, L.d_live8_cnt, L.d_live8_list, L.ng8,
-                                  static_cast<uint8_t*>(ws_flags.p), static_cast<int*>(ws_piv.p), L.d.nV,
-                                  static_cast<int*>(ws_counts.p), stream);
-        std::vector<int> hc(static_cast<size_t>(na));
-        if ((e = hipMemcpyAsync(hc.data(), ws_counts.p, size_t(na) * 4, hipMemcpyDeviceToHost, stream)) != hipSuccess)
+                              flags, plane, stream);
+        int* cnt = static_cast<int*>(ws_counts.p);
+        launch_compact_flags_live(L.d.nV, dnodes, na, L.d_live8_cnt, L.d_live8_list, L.ng8, flags, plane,
+                                  static_cast<int*>(ws_piv.p), static_cast<int*>(ws_bpiv.p), L.d.nV, cnt, stream);
+        std::vector<int> hc(static_cast<size_t>(cover ? 2 * na : na));
+        if ((e = hipMemcpyAsync(hc.data(), cnt, hc.size() * 4, hipMemcpyDeviceToHost, stream)) != hipSuccess)
             return hip_fail(e, "download counts");
         if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "update sync");
-        long long sum = 0;
-        int mx = 0;
+        long long sum = 0, bsum = 0;
+        int mx = 0, mxb = 0;
         for (int i = 0; i < na; ++i) {
             sum += hc[size_t(i)];
             mx = std::max(mx, hc[size_t(i)]);
+            if (cover && hc[size_t(i)] > 0) {
+                bsum += hc[size_t(na + i)];
+                mxb = std::max(mxb, hc[size_t(na + i)]);
+            }
             if (hc[size_t(i)] > 0) this->dirty.nodes[size_t(lvl) - 1].push_back(cl[size_t(i)]);
         }
         if (debug_level())
-            std::fprintf(stderr, "[htg] update lvl %d: %d nodes (nV %d), pivots sum %lld max %d\n", lvl, na, L.d.nV,
-                         sum, mx);
+            std::fprintf(stderr, "[htg] update lvl %d: %d nodes (nV %d), pivots sum %lld max %d, B sum %lld max %d\n",
+                         lvl, na, L.d.nV, sum, mx, bsum, mxb);
+        if (bsum > 0) {  // fold the uncovered endpoints into the cover rows
+            double by_r = 0, upd_r = 0, by_p = 0, upd_p = 0;
+            if (prof_ref().on) {
+                for (int i = 0; i < na; ++i) {
+                    const double c = hc[size_t(i)], bb = hc[size_t(na + i)];
+                    if (c <= 0 || bb <= 0) continue;
+                    const double nl = 8.0 * L.h_live8_cnt[cl[size_t(i)]];
+                    upd_r += c * bb * nl;
+                    by_r += (2 * c * nl + bb * nl) * 2;  // cover rows in + out, B rows in
+                    upd_p += c * c * bb;
+                    by_p += (2 * c * bb + c * c) * 2;
+                }
+            }
+            KTimer kt;
+            int pi = prof_open(HTG_K_COVER_RELAX, by_r, upd_r, kt);
+            bool la = launch_cover_relax(L.d.nV, L.d.ld, dnodes, na, cnt, static_cast<const int*>(ws_piv.p),
+                                         static_cast<const int*>(ws_bpiv.p), L.d.nV, L.d_live8_cnt, L.d_live8_list,
+                                         L.ng8, max8, mx, static_cast<uint16_t*>(L.DU.p), L.node_stride, stream, kt);
+            prof_ref().close(pi, la);
+            pi = prof_open(HTG_K_COVER_PAIRS, by_p, upd_p, kt);
+            la = launch_cover_pairs(L.d.nV, L.d.ld, dnodes, na, cnt, static_cast<const int*>(ws_piv.p),
+                                    static_cast<const int*>(ws_bpiv.p), L.d.nV, mx, static_cast<uint16_t*>(L.DU.p),
+                                    L.node_stride, stream, kt);
+            prof_ref().close(pi, la);
+        }
+        hc.resize(size_t(na));
         if (sum > 0) {
             if ((rc = run_closure(lvl, dnodes, cl, hc))) return rc;
             // S_U refresh for nodes that closed over something (the reference
@@ -1726,6 +1789,7 @@ int htg_set_tuning(htg_map* map, const htg_tuning* t) {
     if (t->pivot_block != 16 && t->pivot_block != 32 && t->pivot_block != 64)
         return fail(HTG_E_INVALID, "pivot_block must be 16, 32 or 64");
     if (t->zq_tile < 1 || t->zq_tile > 16) return fail(HTG_E_INVALID, "zq_tile must be 1..16");
+    if (t->update_pivots != 0 && t->update_pivots != 1) return fail(HTG_E_INVALID, "update_pivots must be 0 or 1");
     map->tune = *t;
     return HTG_OK;
 }

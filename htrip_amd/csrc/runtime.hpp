@@ -268,7 +268,7 @@ struct Map {
     hipEvent_t live_evt = nullptr;  // recorded after the last upload of live_h
     bool live_pending = false;
     ~Map();
-    DevBuf ws_nodes, ws_flags, ws_piv, ws_counts, ws_R, ws_changed, ws_mask, ws_edges, ws_cells, ws_leaves,
+    DevBuf ws_nodes, ws_flags, ws_piv, ws_bpiv, ws_counts, ws_R, ws_changed, ws_mask, ws_edges, ws_cells, ws_leaves,
         ws_leaf_changed;
     DevBuf ws_upd, ws_chg;  // update: packed per-level lists; changed flags (leaves, then every level)
     HostBuf ws_upd_h, ws_chg_h;

@@ -139,7 +139,9 @@ enum {
     HTG_K_Q_PROJECT = 3,
     HTG_K_Q_MEET = 4,
     HTG_K_Q_LIFT = 5, /* fold of row-chunk partials + lift (only when a level splits its rows) */
-    HTG_K_COUNT = 6
+    HTG_K_COVER_RELAX = 6, /* update: cover rows relaxed through the uncovered endpoints */
+    HTG_K_COVER_PAIRS = 7, /* update: second hop on the cover x cover block */
+    HTG_K_COUNT = 8
 };
 typedef struct htg_kernel_stats {
     uint64_t launches;
@@ -221,7 +223,10 @@ typedef struct htg_tuning {
     int32_t pivot_block;   /* pivots per closure block: 16, 32 or 64 */
     int32_t zq_tile;       /* viewpoints per query-projection work item: 1..16 */
     int32_t use_graph;     /* run the build as one captured hipGraph (replayed while its launch shape holds) */
-    int32_t reserved;
+    int32_t update_pivots; /* incremental reveal closes each node over: 0 (default) a vertex cover of the
+                              decreased entries (new ports + endpoints of old-old decreases) after a two-hop
+                              pre-relaxation through the uncovered endpoints; 1 every row with a decreased
+                              entry (the reference's pivot set, src/internal_node.cpp:332-413).  Both are exact. */
 } htg_tuning;
 int htg_get_tuning(const htg_map* map, htg_tuning* out);
 int htg_set_tuning(htg_map* map, const htg_tuning* t);

@@ -7,7 +7,7 @@ tests/test_oracle.py); the compiled reference itself is used too when present.
 import numpy as np
 import pytest
 
-from htrip_amd import grids
+from htrip_amd import grids, serving
 from htrip_amd._lib import GpuMap
 from oracle.oracle import Oracle, grid_bfs
 
@@ -94,6 +94,43 @@ def test_incremental_reveal(gpu_ctx, kind, n, b, g, seed, radius, batch, steps):
     F = GpuMap(O.cells, b, g)
     F.build()
     assert_tables_equal(F, O)
+
+
+@pytest.mark.parametrize("kind,n,b,seed,robots,batch,steps", [
+    ("street", 256, 16, 41, 6, 400, 4),  # ~2400 cells a batch: cover > 64 rows, > 512 uncovered ends
+    ("maze", 128, 8, 43, 4, 120, 6),
+    ("street", 128, 32, 44, 3, 60, 8),
+    ("street", 64, 8, 45, 2, 9, 10),     # small batches: mostly single-block covers
+])
+def test_update_pivot_sets_agree(gpu_ctx, kind, n, b, seed, robots, batch, steps):
+    """Incremental reveal closed over the vertex cover (default) and over
+    every decreased row (the reference's pivot set): identical tables after
+    every step, both equal to the oracle at the end."""
+    print(f"pivot sets {kind} n={n} b={b} seed={seed} batch={robots}x{batch}", flush=True)
+    truth = make(kind, n, seed)
+    rv = serving.RobotReveal(truth, robots, 16, seed + 1, limit=100000)
+    obs = rv.observed.copy()
+    O = Oracle("port", obs, b, 2)
+    O.build()
+    A = GpuMap(obs, b, 2)
+    A.build()
+    R = GpuMap(obs, b, 2)
+    R.set_tuning(update_pivots=1)
+    R.build()
+    assert A.tuning().update_pivots == 0 and R.tuning().update_pivots == 1
+    for st in range(steps):
+        cells = np.vstack([rv.next_batch(r, batch) for r in range(robots)]).astype(np.int32)
+        O.open_cells(cells)
+        A.open_cells(cells)
+        R.open_cells(cells)
+        assert A.last_batch_stats() == R.last_batch_stats() == O.last_batch_stats(), st
+        for lvl in range(1, O.H + 1):
+            for i in range(O.num_nodes(lvl)):
+                np.testing.assert_array_equal(A.node_DU(lvl, i), R.node_DU(lvl, i), err_msg=f"step {st} D_U {lvl},{i}")
+    assert_tables_equal(A, O)
+    assert_tables_equal(R, O)
+    vps = grids.sample_free(O.cells, 32, seed)
+    np.testing.assert_array_equal(A.query(vps), grid_bfs(O.cells, vps))
 
 
 def test_single_cell_openings(gpu_ctx):
