@@ -307,6 +307,9 @@ def run_served(args):
                          (("plan_launch", "plan_seconds"), ("gpu_wait", "gpu_wait_seconds"),
                           ("finish", "finish_seconds"))}
         trials.append((tr, td, s, state0))
+        if args.dump_trials:
+            np.savez_compressed(f"{args.dump_trials}_{len(trials) - 1}.npz", t=tr.t, done=td, type=tr.type,
+                                count=tr.count, version=tr.version, rate=rate)
         log(f"rate {rate:.0f}/s: {s['matrices_per_s']:.0f} matrices/s, p50 {s['p50_ms']:.3f} ms, "
             f"p99 {s['p99_ms']:.3f} ms, updates p50 {s['update_p50_ms']} ms, batch {s['mean_batch_queries']:.2f}, "
             f"worker planning {100 * s['worker_plan_busy']:.0f}%, updating {100 * s['update_busy']:.0f}%, "
@@ -661,7 +664,8 @@ def multi_checks(tr, sess, per_session=1, kmax=64):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (served: windows of requests per rate trial, default 200; others 40)")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--mode", default="served",
                     choices=["plain", "served", "update", "small", "build", "reveal", "multi"])
@@ -676,6 +680,7 @@ def main():
                          "every decreased row (the reference's pivot set)")
     ap.add_argument("--update-requests", type=int, default=1,
                     help="update mode: robot requests coalesced into each onBatchCellsOpened call")
+    ap.add_argument("--dump-trials", default="", help="served: save each trial's arrival/completion times here")
     ap.add_argument("--batch-queries", type=int, default=64)
     ap.add_argument("--batch-viewpoints", type=int, default=8192)
     ap.add_argument("--plain-steps", type=int, default=200)
@@ -685,6 +690,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 200 if args.mode == "served" else 40
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU (one MI355X per box); --gpus must be 1"}))
         sys.exit(2)

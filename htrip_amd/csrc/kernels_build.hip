@@ -435,27 +435,49 @@ void launch_compact_flags_live(int nV, const int* node_list, int n_list, const i
 // B columns and writes C columns.
 //
 // Both hops are one rectangular min-plus product, tiled like the closure
-// update: a workgroup owns 64 C rows x 16 column octets (thread = 4 rows x 8
-// columns, two u16 lanes per v_pk op) and streams the inner set B through LDS
-// in chunks of 32: dup'd A[b][row] = D[row][b] and X[b][octet] (hop 1: the B
-// row's live octet; hop 2: D[c2][b] for 8 consecutive cover entries c2).
-// Grid: x = row tile * column tiles + column tile, y = node slot.
-constexpr int kCoverRows = 64, kCoverOct = 16, kCoverK = 32;
+// update: a workgroup owns 16*RPT C rows x 16 column octets (thread = RPT rows
+// x 8 columns, two u16 lanes per v_pk op) and streams the inner set B through
+// LDS in chunks of 32: dup'd A[b][row] = D[row][b] and X[b][octet] (hop 1:
+// the B row's live octet; hop 2: D[c2][b] for 8 consecutive cover entries
+// c2).  Staging is software-pipelined: the gathers of chunk i+1 are issued
+// before chunk i is computed, with their B indices loaded one chunk earlier
+// still.  Waves whose rows are all past the cover size skip the arithmetic.
+// The host picks RPT so that the few large nodes of the upper levels still
+// fill the chip.  Grid: x = row tile * column tiles + column tile, y = slot.
+constexpr int kCoverOct = 16, kCoverK = 32;
 
-template <bool HOP2>
+template <int RPT> struct LdsRow;
+template <> struct LdsRow<1> {
+    typedef uint32_t T;
+    static __device__ __forceinline__ void get(const T& v, uint32_t* o) { o[0] = v; }
+};
+template <> struct LdsRow<2> {
+    typedef uint2 T;
+    static __device__ __forceinline__ void get(const T& v, uint32_t* o) { o[0] = v.x, o[1] = v.y; }
+};
+template <> struct LdsRow<4> {
+    typedef uint4 T;
+    static __device__ __forceinline__ void get(const T& v, uint32_t* o) { o[0] = v.x, o[1] = v.y, o[2] = v.z, o[3] = v.w; }
+};
+
+template <bool HOP2, int RPT>
 __global__ __launch_bounds__(256) void k_cover_product(int nV, int ld, const int* __restrict__ node_list, int n_list,
                                                        const int* __restrict__ counts, const int* __restrict__ cpiv,
                                                        const int* __restrict__ bpiv, int piv_stride, int col_tiles,
                                                        const int* __restrict__ live8_cnt,
                                                        const int* __restrict__ live8_list, int ng8,
                                                        uint16_t* __restrict__ DU, size_t node_stride) {
-    __shared__ uint4 A4[kCoverK][kCoverRows / 4];  // dup16(D[row][b]): 4 rows of a thread per uint4
+    constexpr int R = 16 * RPT;               // rows per tile
+    constexpr int SA = kCoverK * R / 256;     // A entries staged per thread
+    constexpr int SX = kCoverK * kCoverOct / 256;  // X octets staged per thread
+    typedef typename LdsRow<RPT>::T RowT;
+    __shared__ RowT A4[kCoverK][16];  // dup16(D[row][b]): the RPT rows of a thread
     __shared__ uint4 X4[kCoverK][kCoverOct];
-    __shared__ int rowv[kCoverRows], colv[kCoverOct * 8];
+    __shared__ int rowv[R], colv[kCoverOct * 8];
     const int slot = blockIdx.y;
     const int nc = counts[slot], nb = counts[n_list + slot];
     const int I = blockIdx.x / col_tiles, J = blockIdx.x % col_tiles;
-    if (nb == 0 || I * kCoverRows >= nc) return;
+    if (nb == 0 || I * R >= nc) return;
     const int node = node_list[slot];
     const int n_cols = HOP2 ? nc : 8 * live8_cnt[node];  // columns this node has
     if (J * kCoverOct * 8 >= n_cols) return;
@@ -463,7 +485,7 @@ __global__ __launch_bounds__(256) void k_cover_product(int nV, int ld, const int
     const int* cl = cpiv + size_t(slot) * piv_stride;
     const int* bl = bpiv + size_t(slot) * piv_stride;
     uint16_t* D = DU + size_t(node) * node_stride;
-    if (tid < kCoverRows) rowv[tid] = I * kCoverRows + tid < nc ? cl[I * kCoverRows + tid] : -1;
+    if (tid < R) rowv[tid] = I * R + tid < nc ? cl[I * R + tid] : -1;
     if (tid < kCoverOct * 8) {
         const int e = J * kCoverOct * 8 + tid;  // column entry of this tile
         int v = -1;
@@ -472,11 +494,19 @@ __global__ __launch_bounds__(256) void k_cover_product(int nV, int ld, const int
         colv[tid] = v < nV ? v : -1;
     }
     __syncthreads();
-    // accumulators: rows ty*4 .. +3, columns colv[tx*8 .. +7]
-    uint32_t acc[4][4];
+    const bool busy = (tid >> 6) * 4 * RPT < nc - I * R;  // this wave has rows
+    // staging roles: A entry i is (k = tid/R + (256/R) i, row = tid%R); X
+    // octet i is (k = tid/16 + 16 i, octet = tid%16)
+    const int a_r = tid % R;
+    const int a_row = rowv[a_r];
+    const int x_o = tid & 15;
+    int x_c[8];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const int row = rowv[ty * 4 + r];
+    for (int q = 0; q < 8; ++q) x_c[q] = colv[x_o * 8 + q];
+    uint32_t acc[RPT][4];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        const int row = rowv[ty * RPT + r];
 #pragma unroll
         for (int h = 0; h < 4; ++h) {
             const int c0 = colv[tx * 8 + 2 * h], c1 = colv[tx * 8 + 2 * h + 1];
@@ -485,46 +515,68 @@ __global__ __launch_bounds__(256) void k_cover_product(int nV, int ld, const int
             acc[r][h] = lo | (hi << 16);
         }
     }
-    for (int k0 = 0; k0 < nb; k0 += kCoverK) {
-        const int m = min(kCoverK, nb - k0);
-        __syncthreads();
-        // A: 32 x 64 gathered entries D[row][b]
-        for (int e = tid; e < kCoverK * kCoverRows; e += 256) {
-            const int k = e / kCoverRows, r = e % kCoverRows;
-            const int row = rowv[r];
-            const uint32_t v = k < m && row >= 0 ? dup16(D[size_t(row) * ld + bl[k0 + k]]) : kInf2;
-            reinterpret_cast<uint32_t*>(&A4[k][r >> 2])[r & 3] = v;
+    auto load_idx = [&](int k0, int* ia, int* ix) {
+#pragma unroll
+        for (int i = 0; i < SA; ++i) {
+            const int k = k0 + tid / R + (256 / R) * i;
+            ia[i] = k < nb ? bl[k] : -1;
         }
-        // X: 32 x 16 octets
-        for (int e = tid; e < kCoverK * kCoverOct; e += 256) {
-            const int k = e / kCoverOct, o = e % kCoverOct;
+#pragma unroll
+        for (int i = 0; i < SX; ++i) {
+            const int k = k0 + (tid >> 4) + 16 * i;
+            ix[i] = k < nb ? bl[k] : -1;
+        }
+    };
+    auto gather = [&](const int* ia, const int* ix, uint32_t* va, uint4* vx) {
+#pragma unroll
+        for (int i = 0; i < SA; ++i)
+            va[i] = ia[i] >= 0 && a_row >= 0 ? dup16(D[size_t(a_row) * ld + ia[i]]) : kInf2;
+#pragma unroll
+        for (int i = 0; i < SX; ++i) {
             uint4 v = make_uint4(kInf2, kInf2, kInf2, kInf2);
-            if (k < m) {
-                const int b = bl[k0 + k];
+            if (ix[i] >= 0) {
                 uint16_t* ve = reinterpret_cast<uint16_t*>(&v);
                 if (HOP2) {
 #pragma unroll
-                    for (int q = 0; q < 8; ++q) {
-                        const int c = colv[o * 8 + q];
-                        if (c >= 0) ve[q] = D[size_t(c) * ld + b];
-                    }
-                } else if (colv[o * 8] >= 0) {
-                    v = *reinterpret_cast<const uint4*>(D + size_t(b) * ld + colv[o * 8]);
+                    for (int q = 0; q < 8; ++q)
+                        if (x_c[q] >= 0) ve[q] = D[size_t(x_c[q]) * ld + ix[i]];
+                } else if (x_c[0] >= 0) {
+                    v = *reinterpret_cast<const uint4*>(D + size_t(ix[i]) * ld + x_c[0]);
 #pragma unroll
                     for (int q = 0; q < 8; ++q)  // past nV: padding
-                        if (colv[o * 8 + q] < 0) ve[q] = kInf;
+                        if (x_c[q] < 0) ve[q] = kInf;
                 }
             }
-            X4[k][o] = v;
+            vx[i] = v;
         }
+    };
+    int ia[SA], ix[SX];
+    uint32_t va[SA];
+    uint4 vx[SX];
+    load_idx(0, ia, ix);
+    gather(ia, ix, va, vx);
+    load_idx(kCoverK, ia, ix);
+    for (int k0 = 0; k0 < nb; k0 += kCoverK) {
+        __syncthreads();  // the previous chunk's arithmetic is done
+#pragma unroll
+        for (int i = 0; i < SA; ++i)  // row a_r sits in thread group a_r / RPT, lane a_r % RPT
+            reinterpret_cast<uint32_t*>(&A4[tid / R + (256 / R) * i][a_r / RPT])[a_r % RPT] = va[i];
+#pragma unroll
+        for (int i = 0; i < SX; ++i) X4[(tid >> 4) + 16 * i][x_o] = vx[i];
         __syncthreads();
+        if (k0 + kCoverK < nb) {  // next chunk in flight during this one's arithmetic
+            gather(ia, ix, va, vx);
+            load_idx(k0 + 2 * kCoverK, ia, ix);
+        }
+        if (!busy) continue;
+        const int m = min(kCoverK, nb - k0);
 #pragma unroll 4
         for (int k = 0; k < m; ++k) {
-            const uint4 a = A4[k][ty];
+            uint32_t av[RPT];
+            LdsRow<RPT>::get(A4[k][ty], av);
             const uint4 x = X4[k][tx];
-            const uint32_t av[4] = {a.x, a.y, a.z, a.w};
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
+            for (int r = 0; r < RPT; ++r) {
                 acc[r][0] = relax2(acc[r][0], av[r], x.x);
                 acc[r][1] = relax2(acc[r][1], av[r], x.y);
                 acc[r][2] = relax2(acc[r][2], av[r], x.z);
@@ -533,8 +585,8 @@ __global__ __launch_bounds__(256) void k_cover_product(int nV, int ld, const int
         }
     }
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const int row = rowv[ty * 4 + r];
+    for (int r = 0; r < RPT; ++r) {
+        const int row = rowv[ty * RPT + r];
         if (row < 0) continue;
         uint16_t* Dr = D + size_t(row) * ld;
         if (!HOP2 && colv[tx * 8] >= 0 && colv[tx * 8 + 7] >= 0) {
@@ -549,14 +601,30 @@ __global__ __launch_bounds__(256) void k_cover_product(int nV, int ld, const int
     }
 }
 
+// Rows per thread: the tallest tile that still gives about 1024 workgroups.
+static int cover_rpt(int max_c, int col_tiles, int n_list) {
+    for (int rpt = 4; rpt > 1; rpt /= 2) {
+        const long long blocks = (long long)((max_c + 16 * rpt - 1) / (16 * rpt)) * col_tiles * n_list;
+        if (blocks >= 1024) return rpt;
+    }
+    return 1;
+}
+
+#define HTG_COVER_LAUNCH(HOP2, RPT, CT, L8C, L8L, NG8)                                                             \
+    hipExtLaunchKernelGGL((k_cover_product<HOP2, RPT>),                                                           \
+                          dim3(unsigned(((max_c + 16 * (RPT)-1) / (16 * (RPT))) * (CT)), unsigned(n_list), 1u),   \
+                          dim3(256), 0, st, kt.a, kt.b, 0, nV, ld, node_list, n_list, counts, cpiv, bpiv,         \
+                          piv_stride, CT, L8C, L8L, NG8, DU, node_stride)
+
 bool launch_cover_relax(int nV, int ld, const int* node_list, int n_list, const int* counts, const int* cpiv,
                         const int* bpiv, int piv_stride, const int* live8_cnt, const int* live8_list, int ng8,
                         int max_live8, int max_c, uint16_t* DU, size_t node_stride, hipStream_t st, KTimer kt) {
     if (n_list == 0 || max_live8 == 0 || max_c == 0) return false;
-    const int ct = (max_live8 + kCoverOct - 1) / kCoverOct, rt = (max_c + kCoverRows - 1) / kCoverRows;
-    const dim3 grid{unsigned(rt * ct), unsigned(n_list), 1u};
-    hipExtLaunchKernelGGL(k_cover_product<false>, grid, dim3(256), 0, st, kt.a, kt.b, 0, nV, ld, node_list, n_list,
-                          counts, cpiv, bpiv, piv_stride, ct, live8_cnt, live8_list, ng8, DU, node_stride);
+    const int ct = (max_live8 + kCoverOct - 1) / kCoverOct;
+    const int rpt = cover_rpt(max_c, ct, n_list);
+    if (rpt == 4) HTG_COVER_LAUNCH(false, 4, ct, live8_cnt, live8_list, ng8);
+    else if (rpt == 2) HTG_COVER_LAUNCH(false, 2, ct, live8_cnt, live8_list, ng8);
+    else HTG_COVER_LAUNCH(false, 1, ct, live8_cnt, live8_list, ng8);
     return true;
 }
 
@@ -564,12 +632,15 @@ bool launch_cover_pairs(int nV, int ld, const int* node_list, int n_list, const 
                         const int* bpiv, int piv_stride, int max_c, uint16_t* DU, size_t node_stride, hipStream_t st,
                         KTimer kt) {
     if (n_list == 0 || max_c == 0) return false;
-    const int ct = (max_c + kCoverOct * 8 - 1) / (kCoverOct * 8), rt = (max_c + kCoverRows - 1) / kCoverRows;
-    const dim3 grid{unsigned(rt * ct), unsigned(n_list), 1u};
-    hipExtLaunchKernelGGL(k_cover_product<true>, grid, dim3(256), 0, st, kt.a, kt.b, 0, nV, ld, node_list, n_list,
-                          counts, cpiv, bpiv, piv_stride, ct, nullptr, nullptr, 0, DU, node_stride);
+    const int ct = (max_c + kCoverOct * 8 - 1) / (kCoverOct * 8);
+    const int rpt = cover_rpt(max_c, ct, n_list);
+    const int* nul = nullptr;
+    if (rpt == 4) HTG_COVER_LAUNCH(true, 4, ct, nul, nul, 0);
+    else if (rpt == 2) HTG_COVER_LAUNCH(true, 2, ct, nul, nul, 0);
+    else HTG_COVER_LAUNCH(true, 1, ct, nul, nul, 0);
     return true;
 }
+#undef HTG_COVER_LAUNCH
 
 // ---------------------------------------------------------------------------
 // Row flags -> ascending pivot list (one workgroup per node).

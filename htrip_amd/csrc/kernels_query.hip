@@ -14,6 +14,8 @@
 // viewpoints that share (W, q), so every D_W row read feeds up to M
 // viewpoints from registers.
 #include <hip/hip_runtime.h>
+
+#include <cstdlib>
 #include <hip/hip_ext.h>
 
 #include "geometry.hpp"
@@ -133,7 +135,7 @@ void launch_q_front(int nq, const int* q_k, const long long* q_out, const int* q
 // (src/minplus_oracle.cpp:1113-1154).  wstat (nullable) accumulates the
 // work done: [0] bytes (D slices of distinct (W, q) items, X reads, Zp
 // writes, each counted once), [1] min-plus updates on finite rows.
-template <int M>
+template <int M, int PF>
 __global__ __launch_bounds__(256) void k_q_z(
     int ld, int pc, const uint16_t* __restrict__ DU, size_t node_stride, const int* __restrict__ seam_list,
     const int* __restrict__ seam_off, const int* __restrict__ cols, int ngrp, const int* __restrict__ live_cnt,
@@ -142,8 +144,7 @@ __global__ __launch_bounds__(256) void k_q_z(
     const int* __restrict__ colpos, const int* __restrict__ colext, const int* __restrict__ ext_child,
     const int* __restrict__ ext_cport, uint16_t* __restrict__ Z, int ldz, uint16_t* __restrict__ Xn, int ne,
     unsigned long long* __restrict__ wstat) {
-    constexpr int ACH = 64;  // seam rows staged per wavefront and round
-    constexpr int PF = 8;    // D rows in flight per lane
+    constexpr int ACH = 64;  // seam rows staged per wavefront and round (PF: D rows in flight per lane)
     constexpr int NW = 4;    // wavefronts per workgroup: same columns, interleaved row rounds
     // Padding rows carry INF in xs: min(acc, INF (+) d) leaves acc unchanged,
     // so the loop runs branch-free over whole PF blocks.
@@ -201,10 +202,20 @@ __global__ __launch_bounds__(256) void k_q_z(
         __syncthreads();
         rows_done += nk;
         if (!active) continue;
-        for (int a = 0; a < nkp; a += PF) {
-            uint2 dv[PF];
+        // D rows double-buffered: the next PF rows are in flight while this
+        // block's PF rows are folded in
+        uint2 dv[PF], dn[PF];
+        if (nkp > 0) {
 #pragma unroll
-            for (int u = 0; u < PF; ++u) dv[u] = *reinterpret_cast<const uint2*>(D + roff[w][a + u]);
+            for (int u = 0; u < PF; ++u) dn[u] = *reinterpret_cast<const uint2*>(D + roff[w][u]);
+        }
+        for (int a = 0; a < nkp; a += PF) {
+#pragma unroll
+            for (int u = 0; u < PF; ++u) dv[u] = dn[u];
+            if (a + PF < nkp) {
+#pragma unroll
+                for (int u = 0; u < PF; ++u) dn[u] = *reinterpret_cast<const uint2*>(D + roff[w][a + PF + u]);
+            }
 #pragma unroll
             for (int u = 0; u < PF; ++u) {
                 const uint32_t* xr = xs[w][a + u];
@@ -293,16 +304,29 @@ bool launch_q_z(int M, int ld, int pc, const uint16_t* DU, size_t node_stride, c
                 hipStream_t st, KTimer kt) {
     if (n_items == 0 || col_tiles == 0) return false;
     const dim3 grid{unsigned(n_items), unsigned(col_tiles), unsigned(ksplit)};
-#define HTG_QZ(MM)                                                                                                  \
-    hipExtLaunchKernelGGL((k_q_z<MM>), grid, dim3(256), 0, st, kt.a, kt.b, 0, ld, pc, DU, node_stride, seam_list,   \
-                          seam_off, cols, ngrp,                                                                     \
-                       live_cnt, live_list, ldp, items, ksplit, X, Zp, part_stride, colpos, colext, ext_child, \
-                       ext_cport, Z, ldz, Xn, ne, wstat)
-    if (M <= 1) HTG_QZ(1);
-    else if (M <= 2) HTG_QZ(2);
-    else if (M <= 4) HTG_QZ(4);
-    else if (M <= 8) HTG_QZ(8);
-    else HTG_QZ(16);
+    static const int pf = [] {
+        const char* e = std::getenv("HTG_ZQ_PF");
+        const int v = e ? std::atoi(e) : 8;
+        return v == 16 || v == 32 ? v : 8;
+    }();
+#define HTG_QZ(MM, PP)                                                                                             \
+    hipExtLaunchKernelGGL((k_q_z<MM, PP>), grid, dim3(256), 0, st, kt.a, kt.b, 0, ld, pc, DU, node_stride,         \
+                          seam_list, seam_off, cols, ngrp, live_cnt, live_list, ldp, items, ksplit, X, Zp,         \
+                          part_stride, colpos, colext, ext_child, ext_cport, Z, ldz, Xn, ne, wstat)
+#define HTG_QZM(PP)                  \
+    if (M <= 1) HTG_QZ(1, PP);       \
+    else if (M <= 2) HTG_QZ(2, PP);  \
+    else if (M <= 4) HTG_QZ(4, PP);  \
+    else if (M <= 8) HTG_QZ(8, PP);  \
+    else HTG_QZ(16, PP);
+    if (pf == 32) {
+        HTG_QZM(32)
+    } else if (pf == 16) {
+        HTG_QZM(16)
+    } else {
+        HTG_QZM(8)
+    }
+#undef HTG_QZM
 #undef HTG_QZ
     return true;
 }

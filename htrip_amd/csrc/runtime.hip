@@ -39,8 +39,11 @@ int hip_fail(hipError_t e, const char* what) {
     return fail(HTG_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// Buffers that have to grow double: freeing device or pinned memory waits
+// for the whole device, which stalls every other stream's work.
 int DevBuf::ensure(size_t n) {
     if (n <= bytes) return HTG_OK;
+    if (bytes > 0) n = std::max(n, 2 * bytes);
     release();
     if (n == 0) return HTG_OK;
     hipError_t e = hipMalloc(&p, n);
@@ -60,6 +63,7 @@ void DevBuf::release() {
 
 int HostBuf::ensure(size_t n) {
     if (n <= bytes) return HTG_OK;
+    if (bytes > 0) n = std::max(n, 2 * bytes);
     release();
     if (n == 0) return HTG_OK;
     hipError_t e = hipHostMalloc(&p, n, hipHostMallocDefault);
@@ -431,7 +435,7 @@ static uint64_t build_ws_bytes(int b, int g, int S, int H, int P) {
         const uint64_t w = nn * (uint64_t(d.nV) * (3 + 4 + 4) + 24 + uint64_t(P) * d.ld * 2) + nn * 4 * 4;
         m = std::max(m, w);
     }
-    return m;
+    return m + kUpdateListReserve;
 }
 
 static void geometry(int b, int g, int rows, int cols, int& S, int& H) {
@@ -439,6 +443,21 @@ static void geometry(int b, int g, int rows, int cols, int& S, int& H) {
     while (S < std::max(rows, cols)) S *= g;
     H = 0;
     for (int t = S / b; t > 1; t /= g) ++H;
+}
+
+// Update lists and changed flags, and every level's closure workspace, at
+// their working sizes before the first update (a map that is only ever
+// updated, such as the second copy of a snapshot session, never builds).
+int Map::reserve_update_ws() {
+    size_t nchg = size_t(nl) * nl;
+    for (const LevelGeo& L : lv) nchg += size_t(L.nn);
+    int rc;
+    if ((rc = ws_upd.ensure(kUpdateListReserve)) || (rc = ws_upd_h.ensure(kUpdateListReserve)) ||
+        (rc = ws_chg.ensure(nchg * 4 + 16)) || (rc = ws_chg_h.ensure(nchg * 4 + 16)))
+        return rc;
+    for (int lvl = 1; lvl <= H; ++lvl)
+        if ((rc = ensure_build_ws(lvl))) return rc;
+    return HTG_OK;
 }
 
 int Map::ensure_build_ws(int lvl) {
@@ -1174,7 +1193,11 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
         {  // Row chunks: one (fold in registers, no fold launch) while an item
            // has at most kRowsPerChunk passable seam rows; beyond that, enough
            // chunks for that bound and to give the launch ~2048 workgroups.
-            constexpr int kRowsPerChunk = 512;  // 4 wavefronts x 128 rows
+            static const int kRowsPerChunk = [] {  // 4 wavefronts x 128 rows
+                const char* e = std::getenv("HTG_ZQ_ROWS");
+                const int v = e ? std::atoi(e) : 512;
+                return v >= 64 && v <= 4096 ? v : 512;
+            }();
             int ks = (rows_max + kRowsPerChunk - 1) / kRowsPerChunk;
             if (ks > 1 && waves > 0) ks = std::max(ks, int((2048 + waves - 1) / waves));
             ks = std::min(ks, std::max(1, L.nsmax / 32));
@@ -1497,6 +1520,7 @@ int map_create_on(htg_ctx* ctx, const htg_config* cfg, int32_t rows, int32_t col
     }
     if ((rc = m->init_live()) || (rc = m->refresh_live(nullptr))) return rc;
     if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(e, "upload live groups");
+    if ((rc = m->reserve_update_ws())) return rc;
     m->device_bytes = tables_bytes(b, g, m->S, m->H);
     *out = m.release();
     return HTG_OK;

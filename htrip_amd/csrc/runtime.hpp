@@ -37,6 +37,9 @@ struct DevBuf {
     void release();
 };
 
+// Packed update lists reserved per map (grown, doubling, past it).
+constexpr size_t kUpdateListReserve = size_t(1) << 20;
+
 // Grow-only pinned host buffer.
 struct HostBuf {
     void* p = nullptr;
@@ -244,7 +247,7 @@ struct Map {
     uint64_t epoch = 0;
     htg_batch_stats stats{0, 0, 0};
     uint64_t budget = 0, device_bytes = 0;
-    htg_tuning tune{64, 16, 0, 0};
+    htg_tuning tune{64, 4, 0, 0};
     QueryWS qws;
     std::shared_ptr<Profiler> profp = std::make_shared<Profiler>();
     Profiler& prof_ref() { return *profp; }
@@ -285,6 +288,7 @@ struct Map {
     // free synchronises the whole device).
     int reserve_query_ws(QueryWS& ws, int max_vps, long long max_out) const;
     int ensure_build_ws(int lvl);
+    int reserve_update_ws();
     // Lay out the live block of every level (once, at creation).
     int init_live();
     // Recompute the live column groups and octets of the nodes that contain
