@@ -109,6 +109,30 @@ def plain_rate(gmap, cells, k, steps, warmup, seed=1000):
     return steps / dt, dt, qs[warmup:], res
 
 
+def kernel_phase(gmap, src, batch_q, n_batches, seed=4242, events=True):
+    """Dispatch timing of served-shaped query batches (batch_q queries of k ~
+    logU[8,256], viewpoints from src's revealed cells), one batch at a time on
+    the map's own stream with start/stop events on every dispatch.  The
+    latency trials run without events: a dispatch that carries them can hold
+    the other queues back for milliseconds, and under contention a kernel's
+    duration measures its neighbours as much as itself."""
+    from htrip_amd import grids, serving
+
+    rng = grids.SplitMix64(seed)
+    batches = [[src.sample_viewpoints(serving.loguniform_k(rng, 8, 256), rng) for _ in range(batch_q)]
+               for _ in range(n_batches + 10)]
+    for b in batches[:10]:
+        gmap.query_batch(b)
+    gmap.profile(events, every=1)
+    t0 = time.perf_counter()
+    for b in batches[10:]:
+        gmap.query_batch(b)
+    dt = time.perf_counter() - t0
+    stats = kernel_stats_of(gmap)
+    gmap.profile(False)
+    return stats, n_batches * batch_q / dt
+
+
 def cpu_baseline(gmap, cells, queries, budget_s, expect, served_checks=()):
     """The reference's own CPU AVX2 queryAllPairs (oracle/_ref), one thread, on
     a bounded sample of the same queries, over tables copied from the GPU.
@@ -251,7 +275,6 @@ def run_served(args):
                          max_batch_viewpoints=args.batch_viewpoints, consistency=mode,
                          update_cu_percent=args.update_cu, update_tick_us=args.update_tick_us)
     uf = args.update_frac
-    kstats = {}  # kernel figures over every session of the served phase
 
     def fresh_session():
         """Every trial starts from the same map state: a new session built from
@@ -262,12 +285,10 @@ def run_served(args):
         sid = srv.add_map(rv.observed, B, G)
         build_ms = 1e3 * (time.time() - t1)
         smap = srv.map_of(sid, rv.observed, B, G)
-        smap.profile(True, every=PROFILE_EVERY)
         srv.replay(serving.make_trace(sid, rv, 100, 1e6, 11, update_frac=0.0))
         return rv, sid, smap, build_ms
 
     def close_session(sid, smap):
-        add_stats(kstats, kernel_stats_of(smap))
         srv.remove_map(sid)
 
     # plain per-call path on the same map state (a separate resident copy)
@@ -294,11 +315,14 @@ def run_served(args):
         state0 = rv.observed.copy() if not trials and not args.no_cpu else None
         tr = serving.make_trace(sid, rv, args.window * args.steps, rate, 100 + len(trials), update_frac=uf)
         st0 = srv.stats()
+        c0, w0 = os.times(), time.perf_counter()
         td = srv.replay(tr)
+        c1, w1 = os.times(), time.perf_counter()
         st1 = srv.stats()
         close_session(sid, smap)
         s = serving.latency_summary(tr, td)
         s["rate"] = rate
+        s["cpu_cores"] = ((c1.user - c0.user) + (c1.system - c0.system)) / max(w1 - w0, 1e-9)
         nb = st1["query_batches"] - st0["query_batches"]
         s["mean_batch_queries"] = (st1["queries"] - st0["queries"]) / max(nb, 1)
         s["worker_plan_busy"] = (st1["plan_seconds"] - st0["plan_seconds"]) / max(s["span_s"], 1e-9)
@@ -308,11 +332,13 @@ def run_served(args):
                           ("finish", "finish_seconds"))}
         trials.append((tr, td, s, state0))
         if args.dump_trials:
+            os.makedirs(os.path.dirname(args.dump_trials) or ".", exist_ok=True)
             np.savez_compressed(f"{args.dump_trials}_{len(trials) - 1}.npz", t=tr.t, done=td, type=tr.type,
                                 count=tr.count, version=tr.version, rate=rate)
         log(f"rate {rate:.0f}/s: {s['matrices_per_s']:.0f} matrices/s, p50 {s['p50_ms']:.3f} ms, "
             f"p99 {s['p99_ms']:.3f} ms, updates p50 {s['update_p50_ms']} ms, batch {s['mean_batch_queries']:.2f}, "
             f"worker planning {100 * s['worker_plan_busy']:.0f}%, updating {100 * s['update_busy']:.0f}%, "
+            f"cpu {s['cpu_cores']:.1f} cores, "
             f"batch ms " + " ".join(f"{k} {v:.3f}" for k, v in s["batch_ms"].items()))
         if s["p99_ms"] < 1.0:
             chosen = len(trials) - 1
@@ -320,7 +346,13 @@ def run_served(args):
     if chosen is None:
         chosen = len(trials) - 1
     tr, td, s = trials[chosen][:3]
-    roof, valu, kernels = roofline_of(kstats)
+    # kernel figures: the chosen trial's mean batch size, dispatch-timed alone
+    bq = max(1, int(round(s["mean_batch_queries"])))
+    kst, kphase_rate = kernel_phase(plain, serving.RobotReveal(truth, ROBOTS, RADIUS, seed=5, limit=40000), bq,
+                                    args.kernel_batches)
+    roof, valu, kernels = roofline_of(kst)
+    log(f"kernel phase: {args.kernel_batches} batches of {bq} queries, k_q_z {1e3 * roof['avg_launch_ms']:.1f} us "
+        f"per launch, {roof['achieved']:.0f} GB/s")
 
     cpu = None
     if not args.no_cpu:
@@ -348,6 +380,33 @@ def run_served(args):
         "server_stats": srv.stats(),
         "rate_trials": [t[2] for t in trials],
         "roofline": roof, "valu": valu, "cpu_baseline": cpu, "kernels": kernels,
+        "kernel_phase": {"batch_queries": bq, "batches": args.kernel_batches,
+                         "matrices_per_s_one_stream": kphase_rate},
+    }
+
+
+def run_kernel(args):
+    """The kernel phase of the served mode alone: served-shaped batches on the
+    serving map at its start state, every dispatch timed (the command whose
+    rocprofv3 summary backs the served line's roofline)."""
+    from htrip_amd import _lib, grids, serving
+
+    truth = grids.street_grid(N, SEED)
+    rv = serving.RobotReveal(truth, ROBOTS, RADIUS, seed=5, limit=40000)
+    gmap = _lib.GpuMap(rv.observed, B, G)
+    gmap.build()
+    kst, rate = kernel_phase(gmap, rv, args.batch_size, args.kernel_batches, events=not args.no_events)
+    roof, valu, kernels = roofline_of(kst)
+    log(f"kernel phase: {args.kernel_batches} batches of {args.batch_size} queries: {rate:.0f} matrices/s, "
+        f"k_q_z {1e3 * roof['avg_launch_ms']:.1f} us per launch, {roof['achieved']:.0f} GB/s")
+    return {
+        "metric": "k x k query matrices/s (served-shaped batches, one stream, every dispatch timed), 1024^2 grid",
+        "value": rate, "unit": "matrices/s", "n_gpus": 1, "steps": args.kernel_batches, "warmup": 10,
+        "ms_per_step": 1e3 * args.batch_size / rate, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u16", "data": "synthetic",
+        "config": {"workload": f"1024x1024 street-like grid seed 3, b=32, g=2x2, batches of {args.batch_size} "
+                               "queryAllPairs k~logU[8,256] at the robots' start state", "mode": "kernel"},
+        "roofline": roof, "valu": valu, "cpu_baseline": None, "kernels": kernels,
     }
 
 
@@ -668,7 +727,7 @@ def main():
                     help="timed steps (served: windows of requests per rate trial, default 200; others 40)")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--mode", default="served",
-                    choices=["plain", "served", "update", "small", "build", "reveal", "multi"])
+                    choices=["plain", "served", "update", "small", "build", "reveal", "multi", "kernel"])
     ap.add_argument("--k", type=int, default=64)
     ap.add_argument("--window", type=int, default=64, help="requests per served step")
     ap.add_argument("--slots", type=int, default=3, help="query streams (the update stream makes one more)")
@@ -680,6 +739,10 @@ def main():
                          "every decreased row (the reference's pivot set)")
     ap.add_argument("--update-requests", type=int, default=1,
                     help="update mode: robot requests coalesced into each onBatchCellsOpened call")
+    ap.add_argument("--kernel-batches", type=int, default=400,
+                    help="served and kernel modes: dispatch-timed served-shaped batches for the kernel figures")
+    ap.add_argument("--batch-size", type=int, default=5, help="kernel mode: queries per batch")
+    ap.add_argument("--no-events", action="store_true", help="kernel mode: no timing events (for an outside profiler)")
     ap.add_argument("--dump-trials", default="", help="served: save each trial's arrival/completion times here")
     ap.add_argument("--batch-queries", type=int, default=64)
     ap.add_argument("--batch-viewpoints", type=int, default=8192)
@@ -696,7 +759,7 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU (one MI355X per box); --gpus must be 1"}))
         sys.exit(2)
     line = {"plain": run_plain, "served": run_served, "update": run_update, "small": run_small, "build": run_build,
-            "reveal": run_reveal, "multi": run_multi}[args.mode](args)
+            "reveal": run_reveal, "multi": run_multi, "kernel": run_kernel}[args.mode](args)
     print(json.dumps(line))
 
 

@@ -132,6 +132,9 @@ void launch_q_front(int nq, const int* q_k, const long long* q_out, const int* q
 // live column groups of W (live_list, live_cnt), col_tiles = max over the
 // items of ceil(live_cnt / 64).  With ksplit == 1 the results go straight to
 // Z (position colpos[col]) and the lifted Xn (exterior port colext[col]).
+// Work counters of the query projection: kWorkSlots (bytes, updates) pairs,
+// summed on the host.
+constexpr int kWorkSlots = 64;
 bool launch_q_z(int M, int ld, int pc, const uint16_t* DU, size_t node_stride, const int* seam_list,
                 const int* seam_off, const int* cols, int ngrp, const int* live_cnt, const int* live_list,
                 int col_tiles, int ldp, const int4* items, int n_items, int ksplit, const uint16_t* X, uint16_t* Zp,

@@ -58,7 +58,7 @@ __global__ __launch_bounds__(256) void k_q_front(int nq, int ib, int leaf_blocks
                                                  unsigned long long* __restrict__ wstat) {
     const int n = b * b;
     const int blk = int(blockIdx.x);
-    if (wstat && blk == 0 && threadIdx.x < 2) wstat[threadIdx.x] = 0;
+    if (wstat && blk == 0 && threadIdx.x < 2 * kWorkSlots) wstat[threadIdx.x] = 0;
     if (blk < nq * ib) {
         const int Q = blk / ib;
         const int k = q_k[Q];
@@ -235,6 +235,10 @@ __global__ __launch_bounds__(256) void k_q_z(
             }
         }
     }
+    // work counters: the wavefronts' figures meet in LDS and wavefront 0 adds
+    // them to one of kWorkSlots counter pairs (one same-address atomic per
+    // wavefront would serialise in the L2 and double the kernel's time)
+    __shared__ unsigned long long wsum[NW][2];
     if (wstat && tid == 0) {
         const int nact = min(64, nlive - blockIdx.y * 64);
         const unsigned long long cols4 = 4ull * nact;
@@ -243,8 +247,8 @@ __global__ __launch_bounds__(256) void k_q_z(
             if (blockIdx.y == 0) bytes += (unsigned long long)m * (a_hi - a_lo) * 2;
             bytes += (unsigned long long)m * cols4 * 2;
         }
-        atomicAdd(&wstat[0], bytes);
-        atomicAdd(&wstat[1], (unsigned long long)rows_done * cols4 * m);
+        wsum[w][0] = bytes;
+        wsum[w][1] = (unsigned long long)rows_done * cols4 * m;
     }
     // minimum over the wavefronts, in wavefront 0
     if (w > 0) {
@@ -253,6 +257,11 @@ __global__ __launch_bounds__(256) void k_q_z(
     }
     __syncthreads();
     if (w > 0) return;
+    if (wstat && tid == 0) {
+        const unsigned slot = (blockIdx.x * 131u + blockIdx.y * 17u + blockIdx.z * 7u) % unsigned(kWorkSlots);
+        atomicAdd(&wstat[2 * slot], wsum[0][0] + wsum[1][0] + wsum[2][0] + wsum[3][0]);
+        atomicAdd(&wstat[2 * slot + 1], wsum[0][1] + wsum[1][1] + wsum[2][1] + wsum[3][1]);
+    }
 #pragma unroll
     for (int j = 0; j < NW - 1; ++j)
 #pragma unroll

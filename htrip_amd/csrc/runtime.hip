@@ -88,24 +88,33 @@ hipEvent_t Profiler::get() {
         return e;
     }
     hipEvent_t e = nullptr;
-    if (hipEventCreate(&e) != hipSuccess) {
+    // no system-scope fence when the event completes: that fence writes back
+    // and invalidates the L2, so the next kernel would start cold
+    if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) {
         (void)hipGetLastError();
         return nullptr;
     }
     return e;
 }
-int Profiler::open(int cls, double bytes, double updates, KTimer& kt) {
+int Profiler::open(int cls, double bytes, double updates, KTimer& kt, hipStream_t st) {
     kt = KTimer{};
     if (!on) return -1;
     std::lock_guard<std::mutex> lk(mu);
-    Pending p{cls, get(), get(), bytes, updates};
+    Pending p{cls, get(), get(), bytes, updates, st};
     if (!p.a || !p.b) {
         if (p.a) pool.push_back(p.a);
         if (p.b) pool.push_back(p.b);
         return -1;
     }
-    kt.a = p.a;
-    kt.b = p.b;
+    if (attach) {
+        kt.a = p.a;
+        kt.b = p.b;
+    } else if (hipEventRecord(p.a, st) != hipSuccess) {
+        (void)hipGetLastError();
+        pool.push_back(p.a);
+        pool.push_back(p.b);
+        return -1;
+    }
     const int token = next_token++ & 0x3fffffff;
     this->open_[token] = p;
     return token;
@@ -115,6 +124,10 @@ void Profiler::close(int token, bool launched) {
     std::lock_guard<std::mutex> lk(mu);
     auto it = open_.find(token);
     if (it == open_.end()) return;
+    if (launched && !attach && hipEventRecord(it->second.b, it->second.st) != hipSuccess) {
+        (void)hipGetLastError();
+        launched = false;
+    }
     if (launched) {
         pending.push_back(it->second);
     } else {
@@ -452,8 +465,11 @@ int Map::reserve_update_ws() {
     size_t nchg = size_t(nl) * nl;
     for (const LevelGeo& L : lv) nchg += size_t(L.nn);
     int rc;
+    size_t nn_max = 0;
+    for (const LevelGeo& L : lv) nn_max = std::max(nn_max, size_t(L.nn));
     if ((rc = ws_upd.ensure(kUpdateListReserve)) || (rc = ws_upd_h.ensure(kUpdateListReserve)) ||
-        (rc = ws_chg.ensure(nchg * 4 + 16)) || (rc = ws_chg_h.ensure(nchg * 4 + 16)))
+        (rc = ws_chg.ensure(nchg * 4 + 16)) || (rc = ws_chg_h.ensure(nchg * 4 + 16)) ||
+        (rc = ws_counts_h.ensure(2 * nn_max * 4 + 16)))
         return rc;
     for (int lvl = 1; lvl <= H; ++lvl)
         if ((rc = ensure_build_ws(lvl))) return rc;
@@ -901,9 +917,11 @@ int Map::open_cells(const int32_t* rcs, int n) {
         launch_compact_flags_live(L.d.nV, dnodes, na, L.d_live8_cnt, L.d_live8_list, L.ng8, flags, plane,
                                   static_cast<int*>(ws_piv.p), static_cast<int*>(ws_bpiv.p), L.d.nV, cnt, stream);
         std::vector<int> hc(static_cast<size_t>(cover ? 2 * na : na));
-        if ((e = hipMemcpyAsync(hc.data(), cnt, hc.size() * 4, hipMemcpyDeviceToHost, stream)) != hipSuccess)
+        if ((rc = ws_counts_h.ensure(hc.size() * 4))) return rc;  // pinned: a pageable copy would block
+        if ((e = hipMemcpyAsync(ws_counts_h.p, cnt, hc.size() * 4, hipMemcpyDeviceToHost, stream)) != hipSuccess)
             return hip_fail(e, "download counts");
         if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "update sync");
+        std::memcpy(hc.data(), ws_counts_h.p, hc.size() * 4);
         long long sum = 0, bsum = 0;
         int mx = 0, mxb = 0;
         for (int i = 0; i < na; ++i) {
@@ -1051,7 +1069,9 @@ int Map::query_finish(QueryWS& ws, uint16_t* out_host) {
     if (out_host && ws.total_out) std::memcpy(out_host, ws.h_out.p, size_t(ws.total_out) * 2);
     if (ws.counted) {
         const unsigned long long* w = static_cast<const unsigned long long*>(ws.h_wstat.p);
-        prof_ref().add_work(HTG_K_Q_PROJECT, double(w[0]), double(w[1]));
+        double by = 0, up = 0;
+        for (int i = 0; i < kWorkSlots; ++i) by += double(w[2 * i]), up += double(w[2 * i + 1]);
+        prof_ref().add_work(HTG_K_Q_PROJECT, by, up);
         ws.counted = false;
     }
     prof_ref().harvest();
@@ -1278,7 +1298,7 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
     ws.counted = prof_ref().take();
     unsigned long long* wstat = nullptr;
     if (ws.counted) {
-        if ((rc = ws.d_wstat.ensure(16)) || (rc = ws.h_wstat.ensure(16))) return rc;
+        if ((rc = ws.d_wstat.ensure(kWorkSlots * 16)) || (rc = ws.h_wstat.ensure(kWorkSlots * 16))) return rc;
         wstat = static_cast<unsigned long long*>(ws.d_wstat.p);
     }
     const int2* srcs = reinterpret_cast<const int2*>(dp + o_src);
@@ -1296,7 +1316,7 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
         const int ne = l < H ? L.d.ne : 0;  // no lift above the root
         // bytes and updates of the projection are counted on the device
         KTimer kt;
-        const int pi = ws.counted ? prof_ref().open(HTG_K_Q_PROJECT, 0.0, 0.0, kt) : -1;
+        const int pi = ws.counted ? prof_ref().open(HTG_K_Q_PROJECT, 0.0, 0.0, kt, st) : -1;
         const size_t part_stride = nsl * L.ldp;
         const bool la = launch_q_z(LP.M, L.d.ld, L.d.pc, static_cast<const uint16_t*>(L.DU.p), L.node_stride,
                    static_cast<const int*>(L.d_seam_list.p), static_cast<const int*>(L.d_seam_off.p),
@@ -1309,7 +1329,7 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
         prof_ref().close(pi, la);
         if (LP.ksplit > 1) {
             const int pf =
-                ws.counted ? prof_ref().open(HTG_K_Q_LIFT, double(ns) * (L.nz + 2.0 * ne + L.d.pc) * 2, 0, kt) : -1;
+                ws.counted ? prof_ref().open(HTG_K_Q_LIFT, double(ns) * (L.nz + 2.0 * ne + L.d.pc) * 2, 0, kt, st) : -1;
             const bool lf = launch_q_fold(ns, L.nz, L.ldz, static_cast<const int*>(L.d_zsrc.p), L.ldp, LP.ksplit,
                           static_cast<const uint16_t*>(ws.Zp.p), part_stride, srcs, L.d.s, L.d.nt, L.ngrp,
                           L.d_live_map, Zl, ne, static_cast<const int*>(L.d_ext_col.p),
@@ -1320,7 +1340,7 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
     }
     if (ns > 0 && !P.mtiles.empty()) {
         KTimer kt;
-        const int pm = ws.counted ? prof_ref().open(HTG_K_Q_MEET, P.m_bytes, P.m_updates, kt) : -1;
+        const int pm = ws.counted ? prof_ref().open(HTG_K_Q_MEET, P.m_bytes, P.m_updates, kt, st) : -1;
         const bool lm = launch_q_meet(I(o_tiles), int(P.mtiles.size() / 8), reinterpret_cast<const int4*>(dp + o_groups),
                                       I(o_glist), I(o_zgeo), g2, I(o_si), I(o_qk),
                                       reinterpret_cast<const long long*>(dp + o_qout), Qb, dout, st, kt);
@@ -1332,13 +1352,18 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
             hipSuccess)
             return hip_fail(e, "download result");
     }
-    if (wstat && (e = hipMemcpyAsync(ws.h_wstat.p, wstat, 16, hipMemcpyDeviceToHost, st)) != hipSuccess)
+    if (wstat && (e = hipMemcpyAsync(ws.h_wstat.p, wstat, kWorkSlots * 16, hipMemcpyDeviceToHost, st)) != hipSuccess)
         return hip_fail(e, "download counters");
     if ((e = hipEventRecord(ws.done, st)) != hipSuccess) return hip_fail(e, "record");
     const auto th3 = std::chrono::steady_clock::now();
-    ws.host_s[0] += std::chrono::duration<double>(th1 - th0).count();
-    ws.host_s[1] += std::chrono::duration<double>(th2 - th1).count();
-    ws.host_s[2] += std::chrono::duration<double>(th3 - th2).count();
+    const double ph[3] = {std::chrono::duration<double>(th1 - th0).count(),
+                          std::chrono::duration<double>(th2 - th1).count(),
+                          std::chrono::duration<double>(th3 - th2).count()};
+    for (int i = 0; i < 3; ++i) {
+        ws.host_s[i] += ph[i];
+        ws.host_max[i] = std::max(ws.host_max[i], ph[i]);
+        ws.host_slow[i] += ph[i] > 0.5e-3;
+    }
     ws.host_batches++;
     return HTG_OK;
 }
@@ -1357,8 +1382,8 @@ int Map::reserve_query_ws(QueryWS& ws, int max_vps, long long max_out) const {
     int rc;
     if ((rc = ws.Q.ensure(qsz * 2)) || (rc = ws.Zp.ensure(nsl * zp * 2)) ||
         (rc = ws.d_out.ensure(size_t(max_out) * 2 + 4)) || (rc = ws.h_out.ensure(size_t(max_out) * 2)) ||
-        (rc = ws.d_plan.ensure(plan)) || (rc = ws.h_plan.ensure(plan)) || (rc = ws.d_wstat.ensure(16)) ||
-        (rc = ws.h_wstat.ensure(16)))
+        (rc = ws.d_plan.ensure(plan)) || (rc = ws.h_plan.ensure(plan)) || (rc = ws.d_wstat.ensure(kWorkSlots * 16)) ||
+        (rc = ws.h_wstat.ensure(kWorkSlots * 16)))
         return rc;
     return HTG_OK;
 }
@@ -1377,9 +1402,12 @@ int QueryWS::init(hipStream_t ext) {
 }
 QueryWS::~QueryWS() {
     if (debug_level() >= 2 && host_batches > 0)
-        std::fprintf(stderr, "[htg] query host time per batch (%lld batches): plan %.1f us, pack+upload %.1f us, "
-                     "launch %.1f us\n", host_batches, 1e6 * host_s[0] / host_batches, 1e6 * host_s[1] / host_batches,
-                     1e6 * host_s[2] / host_batches);
+        std::fprintf(stderr,
+                     "[htg] query host time per batch (%lld batches): plan %.1f us (max %.0f, %lld > 0.5 ms), "
+                     "pack+upload %.1f us (max %.0f, %lld), launch %.1f us (max %.0f, %lld)\n",
+                     host_batches, 1e6 * host_s[0] / host_batches, 1e6 * host_max[0], host_slow[0],
+                     1e6 * host_s[1] / host_batches, 1e6 * host_max[1], host_slow[1], 1e6 * host_s[2] / host_batches,
+                     1e6 * host_max[2], host_slow[2]);
     if (stream) (void)hipStreamSynchronize(stream);
     if (done) (void)hipEventDestroy(done);
     if (stream && own_stream) (void)hipStreamDestroy(stream);
@@ -1724,6 +1752,11 @@ int htg_profile_enable(htg_map* map, int on) {
     if (!map) return fail(HTG_E_INVALID, "null map");
     if (on < 0) return fail(HTG_E_INVALID, "sampling period must be >= 0");
     map->prof_ref().reset();
+    static const bool attach = [] {
+        const char* e = std::getenv("HTG_PROFILE_ATTACH");
+        return e && std::atoi(e) != 0;
+    }();
+    map->prof_ref().attach = attach;
     map->prof_ref().every = on > 1 ? on : 1;
     map->prof_ref().tick = 0;
     map->prof_ref().on = on != 0;

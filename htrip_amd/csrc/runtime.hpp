@@ -176,6 +176,8 @@ struct QueryWS {
     DevBuf d_plan, Q, Zp, d_out, d_wstat;  // d_wstat: k_q_z work counters (bytes, updates)
     bool counted = false;                        // this batch's counters go to the profiler
     double host_s[3] = {0, 0, 0};                // host time: planning, packing + upload, launches
+    double host_max[3] = {0, 0, 0};              // longest of each phase
+    long long host_slow[3] = {0, 0, 0};          // phases over 0.5 ms
     long long host_batches = 0;
     // Runs on `ext` when given (not owned), else on a stream of its own.
     int init(hipStream_t ext = nullptr);
@@ -197,7 +199,13 @@ struct Profiler {
         int cls;
         hipEvent_t a, b;
         double bytes, updates;
+        hipStream_t st;
     };
+    // Events recorded on the stream right before and after the dispatch
+    // (default), or attached to the dispatch itself (HTG_PROFILE_ATTACH=1):
+    // an attached pair slows the kernel it times about twofold on gfx950 and
+    // holds other queues back.
+    bool attach = false;
     std::vector<Pending> pending;          // ended, not yet harvested
     std::unordered_map<int, Pending> open_;  // reserved, not yet closed
     int next_token = 0;
@@ -207,8 +215,9 @@ struct Profiler {
     // Reserves the start/stop events of one kernel dispatch (kt, for the
     // launcher); returns a token for close(), or -1 with null events when
     // timing is off.
-    int open(int cls, double bytes, double updates, KTimer& kt);
-    // launched: the dispatch carried the events; otherwise they are recycled.
+    int open(int cls, double bytes, double updates, KTimer& kt, hipStream_t st);
+    // launched: the dispatch was made (its stop event is recorded now, unless
+    // attached); otherwise the events are recycled.
     void close(int token, bool launched);
     // Accumulate every completed pair; pairs still running stay pending.
     void harvest();
@@ -260,7 +269,7 @@ struct Map {
     bool timing_now = false;  // the current build / update was sampled for timing
     int prof_open(int cls, double bytes, double updates, KTimer& kt) {
         kt = KTimer{};
-        return capturing || !timing_now ? -1 : prof_ref().open(cls, bytes, updates, kt);
+        return capturing || !timing_now ? -1 : prof_ref().open(cls, bytes, updates, kt, stream);
     }
     // Build pivots per node of a level (host): the endpoints of seam pairs
     // whose cells are both passable.
@@ -274,7 +283,7 @@ struct Map {
     DevBuf ws_nodes, ws_flags, ws_piv, ws_bpiv, ws_counts, ws_R, ws_changed, ws_mask, ws_edges, ws_cells, ws_leaves,
         ws_leaf_changed;
     DevBuf ws_upd, ws_chg;  // update: packed per-level lists; changed flags (leaves, then every level)
-    HostBuf ws_upd_h, ws_chg_h;
+    HostBuf ws_upd_h, ws_chg_h, ws_counts_h;  // pinned staging of the lists, changed flags, pivot counts
 
     int build();
     int open_cells(const int32_t* rc, int n);
