@@ -161,20 +161,23 @@ def cpu_baseline(gmap, cells, queries, budget_s, expect, served_checks=()):
             api["import_node"](R.h, lvl, i, gmap.node_DU(lvl, i))
     log(f"cpu baseline: GPU tables imported into the reference in {time.time() - t0:.1f}s")
     done, spent = 0, 0.0
-    for q, vps in enumerate(queries):
-        vps = np.ascontiguousarray(vps, dtype=np.int32)
-        ks = np.array([len(vps)], dtype=np.int32)
-        out = np.empty(len(vps) * len(vps), dtype=np.uint16)
-        spent += api["time_queries"](R.h, vps, ks, 1, 1, out.ctypes.data)
-        if not np.array_equal(out.reshape(len(vps), -1), expect[q]):
-            raise SystemExit("reference CPU result differs from the GPU result")
-        done += 1
+    for _ in range(50):  # the query list, cyclically, until the CPU budget is spent
+        for q, vps in enumerate(queries):
+            vps = np.ascontiguousarray(vps, dtype=np.int32)
+            ks = np.array([len(vps)], dtype=np.int32)
+            out = np.empty(len(vps) * len(vps), dtype=np.uint16)
+            spent += api["time_queries"](R.h, vps, ks, 1, 1, out.ctypes.data)
+            if not np.array_equal(out.reshape(len(vps), -1), expect[q]):
+                raise SystemExit("reference CPU result differs from the GPU result")
+            done += 1
+            if spent >= budget_s:
+                break
         if spent >= budget_s:
             break
     log(f"cpu baseline: {done} queries in {spent:.2f}s, all equal to the GPU results")
     one = {"value": done / spent, "unit": "matrices/s", "cores": 1, "kind": "reference",
            "sample": f"{done} queryAllPairs k={len(queries[0])} (1 thread, reference AVX2 sources) on the same map"}
-    one["all_cores"] = cpu_all_cores(R, queries[:max(done, 1) * 4], expect)
+    one["all_cores"] = cpu_all_cores(R, queries, expect)
     return one
 
 
@@ -196,11 +199,14 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_all_cores(R, queries, expect):
+def cpu_all_cores(R, queries, expect, reps=10):
     """The reference's queryAllPairs on every host core (independent queries
-    over std::thread, each with its own scratch), same exactness gate."""
+    over std::thread, each with its own scratch), same exactness gate; the
+    query list `reps` times over."""
     api = R.api
     cores = host_cores()
+    expect = list(expect) * reps
+    queries = list(queries) * reps
     ks = np.array([len(v) for v in queries], dtype=np.int32)
     allv = np.ascontiguousarray(np.concatenate([np.asarray(v, np.int32) for v in queries]), dtype=np.int32)
     out = np.empty(int((ks.astype(np.int64) ** 2).sum()), dtype=np.uint16)
@@ -730,7 +736,7 @@ def main():
                     choices=["plain", "served", "update", "small", "build", "reveal", "multi", "kernel"])
     ap.add_argument("--k", type=int, default=64)
     ap.add_argument("--window", type=int, default=64, help="requests per served step")
-    ap.add_argument("--slots", type=int, default=3, help="query streams (the update stream makes one more)")
+    ap.add_argument("--slots", type=int, default=4, help="query streams (the update stream makes one more)")
     ap.add_argument("--update-cu", type=int, default=0, help="percent of CUs for update kernels (0 = all)")
     ap.add_argument("--update-frac", type=float, default=0.1, help="share of update requests in the stream")
     ap.add_argument("--update-tick-us", type=int, default=0, help="snapshot mode: minimum spacing of update batches")
@@ -749,7 +755,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=200)
     ap.add_argument("--consistency", default="snapshot", choices=["snapshot", "strict"])
     ap.add_argument("--rate-fracs", type=float, nargs="+",
-                    default=[2.0, 1.75, 1.5, 1.3, 1.15, 1.0, 0.85, 0.7, 0.55, 0.4, 0.25])
+                    default=[2.0, 1.6, 1.35, 1.2, 1.1, 1.0, 0.93, 0.86, 0.8, 0.74, 0.68, 0.62, 0.56, 0.5, 0.44,
+                             0.38, 0.32, 0.26, 0.2])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()

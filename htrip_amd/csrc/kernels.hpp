@@ -15,6 +15,14 @@ struct KTimer {
     hipEvent_t a = nullptr, b = nullptr;
 };
 
+// Launch with the timer's start/stop events attached to the dispatch when it
+// carries them, as a plain launch otherwise.
+#define HTG_LAUNCH_T(K, GRID, BLOCK, LDS, ST, KT, ...)                                          \
+    do {                                                                                         \
+        if ((KT).a) hipExtLaunchKernelGGL(K, GRID, BLOCK, LDS, ST, (KT).a, (KT).b, 0, __VA_ARGS__); \
+        else hipLaunchKernelGGL(K, GRID, BLOCK, LDS, ST, __VA_ARGS__);                           \
+    } while (0)
+
 
 // ---- build / update ------------------------------------------------------
 // Tile-restricted BFS from every cell of each listed leaf; writes the leaf's

@@ -118,7 +118,7 @@ bool launch_leaf_apsp(const uint8_t* pass, int S, int b, int nl, const int* leaf
         const int swg = 4 * (64 / lanes);
         const int chunks = (n + swg - 1) / swg;
         const uint32_t lds = uint32_t(size_t(swg) * n * sizeof(uint16_t));
-        hipExtLaunchKernelGGL(kern, dim3(n_list * chunks), dim3(256), lds, st, kt.a, kt.b, 0, pass, S, b, nl, leaf_list,
+        HTG_LAUNCH_T(kern, dim3(n_list * chunks), dim3(256), lds, st, kt, pass, S, b, nl, leaf_list,
                               chunks, leafD, leafS, changed);
     };
     if (b <= 2) go(k_leaf_apsp<2, uint32_t>, 2);
@@ -611,9 +611,9 @@ static int cover_rpt(int max_c, int col_tiles, int n_list) {
 }
 
 #define HTG_COVER_LAUNCH(HOP2, RPT, CT, L8C, L8L, NG8)                                                             \
-    hipExtLaunchKernelGGL((k_cover_product<HOP2, RPT>),                                                           \
+    HTG_LAUNCH_T((k_cover_product<HOP2, RPT>),                                                           \
                           dim3(unsigned(((max_c + 16 * (RPT)-1) / (16 * (RPT))) * (CT)), unsigned(n_list), 1u),   \
-                          dim3(256), 0, st, kt.a, kt.b, 0, nV, ld, node_list, n_list, counts, cpiv, bpiv,         \
+                          dim3(256), 0, st, kt, nV, ld, node_list, n_list, counts, cpiv, bpiv,         \
                           piv_stride, CT, L8C, L8L, NG8, DU, node_stride)
 
 bool launch_cover_relax(int nV, int ld, const int* node_list, int n_list, const int* counts, const int* cpiv,
@@ -883,7 +883,7 @@ bool launch_closure_pivots(int P, int nV, int ld, const int* node_list, int n_li
     if (n_list == 0 || max_live8 == 0) return false;
     const dim3 grid{unsigned((max_live8 + 15) / 16), unsigned(n_list), 1u};
 #define HTG_CP(PP)                                                                                                 \
-    hipExtLaunchKernelGGL(k_closure_pivots<PP>, grid, dim3(256), 0, st, kt.a, kt.b, 0, nV, ld, node_list, pivots,  \
+    HTG_LAUNCH_T(k_closure_pivots<PP>, grid, dim3(256), 0, st, kt, nV, ld, node_list, pivots,  \
                           piv_stride, counts, t, DU, node_stride, live8_cnt, live8_list, ng8, Rbuf)
     if (P == 16) HTG_CP(16);
     else if (P == 32) HTG_CP(32);
@@ -899,7 +899,7 @@ bool launch_closure_update(int P, int nV, int ld, const int* node_list, int n_li
     const int tiles = (max_live8 + 15) / 16;
     const dim3 grid{unsigned(tiles * tiles), unsigned(n_list), 1u};
 #define HTG_CU(PP)                                                                                             \
-    hipExtLaunchKernelGGL(k_closure_update<PP>, grid, dim3(256), 0, st, kt.a, kt.b, 0, nV, ld, node_list, counts, t, \
+    HTG_LAUNCH_T(k_closure_update<PP>, grid, dim3(256), 0, st, kt, nV, ld, node_list, counts, t, \
                           tiles, DU, node_stride, live8_cnt, live8_list, ng8, Rbuf)
     if (P == 16) HTG_CU(16);
     else if (P == 32) HTG_CU(32);

@@ -319,7 +319,7 @@ bool launch_q_z(int M, int ld, int pc, const uint16_t* DU, size_t node_stride, c
         return v == 16 || v == 32 ? v : 8;
     }();
 #define HTG_QZ(MM, PP)                                                                                             \
-    hipExtLaunchKernelGGL((k_q_z<MM, PP>), grid, dim3(256), 0, st, kt.a, kt.b, 0, ld, pc, DU, node_stride,         \
+    HTG_LAUNCH_T((k_q_z<MM, PP>), grid, dim3(256), 0, st, kt, ld, pc, DU, node_stride,         \
                           seam_list, seam_off, cols, ngrp, live_cnt, live_list, ldp, items, ksplit, X, Zp,         \
                           part_stride, colpos, colext, ext_child, ext_cport, Z, ldz, Xn, ne, wstat)
 #define HTG_QZM(PP)                  \
@@ -395,7 +395,7 @@ bool launch_q_fold(int n, int nz, int ldz, const int* src, int ldp, int ksplit, 
                    int g, const uint16_t* Xprev, uint16_t* Xnext, hipStream_t st, KTimer kt) {
     if (n == 0) return false;
     const long long total = (long long)n * (nz + ne);
-    hipExtLaunchKernelGGL(k_q_fold, dim3(unsigned((total + 255) / 256)), dim3(256), 0, st, kt.a, kt.b, 0, n, nz, ldz,
+    HTG_LAUNCH_T(k_q_fold, dim3(unsigned((total + 255) / 256)), dim3(256), 0, st, kt, n, nz, ldz,
                           src, ldp, ksplit, Zp, part_stride, slot_rc, side, nt, ngrp, live_map, Z, ne, ext_col,
                           ext_child, ext_cport, pc, cs, g, Xprev, Xnext);
     return true;
@@ -490,7 +490,7 @@ bool launch_q_meet(const int* tiles, int n_tiles, const int4* groups, const int*
                    const int* slot_i, const int* q_k, const long long* q_out, const uint16_t* Z, uint16_t* out,
                    hipStream_t st, KTimer kt) {
     if (n_tiles == 0) return false;
-    hipExtLaunchKernelGGL(k_q_meet, dim3(unsigned(n_tiles)), dim3(256), 0, st, kt.a, kt.b, 0, tiles, groups, glist,
+    HTG_LAUNCH_T(k_q_meet, dim3(unsigned(n_tiles)), dim3(256), 0, st, kt, tiles, groups, glist,
                           zgeo, g2, slot_i, q_k, q_out, Z, out);
     return true;
 }

@@ -1178,15 +1178,27 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
         P.fills.push_back(make_int4(int(start & 0xffffffffu), int(start >> 32), int(n8), fill_blocks));
         fill_blocks += int((n8 + 255) / 256);
     };
-    unsigned long long div = 1;
+    // per-level slot keys: kc[s] = key / g2^(l-1) identifies (W, q) at level
+    // l, kw[s] = kc[s] / g2 identifies W; shifts when g2 is a power of two
+    std::vector<unsigned long long>& kc = P.kc;
+    std::vector<unsigned long long>& kw = P.kw;
+    kc.resize(size_t(ns));
+    kw.resize(size_t(ns));
+    for (int s2 = 0; s2 < ns; ++s2) kc[size_t(s2)] = sk[size_t(s2)].key;
+    const bool pow2 = (g2 & (g2 - 1)) == 0;
+    int g2shift = 0;
+    while ((1 << g2shift) < g2) ++g2shift;
     for (int l = 1; l <= H; ++l) {
         const LevelGeo& L = lv[l - 1];
         LevelPlan& LP = P.levels[l - 1];
+        for (int s2 = 0; s2 < ns; ++s2)
+            kw[size_t(s2)] = pow2 ? kc[size_t(s2)] >> g2shift : kc[size_t(s2)] / unsigned(g2);
+        auto qdig = [&](int s2) { return int(kc[size_t(s2)] - kw[size_t(s2)] * unsigned(g2)); };
         // (W, q) runs -> projection items
         int maxrun = 0;
         for (int s = 0; s < ns;) {
             int t = s + 1;
-            while (t < ns && sk[t].key / div == sk[s].key / div) ++t;
+            while (t < ns && kc[size_t(t)] == kc[size_t(s)]) ++t;
             maxrun = std::max(maxrun, t - s);
             s = t;
         }
@@ -1197,10 +1209,10 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
         int rows_max = 0;     // passable seam rows of the largest item
         for (int s = 0; s < ns;) {
             int t = s + 1;
-            while (t < ns && sk[t].key / div == sk[s].key / div) ++t;
+            while (t < ns && kc[size_t(t)] == kc[size_t(s)]) ++t;
             const int2 rc0 = P.slot_rc[s];
             const int W = (rc0.x / L.d.s) * L.d.nt + rc0.y / L.d.s;
-            const int q = int((sk[s].key / div) % unsigned(g2));
+            const int q = qdig(s);
             const int tiles = (L.h_live_cnt[W] + 63) / 64;
             LP.col_tiles = std::max(LP.col_tiles, tiles);
             rows_max = std::max(rows_max, L.seam_live[size_t(W) * g2 + q]);
@@ -1236,21 +1248,21 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
             if (l < H) add_fill(xoff[size_t(l)], a8(nsl * size_t(L.d.ne)));
         }
         // (query, W) groups -> meet tiles per target child q2
-        const unsigned long long divW = div * unsigned(g2);
         for (int a0 = 0; a0 < ns;) {
-            const int q = sk[size_t(by_q[size_t(a0)])].q;
-            const unsigned long long w = sk[size_t(by_q[size_t(a0)])].key / divW;
+            const int s0 = by_q[size_t(a0)];
+            const int q = sk[size_t(s0)].q;
+            const unsigned long long w = kw[size_t(s0)];
             int a1 = a0 + 1;
-            while (a1 < ns && sk[size_t(by_q[size_t(a1)])].q == q && sk[size_t(by_q[size_t(a1)])].key / divW == w) ++a1;
+            while (a1 < ns && sk[size_t(by_q[size_t(a1)])].q == q && kw[size_t(by_q[size_t(a1)])] == w) ++a1;
             const int n = a1 - a0;
             if (n >= 2) {
                 const int gi = int(P.groups.size());
                 P.groups.push_back(make_int4(q, 0, int(P.glist.size()), n));
                 P.glist.insert(P.glist.end(), by_q.begin() + a0, by_q.begin() + a1);
                 for (int j0 = 0; j0 < n;) {
-                    const int q2 = int((sk[size_t(by_q[size_t(a0 + j0)])].key / div) % unsigned(g2));
+                    const int q2 = qdig(by_q[size_t(a0 + j0)]);
                     int je = j0 + 1;
-                    while (je < n && int((sk[size_t(by_q[size_t(a0 + je)])].key / div) % unsigned(g2)) == q2) ++je;
+                    while (je < n && qdig(by_q[size_t(a0 + je)]) == q2) ++je;
                     for (int i0 = 0; i0 < n; i0 += 8)
                         for (int jj = j0; jj < je; jj += 8) {
                             const int tt[8] = {gi, q2, i0, jj, je, l - 1, 0, 0};
@@ -1264,7 +1276,7 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
             }
             a0 = a1;
         }
-        div = divW;
+        kc.swap(kw);
     }
     const auto th1 = std::chrono::steady_clock::now();
     // ---- upload (one packed staging buffer)

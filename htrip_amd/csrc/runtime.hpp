@@ -126,6 +126,7 @@ struct SlotKey {
 
 struct QueryPlan {
     std::vector<SlotKey> sk;          // slot order
+    std::vector<unsigned long long> kc, kw;  // per-level keys of the slots: (W, q) and W
     std::vector<int> by_q, qstart;    // meet order: slots grouped by query
     std::vector<int> q_k, q_vp;
     std::vector<long long> q_out;
