@@ -43,6 +43,12 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+# One hardware queue per stream: the served mode runs 4 query slots, the update
+# stream and the plain map's stream.  With HIP's default of 4, two query slots
+# share a queue and run in order behind each other.  Read when the HIP runtime
+# starts, so it is set before the library is loaded.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+
 PEAK_HBM_GBS = 8000.0
 # 256 CUs x 64 lanes x 2 u16 per packed op x 2.4 GHz = 78.6 T u16 ops/s; one
 # min-plus update is a packed add + a packed min: 39.3 T updates/s.

@@ -28,8 +28,14 @@ struct KTimer {
 // Tile-restricted BFS from every cell of each listed leaf; writes the leaf's
 // D (b^2 x b^2) and port summary S ((4b-4)^2).  When changed != nullptr,
 // changed[i] is set to 1 if listed leaf i's S decreased anywhere.
+// Workgroups an update-path launch may occupy at once (of ~2048 resident on
+// the chip): the grid strides over its work so that every workgroup is
+// dispatched at once and query kernels on other streams find free slots.
+constexpr int kBackgroundWG = 1024;
+
+// max_wg > 0 caps the grid (work items are strided over it).
 bool launch_leaf_apsp(const uint8_t* pass, int S, int b, int nl, const int* leaf_list, int n_list, uint16_t* leafD,
-                      uint16_t* leafS, int* changed, hipStream_t st, KTimer kt = {});
+                      uint16_t* leafS, int* changed, hipStream_t st, KTimer kt = {}, int max_wg = 0);
 
 // D_U of each listed node := block-diagonal child summaries, INF elsewhere.
 // Child summaries come from leafS (lvl 1, stride (4b-4)^2) or the previous

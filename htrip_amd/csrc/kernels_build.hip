@@ -34,13 +34,16 @@ __device__ __forceinline__ uint16_t add1(uint16_t a, uint16_t b) {
 // out as whole D rows (coalesced); port sources also write their S row.
 template <int LANES, typename RowT>
 __global__ __launch_bounds__(256) void k_leaf_apsp(const uint8_t* __restrict__ pass, int S, int b, int nl,
-                                                   const int* __restrict__ leaf_list, int chunks,
+                                                   const int* __restrict__ leaf_list, int chunks, int n_items,
                                                    uint16_t* __restrict__ leafD, uint16_t* __restrict__ leafS,
                                                    int* __restrict__ changed) {
     constexpr int SPW = 64 / LANES;
     constexpr int SWG = 4 * SPW;
     extern __shared__ uint16_t dist[];  // [SWG][b*b]
-    const int li = blockIdx.x / chunks, chunk = blockIdx.x % chunks;
+    // work item = (leaf, chunk of SWG sources); a capped grid strides over them
+    for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
+    if (item != int(blockIdx.x)) __syncthreads();  // the previous item's readers of dist are done
+    const int li = item / chunks, chunk = item % chunks;
     const int leaf = leaf_list ? leaf_list[li] : li;
     const int n = b * b;
     const int R0 = (leaf / nl) * b, C0 = (leaf % nl) * b;
@@ -108,18 +111,21 @@ __global__ __launch_bounds__(256) void k_leaf_apsp(const uint8_t* __restrict__ p
             *dst = v;
         }
     }
+    }
 }
 
 bool launch_leaf_apsp(const uint8_t* pass, int S, int b, int nl, const int* leaf_list, int n_list, uint16_t* leafD,
-                      uint16_t* leafS, int* changed, hipStream_t st, KTimer kt) {
+                      uint16_t* leafS, int* changed, hipStream_t st, KTimer kt, int max_wg) {
     if (n_list == 0) return false;
     const int n = b * b;
     auto go = [&](auto kern, int lanes) {
         const int swg = 4 * (64 / lanes);
         const int chunks = (n + swg - 1) / swg;
+        const int items = n_list * chunks;
+        const int grid = max_wg > 0 ? std::min(items, max_wg) : items;
         const uint32_t lds = uint32_t(size_t(swg) * n * sizeof(uint16_t));
-        HTG_LAUNCH_T(kern, dim3(n_list * chunks), dim3(256), lds, st, kt, pass, S, b, nl, leaf_list,
-                              chunks, leafD, leafS, changed);
+        HTG_LAUNCH_T(kern, dim3(unsigned(grid)), dim3(256), lds, st, kt, pass, S, b, nl, leaf_list, chunks, items,
+                     leafD, leafS, changed);
     };
     if (b <= 2) go(k_leaf_apsp<2, uint32_t>, 2);
     else if (b <= 4) go(k_leaf_apsp<4, uint32_t>, 4);
@@ -989,11 +995,13 @@ void launch_project_su_live(int nV, int ld, int ne, const int* vu_ext, const int
 // Whole items (leaf tables): item t of the list is the contiguous block of
 // elems u16 at src + t*elems; vec: elems is a multiple of 8 (16-byte copies).
 template <bool VEC>
-__global__ __launch_bounds__(256) void k_copy_items(const int* __restrict__ items, size_t elems,
+__global__ __launch_bounds__(256) void k_copy_items(const int* __restrict__ items, int n_items, size_t elems,
                                                     const uint16_t* __restrict__ src, uint16_t* __restrict__ dst) {
-    const size_t base = size_t(items[blockIdx.y]) * elems;
     const size_t units = VEC ? elems / 8 : elems;
-    for (size_t u = size_t(blockIdx.x) * 256 + threadIdx.x; u < units; u += size_t(gridDim.x) * 256) {
+    const size_t total = units * size_t(n_items);
+    for (size_t g = size_t(blockIdx.x) * 256 + threadIdx.x; g < total; g += size_t(gridDim.x) * 256) {
+        const size_t it = g / units, u = g - it * units;
+        const size_t base = size_t(items[it]) * elems;
         if (VEC)
             reinterpret_cast<uint4*>(dst + base)[u] = reinterpret_cast<const uint4*>(src + base)[u];
         else
@@ -1006,12 +1014,11 @@ void launch_copy_items(const int* items, int n_items, size_t elems, const uint16
     if (n_items == 0 || elems == 0) return;
     const bool vec = elems % 8 == 0;
     const size_t units = vec ? elems / 8 : elems;
-    const unsigned bx = unsigned(std::min<size_t>((units + 255) / 256, 64));
-    const dim3 grid{bx, unsigned(n_items), 1u};
+    const unsigned blocks = unsigned(std::min<size_t>((units * size_t(n_items) + 255) / 256, kBackgroundWG));
     if (vec)
-        hipLaunchKernelGGL(k_copy_items<true>, grid, dim3(256), 0, st, items, elems, src, dst);
+        hipLaunchKernelGGL(k_copy_items<true>, dim3(blocks), dim3(256), 0, st, items, n_items, elems, src, dst);
     else
-        hipLaunchKernelGGL(k_copy_items<false>, grid, dim3(256), 0, st, items, elems, src, dst);
+        hipLaunchKernelGGL(k_copy_items<false>, dim3(blocks), dim3(256), 0, st, items, n_items, elems, src, dst);
 }
 
 // Live rows of node tables: for each listed node, the D_U rows of its live
@@ -1024,22 +1031,26 @@ __global__ __launch_bounds__(256) void k_copy_live_rows(const int* __restrict__ 
                                                         const int* __restrict__ vu_ext,
                                                         const uint16_t* __restrict__ srcDU, uint16_t* __restrict__ dstDU,
                                                         const uint16_t* __restrict__ srcSU,
-                                                        uint16_t* __restrict__ dstSU) {
-    const int node = nodes[blockIdx.y];
-    const int t = blockIdx.x;
-    if (t >= 8 * live8_cnt[node]) return;
-    const int v = 8 * live8_list[size_t(node) * ng8 + t / 8] + t % 8;
-    if (v >= nV) return;
-    const size_t roff = size_t(node) * node_stride + size_t(v) * ld;
-    const uint4* s4 = reinterpret_cast<const uint4*>(srcDU + roff);
-    uint4* d4 = reinterpret_cast<uint4*>(dstDU + roff);
-    for (int u = threadIdx.x; u < ld / 8; u += 256) d4[u] = s4[u];
-    const int e = vu_ext[v];
-    if (e < 0) return;
-    const size_t soff = (size_t(node) * ne + e) * ne;
-    const uint2* s2 = reinterpret_cast<const uint2*>(srcSU + soff);
-    uint2* d2 = reinterpret_cast<uint2*>(dstSU + soff);
-    for (int u = threadIdx.x; u < ne / 4; u += 256) d2[u] = s2[u];
+                                                        uint16_t* __restrict__ dstSU, int rows_per_node,
+                                                        int n_nodes) {
+    // work item = (node slot, live row slot); a capped grid strides over them
+    for (int w = blockIdx.x; w < rows_per_node * n_nodes; w += gridDim.x) {
+        const int node = nodes[w / rows_per_node];
+        const int t = w % rows_per_node;
+        if (t >= 8 * live8_cnt[node]) continue;
+        const int v = 8 * live8_list[size_t(node) * ng8 + t / 8] + t % 8;
+        if (v >= nV) continue;
+        const size_t roff = size_t(node) * node_stride + size_t(v) * ld;
+        const uint4* s4 = reinterpret_cast<const uint4*>(srcDU + roff);
+        uint4* d4 = reinterpret_cast<uint4*>(dstDU + roff);
+        for (int u = threadIdx.x; u < ld / 8; u += 256) d4[u] = s4[u];
+        const int e = vu_ext[v];
+        if (e < 0) continue;
+        const size_t soff = (size_t(node) * ne + e) * ne;
+        const uint2* s2 = reinterpret_cast<const uint2*>(srcSU + soff);
+        uint2* d2 = reinterpret_cast<uint2*>(dstSU + soff);
+        for (int u = threadIdx.x; u < ne / 4; u += 256) d2[u] = s2[u];
+    }
 }
 
 void launch_copy_live_rows(const int* nodes, int n_nodes, int nV, int ld, size_t node_stride, const int* live8_cnt,
@@ -1047,9 +1058,10 @@ void launch_copy_live_rows(const int* nodes, int n_nodes, int nV, int ld, size_t
                            const uint16_t* srcDU, uint16_t* dstDU, const uint16_t* srcSU, uint16_t* dstSU,
                            hipStream_t st) {
     if (n_nodes == 0 || max_live8 == 0) return;
-    const dim3 grid{unsigned(8 * max_live8), unsigned(n_nodes), 1u};
-    hipLaunchKernelGGL(k_copy_live_rows, grid, dim3(256), 0, st, nodes, nV, ld, node_stride, live8_cnt, live8_list, ng8,
-                       ne, vu_ext, srcDU, dstDU, srcSU, dstSU);
+    const int rows = 8 * max_live8;
+    const unsigned blocks = unsigned(std::min<long long>((long long)rows * n_nodes, kBackgroundWG));
+    hipLaunchKernelGGL(k_copy_live_rows, dim3(blocks), dim3(256), 0, st, nodes, nV, ld, node_stride, live8_cnt,
+                       live8_list, ng8, ne, vu_ext, srcDU, dstDU, srcSU, dstSU, rows, n_nodes);
 }
 
 // Byte fill (a kernel rather than hipMemsetAsync, so that it can sit in a

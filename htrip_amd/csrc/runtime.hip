@@ -868,7 +868,7 @@ int Map::open_cells(const int32_t* rcs, int n) {
     const int lpi = prof_open(HTG_K_LEAF_APSP, leaf_bytes * nd, 0, lkt);
     const bool lla = launch_leaf_apsp(static_cast<const uint8_t*>(d_pass.p), S, b, nl,
                                       reinterpret_cast<const int*>(up + o_leaves), nd, static_cast<uint16_t*>(leafD.p),
-                                      static_cast<uint16_t*>(leafS.p), chg, stream, lkt);
+                                      static_cast<uint16_t*>(leafS.p), chg, stream, lkt, kBackgroundWG);
     prof_ref().close(lpi, lla);
     stats.n_touched_leaves = nd;
     stats.n_pivots = piv;
@@ -1116,6 +1116,11 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
     std::vector<SlotKey>& sk = P.sk;
     sk.clear();
     sk.reserve(size_t(total_vp));
+    // b and g powers of two (the usual case): the key's digits are bit fields
+    int lb = 0, lg = 0;
+    while ((1 << lb) < b) ++lb;
+    while ((1 << lg) < g) ++lg;
+    const bool shifts = (1 << lb) == b && (1 << lg) == g;
     for (int q = 0; q < nq; ++q)
         for (int i = 0; i < ks[q]; ++i) {
             const int v = P.q_vp[q] + i;
@@ -1125,9 +1130,17 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
             P.vp_valid[v] = ok;
             if (!ok) continue;
             unsigned long long key = 0;
-            for (int l = H; l >= 1; --l) {
-                const int cs = lv[l - 1].d.cs;
-                key = key * unsigned(g * g) + unsigned(((r / cs) % g) * g + (c / cs) % g);
+            if (shifts) {
+                const unsigned gm = unsigned(g - 1);
+                for (int l = H; l >= 1; --l) {  // child side of level l: b * g^(l-1)
+                    const int sh = lb + lg * (l - 1);
+                    key = (key << (2 * lg)) | ((((unsigned(r) >> sh) & gm) << lg) | ((unsigned(c) >> sh) & gm));
+                }
+            } else {
+                for (int l = H; l >= 1; --l) {
+                    const int cs = lv[l - 1].d.cs;
+                    key = key * unsigned(g * g) + unsigned(((r / cs) % g) * g + (c / cs) % g);
+                }
             }
             sk.push_back({key, q, i, v});
         }

@@ -165,7 +165,10 @@ int htg_profile_get(htg_map* map, int kernel_class, htg_kernel_stats* out);
  * queries never wait for updates and see the newest applied prefix of them.
  * Up to n_slots query batches are in flight at once on their own streams,
  * and requests that arrive meanwhile join the next batch as soon as a slot
- * frees.  No counterpart in the reference's C++ API. */
+ * frees.  Each stream needs a hardware queue of its own (the process's
+ * GPU_MAX_HW_QUEUES >= n_slots + 1 plus any other streams; HIP's default is
+ * 4): streams that share a queue run in order behind each other.  No
+ * counterpart in the reference's C++ API. */
 enum {
     HTG_CONSISTENCY_STRICT = 0,  /* a query sees exactly the updates submitted before it */
     HTG_CONSISTENCY_SNAPSHOT = 1 /* double-buffered: a query sees the newest applied prefix of updates */
