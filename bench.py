@@ -55,9 +55,6 @@ PEAK_HBM_GBS = 8000.0
 PEAK_UPDATES = 256 * 64 * 2 * 2.4e9 / 2
 
 N, B, G, SEED, ROBOTS, RADIUS = 1024, 32, 2, 3, 16, 64
-# Timing events cost host time per launch, so the served and update modes time
-# one operation in PROFILE_EVERY (counters and durations from the same ones).
-PROFILE_EVERY = 8
 
 
 def log(*a):
@@ -258,9 +255,14 @@ def run_plain(args):
     t0 = time.time()
     gmap.build()
     log(f"build {1e3 * (time.time() - t0):.1f} ms (S={gmap.S} H={gmap.H})")
-    gmap.profile(True, every=PROFILE_EVERY)
+    gmap.profile(False)
     rate, dt, qs, res = plain_rate(gmap, cells, args.k, args.steps, args.warmup)
     log(f"plain: {args.steps} queries k={args.k}: {rate:.0f} matrices/s")
+    # kernel figures from a second pass over the same queries with every
+    # dispatch timed (the timed pass above carries no events)
+    gmap.profile(True, every=1)
+    for q in qs:
+        gmap.query(q)
     roof, valu, kernels = kernel_roofline(gmap)
     cpu = None if args.no_cpu else cpu_baseline(gmap, cells, qs, args.cpu_seconds, res)
     return {
@@ -430,29 +432,37 @@ def run_update(args):
     from htrip_amd import _lib, grids, serving
 
     truth = grids.street_grid(N, SEED)
-    rv = serving.RobotReveal(truth, ROBOTS, RADIUS, seed=5, limit=40000)
-    gmap = _lib.GpuMap(rv.observed, B, G)
-    gmap.set_tuning(update_pivots=PIVOT_SETS[args.update_pivots])
-    gmap.build()
-    rng = grids.SplitMix64(77)
-    lat, sizes, stats = [], [], []
-    for i in range(args.warmup + args.steps):
-        parts = []
-        for _ in range(args.update_requests):  # the server coalesces queued requests like this
-            m = serving.loguniform_k(rng, 16, 512)
-            parts.append(rv.next_batch(rng.below(ROBOTS), m))
-        cells = np.vstack(parts)
-        if i == args.warmup:
-            gmap.profile(True, every=4)
-        t0 = time.perf_counter()
-        gmap.open_cells(cells)
-        dt = time.perf_counter() - t0
-        if i >= args.warmup:
-            lat.append(dt)
-            sizes.append(len(cells))
-            stats.append(gmap.last_batch_stats())
-    lat = np.array(lat)
-    _, _, kernels = kernel_roofline(gmap)
+
+    def stream(timed):
+        """The same reveal sequence on a fresh map: latencies without timing
+        events, or kernel figures with events around every dispatch."""
+        rv = serving.RobotReveal(truth, ROBOTS, RADIUS, seed=5, limit=40000)
+        gmap = _lib.GpuMap(rv.observed, B, G)
+        gmap.set_tuning(update_pivots=PIVOT_SETS[args.update_pivots])
+        gmap.build()
+        rng = grids.SplitMix64(77)
+        lat, sizes, stats = [], [], []
+        for i in range(args.warmup + args.steps):
+            parts = []
+            for _ in range(args.update_requests):  # the server coalesces queued requests like this
+                m = serving.loguniform_k(rng, 16, 512)
+                parts.append(rv.next_batch(rng.below(ROBOTS), m))
+            cells = np.vstack(parts)
+            if timed and i == args.warmup:
+                gmap.profile(True, every=1)
+            t0 = time.perf_counter()
+            gmap.open_cells(cells)
+            dt = time.perf_counter() - t0
+            if i >= args.warmup:
+                lat.append(dt)
+                sizes.append(len(cells))
+                stats.append(gmap.last_batch_stats())
+        kernels = kernel_roofline(gmap)[2] if timed else None
+        gmap.close()
+        return np.array(lat), sizes, stats, kernels
+
+    lat, sizes, stats, _ = stream(False)
+    _, _, _, kernels = stream(True)
     log(f"update: p50 {1e3 * np.median(lat):.3f} ms, p99 {1e3 * np.percentile(lat, 99):.3f} ms, "
         f"mean cells {np.mean(sizes):.0f}")
     for k, v in kernels.items():
@@ -591,7 +601,6 @@ def run_reveal(args):
     gmap.build()
     state = obs.copy()
     batches, views, lat = [], [], []
-    gmap.profile(True)
     t_all = time.perf_counter()
     for st in range(steps):
         cells = order[st * 256:(st + 1) * 256]
@@ -607,7 +616,17 @@ def run_reveal(args):
     lat = np.array(lat) * 1e3
     log(f"512^2 street b=8 reveal: {steps} steps, step p50 {np.percentile(lat, 50):.3f} ms, "
         f"p99 {np.percentile(lat, 99):.3f} ms")
-    _, _, kernels = kernel_roofline(gmap)
+    # kernel figures: the same steps on a second map with every dispatch timed
+    # (the timed steps above carry no events)
+    gk = _lib.GpuMap(obs, 8, 2)
+    gk.set_tuning(update_pivots=PIVOT_SETS[args.update_pivots])
+    gk.build()
+    gk.profile(True, every=1)
+    for st in range(steps):
+        gk.open_cells(batches[st])
+        gk.query(views[st][0])
+    _, _, kernels = kernel_roofline(gk)
+    gk.close()
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as orc
@@ -683,7 +702,8 @@ def run_multi(args):
         srv.remove_map(sid)
     log(f"burst capacity ~{cap:.0f} requests/s")
     trials = []
-    for frac in args.rate_fracs[:5]:
+    chosen = None
+    for frac in args.rate_fracs:
         sess = fresh()
         tr = traffic(sess, args.steps * 8, cap * frac, 800 + len(trials))
         td = srv.replay(tr)
@@ -695,8 +715,11 @@ def run_multi(args):
         trials.append(s)
         log(f"rate {s['rate']:.0f}/s: {s['matrices_per_s']:.0f} matrices/s, p50 {s['p50_ms']:.3f} ms, "
             f"p99 {s['p99_ms']:.3f} ms, updates p50 {s['update_p50_ms']} ms; {checks} spot checks equal BFS")
-    best = max(trials, key=lambda x: x["matrices_per_s"] if x["p99_ms"] < 10.0 else 0.0)
-    return {"metric": "k x k query matrices/s over 8 resident sessions (mixed k <= 512 and reveals)",
+        if s["p99_ms"] < 1.0:  # the highest tested rate with query p99 < 1 ms, as in the served mode
+            chosen = s
+            break
+    best = chosen if chosen is not None else trials[-1]
+    return {"metric": "k x k query matrices/s over 8 resident sessions (mixed k <= 512 and reveals, p99 < 1 ms)",
             "value": best["matrices_per_s"], "unit": "matrices/s", "n_gpus": 1, "steps": args.steps,
             "warmup": 0, "ms_per_step": best["span_s"] * 1e3 / args.steps, "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "u16", "data": "synthetic",
@@ -761,8 +784,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=200)
     ap.add_argument("--consistency", default="snapshot", choices=["snapshot", "strict"])
     ap.add_argument("--rate-fracs", type=float, nargs="+",
-                    default=[2.0, 1.6, 1.35, 1.2, 1.1, 1.0, 0.93, 0.86, 0.8, 0.74, 0.68, 0.62, 0.56, 0.5, 0.44,
-                             0.38, 0.32, 0.26, 0.2])
+                    default=[3.0, 2.7, 2.45, 2.25, 2.1, 1.95, 1.8, 1.68, 1.56, 1.45, 1.35, 1.25, 1.15, 1.05, 0.95,
+                             0.85, 0.75, 0.65, 0.55, 0.45, 0.35, 0.25])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
