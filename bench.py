@@ -354,7 +354,10 @@ def run_served(args):
             f"worker planning {100 * s['worker_plan_busy']:.0f}%, updating {100 * s['update_busy']:.0f}%, "
             f"cpu {s['cpu_cores']:.1f} cores, "
             f"batch ms " + " ".join(f"{k} {v:.3f}" for k, v in s["batch_ms"].items()))
-        if s["p99_ms"] < 1.0:
+        # a rate passes when query p99 < 1 ms and the updates keep up (their p99
+        # lag stays within a few merged batches; a growing backlog would mean
+        # the stream is not sustained)
+        if s["p99_ms"] < 1.0 and (s["update_p99_ms"] is None or s["update_p99_ms"] < args.update_lag_ms):
             chosen = len(trials) - 1
             break
     if chosen is None:
@@ -376,7 +379,8 @@ def run_served(args):
     value = s["matrices_per_s"]
     span = float(td.max() - tr.t.min())
     return {
-        "metric": "k x k query matrices/s (served, p99 < 1 ms target) + p99 latency, 1024^2 grid",
+        "metric": "k x k query matrices/s (served, query p99 < 1 ms, update p99 < "
+                  f"{args.update_lag_ms:g} ms) + p99 latency, 1024^2 grid",
         "value": value, "unit": "matrices/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
         "ms_per_step": span * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "u16", "data": "synthetic",
@@ -385,7 +389,7 @@ def run_served(args):
                                "starts from the robots' start disks",
                    "mode": "served", "consistency": args.consistency, "window": args.window, "slots": args.slots,
                    "max_batch_queries": args.batch_queries, "update_cu_percent": args.update_cu,
-                   "update_tick_us": args.update_tick_us, "update_frac": uf},
+                   "update_tick_us": args.update_tick_us, "update_frac": uf, "update_lag_ms": args.update_lag_ms},
         "p50_ms": s["p50_ms"], "p99_ms": s["p99_ms"], "max_ms": s["max_ms"],
         "update_p50_ms": s["update_p50_ms"], "update_p99_ms": s["update_p99_ms"],
         "arrival_rate_per_s": s["rate"],
@@ -772,6 +776,8 @@ def main():
     ap.add_argument("--update-pivots", default="cover", choices=sorted(PIVOT_SETS),
                     help="update and reveal modes: close over a vertex cover of the decreased entries, or over "
                          "every decreased row (the reference's pivot set)")
+    ap.add_argument("--update-lag-ms", type=float, default=25.0,
+                    help="served: a rate passes only if update p99 latency stays below this (updates keep up)")
     ap.add_argument("--update-requests", type=int, default=1,
                     help="update mode: robot requests coalesced into each onBatchCellsOpened call")
     ap.add_argument("--kernel-batches", type=int, default=400,

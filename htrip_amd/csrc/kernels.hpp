@@ -124,6 +124,8 @@ void launch_copy_live_rows(const int* nodes, int n_nodes, int nV, int ld, size_t
                            hipStream_t st);
 
 // p[0..n) = v (graph-capturable).
+// src and dst 16-byte aligned; either may be pinned host memory.
+void launch_copy_bytes(const void* src, void* dst, size_t bytes, hipStream_t st);
 void launch_fill_u8(uint8_t* p, size_t n, uint8_t v, hipStream_t st);
 
 // pass[r*S + c] = 1 for the listed cells.

@@ -1064,6 +1064,26 @@ void launch_copy_live_rows(const int* nodes, int n_nodes, int nV, int ld, size_t
                        live8_list, ng8, ne, vu_ext, srcDU, dstDU, srcSU, dstSU, rows, n_nodes);
 }
 
+// Copy by a kernel between device memory and pinned host memory (the device
+// reads or writes the host pages directly).  Small per-batch transfers of the
+// query path go this way: a runtime copy can wait behind other streams' work.
+__global__ __launch_bounds__(256) void k_copy_bytes(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                    size_t n16, const uint8_t* __restrict__ src_tail,
+                                                    uint8_t* __restrict__ dst_tail, int tail) {
+    for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n16; i += size_t(gridDim.x) * 256) dst[i] = src[i];
+    if (blockIdx.x == 0 && int(threadIdx.x) < tail) dst_tail[threadIdx.x] = src_tail[threadIdx.x];
+}
+
+void launch_copy_bytes(const void* src, void* dst, size_t bytes, hipStream_t st) {
+    if (bytes == 0) return;
+    const size_t n16 = bytes / 16;
+    const int tail = int(bytes - n16 * 16);
+    const unsigned blocks = unsigned(std::max<size_t>(1, std::min<size_t>((n16 + 255) / 256, 64)));
+    hipLaunchKernelGGL(k_copy_bytes, dim3(blocks), dim3(256), 0, st, static_cast<const uint4*>(src),
+                       static_cast<uint4*>(dst), n16, static_cast<const uint8_t*>(src) + n16 * 16,
+                       static_cast<uint8_t*>(dst) + n16 * 16, tail);
+}
+
 // Byte fill (a kernel rather than hipMemsetAsync, so that it can sit in a
 // captured graph).
 __global__ __launch_bounds__(256) void k_fill_u8(uint8_t* __restrict__ p, size_t n, uint8_t v) {

@@ -1313,8 +1313,15 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
     if ((rc = ws.d_out.ensure(size_t(total_out) * 2 + 4))) return rc;
     if ((rc = ws.h_out.ensure(size_t(total_out) * 2))) return rc;
     hipStream_t st = ws.stream;
-    if ((e = hipMemcpyAsync(ws.d_plan.p, ws.h_plan.p, pk.size, hipMemcpyHostToDevice, st)) != hipSuccess)
+    static const bool kernel_copy = [] {
+        const char* e = std::getenv("HTG_KERNEL_COPY");
+        return !e || std::atoi(e) != 0;
+    }();
+    if (kernel_copy) {
+        launch_copy_bytes(ws.h_plan.p, ws.d_plan.p, pk.size, st);
+    } else if ((e = hipMemcpyAsync(ws.d_plan.p, ws.h_plan.p, pk.size, hipMemcpyHostToDevice, st)) != hipSuccess) {
         return hip_fail(e, "upload plan");
+    }
     char* dp = static_cast<char*>(ws.d_plan.p);
     auto I = [&](size_t o) { return reinterpret_cast<const int*>(dp + o); };
     uint16_t* dout = static_cast<uint16_t*>(ws.d_out.p);
@@ -1373,8 +1380,10 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
     }
     if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "query launch");
     if (to_host) {
-        if ((e = hipMemcpyAsync(ws.h_out.p, ws.d_out.p, size_t(total_out) * 2, hipMemcpyDeviceToHost, st)) !=
-            hipSuccess)
+        if (kernel_copy)
+            launch_copy_bytes(ws.d_out.p, ws.h_out.p, size_t(total_out) * 2, st);
+        else if ((e = hipMemcpyAsync(ws.h_out.p, ws.d_out.p, size_t(total_out) * 2, hipMemcpyDeviceToHost, st)) !=
+                 hipSuccess)
             return hip_fail(e, "download result");
     }
     if (wstat && (e = hipMemcpyAsync(ws.h_wstat.p, wstat, kWorkSlots * 16, hipMemcpyDeviceToHost, st)) != hipSuccess)
