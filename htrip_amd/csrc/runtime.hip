@@ -297,6 +297,13 @@ static void make_level(LevelGeo& L, int lvl, int b, int g, int S, int H) {
     L.seam_live.assign(size_t(L.nn) * g * g, 0);
     L.ng8 = d.ld / 8;
     L.ngrp = L.ncols / 4;
+    // inverse tables for incremental liveness: group of each aligned quad of
+    // V_U ports (-1: not projected), seam flag of each V_U port
+    L.grp_of4.assign(size_t(d.nV / 4), -1);
+    for (int gi = 0; gi < L.ngrp; ++gi) L.grp_of4[size_t(L.zcols[size_t(4 * gi)] / 4)] = gi;
+    L.seam_flag.assign(size_t(d.nV), 0);
+    for (int q = 0; q < g * g; ++q)
+        for (int i = L.seam_off[q]; i < L.seam_off[q + 1]; ++i) L.seam_flag[size_t(q * d.pc + L.seam_list[i])] = 1;
 }
 
 int Map::init_live() {
@@ -337,7 +344,16 @@ int Map::init_live() {
     return HTG_OK;
 }
 
+static int debug_level();
+
 Map::~Map() {
+    if (debug_level() >= 2 && upd_count > 0)
+        std::fprintf(stderr,
+                     "[htg] update host phases per call (%lld calls): cells+live %.0f us, plan %.0f us, levels "
+                     "host %.0f us, levels device wait %.0f us, stats %.0f us; catch-up %.0f us per call (%lld)\n",
+                     upd_count, 1e6 * upd_time[0] / upd_count, 1e6 * upd_time[1] / upd_count,
+                     1e6 * upd_time[2] / upd_count, 1e6 * upd_time[3] / upd_count, 1e6 * upd_time[4] / upd_count,
+                     catchup_count ? 1e6 * upd_time[5] / catchup_count : 0.0, catchup_count);
     if (build_graph) (void)hipGraphExecDestroy(build_graph);
     if (live_evt) {
         (void)hipEventSynchronize(live_evt);
@@ -358,52 +374,79 @@ int Map::refresh_live(const std::vector<int2>* cells) {
         lo = std::min(lo, o);
         hi = std::max(hi, o + bytes);
     };
+    // insert x into the ascending list a[0..n) (absent); returns the new size
+    auto insert_sorted = [](int* a, int n, int x) {
+        int i = n;
+        while (i > 0 && a[i - 1] > x) {
+            a[i] = a[i - 1];
+            --i;
+        }
+        a[i] = x;
+        return n + 1;
+    };
     for (int l = 1; l <= H; ++l) {
         LevelGeo& L = lv[l - 1];
-        const int s = L.d.s, nt = L.d.nt, ng = L.ngrp;
-        std::vector<int> nodes;
-        if (!cells) {
-            nodes.resize(size_t(L.nn));
-            std::iota(nodes.begin(), nodes.end(), 0);
-        } else {
-            for (const int2& c : *cells) nodes.push_back((c.x / s) * nt + c.y / s);
-            std::sort(nodes.begin(), nodes.end());
-            nodes.erase(std::unique(nodes.begin(), nodes.end()), nodes.end());
+        const int s = L.d.s, nt = L.d.nt, ng = L.ngrp, cs = L.d.cs;
+        int a = L.nn, z = -1;  // touched node range
+        if (!cells) {  // every node from scratch
+            for (int W = 0; W < L.nn; ++W) {
+                const int R0 = (W / nt) * s, C0 = (W % nt) * s;
+                auto live_port = [&](int v) {
+                    const short2 p = L.vu_rc[size_t(v)];
+                    return pass[size_t(R0 + p.x) * S + size_t(C0 + p.y)] != 0;
+                };
+                int cnt = 0;
+                for (int gi = 0; gi < ng; ++gi) {
+                    bool live = false;
+                    for (int k = 0; k < 4 && !live; ++k) live = live_port(L.zcols[size_t(4 * gi + k)]);
+                    L.h_live_map[size_t(W) * ng + gi] = live ? 1 : 0;
+                    if (live) L.h_live_list[size_t(W) * ng + cnt++] = gi;
+                }
+                L.h_live_cnt[W] = cnt;
+                for (int q = 0; q < g * g; ++q) {
+                    int ns_live = 0;
+                    for (int i = L.seam_off[q]; i < L.seam_off[q + 1]; ++i)
+                        ns_live += live_port(q * L.d.pc + L.seam_list[i]);
+                    L.seam_live[size_t(W) * g * g + q] = ns_live;
+                }
+                int c8 = 0;
+                for (int o = 0; o < L.ng8; ++o) {
+                    bool live = false;
+                    for (int v = 8 * o; v < 8 * o + 8 && v < L.d.nV && !live; ++v) live = live_port(v);
+                    if (live) L.h_live8_list[size_t(W) * L.ng8 + c8++] = o;
+                }
+                L.h_live8_cnt[W] = c8;
+            }
+            a = 0;
+            z = L.nn - 1;
+        } else {  // cells that became passable: only their own ports change
+            for (const int2& c : *cells) {
+                const int ty = c.x / s, tx = c.y / s, W = ty * nt + tx;
+                const int r = c.x - ty * s, cc = c.y - tx * s;
+                const int pi = port_index(r % cs, cc % cs, cs);
+                if (pi < 0) continue;  // not a port of its node at this level
+                const int q = (r / cs) * g + cc / cs;
+                const int v = q * L.d.pc + pi;
+                a = std::min(a, W);
+                z = std::max(z, W);
+                const int gi = L.grp_of4[size_t(v / 4)];
+                if (gi >= 0 && !L.h_live_map[size_t(W) * ng + gi]) {
+                    L.h_live_map[size_t(W) * ng + gi] = 1;
+                    L.h_live_cnt[W] = insert_sorted(L.h_live_list + size_t(W) * ng, L.h_live_cnt[W], gi);
+                }
+                if (L.seam_flag[size_t(v)]) L.seam_live[size_t(W) * g * g + q]++;
+                int* l8 = L.h_live8_list + size_t(W) * L.ng8;
+                const int n8 = L.h_live8_cnt[W], o = v / 8;
+                if (!std::binary_search(l8, l8 + n8, o)) L.h_live8_cnt[W] = insert_sorted(l8, n8, o);
+            }
         }
-        if (nodes.empty()) continue;
-        for (int W : nodes) {
-            const int R0 = (W / nt) * s, C0 = (W % nt) * s;
-            auto live_port = [&](int v) {
-                const short2 p = L.vu_rc[size_t(v)];
-                return pass[size_t(R0 + p.x) * S + size_t(C0 + p.y)] != 0;
-            };
-            int cnt = 0;
-            for (int gi = 0; gi < ng; ++gi) {
-                bool live = false;
-                for (int k = 0; k < 4 && !live; ++k) live = live_port(L.zcols[size_t(4 * gi + k)]);
-                L.h_live_map[size_t(W) * ng + gi] = live ? 1 : 0;
-                if (live) L.h_live_list[size_t(W) * ng + cnt++] = gi;
-            }
-            L.h_live_cnt[W] = cnt;
-            for (int q = 0; q < g * g; ++q) {
-                int ns_live = 0;
-                for (int i = L.seam_off[q]; i < L.seam_off[q + 1]; ++i) ns_live += live_port(q * L.d.pc + L.seam_list[i]);
-                L.seam_live[size_t(W) * g * g + q] = ns_live;
-            }
-            int c8 = 0;
-            for (int o = 0; o < L.ng8; ++o) {
-                bool live = false;
-                for (int v = 8 * o; v < 8 * o + 8 && v < L.d.nV && !live; ++v) live = live_port(v);
-                if (live) L.h_live8_list[size_t(W) * L.ng8 + c8++] = o;
-            }
-            L.h_live8_cnt[W] = c8;
-        }
-        const size_t a = size_t(nodes.front()), z = size_t(nodes.back()) + 1;
-        touch(L.h_live_cnt + a, (z - a) * 4);
-        touch(L.h_live_list + a * ng, (z - a) * ng * 4);
-        touch(L.h_live8_cnt + a, (z - a) * 4);
-        touch(L.h_live8_list + a * L.ng8, (z - a) * L.ng8 * 4);
-        touch(L.h_live_map + a * ng, (z - a) * ng);
+        if (z < a) continue;
+        const size_t a0 = size_t(a), z0 = size_t(z) + 1;
+        touch(L.h_live_cnt + a0, (z0 - a0) * 4);
+        touch(L.h_live_list + a0 * ng, (z0 - a0) * ng * 4);
+        touch(L.h_live8_cnt + a0, (z0 - a0) * 4);
+        touch(L.h_live8_list + a0 * L.ng8, (z0 - a0) * L.ng8 * 4);
+        touch(L.h_live_map + a0 * ng, (z0 - a0) * ng);
     }
     if (hi <= lo) return HTG_OK;
     if ((e = hipMemcpyAsync(static_cast<char*>(live_d.p) + lo, hbase + lo, hi - lo, hipMemcpyHostToDevice, stream)) !=
@@ -685,6 +728,15 @@ static int debug_level() {
 // through the uncovered endpoints B (k_cover_relax, k_cover_pairs), which
 // leaves the same closed table with far fewer pivots.
 int Map::open_cells(const int32_t* rcs, int n) {
+    using clk = std::chrono::steady_clock;
+    const auto u0 = clk::now();
+    double sync_s = 0;
+    auto sync = [&]() {
+        const auto a = clk::now();
+        const hipError_t r = hipStreamSynchronize(stream);
+        sync_s += std::chrono::duration<double>(clk::now() - a).count();
+        return r;
+    };
     timing_now = prof_ref().take();
     stats = htg_batch_stats{0, 0, 0};
     dirty.clear();
@@ -694,60 +746,67 @@ int Map::open_cells(const int32_t* rcs, int n) {
     int rc;
     const int dr[4] = {-1, 1, 0, 0}, dc[4] = {0, 0, -1, 1};
     auto passable = [&](int r, int c) { return r >= 0 && c >= 0 && r < rows && c < cols && pass[size_t(r) * S + c]; };
-    // 1. group by leaf (first touch order); remember pre-batch passability
+    // 1. leaves touched by the batch (ascending); cells that become passable
     std::vector<int> dirty;
-    std::map<int, std::vector<int2>> by_leaf;
     std::vector<int2> newly;  // cells that became passable
     std::vector<int2> cells;
-    cells.reserve(n);
+    cells.reserve(size_t(n));
+    dirty.reserve(size_t(n));
     for (int i = 0; i < n; ++i) {
         const int r = rcs[2 * i], c = rcs[2 * i + 1];
         if (r < 0 || c < 0 || r >= S || c >= S) continue;
         cells.push_back(make_int2(r, c));
-        const int leaf = (r / b) * nl + c / b;
-        auto& v = by_leaf[leaf];
-        if (v.empty()) dirty.push_back(leaf);
-        v.push_back(make_int2(r, c));
+        dirty.push_back((r / b) * nl + c / b);
     }
+    std::sort(dirty.begin(), dirty.end());
+    dirty.erase(std::unique(dirty.begin(), dirty.end()), dirty.end());
     if (dirty.empty()) return HTG_OK;
-    std::vector<uint8_t> before;  // passability of touched cells before the batch
-    std::map<long long, uint8_t> prev;
-    auto was = [&](int r, int c) -> bool {
-        if (r < 0 || c < 0 || r >= rows || c >= cols) return false;
-        auto it = prev.find((long long)r * S + c);
-        return it != prev.end() ? it->second : pass[size_t(r) * S + c];
-    };
     for (const int2& p : cells) {
         if (p.x >= rows || p.y >= cols) continue;  // padding cells stay impassable
-        const long long key = (long long)p.x * S + p.y;
-        if (!prev.count(key)) prev[key] = pass[size_t(key)];
-        if (!pass[size_t(key)]) {
-            pass[size_t(key)] = 1;
+        const size_t key = size_t(p.x) * S + p.y;
+        if (!pass[key]) {
+            pass[key] = 1;
             cells_code[size_t(p.x) * cols + p.y] = 1;
             newly.push_back(p);
         }
     }
     if (!newly.empty() && (rc = refresh_live(&newly))) return rc;
-    // 2. leaf pivot counts, as the reference's leaf update would collect them
+    const auto u1 = clk::now();
+    // 2. leaf pivot count, as the reference's leaf update would collect it:
+    // the endpoints of intra-leaf edges with an endpoint opened by this batch
     int piv = 0;
-    for (int leaf : dirty) {
-        const int R0 = (leaf / nl) * b, C0 = (leaf % nl) * b;
-        std::vector<int> K;
-        for (const int2& p : by_leaf[leaf]) {
-            if (!passable(p.x, p.y)) continue;
+    {
+        if (cell_mark.size() != size_t(S) * S) cell_mark.assign(size_t(S) * S, 0);
+        std::vector<size_t> marked;  // distinct endpoints, cleared below
+        marked.reserve(newly.size() * 2);
+        auto add = [&](size_t key) {
+            if (!cell_mark[key]) {
+                cell_mark[key] = 1;
+                marked.push_back(key);
+            }
+        };
+        for (const int2& p : newly) {
+            const int R0 = (p.x / b) * b, C0 = (p.y / b) * b;
             for (int k = 0; k < 4; ++k) {
                 const int vr = p.x + dr[k], vc = p.y + dc[k];
                 if (vr < R0 || vr >= R0 + b || vc < C0 || vc >= C0 + b || !passable(vr, vc)) continue;
-                if (was(p.x, p.y) && was(vr, vc)) continue;
-                K.push_back((p.x - R0) * b + (p.y - C0));
-                K.push_back((vr - R0) * b + (vc - C0));
+                add(size_t(p.x) * S + p.y);
+                add(size_t(vr) * S + vc);
             }
         }
-        std::sort(K.begin(), K.end());
-        piv += int(std::unique(K.begin(), K.end()) - K.begin());
+        piv = int(marked.size());
+        for (size_t key : marked) cell_mark[key] = 0;
     }
-    // 3. seam edges by LCA level and node (src/minplus_oracle.cpp:563-598)
-    std::vector<std::map<int, std::vector<std::pair<int, int>>>> seams(H + 1);
+    // 3. seam edges by LCA level and node (src/minplus_oracle.cpp:563-598),
+    // as (level, node, u, v) sorted and unique
+    struct SeamEdge {
+        int lvl, node, u, v;
+        bool operator<(const SeamEdge& o) const {
+            return lvl != o.lvl ? lvl < o.lvl : node != o.node ? node < o.node : u != o.u ? u < o.u : v < o.v;
+        }
+        bool operator==(const SeamEdge& o) const { return lvl == o.lvl && node == o.node && u == o.u && v == o.v; }
+    };
+    std::vector<SeamEdge> seams;
     for (const int2& p : cells) {
         if (!passable(p.x, p.y)) continue;
         for (int k = 0; k < 4; ++k) {
@@ -772,9 +831,11 @@ int Map::open_cells(const int32_t* rcs, int n) {
             };
             int u = vert(p.x, p.y), v = vert(nr, nc);
             if (u > v) std::swap(u, v);
-            seams[lca][node].push_back({u, v});
+            seams.push_back({lca, node, u, v});
         }
     }
+    std::sort(seams.begin(), seams.end());
+    seams.erase(std::unique(seams.begin(), seams.end()), seams.end());
     // 4. candidate nodes per level: every ancestor of a dirty leaf (plus, to
     // be safe, every node with new seam edges).  The child masks of the
     // reference's routing (a parent is dirty when a child summary changed,
@@ -787,34 +848,37 @@ int Map::open_cells(const int32_t* rcs, int n) {
     std::vector<std::vector<int>> cand(size_t(H) + 1), cslot(size_t(H) + 1);
     std::vector<std::vector<int4>> edges(size_t(H) + 1);
     {
-        std::vector<int> below = dirty;  // level 0: dirty leaves in upload order
+        std::vector<int> below = dirty;  // ascending, as every candidate list
         int below_nt = nl;
+        size_t se = 0;  // seam edges of this level start here
         for (int lvl = 1; lvl <= H; ++lvl) {
             const LevelGeo& L = lv[lvl - 1];
+            size_t se_end = se;
+            while (se_end < seams.size() && seams[se_end].lvl == lvl) ++se_end;
             std::vector<int>& c = cand[size_t(lvl)];
+            c.reserve(below.size() + (se_end - se));
             for (int ch : below) c.push_back(((ch / below_nt) / g) * L.d.nt + (ch % below_nt) / g);
-            for (auto& kv : seams[size_t(lvl)]) c.push_back(kv.first);
+            for (size_t k = se; k < se_end; ++k) c.push_back(seams[k].node);
             std::sort(c.begin(), c.end());
             c.erase(std::unique(c.begin(), c.end()), c.end());
-            std::unordered_map<int, int> pos_below;
-            for (size_t k = 0; k < below.size(); ++k) pos_below[below[k]] = int(k);
             std::vector<int>& cs = cslot[size_t(lvl)];
             cs.assign(c.size() * size_t(g2), -1);
+            size_t k = se;
             for (size_t i = 0; i < c.size(); ++i) {
                 const int tx = c[i] % L.d.nt, ty = c[i] / L.d.nt;
                 for (int q = 0; q < g2; ++q) {
-                    auto it = pos_below.find((ty * g + q / g) * below_nt + tx * g + q % g);
-                    if (it != pos_below.end()) cs[i * size_t(g2) + size_t(q)] = it->second;
+                    const int child = (ty * g + q / g) * below_nt + tx * g + q % g;
+                    auto it = std::lower_bound(below.begin(), below.end(), child);
+                    if (it != below.end() && *it == child) cs[i * size_t(g2) + size_t(q)] = int(it - below.begin());
                 }
-                auto sit = seams[size_t(lvl)].find(c[i]);
-                if (sit == seams[size_t(lvl)].end()) continue;
-                auto& lst = sit->second;
-                std::sort(lst.begin(), lst.end());
-                lst.erase(std::unique(lst.begin(), lst.end()), lst.end());
-                for (auto& uv : lst) edges[size_t(lvl)].push_back(make_int4(int(i), uv.first, uv.second, 1));
+                while (k < se_end && seams[k].node < c[i]) ++k;
+                for (; k < se_end && seams[k].node == c[i]; ++k)
+                    edges[size_t(lvl)].push_back(make_int4(int(i), seams[k].u, seams[k].v, 1));
             }
-            below = c;
+            below.swap(c);
+            c = below;
             below_nt = L.d.nt;
+            se = se_end;
         }
     }
     // new ports per level as (candidate slot, V_U index): the cover of the
@@ -837,6 +901,7 @@ int Map::open_cells(const int32_t* rcs, int n) {
             }
         }
     }
+    const auto u2 = clk::now();
     // one packed upload; changed flags: leaves, then each level's candidates
     Packer pk;
     const size_t o_cells = pk.add(newly), o_leaves = pk.add(dirty);
@@ -920,7 +985,7 @@ int Map::open_cells(const int32_t* rcs, int n) {
         if ((rc = ws_counts_h.ensure(hc.size() * 4))) return rc;  // pinned: a pageable copy would block
         if ((e = hipMemcpyAsync(ws_counts_h.p, cnt, hc.size() * 4, hipMemcpyDeviceToHost, stream)) != hipSuccess)
             return hip_fail(e, "download counts");
-        if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "update sync");
+        if ((e = sync()) != hipSuccess) return hip_fail(e, "update sync");
         std::memcpy(hc.data(), ws_counts_h.p, hc.size() * 4);
         long long sum = 0, bsum = 0;
         int mx = 0, mxb = 0;
@@ -975,6 +1040,7 @@ int Map::open_cells(const int32_t* rcs, int n) {
         below_any = sum > 0;
         chg_below = chg_here;
     }
+    const auto u3 = clk::now();
     // stats: a node is a dirty ancestor when a child summary changed or it
     // got new seam edges (the reference's routing)
     if ((e = hipMemcpyAsync(ws_chg_h.p, ws_chg.p, nchg * 4, hipMemcpyDeviceToHost, stream)) != hipSuccess)
@@ -1000,10 +1066,27 @@ int Map::open_cells(const int32_t* rcs, int n) {
     if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "update sync");
     prof_ref().harvest();
     epoch++;
+    const auto u4 = clk::now();
+    auto dt = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); };
+    upd_time[0] += dt(u0, u1);  // cell bookkeeping and live groups
+    upd_time[1] += dt(u1, u2);  // planning: leaf pivots, seams, candidates
+    upd_time[2] += dt(u2, u3) - sync_s;  // level loop, host side
+    upd_time[3] += sync_s;               // level loop, waiting for the device
+    upd_time[4] += dt(u3, u4);           // changed flags, stats
+    upd_count++;
     return HTG_OK;
 }
 
 int Map::catch_up_from(const Map& src, const DirtySet& ds) {
+    const auto c0 = std::chrono::steady_clock::now();
+    struct Acc {
+        Map* m;
+        std::chrono::steady_clock::time_point t;
+        ~Acc() {
+            m->upd_time[5] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
+            m->catchup_count++;
+        }
+    } acc{this, c0};
     hipError_t e;
     const size_t n = size_t(b) * b, pl = size_t(4 * b - 4);
     for (const int2& c : ds.cells) {

@@ -83,6 +83,8 @@ struct LevelGeo {
     size_t node_stride = 0;  // elements per node D_U (nV * ld)
     std::vector<short2> vu_rc;
     std::vector<int> ext_idx, ext_child, ext_cport;
+    std::vector<int> grp_of4;        // projection group of each aligned quad of V_U ports, -1 if none
+    std::vector<uint8_t> seam_flag;  // per V_U port: a seam port of its child
     std::vector<int2> seams;
     std::vector<int> seam_list, seam_off;  // per-child seam ports
     // Query projection: natural columns (V_U indices) the projection computes,
@@ -299,6 +301,9 @@ struct Map {
     int reserve_query_ws(QueryWS& ws, int max_vps, long long max_out) const;
     int ensure_build_ws(int lvl);
     int reserve_update_ws();
+    std::vector<uint8_t> cell_mark;            // per grid cell, all zero between updates (deduplication)
+    double upd_time[6] = {0, 0, 0, 0, 0, 0};  // update phases (printed at HTG_DEBUG >= 2)
+    long long upd_count = 0, catchup_count = 0;
     // Lay out the live block of every level (once, at creation).
     int init_live();
     // Recompute the live column groups and octets of the nodes that contain
