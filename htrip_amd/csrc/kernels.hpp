@@ -151,9 +151,12 @@ void launch_q_front(int nq, const int* q_k, const long long* q_out, const int* q
 // Work counters of the query projection: kWorkSlots (bytes, updates) pairs,
 // summed on the host.
 constexpr int kWorkSlots = 64;
+// items: two int4 per work item, (node, child | first flag << 16, first slot,
+// slots) and (live column groups of the node, seam range lo, hi, first seam
+// port of the child).
 bool launch_q_z(int M, int ld, int pc, const uint16_t* DU, size_t node_stride, const int* seam_list,
-                const int* seam_off, const int* cols, int ngrp, const int* live_cnt, const int* live_list,
-                int col_tiles, int ldp, const int4* items, int n_items, int ksplit, const uint16_t* X, uint16_t* Zp,
+                const int* cols, int ngrp, const int* live_list, int col_tiles, int ldp, const int4* items,
+                int n_items, int ksplit, const uint16_t* X, uint16_t* Zp,
                 size_t part_stride, const int* colpos, const int* colext, const int* ext_child,
                 const int* ext_cport, uint16_t* Z, int ldz, uint16_t* Xn, int ne, unsigned long long* wstat,
                 hipStream_t st, KTimer kt = {});

@@ -138,8 +138,8 @@ void launch_q_front(int nq, const int* q_k, const long long* q_out, const int* q
 template <int M, int PF>
 __global__ __launch_bounds__(256) void k_q_z(
     int ld, int pc, const uint16_t* __restrict__ DU, size_t node_stride, const int* __restrict__ seam_list,
-    const int* __restrict__ seam_off, const int* __restrict__ cols, int ngrp, const int* __restrict__ live_cnt,
-    const int* __restrict__ live_list, int ldp, const int4* __restrict__ items, int ksplit,
+    const int* __restrict__ cols, int ngrp, const int* __restrict__ live_list, int ldp,
+    const int4* __restrict__ items, int ksplit,
     const uint16_t* __restrict__ X, uint16_t* __restrict__ Zp, size_t part_stride,
     const int* __restrict__ colpos, const int* __restrict__ colext, const int* __restrict__ ext_child,
     const int* __restrict__ ext_cport, uint16_t* __restrict__ Z, int ldz, uint16_t* __restrict__ Xn, int ne,
@@ -151,11 +151,11 @@ __global__ __launch_bounds__(256) void k_q_z(
     __shared__ __align__(16) uint32_t xs[NW][ACH][M];  // per wavefront, row-major: one row's M values contiguous
     __shared__ unsigned roff[NW][ACH];
     __shared__ uint2 red[NW - 1][M][64];  // accumulators of wavefronts 1..NW-1 for the final minimum
-    const int4 it = items[blockIdx.x];
+    const int4 it = items[2 * blockIdx.x], it2 = items[2 * blockIdx.x + 1];
     const int W = it.x, q = it.y & 0xffff, s0 = it.z, m = it.w;
     const bool first_item = (it.y >> 16) & 1;  // first item of its (W, q) run: counts the D bytes once
     const int tid = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int nlive = live_cnt[W];
+    const int nlive = it2.x;  // live column groups of W
     const int gi = blockIdx.y * 64 + tid;
     if (blockIdx.y * 64 >= nlive) return;  // whole workgroup past W's live groups
     const bool active = gi < nlive;
@@ -164,10 +164,10 @@ __global__ __launch_bounds__(256) void k_q_z(
     // so one 8-byte load per seam row feeds the lane's 4 columns.
     const int base = cols[c0];
     const uint16_t* D = DU + size_t(W) * node_stride + base;
-    const int* seam = seam_list + seam_off[q];
-    const int ns = seam_off[q + 1] - seam_off[q];
+    const int* seam = seam_list + it2.y;  // child q's seam ports
+    const int ns = it2.z - it2.y;
     const int a_lo = int((long long)ns * blockIdx.z / ksplit), a_hi = int((long long)ns * (blockIdx.z + 1) / ksplit);
-    const unsigned pad_off = unsigned(q * pc + seam[0]) * unsigned(ld);
+    const unsigned pad_off = unsigned(q * pc + it2.w) * unsigned(ld);
     const int rounds = (a_hi - a_lo + NW * ACH - 1) / (NW * ACH);  // the same for every wavefront
     int rows_done = 0;
     uint32_t acc[M][2];
@@ -306,8 +306,8 @@ __global__ __launch_bounds__(256) void k_q_z(
 }
 
 bool launch_q_z(int M, int ld, int pc, const uint16_t* DU, size_t node_stride, const int* seam_list,
-                const int* seam_off, const int* cols, int ngrp, const int* live_cnt, const int* live_list,
-                int col_tiles, int ldp, const int4* items, int n_items, int ksplit, const uint16_t* X, uint16_t* Zp,
+                const int* cols, int ngrp, const int* live_list, int col_tiles, int ldp, const int4* items,
+                int n_items, int ksplit, const uint16_t* X, uint16_t* Zp,
                 size_t part_stride, const int* colpos, const int* colext, const int* ext_child,
                 const int* ext_cport, uint16_t* Z, int ldz, uint16_t* Xn, int ne, unsigned long long* wstat,
                 hipStream_t st, KTimer kt) {
@@ -320,7 +320,7 @@ bool launch_q_z(int M, int ld, int pc, const uint16_t* DU, size_t node_stride, c
     }();
 #define HTG_QZ(MM, PP)                                                                                             \
     HTG_LAUNCH_T((k_q_z<MM, PP>), grid, dim3(256), 0, st, kt, ld, pc, DU, node_stride,         \
-                          seam_list, seam_off, cols, ngrp, live_cnt, live_list, ldp, items, ksplit, X, Zp,         \
+                          seam_list, cols, ngrp, live_list, ldp, items, ksplit, X, Zp,                             \
                           part_stride, colpos, colext, ext_child, ext_cport, Z, ldz, Xn, ne, wstat)
 #define HTG_QZM(PP)                  \
     if (M <= 1) HTG_QZ(1, PP);       \

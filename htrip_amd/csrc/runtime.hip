@@ -1313,7 +1313,11 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
             LP.col_tiles = std::max(LP.col_tiles, tiles);
             rows_max = std::max(rows_max, L.seam_live[size_t(W) * g2 + q]);
             for (int a = s; a < t; a += M) {
+                // two int4 per item: (node, child | first flag, first slot, slots) and the node's live
+                // group count, the child's seam range and its first seam port (no dependent loads)
                 LP.items.push_back(make_int4(W, q | (a == s ? 1 << 16 : 0), a, std::min(M, t - a)));
+                const int sl0 = L.seam_off[size_t(q)], sl1 = L.seam_off[size_t(q) + 1];
+                LP.items.push_back(make_int4(L.h_live_cnt[W], sl0, sl1, sl1 > sl0 ? L.seam_list[size_t(sl0)] : 0));
                 waves += tiles;
             }
             s = t;
@@ -1434,10 +1438,9 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
         const int pi = ws.counted ? prof_ref().open(HTG_K_Q_PROJECT, 0.0, 0.0, kt, st) : -1;
         const size_t part_stride = nsl * L.ldp;
         const bool la = launch_q_z(LP.M, L.d.ld, L.d.pc, static_cast<const uint16_t*>(L.DU.p), L.node_stride,
-                   static_cast<const int*>(L.d_seam_list.p), static_cast<const int*>(L.d_seam_off.p),
-                   static_cast<const int*>(L.d_zcols.p), L.ngrp, L.d_live_cnt,
+                   static_cast<const int*>(L.d_seam_list.p), static_cast<const int*>(L.d_zcols.p), L.ngrp,
                    L.d_live_list, LP.col_tiles, L.ldp,
-                   reinterpret_cast<const int4*>(dp + o_items[l - 1]), int(LP.items.size()), LP.ksplit, Xl,
+                   reinterpret_cast<const int4*>(dp + o_items[l - 1]), int(LP.items.size() / 2), LP.ksplit, Xl,
                    static_cast<uint16_t*>(ws.Zp.p), part_stride, static_cast<const int*>(L.d_colpos.p),
                    static_cast<const int*>(L.d_colext.p), static_cast<const int*>(L.d_ext_child.p),
                    static_cast<const int*>(L.d_ext_cport.p), Zl, L.ldz, Xn, ne, wstat, st, kt);
