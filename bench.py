@@ -67,6 +67,22 @@ def kernel_stats_of(gmap):
     return {c: gmap.kernel_stats(c) for c in range(len(_lib.KERNEL_NAMES))}
 
 
+TRAFFIC_FILE = "profiles/k_q_z_traffic_kernel_phase_1024_street.txt"
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on the kernel-phase workload, from the
+    calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in
+    TRAFFIC_FILE (None when that file is absent)."""
+    try:
+        for line in open(os.path.join(ROOT, TRAFFIC_FILE)):
+            if line.startswith(f"traffic_bytes_per_launch {kernel} = "):
+                return float(line.split("=")[1])
+    except OSError:
+        pass
+    return None
+
+
 def roofline_of(stats):
     """Roofline of the query projection (k_q_z) from accumulated kernel stats:
     bytes and min-plus updates are counted on the device, time from start/stop
@@ -79,7 +95,8 @@ def roofline_of(stats):
     gbs = (z["bytes"] / n) / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     ups = (z["updates"] / n) / (avg_ms * 1e-3) if avg_ms > 0 else 0.0
     roof = {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": gbs / PEAK_HBM_GBS,
-            "traffic": None, "kernel": "k_q_z", "avg_launch_ms": avg_ms, "launches": z["launches"]}
+            "traffic": measured_traffic("k_q_z"), "kernel": "k_q_z", "avg_launch_ms": avg_ms,
+            "launches": z["launches"], "traffic_source": TRAFFIC_FILE}
     valu = {"kernel": "k_q_z", "achieved_updates_per_s": ups, "peak_updates_per_s": PEAK_UPDATES,
             "frac": ups / PEAK_UPDATES}
     kernels = {_lib.KERNEL_NAMES[c]: stats[c] for c in range(len(_lib.KERNEL_NAMES))}
@@ -415,6 +432,20 @@ def run_kernel(args):
     gmap.build()
     kst, rate = kernel_phase(gmap, rv, args.batch_size, args.kernel_batches, events=not args.no_events)
     roof, valu, kernels = roofline_of(kst)
+    if args.calibrate_copy_mib > 0:
+        # one device-to-device copy of known size by the runtime's copy kernel,
+        # the reference for scaling hardware byte counters in a profiled run
+        hip = C.CDLL("libamdhip64.so")
+        nbytes = args.calibrate_copy_mib << 20
+        bufs = [C.c_void_p(), C.c_void_p()]
+        for bp in bufs:
+            if hip.hipMalloc(C.byref(bp), C.c_size_t(nbytes)) != 0:
+                raise SystemExit("hipMalloc for the calibration copy failed")
+        if hip.hipMemcpy(bufs[1], bufs[0], C.c_size_t(nbytes), 3) != 0 or hip.hipDeviceSynchronize() != 0:
+            raise SystemExit("calibration copy failed")
+        for bp in bufs:
+            hip.hipFree(bp)
+        log(f"calibration copy: {args.calibrate_copy_mib} MiB device to device")
     log(f"kernel phase: {args.kernel_batches} batches of {args.batch_size} queries: {rate:.0f} matrices/s, "
         f"k_q_z {1e3 * roof['avg_launch_ms']:.1f} us per launch, {roof['achieved']:.0f} GB/s")
     return {
@@ -784,6 +815,8 @@ def main():
                     help="served and kernel modes: dispatch-timed served-shaped batches for the kernel figures")
     ap.add_argument("--batch-size", type=int, default=5, help="kernel mode: queries per batch")
     ap.add_argument("--no-events", action="store_true", help="kernel mode: no timing events (for an outside profiler)")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="kernel mode: end with one device-to-device copy of this many MiB (counter calibration)")
     ap.add_argument("--dump-trials", default="", help="served: save each trial's arrival/completion times here")
     ap.add_argument("--batch-queries", type=int, default=64)
     ap.add_argument("--batch-viewpoints", type=int, default=8192)

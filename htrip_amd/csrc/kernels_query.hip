@@ -313,29 +313,15 @@ bool launch_q_z(int M, int ld, int pc, const uint16_t* DU, size_t node_stride, c
                 hipStream_t st, KTimer kt) {
     if (n_items == 0 || col_tiles == 0) return false;
     const dim3 grid{unsigned(n_items), unsigned(col_tiles), unsigned(ksplit)};
-    static const int pf = [] {
-        const char* e = std::getenv("HTG_ZQ_PF");
-        const int v = e ? std::atoi(e) : 8;
-        return v == 16 || v == 32 ? v : 8;
-    }();
-#define HTG_QZ(MM, PP)                                                                                             \
-    HTG_LAUNCH_T((k_q_z<MM, PP>), grid, dim3(256), 0, st, kt, ld, pc, DU, node_stride,         \
-                          seam_list, cols, ngrp, live_list, ldp, items, ksplit, X, Zp,                             \
-                          part_stride, colpos, colext, ext_child, ext_cport, Z, ldz, Xn, ne, wstat)
-#define HTG_QZM(PP)                  \
-    if (M <= 1) HTG_QZ(1, PP);       \
-    else if (M <= 2) HTG_QZ(2, PP);  \
-    else if (M <= 4) HTG_QZ(4, PP);  \
-    else if (M <= 8) HTG_QZ(8, PP);  \
-    else HTG_QZ(16, PP);
-    if (pf == 32) {
-        HTG_QZM(32)
-    } else if (pf == 16) {
-        HTG_QZM(16)
-    } else {
-        HTG_QZM(8)
-    }
-#undef HTG_QZM
+#define HTG_QZ(MM)                                                                                                  \
+    HTG_LAUNCH_T((k_q_z<MM, 8>), grid, dim3(256), 0, st, kt, ld, pc, DU, node_stride, seam_list, cols, ngrp,        \
+                 live_list, ldp, items, ksplit, X, Zp, part_stride, colpos, colext, ext_child, ext_cport, Z, ldz,   \
+                 Xn, ne, wstat)
+    if (M <= 1) HTG_QZ(1);
+    else if (M <= 2) HTG_QZ(2);
+    else if (M <= 4) HTG_QZ(4);
+    else if (M <= 8) HTG_QZ(8);
+    else HTG_QZ(16);
 #undef HTG_QZ
     return true;
 }

@@ -1325,11 +1325,7 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
         {  // Row chunks: one (fold in registers, no fold launch) while an item
            // has at most kRowsPerChunk passable seam rows; beyond that, enough
            // chunks for that bound and to give the launch ~2048 workgroups.
-            static const int kRowsPerChunk = [] {  // 4 wavefronts x 128 rows
-                const char* e = std::getenv("HTG_ZQ_ROWS");
-                const int v = e ? std::atoi(e) : 512;
-                return v >= 64 && v <= 4096 ? v : 512;
-            }();
+            constexpr int kRowsPerChunk = 512;  // 4 wavefronts x 128 rows
             int ks = (rows_max + kRowsPerChunk - 1) / kRowsPerChunk;
             if (ks > 1 && waves > 0) ks = std::max(ks, int((2048 + waves - 1) / waves));
             ks = std::min(ks, std::max(1, L.nsmax / 32));
