@@ -20,7 +20,7 @@ HTG_OK, HTG_E_INVALID, HTG_E_NOMEM, HTG_E_HIP, HTG_E_STATE, HTG_E_RANGE = 0, -1,
 EXPORTS = [
     "htg_ctx_create", "htg_ctx_destroy", "htg_last_error", "htg_ctx_mem_info",
     "htg_map_create", "htg_map_destroy", "htg_map_clone", "htg_map_footprint", "htg_map_info_get",
-    "htg_build", "htg_open_cells", "htg_open_cell", "htg_query_all_pairs", "htg_query_batch",
+    "htg_build", "htg_open_cells", "htg_open_cell", "htg_block_cells", "htg_query_all_pairs", "htg_query_batch",
     "htg_last_batch_stats", "htg_sync",
     "htg_read_leaf_D", "htg_read_leaf_S", "htg_node_dims", "htg_read_node_DU", "htg_read_node_SU",
     "htg_profile_enable", "htg_profile_get",
@@ -92,6 +92,7 @@ def load(path: str = LIB_PATH):
         "htg_build": (C.c_int, [vp]),
         "htg_open_cells": (C.c_int, [vp, pi32, i32]),
         "htg_open_cell": (C.c_int, [vp, i32, i32]),
+        "htg_block_cells": (C.c_int, [vp, pi32, i32]),
         "htg_query_all_pairs": (C.c_int, [vp, pi32, i32, pu16]),
         "htg_query_batch": (C.c_int, [vp, pi32, pi32, i32, pu16]),
         "htg_last_batch_stats": (C.c_int, [vp, C.POINTER(BatchStats)]),
@@ -217,6 +218,14 @@ class GpuMap:
         if 0 <= r < self.cells.shape[0] and 0 <= c < self.cells.shape[1]:
             self.cells[r, c] = 1
         check(self.lib.htg_open_cell(self.h, r, c))
+
+    def block_cells(self, rc) -> None:
+        """onCellBlocked for each listed cell: passable cells become Obstacle."""
+        rc = np.ascontiguousarray(rc, dtype=np.int32).reshape(-1, 2)
+        ok = (rc[:, 0] >= 0) & (rc[:, 1] >= 0) & (rc[:, 0] < self.cells.shape[0]) & (rc[:, 1] < self.cells.shape[1])
+        r, c = rc[ok, 0], rc[ok, 1]
+        self.cells[r, c] = np.where(self.cells[r, c] == 1, 2, self.cells[r, c])
+        check(self.lib.htg_block_cells(self.h, ptr(rc, C.c_int32), len(rc)))
 
     def query(self, vps) -> np.ndarray:
         vps = np.ascontiguousarray(vps, dtype=np.int32).reshape(-1, 2)

@@ -129,7 +129,7 @@ void launch_copy_bytes(const void* src, void* dst, size_t bytes, hipStream_t st)
 void launch_fill_u8(uint8_t* p, size_t n, uint8_t v, hipStream_t st);
 
 // pass[r*S + c] = 1 for the listed cells.
-void launch_set_pass(uint8_t* pass, int S, const int2* cells, int n, hipStream_t st);
+void launch_set_pass(uint8_t* pass, int S, const int2* cells, int n, hipStream_t st, uint8_t v = 1);
 
 // ---- query -----------------------------------------------------------------
 // One launch: out matrices (INF, 0 on the diagonal of valid viewpoints, leaf

@@ -1096,14 +1096,14 @@ void launch_fill_u8(uint8_t* p, size_t n, uint8_t v, hipStream_t st) {
     hipLaunchKernelGGL(k_fill_u8, dim3(blocks), dim3(256), 0, st, p, n, v);
 }
 
-__global__ void k_set_pass(uint8_t* pass, int S, const int2* cells, int n) {
+__global__ void k_set_pass(uint8_t* pass, int S, const int2* cells, int n, uint8_t v) {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < n) pass[size_t(cells[i].x) * S + cells[i].y] = 1;
+    if (i < n) pass[size_t(cells[i].x) * S + cells[i].y] = v;
 }
 
-void launch_set_pass(uint8_t* pass, int S, const int2* cells, int n, hipStream_t st) {
+void launch_set_pass(uint8_t* pass, int S, const int2* cells, int n, hipStream_t st, uint8_t v) {
     if (n == 0) return;
-    hipLaunchKernelGGL(k_set_pass, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, pass, S, cells, n);
+    hipLaunchKernelGGL(k_set_pass, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, pass, S, cells, n, v);
 }
 
 }  // namespace htg

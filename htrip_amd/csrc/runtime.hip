@@ -587,11 +587,13 @@ const uint16_t* Map::child_summaries(int lvl, size_t& stride) const {
     return static_cast<const uint16_t*>(C.SU.p);
 }
 
-std::vector<int> Map::build_pivot_counts(int lvl) const {
+std::vector<int> Map::build_pivot_counts(int lvl, const std::vector<int>* nodes) const {
     const LevelGeo& L = lv[size_t(lvl) - 1];
-    std::vector<int> cnt(size_t(L.nn), 0);
+    const int n_out = nodes ? int(nodes->size()) : L.nn;
+    std::vector<int> cnt(size_t(n_out), 0);
     std::vector<uint8_t> mark(size_t(L.d.nV));
-    for (int W = 0; W < L.nn; ++W) {
+    for (int i = 0; i < n_out; ++i) {
+        const int W = nodes ? (*nodes)[size_t(i)] : i;
         const int R0 = (W / L.d.nt) * L.d.s, C0 = (W % L.d.nt) * L.d.s;
         std::fill(mark.begin(), mark.end(), 0);
         int c = 0;
@@ -602,9 +604,97 @@ std::vector<int> Map::build_pivot_counts(int lvl) const {
             c += !mark[size_t(e.x)] + !mark[size_t(e.y)];
             mark[size_t(e.x)] = mark[size_t(e.y)] = 1;
         }
-        cnt[size_t(W)] = c;
+        cnt[size_t(i)] = c;
     }
     return cnt;
+}
+
+// onCellBlocked (src/minplus_oracle.cpp:247-302), for a list of cells: they
+// become impassable (Obstacle); their leaves are recomputed by the BFS kernel
+// and every ancestor node is rebuilt from its children with the build's
+// kernels on those nodes only (assemble, seam edges, closure, S_U).  Other
+// nodes' tables do not depend on these cells.  The live lists stay a
+// superset: a port that turned impassable keeps an INF row and column, which
+// the live-row kernels and the snapshot copy rely on.
+int Map::block_cells(const int32_t* rcs, int n) {
+    hipError_t e;
+    int rc;
+    timing_now = prof_ref().take();
+    std::vector<int2> closed;
+    std::vector<int> leaves;
+    for (int i = 0; i < n; ++i) {
+        const int r = rcs[2 * i], c = rcs[2 * i + 1];
+        if (r < 0 || c < 0 || r >= rows || c >= cols) continue;
+        const size_t key = size_t(r) * S + c;
+        if (!pass[key]) continue;  // already impassable: no table changes
+        pass[key] = 0;
+        cells_code[size_t(r) * cols + c] = 2;
+        closed.push_back(make_int2(r, c));
+        leaves.push_back((r / b) * nl + c / b);
+    }
+    epoch++;  // the reference bumps it on every call (src/minplus_oracle.cpp:300)
+    if (closed.empty()) return HTG_OK;
+    std::sort(leaves.begin(), leaves.end());
+    leaves.erase(std::unique(leaves.begin(), leaves.end()), leaves.end());
+    // ancestors per level, ascending
+    std::vector<std::vector<int>> anc(size_t(H) + 1);
+    {
+        std::vector<int> below = leaves;
+        int below_nt = nl;
+        for (int lvl = 1; lvl <= H; ++lvl) {
+            const LevelGeo& L = lv[size_t(lvl) - 1];
+            std::vector<int>& a = anc[size_t(lvl)];
+            for (int ch : below) a.push_back(((ch / below_nt) / g) * L.d.nt + (ch % below_nt) / g);
+            std::sort(a.begin(), a.end());
+            a.erase(std::unique(a.begin(), a.end()), a.end());
+            below = a;
+            below_nt = L.d.nt;
+        }
+    }
+    Packer pk;
+    const size_t o_cells = pk.add(closed), o_leaves = pk.add(leaves);
+    std::vector<size_t> o_anc(size_t(H) + 1);
+    for (int lvl = 1; lvl <= H; ++lvl) o_anc[size_t(lvl)] = pk.add(anc[size_t(lvl)]);
+    if ((rc = ws_upd_h.ensure(pk.size)) || (rc = ws_upd.ensure(pk.size))) return rc;
+    pk.write(static_cast<char*>(ws_upd_h.p));
+    if ((e = hipMemcpyAsync(ws_upd.p, ws_upd_h.p, pk.size, hipMemcpyHostToDevice, stream)) != hipSuccess)
+        return hip_fail(e, "upload block lists");
+    const char* up = static_cast<const char*>(ws_upd.p);
+    launch_set_pass(static_cast<uint8_t*>(d_pass.p), S, reinterpret_cast<const int2*>(up + o_cells), int(closed.size()),
+                    stream, 0);
+    const double leaf_bytes = (double(b) * b * b * b + double(4 * b - 4) * (4 * b - 4)) * 2 + double(b) * b;
+    KTimer lkt;
+    const int lpi = prof_open(HTG_K_LEAF_APSP, leaf_bytes * double(leaves.size()), 0, lkt);
+    const bool lla = launch_leaf_apsp(static_cast<const uint8_t*>(d_pass.p), S, b, nl,
+                                      reinterpret_cast<const int*>(up + o_leaves), int(leaves.size()),
+                                      static_cast<uint16_t*>(leafD.p), static_cast<uint16_t*>(leafS.p), nullptr, stream,
+                                      lkt, kBackgroundWG);
+    prof_ref().close(lpi, lla);
+    for (int lvl = 1; lvl <= H; ++lvl) {
+        LevelGeo& L = lv[size_t(lvl) - 1];
+        const std::vector<int>& hn = anc[size_t(lvl)];
+        const int na = int(hn.size());
+        const int* nodes = reinterpret_cast<const int*>(up + o_anc[size_t(lvl)]);
+        if ((rc = ensure_build_ws(lvl))) return rc;
+        launch_fill_u8(static_cast<uint8_t*>(ws_flags.p), size_t(na) * L.d.nV, 0, stream);
+        size_t cstride;
+        const uint16_t* csum = child_summaries(lvl, cstride);
+        launch_node_assemble(lvl, g, L.d.pc, L.d.nV, L.d.ld, L.d.nt, csum, cstride, nodes, na,
+                             static_cast<uint16_t*>(L.DU.p), L.node_stride, stream);
+        launch_seam_relax(L.d.nV, L.d.ld, L.d.s, L.d.nt, static_cast<const int2*>(L.d_seams.p), int(L.seams.size()),
+                          static_cast<const short2*>(L.d_vu_rc.p), static_cast<const uint8_t*>(d_pass.p), S, nodes, na,
+                          static_cast<uint16_t*>(L.DU.p), L.node_stride, static_cast<uint8_t*>(ws_flags.p), stream);
+        launch_compact_flags(L.d.nV, nodes, na, static_cast<uint8_t*>(ws_flags.p), static_cast<int*>(ws_piv.p), L.d.nV,
+                             static_cast<int*>(ws_counts.p), stream);
+        if ((rc = run_closure(lvl, nodes, hn, build_pivot_counts(lvl, &hn)))) return rc;
+        launch_project_su(L.d.nV, L.d.ld, L.d.ne, static_cast<const int*>(L.d_ext_idx.p), nodes, na,
+                          static_cast<const uint16_t*>(L.DU.p), L.node_stride, static_cast<uint16_t*>(L.SU.p), nullptr,
+                          stream);
+    }
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "block launch");
+    if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "block sync");
+    prof_ref().harvest();
+    return HTG_OK;
 }
 
 // Build (src/minplus_oracle.cpp:48-137): leaf APSP, then per level assemble
@@ -1780,6 +1870,17 @@ int htg_open_cell(htg_map* map, int32_t r, int32_t c) {
     int rcode = map->open_cells(rcv, 1);
     map->stats = keep;
     return rcode;
+    HTG_GUARD_END
+}
+
+int htg_block_cells(htg_map* map, const int32_t* rc, int32_t n) {
+    HTG_GUARD_BEGIN
+    if (!map || (n > 0 && !rc)) return fail(HTG_E_INVALID, "null argument");
+    if (n < 0) return fail(HTG_E_INVALID, "negative count");
+    if (!map->built) return fail(HTG_E_STATE, "map not built");
+    hipError_t e = hipSetDevice(map->ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    return map->block_cells(rc, n);
     HTG_GUARD_END
 }
 

@@ -276,7 +276,8 @@ struct Map {
     }
     // Build pivots per node of a level (host): the endpoints of seam pairs
     // whose cells are both passable.
-    std::vector<int> build_pivot_counts(int lvl) const;
+    // (every node of the level, or the listed ones in that order)
+    std::vector<int> build_pivot_counts(int lvl, const std::vector<int>* nodes = nullptr) const;
     int enqueue_build(const std::vector<std::vector<int>>& hc);
     HostBuf live_h;  // live arrays of every level (pinned)
     DevBuf live_d;
@@ -290,6 +291,7 @@ struct Map {
 
     int build();
     int open_cells(const int32_t* rc, int n);
+    int block_cells(const int32_t* rc, int n);
     int query(const int32_t* rc, const int32_t* ks, int nq, uint16_t* out_host, QueryWS& ws);
     // Asynchronous halves of query(): plan + launch on ws.stream (results
     // copied to ws.h_out when to_host), then wait for ws.done.

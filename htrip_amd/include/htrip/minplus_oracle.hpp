@@ -156,6 +156,17 @@ public:
         ++update_counter;
     }
 
+    // src/minplus_oracle.cpp:247-302.  The cell must already be blocked in
+    // grid; the device map marks it Obstacle, recomputes its leaf and rebuilds
+    // the ancestor nodes along its branch.
+    void onCellBlocked(Point p, const GridMap& /*grid*/) {
+        if (!map_) return;
+        const int32_t rc[2] = {p.r, p.c};
+        detail::check(htg_block_cells(map_.get(), rc, 1));
+        ++update_counter;
+        epoch++;
+    }
+
     // src/minplus_oracle.cpp:530-729.  Cells must already be Free in grid
     // (README.md:71-91); cells the grid marks Obstacle carry no edges in the
     // reference (edgeWeight is INF) and are skipped here.

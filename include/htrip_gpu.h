@@ -105,6 +105,12 @@ int htg_build(htg_map* map);
 int htg_open_cells(htg_map* map, const int32_t* rc, int32_t n);
 /* MinPlusOracle::onCellOpened (src/minplus_oracle.cpp:139-177). */
 int htg_open_cell(htg_map* map, int32_t r, int32_t c);
+/* MinPlusOracle::onCellBlocked (src/minplus_oracle.cpp:247-302), for n cells
+ * at once: the listed cells become impassable (Obstacle); their leaves are
+ * recomputed and every ancestor node is rebuilt from its children.  Cells out
+ * of range or already impassable are skipped; last_batch_stats is untouched.
+ * Not supported inside an htg_server session (its maps are reveal-only). */
+int htg_block_cells(htg_map* map, const int32_t* rc, int32_t n);
 /* MinPlusOracle::queryAllPairs (src/minplus_oracle.cpp:934-1426).  rc holds k
  * (row, col) viewpoints; out receives k*k distances, row-major, host memory.
  * Impassable or out-of-range viewpoints give an INF row and column, INF

@@ -133,6 +133,46 @@ def test_update_pivot_sets_agree(gpu_ctx, kind, n, b, seed, robots, batch, steps
     np.testing.assert_array_equal(A.query(vps), grid_bfs(O.cells, vps))
 
 
+@pytest.mark.parametrize("kind,n,b,seed", [("street", 96, 8, 51), ("maze", 64, 8, 52), ("street", 128, 16, 53)])
+def test_blocked_cells_equal_a_fresh_build(gpu_ctx, kind, n, b, seed):
+    """onCellBlocked (batched): after each blocking step every table equals a
+    fresh build of the same grid (no other node depends on the blocked cells),
+    queries equal BFS, last_batch_stats is untouched; reveals after blocking
+    (including of blocked cells) stay exact."""
+    print(f"block {kind} n={n} b={b} seed={seed}", flush=True)
+    truth = make(kind, n, seed)
+    start = grids.sample_free(truth, 1, seed)[0]
+    obs = grids.disk_observed(truth, [tuple(start)], n // 3)
+    G = GpuMap(obs, b, 2)
+    G.build()
+    order = grids.frontier_order(truth, obs)
+    G.open_cells(order[:60])
+    stats = G.last_batch_stats()
+    rng = np.random.default_rng(seed)
+    for step, m in enumerate([1, 1, 5, 40, 3]):
+        free = np.argwhere(G.cells == 1)
+        pick = free[rng.choice(len(free), size=m, replace=False)].astype(np.int32)
+        if step == 1:  # a port cell on a leaf boundary, when one is free
+            edge = free[(free[:, 0] % b == 0) | (free[:, 1] % b == b - 1)]
+            if len(edge):
+                pick = edge[:1].astype(np.int32)
+        G.block_cells(pick)
+        assert G.last_batch_stats() == stats
+        O = Oracle("port", G.cells, b, 2)
+        assert O.build()
+        assert_tables_equal(G, O)
+        vps = grids.sample_free(G.cells, 24, seed + step)
+        np.testing.assert_array_equal(G.query(vps), grid_bfs(G.cells, vps), err_msg=f"step {step}")
+    # reveal again, reopening some blocked cells too
+    blocked = np.argwhere((G.cells == 2) & (truth == 1))[:10].astype(np.int32)
+    G.open_cells(np.vstack([blocked, order[60:200]]))
+    O = Oracle("port", G.cells, b, 2)
+    assert O.build()
+    assert_tables_equal(G, O)
+    vps = grids.sample_free(G.cells, 32, seed + 99)
+    np.testing.assert_array_equal(G.query(vps), grid_bfs(G.cells, vps))
+
+
 def test_single_cell_openings(gpu_ctx):
     """onCellOpened one cell at a time (Suite 5 style) including seam cells."""
     truth = grids.street_grid(64, 31)
