@@ -1271,6 +1271,8 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
     // ---- plan
     QueryPlan& P = ws.plan;
     P.clear();
+    ws.counted = prof_ref().take();  // this batch carries kernel timing
+    const bool counting = ws.counted;
     P.q_k.assign(ks, ks + nq);
     P.q_out.resize(nq);
     P.q_vp.resize(nq);
@@ -1317,9 +1319,26 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
             }
             sk.push_back({key, q, i, v});
         }
-    std::sort(sk.begin(), sk.end(), [](const SlotKey& a, const SlotKey& c) {
-        return a.key != c.key ? a.key < c.key : a.vp < c.vp;  // vp orders (q, i)
-    });
+    const auto tk0 = std::chrono::steady_clock::now();
+    // order by (key, vp): sk is built in vp order, so a stable counting sort on
+    // the key does it when the key is short (2 H log2(g) bits)
+    const int key_bits = shifts ? 2 * lg * H : 64;
+    if (key_bits <= 16 && sk.size() > 32 && (size_t(1) << key_bits) <= 32 * sk.size()) {
+        std::vector<int>& cnt = P.key_count;
+        cnt.assign((size_t(1) << key_bits) + 1, 0);
+        for (const SlotKey& k : sk) cnt[size_t(k.key) + 1]++;
+        for (size_t i = 1; i < cnt.size(); ++i) cnt[i] += cnt[i - 1];
+        std::vector<SlotKey>& tmp = P.sk_tmp;
+        tmp.resize(sk.size());
+        for (const SlotKey& k : sk) tmp[size_t(cnt[size_t(k.key)]++)] = k;
+        sk.swap(tmp);
+    } else {
+        std::sort(sk.begin(), sk.end(), [](const SlotKey& a, const SlotKey& c) {
+            return a.key != c.key ? a.key < c.key : a.vp < c.vp;  // vp orders (q, i)
+        });
+    }
+    const auto tk1 = std::chrono::steady_clock::now();
+    double t_meet = 0;
     const int ns = int(sk.size());
     P.slot_rc.resize(ns);
     P.slot_q.resize(ns);
@@ -1434,6 +1453,7 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
             if (l < H) add_fill(xoff[size_t(l)], a8(nsl * size_t(L.d.ne)));
         }
         // (query, W) groups -> meet tiles per target child q2
+        const auto tm0 = std::chrono::steady_clock::now();
         for (int a0 = 0; a0 < ns;) {
             const int s0 = by_q[size_t(a0)];
             const int q = sk[size_t(s0)].q;
@@ -1451,20 +1471,33 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
                     while (je < n && qdig(by_q[size_t(a0 + je)]) == q2) ++je;
                     for (int i0 = 0; i0 < n; i0 += 8)
                         for (int jj = j0; jj < je; jj += 8) {
-                            const int tt[8] = {gi, q2, i0, jj, je, l - 1, 0, 0};
-                            P.mtiles.insert(P.mtiles.end(), tt, tt + 8);
-                            P.m_updates += double(std::min(8, n - i0)) * std::min(8, je - jj) * L.zlen[size_t(q2)];
-                            P.m_bytes +=
-                                double(std::min(8, n - i0) + std::min(8, je - jj)) * L.zlen[size_t(q2)] * 2;
+                            const size_t o = P.mtiles.size();
+                            P.mtiles.resize(o + 8);
+                            int* tt = P.mtiles.data() + o;
+                            tt[0] = gi, tt[1] = q2, tt[2] = i0, tt[3] = jj, tt[4] = je, tt[5] = l - 1;
+                            tt[6] = tt[7] = 0;
                         }
+                    if (counting) {  // the meet's work figures, for kernel timing only
+                        const double zl = L.zlen[size_t(q2)];
+                        for (int i0 = 0; i0 < n; i0 += 8)
+                            for (int jj = j0; jj < je; jj += 8) {
+                                P.m_updates += double(std::min(8, n - i0)) * std::min(8, je - jj) * zl;
+                                P.m_bytes += double(std::min(8, n - i0) + std::min(8, je - jj)) * zl * 2;
+                            }
+                    }
                     j0 = je;
                 }
             }
             a0 = a1;
         }
+        t_meet += std::chrono::duration<double>(std::chrono::steady_clock::now() - tm0).count();
         kc.swap(kw);
     }
     const auto th1 = std::chrono::steady_clock::now();
+    ws.host_sub[0] += std::chrono::duration<double>(tk0 - th0).count();  // validation, keys
+    ws.host_sub[1] += std::chrono::duration<double>(tk1 - tk0).count();  // sort
+    ws.host_sub[2] += t_meet;                                            // meet tiles
+    ws.host_sub[3] += std::chrono::duration<double>(th1 - tk1).count() - t_meet;  // layout, items
     // ---- upload (one packed staging buffer)
     Packer pk;
     const size_t o_qk = pk.add(P.q_k), o_qout = pk.add(P.q_out), o_qvp = pk.add(P.q_vp), o_vprc = pk.add(P.vp_rc),
@@ -1500,7 +1533,6 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
     uint16_t* dout = static_cast<uint16_t*>(ws.d_out.p);
     const uint16_t* dleafD = static_cast<const uint16_t*>(leafD.p);
     const auto th2 = std::chrono::steady_clock::now();
-    ws.counted = prof_ref().take();
     unsigned long long* wstat = nullptr;
     if (ws.counted) {
         if ((rc = ws.d_wstat.ensure(kWorkSlots * 16)) || (rc = ws.h_wstat.ensure(kWorkSlots * 16))) return rc;
@@ -1614,6 +1646,10 @@ QueryWS::~QueryWS() {
                      host_batches, 1e6 * host_s[0] / host_batches, 1e6 * host_max[0], host_slow[0],
                      1e6 * host_s[1] / host_batches, 1e6 * host_max[1], host_slow[1], 1e6 * host_s[2] / host_batches,
                      1e6 * host_max[2], host_slow[2]);
+    if (debug_level() >= 2 && host_batches > 0)
+        std::fprintf(stderr, "[htg]   planning: keys %.1f us, sort %.1f us, meet tiles %.1f us, layout+items %.1f us\n",
+                     1e6 * host_sub[0] / host_batches, 1e6 * host_sub[1] / host_batches,
+                     1e6 * host_sub[2] / host_batches, 1e6 * host_sub[3] / host_batches);
     if (stream) (void)hipStreamSynchronize(stream);
     if (done) (void)hipEventDestroy(done);
     if (stream && own_stream) (void)hipStreamDestroy(stream);

@@ -128,6 +128,8 @@ struct SlotKey {
 
 struct QueryPlan {
     std::vector<SlotKey> sk;          // slot order
+    std::vector<SlotKey> sk_tmp;      // counting-sort output
+    std::vector<int> key_count;       // counting-sort buckets
     std::vector<unsigned long long> kc, kw;  // per-level keys of the slots: (W, q) and W
     std::vector<int> by_q, qstart;    // meet order: slots grouped by query
     std::vector<int> q_k, q_vp;
@@ -181,6 +183,7 @@ struct QueryWS {
     double host_s[3] = {0, 0, 0};                // host time: planning, packing + upload, launches
     double host_max[3] = {0, 0, 0};              // longest of each phase
     long long host_slow[3] = {0, 0, 0};          // phases over 0.5 ms
+    double host_sub[4] = {0, 0, 0, 0};           // planning: keys, sort, meet tiles, layout + items
     long long host_batches = 0;
     // Runs on `ext` when given (not owned), else on a stream of its own.
     int init(hipStream_t ext = nullptr);
