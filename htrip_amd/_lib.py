@@ -21,7 +21,7 @@ EXPORTS = [
     "htg_ctx_create", "htg_ctx_destroy", "htg_last_error", "htg_ctx_mem_info",
     "htg_map_create", "htg_map_destroy", "htg_map_clone", "htg_map_footprint", "htg_map_info_get",
     "htg_build", "htg_open_cells", "htg_open_cell", "htg_block_cells", "htg_query_all_pairs", "htg_query_batch",
-    "htg_last_batch_stats", "htg_sync",
+    "htg_lift_point", "htg_last_batch_stats", "htg_sync",
     "htg_read_leaf_D", "htg_read_leaf_S", "htg_node_dims", "htg_read_node_DU", "htg_read_node_SU",
     "htg_profile_enable", "htg_profile_get",
     "htg_get_tuning", "htg_set_tuning", "htg_autotune",
@@ -95,6 +95,7 @@ def load(path: str = LIB_PATH):
         "htg_block_cells": (C.c_int, [vp, pi32, i32]),
         "htg_query_all_pairs": (C.c_int, [vp, pi32, i32, pu16]),
         "htg_query_batch": (C.c_int, [vp, pi32, pi32, i32, pu16]),
+        "htg_lift_point": (C.c_int, [vp, i32, i32, pu16, C.c_int64, pi32, pi32, pi32, pi32]),
         "htg_last_batch_stats": (C.c_int, [vp, C.POINTER(BatchStats)]),
         "htg_sync": (C.c_int, [vp]),
         "htg_read_leaf_D": (C.c_int, [vp, i32, pu16]),
@@ -246,6 +247,18 @@ class GpuMap:
             res.append(out[o:o + k * k].reshape(k, k))
             o += int(k) * int(k)
         return res
+
+    def lift_point(self, r: int, c: int):
+        """liftPoint: (X, parent_node_idx, child_quad, child_offset); X[l] is the
+        distance vector of level l (leaf ports, then each ancestor's exterior)."""
+        H = self.H
+        cap = 4 * self.b - 4 + sum(self.node_dims(l)[2] for l in range(1, H + 1))
+        x = np.empty(cap, dtype=np.uint16)
+        lens, par, quad, off = (np.empty(H + 1, dtype=np.int32) for _ in range(4))
+        check(self.lib.htg_lift_point(self.h, int(r), int(c), ptr(x, C.c_uint16), cap, ptr(lens, C.c_int32),
+                                      ptr(par, C.c_int32), ptr(quad, C.c_int32), ptr(off, C.c_int32)))
+        o = np.concatenate([[0], np.cumsum(lens)])
+        return [x[o[l]:o[l + 1]] for l in range(H + 1)], par, quad, off
 
     def info(self) -> MapInfo:
         i = MapInfo()

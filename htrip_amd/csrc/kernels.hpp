@@ -167,6 +167,14 @@ bool launch_q_fold(int n, int nz, int ldz, const int* src, int ldp, int ksplit, 
                    size_t part_stride, const int2* slot_rc, int side, int nt, int ngrp, const uint8_t* live_map,
                    uint16_t* Z, int ne, const int* ext_col, const int* ext_child, const int* ext_cport, int pc, int cs,
                    int g, const uint16_t* Xprev, uint16_t* Xnext, hipStream_t st, KTimer kt = {});
+// liftPoint: X0[i] = row[cell of leaf port i] (row: the point's row of its leaf D).
+void launch_lift_leaf(const uint16_t* row, int b, uint16_t* X0, hipStream_t st);
+// liftPoint, one level: Xo[e] = min_p X[p] (+) D[row0 + p][column of exterior
+// port e] over the pc rows of the point's child, in nchunks chunks of
+// rows_per_chunk rows (part: nchunks x ld scratch).
+void launch_lift_level(const uint16_t* D, int ld, int row0, int pc, int rows_per_chunk, int nchunks,
+                       const uint16_t* X, uint16_t* part, const int* ext_child, const int* ext_cport, int ne,
+                       uint16_t* Xo, hipStream_t st);
 // Meet tiles of every level (8 ints each: group, q2, i0, j0, j_end, level):
 // out[i][j] = min(out, min over the q2 seam block of Z_level[i] (+) Z_level[j]).
 bool launch_q_meet(const int* tiles, int n_tiles, const int4* groups, const int* glist, const int* zgeo, int g2,

@@ -481,4 +481,66 @@ bool launch_q_meet(const int* tiles, int n_tiles, const int4* groups, const int*
     return true;
 }
 
+// ---------------------------------------------------------------------------
+// liftPoint (src/minplus_oracle.cpp:731-823): distances from one cell to the
+// ports of its leaf and of every ancestor's exterior.
+//
+// Level 0: X0[i] = D_leaf[u][cell of port i] (one row of the leaf table).
+__global__ __launch_bounds__(256) void k_lift_leaf(const uint16_t* __restrict__ row, int b,
+                                                   uint16_t* __restrict__ X0) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= 4 * b - 4) return;
+    int r, c;
+    port_rc(i, b, r, c);
+    X0[i] = row[r * b + c];
+}
+
+// Level l: the rows of child q in the node's D_U, against the child's lift.
+// Workgroup (x, y) covers 512 columns (two per lane, u32 loads) and the rows
+// [y * rows_per_chunk, ...) of the child: part[y][j] = min_p X[p] (+) D[row0 + p][j].
+// X[p] is the same for every lane (a scalar load), and rows with X[p] = INF
+// are skipped as the reference skips inactive ports.
+__global__ __launch_bounds__(256) void k_lift_rows(const uint16_t* __restrict__ D, int ld, int row0, int pc,
+                                                   int rows_per_chunk, const uint16_t* __restrict__ X,
+                                                   uint16_t* __restrict__ part) {
+    const int j2 = blockIdx.x * 256 + threadIdx.x;  // column pair
+    const int p0 = blockIdx.y * rows_per_chunk;
+    const int p1 = min(pc, p0 + rows_per_chunk);
+    if (2 * j2 >= ld) return;
+    uint32_t acc = kInf2;
+    for (int p = p0; p < p1; ++p) {
+        const uint32_t x = X[p];
+        if (x >= kInf) continue;
+        const uint32_t d = reinterpret_cast<const uint32_t*>(D + size_t(row0 + p) * ld)[j2];
+        acc = qrelax2(acc, qdup(x), d);
+    }
+    reinterpret_cast<uint32_t*>(part + size_t(blockIdx.y) * ld)[j2] = acc;
+}
+
+// Xo[e] = min over row chunks of part[chunk][column of exterior port e].
+__global__ __launch_bounds__(256) void k_lift_ext(const uint16_t* __restrict__ part, int ld, int nchunks,
+                                                  const int* __restrict__ ext_child,
+                                                  const int* __restrict__ ext_cport, int pc, int ne,
+                                                  uint16_t* __restrict__ Xo) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= ne) return;
+    const int col = ext_child[e] * pc + ext_cport[e];
+    uint16_t v = kInf;
+    for (int ch = 0; ch < nchunks; ++ch) v = min(v, part[size_t(ch) * ld + col]);
+    Xo[e] = v;
+}
+
+void launch_lift_leaf(const uint16_t* row, int b, uint16_t* X0, hipStream_t st) {
+    hipLaunchKernelGGL(k_lift_leaf, dim3(unsigned((4 * b - 4 + 255) / 256)), dim3(256), 0, st, row, b, X0);
+}
+
+void launch_lift_level(const uint16_t* D, int ld, int row0, int pc, int rows_per_chunk, int nchunks,
+                       const uint16_t* X, uint16_t* part, const int* ext_child, const int* ext_cport, int ne,
+                       uint16_t* Xo, hipStream_t st) {
+    hipLaunchKernelGGL(k_lift_rows, dim3(unsigned((ld / 2 + 255) / 256), unsigned(nchunks)), dim3(256), 0, st, D,
+                       ld, row0, pc, rows_per_chunk, X, part);
+    hipLaunchKernelGGL(k_lift_ext, dim3(unsigned((ne + 255) / 256)), dim3(256), 0, st, part, ld, nchunks, ext_child,
+                       ext_cport, pc, ne, Xo);
+}
+
 }  // namespace htg

@@ -1236,6 +1236,68 @@ int Map::query(const int32_t* rcs, const int32_t* ks, int nq, uint16_t* out_host
     return query_finish(ws, out_host);
 }
 
+// liftPoint (src/minplus_oracle.cpp:731-823).  Level 0 is the point's row of
+// its leaf table at the leaf's ports; level l is the vector-matrix product of
+// level l-1 with the rows of the point's child in the ancestor's D_U, read at
+// the ancestor's exterior ports (the reference's child_to_ext submatrix,
+// src/internal_node.cpp:435-455).  One stream, synchronous.
+int Map::lift_point(int r, int c, uint16_t* x, long long x_cap, int32_t* lens, int32_t* parent, int32_t* quad,
+                    int32_t* offset) {
+    if (r < 0 || c < 0 || r >= S || c >= S) return fail(HTG_E_RANGE, "point outside the hierarchy");
+    std::vector<long long> off(size_t(H) + 2, 0);
+    lens[0] = 4 * b - 4;
+    for (int l = 1; l <= H; ++l) lens[l] = lv[size_t(l) - 1].d.ne;
+    for (int l = 0; l <= H; ++l) off[size_t(l) + 1] = off[size_t(l)] + lens[l];
+    const long long total = off[size_t(H) + 1];
+    if (x_cap < total) return fail(HTG_E_RANGE, "lift buffer too small");
+    parent[0] = quad[0] = -1;
+    offset[0] = 0;
+    int tx = c / b, ty = r / b;
+    for (int l = 1; l <= H; ++l) {
+        const LevelGeo& L = lv[size_t(l) - 1];
+        parent[l] = (ty / g) * L.d.nt + tx / g;
+        quad[l] = (ty % g) * g + tx % g;
+        offset[l] = quad[l] * L.d.pc;
+        tx /= g;
+        ty /= g;
+    }
+    // row-chunk scratch: enough workgroups to spread the rows over the chip
+    std::vector<int> rpc_l(size_t(H) + 1, 0), nch_l(size_t(H) + 1, 0);
+    size_t part = 8;
+    for (int l = 1; l <= H; ++l) {
+        const LevelGeo& L = lv[size_t(l) - 1];
+        const int col_blocks = (L.d.ld / 2 + 255) / 256;
+        int nch = std::max(1, std::min(512 / col_blocks, (L.d.pc + 15) / 16));
+        const int rpc = (L.d.pc + nch - 1) / nch;
+        nch = (L.d.pc + rpc - 1) / rpc;
+        rpc_l[size_t(l)] = rpc;
+        nch_l[size_t(l)] = nch;
+        part = std::max(part, size_t(nch) * L.d.ld);
+    }
+    int rc;
+    if ((rc = lift_x.ensure(size_t(total) * 2 + 16)) || (rc = lift_part.ensure(part * 2)) ||
+        (rc = lift_h.ensure(size_t(total) * 2 + 16)))
+        return rc;
+    uint16_t* X = static_cast<uint16_t*>(lift_x.p);
+    const size_t n = size_t(b) * b;
+    const int leaf = (r / b) * nl + c / b, u = (r % b) * b + c % b;
+    launch_lift_leaf(static_cast<const uint16_t*>(leafD.p) + size_t(leaf) * n * n + size_t(u) * n, b, X, stream);
+    for (int l = 1; l <= H; ++l) {
+        const LevelGeo& L = lv[size_t(l) - 1];
+        launch_lift_level(static_cast<const uint16_t*>(L.DU.p) + size_t(parent[l]) * L.node_stride, L.d.ld,
+                          quad[l] * L.d.pc, L.d.pc, rpc_l[size_t(l)], nch_l[size_t(l)], X + off[size_t(l) - 1],
+                          static_cast<uint16_t*>(lift_part.p), static_cast<const int*>(L.d_ext_child.p),
+                          static_cast<const int*>(L.d_ext_cport.p), L.d.ne, X + off[size_t(l)], stream);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "lift launch");
+    if ((e = hipMemcpyAsync(lift_h.p, X, size_t(total) * 2, hipMemcpyDeviceToHost, stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(stream)) != hipSuccess)
+        return hip_fail(e, "lift download");
+    std::memcpy(x, lift_h.p, size_t(total) * 2);
+    return HTG_OK;
+}
+
 int Map::query_finish(QueryWS& ws, uint16_t* out_host) {
     hipError_t e = hipEventSynchronize(ws.done);
     if (e != hipSuccess) return hip_fail(e, "query sync");
@@ -1929,6 +1991,18 @@ int htg_query_batch(htg_map* map, const int32_t* rc, const int32_t* ks, int32_t 
     // queries read what the map's update stream wrote
     if ((e = hipStreamSynchronize(map->stream)) != hipSuccess) return hip_fail(e, "update sync");
     return map->query(rc, ks, nq, out, map->qws);
+    HTG_GUARD_END
+}
+
+int htg_lift_point(htg_map* map, int32_t r, int32_t c, uint16_t* x, int64_t x_cap, int32_t* lens,
+                   int32_t* parent_node_idx, int32_t* child_quad, int32_t* child_offset) {
+    HTG_GUARD_BEGIN
+    if (!map || !x || !lens || !parent_node_idx || !child_quad || !child_offset)
+        return fail(HTG_E_INVALID, "null argument");
+    if (!map->built) return fail(HTG_E_STATE, "map not built");
+    hipError_t e = hipSetDevice(map->ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    return map->lift_point(r, c, x, x_cap, lens, parent_node_idx, child_quad, child_offset);
     HTG_GUARD_END
 }
 

@@ -296,6 +296,12 @@ struct Map {
     int open_cells(const int32_t* rc, int n);
     int block_cells(const int32_t* rc, int n);
     int query(const int32_t* rc, const int32_t* ks, int nq, uint16_t* out_host, QueryWS& ws);
+    // liftPoint of cell (r, c): x receives levels 0..H back to back (lens[l]
+    // entries each), meta the ancestor node, child quad and child offset.
+    int lift_point(int r, int c, uint16_t* x, long long x_cap, int32_t* lens, int32_t* parent, int32_t* quad,
+                   int32_t* offset);
+    DevBuf lift_x, lift_part;  // liftPoint: the lifts of every level; row-chunk partial minima
+    HostBuf lift_h;
     // Asynchronous halves of query(): plan + launch on ws.stream (results
     // copied to ws.h_out when to_host), then wait for ws.done.
     int query_launch(const int32_t* rc, const int32_t* ks, int nq, QueryWS& ws, bool to_host);

@@ -1,8 +1,9 @@
 #pragma once
 // Drop-in replacement of the reference's include/htrip/minplus_oracle.hpp:
 // htrip::MinPlusOracle (alias HTripOracle) with the same hot-path signatures
-// (build, onCellOpened, onBatchCellsOpened, queryAllPairs with
-// HTripQueryScratch, queryDistance) and the public metadata callers read
+// (build, onCellOpened, onBatchCellsOpened, onCellBlocked, queryAllPairs with
+// HTripQueryScratch, queryAllPairsIncremental, queryDistance, liftPoint) and
+// the public metadata callers read
 // (config, S, H, num_leaf_tiles_x/y, leaves.size(), depth(), findLcaLevel,
 // totalMemoryBytes, last_batch_stats).  The index itself lives in HBM behind
 // the C ABI of include/htrip_gpu.h; link with -lhtrip_gpu.
@@ -14,6 +15,7 @@
 //    besides build()'s bool).
 //  * hybrid_k_threshold is accepted and ignored: the GPU query is exact for
 //    every k, as the reference's BFS fallback is (minplus_oracle.cpp:951-958).
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -215,6 +217,86 @@ public:
     [[nodiscard]] dist_t queryDistance(Point s, Point t, const GridMap& grid) const {
         auto m = queryAllPairs({s, t}, grid);
         return m[1];
+    }
+
+    // include/htrip/minplus_oracle.hpp:177-183
+    struct LiftedViewpoint {
+        std::vector<std::vector<dist_t>> X;    // [lvl][port]
+        std::vector<std::vector<int>> active;  // [lvl]: ports with X < INF, by increasing X
+        std::vector<int> parent_node_idx;      // [lvl]: ancestor node at lvl (-1 at 0)
+        std::vector<int> child_quad;           // [lvl]: quadrant of the child in that node (-1 at 0)
+        std::vector<int> child_offset;         // [lvl]: start of that child's ports in the node's V_U
+    };
+
+    // src/minplus_oracle.cpp:731-823.  X comes from the device; the active
+    // lists hold the same ports as the reference's, ordered by X and, among
+    // equal distances, by port index (the reference's std::sort leaves the
+    // order of ties unspecified).
+    void liftPoint(Point p, LiftedViewpoint& lifted) const {
+        if (!map_) throw std::runtime_error("htrip: liftPoint before build");
+        size_t cap = size_t(4 * config.tile_size_b - 4);
+        for (int lvl = 1; lvl <= H; ++lvl) {
+            int32_t nn = 0, nv = 0, ne = 0;
+            detail::check(htg_node_dims(map_.get(), lvl, &nn, &nv, &ne));
+            cap += size_t(ne);
+        }
+        const size_t n = size_t(H) + 1;
+        std::vector<dist_t> x(cap);
+        std::vector<int32_t> lens(n), par(n), quad(n), off(n);
+        detail::check(htg_lift_point(map_.get(), p.r, p.c, x.data(), int64_t(cap), lens.data(), par.data(),
+                                     quad.data(), off.data()));
+        lifted.X.resize(n);
+        lifted.active.resize(n);
+        lifted.parent_node_idx.assign(par.begin(), par.end());
+        lifted.child_quad.assign(quad.begin(), quad.end());
+        lifted.child_offset.assign(off.begin(), off.end());
+        size_t o = 0;
+        for (size_t l = 0; l < n; ++l) {
+            auto& X = lifted.X[l];
+            X.assign(x.begin() + std::ptrdiff_t(o), x.begin() + std::ptrdiff_t(o + size_t(lens[l])));
+            o += size_t(lens[l]);
+            auto& a = lifted.active[l];
+            a.clear();
+            for (int i = 0; i < int(X.size()); ++i)
+                if (X[size_t(i)] < INF) a.push_back(i);
+            std::stable_sort(a.begin(), a.end(), [&X](int u, int v) { return X[size_t(u)] < X[size_t(v)]; });
+        }
+    }
+
+    // include/htrip/minplus_oracle.hpp:262-272.  The device recomputes all k^2
+    // pairs in one batched query, so the state keeps the viewpoints, the
+    // stamps and the matrix but no per-viewpoint lifts.
+    struct HTripIncrementalState {
+        bool initialized = false;
+        uint64_t epoch_seen = 0;
+        uint32_t last_update_counter = 0;
+        std::vector<Point> vp_pts;
+        std::vector<dist_t> matrix;
+    };
+
+    // src/minplus_oracle.cpp:1444-2002.  Same contract: the result equals
+    // queryAllPairs on the current map.  k = 0 clears the state (:1450-1454);
+    // an unchanged viewpoint list with no update since the last call returns
+    // the cached matrix without touching the device (:1522-1528); anything
+    // else (first call, new epoch, other viewpoints, updates) runs the full
+    // query on the device and re-stamps the state.
+    void queryAllPairsIncremental(const std::vector<Point>& viewpoints, const GridMap& grid,
+                                  std::vector<dist_t>& out_matrix, HTripQueryScratch& scratch,
+                                  HTripIncrementalState& state) const {
+        if (viewpoints.empty()) {
+            out_matrix.clear();
+            state.initialized = false;
+            return;
+        }
+        const bool same = state.initialized && state.epoch_seen == epoch && state.vp_pts == viewpoints;
+        if (!(same && state.last_update_counter == update_counter)) {
+            queryAllPairs(viewpoints, grid, state.matrix, scratch);
+            state.vp_pts = viewpoints;
+            state.epoch_seen = epoch;
+            state.last_update_counter = update_counter;
+            state.initialized = true;
+        }
+        out_matrix = state.matrix;
     }
 
     // src/minplus_oracle.cpp:2116-2134

@@ -116,6 +116,19 @@ int htg_block_cells(htg_map* map, const int32_t* rc, int32_t n);
  * Impassable or out-of-range viewpoints give an INF row and column, INF
  * diagonal (src/minplus_oracle.cpp:961-965, :993-997). */
 int htg_query_all_pairs(htg_map* map, const int32_t* rc, int32_t k, uint16_t* out);
+/* MinPlusOracle::liftPoint (src/minplus_oracle.cpp:731-823,
+ * include/htrip/minplus_oracle.hpp:177-183, :361).  Distances from cell (r, c)
+ * to the ports of its leaf (level 0, 4b-4 entries in port order) and, for
+ * l = 1..H, to the exterior ports of its level-l ancestor (nExt entries in
+ * extPorts order), each restricted to that tile's region.  x receives the H+1
+ * vectors back to back (x_cap entries available; lens[l] gets each length);
+ * parent_node_idx, child_quad and child_offset get H+1 entries with entry 0 =
+ * -1, -1, 0 as in the reference.  The active lists of the reference's
+ * LiftedViewpoint are the finite entries of X (the drop-in derives them).
+ * HTG_E_RANGE for a cell outside the S x S hierarchy (the reference indexes
+ * out of bounds there) or x_cap too small. */
+int htg_lift_point(htg_map* map, int32_t r, int32_t c, uint16_t* x, int64_t x_cap, int32_t* lens,
+                   int32_t* parent_node_idx, int32_t* child_quad, int32_t* child_offset);
 /* Same, for nq queries in one call: query q has ks[q] viewpoints (packed back to
  * back in rc) and its k*k result at out + sum_{p<q} ks[p]^2.  One coalesced set
  * of launches serves all of them. */

@@ -13,6 +13,7 @@
 //   * leaf D = tile-restricted BFS, D[i][i] = 0 for every cell (src/leaf_tile.cpp:56-208)
 //   * node D_U = closure over V_U with seam pivots (src/internal_node.cpp:94-245)
 //   * incremental batch reveal with pivot closures (src/minplus_oracle.cpp:530-729)
+//   * liftPoint (src/minplus_oracle.cpp:731-823)
 //   * all-pairs query by lift + inter-child meet + same-child shortcut
 //     (src/minplus_oracle.cpp:934-1419), without the data-dependent pruning
 //     (sorted active lists, early breaks, Manhattan "closed" flags), which the
@@ -520,6 +521,42 @@ struct Oracle {
         last_pivots = piv;
     }
 
+    // src/minplus_oracle.cpp:731-823 (liftPoint) restated: level 0 is the
+    // cell's row of its leaf D at the leaf ports; level l takes level l-1
+    // through the child's rows of the ancestor's D_U at the exterior ports
+    // (the child_to_ext block, src/internal_node.cpp:435-455).
+    void lift_point(Pt p, std::vector<std::vector<dist_t>>& X, int* par, int* quad, int* off) const {
+        X.assign(size_t(H) + 1, {});
+        int tx = p.c / b, ty = p.r / b;
+        const Leaf& L = leaves[size_t(ty) * nl + tx];
+        const int n = b * b, u = (p.r - L.R0) * b + (p.c - L.C0);
+        for (const Pt& q : L.ports) X[0].push_back(L.D[size_t(u) * n + (q.r - L.R0) * b + (q.c - L.C0)]);
+        par[0] = quad[0] = -1;
+        off[0] = 0;
+        for (int lvl = 1; lvl <= H; ++lvl) {
+            const int ptx = tx / g, pty = ty / g, q = (ty % g) * g + tx % g;
+            const int pi = tile_index_at(lvl, ptx, pty);
+            const Node& N = levels[size_t(lvl) - 1][size_t(pi)];
+            par[lvl] = pi;
+            quad[lvl] = q;
+            off[lvl] = N.off[size_t(q)];
+            const int nV = N.nV(), ne = N.nExt();
+            auto& xo = X[size_t(lvl)];
+            xo.assign(size_t(ne), INF);
+            const auto& xi = X[size_t(lvl) - 1];
+            for (size_t cp = 0; cp < xi.size(); ++cp) {
+                if (xi[cp] >= INF) continue;
+                const dist_t* row = &N.DU[(size_t(N.off[size_t(q)]) + cp) * nV];
+                for (int e = 0; e < ne; ++e) {
+                    const dist_t c = add_dist(xi[cp], row[N.extIdx[size_t(e)]]);
+                    if (c < xo[size_t(e)]) xo[size_t(e)] = c;
+                }
+            }
+            tx = ptx;
+            ty = pty;
+        }
+    }
+
     // src/minplus_oracle.cpp:934-1419 restated without pruning.
     void query(const std::vector<Pt>& vp, dist_t* out) const {
         const size_t k = vp.size();
@@ -741,6 +778,32 @@ void orc_region_bfs(void* h, int R0, int C0, int hh, int ww, int sr, int sc, uin
             out[v] = d[v] >= INF ? INF : uint16_t(d[v]);
             q.push_back(v);
         }
+    }
+}
+// liftPoint: X levels back to back (lens[l] entries each), par/quad/off H+1
+// entries, active[l] = ports with X < INF by increasing X (ties by index),
+// padded with -1 to lens[l].
+void orc_lift_point(void* h, int r, int c, uint16_t* X, int32_t* lens, int32_t* par, int32_t* quad, int32_t* off,
+                    int32_t* active) {
+    const Oracle* o = static_cast<Oracle*>(h);
+    std::vector<std::vector<dist_t>> x;
+    std::vector<int> pv(size_t(o->H) + 1), qv(size_t(o->H) + 1), ov(size_t(o->H) + 1);
+    o->lift_point(Pt{r, c}, x, pv.data(), qv.data(), ov.data());
+    size_t w = 0;
+    for (size_t l = 0; l < x.size(); ++l) {
+        lens[l] = int32_t(x[l].size());
+        par[l] = pv[l];
+        quad[l] = qv[l];
+        off[l] = ov[l];
+        std::vector<int> a;
+        for (int i = 0; i < int(x[l].size()); ++i)
+            if (x[l][size_t(i)] < INF) a.push_back(i);
+        std::stable_sort(a.begin(), a.end(), [&](int u, int v) { return x[l][size_t(u)] < x[l][size_t(v)]; });
+        for (size_t i = 0; i < x[l].size(); ++i) {
+            X[w + i] = x[l][i];
+            active[w + i] = i < a.size() ? a[i] : -1;
+        }
+        w += x[l].size();
     }
 }
 int orc_S(void* h) { return static_cast<Oracle*>(h)->S; }

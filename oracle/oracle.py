@@ -78,6 +78,7 @@ def _lib(kind: str):
         "node_SU": fn("node_SU", None, vp, ci, ci, _u16p),
         "last_batch_stats": fn("last_batch_stats", None, vp, _i32p),
         "total_memory_bytes": fn("total_memory_bytes", C.c_uint64, vp),
+        "lift_point": fn("lift_point", None, vp, ci, ci, _u16p, _i32p, _i32p, _i32p, _i32p, _i32p),
     }
     if kind == "port":
         api["region_bfs"] = fn("region_bfs", None, vp, ci, ci, ci, ci, ci, ci, _u16p)
@@ -87,6 +88,7 @@ def _lib(kind: str):
         api["import_leaf"] = fn("import_leaf", None, vp, ci, _u16p)
         api["import_node"] = fn("import_node", None, vp, ci, ci, _u16p)
         api["time_queries"] = fn("time_queries", C.c_double, vp, _i32p, _i32p, ci, ci, C.c_void_p)
+        api["time_lifts"] = fn("time_lifts", C.c_double, vp, _i32p, ci)
     _libs[kind] = api
     return api
 
@@ -191,6 +193,25 @@ class Oracle:
 
     def total_memory_bytes(self) -> int:
         return int(self.api["total_memory_bytes"](self.h))
+
+    def lift_point(self, r: int, c: int):
+        """liftPoint: (X, parent_node_idx, child_quad, child_offset, active) with
+        X[l] and active[l] per level (src/minplus_oracle.cpp:731-823)."""
+        H = self.H
+        cap = 4 * self.b - 4 + sum(self.api["node_nExt"](self.h, l, 0) for l in range(1, H + 1))
+        x = np.empty(cap, dtype=np.uint16)
+        act = np.empty(cap, dtype=np.int32)
+        lens, par, quad, off = (np.empty(H + 1, dtype=np.int32) for _ in range(4))
+        self.api["lift_point"](self.h, int(r), int(c), x, lens, par, quad, off, act)
+        o = np.concatenate([[0], np.cumsum(lens)])
+        X = [x[o[l]:o[l + 1]].copy() for l in range(H + 1)]
+        A = [act[o[l]:o[l + 1]][act[o[l]:o[l + 1]] >= 0].copy() for l in range(H + 1)]
+        return X, par, quad, off, A
+
+    def time_lifts(self, rc) -> float:
+        """Seconds for liftPoint on each listed cell (compiled reference only)."""
+        rc = np.ascontiguousarray(rc, dtype=np.int32).reshape(-1, 2)
+        return float(self.api["time_lifts"](self.h, rc, len(rc)))
 
     def region_bfs(self, R0: int, C0: int, h: int, w: int, sr: int, sc: int) -> np.ndarray:
         out = np.empty((h, w), dtype=np.uint16)

@@ -190,4 +190,32 @@ double ref_time_queries(void* p, const int32_t* rc, const int32_t* ks, int nq, i
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// MinPlusOracle::liftPoint: X levels back to back (lens[l] entries each),
+// par/quad/off H+1 entries, active[l] in the reference's order padded with -1
+// to lens[l].
+void ref_lift_point(void* p, int r, int c, uint16_t* X, int32_t* lens, int32_t* par, int32_t* quad, int32_t* off,
+                    int32_t* active) {
+    auto* h = static_cast<RefHandle*>(p);
+    MinPlusOracle::LiftedViewpoint lv;
+    h->oracle.liftPoint(Point{r, c}, lv);
+    size_t o = 0;
+    for (size_t l = 0; l < lv.X.size(); ++l) {
+        lens[l] = int32_t(lv.X[l].size());
+        par[l] = lv.parent_node_idx[l];
+        quad[l] = lv.child_quad[l];
+        off[l] = lv.child_offset[l];
+        std::memcpy(X + o, lv.X[l].data(), lv.X[l].size() * 2);
+        for (size_t i = 0; i < lv.X[l].size(); ++i) active[o + i] = i < lv.active[l].size() ? lv.active[l][i] : -1;
+        o += lv.X[l].size();
+    }
+}
+// Seconds for n liftPoint calls on the listed cells (one thread).
+double ref_time_lifts(void* p, const int32_t* rc, int n) {
+    auto* h = static_cast<RefHandle*>(p);
+    MinPlusOracle::LiftedViewpoint lv;
+    auto t0 = std::chrono::steady_clock::now();
+    for (int i = 0; i < n; ++i) h->oracle.liftPoint(Point{rc[2 * i], rc[2 * i + 1]}, lv);
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
 }  // extern "C"

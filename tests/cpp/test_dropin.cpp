@@ -2,9 +2,11 @@
 // compiled against htrip_amd/include/htrip/*.hpp and linked with
 // libhtrip_gpu.so.  Mirrors suites of the reference's apps/test_exactness.cpp
 // (known answers 1 and 37, blocked viewpoint INF, batch openings, copy
-// semantics) and checks every matrix against a BFS written here.
+// semantics, incremental refresh, reopened viewpoint, liftPoint) and checks
+// every matrix against a BFS written here.
 // Exit status 0 = all passed.
 #include <cstdio>
+#include <cstdlib>
 #include <queue>
 #include <random>
 #include <vector>
@@ -144,6 +146,113 @@ int main() {
         std::vector<Point> vps = {{1, 1}, {30, 30}, {39, 2}, {60, 60}};
         expect(matches_bfs(snapshot, g0, vps), "copied oracle keeps the pre-update state");
         expect(o.totalMemoryBytes() > 0 && o.depth() == o.H, "totalMemoryBytes / depth");
+    }
+    {  // Suite 7: incremental refresh == full query after every opening (:520-580),
+       // with an unrelated full query on the same scratch in between
+        std::mt19937_64 rng(99);
+        GridMap g(64, 64, CellState::Unknown);
+        for (int r = 0; r < 64; ++r)
+            for (int c = 0; c < 64; ++c) {
+                const bool wall = (r % 10 == 0 && c % 7 != 3) || (c % 13 == 0 && r % 9 != 4);
+                g.setObserved(r, c, wall ? CellState::Obstacle : (r < 30 ? CellState::Free : CellState::Unknown));
+            }
+        HierarchyConfig cfg;
+        cfg.tile_size_b = 16;
+        MinPlusOracle o(cfg);
+        o.build(g);
+        std::vector<Point> vps;
+        for (int i = 0; vps.size() < 24; ++i) {
+            const Point p{int(rng() % 64), int(rng() % 64)};
+            if (g.isPassable(p) || p.r >= 30) vps.push_back(p);
+        }
+        MinPlusOracle::HTripQueryScratch scratch;
+        MinPlusOracle::HTripIncrementalState state;
+        std::vector<dist_t> inc;
+        o.queryAllPairsIncremental(vps, g, inc, scratch, state);
+        bool ok = inc == o.queryAllPairs(vps, g);
+        int opened = 0;
+        for (int step = 0; step < 400 && opened < 40; ++step) {
+            const Point p{30 + int(rng() % 34), int(rng() % 64)};
+            if (g.getObserved(p) != CellState::Unknown) continue;
+            g.openDoorway(p);
+            o.onCellOpened(p, g);
+            ++opened;
+            o.queryAllPairsIncremental(vps, g, inc, scratch, state);
+            if (opened % 5 == 0) {
+                std::vector<dist_t> throwaway;
+                std::vector<Point> other(vps.rbegin(), vps.rbegin() + 10);
+                o.queryAllPairs(other, g, throwaway, scratch);
+            }
+            ok = ok && inc == o.queryAllPairs(vps, g) && inc == bfs_all_pairs(g, vps);
+        }
+        const uint32_t seen = state.last_update_counter;
+        o.queryAllPairsIncremental(vps, g, inc, scratch, state);  // no update since: cached
+        expect(ok && opened == 40, "suite 7 incremental == full == BFS after 40 openings");
+        expect(state.last_update_counter == seen && inc == bfs_all_pairs(g, vps), "suite 7 cached refresh");
+        std::vector<dist_t> empty_out{1};
+        o.queryAllPairsIncremental({}, g, empty_out, scratch, state);
+        expect(empty_out.empty() && !state.initialized, "suite 7 k = 0 clears the state");
+    }
+    {  // Suite 9b: a viewpoint blocked during a reset and reopened regains 0 (:717-748)
+        GridMap g(32, 32, CellState::Free);
+        for (int r = 0; r < 32; ++r) g.blockCorridor({r, 9});
+        g.openDoorway({5, 9});
+        HierarchyConfig cfg;
+        cfg.tile_size_b = 16;
+        MinPlusOracle o(cfg);
+        o.build(g);
+        std::vector<Point> vps = {{1, 1}, {3, 3}, {20, 20}, {30, 2}};
+        MinPlusOracle::HTripQueryScratch scratch;
+        MinPlusOracle::HTripIncrementalState state;
+        std::vector<dist_t> m;
+        o.queryAllPairsIncremental(vps, g, m, scratch, state);
+        g.blockCorridor(vps[1]);
+        o.onCellBlocked(vps[1], g);
+        o.queryAllPairsIncremental(vps, g, m, scratch, state);
+        const bool blocked_inf = m[1 * 4 + 1] == INF && m[0 * 4 + 1] == INF;
+        g.openDoorway(vps[1]);
+        o.onCellOpened(vps[1], g);
+        o.queryAllPairsIncremental(vps, g, m, scratch, state);
+        expect(blocked_inf && m[1 * 4 + 1] == 0 && m == bfs_all_pairs(g, vps), "suite 9b reopened viewpoint");
+    }
+    {  // liftPoint (src/minplus_oracle.cpp:731-823): level 0 = BFS inside the
+       // leaf, level H = BFS over the whole map (the root tile), both at the
+       // tile's ports; consistency with the matrix of a port pair (suite 15 style)
+        GridMap g(64, 64, CellState::Free);
+        for (int r = 0; r < 64; ++r)
+            if (r != 40) g.blockCorridor({r, 20});
+        for (int c = 0; c < 64; ++c)
+            if (c != 50) g.blockCorridor({33, c});
+        HierarchyConfig cfg;
+        cfg.tile_size_b = 8;
+        MinPlusOracle o(cfg);
+        o.build(g);
+        const Point p{10, 12};
+        MinPlusOracle::LiftedViewpoint lv;
+        o.liftPoint(p, lv);
+        bool ok = int(lv.X.size()) == o.H + 1 && lv.parent_node_idx[0] == -1 && lv.child_quad[0] == -1;
+        // level 0 inside leaf (1, 1): rows 8..15, cols 8..15
+        const int b = 8;
+        auto ports = [](int R0, int C0, int s) {
+            std::vector<Point> v;
+            for (int c = 0; c < s; ++c) v.push_back({R0, C0 + c});
+            for (int c = 0; c < s; ++c) v.push_back({R0 + s - 1, C0 + c});
+            for (int r = 1; r < s - 1; ++r) v.push_back({R0 + r, C0});
+            for (int r = 1; r < s - 1; ++r) v.push_back({R0 + r, C0 + s - 1});
+            return v;
+        };
+        const auto lp = ports(8, 8, b);
+        for (size_t i = 0; i < lp.size(); ++i)  // an open leaf: Manhattan distance
+            ok = ok && lv.X[0][i] == dist_t(std::abs(lp[i].r - p.r) + std::abs(lp[i].c - p.c));
+        const auto rp = ports(0, 0, 64);
+        std::vector<Point> pair = {p};
+        pair.insert(pair.end(), rp.begin(), rp.end());
+        const auto d = bfs_all_pairs(g, pair);
+        for (size_t e = 0; e < rp.size(); ++e) ok = ok && lv.X[size_t(o.H)][e] == d[1 + e];
+        for (size_t l = 0; l < lv.X.size(); ++l)
+            for (size_t i = 1; i < lv.active[l].size(); ++i)
+                ok = ok && lv.X[l][size_t(lv.active[l][i - 1])] <= lv.X[l][size_t(lv.active[l][i])];
+        expect(ok, "liftPoint levels 0 and H == BFS, active sorted");
     }
     std::printf("%s (%d failure%s)\n", failures ? "FAILED" : "ALL PASSED", failures, failures == 1 ? "" : "s");
     return failures ? 1 : 0;

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from htrip_amd import grids, serving
-from htrip_amd._lib import GpuMap
+from htrip_amd._lib import HTG_E_RANGE, GpuMap, HtgError
 from oracle.oracle import Oracle, grid_bfs
 
 pytestmark = pytest.mark.gpu
@@ -227,6 +227,44 @@ def test_saturation_maze(gpu_ctx):
     expect = grid_bfs(cells, vps)
     assert expect[expect < 60000].max() > 4000
     np.testing.assert_array_equal(G.query(vps), expect)
+
+
+@pytest.mark.parametrize("kind,n,b,g,seed", CASES)
+def test_lift_point_equals_the_oracle(gpu_ctx, kind, n, b, g, seed):
+    """liftPoint (src/minplus_oracle.cpp:731-823): every level's vector and the
+    ancestor metadata equal the oracle's, for free, blocked, border and
+    padding cells, after the build, after reveals and after a blocking step."""
+    print(f"lift {kind} n={n} b={b} g={g} seed={seed}", flush=True)
+    truth = make(kind, n, seed)
+    obs = grids.disk_observed(truth, [tuple(grids.sample_free(truth, 1, seed)[0])], n // 3)
+    G = GpuMap(obs, b, g)
+    G.build()
+    order = grids.frontier_order(truth, obs)
+    S = G.S
+
+    def check(tag):
+        O = Oracle("port", G.cells, b, g)
+        assert O.build()
+        pts = [tuple(p) for p in grids.sample_free(G.cells, 8, seed)] + [(0, 0), (n - 1, n - 1), (S - 1, S - 1)]
+        if b < n:
+            pts.append((b - 1, b))
+        for r, c in pts:
+            X, par, quad, off = G.lift_point(r, c)
+            Xo, po, qo, oo, _ = O.lift_point(r, c)
+            assert len(X) == len(Xo) == G.H + 1
+            for lvl in range(G.H + 1):
+                np.testing.assert_array_equal(X[lvl], Xo[lvl], err_msg=f"{tag} ({r},{c}) level {lvl}")
+            assert (par == po).all() and (quad == qo).all() and (off == oo).all(), (tag, r, c)
+
+    check("built")
+    G.open_cells(order[:150])
+    check("revealed")
+    free = np.argwhere(G.cells == 1)
+    G.block_cells(free[:: max(1, len(free) // 7)][:7].astype(np.int32))
+    check("blocked")
+    with pytest.raises(HtgError) as ei:
+        G.lift_point(S, 0)
+    assert ei.value.code == HTG_E_RANGE
 
 
 def test_cpp_dropin_program(gpu_ctx):

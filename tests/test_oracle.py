@@ -153,6 +153,80 @@ def test_closed_forms(kind, b, g):
     np.testing.assert_array_equal(o.query(vps), grid_bfs(cells, vps))
 
 
+def lift_closed_form(o, r, c):
+    """liftPoint's closed form: level l holds the BFS distances from (r, c)
+    restricted to its level-l tile (side b g^l) at that tile's ports, in
+    canonical order; the tile's index, the child quadrant and the child's
+    offset in V_U follow from the coordinates."""
+    b, g = o.b, o.g
+    X, par, quad, off = [], [-1], [-1], [0]
+    for lvl in range(o.H + 1):
+        s = b * g ** lvl
+        R0, C0 = (r // s) * s, (c // s) * s
+        reg = o.region_bfs(R0, C0, s, s, r, c)
+        X.append(np.array([reg[pr - R0, pc - C0] for pr, pc in port_cells(R0, C0, s)], dtype=np.uint16))
+        if lvl:
+            cs = s // g
+            q = ((r // cs) % g) * g + (c // cs) % g
+            par.append((r // s) * max(1, o.S // s) + c // s)
+            quad.append(q)
+            off.append(q * (4 * cs - 4))
+    return X, par, quad, off
+
+
+def check_lift(got, o, r, c):
+    X, par, quad, off, act = got
+    eX, epar, equad, eoff = lift_closed_form(o, r, c)
+    assert len(X) == len(eX) == o.H + 1
+    for lvl in range(o.H + 1):
+        np.testing.assert_array_equal(X[lvl], eX[lvl], err_msg=f"({r},{c}) level {lvl}")
+        fin = np.nonzero(X[lvl] < orc.INF)[0]
+        assert sorted(act[lvl].tolist()) == fin.tolist()          # the finite ports
+        assert (np.diff(X[lvl][act[lvl]].astype(int)) >= 0).all()  # by increasing distance
+    assert list(par) == epar and list(quad) == equad and list(off) == eoff
+
+
+@pytest.mark.parametrize("kind,n,b,g", [("street", 64, 8, 2), ("maze", 64, 4, 2), ("street", 64, 4, 4),
+                                        ("street", 40, 8, 2)])
+def test_lift_point_closed_form(kind, n, b, g):
+    """liftPoint (src/minplus_oracle.cpp:731-823) against region BFS, before and
+    after reveals, for free cells, blocked cells and cells on tile borders."""
+    truth = grids.street_grid(n, 21) if kind == "street" else grids.maze_grid(n, 21)
+    obs = grids.disk_observed(truth, [tuple(grids.sample_free(truth, 1, 22)[0])], n // 3)
+    o = Oracle("port", obs, b, g)
+    o.build()
+    order = grids.frontier_order(truth, obs)
+    pts = [tuple(p) for p in grids.sample_free(obs, 6, 23)] + [(0, 0), (b - 1, b), (n - 1, n - 1)]
+    for step in range(2):
+        for r, c in pts:
+            check_lift(o.lift_point(r, c), o, r, c)
+        o.open_cells(order[:120])
+        order = order[120:]
+
+
+@pytest.mark.skipif(not orc.available("reference"), reason="oracle/_ref not built (reference sources absent)")
+@pytest.mark.parametrize("seed", [31, 32])
+def test_lift_point_matches_compiled_reference(seed):
+    truth = grids.street_grid(64, seed)
+    obs = grids.disk_observed(truth, [tuple(grids.sample_free(truth, 1, seed)[0])], 22)
+    order = grids.frontier_order(truth, obs)
+    P, R = Oracle("port", obs, 8, 2), Oracle("reference", obs, 8, 2)
+    P.build()
+    R.build()
+    for st in range(3):
+        for r, c in [tuple(p) for p in grids.sample_free(P.cells, 8, seed + st)] + [(5, 7), (63, 0)]:
+            Xp, pp, qp, op, ap = P.lift_point(r, c)
+            Xr, pr, qr, or_, ar = R.lift_point(r, c)
+            for lvl in range(P.H + 1):
+                np.testing.assert_array_equal(Xp[lvl], Xr[lvl])
+                assert sorted(ap[lvl].tolist()) == sorted(ar[lvl].tolist())
+                assert (np.diff(Xr[lvl][ar[lvl]].astype(int)) >= 0).all()
+            assert (pp == pr).all() and (qp == qr).all() and (op == or_).all()
+        batch = order[st * 60:(st + 1) * 60]
+        P.open_cells(batch)
+        R.open_cells(batch)
+
+
 def test_open_grid_is_manhattan():
     cells = np.ones((48, 48), np.uint8)
     o = Oracle("port", cells, 8, 2)
