@@ -153,18 +153,11 @@ def kernel_phase(gmap, src, batch_q, n_batches, seed=4242, events=True):
     return stats, n_batches * batch_q / dt
 
 
-def cpu_baseline(gmap, cells, queries, budget_s, expect, served_checks=()):
-    """The reference's own CPU AVX2 queryAllPairs (oracle/_ref), one thread, on
-    a bounded sample of the same queries, over tables copied from the GPU.
-    Also the exactness gate: every timed CPU result must equal the GPU result,
-    and the sampled served results must equal BFS on their map state."""
+def import_reference(gmap, cells):
+    """The compiled reference (oracle/_ref) laid out over tables copied from the
+    GPU, so that its query path runs without its CPU build; None without it."""
     from oracle import oracle as orc
 
-    for state, vps, got in served_checks:
-        if not np.array_equal(orc.grid_bfs(state, vps), got):
-            raise SystemExit("served result differs from BFS on its map state")
-    if served_checks:
-        log(f"exactness: {len(served_checks)} served matrices equal BFS on their snapshot states")
     if not orc.available("reference"):
         return None
     R = orc.Oracle("reference", cells, B, G)
@@ -180,6 +173,25 @@ def cpu_baseline(gmap, cells, queries, budget_s, expect, served_checks=()):
         for i in range(gmap.num_nodes(lvl)):
             api["import_node"](R.h, lvl, i, gmap.node_DU(lvl, i))
     log(f"cpu baseline: GPU tables imported into the reference in {time.time() - t0:.1f}s")
+    return R
+
+
+def cpu_baseline(gmap, cells, queries, budget_s, expect, served_checks=()):
+    """The reference's own CPU AVX2 queryAllPairs (oracle/_ref), one thread, on
+    a bounded sample of the same queries, over tables copied from the GPU.
+    Also the exactness gate: every timed CPU result must equal the GPU result,
+    and the sampled served results must equal BFS on their map state."""
+    from oracle import oracle as orc
+
+    for state, vps, got in served_checks:
+        if not np.array_equal(orc.grid_bfs(state, vps), got):
+            raise SystemExit("served result differs from BFS on its map state")
+    if served_checks:
+        log(f"exactness: {len(served_checks)} served matrices equal BFS on their snapshot states")
+    R = import_reference(gmap, cells)
+    if R is None:
+        return None
+    api = R.api
     done, spent = 0, 0.0
     for _ in range(50):  # the query list, cyclically, until the CPU budget is spent
         for q, vps in enumerate(queries):
@@ -579,6 +591,139 @@ def run_small(args):
             "build_ms": build_ms, "cpu_baseline": cpu}
 
 
+def run_lift(args):
+    """liftPoint on the 1024^2 street-like grid (b=32): one call at a time, each
+    synchronous and including the copy of its H+1 vectors to the host, as the
+    reference's API returns them.  The reference's liftPoint runs beside it over
+    the same tables, and must return the same vectors."""
+    from htrip_amd import _lib, grids
+
+    cells = grids.street_grid(N, SEED)
+    gmap = _lib.GpuMap(cells, B, G)
+    gmap.build()
+    H = gmap.H
+    cap = 4 * B - 4 + sum(gmap.node_dims(l)[2] for l in range(1, H + 1))
+    x = np.empty(cap, dtype=np.uint16)
+    meta = [np.empty(H + 1, dtype=np.int32) for _ in range(4)]
+    P = C.POINTER(C.c_int32)
+    xp = x.ctypes.data_as(C.POINTER(C.c_uint16))
+    mp = [m.ctypes.data_as(P) for m in meta]
+    pts = grids.sample_free(cells, 20 * args.steps + args.warmup, 77).astype(np.int32)  # 800 timed by default
+    lat = []
+    for i, (r, c) in enumerate(pts):
+        t0 = time.perf_counter()
+        rc = gmap.lib.htg_lift_point(gmap.h, int(r), int(c), xp, cap, *mp)
+        t1 = time.perf_counter()
+        if rc != 0:
+            raise SystemExit(f"htg_lift_point failed: {rc}")
+        if i >= args.warmup:
+            lat.append(t1 - t0)
+    lat = np.array(lat)
+    rate = len(lat) / lat.sum()
+    log(f"lift: {len(lat)} calls, {rate:.0f} lifts/s, p50 {1e6 * np.median(lat):.1f} us, "
+        f"p99 {1e6 * np.percentile(lat, 99):.1f} us")
+    cpu = None
+    if not args.no_cpu:
+        R = import_reference(gmap, cells)
+        if R is not None:
+            for r, c in pts[:16]:  # exactness gate
+                Xg = gmap.lift_point(int(r), int(c))[0]
+                Xr = R.lift_point(int(r), int(c))[0]
+                if any(not np.array_equal(a, b) for a, b in zip(Xg, Xr)):
+                    raise SystemExit("reference liftPoint differs from the GPU")
+            log("cpu baseline: 16 reference lifts equal the GPU lifts")
+            done, spent = 0, 0.0
+            while spent < args.cpu_seconds and done < len(pts):
+                chunk = pts[done:done + 64]
+                spent += R.time_lifts(chunk)
+                done += len(chunk)
+            cpu = {"value": done / spent, "unit": "lifts/s", "cores": 1, "kind": "reference",
+                   "sample": f"{done} liftPoint calls (1 thread, reference sources) on the same map"}
+    return {"metric": "liftPoint calls/s (one synchronous call at a time, 1024^2 grid)", "value": rate,
+            "unit": "lifts/s", "n_gpus": 1, "steps": len(lat), "warmup": args.warmup,
+            "ms_per_step": 1e3 / rate, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "u16", "data": "synthetic",
+            "config": {"workload": "1024x1024 street-like grid seed 3, b=32, g=2x2, liftPoint of free cells",
+                       "mode": "lift"},
+            "p50_us": 1e6 * float(np.median(lat)), "p99_us": 1e6 * float(np.percentile(lat, 99)),
+            "cpu_baseline": cpu}
+
+
+def run_incremental(args):
+    """queryAllPairsIncremental after each single-cell opening: 256^2
+    street-like grid (seed 1), b=16, k=64 fixed viewpoints, robots revealing
+    the frontier one cell at a time.  GPU: onCellOpened, then the refresh (one
+    batched query of all k^2 pairs).  Reference: the same openings, then its
+    version-stamped incremental refresh.  The GPU refresh is
+    htg_query_all_pairs_incremental: the cached matrix when no viewpoint's
+    leaf or ancestor table was rewritten, one batched query otherwise.  Every
+    refreshed matrix is compared with the reference's."""
+    from htrip_amd import _lib, grids
+    from oracle import oracle as orc
+
+    truth = grids.street_grid(256, 1)
+    obs = grids.disk_observed(truth, [tuple(grids.sample_free(truth, 1, 5)[0])], 60)
+    order = grids.frontier_order(truth, obs)
+    vps = grids.sample_free(truth, 64, 6).astype(np.int32)
+    gmap = _lib.GpuMap(obs, 16, 2)
+    gmap.build()
+    steps = min(10 * args.steps, len(order) - args.warmup)  # 400 refreshes by default
+    R = None
+    if not args.no_cpu and orc.available("reference"):
+        R = orc.Oracle("reference", obs, 16, 2)
+        assert R.build()
+        inc = R.api["inc_create"]()
+    t_gpu, t_ref = [], []
+    recomputed = 0
+    out_r = np.empty(64 * 64, dtype=np.uint16)
+    st = _lib.IncState()
+    m = np.empty((64, 64), dtype=np.uint16)  # caller-owned output, as the reference's out_matrix
+    nr = C.c_int32()
+    P32 = C.POINTER(C.c_int32)
+    vp_p, m_p = vps.ctypes.data_as(P32), m.ctypes.data_as(C.POINTER(C.c_uint16))
+    refresh = gmap.lib.htg_query_all_pairs_incremental
+    for i in range(args.warmup + steps):
+        r, c = (int(v) for v in order[i])
+        gmap.open_cell(r, c)
+        gmap.sync()
+        t0 = time.perf_counter()
+        rc = refresh(gmap.h, st.h, vp_p, 64, m_p, C.byref(nr))
+        t1 = time.perf_counter()
+        if rc != 0:
+            raise SystemExit(f"htg_query_all_pairs_incremental failed: {rc}")
+        if i >= args.warmup:
+            recomputed += nr.value > 0
+        if R is not None:
+            R.open_cell(r, c)
+            dt = R.api["inc_query"](R.h, inc, vps, 64, out_r)
+            if not np.array_equal(out_r.reshape(64, 64), m):
+                raise SystemExit("reference incremental refresh differs from the GPU")
+        if i >= args.warmup:
+            t_gpu.append(t1 - t0)
+            if R is not None:
+                t_ref.append(dt)
+    if R is not None:
+        R.api["inc_destroy"](inc)
+    t_gpu = np.array(t_gpu)
+    rate = len(t_gpu) / t_gpu.sum()
+    st.close()
+    log(f"incremental: {len(t_gpu)} refreshes ({recomputed} recomputed), GPU p50 {1e6 * np.median(t_gpu):.1f} us"
+        + (f", reference p50 {1e6 * np.median(t_ref):.0f} us" if t_ref else ""))
+    cpu = None
+    if t_ref:
+        cpu = {"value": len(t_ref) / float(np.sum(t_ref)), "unit": "refreshes/s", "cores": 1, "kind": "reference",
+               "sample": f"{len(t_ref)} queryAllPairsIncremental refreshes (1 thread), every result equal to the GPU's",
+               "p50_us": 1e6 * float(np.median(t_ref)), "p99_us": 1e6 * float(np.percentile(t_ref, 99))}
+    return {"metric": "incremental k=64 refreshes/s after single-cell openings (256^2 street grid, b=16)",
+            "value": rate, "unit": "refreshes/s", "n_gpus": 1, "steps": len(t_gpu), "warmup": args.warmup,
+            "ms_per_step": 1e3 / rate, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "u16", "data": "synthetic",
+            "config": {"workload": "256x256 street-like grid seed 1, b=16, g=2x2, k=64, one cell opened per refresh",
+                       "mode": "incremental"},
+            "p50_us": 1e6 * float(np.median(t_gpu)), "p99_us": 1e6 * float(np.percentile(t_gpu, 99)),
+            "recomputed": recomputed, "cpu_baseline": cpu}
+
+
 def run_build(args):
     """512^2 DFS maze (seed 2, corridor and wall width 1), b=16: the batched
     build (every node of a level per launch, the whole build as one captured
@@ -797,7 +942,8 @@ def main():
                     help="timed steps (served: windows of requests per rate trial, default 200; others 40)")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--mode", default="served",
-                    choices=["plain", "served", "update", "small", "build", "reveal", "multi", "kernel"])
+                    choices=["plain", "served", "update", "small", "build", "reveal", "multi", "kernel", "lift",
+                             "incremental"])
     ap.add_argument("--k", type=int, default=64)
     ap.add_argument("--window", type=int, default=64, help="requests per served step")
     ap.add_argument("--slots", type=int, default=4, help="query streams (the update stream makes one more)")
@@ -834,7 +980,8 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU (one MI355X per box); --gpus must be 1"}))
         sys.exit(2)
     line = {"plain": run_plain, "served": run_served, "update": run_update, "small": run_small, "build": run_build,
-            "reveal": run_reveal, "multi": run_multi, "kernel": run_kernel}[args.mode](args)
+            "reveal": run_reveal, "multi": run_multi, "kernel": run_kernel, "lift": run_lift,
+            "incremental": run_incremental}[args.mode](args)
     print(json.dumps(line))
 
 

@@ -21,7 +21,7 @@ EXPORTS = [
     "htg_ctx_create", "htg_ctx_destroy", "htg_last_error", "htg_ctx_mem_info",
     "htg_map_create", "htg_map_destroy", "htg_map_clone", "htg_map_footprint", "htg_map_info_get",
     "htg_build", "htg_open_cells", "htg_open_cell", "htg_block_cells", "htg_query_all_pairs", "htg_query_batch",
-    "htg_lift_point", "htg_last_batch_stats", "htg_sync",
+    "htg_lift_point", "htg_inc_create", "htg_inc_destroy", "htg_query_all_pairs_incremental", "htg_last_batch_stats", "htg_sync",
     "htg_read_leaf_D", "htg_read_leaf_S", "htg_node_dims", "htg_read_node_DU", "htg_read_node_SU",
     "htg_profile_enable", "htg_profile_get",
     "htg_get_tuning", "htg_set_tuning", "htg_autotune",
@@ -96,6 +96,9 @@ def load(path: str = LIB_PATH):
         "htg_query_all_pairs": (C.c_int, [vp, pi32, i32, pu16]),
         "htg_query_batch": (C.c_int, [vp, pi32, pi32, i32, pu16]),
         "htg_lift_point": (C.c_int, [vp, i32, i32, pu16, C.c_int64, pi32, pi32, pi32, pi32]),
+        "htg_inc_create": (C.c_int, [C.POINTER(vp)]),
+        "htg_inc_destroy": (C.c_int, [vp]),
+        "htg_query_all_pairs_incremental": (C.c_int, [vp, vp, pi32, i32, pu16, pi32]),
         "htg_last_batch_stats": (C.c_int, [vp, C.POINTER(BatchStats)]),
         "htg_sync": (C.c_int, [vp]),
         "htg_read_leaf_D": (C.c_int, [vp, i32, pu16]),
@@ -163,6 +166,27 @@ def footprint(b: int, g: int, rows: int, cols: int) -> int:
     out = C.c_uint64()
     check(lib.htg_map_footprint(C.byref(cfg), rows, cols, C.byref(out)))
     return out.value
+
+
+class IncState:
+    """The caller-owned state of queryAllPairsIncremental (htg_inc)."""
+
+    def __init__(self):
+        self.lib = load()
+        h = C.c_void_p()
+        check(self.lib.htg_inc_create(C.byref(h)))
+        self.h = h
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.htg_inc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class GpuMap:
@@ -247,6 +271,18 @@ class GpuMap:
             res.append(out[o:o + k * k].reshape(k, k))
             o += int(k) * int(k)
         return res
+
+    def query_incremental(self, state: "IncState", vps):
+        """queryAllPairsIncremental: (k x k matrix, recomputed) where recomputed
+        is k when the device ran the query and 0 when the cached matrix was
+        still exact."""
+        vps = np.ascontiguousarray(vps, dtype=np.int32).reshape(-1, 2)
+        k = len(vps)
+        out = np.empty((k, k), dtype=np.uint16)
+        n = C.c_int32()
+        check(self.lib.htg_query_all_pairs_incremental(self.h, state.h, ptr(vps, C.c_int32), k,
+                                                       ptr(out, C.c_uint16), C.byref(n)))
+        return out, n.value
 
     def lift_point(self, r: int, c: int):
         """liftPoint: (X, parent_node_idx, child_quad, child_offset); X[l] is the

@@ -694,6 +694,7 @@ int Map::block_cells(const int32_t* rcs, int n) {
     if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "block launch");
     if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "block sync");
     prof_ref().harvest();
+    stamp(leaves, anc, 1);
     return HTG_OK;
 }
 
@@ -795,6 +796,62 @@ int Map::build() {
     prof_ref().harvest();
     built = true;
     epoch++;
+    gen++;
+    clock++;
+    leaf_ver.assign(size_t(nl) * nl, 0);
+    node_ver.assign(size_t(H), {});
+    for (int l = 0; l < H; ++l) node_ver[size_t(l)].assign(size_t(lv[size_t(l)].nn), 0);
+    return HTG_OK;
+}
+
+void Map::stamp(const std::vector<int>& leaves, const std::vector<std::vector<int>>& nodes, int first_level) {
+    ++clock;
+    for (int x : leaves) leaf_ver[size_t(x)] = clock;
+    for (size_t i = size_t(first_level); i < nodes.size(); ++i) {
+        const size_t l = i - size_t(first_level);  // level l + 1
+        if (l >= node_ver.size()) break;
+        for (int w : nodes[i]) node_ver[l][size_t(w)] = clock;
+    }
+}
+
+uint64_t Map::chain_version(int r, int c) const {
+    int tx = c / b, ty = r / b;
+    uint64_t v = leaf_ver[size_t(ty) * nl + tx];
+    for (int l = 1; l <= H; ++l) {
+        tx /= g;
+        ty /= g;
+        v = std::max(v, node_ver[size_t(l) - 1][size_t(ty) * lv[size_t(l) - 1].d.nt + tx]);
+    }
+    return v;
+}
+
+int Map::query_incremental(htg_inc& st, const int32_t* rc, int k, uint16_t* out, int32_t* n_recomputed) {
+    if (n_recomputed) *n_recomputed = 0;
+    if (k == 0) {
+        st.valid = false;
+        return HTG_OK;
+    }
+    bool fresh = st.valid && st.map_uid == uid && st.gen == gen && st.rc.size() == size_t(2 * k) &&
+                 std::memcmp(st.rc.data(), rc, size_t(2 * k) * 4) == 0;
+    for (int i = 0; fresh && i < k; ++i) {
+        const int r = rc[2 * i], c = rc[2 * i + 1];
+        if (r < 0 || c < 0 || r >= S || c >= S) continue;  // outside the hierarchy: INF for good
+        if (chain_version(r, c) > st.clock) fresh = false;
+    }
+    if (!fresh) {
+        st.valid = false;
+        st.m.resize(size_t(k) * k);
+        const int32_t ks = k;
+        const int e = query(rc, &ks, 1, st.m.data(), qws);
+        if (e) return e;
+        st.rc.assign(rc, rc + 2 * k);
+        st.map_uid = uid;
+        st.gen = gen;
+        st.clock = clock;
+        st.valid = true;
+        if (n_recomputed) *n_recomputed = k;
+    }
+    std::memcpy(out, st.m.data(), size_t(k) * k * 2);
     return HTG_OK;
 }
 
@@ -1029,6 +1086,7 @@ int Map::open_cells(const int32_t* rcs, int n) {
     stats.n_pivots = piv;
     if (H == 0) {
         if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "update sync");
+        stamp(this->dirty.leaves, this->dirty.nodes, 0);
         return HTG_OK;  // the reference returns before bumping epoch here
     }
     bool below_any = true;  // the level below may have changed flags set
@@ -1155,6 +1213,7 @@ int Map::open_cells(const int32_t* rcs, int n) {
     stats.n_dirty_ancestors = anc;
     if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "update sync");
     prof_ref().harvest();
+    stamp(this->dirty.leaves, this->dirty.nodes, 0);
     epoch++;
     const auto u4 = clk::now();
     auto dt = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); };
@@ -1223,6 +1282,7 @@ int Map::catch_up_from(const Map& src, const DirtySet& ds) {
     if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "catch-up sync");
     stats = src.stats;
     epoch = src.epoch;
+    stamp(ds.leaves, ds.nodes, 0);
     return HTG_OK;
 }
 
@@ -1801,7 +1861,9 @@ int map_create_on(htg_ctx* ctx, const htg_config* cfg, int32_t rows, int32_t col
     size_t fr = 0, tot = 0;
     if (hipMemGetInfo(&fr, &tot) == hipSuccess && need > fr)
         return fail(HTG_E_NOMEM, "map needs " + std::to_string(need) + " bytes, device has " + std::to_string(fr));
+    static std::atomic<uint64_t> next_uid{0};
     std::unique_ptr<htg_map> m(new htg_map());
+    m->uid = ++next_uid;
     m->ctx = ctx;
     m->cfg = *cfg;
     m->b = b;
@@ -1871,6 +1933,10 @@ int map_clone_on(const htg_map* src, uint64_t hbm_budget, hipStream_t shared, ht
     m->built = src->built;
     m->stats = src->stats;
     m->epoch = src->epoch;
+    m->gen = src->gen;
+    m->clock = src->clock;
+    m->leaf_ver = src->leaf_ver;
+    m->node_ver = src->node_ver;
     hipError_t e;
     const hipStream_t st = m->stream;
     (void)hipStreamSynchronize(src->stream);
@@ -2003,6 +2069,32 @@ int htg_lift_point(htg_map* map, int32_t r, int32_t c, uint16_t* x, int64_t x_ca
     hipError_t e = hipSetDevice(map->ctx->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     return map->lift_point(r, c, x, x_cap, lens, parent_node_idx, child_quad, child_offset);
+    HTG_GUARD_END
+}
+
+int htg_inc_create(htg_inc** out) {
+    HTG_GUARD_BEGIN
+    if (!out) return fail(HTG_E_INVALID, "null argument");
+    *out = new htg_inc();
+    return HTG_OK;
+    HTG_GUARD_END
+}
+
+int htg_inc_destroy(htg_inc* state) {
+    delete state;
+    return HTG_OK;
+}
+
+int htg_query_all_pairs_incremental(htg_map* map, htg_inc* state, const int32_t* rc, int32_t k, uint16_t* out,
+                                    int32_t* n_recomputed) {
+    HTG_GUARD_BEGIN
+    if (!map || !state || (k > 0 && (!rc || !out))) return fail(HTG_E_INVALID, "null argument");
+    if (k < 0) return fail(HTG_E_INVALID, "negative k");
+    if (!map->built) return fail(HTG_E_STATE, "map not built");
+    hipError_t e = hipSetDevice(map->ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    if ((e = hipStreamSynchronize(map->stream)) != hipSuccess) return hip_fail(e, "update sync");
+    return map->query_incremental(*state, rc, k, out, n_recomputed);
     HTG_GUARD_END
 }
 

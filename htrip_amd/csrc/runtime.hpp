@@ -302,6 +302,19 @@ struct Map {
                    int32_t* offset);
     DevBuf lift_x, lift_part;  // liftPoint: the lifts of every level; row-chunk partial minima
     HostBuf lift_h;
+    // Table versions for incremental queries.  uid names this map for the
+    // process's lifetime, gen counts full builds, and every update that
+    // rewrites tables advances clock and stamps those leaves and nodes with
+    // it.  A query result depends only on the tables of its viewpoints'
+    // leaves and ancestor chains, so it is unchanged while none of them is
+    // stamped after the result was taken.
+    uint64_t uid = 0, gen = 0, clock = 0;
+    std::vector<uint64_t> leaf_ver;
+    std::vector<std::vector<uint64_t>> node_ver;  // node_ver[l-1][node]
+    void stamp(const std::vector<int>& leaves, const std::vector<std::vector<int>>& nodes, int first_level);
+    // Latest stamp on the leaf and ancestor chain of cell (r, c).
+    uint64_t chain_version(int r, int c) const;
+    int query_incremental(htg_inc& st, const int32_t* rc, int k, uint16_t* out, int32_t* n_recomputed);
     // Asynchronous halves of query(): plan + launch on ws.stream (results
     // copied to ws.h_out when to_host), then wait for ws.done.
     int query_launch(const int32_t* rc, const int32_t* ks, int nq, QueryWS& ws, bool to_host);
@@ -332,6 +345,16 @@ struct Map {
 }  // namespace htg
 
 struct htg_map : htg::Map {};
+
+// Caller-owned state of MinPlusOracle::queryAllPairsIncremental
+// (include/htrip/minplus_oracle.hpp:262-272): the map and build it was taken
+// on, the clock at that time, the viewpoints and the matrix.
+struct htg_inc {
+    bool valid = false;
+    uint64_t map_uid = 0, gen = 0, clock = 0;
+    std::vector<int32_t> rc;
+    std::vector<uint16_t> m;
+};
 
 struct htg_ctx {
     int device = 0;

@@ -263,40 +263,79 @@ public:
         }
     }
 
-    // include/htrip/minplus_oracle.hpp:262-272.  The device recomputes all k^2
-    // pairs in one batched query, so the state keeps the viewpoints, the
-    // stamps and the matrix but no per-viewpoint lifts.
+    // include/htrip/minplus_oracle.hpp:262-272.  The device-side state
+    // (htg_inc) keeps the viewpoints, the matrix and the table clock it was
+    // taken at; the public fields mirror the reference's for callers that
+    // read them.  Copies start from an empty state (the next call recomputes).
     struct HTripIncrementalState {
         bool initialized = false;
         uint64_t epoch_seen = 0;
         uint32_t last_update_counter = 0;
         std::vector<Point> vp_pts;
         std::vector<dist_t> matrix;
+        int last_recomputed = 0;  // viewpoints recomputed by the last call (k or 0)
+
+        HTripIncrementalState() = default;
+        HTripIncrementalState(const HTripIncrementalState& o)
+            : initialized(false), epoch_seen(o.epoch_seen), last_update_counter(o.last_update_counter),
+              vp_pts(o.vp_pts), matrix(o.matrix) {}
+        HTripIncrementalState& operator=(const HTripIncrementalState& o) {
+            if (this != &o) {
+                *this = HTripIncrementalState(o);
+            }
+            return *this;
+        }
+        HTripIncrementalState(HTripIncrementalState&&) noexcept = default;
+        HTripIncrementalState& operator=(HTripIncrementalState&&) noexcept = default;
+
+        htg_inc* handle() {
+            if (!h_) {
+                htg_inc* p = nullptr;
+                detail::check(htg_inc_create(&p));
+                h_.reset(p);
+            }
+            return h_.get();
+        }
+
+    private:
+        struct IncDeleter {
+            void operator()(htg_inc* p) const noexcept { htg_inc_destroy(p); }
+        };
+        std::unique_ptr<htg_inc, IncDeleter> h_;
     };
 
     // src/minplus_oracle.cpp:1444-2002.  Same contract: the result equals
-    // queryAllPairs on the current map.  k = 0 clears the state (:1450-1454);
-    // an unchanged viewpoint list with no update since the last call returns
-    // the cached matrix without touching the device (:1522-1528); anything
-    // else (first call, new epoch, other viewpoints, updates) runs the full
-    // query on the device and re-stamps the state.
-    void queryAllPairsIncremental(const std::vector<Point>& viewpoints, const GridMap& grid,
+    // queryAllPairs on the current map.  k = 0 clears the state (:1450-1454).
+    // The device keeps a version stamp per leaf and node table; the cached
+    // matrix is returned without device work while no update has rewritten
+    // the leaf or an ancestor of any viewpoint since it was computed (no
+    // entry can have changed), otherwise one batched query recomputes it.
+    void queryAllPairsIncremental(const std::vector<Point>& viewpoints, const GridMap& /*grid*/,
                                   std::vector<dist_t>& out_matrix, HTripQueryScratch& scratch,
                                   HTripIncrementalState& state) const {
-        if (viewpoints.empty()) {
-            out_matrix.clear();
+        const size_t k = viewpoints.size();
+        if (k == 0 || !map_) {
+            if (map_) detail::check(htg_query_all_pairs_incremental(map_.get(), state.handle(), nullptr, 0,
+                                                                    nullptr, nullptr));
+            out_matrix.assign(k * k, INF);
             state.initialized = false;
             return;
         }
-        const bool same = state.initialized && state.epoch_seen == epoch && state.vp_pts == viewpoints;
-        if (!(same && state.last_update_counter == update_counter)) {
-            queryAllPairs(viewpoints, grid, state.matrix, scratch);
-            state.vp_pts = viewpoints;
-            state.epoch_seen = epoch;
-            state.last_update_counter = update_counter;
-            state.initialized = true;
+        scratch.rc.resize(2 * k);
+        for (size_t i = 0; i < k; ++i) {
+            scratch.rc[2 * i] = viewpoints[i].r;
+            scratch.rc[2 * i + 1] = viewpoints[i].c;
         }
-        out_matrix = state.matrix;
+        out_matrix.resize(k * k);
+        int32_t n = 0;
+        detail::check(htg_query_all_pairs_incremental(map_.get(), state.handle(), scratch.rc.data(), int32_t(k),
+                                                      out_matrix.data(), &n));
+        state.last_recomputed = n;
+        state.initialized = true;
+        state.epoch_seen = epoch;
+        state.last_update_counter = update_counter;
+        state.vp_pts = viewpoints;
+        state.matrix = out_matrix;
     }
 
     // src/minplus_oracle.cpp:2116-2134

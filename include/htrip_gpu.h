@@ -41,6 +41,7 @@ enum {
 
 typedef struct htg_ctx htg_ctx;
 typedef struct htg_map htg_map;
+typedef struct htg_inc htg_inc;
 typedef struct htg_server htg_server;
 
 /* include/htrip/types.hpp:199-225 (HierarchyConfig): tile_size_b, group_w,
@@ -129,6 +130,20 @@ int htg_query_all_pairs(htg_map* map, const int32_t* rc, int32_t k, uint16_t* ou
  * out of bounds there) or x_cap too small. */
 int htg_lift_point(htg_map* map, int32_t r, int32_t c, uint16_t* x, int64_t x_cap, int32_t* lens,
                    int32_t* parent_node_idx, int32_t* child_quad, int32_t* child_offset);
+/* MinPlusOracle::queryAllPairsIncremental (src/minplus_oracle.cpp:1444-2002,
+ * state include/htrip/minplus_oracle.hpp:262-272).  `state` (htg_inc_create)
+ * keeps the last viewpoints and matrix; out receives k*k distances equal to
+ * htg_query_all_pairs on the current map.  k = 0 clears the state.  The state
+ * is refreshed by one batched query when it is new, was taken on another map
+ * or before a build, lists other viewpoints, or when an update since then
+ * rewrote the leaf or an ancestor node of one of the viewpoints; otherwise
+ * the cached matrix is returned with no device work (no such table changed,
+ * so no entry can have changed).  n_recomputed (may be null) gets k when the
+ * matrix was recomputed, 0 when it was cached. */
+int htg_inc_create(htg_inc** out);
+int htg_inc_destroy(htg_inc* state);
+int htg_query_all_pairs_incremental(htg_map* map, htg_inc* state, const int32_t* rc, int32_t k, uint16_t* out,
+                                    int32_t* n_recomputed);
 /* Same, for nq queries in one call: query q has ks[q] viewpoints (packed back to
  * back in rc) and its k*k result at out + sum_{p<q} ks[p]^2.  One coalesced set
  * of launches serves all of them. */

@@ -89,6 +89,9 @@ def _lib(kind: str):
         api["import_node"] = fn("import_node", None, vp, ci, ci, _u16p)
         api["time_queries"] = fn("time_queries", C.c_double, vp, _i32p, _i32p, ci, ci, C.c_void_p)
         api["time_lifts"] = fn("time_lifts", C.c_double, vp, _i32p, ci)
+        api["inc_create"] = fn("inc_create", vp)
+        api["inc_destroy"] = fn("inc_destroy", None, vp)
+        api["inc_query"] = fn("inc_query", C.c_double, vp, vp, _i32p, ci, _u16p)
     _libs[kind] = api
     return api
 

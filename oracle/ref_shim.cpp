@@ -209,6 +209,26 @@ void ref_lift_point(void* p, int r, int c, uint16_t* X, int32_t* lens, int32_t* 
         o += lv.X[l].size();
     }
 }
+// MinPlusOracle::queryAllPairsIncremental with a persistent state: the
+// handle keeps the scratch, the state and the matrix between calls.
+struct IncHandle {
+    MinPlusOracle::HTripQueryScratch scratch;
+    MinPlusOracle::HTripIncrementalState state;
+    std::vector<dist_t> m;
+};
+void* ref_inc_create() { return new IncHandle(); }
+void ref_inc_destroy(void* q) { delete static_cast<IncHandle*>(q); }
+// Seconds for one incremental refresh of the k listed viewpoints; out gets k*k.
+double ref_inc_query(void* p, void* q, const int32_t* rc, int k, uint16_t* out) {
+    auto* h = static_cast<RefHandle*>(p);
+    auto* inc = static_cast<IncHandle*>(q);
+    const auto pts = to_pts(rc, k);
+    auto t0 = std::chrono::steady_clock::now();
+    h->oracle.queryAllPairsIncremental(pts, h->grid, inc->m, inc->scratch, inc->state);
+    const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::memcpy(out, inc->m.data(), inc->m.size() * 2);
+    return dt;
+}
 // Seconds for n liftPoint calls on the listed cells (one thread).
 double ref_time_lifts(void* p, const int32_t* rc, int n) {
     auto* h = static_cast<RefHandle*>(p);

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from htrip_amd import grids, serving
-from htrip_amd._lib import HTG_E_RANGE, GpuMap, HtgError
+from htrip_amd._lib import HTG_E_RANGE, GpuMap, HtgError, IncState
 from oracle.oracle import Oracle, grid_bfs
 
 pytestmark = pytest.mark.gpu
@@ -265,6 +265,62 @@ def test_lift_point_equals_the_oracle(gpu_ctx, kind, n, b, g, seed):
     with pytest.raises(HtgError) as ei:
         G.lift_point(S, 0)
     assert ei.value.code == HTG_E_RANGE
+
+
+@pytest.mark.parametrize("kind,n,b,seed", [("street", 128, 16, 61), ("maze", 96, 8, 62), ("street", 64, 8, 63)])
+def test_incremental_query_equals_the_full_query(gpu_ctx, kind, n, b, seed):
+    """queryAllPairsIncremental (src/minplus_oracle.cpp:1444-2002): after reveal
+    batches, single openings, blockings and a rebuild, the refreshed matrix
+    equals the full query and BFS; with no update in between, or updates that
+    touch no viewpoint's chain, it is returned from the cache."""
+    print(f"incremental {kind} n={n} b={b} seed={seed}", flush=True)
+    truth = make(kind, n, seed)
+    obs = grids.disk_observed(truth, [tuple(grids.sample_free(truth, 1, seed)[0])], n // 3)
+    G = GpuMap(obs, b, 2)
+    G.build()
+    order = grids.frontier_order(truth, obs)
+    vps = grids.sample_free(truth, 40, seed + 1)  # some not revealed yet
+    st = IncState()
+    m, nr = G.query_incremental(st, vps)
+    assert nr == len(vps)
+    np.testing.assert_array_equal(m, grid_bfs(G.cells, vps))
+    m2, nr = G.query_incremental(st, vps)
+    assert nr == 0 and np.array_equal(m, m2)
+    rng = np.random.default_rng(seed)
+    pos, cached = 0, 0
+    for step in range(24):
+        kind_of = step % 4
+        if kind_of == 0:
+            m_ = int(rng.integers(5, 40))
+            G.open_cells(order[pos:pos + m_])
+            pos += m_
+        elif kind_of == 1:
+            r, c = order[pos]
+            pos += 1
+            G.open_cell(int(r), int(c))
+        elif kind_of == 2 and step % 8 == 2:
+            free = np.argwhere(G.cells == 1)
+            G.block_cells(free[rng.integers(len(free))][None].astype(np.int32))
+        m, nr = G.query_incremental(st, vps)
+        assert nr in (0, len(vps))
+        cached += nr == 0
+        np.testing.assert_array_equal(m, G.query(vps), err_msg=f"step {step}")
+        np.testing.assert_array_equal(m, grid_bfs(G.cells, vps), err_msg=f"step {step}")
+    assert cached >= 3  # at least the steps without an update
+    # other viewpoints, a rebuild, another map: recomputed; k = 0 clears
+    _, nr = G.query_incremental(st, vps[:30])
+    assert nr == 30
+    G.build()
+    m, nr = G.query_incremental(st, vps[:30])
+    assert nr == 30 and np.array_equal(m, grid_bfs(G.cells, vps[:30]))
+    H = G.clone()
+    m, nr = H.query_incremental(st, vps[:30])
+    assert nr == 30 and np.array_equal(m, grid_bfs(G.cells, vps[:30]))
+    m, nr = H.query_incremental(st, vps[:0])
+    assert m.shape == (0, 0) and nr == 0
+    _, nr = H.query_incremental(st, vps[:30])
+    assert nr == 30
+    st.close()
 
 
 def test_cpp_dropin_program(gpu_ctx):
