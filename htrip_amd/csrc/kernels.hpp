@@ -171,7 +171,8 @@ bool launch_q_fold(int n, int nz, int ldz, const int* src, int ldp, int ksplit, 
 void launch_lift_leaf(const uint16_t* row, int b, uint16_t* X0, hipStream_t st);
 // liftPoint, one level: Xo[e] = min_p X[p] (+) D[row0 + p][column of exterior
 // port e] over the pc rows of the point's child, in nchunks chunks of
-// rows_per_chunk rows (part: nchunks x ld scratch).
+// rows_per_chunk (<= kLiftRowsPerChunk) rows (part: nchunks x ld scratch).
+constexpr int kLiftRowsPerChunk = 64;
 void launch_lift_level(const uint16_t* D, int ld, int row0, int pc, int rows_per_chunk, int nchunks,
                        const uint16_t* X, uint16_t* part, const int* ext_child, const int* ext_cport, int ne,
                        uint16_t* Xo, hipStream_t st);

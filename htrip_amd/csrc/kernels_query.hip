@@ -497,36 +497,56 @@ __global__ __launch_bounds__(256) void k_lift_leaf(const uint16_t* __restrict__ 
 
 // Level l: the rows of child q in the node's D_U, against the child's lift.
 // Workgroup (x, y) covers 512 columns (two per lane, u32 loads) and the rows
-// [y * rows_per_chunk, ...) of the child: part[y][j] = min_p X[p] (+) D[row0 + p][j].
-// X[p] is the same for every lane (a scalar load), and rows with X[p] = INF
-// are skipped as the reference skips inactive ports.
+// [y * rows_per_chunk, ...) of the child (at most kLiftRows):
+//     part[y][j] = min_p X[p] (+) D[row0 + p][j].
+// The chunk's X values sit in LDS; rows are read 16 at a time so that 16
+// loads per lane are in flight, and rows with X[p] = INF are not read (the
+// reference skips inactive ports the same way).
+constexpr int kLiftRows = kLiftRowsPerChunk;
 __global__ __launch_bounds__(256) void k_lift_rows(const uint16_t* __restrict__ D, int ld, int row0, int pc,
                                                    int rows_per_chunk, const uint16_t* __restrict__ X,
                                                    uint16_t* __restrict__ part) {
-    const int j2 = blockIdx.x * 256 + threadIdx.x;  // column pair
+    __shared__ uint32_t xs[kLiftRows];
     const int p0 = blockIdx.y * rows_per_chunk;
-    const int p1 = min(pc, p0 + rows_per_chunk);
+    const int n = min(pc - p0, rows_per_chunk);
+    if (int(threadIdx.x) < n) xs[threadIdx.x] = qdup(X[p0 + threadIdx.x]);
+    __syncthreads();
+    const int j2 = blockIdx.x * 256 + threadIdx.x;  // column pair
     if (2 * j2 >= ld) return;
+    const size_t ldw = size_t(ld) / 2;
+    const uint32_t* Dp = reinterpret_cast<const uint32_t*>(D + size_t(row0 + p0) * ld) + j2;
     uint32_t acc = kInf2;
-    for (int p = p0; p < p1; ++p) {
-        const uint32_t x = X[p];
-        if (x >= kInf) continue;
-        const uint32_t d = reinterpret_cast<const uint32_t*>(D + size_t(row0 + p) * ld)[j2];
-        acc = qrelax2(acc, qdup(x), d);
+    for (int p = 0; p < n; p += 16) {
+        uint32_t x[16], d[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) x[k] = p + k < n ? xs[p + k] : kInf2;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) d[k] = x[k] != kInf2 ? Dp[size_t(p + k) * ldw] : kInf2;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) acc = qrelax2(acc, x[k], d[k]);
     }
     reinterpret_cast<uint32_t*>(part + size_t(blockIdx.y) * ld)[j2] = acc;
 }
 
-// Xo[e] = min over row chunks of part[chunk][column of exterior port e].
+// Xo[e] = min over row chunks of part[chunk][column of exterior port e]
+// (eight independent loads in flight per lane).
 __global__ __launch_bounds__(256) void k_lift_ext(const uint16_t* __restrict__ part, int ld, int nchunks,
                                                   const int* __restrict__ ext_child,
                                                   const int* __restrict__ ext_cport, int pc, int ne,
                                                   uint16_t* __restrict__ Xo) {
     const int e = blockIdx.x * 256 + threadIdx.x;
     if (e >= ne) return;
-    const int col = ext_child[e] * pc + ext_cport[e];
+    const uint16_t* col = part + ext_child[e] * pc + ext_cport[e];
     uint16_t v = kInf;
-    for (int ch = 0; ch < nchunks; ++ch) v = min(v, part[size_t(ch) * ld + col]);
+    int ch = 0;
+    for (; ch + 8 <= nchunks; ch += 8) {
+        uint16_t t[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) t[k] = col[size_t(ch + k) * ld];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v = min(v, t[k]);
+    }
+    for (; ch < nchunks; ++ch) v = min(v, col[size_t(ch) * ld]);
     Xo[e] = v;
 }
 

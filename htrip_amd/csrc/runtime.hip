@@ -1328,7 +1328,7 @@ int Map::lift_point(int r, int c, uint16_t* x, long long x_cap, int32_t* lens, i
         const LevelGeo& L = lv[size_t(l) - 1];
         const int col_blocks = (L.d.ld / 2 + 255) / 256;
         int nch = std::max(1, std::min(512 / col_blocks, (L.d.pc + 15) / 16));
-        const int rpc = (L.d.pc + nch - 1) / nch;
+        const int rpc = std::min(kLiftRowsPerChunk, (L.d.pc + nch - 1) / nch);
         nch = (L.d.pc + rpc - 1) / rpc;
         rpc_l[size_t(l)] = rpc;
         nch_l[size_t(l)] = nch;
