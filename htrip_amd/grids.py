@@ -15,6 +15,8 @@ an explicit integer mapping) so every grid replays bit for bit on any host:
   which the remaining free cells are revealed (BFS over ground truth from the
   observed set).
 * ``sample_free``  -- distinct viewpoints drawn uniformly from free cells.
+* ``load_movingai`` -- a MovingAI .map file (or .map.gz) as ground truth,
+  parsed as the reference's GridMap::loadMovingAI does.
 
 Cell codes follow include/htrip/types.hpp:101-105: 0 Unknown, 1 Free,
 2 Obstacle.  Grids are ``numpy.uint8`` arrays of shape (rows, cols).
@@ -164,3 +166,60 @@ def reveal_scenario(truth: np.ndarray, seed: int, radius: int = 64):
     start = sample_free(truth, 1, seed)[0]
     obs = disk_observed(truth, [tuple(start)], radius)
     return obs, frontier_order(truth, obs)
+
+
+def _second_int(line: bytes) -> int:
+    """The integer a stream extraction reads after the first token (0 when it fails)."""
+    import re
+
+    parts = line.split(None, 1)
+    m = re.match(rb"[+-]?\d+", parts[1]) if len(parts) > 1 else None
+    return int(m.group(0)) if m else 0
+
+
+def load_movingai(src) -> np.ndarray:
+    """Ground truth of a MovingAI .map (a path, ``.map.gz`` path, bytes or str)
+    as cell codes, following GridMap::loadMovingAI (src/grid_map.cpp:16-76):
+    header lines up to one that is exactly ``map`` (trailing CR, space and tab
+    ignored), where lines starting with ``height`` / ``width`` give the size;
+    then one line per row, where '.', 'G' and 'S' are Free (1) and every other
+    character, and every cell past a short line or a missing row, is an
+    Obstacle (2).  Raises ValueError where the reference returns false."""
+    import gzip
+    import os
+
+    if isinstance(src, (str, os.PathLike)) and os.path.exists(src):
+        with open(src, "rb") as f:
+            data = f.read()
+        if str(src).endswith(".gz"):
+            data = gzip.decompress(data)
+    elif isinstance(src, bytes):
+        data = src
+    else:
+        data = str(src).encode()
+    lines = data.split(b"\n")
+    h = w = 0
+    i = 0
+    while i < len(lines):
+        line = lines[i].rstrip(b"\r \t")
+        i += 1
+        if line.startswith(b"height"):
+            h = _second_int(line)
+        elif line.startswith(b"width"):
+            w = _second_int(line)
+        elif line == b"map":
+            break
+    if h <= 0 or w <= 0:
+        raise ValueError("not a MovingAI map: missing or non-positive height / width")
+    out = np.full((h, w), 2, dtype=np.uint8)
+    free = np.frombuffer(b".GS", dtype=np.uint8)
+    # std::getline yields a last, empty piece only when the file does not end
+    # with a newline; a trailing "\n" ends the stream
+    rows = lines[i:]
+    if rows and rows[-1] == b"" and data.endswith(b"\n"):
+        rows = rows[:-1]
+    for r, line in enumerate(rows[:h]):
+        row = np.frombuffer(line[:w], dtype=np.uint8)
+        out[r, :len(row)] = np.where(np.isin(row, free), 1, 2)
+    return out
+

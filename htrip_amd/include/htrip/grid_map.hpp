@@ -72,20 +72,27 @@ public:
         return true;
     }
 
-    // MovingAI .map: header (type/height/width), "map", then rows of
-    // characters; '.', 'G' and 'S' are free, everything else is an obstacle.
-    // observed starts all Unknown.
+    // src/grid_map.cpp:16-76.  MovingAI .map: header lines up to a line that is
+    // exactly "map" (trailing CR, space and tab ignored); lines starting with
+    // "height" / "width" give the size.  Then one line per row: '.', 'G' and
+    // 'S' are Free, every other character is an Obstacle, and so are cells past
+    // a short line or a missing row.  observed starts all Unknown.
     bool loadMovingAI(const std::string& filepath) {
         std::ifstream f(filepath);
         if (!f.is_open()) return false;
-        std::string line, key;
-        int h = -1, w = -1;
+        std::string line;
+        int h = 0, w = 0;
         while (std::getline(f, line)) {
+            while (!line.empty() && (line.back() == '\r' || line.back() == ' ' || line.back() == '\t')) line.pop_back();
             std::istringstream ss(line);
-            ss >> key;
-            if (key == "height") ss >> h;
-            else if (key == "width") ss >> w;
-            else if (key == "map") break;
+            std::string key;
+            if (line.rfind("height", 0) == 0) {
+                ss >> key >> h;
+            } else if (line.rfind("width", 0) == 0) {
+                ss >> key >> w;
+            } else if (line == "map") {
+                break;
+            }
         }
         if (h <= 0 || w <= 0) return false;
         rows = h;

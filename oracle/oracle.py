@@ -89,11 +89,24 @@ def _lib(kind: str):
         api["import_node"] = fn("import_node", None, vp, ci, ci, _u16p)
         api["time_queries"] = fn("time_queries", C.c_double, vp, _i32p, _i32p, ci, ci, C.c_void_p)
         api["time_lifts"] = fn("time_lifts", C.c_double, vp, _i32p, ci)
+        api["load_movingai"] = fn("load_movingai", ci, C.c_char_p, _i32p, _i32p, _u8p, C.c_int64)
         api["inc_create"] = fn("inc_create", vp)
         api["inc_destroy"] = fn("inc_destroy", None, vp)
         api["inc_query"] = fn("inc_query", C.c_double, vp, vp, _i32p, ci, _u16p)
     _libs[kind] = api
     return api
+
+
+def reference_load_movingai(path: str):
+    """The compiled reference's GridMap::loadMovingAI on a file: its ground
+    truth codes, or None where it returns false."""
+    api = _lib("reference")
+    rows, cols = np.zeros(1, np.int32), np.zeros(1, np.int32)
+    if not api["load_movingai"](path.encode(), rows, cols, np.zeros(1, np.uint8), 0):
+        return None
+    out = np.empty(int(rows[0]) * int(cols[0]), dtype=np.uint8)
+    api["load_movingai"](path.encode(), rows, cols, out, out.size)
+    return out.reshape(int(rows[0]), int(cols[0]))
 
 
 def grid_bfs(cells: np.ndarray, vps: np.ndarray) -> np.ndarray:

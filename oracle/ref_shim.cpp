@@ -190,6 +190,17 @@ double ref_time_queries(void* p, const int32_t* rc, const int32_t* ks, int nq, i
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// GridMap::loadMovingAI: 1 on success with rows/cols set and, when they fit
+// in cap cells, the ground truth codes written to out; 0 where it returns false.
+int ref_load_movingai(const char* path, int32_t* rows, int32_t* cols, uint8_t* out, int64_t cap) {
+    GridMap g;
+    if (!g.loadMovingAI(path)) return 0;
+    *rows = g.rows;
+    *cols = g.cols;
+    if (int64_t(g.rows) * g.cols <= cap)
+        for (size_t i = 0; i < g.ground_truth.size(); ++i) out[i] = uint8_t(g.ground_truth[i]);
+    return 1;
+}
 // MinPlusOracle::liftPoint: X levels back to back (lens[l] entries each),
 // par/quad/off H+1 entries, active[l] in the reference's order padded with -1
 // to lens[l].
