@@ -278,7 +278,11 @@ def served_checks(tr, state0, base, n=4, kmax=96):
 def run_plain(args):
     from htrip_amd import _lib, grids
 
-    cells = grids.street_grid(N, SEED)
+    if args.map:  # a shipped MovingAI map, fully revealed (campaign_support.cpp:150-159)
+        cells = grids.load_movingai(os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden",
+                                                 "maps", args.map + ".map.gz"))
+    else:
+        cells = grids.street_grid(N, SEED)
     gmap = _lib.GpuMap(cells, B, G)
     gmap.profile(True)
     t0 = time.time()
@@ -299,7 +303,8 @@ def run_plain(args):
         "value": rate, "unit": "matrices/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": dt * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "u16", "data": "synthetic",
-        "config": {"workload": "1024x1024 street-like grid seed 3, b=32, g=2x2, plain queryAllPairs k=64",
+        "config": {"workload": (f"MovingAI {args.map} fully revealed" if args.map else
+                                "1024x1024 street-like grid seed 3") + f", b=32, g=2x2, plain queryAllPairs k={args.k}",
                    "mode": "plain", "k": args.k},
         "roofline": roof, "valu": valu, "cpu_baseline": cpu, "kernels": kernels,
     }
@@ -945,6 +950,8 @@ def main():
                     choices=["plain", "served", "update", "small", "build", "reveal", "multi", "kernel", "lift",
                              "incremental"])
     ap.add_argument("--k", type=int, default=64)
+    ap.add_argument("--map", default="", help="plain mode: a shipped MovingAI map (tests/golden/maps) instead of "
+                                              "the synthetic grid, e.g. Boston_0_1024")
     ap.add_argument("--window", type=int, default=64, help="requests per served step")
     ap.add_argument("--slots", type=int, default=4, help="query streams (the update stream makes one more)")
     ap.add_argument("--update-cu", type=int, default=0, help="percent of CUs for update kernels (0 = all)")

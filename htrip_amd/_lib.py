@@ -21,7 +21,7 @@ EXPORTS = [
     "htg_ctx_create", "htg_ctx_destroy", "htg_last_error", "htg_ctx_mem_info",
     "htg_map_create", "htg_map_destroy", "htg_map_clone", "htg_map_footprint", "htg_map_info_get",
     "htg_build", "htg_open_cells", "htg_open_cell", "htg_block_cells", "htg_query_all_pairs", "htg_query_batch",
-    "htg_lift_point", "htg_inc_create", "htg_inc_destroy", "htg_query_all_pairs_incremental", "htg_last_batch_stats", "htg_sync",
+    "htg_lift_point", "htg_lift_root", "htg_inc_create", "htg_inc_destroy", "htg_query_all_pairs_incremental", "htg_last_batch_stats", "htg_sync",
     "htg_read_leaf_D", "htg_read_leaf_S", "htg_node_dims", "htg_read_node_DU", "htg_read_node_SU",
     "htg_profile_enable", "htg_profile_get",
     "htg_get_tuning", "htg_set_tuning", "htg_autotune",
@@ -96,6 +96,7 @@ def load(path: str = LIB_PATH):
         "htg_query_all_pairs": (C.c_int, [vp, pi32, i32, pu16]),
         "htg_query_batch": (C.c_int, [vp, pi32, pi32, i32, pu16]),
         "htg_lift_point": (C.c_int, [vp, i32, i32, pu16, C.c_int64, pi32, pi32, pi32, pi32]),
+        "htg_lift_root": (C.c_int, [C.POINTER(vp), i32, i32, pu8]),
         "htg_inc_create": (C.c_int, [C.POINTER(vp)]),
         "htg_inc_destroy": (C.c_int, [vp]),
         "htg_query_all_pairs_incremental": (C.c_int, [vp, vp, pi32, i32, pu16, pi32]),
@@ -271,6 +272,21 @@ class GpuMap:
             res.append(out[o:o + k * k].reshape(k, k))
             o += int(k) * int(k)
         return res
+
+    def lift_root(self, cells: np.ndarray) -> None:
+        """liftRoot over the grown grid ``cells``: the square grows by g and
+        gains a level.  Inside the old square the map keeps its own state."""
+        cells = np.ascontiguousarray(cells, dtype=np.uint8)
+        S0 = self.S
+        r0, c0 = self.cells.shape
+        h = C.c_void_p(self.h.value)
+        check(self.lib.htg_lift_root(C.byref(h), cells.shape[0], cells.shape[1], ptr(cells, C.c_uint8)))
+        self.h = h
+        merged = cells.copy()
+        band = merged[:S0, :S0]
+        band[band == 1] = 0  # the old square's cells beyond the old grid stay impassable
+        merged[:r0, :c0] = self.cells
+        self.cells = merged
 
     def query_incremental(self, state: "IncState", vps):
         """queryAllPairsIncremental: (k x k matrix, recomputed) where recomputed

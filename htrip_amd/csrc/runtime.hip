@@ -494,9 +494,9 @@ static uint64_t build_ws_bytes(int b, int g, int S, int H, int P) {
     return m + kUpdateListReserve;
 }
 
-static void geometry(int b, int g, int rows, int cols, int& S, int& H) {
+static void geometry(int b, int g, int rows, int cols, int& S, int& H, int min_S = 0) {
     S = b;
-    while (S < std::max(rows, cols)) S *= g;
+    while (S < std::max(rows, cols) || S < min_S) S *= g;
     H = 0;
     for (int t = S / b; t > 1; t /= g) ++H;
 }
@@ -617,8 +617,6 @@ std::vector<int> Map::build_pivot_counts(int lvl, const std::vector<int>* nodes)
 // superset: a port that turned impassable keeps an INF row and column, which
 // the live-row kernels and the snapshot copy rely on.
 int Map::block_cells(const int32_t* rcs, int n) {
-    hipError_t e;
-    int rc;
     timing_now = prof_ref().take();
     std::vector<int2> closed;
     std::vector<int> leaves;
@@ -651,6 +649,18 @@ int Map::block_cells(const int32_t* rcs, int n) {
             below_nt = L.d.nt;
         }
     }
+    return rebuild(closed, leaves, anc);
+}
+
+// Recompute the listed leaves (BFS from the passability) and rebuild the
+// listed nodes of every level from their children with the build kernels
+// (assemble, seam edges, closure over the seam endpoints, S_U); closed cells
+// are cleared in the device passability first.  anc[lvl] lists level lvl's
+// nodes (ascending; anc[0] unused).  Stamps every listed table.
+int Map::rebuild(const std::vector<int2>& closed, const std::vector<int>& leaves,
+                 const std::vector<std::vector<int>>& anc) {
+    hipError_t e;
+    int rc;
     Packer pk;
     const size_t o_cells = pk.add(closed), o_leaves = pk.add(leaves);
     std::vector<size_t> o_anc(size_t(H) + 1);
@@ -660,8 +670,9 @@ int Map::block_cells(const int32_t* rcs, int n) {
     if ((e = hipMemcpyAsync(ws_upd.p, ws_upd_h.p, pk.size, hipMemcpyHostToDevice, stream)) != hipSuccess)
         return hip_fail(e, "upload block lists");
     const char* up = static_cast<const char*>(ws_upd.p);
-    launch_set_pass(static_cast<uint8_t*>(d_pass.p), S, reinterpret_cast<const int2*>(up + o_cells), int(closed.size()),
-                    stream, 0);
+    if (!closed.empty())
+        launch_set_pass(static_cast<uint8_t*>(d_pass.p), S, reinterpret_cast<const int2*>(up + o_cells),
+                        int(closed.size()), stream, 0);
     const double leaf_bytes = (double(b) * b * b * b + double(4 * b - 4) * (4 * b - 4)) * 2 + double(b) * b;
     KTimer lkt;
     const int lpi = prof_open(HTG_K_LEAF_APSP, leaf_bytes * double(leaves.size()), 0, lkt);
@@ -674,6 +685,7 @@ int Map::block_cells(const int32_t* rcs, int n) {
         LevelGeo& L = lv[size_t(lvl) - 1];
         const std::vector<int>& hn = anc[size_t(lvl)];
         const int na = int(hn.size());
+        if (na == 0) continue;
         const int* nodes = reinterpret_cast<const int*>(up + o_anc[size_t(lvl)]);
         if ((rc = ensure_build_ws(lvl))) return rc;
         launch_fill_u8(static_cast<uint8_t*>(ws_flags.p), size_t(na) * L.d.nV, 0, stream);
@@ -691,8 +703,8 @@ int Map::block_cells(const int32_t* rcs, int n) {
                           static_cast<const uint16_t*>(L.DU.p), L.node_stride, static_cast<uint16_t*>(L.SU.p), nullptr,
                           stream);
     }
-    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "block launch");
-    if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "block sync");
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "rebuild launch");
+    if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "rebuild sync");
     prof_ref().harvest();
     stamp(leaves, anc, 1);
     return HTG_OK;
@@ -796,12 +808,16 @@ int Map::build() {
     prof_ref().harvest();
     built = true;
     epoch++;
+    reset_versions();
+    return HTG_OK;
+}
+
+void Map::reset_versions() {
     gen++;
     clock++;
     leaf_ver.assign(size_t(nl) * nl, 0);
     node_ver.assign(size_t(H), {});
     for (int l = 0; l < H; ++l) node_ver[size_t(l)].assign(size_t(lv[size_t(l)].nn), 0);
-    return HTG_OK;
 }
 
 void Map::stamp(const std::vector<int>& leaves, const std::vector<std::vector<int>>& nodes, int first_level) {
@@ -1842,7 +1858,7 @@ int htg_map_footprint(const htg_config* cfg, int32_t rows, int32_t cols, uint64_
 
 namespace htg {
 int map_create_on(htg_ctx* ctx, const htg_config* cfg, int32_t rows, int32_t cols, const uint8_t* cells,
-                  uint64_t hbm_budget, hipStream_t shared, htg_map** out) {
+                  uint64_t hbm_budget, hipStream_t shared, htg_map** out, int min_S) {
     HTG_GUARD_BEGIN
     if (!ctx || !cfg || !cells || !out) return fail(HTG_E_INVALID, "null argument");
     // include/htrip/types.hpp:221-224 plus this build's limits: 2 <= b <= 64
@@ -1853,9 +1869,11 @@ int map_create_on(htg_ctx* ctx, const htg_config* cfg, int32_t rows, int32_t col
     if (rows <= 0 || cols <= 0) return fail(HTG_E_INVALID, "empty grid");
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
-    uint64_t need = 0;
-    int rc = htg_map_footprint(cfg, rows, cols, &need);
-    if (rc) return rc;
+    if (min_S > (1 << 16)) return fail(HTG_E_INVALID, "hierarchy side too large");
+    int rc;
+    int S0, H0;
+    geometry(b, g, rows, cols, S0, H0, min_S);
+    const uint64_t need = tables_bytes(b, g, S0, H0) + build_ws_bytes(b, g, S0, H0, 64);
     if (hbm_budget && need > hbm_budget)
         return fail(HTG_E_NOMEM, "map needs " + std::to_string(need) + " bytes, budget " + std::to_string(hbm_budget));
     size_t fr = 0, tot = 0;
@@ -1871,7 +1889,8 @@ int map_create_on(htg_ctx* ctx, const htg_config* cfg, int32_t rows, int32_t col
     m->rows = rows;
     m->cols = cols;
     m->budget = hbm_budget;
-    geometry(b, g, rows, cols, m->S, m->H);
+    m->min_S = min_S;
+    geometry(b, g, rows, cols, m->S, m->H, min_S);
     m->nl = m->S / b;
     m->cells_code.assign(cells, cells + size_t(rows) * cols);
     m->pass.assign(size_t(m->S) * m->S, 0);
@@ -1925,7 +1944,8 @@ int map_clone_on(const htg_map* src, uint64_t hbm_budget, hipStream_t shared, ht
     HTG_GUARD_BEGIN
     if (!src || !out) return fail(HTG_E_INVALID, "null argument");
     htg_map* m = nullptr;
-    int rc = map_create_on(src->ctx, &src->cfg, src->rows, src->cols, src->cells_code.data(), hbm_budget, shared, &m);
+    int rc = map_create_on(src->ctx, &src->cfg, src->rows, src->cols, src->cells_code.data(), hbm_budget, shared, &m,
+                           src->min_S);
     if (rc) return rc;
     std::unique_ptr<htg_map> hold(m);
     m->pass = src->pass;
@@ -1955,6 +1975,85 @@ int map_clone_on(const htg_map* src, uint64_t hbm_budget, hipStream_t shared, ht
             return hip_fail(e, "clone copy");
     }
     if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(e, "clone copy");
+    *out = hold.release();
+    return HTG_OK;
+    HTG_GUARD_END
+}
+
+// liftRoot (src/minplus_oracle.cpp:2008-2114).  The square grows from S to
+// g S and gains a root level.  Leaves and nodes inside the old square keep
+// their tables (copied into their places in the larger level arrays); the
+// leaves and nodes outside it and the new root are computed from the grid,
+// leaves by BFS and nodes from their children, as the reference computes
+// them (LeafTile::recomputeFromGrid, InternalNode::recomputeFromChildren).
+// Inside the old square the map keeps its own cell state, which its kept
+// tables describe; cells there that were outside the old grid stay
+// impassable until opened.
+int lift_root(htg_map* src, int32_t rows, int32_t cols, const uint8_t* cells, htg_map** out) {
+    HTG_GUARD_BEGIN
+    if (!src || !cells || !out) return fail(HTG_E_INVALID, "null argument");
+    if (!src->built) return fail(HTG_E_STATE, "map not built");
+    const int b = src->b, g = src->g, S0 = src->S, nl0 = src->nl, H0 = src->H;
+    const long long S1 = (long long)S0 * g;
+    if (S1 > (1 << 16)) return fail(HTG_E_INVALID, "hierarchy side too large");
+    if (rows < src->rows || cols < src->cols || rows > S1 || cols > S1)
+        return fail(HTG_E_INVALID, "the grid must contain the old one and fit the lifted square");
+    hipError_t e = hipSetDevice(src->ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    if ((e = hipStreamSynchronize(src->stream)) != hipSuccess) return hip_fail(e, "sync");
+    std::vector<uint8_t> merged(size_t(rows) * cols);
+    for (int r = 0; r < rows; ++r)
+        for (int c = 0; c < cols; ++c) {
+            uint8_t v = cells[size_t(r) * cols + c];
+            if (r < S0 && c < S0) {
+                if (r < src->rows && c < src->cols) v = src->cells_code[size_t(r) * src->cols + c];
+                if (!src->pass[size_t(r) * S0 + c] && v == 1) v = 0;  // kept tables treat it as impassable
+            }
+            merged[size_t(r) * cols + c] = v;
+        }
+    htg_map* m = nullptr;
+    int rc = map_create_on(src->ctx, &src->cfg, rows, cols, merged.data(), src->budget, nullptr, &m, int(S1));
+    if (rc) return rc;
+    std::unique_ptr<htg_map> hold(m);
+    if (m->S != S1 || m->H != H0 + 1) return fail(HTG_E_INVALID, "unexpected lifted geometry");
+    m->tune = src->tune;
+    m->stats = src->stats;
+    m->epoch = src->epoch + 1;  // src/minplus_oracle.cpp:2113
+    const hipStream_t st = m->stream;
+    const auto d2d = hipMemcpyDeviceToDevice;
+    const int nl1 = m->nl;
+    const size_t n = size_t(b) * b, pl = size_t(4 * b - 4);
+    // old leaf (tx, ty) moves from ty nl0 + tx to ty nl1 + tx: one 2-D copy per array
+    if ((e = hipMemcpy2DAsync(m->leafD.p, size_t(nl1) * n * n * 2, src->leafD.p, size_t(nl0) * n * n * 2,
+                              size_t(nl0) * n * n * 2, size_t(nl0), d2d, st)) != hipSuccess ||
+        (e = hipMemcpy2DAsync(m->leafS.p, size_t(nl1) * pl * pl * 2, src->leafS.p, size_t(nl0) * pl * pl * 2,
+                              size_t(nl0) * pl * pl * 2, size_t(nl0), d2d, st)) != hipSuccess)
+        return hip_fail(e, "lift copy leaves");
+    std::vector<int> leaves;
+    for (int ty = 0; ty < nl1; ++ty)
+        for (int tx = 0; tx < nl1; ++tx)
+            if (tx >= nl0 || ty >= nl0) leaves.push_back(ty * nl1 + tx);
+    std::vector<std::vector<int>> nodes(size_t(H0) + 2);
+    for (int l = 1; l <= H0; ++l) {
+        const LevelGeo& Lo = src->lv[size_t(l) - 1];
+        const LevelGeo& Ln = m->lv[size_t(l) - 1];
+        if (Lo.node_stride != Ln.node_stride || Lo.d.ne != Ln.d.ne) return fail(HTG_E_INVALID, "level layout differs");
+        const int nt0 = Lo.d.nt, nt1 = Ln.d.nt;
+        const size_t du = Lo.node_stride * 2, su = size_t(Lo.d.ne) * Lo.d.ne * 2;
+        if ((e = hipMemcpy2DAsync(Ln.DU.p, size_t(nt1) * du, Lo.DU.p, size_t(nt0) * du, size_t(nt0) * du, size_t(nt0),
+                                  d2d, st)) != hipSuccess ||
+            (e = hipMemcpy2DAsync(Ln.SU.p, size_t(nt1) * su, Lo.SU.p, size_t(nt0) * su, size_t(nt0) * su, size_t(nt0),
+                                  d2d, st)) != hipSuccess)
+            return hip_fail(e, "lift copy nodes");
+        for (int ty = 0; ty < nt1; ++ty)
+            for (int tx = 0; tx < nt1; ++tx)
+                if (tx >= nt0 || ty >= nt0) nodes[size_t(l)].push_back(ty * nt1 + tx);
+    }
+    nodes[size_t(H0) + 1].push_back(0);  // the new root
+    m->reset_versions();
+    m->timing_now = false;
+    if ((rc = m->rebuild({}, leaves, nodes))) return rc;
+    m->built = true;
     *out = hold.release();
     return HTG_OK;
     HTG_GUARD_END
@@ -2069,6 +2168,18 @@ int htg_lift_point(htg_map* map, int32_t r, int32_t c, uint16_t* x, int64_t x_ca
     hipError_t e = hipSetDevice(map->ctx->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     return map->lift_point(r, c, x, x_cap, lens, parent_node_idx, child_quad, child_offset);
+    HTG_GUARD_END
+}
+
+int htg_lift_root(htg_map** map, int32_t rows, int32_t cols, const uint8_t* cells) {
+    HTG_GUARD_BEGIN
+    if (!map || !*map) return fail(HTG_E_INVALID, "null map");
+    htg_map* m = nullptr;
+    const int rc = htg::lift_root(*map, rows, cols, cells, &m);
+    if (rc) return rc;
+    htg_map_destroy(*map);
+    *map = m;
+    return HTG_OK;
     HTG_GUARD_END
 }
 

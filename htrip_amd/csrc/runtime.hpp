@@ -249,6 +249,7 @@ struct Map {
     htg_ctx* ctx = nullptr;
     htg_config cfg{};
     int b = 16, g = 2, rows = 0, cols = 0, S = 0, H = 0, nl = 0;
+    int min_S = 0;  // side forced by a lifted root (0: the smallest square over the grid)
     std::vector<uint8_t> cells_code;  // rows*cols cell codes as last known
     std::vector<uint8_t> pass;        // S*S passable flags (host mirror of d_pass)
     DevBuf d_pass, leafD, leafS;
@@ -295,6 +296,9 @@ struct Map {
     int build();
     int open_cells(const int32_t* rc, int n);
     int block_cells(const int32_t* rc, int n);
+    int rebuild(const std::vector<int2>& closed, const std::vector<int>& leaves,
+                const std::vector<std::vector<int>>& anc);
+    void reset_versions();
     int query(const int32_t* rc, const int32_t* ks, int nq, uint16_t* out_host, QueryWS& ws);
     // liftPoint of cell (r, c): x receives levels 0..H back to back (lens[l]
     // entries each), meta the ancestor node, child quad and child offset.
@@ -362,7 +366,11 @@ struct htg_ctx {
 
 namespace htg {
 // htg_map_create / htg_map_clone on a given stream (nullptr: a new one).
+// min_S: the hierarchy square is at least this side (a lifted root).
 int map_create_on(htg_ctx* ctx, const htg_config* cfg, int32_t rows, int32_t cols, const uint8_t* cells,
-                  uint64_t hbm_budget, hipStream_t shared, htg_map** out);
+                  uint64_t hbm_budget, hipStream_t shared, htg_map** out, int min_S = 0);
+// liftRoot: a new map of side g S over the grown grid, holding src's tables
+// in its old square and building the rest; src is left unchanged.
+int lift_root(htg_map* src, int32_t rows, int32_t cols, const uint8_t* cells, htg_map** out);
 int map_clone_on(const htg_map* src, uint64_t hbm_budget, hipStream_t shared, htg_map** out);
 }  // namespace htg

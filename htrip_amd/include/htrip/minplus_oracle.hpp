@@ -1,8 +1,9 @@
 #pragma once
 // Drop-in replacement of the reference's include/htrip/minplus_oracle.hpp:
 // htrip::MinPlusOracle (alias HTripOracle) with the same hot-path signatures
-// (build, onCellOpened, onBatchCellsOpened, onCellBlocked, queryAllPairs with
-// HTripQueryScratch, queryAllPairsIncremental, queryDistance, liftPoint) and
+// (build, onCellOpened, onBatchCellsOpened, onCellBlocked, liftRoot,
+// queryAllPairs with HTripQueryScratch, queryAllPairsIncremental,
+// queryDistance, liftPoint) and
 // the public metadata callers read
 // (config, S, H, num_leaf_tiles_x/y, leaves.size(), depth(), findLcaLevel,
 // totalMemoryBytes, last_batch_stats).  The index itself lives in HBM behind
@@ -105,25 +106,25 @@ public:
         detail::check(rc);
         map_.reset(m);
         detail::check(htg_build(m));
-        htg_map_info info{};
-        detail::check(htg_map_info_get(m, &info));
-        S = info.S;
-        H = info.H;
-        num_leaf_tiles_x = num_leaf_tiles_y = info.num_leaf_tiles_x;
-        leaves.clear();
-        const int b = config.tile_size_b;
-        for (int ty = 0; ty < num_leaf_tiles_y; ++ty)
-            for (int tx = 0; tx < num_leaf_tiles_x; ++tx) leaves.push_back({tx, ty, ty * b, tx * b, b});
-        levels.assign(size_t(H), {});
-        for (int lvl = 1; lvl <= H; ++lvl) {
-            int side = b;
-            for (int l = 0; l < lvl; ++l) side *= config.group_w;
-            const int nt = S / side > 0 ? S / side : 1;
-            for (int ty = 0; ty < nt; ++ty)
-                for (int tx = 0; tx < nt; ++tx) levels[size_t(lvl - 1)].push_back({lvl, tx, ty, ty * side, tx * side, side});
-        }
+        readMeta();
         epoch++;
         return true;
+    }
+
+    // src/minplus_oracle.cpp:2008-2114.  grid is the grown map (it contains
+    // the old one, within g*S on each side); S doubles (x group_w) and H grows
+    // by one.  Kept tables and the map's cell state inside the old square are
+    // the device's; the rest is computed from grid.observed.
+    void liftRoot(const GridMap& grid) {
+        if (!map_) throw std::runtime_error("htrip: liftRoot before build");
+        std::vector<uint8_t> cells(grid.observed.size());
+        for (size_t i = 0; i < cells.size(); ++i) cells[i] = uint8_t(grid.observed[i]);
+        htg_map* m = map_.get();
+        detail::check(htg_lift_root(&m, grid.rows, grid.cols, cells.data()));
+        (void)map_.release();  // destroyed by htg_lift_root
+        map_.reset(m);
+        readMeta();
+        epoch++;
     }
 
     [[nodiscard]] int depth() const noexcept { return H; }
@@ -369,6 +370,26 @@ private:
         void operator()(htg_map* m) const noexcept { htg_map_destroy(m); }
     };
     std::unique_ptr<htg_map, MapDeleter> map_;
+
+    void readMeta() {
+        htg_map_info info{};
+        detail::check(htg_map_info_get(map_.get(), &info));
+        S = info.S;
+        H = info.H;
+        num_leaf_tiles_x = num_leaf_tiles_y = info.num_leaf_tiles_x;
+        leaves.clear();
+        const int b = config.tile_size_b;
+        for (int ty = 0; ty < num_leaf_tiles_y; ++ty)
+            for (int tx = 0; tx < num_leaf_tiles_x; ++tx) leaves.push_back({tx, ty, ty * b, tx * b, b});
+        levels.assign(size_t(H), {});
+        for (int lvl = 1; lvl <= H; ++lvl) {
+            int side = b;
+            for (int l = 0; l < lvl; ++l) side *= config.group_w;
+            const int nt = S / side > 0 ? S / side : 1;
+            for (int ty = 0; ty < nt; ++ty)
+                for (int tx = 0; tx < nt; ++tx) levels[size_t(lvl - 1)].push_back({lvl, tx, ty, ty * side, tx * side, side});
+        }
+    }
 
     void copyMeta(const MinPlusOracle& o) {
         config = o.config;

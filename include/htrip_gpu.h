@@ -112,6 +112,17 @@ int htg_open_cell(htg_map* map, int32_t r, int32_t c);
  * of range or already impassable are skipped; last_batch_stats is untouched.
  * Not supported inside an htg_server session (its maps are reveal-only). */
 int htg_block_cells(htg_map* map, const int32_t* rc, int32_t n);
+/* MinPlusOracle::liftRoot (src/minplus_oracle.cpp:2008-2114): the hierarchy's
+ * square grows from S to g*S and gains a root level.  rows x cols cell codes
+ * give the grown grid (it contains the old grid; rows, cols <= g*S).  Leaves
+ * and nodes inside the old square keep their tables, and the map keeps its own
+ * cell state there (cells of the old square outside the old grid stay
+ * impassable until opened); the other leaves and nodes and the new root are
+ * computed from the given cells.  On success *map is replaced by the grown
+ * map's handle and the old handle is destroyed (incremental states taken on it
+ * recompute, as the reference's epoch bump forces); on failure *map is kept.
+ * Not for maps inside an htg_server session. */
+int htg_lift_root(htg_map** map, int32_t rows, int32_t cols, const uint8_t* cells);
 /* MinPlusOracle::queryAllPairs (src/minplus_oracle.cpp:934-1426).  rc holds k
  * (row, col) viewpoints; out receives k*k distances, row-major, host memory.
  * Impassable or out-of-range viewpoints give an INF row and column, INF

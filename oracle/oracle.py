@@ -90,6 +90,7 @@ def _lib(kind: str):
         api["time_queries"] = fn("time_queries", C.c_double, vp, _i32p, _i32p, ci, ci, C.c_void_p)
         api["time_lifts"] = fn("time_lifts", C.c_double, vp, _i32p, ci)
         api["load_movingai"] = fn("load_movingai", ci, C.c_char_p, _i32p, _i32p, _u8p, C.c_int64)
+        api["lift_root"] = fn("lift_root", None, vp, ci, ci, _u8p)
         api["inc_create"] = fn("inc_create", vp)
         api["inc_destroy"] = fn("inc_destroy", None, vp)
         api["inc_query"] = fn("inc_query", C.c_double, vp, vp, _i32p, ci, _u16p)
@@ -223,6 +224,11 @@ class Oracle:
         X = [x[o[l]:o[l + 1]].copy() for l in range(H + 1)]
         A = [act[o[l]:o[l + 1]][act[o[l]:o[l + 1]] >= 0].copy() for l in range(H + 1)]
         return X, par, quad, off, A
+
+    def lift_root(self, cells: np.ndarray) -> None:
+        """liftRoot over the grown grid (compiled reference only)."""
+        self.cells = np.ascontiguousarray(cells, dtype=np.uint8).copy()
+        self.api["lift_root"](self.h, self.cells.shape[0], self.cells.shape[1], self.cells)
 
     def time_lifts(self, rc) -> float:
         """Seconds for liftPoint on each listed cell (compiled reference only)."""

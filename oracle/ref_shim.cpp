@@ -190,6 +190,13 @@ double ref_time_queries(void* p, const int32_t* rc, const int32_t* ks, int nq, i
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// MinPlusOracle::liftRoot on a grown grid given as rows x cols observed codes.
+void ref_lift_root(void* p, int rows, int cols, const uint8_t* cells) {
+    auto* h = static_cast<RefHandle*>(p);
+    h->grid = GridMap(rows, cols, CellState::Unknown);
+    for (size_t i = 0; i < size_t(rows) * cols; ++i) h->grid.observed[i] = static_cast<CellState>(cells[i]);
+    h->oracle.liftRoot(h->grid);
+}
 // GridMap::loadMovingAI: 1 on success with rows/cols set and, when they fit
 // in cap cells, the ground truth codes written to out; 0 where it returns false.
 int ref_load_movingai(const char* path, int32_t* rows, int32_t* cols, uint8_t* out, int64_t cap) {

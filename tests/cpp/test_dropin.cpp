@@ -2,7 +2,7 @@
 // compiled against htrip_amd/include/htrip/*.hpp and linked with
 // libhtrip_gpu.so.  Mirrors suites of the reference's apps/test_exactness.cpp
 // (known answers 1 and 37, blocked viewpoint INF, batch openings, copy
-// semantics, incremental refresh, reopened viewpoint, liftPoint) and checks
+// semantics, incremental refresh, reopened viewpoint, liftPoint, liftRoot) and checks
 // every matrix against a BFS written here.
 // Exit status 0 = all passed.
 #include <cstdio>
@@ -252,6 +252,50 @@ int main() {
             for (size_t i = 1; i < lv.active[l].size(); ++i)
                 ok = ok && lv.X[l][size_t(lv.active[l][i - 1])] <= lv.X[l][size_t(lv.active[l][i])];
         expect(ok, "liftPoint levels 0 and H == BFS, active sorted");
+    }
+    {  // Suite 3: liftRoot 16x16 (H=1) -> 32x32 and 8x8 (H=0) -> 16x16 (:309-366)
+        GridMap g16(16, 16), g32(32, 32), g8(8, 8);
+        for (int r = 0; r < 32; ++r)
+            for (int c = 0; c < 32; ++c) {
+                g32.openDoorway({r, c});
+                if (r < 16 && c < 16) g16.openDoorway({r, c});
+                if (r < 8 && c < 8) g8.openDoorway({r, c});
+            }
+        HierarchyConfig cfg;
+        cfg.tile_size_b = 8;
+        MinPlusOracle o(cfg);
+        o.build(g16);
+        o.liftRoot(g32);
+        expect(o.S == 32 && o.H == 2 && matches_bfs(o, g32, {{2, 2}, {5, 5}, {2, 20}, {20, 20}, {25, 10}}),
+               "suite 3 liftRoot 16 -> 32 == BFS");
+        MinPlusOracle o0(cfg);
+        o0.build(g8);
+        const bool h0 = o0.depth() == 0;
+        o0.liftRoot(g16);
+        expect(h0 && o0.depth() == 1 && matches_bfs(o0, g16, {{1, 1}, {3, 3}, {6, 6}, {10, 10}, {14, 14}}),
+               "suite 3 liftRoot from H = 0 == BFS");
+    }
+    {  // Suite 16.4: liftRoot after batch openings (:1165-1208)
+        GridMap g(64, 64);
+        for (int r = 0; r < 64; ++r)
+            for (int c = 0; c < 64; ++c) g.setObserved(r, c, r == 0 ? CellState::Obstacle : CellState::Free);
+        HierarchyConfig cfg;
+        cfg.tile_size_b = 8;
+        MinPlusOracle o(cfg);
+        o.build(g);
+        std::vector<Point> row0;
+        for (int c = 0; c < 64; ++c) {
+            g.setObserved(0, c, CellState::Free);
+            row0.push_back({0, c});
+        }
+        o.onBatchCellsOpened(row0, g);
+        GridMap g128(128, 128);
+        for (int r = 0; r < 128; ++r)
+            for (int c = 0; c < 128; ++c)
+                g128.setObserved(r, c, (r < 64 && c < 64 && !g.isPassable({r, c})) ? CellState::Obstacle : CellState::Free);
+        o.liftRoot(g128);
+        const dist_t d = o.queryDistance({0, 0}, {0, 100}, g128);
+        expect(d == bfs_all_pairs(g128, {{0, 0}, {0, 100}})[1] && d == 100, "suite 16.4 liftRoot after a batch, d = 100");
     }
     std::printf("%s (%d failure%s)\n", failures ? "FAILED" : "ALL PASSED", failures, failures == 1 ? "" : "s");
     return failures ? 1 : 0;

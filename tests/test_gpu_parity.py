@@ -323,6 +323,60 @@ def test_incremental_query_equals_the_full_query(gpu_ctx, kind, n, b, seed):
     st.close()
 
 
+@pytest.mark.parametrize("kind,n,b,g,seed", [("street", 64, 8, 2, 71), ("street", 8, 8, 2, 72), ("maze", 64, 8, 2, 73),
+                                              ("street", 16, 4, 4, 74), ("street", 64, 16, 2, 75)])
+def test_lift_root_equals_a_fresh_build(gpu_ctx, kind, n, b, g, seed):
+    """liftRoot (src/minplus_oracle.cpp:2008-2114): after reveals, the square
+    grows by g with one more level; every table equals the oracle's fresh
+    build of the grown grid, queries (across the old border too) equal BFS,
+    incremental states recompute, and reveals, lifts and clones afterwards
+    stay exact."""
+    print(f"lift root {kind} n={n} b={b} g={g} seed={seed}", flush=True)
+    truth = make(kind, n, seed)
+    obs = grids.disk_observed(truth, [tuple(grids.sample_free(truth, 1, seed)[0])], max(3, n // 2))
+    G = GpuMap(obs, b, g)
+    assert G.build()
+    order = grids.frontier_order(truth, obs)
+    G.open_cells(order[:len(order) // 2])
+    S0, H0 = G.S, G.H
+    assert S0 == n
+    st = IncState()
+    vps0 = grids.sample_free(G.cells, 12, seed)
+    G.query_incremental(st, vps0)
+    S1 = S0 * g
+    big_truth = make(kind, S1, seed + 1)
+    big_truth[:n, :n] = truth
+    big_obs = big_truth.copy()
+    big_obs[:n, :n] = G.cells
+    big_obs[n:n + max(2, n // 4), :] = np.where(big_obs[n:n + max(2, n // 4), :] == 1, 0, big_obs[n:n + max(2, n // 4), :])
+    G.lift_root(big_obs)
+    assert G.S == S1 and G.H == H0 + 1
+    np.testing.assert_array_equal(G.cells, big_obs)
+    O = Oracle("port", G.cells, b, g)
+    assert O.build()
+    assert_tables_equal(G, O)
+    vps = np.vstack([grids.sample_free(G.cells[:n, :n], 8, seed + 2), grids.sample_free(G.cells, 24, seed + 3)])
+    np.testing.assert_array_equal(G.query(vps), grid_bfs(G.cells, vps))
+    m, nr = G.query_incremental(st, vps0)
+    assert nr == len(vps0) and np.array_equal(m, grid_bfs(G.cells, vps0))
+    # reveals afterwards: the unknown band and the rest of the old frontier
+    band = np.argwhere((G.cells == 0) & (big_truth == 1)).astype(np.int32)
+    G.open_cells(np.vstack([band, order[len(order) // 2:]]).astype(np.int32))
+    O = Oracle("port", G.cells, b, g)
+    assert O.build()
+    assert_tables_equal(G, O)
+    np.testing.assert_array_equal(G.query(vps), grid_bfs(G.cells, vps))
+    r, c = (int(v) for v in grids.sample_free(G.cells, 1, seed + 4)[0])
+    X, par, quad, off = G.lift_point(r, c)
+    Xo, po, qo, oo, _ = O.lift_point(r, c)
+    for lvl in range(G.H + 1):
+        np.testing.assert_array_equal(X[lvl], Xo[lvl])
+    C2 = G.clone()
+    assert C2.S == S1
+    np.testing.assert_array_equal(C2.query(vps), grid_bfs(G.cells, vps))
+    st.close()
+
+
 def test_cpp_dropin_program(gpu_ctx):
     """The reference-API C++ program (tests/cpp/test_dropin.cpp) against the GPU library."""
     import os

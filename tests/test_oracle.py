@@ -227,6 +227,37 @@ def test_lift_point_matches_compiled_reference(seed):
         R.open_cells(batch)
 
 
+def grown(cells, S1, fill):
+    """cells placed in the top-left corner of an S1 x S1 grid of ``fill``."""
+    out = np.full((S1, S1), fill, np.uint8)
+    out[:cells.shape[0], :cells.shape[1]] = cells
+    return out
+
+
+@pytest.mark.skipif(not orc.available("reference"), reason="oracle/_ref not built (reference sources absent)")
+@pytest.mark.parametrize("kind,n,b,g", [("street", 32, 8, 2), ("street", 8, 8, 2), ("maze", 64, 8, 2),
+                                        ("street", 16, 4, 4)])
+def test_reference_lift_root_equals_a_fresh_build(kind, n, b, g):
+    """liftRoot (src/minplus_oracle.cpp:2008-2114) keeps the old square's tables
+    and computes the rest from the grid: on a grid that fills its square (the
+    reference's own use, apps/test_exactness.cpp:309-366, :1165-1208) every
+    table equals the oracle's fresh build of the grown grid, which pins the
+    library's lifted tables to the reference's."""
+    truth = (grids.street_grid(n, 41) if kind == "street" else grids.maze_grid(n, 41))
+    R = Oracle("reference", truth, b, g)
+    R.build()
+    R.open_cells(np.argwhere(truth == 1)[:5])  # updates before the lift
+    big = grown(R.cells, n * g, 1)
+    big[n:, :n:3] = 2  # some walls in the new area
+    R.lift_root(big)
+    assert R.S == n * g
+    P = Oracle("port", big, b, g)
+    P.build()
+    assert digests(P) == digests(R)
+    vps = grids.sample_free(big, 24, 3)
+    np.testing.assert_array_equal(R.query(vps), grid_bfs(big, vps))
+
+
 def test_open_grid_is_manhattan():
     cells = np.ones((48, 48), np.uint8)
     o = Oracle("port", cells, 8, 2)
