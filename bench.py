@@ -729,6 +729,62 @@ def run_incremental(args):
             "recomputed": recomputed, "cpu_baseline": cpu}
 
 
+def run_liftroot(args):
+    """liftRoot: a 256^2 corner of the 512^2 street-like grid (seed 3), fully
+    revealed, b=16, grows to the whole 512^2 grid (S 256 -> 512, H 4 -> 5).
+    GPU: htg_lift_root (new map, kept tables copied, 3/4 of the leaves and
+    nodes and the new root built); reference: its liftRoot on its own build of
+    the same corner.  Every lifted map must answer queries as BFS does."""
+    from htrip_amd import _lib, grids
+    from oracle import oracle as orc
+
+    big = grids.street_grid(512, SEED)
+    small = big[:256, :256].copy()
+    vps = grids.sample_free(big, 32, 9)
+    expect = orc.grid_bfs(big, vps)
+    ts = []
+    for rep in range(args.warmup + 5):
+        gmap = _lib.GpuMap(small, 16, 2)
+        gmap.build()
+        gmap.sync()
+        t0 = time.perf_counter()
+        gmap.lift_root(big)
+        t1 = time.perf_counter()
+        if not np.array_equal(gmap.query(vps), expect):
+            raise SystemExit("lifted GPU map differs from BFS")
+        if rep >= args.warmup:
+            ts.append(t1 - t0)
+        gmap.close()
+    tc = []  # a new map of the grown grid, created and built (the lift allocates its map too)
+    for _ in range(5):
+        t0 = time.perf_counter()
+        fresh = _lib.GpuMap(big, 16, 2)
+        fresh.build()
+        tc.append(time.perf_counter() - t0)
+        fresh.close()
+    fresh_ms = 1e3 * float(np.median(tc))
+    gpu_ms = 1e3 * float(np.median(ts))
+    log(f"liftroot: GPU {gpu_ms:.2f} ms (creating and building the grown map instead: {fresh_ms:.2f} ms)")
+    cpu = None
+    if not args.no_cpu and orc.available("reference"):
+        R = orc.Oracle("reference", small, 16, 2)
+        assert R.build()
+        t0 = time.perf_counter()
+        R.lift_root(big)
+        t_ref = time.perf_counter() - t0
+        if not np.array_equal(R.query(vps), expect):
+            raise SystemExit("reference lifted map differs from BFS")
+        log(f"liftroot: reference {1e3 * t_ref:.1f} ms")
+        cpu = {"value": 1e3 * t_ref, "unit": "ms", "cores": 1, "kind": "reference",
+               "sample": "one liftRoot 256^2 -> 512^2 (1 thread), queries equal BFS"}
+    return {"metric": "liftRoot latency, 256^2 -> 512^2 street grid, b=16", "value": gpu_ms, "unit": "ms",
+            "n_gpus": 1, "steps": len(ts), "warmup": args.warmup, "ms_per_step": gpu_ms, "higher_is_better": False,
+            "scaling": "weak", "vs_baseline": None, "dtype": "u16", "data": "synthetic",
+            "config": {"workload": "512x512 street-like grid seed 3 (its 256x256 corner built first), b=16, g=2x2",
+                       "mode": "liftroot"},
+            "fresh_create_build_ms": fresh_ms, "cpu_baseline": cpu}
+
+
 def run_build(args):
     """512^2 DFS maze (seed 2, corridor and wall width 1), b=16: the batched
     build (every node of a level per launch, the whole build as one captured
@@ -948,7 +1004,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--mode", default="served",
                     choices=["plain", "served", "update", "small", "build", "reveal", "multi", "kernel", "lift",
-                             "incremental"])
+                             "incremental", "liftroot"])
     ap.add_argument("--k", type=int, default=64)
     ap.add_argument("--map", default="", help="plain mode: a shipped MovingAI map (tests/golden/maps) instead of "
                                               "the synthetic grid, e.g. Boston_0_1024")
@@ -988,7 +1044,7 @@ def main():
         sys.exit(2)
     line = {"plain": run_plain, "served": run_served, "update": run_update, "small": run_small, "build": run_build,
             "reveal": run_reveal, "multi": run_multi, "kernel": run_kernel, "lift": run_lift,
-            "incremental": run_incremental}[args.mode](args)
+            "incremental": run_incremental, "liftroot": run_liftroot}[args.mode](args)
     print(json.dumps(line))
 
 
