@@ -91,6 +91,7 @@ def _lib(kind: str):
         api["time_lifts"] = fn("time_lifts", C.c_double, vp, _i32p, ci)
         api["load_movingai"] = fn("load_movingai", ci, C.c_char_p, _i32p, _i32p, _u8p, C.c_int64)
         api["lift_root"] = fn("lift_root", None, vp, ci, ci, _u8p)
+        api["frontiers"] = fn("frontiers", ci, ci, ci, _u8p, ci, ci, _i32p, ci, _i32p, ci, _u8p, _u8p)
         api["inc_create"] = fn("inc_create", vp)
         api["inc_destroy"] = fn("inc_destroy", None, vp)
         api["inc_query"] = fn("inc_query", C.c_double, vp, vp, _i32p, ci, _u16p)
@@ -108,6 +109,26 @@ def reference_load_movingai(path: str):
     out = np.empty(int(rows[0]) * int(cols[0]), dtype=np.uint8)
     api["load_movingai"](path.encode(), rows, cols, out, out.size)
     return out.reshape(int(rows[0]), int(cols[0]))
+
+
+def reference_frontiers(obs: np.ndarray, min_size: int = 2, max_size: int = 16):
+    """The compiled reference's FrontierManager on an observed grid:
+    (clearance, frontier flags, [(id, (r, c), cells)])."""
+    api = _lib("reference")
+    obs = np.ascontiguousarray(obs, dtype=np.uint8)
+    R, Cn = obs.shape
+    clr = np.empty_like(obs)
+    front = np.empty_like(obs)
+    cap = R * Cn
+    meta = np.empty(4 * cap, dtype=np.int32)
+    cells = np.empty(2 * cap, dtype=np.int32)
+    n = api["frontiers"](R, Cn, obs, min_size, max_size, meta, cap, cells, cap, clr, front)
+    out, w = [], 0
+    for i in range(n):
+        cid, r, c, sz = (int(v) for v in meta[4 * i:4 * i + 4])
+        out.append((cid, (r, c), [tuple(int(v) for v in cells[2 * j:2 * j + 2]) for j in range(w, w + sz)]))
+        w += sz
+    return clr, front.astype(bool), out
 
 
 def grid_bfs(cells: np.ndarray, vps: np.ndarray) -> np.ndarray:

@@ -2,7 +2,7 @@
 //
 // oracle/Makefile compiles this file together with the reference's own
 // sources (src/grid_map.cpp, leaf_tile.cpp, internal_node.cpp,
-// minplus_oracle.cpp, bfs_oracle.cpp, dijkstra_oracle.cpp, read in place
+// minplus_oracle.cpp, bfs_oracle.cpp, dijkstra_oracle.cpp, frontier.cpp, read in place
 // from the reference checkout) into oracle/_ref/libhtrip_ref.so.  The result
 // is the reference's CPU AVX2 path itself, used by tests/ to pin the
 // restatement in htrip_oracle.cpp and the GPU tables, and by bench.py as the
@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "htrip/bfs_oracle.hpp"
+#include "htrip/frontier.hpp"
 #include "htrip/grid_map.hpp"
 #include "htrip/minplus_oracle.hpp"
 
@@ -190,6 +191,36 @@ double ref_time_queries(void* p, const int32_t* rc, const int32_t* ks, int nq, i
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// FrontierManager::detectAndCluster on an observed grid.  Returns the number
+// of clusters; the first cap_clusters get (id, representative r, c, size) in
+// meta and their cells back to back in cells (up to cap_cells pairs).  clr
+// (rows*cols) receives GridMap::getClearance and front the frontier flags.
+int ref_frontiers(int rows, int cols, const uint8_t* obs, int min_size, int max_size, int32_t* meta,
+                  int cap_clusters, int32_t* cells, int cap_cells, uint8_t* clr, uint8_t* front) {
+    GridMap g(rows, cols, CellState::Unknown);
+    for (size_t i = 0; i < size_t(rows) * cols; ++i) g.observed[i] = static_cast<CellState>(obs[i]);
+    for (int r = 0; r < rows; ++r)
+        for (int c = 0; c < cols; ++c) {
+            clr[size_t(r) * cols + c] = g.getClearance(r, c);
+            front[size_t(r) * cols + c] = FrontierManager::isFrontierCell(g, Point{r, c}) ? 1 : 0;
+        }
+    FrontierManager fm(min_size, max_size);
+    const auto cl = fm.detectAndCluster(g);
+    int w = 0;
+    for (size_t i = 0; i < cl.size() && int(i) < cap_clusters; ++i) {
+        meta[4 * i] = cl[i].id;
+        meta[4 * i + 1] = cl[i].representative.r;
+        meta[4 * i + 2] = cl[i].representative.c;
+        meta[4 * i + 3] = cl[i].size();
+        for (const Point& p : cl[i].cells)
+            if (w < cap_cells) {
+                cells[2 * w] = p.r;
+                cells[2 * w + 1] = p.c;
+                ++w;
+            }
+    }
+    return int(cl.size());
+}
 // MinPlusOracle::liftRoot on a grown grid given as rows x cols observed codes.
 void ref_lift_root(void* p, int rows, int cols, const uint8_t* cells) {
     auto* h = static_cast<RefHandle*>(p);

@@ -7,7 +7,7 @@ import os
 import numpy as np
 import pytest
 
-from htrip_amd import grids
+from htrip_amd import frontier, grids
 from htrip_amd._lib import GpuMap, IncState
 from oracle.oracle import INF, Oracle, grid_bfs
 from tests.test_gpu_parity import assert_tables_equal
@@ -60,8 +60,9 @@ def test_fully_revealed_map(gpu_ctx, name, b):
 @pytest.mark.parametrize("name,b", [("Berlin_0_1024", 32), ("maze512-1-0", 16)])
 def test_frontier_reveal_on_a_shipped_map(gpu_ctx, name, b):
     """Robots' disks revealed, then frontier batches: after every batch the
-    matrix equals BFS on the observed map, and the incremental refresh equals
-    the full query."""
+    matrix equals BFS on the observed map, the incremental refresh equals the
+    full query, and so does a query over the robots and the frontier
+    viewpoints (htrip_amd.frontier)."""
     print(f"reveal {name} b={b}", flush=True)
     truth = load(name)
     comp = grids._largest_component(truth == 1)
@@ -78,4 +79,8 @@ def test_frontier_reveal_on_a_shipped_map(gpu_ctx, name, b):
         np.testing.assert_array_equal(full, grid_bfs(G.cells, vps), err_msg=f"step {step}")
         inc, _ = G.query_incremental(st, vps)
         np.testing.assert_array_equal(inc, full)
+        # an explorer's query: the robots and the frontier clusters' viewpoints
+        fv = frontier.frontier_viewpoints(G.cells, starts, 64)
+        assert len(fv) > len(starts)
+        np.testing.assert_array_equal(G.query(fv), grid_bfs(G.cells, fv), err_msg=f"frontier step {step}")
     st.close()
