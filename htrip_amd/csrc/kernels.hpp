@@ -176,10 +176,11 @@ constexpr int kLiftRowsPerChunk = 64;
 void launch_lift_level(const uint16_t* D, int ld, int row0, int pc, int rows_per_chunk, int nchunks,
                        const uint16_t* X, uint16_t* part, const int* ext_child, const int* ext_cport, int ne,
                        uint16_t* Xo, hipStream_t st);
-// Meet tiles of every level (8 ints each: group, q2, i0, j0, j_end, level):
+// Meet tiles of every level (8 ints each: group size, q2, i0, j0, j_end,
+// level, the group's offset in glist, its query):
 // out[i][j] = min(out, min over the q2 seam block of Z_level[i] (+) Z_level[j]).
-bool launch_q_meet(const int* tiles, int n_tiles, const int4* groups, const int* glist, const int* zgeo, int g2,
-                   const int* slot_i, const int* q_k, const long long* q_out, const uint16_t* Z, uint16_t* out,
-                   hipStream_t st, KTimer kt = {});
+bool launch_q_meet(const int* tiles, int n_tiles, const int* glist, const int* zgeo, int g2, const int* slot_i,
+                   const int* q_k, const long long* q_out, const uint16_t* Z, uint16_t* out, hipStream_t st,
+                   KTimer kt = {});
 
 }  // namespace htg

@@ -94,9 +94,13 @@ __global__ __launch_bounds__(256) void k_q_front(int nq, int ib, int leaf_blocks
         X0[gid] = leafD[leaf * n * n + size_t((rc.x % b) * b + rc.y % b) * n + pr * b + pcc];
         return;
     }
+    // the fill range of this block: ranges are in order of their first block,
+    // so r counts the ranges after the first that start at or before fb (one
+    // load per lane and a ballot instead of a walk of dependent loads)
     const int fb = lb - leaf_blocks;
-    int r = 0;
-    while (r + 1 < n_fill && fills[r + 1].w <= fb) ++r;
+    const int lane = int(threadIdx.x & 63u);
+    const bool starts_before = lane >= 1 && lane < n_fill && fills[lane].w <= fb;
+    const int r = __popcll(__ballot(starts_before));
     const int4 f = fills[r];
     const long long u = (long long)(fb - f.w) * 256 + threadIdx.x;
     if (u >= f.z) return;
@@ -112,7 +116,7 @@ void launch_q_front(int nq, const int* q_k, const long long* q_out, const int* q
     const int ib = int(((long long)kmax * kmax + 255) / 256);
     const int leaf_blocks = int(((long long)n_slots * (4 * b - 4) + 255) / 256);
     const long long blocks = (long long)nq * ib + leaf_blocks + fill_blocks;
-    if (blocks == 0) return;
+    if (blocks == 0 || n_fill > 64) return;  // at most 64 fill ranges (2 per level)
     hipLaunchKernelGGL(k_q_front, dim3(unsigned(blocks)), dim3(256), 0, st, nq, ib, leaf_blocks, q_k, q_out, q_vp,
                        vp_rc, vp_valid, n_slots, slot_rc, b, nl, leafD, out, X0, n_fill, fills, fill_base, wstat);
 }
@@ -164,18 +168,46 @@ __global__ __launch_bounds__(256) void k_q_z(
     // so one 8-byte load per seam row feeds the lane's 4 columns.
     const int base = cols[c0];
     const uint16_t* D = DU + size_t(W) * node_stride + base;
+    // The epilogue's metadata (Z positions, exterior ports, the slot's own
+    // X at those ports), loaded now by the wavefront that writes the result:
+    // its dependent loads overlap the row loads instead of following them.
+    int pos[4] = {-1, -1, -1, -1}, ext[4] = {-1, -1, -1, -1};
+    uint16_t ownx[M][4];
+#pragma unroll
+    for (int v = 0; v < M; ++v)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) ownx[v][k] = kInf;
+    if (ksplit == 1 && w == 0 && active) {
+        const int4 cp = *reinterpret_cast<const int4*>(colpos + c0);
+        const int4 ce = *reinterpret_cast<const int4*>(colext + c0);
+        pos[0] = cp.x, pos[1] = cp.y, pos[2] = cp.z, pos[3] = cp.w;
+        ext[0] = ce.x, ext[1] = ce.y, ext[2] = ce.z, ext[3] = ce.w;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int own = (ext[k] >= 0 && ext_child[ext[k]] == q) ? ext_cport[ext[k]] : -1;
+            if (own < 0) continue;
+#pragma unroll
+            for (int v = 0; v < M; ++v)
+                if (v < m) ownx[v][k] = X[size_t(s0 + v) * pc + own];
+        }
+    }
     const int* seam = seam_list + it2.y;  // child q's seam ports
     const int ns = it2.z - it2.y;
     const int a_lo = int((long long)ns * blockIdx.z / ksplit), a_hi = int((long long)ns * (blockIdx.z + 1) / ksplit);
     const unsigned pad_off = unsigned(q * pc + it2.w) * unsigned(ld);
-    const int rounds = (a_hi - a_lo + NW * ACH - 1) / (NW * ACH);  // the same for every wavefront
+    // rows per wavefront and round: the item's rows spread over all NW
+    // wavefronts (a multiple of PF, at most ACH), so that a short seam list
+    // costs each wavefront few dependent D-row round trips
+    const int tot = a_hi - a_lo;
+    const int chunk = min(ACH, max(PF, ((tot + NW - 1) / NW + PF - 1) / PF * PF));
+    const int rounds = (tot + NW * chunk - 1) / (NW * chunk);  // the same for every wavefront
     int rows_done = 0;
     uint32_t acc[M][2];
 #pragma unroll
     for (int v = 0; v < M; ++v) acc[v][0] = acc[v][1] = kInf2;
     for (int r = 0; r < rounds; ++r) {
-        const int a0 = a_lo + (r * NW + w) * ACH;
-        const int na = max(0, min(ACH, a_hi - a0));
+        const int a0 = a_lo + (r * NW + w) * chunk;
+        const int na = max(0, min(chunk, a_hi - a0));
         const int sr = tid < na ? seam[a0 + tid] : seam[0];
         uint32_t xv[M];
         bool fin = false;
@@ -278,12 +310,6 @@ __global__ __launch_bounds__(256) void k_q_z(
             if (v < m) *reinterpret_cast<uint2*>(P + size_t(s0 + v) * ldp + c0) = make_uint2(acc[v][0], acc[v][1]);
         return;
     }
-    const int4 cp = *reinterpret_cast<const int4*>(colpos + c0);
-    const int4 ce = *reinterpret_cast<const int4*>(colext + c0);
-    const int pos[4] = {cp.x, cp.y, cp.z, cp.w}, ext[4] = {ce.x, ce.y, ce.z, ce.w};
-    int own[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) own[k] = (ext[k] >= 0 && ext_child[ext[k]] == q) ? ext_cport[ext[k]] : -1;
 #pragma unroll
     for (int v = 0; v < M; ++v) {
         if (v >= m) continue;
@@ -293,14 +319,7 @@ __global__ __launch_bounds__(256) void k_q_z(
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             if (pos[k] >= 0) Z[sl * ldz + pos[k]] = val[k];
-            if (ext[k] >= 0) {
-                uint16_t x = val[k];
-                if (own[k] >= 0) {
-                    const uint16_t d = X[sl * pc + own[k]];
-                    x = d < x ? d : x;
-                }
-                Xn[sl * ne + ext[k]] = x;
-            }
+            if (ext[k] >= 0) Xn[sl * ne + ext[k]] = ownx[v][k] < val[k] ? ownx[v][k] : val[k];
         }
     }
 }
@@ -412,27 +431,36 @@ __device__ __forceinline__ void atomic_min_u16(uint16_t* o, uint16_t v) {
     }
 }
 
-__global__ __launch_bounds__(256) void k_q_meet(const int* __restrict__ tiles, const int4* __restrict__ groups,
-                                                const int* __restrict__ glist, const int* __restrict__ zgeo, int g2,
-                                                const int* __restrict__ slot_i, const int* __restrict__ q_k,
-                                                const long long* __restrict__ q_out, const uint16_t* __restrict__ Zall,
-                                                uint16_t* __restrict__ out) {
+__global__ __launch_bounds__(256) void k_q_meet(const int* __restrict__ tiles, const int* __restrict__ glist,
+                                                const int* __restrict__ zgeo, int g2, const int* __restrict__ slot_i,
+                                                const int* __restrict__ q_k, const long long* __restrict__ q_out,
+                                                const uint16_t* __restrict__ Zall, uint16_t* __restrict__ out) {
     constexpr int CH = 256;      // packed pairs per round (512 columns)
     constexpr int LDR = CH + 4;  // row stride: the 8 B rows a wavefront reads fall in distinct banks
     __shared__ uint32_t A[8][LDR];
     __shared__ uint32_t Bm[8][LDR];
-    const int* t = tiles + size_t(blockIdx.x) * 8;
-    const int gi = t[0], q2 = t[1], i0 = t[2], j0 = t[3], jend = t[4], lvl = t[5];
+    // tile: group size, target child q2, first rows i0 / j0, j end, level,
+    // the group's offset in glist, its query
+    const int4 t0 = *reinterpret_cast<const int4*>(tiles + size_t(blockIdx.x) * 8);
+    const int4 t1 = *reinterpret_cast<const int4*>(tiles + size_t(blockIdx.x) * 8 + 4);
+    const int n = t0.x, q2 = t0.y, i0 = t0.z, j0 = t0.w, jend = t1.x, lvl = t1.y, Q = t1.w;
+    const int* lst = glist + t1.z;
     const int* zg = zgeo + size_t(lvl) * (4 + 2 * g2);
-    const uint16_t* Z = Zall + ((size_t(unsigned(zg[1])) << 32) | unsigned(zg[0]));
-    const int ldz = zg[2];
-    const int4 grp = groups[gi];
-    const int* lst = glist + grp.z;
-    const int n = grp.w;
     const int tid = threadIdx.x;
     const int sub = tid & 3, pair = tid >> 2, pi = pair >> 3, pj = pair & 7;
-    const int off = zg[4 + q2], len2 = zg[4 + g2 + q2] / 2;  // block length is even
     const int ni = min(8, n - i0), nj = min(8, jend - j0);
+    // the result's address, resolved now (its dependent loads overlap the row loads)
+    const bool writer = sub == 0 && pi < ni && pj < nj;
+    int si = 0, sj = 0;
+    size_t oaddr = 0;
+    if (writer) {
+        si = lst[i0 + pi];
+        sj = lst[j0 + pj];
+        oaddr = size_t(q_out[Q]) + size_t(slot_i[si]) * q_k[Q] + slot_i[sj];
+    }
+    const uint16_t* Z = Zall + ((size_t(unsigned(zg[1])) << 32) | unsigned(zg[0]));
+    const int ldz = zg[2];
+    const int off = zg[4 + q2], len2 = zg[4 + g2 + q2] / 2;  // block length is even
     // staging: thread (row, l16) loads columns l16, l16 + 16, ... of one of
     // the 16 rows (8 of A, then 8 of B)
     const int row = tid >> 4, l16 = tid & 15;
@@ -463,21 +491,18 @@ __global__ __launch_bounds__(256) void k_q_meet(const int* __restrict__ tiles, c
     uint32_t acc = qmin2(acc0, acc1);
     acc = qmin2(acc, uint32_t(__shfl_xor(int(acc), 1)));
     acc = qmin2(acc, uint32_t(__shfl_xor(int(acc), 2)));
-    if (sub != 0 || pi >= ni || pj >= nj) return;
-    const int si = lst[i0 + pi], sj = lst[j0 + pj];
-    if (si == sj) return;
+    if (!writer || si == sj) return;
     const uint16_t best = min(uint16_t(acc & 0xffffu), uint16_t(acc >> 16));
     if (best >= kInf) return;
-    const int Q = grp.x;
-    atomic_min_u16(out + q_out[Q] + size_t(slot_i[si]) * q_k[Q] + slot_i[sj], best);
+    atomic_min_u16(out + oaddr, best);
 }
 
-bool launch_q_meet(const int* tiles, int n_tiles, const int4* groups, const int* glist, const int* zgeo, int g2,
-                   const int* slot_i, const int* q_k, const long long* q_out, const uint16_t* Z, uint16_t* out,
-                   hipStream_t st, KTimer kt) {
+bool launch_q_meet(const int* tiles, int n_tiles, const int* glist, const int* zgeo, int g2, const int* slot_i,
+                   const int* q_k, const long long* q_out, const uint16_t* Z, uint16_t* out, hipStream_t st,
+                   KTimer kt) {
     if (n_tiles == 0) return false;
-    HTG_LAUNCH_T(k_q_meet, dim3(unsigned(n_tiles)), dim3(256), 0, st, kt, tiles, groups, glist,
-                          zgeo, g2, slot_i, q_k, q_out, Z, out);
+    HTG_LAUNCH_T(k_q_meet, dim3(unsigned(n_tiles)), dim3(256), 0, st, kt, tiles, glist, zgeo, g2, slot_i, q_k,
+                 q_out, Z, out);
     return true;
 }
 

@@ -1600,8 +1600,7 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
             while (a1 < ns && sk[size_t(by_q[size_t(a1)])].q == q && kw[size_t(by_q[size_t(a1)])] == w) ++a1;
             const int n = a1 - a0;
             if (n >= 2) {
-                const int gi = int(P.groups.size());
-                P.groups.push_back(make_int4(q, 0, int(P.glist.size()), n));
+                const int goff = int(P.glist.size());
                 P.glist.insert(P.glist.end(), by_q.begin() + a0, by_q.begin() + a1);
                 for (int j0 = 0; j0 < n;) {
                     const int q2 = qdig(by_q[size_t(a0 + j0)]);
@@ -1612,8 +1611,8 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
                             const size_t o = P.mtiles.size();
                             P.mtiles.resize(o + 8);
                             int* tt = P.mtiles.data() + o;
-                            tt[0] = gi, tt[1] = q2, tt[2] = i0, tt[3] = jj, tt[4] = je, tt[5] = l - 1;
-                            tt[6] = tt[7] = 0;
+                            tt[0] = n, tt[1] = q2, tt[2] = i0, tt[3] = jj, tt[4] = je, tt[5] = l - 1;
+                            tt[6] = goff, tt[7] = q;
                         }
                     if (counting) {  // the meet's work figures, for kernel timing only
                         const double zl = L.zlen[size_t(q2)];
@@ -1640,7 +1639,7 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
     Packer pk;
     const size_t o_qk = pk.add(P.q_k), o_qout = pk.add(P.q_out), o_qvp = pk.add(P.q_vp), o_vprc = pk.add(P.vp_rc),
                  o_vpvalid = pk.add(P.vp_valid), o_src = pk.add(P.slot_rc), o_si = pk.add(P.slot_i),
-                 o_tiles = pk.add(P.mtiles), o_groups = pk.add(P.groups), o_glist = pk.add(P.glist),
+                 o_tiles = pk.add(P.mtiles), o_glist = pk.add(P.glist),
                  o_zgeo = pk.add(P.zgeo), o_fills = pk.add(P.fills);
     std::vector<size_t> o_items(H);
     size_t maxpart = 0;
@@ -1715,8 +1714,7 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
     if (ns > 0 && !P.mtiles.empty()) {
         KTimer kt;
         const int pm = ws.counted ? prof_ref().open(HTG_K_Q_MEET, P.m_bytes, P.m_updates, kt, st) : -1;
-        const bool lm = launch_q_meet(I(o_tiles), int(P.mtiles.size() / 8), reinterpret_cast<const int4*>(dp + o_groups),
-                                      I(o_glist), I(o_zgeo), g2, I(o_si), I(o_qk),
+        const bool lm = launch_q_meet(I(o_tiles), int(P.mtiles.size() / 8), I(o_glist), I(o_zgeo), g2, I(o_si), I(o_qk),
                                       reinterpret_cast<const long long*>(dp + o_qout), Qb, dout, st, kt);
         prof_ref().close(pm, lm);
     }

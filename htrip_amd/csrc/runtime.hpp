@@ -140,8 +140,7 @@ struct QueryPlan {
     std::vector<int> slot_q, slot_i;
     std::vector<LevelPlan> levels;
     // meet, every level in one launch
-    std::vector<int> mtiles;   // 8 ints each: group, q2, i0, j0, j_end, level
-    std::vector<int4> groups;  // (query, 0, glist offset, count)
+    std::vector<int> mtiles;   // 8 ints each: group size, q2, i0, j0, j_end, level, glist offset, query
     std::vector<int> glist;    // slot indices
     std::vector<int> zgeo;     // per level: Z offset lo, hi, ldz, 0, zoff[g2], zlen[g2]
     std::vector<int4> fills;   // INF fills of Q: (start lo, start hi, n8, first block)
@@ -160,7 +159,6 @@ struct QueryPlan {
             l.items.clear();
         }
         mtiles.clear();
-        groups.clear();
         glist.clear();
         zgeo.clear();
         fills.clear();
