@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from htrip_amd import grids, serving
-from htrip_amd._lib import HTG_E_RANGE, GpuMap, HtgError, IncState
+from htrip_amd._lib import HTG_E_INVALID, HTG_E_RANGE, GpuMap, HtgError, IncState
 from oracle.oracle import Oracle, grid_bfs
 
 pytestmark = pytest.mark.gpu
@@ -374,6 +374,11 @@ def test_lift_root_equals_a_fresh_build(gpu_ctx, kind, n, b, g, seed):
     C2 = G.clone()
     assert C2.S == S1
     np.testing.assert_array_equal(C2.query(vps), grid_bfs(G.cells, vps))
+    # a grid that does not contain the current one is refused and the map stays usable
+    with pytest.raises(HtgError) as ei:
+        G.lift_root(G.cells[:S1 // 2, :S1 // 2])
+    assert ei.value.code == HTG_E_INVALID and G.S == S1
+    np.testing.assert_array_equal(G.query(vps), grid_bfs(G.cells, vps))
     st.close()
 
 
