@@ -821,6 +821,7 @@ void Map::reset_versions() {
 }
 
 void Map::stamp(const std::vector<int>& leaves, const std::vector<std::vector<int>>& nodes, int first_level) {
+    if (leaf_ver.size() != size_t(nl) * nl || node_ver.size() != size_t(H)) reset_versions();  // never built
     ++clock;
     for (int x : leaves) leaf_ver[size_t(x)] = clock;
     for (size_t i = size_t(first_level); i < nodes.size(); ++i) {
