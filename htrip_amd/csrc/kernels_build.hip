@@ -26,6 +26,23 @@ __device__ __forceinline__ uint16_t add1(uint16_t a, uint16_t b) {
     return s >= kInf ? kInf : uint16_t(s);
 }
 
+// Value of the previous / next lane of the wavefront through a DPP wavefront
+// shift (wave_shr:1 / wave_shl:1, bound_ctrl: 0 past the ends): one VALU move
+// instead of an LDS permute round trip.  Lanes at source-group boundaries are
+// masked by the caller.
+__device__ __forceinline__ uint32_t lane_prev(uint32_t v) {
+    return uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x138, 0xf, 0xf, true));
+}
+__device__ __forceinline__ uint32_t lane_next(uint32_t v) {
+    return uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x130, 0xf, 0xf, true));
+}
+__device__ __forceinline__ uint64_t lane_prev(uint64_t v) {
+    return uint64_t(lane_prev(uint32_t(v))) | (uint64_t(lane_prev(uint32_t(v >> 32))) << 32);
+}
+__device__ __forceinline__ uint64_t lane_next(uint64_t v) {
+    return uint64_t(lane_next(uint32_t(v))) | (uint64_t(lane_next(uint32_t(v >> 32))) << 32);
+}
+
 // ---------------------------------------------------------------------------
 // Leaf all-pairs BFS (src/leaf_tile.cpp:56-153 + projection :198-207).
 // One wavefront runs 64/LANES BFS sources at once; lane r of a source group
@@ -70,8 +87,8 @@ __global__ __launch_bounds__(256) void k_leaf_apsp(const uint8_t* __restrict__ p
     uint16_t d = 0;
     while (__ballot(front != 0)) {
         ++d;
-        RowT up = __shfl_up(front, 1, LANES);
-        RowT dn = __shfl_down(front, 1, LANES);
+        RowT up = lane_prev(front);  // row r - 1 of the same source (r > 0)
+        RowT dn = lane_next(front);  // row r + 1 (r < LANES - 1)
         if (r == 0) up = 0;
         if (r == LANES - 1) dn = 0;
         RowT nb = ((front << 1) | (front >> 1) | up | dn) & prow & ~vis;
