@@ -2,13 +2,15 @@
 
 Bar: bit-exact (integer tropical arithmetic).  The oracle is the CPU
 restatement in oracle/ (pinned against the compiled reference and BFS in
-tests/test_oracle.py); the compiled reference itself is used too when present.
+tests/test_oracle.py); test_tables_equal_the_compiled_reference also compares
+the GPU directly with the compiled reference when oracle/_ref is built.
 """
 import numpy as np
 import pytest
 
 from htrip_amd import grids, serving
 from htrip_amd._lib import HTG_E_INVALID, HTG_E_RANGE, GpuMap, HtgError, IncState
+from oracle import oracle as orc
 from oracle.oracle import Oracle, grid_bfs
 
 pytestmark = pytest.mark.gpu
@@ -380,6 +382,31 @@ def test_lift_root_equals_a_fresh_build(gpu_ctx, kind, n, b, g, seed):
     assert ei.value.code == HTG_E_INVALID and G.S == S1
     np.testing.assert_array_equal(G.query(vps), grid_bfs(G.cells, vps))
     st.close()
+
+
+@pytest.mark.skipif(not orc.available("reference"), reason="oracle/_ref not built (reference sources absent)")
+@pytest.mark.parametrize("kind,n,b,g,seed", [("street", 64, 8, 2, 81), ("maze", 64, 16, 2, 82), ("street", 64, 4, 4, 83)])
+def test_tables_equal_the_compiled_reference(gpu_ctx, kind, n, b, g, seed):
+    """Against the reference's own CPU AVX2 code (oracle/_ref): every leaf and
+    node table after the build and after each reveal batch, last_batch_stats,
+    and the query matrices."""
+    print(f"reference {kind} n={n} b={b} g={g} seed={seed}", flush=True)
+    truth = make(kind, n, seed)
+    obs = grids.disk_observed(truth, [tuple(grids.sample_free(truth, 1, seed)[0])], n // 3)
+    G = GpuMap(obs, b, g)
+    assert G.build()
+    R = Oracle("reference", obs, b, g)
+    assert R.build()
+    assert_tables_equal(G, R)
+    order = grids.frontier_order(truth, obs)
+    for st in range(3):
+        batch = order[st * 90:(st + 1) * 90]
+        G.open_cells(batch)
+        R.open_cells(batch)
+        assert G.last_batch_stats() == R.last_batch_stats(), st
+        assert_tables_equal(G, R)
+        vps = grids.sample_free(G.cells, 24, seed + st)
+        np.testing.assert_array_equal(G.query(vps), R.query(vps))
 
 
 def test_cpp_dropin_program(gpu_ctx):
