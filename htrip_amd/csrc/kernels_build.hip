@@ -628,7 +628,7 @@ __global__ __launch_bounds__(256) void k_cover_product(int nV, int ld, const int
 static int cover_rpt(int max_c, int col_tiles, int n_list) {
     for (int rpt = 4; rpt > 1; rpt /= 2) {
         const long long blocks = (long long)((max_c + 16 * rpt - 1) / (16 * rpt)) * col_tiles * n_list;
-        if (blocks >= 1024) return rpt;
+        if (blocks >= 512) return rpt;
     }
     return 1;
 }
