@@ -31,7 +31,7 @@ struct KTimer {
 // Workgroups an update-path launch may occupy at once (of ~2048 resident on
 // the chip): the grid strides over its work so that every workgroup is
 // dispatched at once and query kernels on other streams find free slots.
-constexpr int kBackgroundWG = 1024;
+constexpr int kBackgroundWG = 512;
 
 // max_wg > 0 caps the grid (work items are strided over it).
 bool launch_leaf_apsp(const uint8_t* pass, int S, int b, int nl, const int* leaf_list, int n_list, uint16_t* leafD,
