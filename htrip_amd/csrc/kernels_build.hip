@@ -624,7 +624,8 @@ __global__ __launch_bounds__(256) void k_cover_product(int nV, int ld, const int
     }
 }
 
-// Rows per thread: the tallest tile that still gives about 1024 workgroups.
+// Rows per thread: the tallest tile that still gives 512 or more workgroups
+// (update work that leaves room for query batches on other streams).
 static int cover_rpt(int max_c, int col_tiles, int n_list) {
     for (int rpt = 4; rpt > 1; rpt /= 2) {
         const long long blocks = (long long)((max_c + 16 * rpt - 1) / (16 * rpt)) * col_tiles * n_list;
