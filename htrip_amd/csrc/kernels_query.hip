@@ -39,6 +39,22 @@ __device__ __forceinline__ uint32_t qmin2(uint32_t a, uint32_t b) {
                                                                   __builtin_bit_cast(us2q, b)));
 }
 
+// XCD-aware work order.  The hardware deals a launch's workgroups round-robin
+// over the 8 XCDs, each with its own L2.  This bijection hands each XCD runs
+// of G consecutive logical items (its k-th workgroup takes item
+// (k / G) * 8G + x G + k % G), so that neighbouring items, which share rows,
+// share one L2, while the runs still rotate over all XCDs and a range of
+// expensive items does not land on one of them.  The last n mod 8G items keep
+// the hardware order.
+template <unsigned G>
+__device__ __forceinline__ unsigned xcd_logical(unsigned hw, unsigned n) {
+    constexpr unsigned X = 8;
+    const unsigned full = n / (X * G) * (X * G);
+    if (hw >= full) return hw;
+    const unsigned x = hw % X, k = hw / X;
+    return (k / G) * (X * G) + x * G + k % G;
+}
+
 // Front of the query, one 1-D launch with three block ranges:
 //  [0, nq * ib): output init (src/minplus_oracle.cpp:940-965, same-leaf pairs
 //     :1027-1048) of query blk / ib: INF, 0 on the diagonal of valid
@@ -122,7 +138,8 @@ void launch_q_front(int nq, const int* q_k, const long long* q_out, const int* q
 }
 
 // Projection Z (see file comment), split over the seam rows: work item
-// (W, q, first slot, count), column tile blockIdx.y and row chunk blockIdx.z;
+// (W, q, first slot, count), column tile and row chunk (the grid's x, y, z in
+// an XCD-aware order);
 // four wavefronts share the columns and take interleaved rounds of rows.
 // Only live column groups are computed: live_list[W] holds the groups of 4
 // projection columns with at least one passable port (the others are INF and
@@ -155,13 +172,18 @@ __global__ __launch_bounds__(256) void k_q_z(
     __shared__ __align__(16) uint32_t xs[NW][ACH][M];  // per wavefront, row-major: one row's M values contiguous
     __shared__ unsigned roff[NW][ACH];
     __shared__ uint2 red[NW - 1][M][64];  // accumulators of wavefronts 1..NW-1 for the final minimum
-    const int4 it = items[2 * blockIdx.x], it2 = items[2 * blockIdx.x + 1];
+    // logical (item, column tile, row chunk): items of one (W, q) read the same
+    // D rows, so runs of consecutive items go to one XCD
+    const unsigned lin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    const unsigned L = xcd_logical<4>(lin, gridDim.x * gridDim.y * gridDim.z);
+    const int bx = int(L % gridDim.x), by = int((L / gridDim.x) % gridDim.y), bz = int(L / (gridDim.x * gridDim.y));
+    const int4 it = items[2 * bx], it2 = items[2 * bx + 1];
     const int W = it.x, q = it.y & 0xffff, s0 = it.z, m = it.w;
     const bool first_item = (it.y >> 16) & 1;  // first item of its (W, q) run: counts the D bytes once
     const int tid = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int nlive = it2.x;  // live column groups of W
-    const int gi = blockIdx.y * 64 + tid;
-    if (blockIdx.y * 64 >= nlive) return;  // whole workgroup past W's live groups
+    const int gi = by * 64 + tid;
+    if (by * 64 >= nlive) return;  // whole workgroup past W's live groups
     const bool active = gi < nlive;
     const int c0 = active ? live_list[size_t(W) * ngrp + gi] * 4 : 0;
     // Projection columns come in aligned groups of 4 consecutive V_U ports,
@@ -193,7 +215,7 @@ __global__ __launch_bounds__(256) void k_q_z(
     }
     const int* seam = seam_list + it2.y;  // child q's seam ports
     const int ns = it2.z - it2.y;
-    const int a_lo = int((long long)ns * blockIdx.z / ksplit), a_hi = int((long long)ns * (blockIdx.z + 1) / ksplit);
+    const int a_lo = int((long long)ns * bz / ksplit), a_hi = int((long long)ns * (bz + 1) / ksplit);
     const unsigned pad_off = unsigned(q * pc + it2.w) * unsigned(ld);
     // rows per wavefront and round: the item's rows spread over all NW
     // wavefronts (a multiple of PF, at most ACH), so that a short seam list
@@ -272,11 +294,11 @@ __global__ __launch_bounds__(256) void k_q_z(
     // wavefront would serialise in the L2 and double the kernel's time)
     __shared__ unsigned long long wsum[NW][2];
     if (wstat && tid == 0) {
-        const int nact = min(64, nlive - blockIdx.y * 64);
+        const int nact = min(64, nlive - by * 64);
         const unsigned long long cols4 = 4ull * nact;
         unsigned long long bytes = first_item ? (unsigned long long)rows_done * cols4 * 2 : 0ull;
         if (w == 0) {
-            if (blockIdx.y == 0) bytes += (unsigned long long)m * (a_hi - a_lo) * 2;
+            if (by == 0) bytes += (unsigned long long)m * (a_hi - a_lo) * 2;
             bytes += (unsigned long long)m * cols4 * 2;
         }
         wsum[w][0] = bytes;
@@ -290,7 +312,7 @@ __global__ __launch_bounds__(256) void k_q_z(
     __syncthreads();
     if (w > 0) return;
     if (wstat && tid == 0) {
-        const unsigned slot = (blockIdx.x * 131u + blockIdx.y * 17u + blockIdx.z * 7u) % unsigned(kWorkSlots);
+        const unsigned slot = (unsigned(bx) * 131u + unsigned(by) * 17u + unsigned(bz) * 7u) % unsigned(kWorkSlots);
         atomicAdd(&wstat[2 * slot], wsum[0][0] + wsum[1][0] + wsum[2][0] + wsum[3][0]);
         atomicAdd(&wstat[2 * slot + 1], wsum[0][1] + wsum[1][1] + wsum[2][1] + wsum[3][1]);
     }
@@ -304,7 +326,7 @@ __global__ __launch_bounds__(256) void k_q_z(
         }
     if (!active) return;
     if (ksplit > 1) {
-        uint16_t* P = Zp + size_t(blockIdx.z) * part_stride;
+        uint16_t* P = Zp + size_t(bz) * part_stride;
 #pragma unroll
         for (int v = 0; v < M; ++v)
             if (v < m) *reinterpret_cast<uint2*>(P + size_t(s0 + v) * ldp + c0) = make_uint2(acc[v][0], acc[v][1]);
@@ -441,8 +463,11 @@ __global__ __launch_bounds__(256) void k_q_meet(const int* __restrict__ tiles, c
     __shared__ uint32_t Bm[8][LDR];
     // tile: group size, target child q2, first rows i0 / j0, j end, level,
     // the group's offset in glist, its query
-    const int4 t0 = *reinterpret_cast<const int4*>(tiles + size_t(blockIdx.x) * 8);
-    const int4 t1 = *reinterpret_cast<const int4*>(tiles + size_t(blockIdx.x) * 8 + 4);
+    // logical tile: tiles of one group and target child share Z rows, so runs
+    // of consecutive tiles go to one XCD
+    const unsigned bt = xcd_logical<4>(blockIdx.x, gridDim.x);
+    const int4 t0 = *reinterpret_cast<const int4*>(tiles + size_t(bt) * 8);
+    const int4 t1 = *reinterpret_cast<const int4*>(tiles + size_t(bt) * 8 + 4);
     const int n = t0.x, q2 = t0.y, i0 = t0.z, j0 = t0.w, jend = t1.x, lvl = t1.y, Q = t1.w;
     const int* lst = glist + t1.z;
     const int* zg = zgeo + size_t(lvl) * (4 + 2 * g2);
