@@ -86,7 +86,7 @@ def measured_traffic(kernel):
 def roofline_of(stats):
     """Roofline of the query projection (k_q_z) from accumulated kernel stats:
     bytes and min-plus updates are counted on the device, time from start/stop
-    events attached to each dispatch."""
+    events recorded on the query stream around each dispatch."""
     from htrip_amd import _lib
 
     z = stats[_lib.K_Q_PROJECT]

@@ -188,7 +188,7 @@ struct QueryWS {
     ~QueryWS();
 };
 
-// Kernel timing (see htg_profile_enable): start/stop events attached to each
+// Kernel timing (see htg_profile_enable): start/stop events around each
 // tracked dispatch.  Thread-safe; a session's two snapshot maps share one
 // instance.
 struct Profiler {

@@ -172,11 +172,14 @@ int htg_read_node_DU(htg_map* map, int32_t lvl, int32_t idx, uint16_t* out); /* 
 int htg_read_node_SU(htg_map* map, int32_t lvl, int32_t idx, uint16_t* out); /* nExt x nExt */
 
 /* ---- kernel timing ----------------------------------------------------------
- * When enabled, the runtime attaches start/stop hipEvents to every dispatch of
- * the kernel classes below (hipExtLaunchKernelGGL, so the interval is the
- * kernel's own execution), and accumulates
- * launch counts, elapsed time and the algorithmic bytes / min-plus updates of
- * each launch.  Enabling resets the counters. */
+ * When enabled, the runtime records start/stop hipEvents on the launching
+ * stream immediately before and after every dispatch of the kernel classes
+ * below (so the interval includes the dispatch's start latency), and
+ * accumulates launch counts, elapsed time and the algorithmic bytes / min-plus
+ * updates of each launch.  HTG_PROFILE_ATTACH=1 in the environment attaches
+ * the events to the dispatch itself instead (hipExtLaunchKernelGGL), which
+ * slows the timed kernel about twofold on gfx950.  Enabling resets the
+ * counters. */
 enum {
     HTG_K_LEAF_APSP = 0,
     HTG_K_CLOSURE_PIVOTS = 1,
