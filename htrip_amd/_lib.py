@@ -14,7 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libhtrip_gpu.so")
 INF = 60000
 
-HTG_OK, HTG_E_INVALID, HTG_E_NOMEM, HTG_E_HIP, HTG_E_STATE, HTG_E_RANGE = 0, -1, -2, -3, -4, -5
+HTG_OK, HTG_E_INVALID, HTG_E_NOMEM, HTG_E_HIP, HTG_E_STATE, HTG_E_RANGE, HTG_E_CANCELLED = 0, -1, -2, -3, -4, -5, -6
+FAULT_UPDATE, FAULT_QUERY, FAULT_BUILD = 1, 2, 3
 
 # Every symbol include/htrip_gpu.h declares (checked by tests/test_abi.py).
 EXPORTS = [
@@ -25,6 +26,8 @@ EXPORTS = [
     "htg_read_leaf_D", "htg_read_leaf_S", "htg_node_dims", "htg_read_node_DU", "htg_read_node_SU",
     "htg_profile_enable", "htg_profile_get",
     "htg_get_tuning", "htg_set_tuning", "htg_autotune",
+    "htg_query_ws_create", "htg_query_ws_destroy", "htg_query_ws_bytes", "htg_query_all_pairs_ws",
+    "htg_query_batch_ws", "htg_inject_fault",
 ]
 
 K_LEAF_APSP, K_CLOSURE_PIVOTS, K_CLOSURE_UPDATE, K_Q_PROJECT, K_Q_MEET, K_Q_LIFT, K_COVER_RELAX, K_COVER_PAIRS = range(8)
@@ -113,6 +116,12 @@ def load(path: str = LIB_PATH):
         "htg_set_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
         "htg_autotune": (C.c_int, [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int32, C.c_int32,
                                    C.POINTER(Tuning)]),
+        "htg_query_ws_create": (C.c_int, [vp, C.POINTER(vp)]),
+        "htg_query_ws_destroy": (C.c_int, [vp]),
+        "htg_query_ws_bytes": (C.c_int, [vp, C.POINTER(u64)]),
+        "htg_query_all_pairs_ws": (C.c_int, [vp, vp, pi32, i32, pu16]),
+        "htg_query_batch_ws": (C.c_int, [vp, vp, pi32, pi32, i32, pu16]),
+        "htg_inject_fault": (C.c_int, [vp, i32, i32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -190,6 +199,34 @@ class IncState:
             pass
 
 
+class QueryWorkspace:
+    """Caller-owned query workspace (htg_query_ws; the reference's
+    HTripQueryScratch): its own stream and device buffers, so that queries
+    through distinct workspaces run on one map from several threads."""
+
+    def __init__(self, ctx: "Context | None" = None):
+        self.ctx = ctx or default_context()
+        self.lib = self.ctx.lib
+        self.h = C.c_void_p()
+        check(self.lib.htg_query_ws_create(self.ctx.h, C.byref(self.h)))
+
+    def device_bytes(self) -> int:
+        n = C.c_uint64()
+        check(self.lib.htg_query_ws_bytes(self.h, C.byref(n)))
+        return n.value
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.htg_query_ws_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class GpuMap:
     """One resident hierarchy (htg_map): the GPU MinPlusOracle over a cell array.
 
@@ -235,15 +272,22 @@ class GpuMap:
         return True
 
     def open_cells(self, rc) -> None:
+        """onBatchCellsOpened.  self.cells follows the map's cell state: it
+        takes the batch unless the map refused the call (HTG_E_STATE), also
+        when the update failed part way (the map's state holds the batch)."""
         rc = np.ascontiguousarray(rc, dtype=np.int32).reshape(-1, 2)
-        ok = (rc[:, 0] >= 0) & (rc[:, 1] >= 0) & (rc[:, 0] < self.cells.shape[0]) & (rc[:, 1] < self.cells.shape[1])
-        self.cells[rc[ok, 0], rc[ok, 1]] = 1
-        check(self.lib.htg_open_cells(self.h, ptr(rc, C.c_int32), len(rc)))
+        code = self.lib.htg_open_cells(self.h, ptr(rc, C.c_int32), len(rc))
+        if code != HTG_E_STATE:
+            ok = (rc[:, 0] >= 0) & (rc[:, 1] >= 0) & (rc[:, 0] < self.cells.shape[0]) & \
+                (rc[:, 1] < self.cells.shape[1])
+            self.cells[rc[ok, 0], rc[ok, 1]] = 1
+        check(code)
 
     def open_cell(self, r: int, c: int) -> None:
-        if 0 <= r < self.cells.shape[0] and 0 <= c < self.cells.shape[1]:
+        code = self.lib.htg_open_cell(self.h, r, c)
+        if code != HTG_E_STATE and 0 <= r < self.cells.shape[0] and 0 <= c < self.cells.shape[1]:
             self.cells[r, c] = 1
-        check(self.lib.htg_open_cell(self.h, r, c))
+        check(code)
 
     def block_cells(self, rc) -> None:
         """onCellBlocked for each listed cell: passable cells become Obstacle."""
@@ -260,13 +304,28 @@ class GpuMap:
         check(self.lib.htg_query_all_pairs(self.h, ptr(vps, C.c_int32), k, ptr(out, C.c_uint16)))
         return out
 
-    def query_batch(self, vps_list) -> list:
+    def query_ws(self, ws: QueryWorkspace, vps) -> np.ndarray:
+        """queryAllPairs through a caller-owned workspace (thread-safe across workspaces)."""
+        vps = np.ascontiguousarray(vps, dtype=np.int32).reshape(-1, 2)
+        k = len(vps)
+        out = np.empty((k, k), dtype=np.uint16)
+        check(self.lib.htg_query_all_pairs_ws(self.h, ws.h, ptr(vps, C.c_int32), k, ptr(out, C.c_uint16)))
+        return out
+
+    def inject_fault(self, site: int, countdown: int = 1) -> None:
+        check(self.lib.htg_inject_fault(self.h, site, countdown))
+
+    def query_batch(self, vps_list, ws: QueryWorkspace | None = None) -> list:
         ks = np.array([len(v) for v in vps_list], dtype=np.int32)
         allv = np.ascontiguousarray(np.concatenate([np.asarray(v, np.int32).reshape(-1, 2) for v in vps_list])
                                     if len(vps_list) else np.zeros((0, 2), np.int32), dtype=np.int32)
         out = np.empty(int((ks.astype(np.int64) ** 2).sum()), dtype=np.uint16)
-        check(self.lib.htg_query_batch(self.h, ptr(allv, C.c_int32), ptr(ks, C.c_int32), len(ks),
-                                       ptr(out, C.c_uint16)))
+        if ws is None:
+            check(self.lib.htg_query_batch(self.h, ptr(allv, C.c_int32), ptr(ks, C.c_int32), len(ks),
+                                           ptr(out, C.c_uint16)))
+        else:
+            check(self.lib.htg_query_batch_ws(self.h, ws.h, ptr(allv, C.c_int32), ptr(ks, C.c_int32), len(ks),
+                                              ptr(out, C.c_uint16)))
         res, o = [], 0
         for k in ks:
             res.append(out[o:o + k * k].reshape(k, k))

@@ -1,5 +1,6 @@
 // Launch wrappers for the gfx950 kernels (kernels_build.hip, kernels_query.hip).
-// Each wrapper validates nothing: the runtime checks shapes before calling.
+// The runtime checks shapes before calling; a wrapper whose kernel has a hard
+// limit of its own returns false for arguments beyond it.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -137,8 +138,9 @@ void launch_set_pass(uint8_t* pass, int S, const int2* cells, int n, hipStream_t
 // and X0[slot][p] = leaf D row of the slot's cell at leaf port p; clears
 // wstat (nullable).
 // INF fills: n_fill ranges (start lo, start hi, n8, first block) of n8 units
-// of 8 u16 from fill_base, fill_blocks blocks in all.
-void launch_q_front(int nq, const int* q_k, const long long* q_out, const int* q_vp, const int2* vp_rc,
+// of 8 u16 from fill_base, fill_blocks blocks in all.  Returns false without
+// launching when n_fill > 64 (one ballot finds a block's range).
+bool launch_q_front(int nq, const int* q_k, const long long* q_out, const int* q_vp, const int2* vp_rc,
                     const uint8_t* vp_valid, int kmax, int n_slots, const int2* slot_rc, int b, int nl,
                     const uint16_t* leafD, uint16_t* out, uint16_t* X0, int n_fill, const int4* fills,
                     int fill_blocks, uint16_t* fill_base, unsigned long long* wstat, hipStream_t st);

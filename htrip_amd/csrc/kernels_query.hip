@@ -128,17 +128,20 @@ __global__ __launch_bounds__(256) void k_q_front(int nq, int ib, int leaf_blocks
     reinterpret_cast<uint4*>(fill_base + start)[u] = make_uint4(kInf2, kInf2, kInf2, kInf2);
 }
 
-void launch_q_front(int nq, const int* q_k, const long long* q_out, const int* q_vp, const int2* vp_rc,
+bool launch_q_front(int nq, const int* q_k, const long long* q_out, const int* q_vp, const int2* vp_rc,
                     const uint8_t* vp_valid, int kmax, int n_slots, const int2* slot_rc, int b, int nl,
                     const uint16_t* leafD, uint16_t* out, uint16_t* X0, int n_fill, const int4* fills,
                     int fill_blocks, uint16_t* fill_base, unsigned long long* wstat, hipStream_t st) {
-    if (nq == 0) return;
+    // the block's fill range is found with one 64-lane ballot over the ranges
+    if (n_fill > 64 || n_fill < 0) return false;
+    if (nq == 0) return true;
     const int ib = int(((long long)kmax * kmax + 255) / 256);
     const int leaf_blocks = int(((long long)n_slots * (4 * b - 4) + 255) / 256);
     const long long blocks = (long long)nq * ib + leaf_blocks + fill_blocks;
-    if (blocks == 0 || n_fill > 64) return;  // at most 64 fill ranges (2 per level)
+    if (blocks == 0) return true;
     hipLaunchKernelGGL(k_q_front, dim3(unsigned(blocks)), dim3(256), 0, st, nq, ib, leaf_blocks, q_k, q_out, q_vp,
                        vp_rc, vp_valid, n_slots, slot_rc, b, nl, leafD, out, X0, n_fill, fills, fill_base, wstat);
+    return true;
 }
 
 // Projection Z (see file comment), split over the seam rows: work item

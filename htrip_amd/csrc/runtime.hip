@@ -753,6 +753,20 @@ int Map::build() {
     hipError_t e;
     int rc;
     timing_now = prof_ref().take();
+    if (broken) {
+        // a failed update left the device passability and live lists behind
+        // the host's cell state: bring them level before rebuilding
+        if (live_pending) {
+            if ((e = hipEventSynchronize(live_evt)) != hipSuccess) return hip_fail(e, "live upload wait");
+            live_pending = false;
+        }
+        if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "recovery sync");
+        if ((e = hipMemcpyAsync(d_pass.p, pass.data(), pass.size(), hipMemcpyHostToDevice, stream)) != hipSuccess ||
+            (e = hipStreamSynchronize(stream)) != hipSuccess)
+            return hip_fail(e, "recovery: upload grid");
+        if ((rc = refresh_live(nullptr))) return rc;
+    }
+    if (fault_hit(HTG_FAULT_BUILD)) return fail(HTG_E_HIP, "injected fault (build)");
     std::vector<std::vector<int>> hc(size_t(H) + 1);
     std::vector<int> sig;
     int ws_lvl = 0;
@@ -807,6 +821,7 @@ int Map::build() {
     if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "build sync");
     prof_ref().harvest();
     built = true;
+    broken = false;
     epoch++;
     reset_versions();
     return HTG_OK;
@@ -1108,7 +1123,8 @@ int Map::open_cells(const int32_t* rcs, int n) {
     }
     bool below_any = true;  // the level below may have changed flags set
     const int* chg_below = chg;
-    int top = 0;  // highest level processed
+    int top = 0;        // highest level processed
+    int top_first = 0;  // lowest level processed
     for (int lvl = 1; lvl <= H; ++lvl) {
         LevelGeo& L = lv[lvl - 1];
         const std::vector<int>& cl = cand[size_t(lvl)];
@@ -1122,6 +1138,7 @@ int Map::open_cells(const int32_t* rcs, int n) {
             continue;
         }
         top = lvl;
+        if (top_first == 0) top_first = lvl;
         if ((rc = ensure_build_ws(lvl))) return rc;
         const int* dnodes = reinterpret_cast<const int*>(up + o_cand[size_t(lvl)]);
         int max8 = 0;
@@ -1204,6 +1221,7 @@ int Map::open_cells(const int32_t* rcs, int n) {
         }
         below_any = sum > 0;
         chg_below = chg_here;
+        if (lvl == top_first && fault_hit(HTG_FAULT_UPDATE)) return fail(HTG_E_HIP, "injected fault (update)");
     }
     const auto u3 = clk::now();
     // stats: a node is a dirty ancestor when a child summary changed or it
@@ -1300,6 +1318,36 @@ int Map::catch_up_from(const Map& src, const DirtySet& ds) {
     stats = src.stats;
     epoch = src.epoch;
     stamp(ds.leaves, ds.nodes, 0);
+    return HTG_OK;
+}
+
+int Map::copy_all_from(const Map& src) {
+    if (src.S != S || src.H != H || src.b != b || src.g != g || src.rows != rows || src.cols != cols)
+        return fail(HTG_E_INVALID, "copy between maps of different geometry");
+    hipError_t e;
+    if ((e = hipStreamSynchronize(src.stream)) != hipSuccess || (e = hipStreamSynchronize(stream)) != hipSuccess)
+        return hip_fail(e, "full copy sync");
+    pass = src.pass;
+    cells_code = src.cells_code;
+    const auto d2d = hipMemcpyDeviceToDevice;
+    if ((e = hipMemcpyAsync(d_pass.p, src.d_pass.p, size_t(S) * S, d2d, stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(leafD.p, src.leafD.p, leafD.bytes, d2d, stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(leafS.p, src.leafS.p, leafS.bytes, d2d, stream)) != hipSuccess)
+        return hip_fail(e, "full copy");
+    for (size_t l = 0; l < lv.size(); ++l) {
+        const size_t du = size_t(lv[l].nn) * lv[l].node_stride * 2, su = size_t(lv[l].nn) * lv[l].d.ne * lv[l].d.ne * 2;
+        if ((e = hipMemcpyAsync(lv[l].DU.p, src.lv[l].DU.p, du, d2d, stream)) != hipSuccess ||
+            (e = hipMemcpyAsync(lv[l].SU.p, src.lv[l].SU.p, su, d2d, stream)) != hipSuccess)
+            return hip_fail(e, "full copy");
+    }
+    int rc;
+    if ((rc = refresh_live(nullptr))) return rc;
+    if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "full copy sync");
+    stats = src.stats;
+    epoch = src.epoch;
+    built = src.built;
+    broken = false;
+    reset_versions();
     return HTG_OK;
 }
 
@@ -1656,6 +1704,8 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
     // the meet's 16-bit atomics address whole 32-bit words
     if ((rc = ws.d_out.ensure(size_t(total_out) * 2 + 4))) return rc;
     if ((rc = ws.h_out.ensure(size_t(total_out) * 2))) return rc;
+    if (P.fills.size() > 64) return fail(HTG_E_INVALID, "query plan has more than 64 fill ranges");
+    if (fault_hit(HTG_FAULT_QUERY)) return fail(HTG_E_HIP, "injected fault (query launch)");
     hipStream_t st = ws.stream;
     static const bool kernel_copy = [] {
         const char* e = std::getenv("HTG_KERNEL_COPY");
@@ -1678,10 +1728,11 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
     }
     const int2* srcs = reinterpret_cast<const int2*>(dp + o_src);
     uint16_t* Qb = static_cast<uint16_t*>(ws.Q.p);
-    launch_q_front(nq, I(o_qk), reinterpret_cast<const long long*>(dp + o_qout), I(o_qvp),
-                   reinterpret_cast<const int2*>(dp + o_vprc), reinterpret_cast<const uint8_t*>(dp + o_vpvalid), kmax,
-                   H > 0 ? ns : 0, srcs, b, nl, dleafD, dout, Qb + xoff[0], int(P.fills.size()),
-                   reinterpret_cast<const int4*>(dp + o_fills), fill_blocks, Qb, wstat, st);
+    if (!launch_q_front(nq, I(o_qk), reinterpret_cast<const long long*>(dp + o_qout), I(o_qvp),
+                        reinterpret_cast<const int2*>(dp + o_vprc), reinterpret_cast<const uint8_t*>(dp + o_vpvalid),
+                        kmax, H > 0 ? ns : 0, srcs, b, nl, dleafD, dout, Qb + xoff[0], int(P.fills.size()),
+                        reinterpret_cast<const int4*>(dp + o_fills), fill_blocks, Qb, wstat, st))
+        return fail(HTG_E_INVALID, "query front: more than 64 fill ranges");
     for (int l = 1; l <= H && ns > 0; ++l) {
         const LevelGeo& L = lv[l - 1];
         const LevelPlan& LP = P.levels[l - 1];
@@ -1806,6 +1857,15 @@ QueryWS::~QueryWS() {
     catch (...) {                                                       \
         return htg::fail(HTG_E_INVALID, "unexpected exception");        \
     }
+
+namespace {
+// Calls on a map need a built map whose last update did not fail.
+int usable(const htg_map* m) {
+    if (!m->built) return fail(HTG_E_STATE, "map not built");
+    if (m->broken) return fail(HTG_E_STATE, "an update of this map failed: it needs htg_build");
+    return HTG_OK;
+}
+}  // namespace
 
 extern "C" {
 
@@ -1942,6 +2002,7 @@ int map_create_on(htg_ctx* ctx, const htg_config* cfg, int32_t rows, int32_t col
 int map_clone_on(const htg_map* src, uint64_t hbm_budget, hipStream_t shared, htg_map** out) {
     HTG_GUARD_BEGIN
     if (!src || !out) return fail(HTG_E_INVALID, "null argument");
+    if (src->broken) return fail(HTG_E_STATE, "an update of this map failed: it needs htg_build");
     htg_map* m = nullptr;
     int rc = map_create_on(src->ctx, &src->cfg, src->rows, src->cols, src->cells_code.data(), hbm_budget, shared, &m,
                            src->min_S);
@@ -2113,17 +2174,19 @@ int htg_build(htg_map* map) {
 int htg_open_cells(htg_map* map, const int32_t* rc, int32_t n) {
     HTG_GUARD_BEGIN
     if (!map || (n > 0 && !rc)) return fail(HTG_E_INVALID, "null argument");
-    if (!map->built) return fail(HTG_E_STATE, "map not built");
+    if (int u = usable(map)) return u;
     hipError_t e = hipSetDevice(map->ctx->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
-    return map->open_cells(rc, n);
+    const int r = map->open_cells(rc, n);
+    if (r != HTG_OK) map->broken = true;  // the cell state already includes the batch
+    return r;
     HTG_GUARD_END
 }
 
 int htg_open_cell(htg_map* map, int32_t r, int32_t c) {
     HTG_GUARD_BEGIN
     if (!map) return fail(HTG_E_INVALID, "null map");
-    if (!map->built) return fail(HTG_E_STATE, "map not built");
+    if (int u = usable(map)) return u;
     hipError_t e = hipSetDevice(map->ctx->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     // onCellOpened does not touch last_batch_stats (src/minplus_oracle.cpp:139-177)
@@ -2131,6 +2194,7 @@ int htg_open_cell(htg_map* map, int32_t r, int32_t c) {
     const int32_t rcv[2] = {r, c};
     int rcode = map->open_cells(rcv, 1);
     map->stats = keep;
+    if (rcode != HTG_OK) map->broken = true;
     return rcode;
     HTG_GUARD_END
 }
@@ -2139,17 +2203,34 @@ int htg_block_cells(htg_map* map, const int32_t* rc, int32_t n) {
     HTG_GUARD_BEGIN
     if (!map || (n > 0 && !rc)) return fail(HTG_E_INVALID, "null argument");
     if (n < 0) return fail(HTG_E_INVALID, "negative count");
-    if (!map->built) return fail(HTG_E_STATE, "map not built");
+    if (int u = usable(map)) return u;
     hipError_t e = hipSetDevice(map->ctx->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
-    return map->block_cells(rc, n);
+    const int r = map->block_cells(rc, n);
+    if (r != HTG_OK) map->broken = true;
+    return r;
     HTG_GUARD_END
+}
+
+// Viewpoint count of a batch; HTG_E_INVALID for a negative k or missing rc.
+static int check_batch(const int32_t* rc, const int32_t* ks, int32_t nq, uint16_t* out) {
+    if (nq < 0) return fail(HTG_E_INVALID, "negative query count");
+    if (nq > 0 && (!ks || !out)) return fail(HTG_E_INVALID, "null argument");
+    long long total = 0;
+    for (int q = 0; q < nq; ++q) {
+        if (ks[q] < 0) return fail(HTG_E_INVALID, "negative k");
+        total += ks[q];
+    }
+    if (total > 0 && !rc) return fail(HTG_E_INVALID, "null viewpoints");
+    return HTG_OK;
 }
 
 int htg_query_batch(htg_map* map, const int32_t* rc, const int32_t* ks, int32_t nq, uint16_t* out) {
     HTG_GUARD_BEGIN
-    if (!map || (nq > 0 && (!ks || !out))) return fail(HTG_E_INVALID, "null argument");
-    if (!map->built) return fail(HTG_E_STATE, "map not built");
+    if (int c = check_batch(rc, ks, nq, out)) return c;
+    if (!map) return fail(HTG_E_INVALID, "null map");
+    if (int u = usable(map)) return u;
+    std::lock_guard<std::mutex> lk(map->qmu);
     hipError_t e = hipSetDevice(map->ctx->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     // queries read what the map's update stream wrote
@@ -2158,12 +2239,74 @@ int htg_query_batch(htg_map* map, const int32_t* rc, const int32_t* ks, int32_t 
     HTG_GUARD_END
 }
 
+int htg_query_ws_create(htg_ctx* ctx, htg_query_ws** out) {
+    HTG_GUARD_BEGIN
+    if (!ctx || !out) return fail(HTG_E_INVALID, "null argument");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    std::unique_ptr<htg_query_ws> w(new htg_query_ws());
+    w->ctx = ctx;
+    if (int rc = w->ws.init()) return rc;  // a stream of its own
+    if ((e = hipEventCreateWithFlags(&w->after, hipEventDisableTiming)) != hipSuccess)
+        return hip_fail(e, "hipEventCreate");
+    *out = w.release();
+    return HTG_OK;
+    HTG_GUARD_END
+}
+
+int htg_query_ws_destroy(htg_query_ws* ws) {
+    if (!ws) return HTG_OK;
+    (void)hipSetDevice(ws->ctx->device);
+    if (ws->ws.stream) (void)hipStreamSynchronize(ws->ws.stream);
+    delete ws;
+    return HTG_OK;
+}
+
+int htg_query_ws_bytes(const htg_query_ws* ws, uint64_t* bytes) {
+    if (!ws || !bytes) return fail(HTG_E_INVALID, "null argument");
+    const QueryWS& w = ws->ws;
+    *bytes = uint64_t(w.d_plan.bytes + w.Q.bytes + w.Zp.bytes + w.d_out.bytes + w.d_wstat.bytes);
+    return HTG_OK;
+}
+
+int htg_query_batch_ws(htg_map* map, htg_query_ws* ws, const int32_t* rc, const int32_t* ks, int32_t nq,
+                       uint16_t* out) {
+    HTG_GUARD_BEGIN
+    if (int c = check_batch(rc, ks, nq, out)) return c;
+    if (!map || !ws) return fail(HTG_E_INVALID, "null argument");
+    if (ws->ctx->device != map->ctx->device) return fail(HTG_E_INVALID, "workspace of another device");
+    if (int u = usable(map)) return u;
+    hipError_t e = hipSetDevice(map->ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    // the workspace's stream runs after every update enqueued on the map so far
+    if ((e = hipEventRecord(ws->after, map->stream)) != hipSuccess ||
+        (e = hipStreamWaitEvent(ws->ws.stream, ws->after, 0)) != hipSuccess)
+        return hip_fail(e, "order after updates");
+    return map->query(rc, ks, nq, out, ws->ws);
+    HTG_GUARD_END
+}
+
+int htg_query_all_pairs_ws(htg_map* map, htg_query_ws* ws, const int32_t* rc, int32_t k, uint16_t* out) {
+    return htg_query_batch_ws(map, ws, rc, &k, 1, out);
+}
+
+int htg_inject_fault(htg_map* map, int32_t site, int32_t countdown) {
+    if (!map) return fail(HTG_E_INVALID, "null map");
+    if (countdown < 0 || (countdown > 0 && (site < HTG_FAULT_UPDATE || site > HTG_FAULT_BUILD)))
+        return fail(HTG_E_INVALID, "bad fault site or countdown");
+    map->fault_site = 0;
+    map->fault_count = countdown;
+    map->fault_site = countdown > 0 ? site : 0;
+    return HTG_OK;
+}
+
 int htg_lift_point(htg_map* map, int32_t r, int32_t c, uint16_t* x, int64_t x_cap, int32_t* lens,
                    int32_t* parent_node_idx, int32_t* child_quad, int32_t* child_offset) {
     HTG_GUARD_BEGIN
     if (!map || !x || !lens || !parent_node_idx || !child_quad || !child_offset)
         return fail(HTG_E_INVALID, "null argument");
-    if (!map->built) return fail(HTG_E_STATE, "map not built");
+    if (int u = usable(map)) return u;
+    std::lock_guard<std::mutex> lk(map->qmu);
     hipError_t e = hipSetDevice(map->ctx->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     return map->lift_point(r, c, x, x_cap, lens, parent_node_idx, child_quad, child_offset);
@@ -2173,6 +2316,7 @@ int htg_lift_point(htg_map* map, int32_t r, int32_t c, uint16_t* x, int64_t x_ca
 int htg_lift_root(htg_map** map, int32_t rows, int32_t cols, const uint8_t* cells) {
     HTG_GUARD_BEGIN
     if (!map || !*map) return fail(HTG_E_INVALID, "null map");
+    if ((*map)->broken) return fail(HTG_E_STATE, "an update of this map failed: it needs htg_build");
     htg_map* m = nullptr;
     const int rc = htg::lift_root(*map, rows, cols, cells, &m);
     if (rc) return rc;
@@ -2200,7 +2344,8 @@ int htg_query_all_pairs_incremental(htg_map* map, htg_inc* state, const int32_t*
     HTG_GUARD_BEGIN
     if (!map || !state || (k > 0 && (!rc || !out))) return fail(HTG_E_INVALID, "null argument");
     if (k < 0) return fail(HTG_E_INVALID, "negative k");
-    if (!map->built) return fail(HTG_E_STATE, "map not built");
+    if (int u = usable(map)) return u;
+    std::lock_guard<std::mutex> lk(map->qmu);
     hipError_t e = hipSetDevice(map->ctx->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     if ((e = hipStreamSynchronize(map->stream)) != hipSuccess) return hip_fail(e, "update sync");
@@ -2313,8 +2458,9 @@ int htg_get_tuning(const htg_map* map, htg_tuning* out) {
 int htg_autotune(htg_map* map, const int32_t* rc, const int32_t* ks, int32_t nq, int32_t reps, htg_tuning* out) {
     HTG_GUARD_BEGIN
     if (!map) return fail(HTG_E_INVALID, "null map");
-    if (!map->built) return fail(HTG_E_STATE, "map not built");
+    if (int u = usable(map)) return u;
     if (nq > 0 && (!rc || !ks)) return fail(HTG_E_INVALID, "null sample");
+    std::lock_guard<std::mutex> lk(map->qmu);
     hipError_t e = hipSetDevice(map->ctx->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     if ((e = hipStreamSynchronize(map->stream)) != hipSuccess) return hip_fail(e, "sync");

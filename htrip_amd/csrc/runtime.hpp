@@ -263,6 +263,21 @@ struct Map {
     uint64_t budget = 0, device_bytes = 0;
     htg_tuning tune{64, 4, 0, 0};
     QueryWS qws;
+    // Held by the calls that use the map's own workspaces (qws, lift_*), so
+    // that concurrent queries on one map serialise instead of sharing them.
+    std::mutex qmu;
+    // An update failed after it had changed the cell state: tables may not
+    // match it, so every call but build answers HTG_E_STATE.
+    bool broken = false;
+    // Fault injection (htg_inject_fault): the fault_count-th arrival at
+    // fault_site fails as a HIP error would.
+    std::atomic<int> fault_site{0}, fault_count{0};
+    bool fault_hit(int site) {
+        if (fault_site.load() != site) return false;
+        if (fault_count.fetch_sub(1) != 1) return false;
+        fault_site = 0;
+        return true;
+    }
     std::shared_ptr<Profiler> profp = std::make_shared<Profiler>();
     Profiler& prof_ref() { return *profp; }
     DirtySet dirty;  // filled by open_cells
@@ -342,6 +357,9 @@ struct Map {
     // Make this map equal to src after src applied one update with dirty set
     // ds (this map equal to src before it): copies only the rewritten tables.
     int catch_up_from(const Map& src, const DirtySet& ds);
+    // Make this map equal to src (same geometry) by copying every table and
+    // the cell state; clears `broken`.
+    int copy_all_from(const Map& src);
 };
 
 }  // namespace htg
@@ -360,6 +378,18 @@ struct htg_inc {
 
 struct htg_ctx {
     int device = 0;
+};
+
+// Caller-owned query workspace (HTripQueryScratch,
+// include/htrip/minplus_oracle.hpp:189-242): its own stream and buffers;
+// `after` orders its queries behind the map's update stream.
+struct htg_query_ws {
+    htg_ctx* ctx = nullptr;
+    htg::QueryWS ws;
+    hipEvent_t after = nullptr;
+    ~htg_query_ws() {
+        if (after) (void)hipEventDestroy(after);
+    }
 };
 
 namespace htg {

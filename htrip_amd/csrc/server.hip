@@ -76,6 +76,7 @@ struct Session {
     int64_t submitted_updates = 0;
     int64_t applied = 0;  // update requests applied to the front copy
     bool back_stale = false;
+    bool full_resync = false;  // the back copy must copy every table (its last update was recovered by a rebuild)
     DirtySet last_dirty;
     Clock::time_point last_update{};  // start of the last update batch (snapshot mode)
 };
@@ -127,7 +128,27 @@ struct htg_server {
     void run_updates();
     bool pending_work() const;
     bool update_one(std::unique_lock<std::mutex>& lk);
+    // Applies a merged update batch to m; a failure is recovered by
+    // rebuilding m from its cell state and applying the batch again.
+    int apply_update(htg_map* m, const std::vector<int32_t>& cells, bool& recovered);
 };
+
+int htg_server::apply_update(htg_map* m, const std::vector<int32_t>& cells, bool& recovered) {
+    recovered = false;
+    int st = m->open_cells(cells.data(), int(cells.size() / 2));
+    if (st == HTG_OK) return st;
+    // The cell state already holds the batch (or part of it): rebuild from it,
+    // then open the batch again (cells already open are skipped).
+    m->broken = true;
+    if (m->build() != HTG_OK) return st;
+    const int again = m->open_cells(cells.data(), int(cells.size() / 2));
+    if (again != HTG_OK) {
+        m->broken = true;
+        return again;
+    }
+    recovered = true;
+    return HTG_OK;
+}
 
 bool htg_server::pending_work() const {
     for (auto& kv : sessions)
@@ -191,10 +212,12 @@ void htg_server::slot_loop(Batch* b) {
             htg_map* m = ss.maps[0];
             lk.unlock();
             const auto tu0 = Clock::now();
-            const int st = m->open_cells(cells.data(), int(cells.size() / 2));
+            bool recovered = false;
+            const int st = apply_update(m, cells, recovered);
             const double tu = std::chrono::duration<double>(Clock::now() - tu0).count();
             lk.lock();
             stats.update_seconds += tu;
+            stats.recovered_faults += recovered;
             ss.updating--;
             ss.applied += int64_t(run.size());
             stats.update_batches++;
@@ -272,22 +295,38 @@ bool htg_server::update_one(std::unique_lock<std::mutex>& lk) {
         for (Request* r : run) cells.insert(cells.end(), r->rc.begin(), r->rc.end());
         htg_map* fm = ss.maps[ss.front];
         htg_map* bm = ss.maps[back];
-        const bool stale = ss.back_stale;
+        const bool stale = ss.back_stale, full = ss.full_resync;
         DirtySet prev = ss.last_dirty;
         lk.unlock();
         const auto tu0 = Clock::now();
         ss.last_update = tu0;
-        int st = stale ? bm->catch_up_from(*fm, prev) : HTG_OK;
-        if (st == HTG_OK) st = bm->open_cells(cells.data(), int(cells.size() / 2));
+        int st = !stale ? HTG_OK : full ? bm->copy_all_from(*fm) : bm->catch_up_from(*fm, prev);
+        bool recovered = false;
+        if (st != HTG_OK) {  // the back copy is part way through the catch-up: rebuild it from its cells
+            bm->broken = true;
+            if (bm->copy_all_from(*fm) == HTG_OK) {
+                st = HTG_OK;
+                recovered = true;
+            }
+        }
+        if (st == HTG_OK) {
+            bool r2 = false;
+            st = apply_update(bm, cells, r2);
+            recovered |= r2;
+        }
         const double tu = std::chrono::duration<double>(Clock::now() - tu0).count();
         lk.lock();
         stats.update_seconds += tu;
+        stats.recovered_faults += recovered;
         ss.updating--;
         if (st == HTG_OK) {
             ss.front = back;
             ss.applied += int64_t(run.size());
             ss.last_dirty = bm->dirty;
             ss.back_stale = true;
+            // after a recovery the tables the batch changed are not all in
+            // bm->dirty: the other copy catches up by copying everything
+            ss.full_resync = recovered;
         }
         stats.update_batches++;
         stats.updates += run.size();
@@ -503,6 +542,41 @@ int htg_server_stats_get(htg_server* s, htg_server_stats* out) {
     out->hbm_used = s->used;
     out->sessions = int32_t(s->sessions.size());
     return HTG_OK;
+}
+
+int htg_server_cancel(htg_server* s, uint64_t ticket) {
+    if (!s) return fail(HTG_E_INVALID, "null server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return fail(HTG_E_RANGE, "unknown ticket");
+    Request* r = it->second.get();
+    if (r->done) return fail(HTG_E_STATE, "request already completed");
+    auto si = s->sessions.find(r->session);
+    if (si == s->sessions.end()) return fail(HTG_E_STATE, "request already completed");
+    Session& ss = si->second;
+    for (std::deque<Request*>* q : {&ss.pending, &ss.upd_pending}) {
+        for (auto qi = q->begin(); qi != q->end(); ++qi) {
+            if (*qi != r) continue;
+            q->erase(qi);
+            mark_done(r, HTG_E_CANCELLED, Clock::now());
+            s->cv_done.notify_all();
+            s->cv_work.notify_all();  // a strict update may have waited behind it
+            return HTG_OK;
+        }
+    }
+    return fail(HTG_E_STATE, "request already running");
+}
+
+int htg_server_inject_fault(htg_server* s, int32_t session, int32_t site, int32_t countdown) {
+    if (!s) return fail(HTG_E_INVALID, "null server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->sessions.find(session);
+    if (it == s->sessions.end()) return fail(HTG_E_RANGE, "no such session");
+    Session& ss = it->second;
+    // updates go to copy 0 (strict) or the back copy (snapshot); queries to
+    // copy 0 or the front copy
+    const int mi = !s->snapshot() ? 0 : site == HTG_FAULT_QUERY ? ss.front : 1 - ss.front;
+    return htg_inject_fault(ss.maps[mi], site, countdown);
 }
 
 int htg_server_map(htg_server* s, int32_t session, htg_map** out) {

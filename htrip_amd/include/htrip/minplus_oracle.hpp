@@ -70,11 +70,81 @@ public:
     };
     LastBatchStats last_batch_stats{};
 
-    // Caller-owned query workspace (minplus_oracle.hpp:189-238): the device
-    // workspace belongs to the map; this keeps the host staging of a call.
+    // Caller-owned query workspace (minplus_oracle.hpp:189-238).  It owns a
+    // device workspace (htg_query_ws: stream, plan and lift buffers, result
+    // staging; created by the first query through it and grown, never
+    // shrunk), so that const queryAllPairs calls with distinct scratch may run
+    // on one oracle from several threads at once, as the reference allows.
+    // The public host fields keep the reference's names and types for source
+    // compatibility and are sized by ensureCapacity as there; the GPU query
+    // does not fill them.  A copy starts with a workspace of its own.
     struct HTripQueryScratch {
-        std::vector<int32_t> rc;
-        void ensureCapacity(size_t k, int /*H*/, size_t /*num_leaves*/) { rc.reserve(2 * k); }
+        struct ViewpointLift {
+            Point pt;
+            int leaf_tx = 0, leaf_ty = 0;
+            std::vector<std::vector<dist_t>> X;    // [lvl][port]
+            std::vector<std::vector<int>> active;  // [lvl][idx]
+        };
+        std::vector<ViewpointLift> vp_data;
+        std::vector<std::vector<int>> leaf_vps;
+        std::vector<std::vector<int>> node_vps[2];
+        std::vector<dist_t> proj_buf;
+        std::vector<dist_t> block_buf;
+        std::vector<int> row_marks;
+        std::vector<int> row_list;
+        std::vector<uint8_t> closed;
+        std::vector<uint8_t> vp_new;
+        std::vector<int32_t> rc;  // host staging of the viewpoints
+
+        // Grow-only: k viewpoints with H + 1 lift levels, num_leaves leaves.
+        void ensureCapacity(size_t k, int H, size_t num_leaves) {
+            const size_t levels = size_t(H + 1);
+            if (vp_data.size() < k) vp_data.resize(k);
+            for (size_t i = 0; i < k; ++i) {
+                if (vp_data[i].X.size() < levels) vp_data[i].X.resize(levels);
+                if (vp_data[i].active.size() < levels) vp_data[i].active.resize(levels);
+            }
+            if (leaf_vps.size() < num_leaves) leaf_vps.resize(num_leaves);
+            for (auto& nv : node_vps)
+                if (nv.size() < num_leaves) nv.resize(num_leaves);
+            if (closed.size() < k * k) closed.resize(k * k);
+            if (vp_new.size() < k) vp_new.resize(k);
+            if (rc.capacity() < 2 * k) rc.reserve(2 * k);
+        }
+
+        HTripQueryScratch() = default;
+        HTripQueryScratch(const HTripQueryScratch& o)
+            : vp_data(o.vp_data), leaf_vps(o.leaf_vps), node_vps{o.node_vps[0], o.node_vps[1]},
+              proj_buf(o.proj_buf), block_buf(o.block_buf), row_marks(o.row_marks), row_list(o.row_list),
+              closed(o.closed), vp_new(o.vp_new), rc(o.rc) {}
+        HTripQueryScratch& operator=(const HTripQueryScratch& o) {
+            if (this != &o) *this = HTripQueryScratch(o);
+            return *this;
+        }
+        HTripQueryScratch(HTripQueryScratch&&) noexcept = default;
+        HTripQueryScratch& operator=(HTripQueryScratch&&) noexcept = default;
+
+        // The device workspace (created on first use).
+        htg_query_ws* workspace() {
+            if (!ws_) {
+                htg_query_ws* w = nullptr;
+                detail::check(htg_query_ws_create(detail::device_context(), &w));
+                ws_.reset(w);
+            }
+            return ws_.get();
+        }
+        // Device bytes the workspace holds (0 before the first query).
+        [[nodiscard]] uint64_t deviceBytes() const {
+            uint64_t n = 0;
+            if (ws_) detail::check(htg_query_ws_bytes(ws_.get(), &n));
+            return n;
+        }
+
+    private:
+        struct WsDeleter {
+            void operator()(htg_query_ws* w) const noexcept { htg_query_ws_destroy(w); }
+        };
+        std::unique_ptr<htg_query_ws, WsDeleter> ws_;
     };
 
     MinPlusOracle() = default;
@@ -191,7 +261,10 @@ public:
         epoch++;
     }
 
-    // src/minplus_oracle.cpp:934-1419
+    // src/minplus_oracle.cpp:934-1419.  Runs on the scratch's own device
+    // workspace and stream: distinct scratch objects may query one oracle
+    // from several threads at once (no update may run meanwhile, as in the
+    // reference).
     void queryAllPairs(const std::vector<Point>& viewpoints, const GridMap& /*grid*/, std::vector<dist_t>& out,
                        HTripQueryScratch& scratch) const {
         const size_t k = viewpoints.size();
@@ -206,7 +279,8 @@ public:
             scratch.rc[2 * i] = viewpoints[i].r;
             scratch.rc[2 * i + 1] = viewpoints[i].c;
         }
-        detail::check(htg_query_all_pairs(map_.get(), scratch.rc.data(), int32_t(k), out.data()));
+        detail::check(htg_query_all_pairs_ws(map_.get(), scratch.workspace(), scratch.rc.data(), int32_t(k),
+                                             out.data()));
     }
     // src/minplus_oracle.cpp:1421-1426
     [[nodiscard]] std::vector<dist_t> queryAllPairs(const std::vector<Point>& viewpoints, const GridMap& grid) const {
@@ -214,8 +288,15 @@ public:
         queryAllPairs(viewpoints, grid, m, query_scratch);
         return m;
     }
-    // src/minplus_oracle.cpp:829-928: the matrix query on the pair.
+    // src/minplus_oracle.cpp:829-928.  The early answers are the reference's
+    // (INF for an endpoint that grid does not hold passable or that lies
+    // outside the leaf tiles, 0 for s == t); otherwise the 2-viewpoint
+    // batched query, which lifts both points and meets them at every common
+    // ancestor, as the reference's LCA meet does.
     [[nodiscard]] dist_t queryDistance(Point s, Point t, const GridMap& grid) const {
+        if (!grid.isPassable(s) || !grid.isPassable(t)) return INF;
+        if (s == t) return 0;
+        if (!inLeafRange(s) || !inLeafRange(t)) return INF;
         auto m = queryAllPairs({s, t}, grid);
         return m[1];
     }
@@ -228,6 +309,9 @@ public:
         std::vector<int> child_quad;           // [lvl]: quadrant of the child in that node (-1 at 0)
         std::vector<int> child_offset;         // [lvl]: start of that child's ports in the node's V_U
     };
+    // queryDistance that also returns both lifts (minplus_oracle.hpp:317-319).
+    [[nodiscard]] dist_t queryDistance(Point s, Point t, const GridMap& grid, LiftedViewpoint& X_s,
+                                       LiftedViewpoint& Y_t) const;
 
     // src/minplus_oracle.cpp:731-823.  X comes from the device; the active
     // lists hold the same ports as the reference's, ordered by X and, among
@@ -264,6 +348,13 @@ public:
         }
     }
 
+private:
+    [[nodiscard]] bool inLeafRange(Point p) const noexcept {
+        const int b = config.tile_size_b;
+        return p.r >= 0 && p.c >= 0 && p.c / b < num_leaf_tiles_x && p.r / b < num_leaf_tiles_y;
+    }
+
+public:
     // include/htrip/minplus_oracle.hpp:262-272.  The device-side state
     // (htg_inc) keeps the viewpoints, the matrix and the table clock it was
     // taken at; the public fields mirror the reference's for callers that
@@ -347,6 +438,33 @@ public:
         return size_t(info.total_memory_bytes);
     }
 
+    // src/minplus_oracle.cpp:2136-2182: the leaf D + S and node D_U + S_U
+    // bytes of the tiles that lie inside grid and hold at least one Free cell
+    // of it (child_to_ext is not counted, as there).
+    [[nodiscard]] size_t effectiveMemoryBytes(const GridMap& grid) const {
+        if (!map_) return 0;
+        auto has_free = [&grid](int R0, int C0, int side) {
+            if (R0 >= grid.rows || C0 >= grid.cols) return false;
+            const int r1 = std::min(grid.rows, R0 + side), c1 = std::min(grid.cols, C0 + side);
+            for (int r = R0; r < r1; ++r)
+                for (int c = C0; c < c1; ++c)
+                    if (grid.getObserved({r, c}) == CellState::Free) return true;
+            return false;
+        };
+        const size_t b = size_t(config.tile_size_b), pl = 4 * b - 4;
+        size_t total = 0;
+        for (const LeafTile& l : leaves)
+            if (has_free(l.R0, l.C0, l.b)) total += (b * b * b * b + pl * pl) * sizeof(dist_t);
+        for (int lvl = 1; lvl <= H; ++lvl) {
+            int32_t nn = 0, nv = 0, ne = 0;
+            detail::check(htg_node_dims(map_.get(), lvl, &nn, &nv, &ne));
+            for (const InternalNode& n : levels[size_t(lvl - 1)])
+                if (has_free(n.R0, n.C0, n.s))
+                    total += (size_t(nv) * size_t(nv) + size_t(ne) * size_t(ne)) * sizeof(dist_t);
+        }
+        return total;
+    }
+
     // Table copies (the reference exposes LeafTile::D and InternalNode::D_U).
     [[nodiscard]] std::vector<dist_t> readLeafD(int leaf) const {
         const size_t n = size_t(config.tile_size_b) * config.tile_size_b;
@@ -404,6 +522,16 @@ private:
         last_batch_stats = o.last_batch_stats;
     }
 };
+
+inline dist_t MinPlusOracle::queryDistance(Point s, Point t, const GridMap& grid, LiftedViewpoint& X_s,
+                                           LiftedViewpoint& Y_t) const {
+    if (!grid.isPassable(s) || !grid.isPassable(t)) return INF;
+    if (s == t) return 0;
+    if (!inLeafRange(s) || !inLeafRange(t)) return INF;
+    liftPoint(s, X_s);
+    liftPoint(t, Y_t);
+    return queryDistance(s, t, grid);
+}
 
 using HTripOracle = MinPlusOracle;  // minplus_oracle.hpp:411
 

@@ -31,7 +31,8 @@ class ServerConfig(C.Structure):
 
 class ServerStats(C.Structure):
     _fields_ = [("query_batches", C.c_uint64), ("queries", C.c_uint64), ("update_batches", C.c_uint64),
-                ("updates", C.c_uint64), ("hbm_used", C.c_uint64), ("sessions", C.c_int32), ("reserved", C.c_int32),
+                ("updates", C.c_uint64), ("hbm_used", C.c_uint64), ("sessions", C.c_int32),
+                ("recovered_faults", C.c_int32),
                 ("plan_seconds", C.c_double), ("update_seconds", C.c_double), ("gpu_wait_seconds", C.c_double),
                 ("finish_seconds", C.c_double)]
 
@@ -54,6 +55,8 @@ def _bind(lib):
         "htg_server_stats_get": (C.c_int, [vp, C.POINTER(ServerStats)]),
         "htg_server_map": (C.c_int, [vp, i32, C.POINTER(vp)]),
         "htg_server_replay": (C.c_int, [vp, i32, pf64, pi32, pi32, pi32, pi64, pi32, pi64, pu16, pf64, pi64]),
+        "htg_server_cancel": (C.c_int, [vp, u64]),
+        "htg_server_inject_fault": (C.c_int, [vp, i32, i32, i32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -64,7 +67,8 @@ def _bind(lib):
 
 SERVER_EXPORTS = ["htg_server_create", "htg_server_destroy", "htg_server_add_map", "htg_server_remove_map",
                   "htg_server_submit_query", "htg_server_submit_update", "htg_server_wait", "htg_server_wait_ex",
-                  "htg_server_stats_get", "htg_server_map", "htg_server_replay"]
+                  "htg_server_stats_get", "htg_server_map", "htg_server_replay", "htg_server_cancel",
+                  "htg_server_inject_fault"]
 
 
 class Server:
@@ -130,7 +134,23 @@ class Server:
     def stats(self) -> dict:
         s = ServerStats()
         check(self.lib.htg_server_stats_get(self.h, C.byref(s)))
-        return {f: getattr(s, f) for f, _ in ServerStats._fields_ if f != "reserved"}
+        return {f: getattr(s, f) for f, _ in ServerStats._fields_}
+
+    def cancel(self, ticket: int) -> int:
+        """htg_server_cancel: HTG_OK when the request was cancelled before it
+        ran, else the (negative) reason code; never raises."""
+        return int(self.lib.htg_server_cancel(self.h, ticket))
+
+    def inject_fault(self, sid: int, site: int, countdown: int = 1) -> None:
+        check(self.lib.htg_server_inject_fault(self.h, sid, site, countdown))
+
+    def wait_status(self, ticket: int) -> int:
+        """Waits for a request and returns its status code instead of raising."""
+        lat = C.c_double()
+        ver = C.c_int64()
+        rc = self.lib.htg_server_wait_ex(self.h, ticket, C.byref(lat), C.byref(ver))
+        self._keep.pop(ticket, None)
+        return int(rc)
 
     def map_of(self, sid: int, cells: np.ndarray, b: int, g: int = 2) -> _lib.GpuMap:
         """A non-owning GpuMap view of a session (for reading tables between requests)."""

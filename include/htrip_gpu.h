@@ -12,11 +12,16 @@
  * Cell codes follow include/htrip/types.hpp:101-105: 0 Unknown, 1 Free,
  * 2 Obstacle; only Free cells are passable (include/htrip/grid_map.hpp:142-145).
  *
- * Threading: a map handle is used by one host thread at a time (like the
- * reference, whose const queries share mutable scratch,
- * include/htrip/minplus_oracle.hpp:240-242).  Different handles may be used
- * concurrently from different threads.  The server API owns its maps'
- * ordering itself.
+ * Threading: updates (build, open, block, lift_root) of a map handle are used
+ * by one host thread at a time, with no query running on the map, as in the
+ * reference.  Queries that go through caller-owned workspaces
+ * (htg_query_ws_*, the counterpart of HTripQueryScratch,
+ * include/htrip/minplus_oracle.hpp:189-242, :337-341) may run on one map from
+ * several threads at once, one workspace per thread.  The queries that use
+ * the map's own workspace (htg_query_all_pairs, htg_query_batch,
+ * htg_query_all_pairs_incremental, htg_lift_point) serialise on a per-map
+ * lock.  Different handles may be used concurrently from different threads.
+ * The server API owns its maps' ordering itself.
  */
 #ifndef HTRIP_GPU_H
 #define HTRIP_GPU_H
@@ -35,13 +40,16 @@ enum {
     HTG_E_INVALID = -1, /* bad argument or configuration */
     HTG_E_NOMEM = -2,   /* admission refused: HBM budget or device memory */
     HTG_E_HIP = -3,     /* HIP runtime error */
-    HTG_E_STATE = -4,   /* call not valid in the handle's state (e.g. query before build) */
-    HTG_E_RANGE = -5    /* index out of range */
+    HTG_E_STATE = -4,   /* call not valid in the handle's state (e.g. query before build, or a map
+                           whose last update failed: it needs htg_build) */
+    HTG_E_RANGE = -5,   /* index out of range */
+    HTG_E_CANCELLED = -6 /* server: the request was cancelled before it ran */
 };
 
 typedef struct htg_ctx htg_ctx;
 typedef struct htg_map htg_map;
 typedef struct htg_inc htg_inc;
+typedef struct htg_query_ws htg_query_ws;
 typedef struct htg_server htg_server;
 
 /* include/htrip/types.hpp:199-225 (HierarchyConfig): tile_size_b, group_w,
@@ -162,6 +170,38 @@ int htg_query_batch(htg_map* map, const int32_t* rc, const int32_t* ks, int32_t 
 int htg_last_batch_stats(const htg_map* map, htg_batch_stats* out);
 int htg_sync(htg_map* map);
 
+/* ---- caller-owned query workspace --------------------------------------
+ * MinPlusOracle::HTripQueryScratch (include/htrip/minplus_oracle.hpp:189-242)
+ * and the const queryAllPairs that takes it (:337-341): a reusable,
+ * grow-only workspace (plan, per-level lift and projection buffers, result
+ * staging) with a stream of its own.  Queries through distinct workspaces
+ * may run on one map from several threads at once (the reference's callers
+ * do so with distinct scratch, src/robot_simulator.cpp:1541-1546, :1768); a
+ * workspace serves one call at a time.  A query through a workspace sees
+ * every update the map finished before the call (its stream waits for the
+ * map's update stream).  A workspace may serve any map of its context's
+ * device. */
+int htg_query_ws_create(htg_ctx* ctx, htg_query_ws** out);
+int htg_query_ws_destroy(htg_query_ws* ws);
+/* Device bytes the workspace holds (it grows to the largest call so far). */
+int htg_query_ws_bytes(const htg_query_ws* ws, uint64_t* bytes);
+int htg_query_all_pairs_ws(htg_map* map, htg_query_ws* ws, const int32_t* rc, int32_t k, uint16_t* out);
+int htg_query_batch_ws(htg_map* map, htg_query_ws* ws, const int32_t* rc, const int32_t* ks, int32_t nq,
+                       uint16_t* out);
+
+/* ---- fault injection (test hook) ----------------------------------------
+ * The countdown-th time (1 = the next time) the map reaches `site`, the call
+ * fails there with HTG_E_HIP as a HIP error would.  countdown 0 disarms.
+ * An update that fails this way (or by a real error) has already changed
+ * the map's cell state, so the map answers HTG_E_STATE until htg_build
+ * rebuilds it from that state.  No counterpart in the reference. */
+enum {
+    HTG_FAULT_UPDATE = 1, /* htg_open_cells / htg_open_cell, after its first level's kernels were enqueued */
+    HTG_FAULT_QUERY = 2,  /* a query launch, before any kernel of the batch */
+    HTG_FAULT_BUILD = 3   /* htg_build, before the first kernel */
+};
+int htg_inject_fault(htg_map* map, int32_t site, int32_t countdown);
+
 /* ---- table access (the reference keeps these as public members:
  * LeafTile::D/S, InternalNode::D_U/S_U, include/htrip/leaf_tile.hpp,
  * include/htrip/internal_node.hpp).  Copies to host memory. --------------- */
@@ -233,7 +273,8 @@ typedef struct htg_server_config {
 } htg_server_config;
 typedef struct htg_server_stats {
     uint64_t query_batches, queries, update_batches, updates, hbm_used;
-    int32_t sessions, reserved;
+    int32_t sessions;
+    int32_t recovered_faults; /* update batches that failed and were recovered by rebuilding the copy */
     double plan_seconds;   /* worker-thread time spent planning and launching query batches */
     double update_seconds; /* time spent applying update batches (catch-up included) */
     double gpu_wait_seconds; /* slot time from the last launch until the batch's results were on the host */
@@ -256,6 +297,17 @@ int htg_server_wait(htg_server* s, uint64_t ticket, double* latency_s);
  * submission order) its result includes; for an update, its index. */
 int htg_server_wait_ex(htg_server* s, uint64_t ticket, double* latency_s, int64_t* version);
 int htg_server_stats_get(htg_server* s, htg_server_stats* out);
+/* Cancels a request that has not started: it completes with HTG_E_CANCELLED
+ * (htg_server_wait returns that code) and has no effect on its session; a
+ * cancelled update is never applied.  HTG_E_STATE when the request already
+ * runs or has completed, HTG_E_RANGE for an unknown ticket.  A failed update
+ * batch is recovered by rebuilding the session's copy from its cell state
+ * (which includes the batch): its requests complete with HTG_OK and
+ * recovered_faults counts it; other sessions are not touched. */
+int htg_server_cancel(htg_server* s, uint64_t ticket);
+/* Test hook: arms htg_inject_fault on the session's map copies (both copies
+ * in snapshot mode: the next update of either). */
+int htg_server_inject_fault(htg_server* s, int32_t session, int32_t site, int32_t countdown);
 /* The session's map, for reading tables; do not use it while requests run. */
 int htg_server_map(htg_server* s, int32_t session, htg_map** out);
 /* Trace replay: submits request i at t_arrival[i] seconds after the call

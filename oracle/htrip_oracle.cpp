@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 namespace orc {
@@ -659,41 +660,57 @@ struct Oracle {
 
 // src/bfs_oracle.cpp:79-202, with a visited bitmap so that a distance that
 // reaches INF cannot be mistaken for "unvisited"; reports min(d, INF).
-static void bfs_all_pairs(const Grid& G, const std::vector<Pt>& vp, dist_t* out) {
+// Row i of out (k entries) from source i; the buffers belong to the caller.
+static void bfs_row(const Grid& G, const std::vector<Pt>& vp, size_t i, dist_t* out, std::vector<uint32_t>& dist,
+                    std::vector<uint8_t>& seen, std::vector<int>& q) {
     const size_t k = vp.size();
-    std::fill(out, out + k * k, INF);
-    const size_t cells = size_t(G.rows) * G.cols;
-    std::vector<uint32_t> dist(cells);
-    std::vector<uint8_t> seen(cells);
-    std::vector<int> q;
-    q.reserve(cells);
-    for (size_t i = 0; i < k; ++i) {
-        if (!G.passable(vp[i].r, vp[i].c)) continue;
-        std::fill(seen.begin(), seen.end(), 0);
-        q.clear();
-        int s = vp[i].r * G.cols + vp[i].c;
-        seen[s] = 1;
-        dist[s] = 0;
-        q.push_back(s);
-        for (size_t h = 0; h < q.size(); ++h) {
-            int u = q[h], ur = u / G.cols, uc = u % G.cols;
-            const int dr[4] = {-1, 1, 0, 0}, dc[4] = {0, 0, -1, 1};
-            for (int d = 0; d < 4; ++d) {
-                int vr = ur + dr[d], vc = uc + dc[d];
-                if (!G.passable(vr, vc)) continue;
-                int v = vr * G.cols + vc;
-                if (seen[v]) continue;
-                seen[v] = 1;
-                dist[v] = dist[u] + 1;
-                q.push_back(v);
-            }
-        }
-        for (size_t j = 0; j < k; ++j) {
-            if (!G.passable(vp[j].r, vp[j].c)) continue;
-            int t = vp[j].r * G.cols + vp[j].c;
-            if (seen[t]) out[i * k + j] = dist[t] >= INF ? INF : dist_t(dist[t]);
+    std::fill(out + i * k, out + (i + 1) * k, INF);
+    if (!G.passable(vp[i].r, vp[i].c)) return;
+    std::fill(seen.begin(), seen.end(), 0);
+    q.clear();
+    int s = vp[i].r * G.cols + vp[i].c;
+    seen[s] = 1;
+    dist[s] = 0;
+    q.push_back(s);
+    for (size_t h = 0; h < q.size(); ++h) {
+        int u = q[h], ur = u / G.cols, uc = u % G.cols;
+        const int dr[4] = {-1, 1, 0, 0}, dc[4] = {0, 0, -1, 1};
+        for (int d = 0; d < 4; ++d) {
+            int vr = ur + dr[d], vc = uc + dc[d];
+            if (!G.passable(vr, vc)) continue;
+            int v = vr * G.cols + vc;
+            if (seen[v]) continue;
+            seen[v] = 1;
+            dist[v] = dist[u] + 1;
+            q.push_back(v);
         }
     }
+    for (size_t j = 0; j < k; ++j) {
+        if (!G.passable(vp[j].r, vp[j].c)) continue;
+        int t = vp[j].r * G.cols + vp[j].c;
+        if (seen[t]) out[i * k + j] = dist[t] >= INF ? INF : dist_t(dist[t]);
+    }
+}
+
+// All pairs, sources spread over `threads` threads (rows are independent).
+static void bfs_all_pairs(const Grid& G, const std::vector<Pt>& vp, dist_t* out, int threads = 1) {
+    const size_t k = vp.size();
+    const size_t cells = size_t(G.rows) * G.cols;
+    const int nt = int(std::max<size_t>(1, std::min<size_t>(size_t(std::max(threads, 1)), k)));
+    auto work = [&](int t) {
+        std::vector<uint32_t> dist(cells);
+        std::vector<uint8_t> seen(cells);
+        std::vector<int> q;
+        q.reserve(cells);
+        for (size_t i = size_t(t); i < k; i += size_t(nt)) bfs_row(G, vp, i, out, dist, seen, q);
+    };
+    if (nt == 1) {
+        work(0);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; ++t) pool.emplace_back(work, t);
+    for (auto& th : pool) th.join();
 }
 
 }  // namespace orc
@@ -751,6 +768,14 @@ void orc_grid_bfs(int rows, int cols, const uint8_t* cells, const int32_t* rc, i
     G.cols = cols;
     G.obs.assign(cells, cells + size_t(rows) * cols);
     bfs_all_pairs(G, to_pts(rc, k), out);
+}
+// Same, with the sources spread over `threads` threads.
+void orc_grid_bfs_mt(int rows, int cols, const uint8_t* cells, const int32_t* rc, int k, uint16_t* out, int threads) {
+    Grid G;
+    G.rows = rows;
+    G.cols = cols;
+    G.obs.assign(cells, cells + size_t(rows) * cols);
+    bfs_all_pairs(G, to_pts(rc, k), out, threads);
 }
 // BFS restricted to the rectangle [R0,R0+h) x [C0,C0+w) from (sr,sc); out is h*w.
 // Source distance 0 even when the source is impassable (leaf_tile.cpp:63-65).

@@ -83,6 +83,7 @@ def _lib(kind: str):
     if kind == "port":
         api["region_bfs"] = fn("region_bfs", None, vp, ci, ci, ci, ci, ci, ci, _u16p)
         api["grid_bfs"] = fn("grid_bfs", None, ci, ci, _u8p, _i32p, ci, _u16p)
+        api["grid_bfs_mt"] = fn("grid_bfs_mt", None, ci, ci, _u8p, _i32p, ci, _u16p, ci)
     else:
         api["import_begin"] = fn("import_begin", None, vp)
         api["import_leaf"] = fn("import_leaf", None, vp, ci, _u16p)
@@ -95,6 +96,7 @@ def _lib(kind: str):
         api["inc_create"] = fn("inc_create", vp)
         api["inc_destroy"] = fn("inc_destroy", None, vp)
         api["inc_query"] = fn("inc_query", C.c_double, vp, vp, _i32p, ci, _u16p)
+        api["effective_memory_bytes"] = fn("effective_memory_bytes", C.c_uint64, vp, ci, ci, _u8p)
     _libs[kind] = api
     return api
 
@@ -131,14 +133,27 @@ def reference_frontiers(obs: np.ndarray, min_size: int = 2, max_size: int = 16):
     return clr, front.astype(bool), out
 
 
-def grid_bfs(cells: np.ndarray, vps: np.ndarray) -> np.ndarray:
-    """Global BFS all-pairs on a cell array: min(d, INF), INF for impassable endpoints."""
+def bfs_threads() -> int:
+    """Host threads for the BFS checker: the CPU share (at most 16)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def grid_bfs(cells: np.ndarray, vps: np.ndarray, threads: int = 1) -> np.ndarray:
+    """Global BFS all-pairs on a cell array: min(d, INF), INF for impassable
+    endpoints; threads > 1 spreads the sources over host threads (same rows)."""
     api = _lib("port")
     cells = np.ascontiguousarray(cells, dtype=np.uint8)
     vps = np.ascontiguousarray(vps, dtype=np.int32).reshape(-1, 2)
     k = len(vps)
     out = np.empty((k, k), dtype=np.uint16)
-    api["grid_bfs"](cells.shape[0], cells.shape[1], cells, vps, k, out)
+    if threads > 1:
+        api["grid_bfs_mt"](cells.shape[0], cells.shape[1], cells, vps, k, out, threads)
+    else:
+        api["grid_bfs"](cells.shape[0], cells.shape[1], cells, vps, k, out)
     return out
 
 

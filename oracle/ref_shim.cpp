@@ -109,6 +109,12 @@ void ref_last_batch_stats(void* p, int32_t* out3) {
     out3[2] = s.n_pivots;
 }
 uint64_t ref_total_memory_bytes(void* p) { return static_cast<RefHandle*>(p)->oracle.totalMemoryBytes(); }
+// effectiveMemoryBytes over a caller-given grid (rows x cols cell codes).
+uint64_t ref_effective_memory_bytes(void* p, int rows, int cols, const uint8_t* cells) {
+    GridMap g(rows, cols, CellState::Unknown);
+    for (size_t i = 0; i < size_t(rows) * cols; ++i) g.observed[i] = static_cast<CellState>(cells[i]);
+    return static_cast<RefHandle*>(p)->oracle.effectiveMemoryBytes(g);
+}
 
 // ---- Table import: lay out the reference hierarchy from tables computed
 // elsewhere (the GPU), so that the reference query path can be timed on

@@ -56,3 +56,51 @@ def test_cpp_dropin_header_compiles(tmp_path):
 
     build.build_dropin_test(str(out))
     assert out.exists()
+
+
+def test_query_argument_checks_need_no_device():
+    """Bad query arguments are refused before any device work: missing
+    viewpoints for k > 0, negative k, null workspace handles."""
+    lib = _lib.load()
+    ks = (C.c_int32 * 2)(3, 2)
+    out = (C.c_uint16 * 16)()
+    assert lib.htg_query_batch(None, None, ks, 2, out) == _lib.HTG_E_INVALID
+    assert "null viewpoints" in lib.htg_last_error().decode()
+    neg = (C.c_int32 * 1)(-1)
+    rc = (C.c_int32 * 2)(0, 0)
+    assert lib.htg_query_batch(None, rc, neg, 1, out) == _lib.HTG_E_INVALID
+    assert "negative k" in lib.htg_last_error().decode()
+    assert lib.htg_query_batch_ws(None, None, None, ks, 2, out) == _lib.HTG_E_INVALID
+    assert lib.htg_query_all_pairs_ws(None, None, rc, 1, out) == _lib.HTG_E_INVALID
+    h = C.c_void_p()
+    assert lib.htg_query_ws_create(None, C.byref(h)) == _lib.HTG_E_INVALID
+    assert lib.htg_inject_fault(None, _lib.FAULT_UPDATE, 1) == _lib.HTG_E_INVALID
+    n = C.c_uint64()
+    assert lib.htg_query_ws_bytes(None, C.byref(n)) == _lib.HTG_E_INVALID
+    from htrip_amd import serving
+
+    serving._bind(lib)
+    assert lib.htg_server_cancel(None, 1) == _lib.HTG_E_INVALID
+    assert lib.htg_server_inject_fault(None, 1, 1, 1) == _lib.HTG_E_INVALID
+
+
+def test_query_front_refuses_more_than_64_fill_ranges():
+    """The front kernel finds a block's fill range with one 64-lane ballot, so
+    its launcher refuses more ranges (returns false, launches nothing) rather
+    than leaving gaps uninitialised.  Called with no device: the guard comes
+    before any HIP call."""
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True).stdout
+    sym = [ln.split()[-1] for ln in out.splitlines() if "launch_q_front" in ln]
+    assert len(sym) == 1
+    lib = C.CDLL(_lib.LIB_PATH)
+    f = getattr(lib, sym[0])
+    f.restype = C.c_bool
+    vp, ci = C.c_void_p, C.c_int
+    # (nq, q_k, q_out, q_vp, vp_rc, vp_valid, kmax, n_slots, slot_rc, b, nl, leafD, out, X0, n_fill, fills,
+    #  fill_blocks, fill_base, wstat, stream)
+    f.argtypes = [ci, vp, vp, vp, vp, vp, ci, ci, vp, ci, ci, vp, vp, vp, ci, vp, ci, vp, vp, vp]
+    args = lambda n_fill, nq: [nq, None, None, None, None, None, 4, 0, None, 8, 1, None, None, None, n_fill, None,  # noqa
+                               0, None, None, None]
+    assert f(*args(65, 1)) is False
+    assert f(*args(-1, 1)) is False
+    assert f(*args(64, 0)) is True  # in range, nothing to launch
