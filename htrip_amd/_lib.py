@@ -27,7 +27,7 @@ EXPORTS = [
     "htg_profile_enable", "htg_profile_get",
     "htg_get_tuning", "htg_set_tuning", "htg_autotune",
     "htg_query_ws_create", "htg_query_ws_destroy", "htg_query_ws_bytes", "htg_query_all_pairs_ws",
-    "htg_query_batch_ws", "htg_inject_fault",
+    "htg_query_batch_ws", "htg_inject_fault", "htg_query_batch_noalloc",
 ]
 
 K_LEAF_APSP, K_CLOSURE_PIVOTS, K_CLOSURE_UPDATE, K_Q_PROJECT, K_Q_MEET, K_Q_LIFT, K_COVER_RELAX, K_COVER_PAIRS = range(8)
@@ -122,6 +122,7 @@ def load(path: str = LIB_PATH):
         "htg_query_all_pairs_ws": (C.c_int, [vp, vp, pi32, i32, pu16]),
         "htg_query_batch_ws": (C.c_int, [vp, vp, pi32, pi32, i32, pu16]),
         "htg_inject_fault": (C.c_int, [vp, i32, i32]),
+        "htg_query_batch_noalloc": (C.c_int, [vp, pi32, pi32, i32, pu16]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -1438,9 +1438,13 @@ int Map::query_finish(QueryWS& ws, uint16_t* out_host) {
     return HTG_OK;
 }
 
-int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws, bool to_host) {
-    hipError_t e;
-    int rc;
+// Host planning of one query batch into P, with the packed plan and the
+// sizes of its device regions in B.  Both are reused from call to call
+// (their vectors keep their capacity), so planning allocates nothing once
+// they have grown to the batch sizes in use.  host_sub (nullable, 4 entries)
+// accumulates the planning sub-phase times.
+int Map::plan_batch(const int32_t* rcs, const int32_t* ks, int nq, QueryPlan& P, BatchLayout& B, bool counting,
+                    double* host_sub) const {
     const auto th0 = std::chrono::steady_clock::now();
     long long total_vp = 0, total_out = 0;
     int kmax = 0;
@@ -1450,16 +1454,11 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
         total_out += (long long)ks[q] * ks[q];
         kmax = std::max(kmax, ks[q]);
     }
-    ws.total_out = total_out;
-    if (total_out == 0) {
-        hipError_t e0 = hipEventRecord(ws.done, ws.stream);
-        return e0 == hipSuccess ? HTG_OK : hip_fail(e0, "record");
-    }
+    B.total_vp = total_vp;
+    B.total_out = total_out;
+    if (total_out == 0) return HTG_OK;
     // ---- plan
-    QueryPlan& P = ws.plan;
     P.clear();
-    ws.counted = prof_ref().take();  // this batch carries kernel timing
-    const bool counting = ws.counted;
     P.q_k.assign(ks, ks + nq);
     P.q_out.resize(nq);
     P.q_vp.resize(nq);
@@ -1551,7 +1550,10 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
     const int g2 = g * g;
     const size_t nsl = size_t(std::max(ns, 1));
     // Q layout: X_0 .. X_{H-1}, then Z_1 .. Z_H
-    std::vector<size_t> xoff(size_t(H) + 1, 0), zoffq(size_t(H) + 1, 0);
+    std::vector<size_t>& xoff = B.xoff;
+    std::vector<size_t>& zoffq = B.zoffq;
+    xoff.assign(size_t(H) + 1, 0);
+    zoffq.assign(size_t(H) + 1, 0);
     size_t qsz = 0;
     auto a8 = [](size_t x) { return (x + 7) / 8 * 8; };
     for (int l = 0; l < H; ++l) {
@@ -1680,31 +1682,114 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
         kc.swap(kw);
     }
     const auto th1 = std::chrono::steady_clock::now();
-    ws.host_sub[0] += std::chrono::duration<double>(tk0 - th0).count();  // validation, keys
-    ws.host_sub[1] += std::chrono::duration<double>(tk1 - tk0).count();  // sort
-    ws.host_sub[2] += t_meet;                                            // meet tiles
-    ws.host_sub[3] += std::chrono::duration<double>(th1 - tk1).count() - t_meet;  // layout, items
+    if (host_sub) {
+        host_sub[0] += std::chrono::duration<double>(tk0 - th0).count();  // validation, keys
+        host_sub[1] += std::chrono::duration<double>(tk1 - tk0).count();  // sort
+        host_sub[2] += t_meet;                                            // meet tiles
+        host_sub[3] += std::chrono::duration<double>(th1 - tk1).count() - t_meet;  // layout, items
+    }
     // ---- upload (one packed staging buffer)
-    Packer pk;
-    const size_t o_qk = pk.add(P.q_k), o_qout = pk.add(P.q_out), o_qvp = pk.add(P.q_vp), o_vprc = pk.add(P.vp_rc),
-                 o_vpvalid = pk.add(P.vp_valid), o_src = pk.add(P.slot_rc), o_si = pk.add(P.slot_i),
-                 o_tiles = pk.add(P.mtiles), o_glist = pk.add(P.glist),
-                 o_zgeo = pk.add(P.zgeo), o_fills = pk.add(P.fills);
-    std::vector<size_t> o_items(H);
+    Packer& pk = B.pk;
+    pk.clear();
+    B.o_qk = pk.add(P.q_k), B.o_qout = pk.add(P.q_out), B.o_qvp = pk.add(P.q_vp), B.o_vprc = pk.add(P.vp_rc);
+    B.o_vpvalid = pk.add(P.vp_valid), B.o_src = pk.add(P.slot_rc), B.o_si = pk.add(P.slot_i);
+    B.o_tiles = pk.add(P.mtiles), B.o_glist = pk.add(P.glist), B.o_zgeo = pk.add(P.zgeo), B.o_fills = pk.add(P.fills);
+    B.o_items.assign(size_t(H), 0);
     size_t maxpart = 0;
     for (int l = 0; l < H; ++l) {
-        o_items[l] = pk.add(P.levels[l].items);
-        maxpart = std::max(maxpart, size_t(P.levels[l].ksplit) * lv[l].ldp);
+        B.o_items[size_t(l)] = pk.add(P.levels[size_t(l)].items);
+        maxpart = std::max(maxpart, size_t(P.levels[size_t(l)].ksplit) * lv[size_t(l)].ldp);
     }
+    B.kmax = kmax;
+    B.ns = ns;
+    B.nsl = nsl;
+    B.qsz = qsz;
+    B.maxpart = maxpart;
+    B.fill_blocks = fill_blocks;
+    if (P.fills.size() > 64) return fail(HTG_E_INVALID, "query plan has more than 64 fill ranges");
+    return HTG_OK;
+}
+
+// The launches of one planned batch on stream st: front (output init,
+// level-0 lift, INF fills), per level the projection (and the fold of its
+// row chunks), then the meet of every level.  dp holds B.pk's packed plan;
+// Qb, Zp and dout have B's sizes.
+int Map::enqueue_batch(const QueryPlan& P, const BatchLayout& B, const char* dp, uint16_t* Qb, uint16_t* Zp,
+                       uint16_t* dout, unsigned long long* wstat, bool counted, hipStream_t st) {
+    auto I = [&](size_t o) { return reinterpret_cast<const int*>(dp + o); };
+    const uint16_t* dleafD = static_cast<const uint16_t*>(leafD.p);
+    const int nq = int(P.q_k.size());
+    const int g2 = g * g;
+    const int2* srcs = reinterpret_cast<const int2*>(dp + B.o_src);
+    if (!launch_q_front(nq, I(B.o_qk), reinterpret_cast<const long long*>(dp + B.o_qout), I(B.o_qvp),
+                        reinterpret_cast<const int2*>(dp + B.o_vprc), reinterpret_cast<const uint8_t*>(dp + B.o_vpvalid),
+                        B.kmax, H > 0 ? B.ns : 0, srcs, b, nl, dleafD, dout, Qb + B.xoff[0], int(P.fills.size()),
+                        reinterpret_cast<const int4*>(dp + B.o_fills), B.fill_blocks, Qb, wstat, st))
+        return fail(HTG_E_INVALID, "query front: more than 64 fill ranges");
+    for (int l = 1; l <= H && B.ns > 0; ++l) {
+        const LevelGeo& L = lv[l - 1];
+        const LevelPlan& LP = P.levels[l - 1];
+        const uint16_t* Xl = Qb + B.xoff[size_t(l) - 1];            // X of this level (stride pc)
+        uint16_t* Xn = l < H ? Qb + B.xoff[size_t(l)] : nullptr;  // lifted X (stride ne)
+        uint16_t* Zl = Qb + B.zoffq[size_t(l)];
+        const int ne = l < H ? L.d.ne : 0;  // no lift above the root
+        // bytes and updates of the projection are counted on the device
+        KTimer kt;
+        const int pi = counted ? prof_ref().open(HTG_K_Q_PROJECT, 0.0, 0.0, kt, st) : -1;
+        const size_t part_stride = B.nsl * L.ldp;
+        const bool la = launch_q_z(LP.M, L.d.ld, L.d.pc, static_cast<const uint16_t*>(L.DU.p), L.node_stride,
+                   static_cast<const int*>(L.d_seam_list.p), static_cast<const int*>(L.d_zcols.p), L.ngrp,
+                   L.d_live_list, LP.col_tiles, L.ldp,
+                   reinterpret_cast<const int4*>(dp + B.o_items[size_t(l) - 1]), int(LP.items.size() / 2), LP.ksplit, Xl,
+                   Zp, part_stride, static_cast<const int*>(L.d_colpos.p),
+                   static_cast<const int*>(L.d_colext.p), static_cast<const int*>(L.d_ext_child.p),
+                   static_cast<const int*>(L.d_ext_cport.p), Zl, L.ldz, Xn, ne, wstat, st, kt);
+        prof_ref().close(pi, la);
+        if (LP.ksplit > 1) {
+            const int pf =
+                counted ? prof_ref().open(HTG_K_Q_LIFT, double(B.ns) * (L.nz + 2.0 * ne + L.d.pc) * 2, 0, kt, st) : -1;
+            const bool lf = launch_q_fold(B.ns, L.nz, L.ldz, static_cast<const int*>(L.d_zsrc.p), L.ldp, LP.ksplit,
+                          Zp, part_stride, srcs, L.d.s, L.d.nt, L.ngrp,
+                          L.d_live_map, Zl, ne, static_cast<const int*>(L.d_ext_col.p),
+                          static_cast<const int*>(L.d_ext_child.p), static_cast<const int*>(L.d_ext_cport.p), L.d.pc,
+                          L.d.cs, g, Xl, Xn, st, kt);
+            prof_ref().close(pf, lf);
+        }
+    }
+    if (B.ns > 0 && !P.mtiles.empty()) {
+        KTimer kt;
+        const int pm = counted ? prof_ref().open(HTG_K_Q_MEET, P.m_bytes, P.m_updates, kt, st) : -1;
+        const bool lm = launch_q_meet(I(B.o_tiles), int(P.mtiles.size() / 8), I(B.o_glist), I(B.o_zgeo), g2, I(B.o_si), I(B.o_qk),
+                                      reinterpret_cast<const long long*>(dp + B.o_qout), Qb, dout, st, kt);
+        prof_ref().close(pm, lm);
+    }
+    hipError_t e;
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "query launch");
+    return HTG_OK;
+}
+
+int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws, bool to_host) {
+    hipError_t e;
+    int rc;
+    const auto th0 = std::chrono::steady_clock::now();
+    QueryPlan& P = ws.plan;
+    BatchLayout& B = ws.lay;
+    ws.counted = prof_ref().take();  // this batch carries kernel timing
+    if ((rc = plan_batch(rcs, ks, nq, P, B, ws.counted, ws.host_sub))) return rc;
+    ws.total_out = B.total_out;
+    if (B.total_out == 0) {
+        ws.counted = false;
+        hipError_t e0 = hipEventRecord(ws.done, ws.stream);
+        return e0 == hipSuccess ? HTG_OK : hip_fail(e0, "record");
+    }
+    const auto th1 = std::chrono::steady_clock::now();
+    // ---- upload (one packed staging buffer)
+    const Packer& pk = B.pk;
     if ((rc = ws.h_plan.ensure(pk.size))) return rc;
     pk.write(static_cast<char*>(ws.h_plan.p));
-    if ((rc = ws.d_plan.ensure(pk.size))) return rc;
-    if ((rc = ws.Q.ensure(std::max<size_t>(qsz, 8) * 2))) return rc;
-    if ((rc = ws.Zp.ensure(nsl * std::max<size_t>(maxpart, 8) * 2))) return rc;
-    // the meet's 16-bit atomics address whole 32-bit words
-    if ((rc = ws.d_out.ensure(size_t(total_out) * 2 + 4))) return rc;
-    if ((rc = ws.h_out.ensure(size_t(total_out) * 2))) return rc;
-    if (P.fills.size() > 64) return fail(HTG_E_INVALID, "query plan has more than 64 fill ranges");
+    if ((rc = ws.d_plan.ensure(pk.size)) || (rc = ws.Q.ensure(B.q_bytes())) || (rc = ws.Zp.ensure(B.zp_bytes())) ||
+        (rc = ws.d_out.ensure(B.out_bytes())) || (rc = ws.h_out.ensure(size_t(B.total_out) * 2)))
+        return rc;
     if (fault_hit(HTG_FAULT_QUERY)) return fail(HTG_E_HIP, "injected fault (query launch)");
     hipStream_t st = ws.stream;
     static const bool kernel_copy = [] {
@@ -1716,61 +1801,17 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
     } else if ((e = hipMemcpyAsync(ws.d_plan.p, ws.h_plan.p, pk.size, hipMemcpyHostToDevice, st)) != hipSuccess) {
         return hip_fail(e, "upload plan");
     }
-    char* dp = static_cast<char*>(ws.d_plan.p);
-    auto I = [&](size_t o) { return reinterpret_cast<const int*>(dp + o); };
-    uint16_t* dout = static_cast<uint16_t*>(ws.d_out.p);
-    const uint16_t* dleafD = static_cast<const uint16_t*>(leafD.p);
     const auto th2 = std::chrono::steady_clock::now();
     unsigned long long* wstat = nullptr;
     if (ws.counted) {
         if ((rc = ws.d_wstat.ensure(kWorkSlots * 16)) || (rc = ws.h_wstat.ensure(kWorkSlots * 16))) return rc;
         wstat = static_cast<unsigned long long*>(ws.d_wstat.p);
     }
-    const int2* srcs = reinterpret_cast<const int2*>(dp + o_src);
-    uint16_t* Qb = static_cast<uint16_t*>(ws.Q.p);
-    if (!launch_q_front(nq, I(o_qk), reinterpret_cast<const long long*>(dp + o_qout), I(o_qvp),
-                        reinterpret_cast<const int2*>(dp + o_vprc), reinterpret_cast<const uint8_t*>(dp + o_vpvalid),
-                        kmax, H > 0 ? ns : 0, srcs, b, nl, dleafD, dout, Qb + xoff[0], int(P.fills.size()),
-                        reinterpret_cast<const int4*>(dp + o_fills), fill_blocks, Qb, wstat, st))
-        return fail(HTG_E_INVALID, "query front: more than 64 fill ranges");
-    for (int l = 1; l <= H && ns > 0; ++l) {
-        const LevelGeo& L = lv[l - 1];
-        const LevelPlan& LP = P.levels[l - 1];
-        const uint16_t* Xl = Qb + xoff[size_t(l) - 1];            // X of this level (stride pc)
-        uint16_t* Xn = l < H ? Qb + xoff[size_t(l)] : nullptr;  // lifted X (stride ne)
-        uint16_t* Zl = Qb + zoffq[size_t(l)];
-        const int ne = l < H ? L.d.ne : 0;  // no lift above the root
-        // bytes and updates of the projection are counted on the device
-        KTimer kt;
-        const int pi = ws.counted ? prof_ref().open(HTG_K_Q_PROJECT, 0.0, 0.0, kt, st) : -1;
-        const size_t part_stride = nsl * L.ldp;
-        const bool la = launch_q_z(LP.M, L.d.ld, L.d.pc, static_cast<const uint16_t*>(L.DU.p), L.node_stride,
-                   static_cast<const int*>(L.d_seam_list.p), static_cast<const int*>(L.d_zcols.p), L.ngrp,
-                   L.d_live_list, LP.col_tiles, L.ldp,
-                   reinterpret_cast<const int4*>(dp + o_items[l - 1]), int(LP.items.size() / 2), LP.ksplit, Xl,
-                   static_cast<uint16_t*>(ws.Zp.p), part_stride, static_cast<const int*>(L.d_colpos.p),
-                   static_cast<const int*>(L.d_colext.p), static_cast<const int*>(L.d_ext_child.p),
-                   static_cast<const int*>(L.d_ext_cport.p), Zl, L.ldz, Xn, ne, wstat, st, kt);
-        prof_ref().close(pi, la);
-        if (LP.ksplit > 1) {
-            const int pf =
-                ws.counted ? prof_ref().open(HTG_K_Q_LIFT, double(ns) * (L.nz + 2.0 * ne + L.d.pc) * 2, 0, kt, st) : -1;
-            const bool lf = launch_q_fold(ns, L.nz, L.ldz, static_cast<const int*>(L.d_zsrc.p), L.ldp, LP.ksplit,
-                          static_cast<const uint16_t*>(ws.Zp.p), part_stride, srcs, L.d.s, L.d.nt, L.ngrp,
-                          L.d_live_map, Zl, ne, static_cast<const int*>(L.d_ext_col.p),
-                          static_cast<const int*>(L.d_ext_child.p), static_cast<const int*>(L.d_ext_cport.p), L.d.pc,
-                          L.d.cs, g, Xl, Xn, st, kt);
-            prof_ref().close(pf, lf);
-        }
-    }
-    if (ns > 0 && !P.mtiles.empty()) {
-        KTimer kt;
-        const int pm = ws.counted ? prof_ref().open(HTG_K_Q_MEET, P.m_bytes, P.m_updates, kt, st) : -1;
-        const bool lm = launch_q_meet(I(o_tiles), int(P.mtiles.size() / 8), I(o_glist), I(o_zgeo), g2, I(o_si), I(o_qk),
-                                      reinterpret_cast<const long long*>(dp + o_qout), Qb, dout, st, kt);
-        prof_ref().close(pm, lm);
-    }
-    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "query launch");
+    if ((rc = enqueue_batch(P, B, static_cast<const char*>(ws.d_plan.p), static_cast<uint16_t*>(ws.Q.p),
+                            static_cast<uint16_t*>(ws.Zp.p), static_cast<uint16_t*>(ws.d_out.p), wstat, ws.counted,
+                            st)))
+        return rc;
+    const long long total_out = B.total_out;
     if (to_host) {
         if (kernel_copy)
             launch_copy_bytes(ws.d_out.p, ws.h_out.p, size_t(total_out) * 2, st);
@@ -1858,6 +1899,45 @@ QueryWS::~QueryWS() {
         return htg::fail(HTG_E_INVALID, "unexpected exception");        \
     }
 
+// ---------------------------------------------------------------------------
+// Device scratch of htg_query_batch_noalloc: one file-scope array per device
+// (each GPU the library runs on holds its own copy until the process ends),
+// carved per call into the packed plan, the X / Z regions, the row-chunk
+// partials and the result.  Host code never uses the array's name as a
+// device address: htg_ctx_create looks its address up once per device
+// (hipGetSymbolAddress, with that device current) and keeps it in
+// g_noalloc[device].ptr.
+namespace {
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+constexpr int kMaxDevices = 64;
+}  // namespace
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+struct NoAllocSlot {
+    std::once_flag once;
+    hipError_t err = hipSuccess;
+    void* ptr = nullptr;  // this device's copy of `scratch`
+    // Calls on one device share its array, so they run one at a time; the
+    // host planning state is reused from call to call.
+    std::mutex mu;
+    QueryPlan plan;
+    BatchLayout lay;
+    std::vector<char> host_plan;
+};
+NoAllocSlot g_noalloc[kMaxDevices];
+
+int noalloc_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return fail(HTG_E_INVALID, "device index beyond the scratch table");
+    NoAllocSlot& sl = g_noalloc[device];
+    std::call_once(sl.once, [&sl] { sl.err = hipGetSymbolAddress(&sl.ptr, HIP_SYMBOL(scratch)); });
+    if (sl.err != hipSuccess) return hip_fail(sl.err, "hipGetSymbolAddress(scratch)");
+    return HTG_OK;
+}
+
+size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+}  // namespace
+
 namespace {
 // Calls on a map need a built map whose last update did not fail.
 int usable(const htg_map* m) {
@@ -1879,6 +1959,7 @@ int htg_ctx_create(int device, htg_ctx** out) {
     if (e != hipSuccess) return hip_fail(e, "hipGetDeviceCount");
     if (device < 0 || device >= n) return fail(HTG_E_INVALID, "no such device");
     if ((e = hipSetDevice(device)) != hipSuccess) return hip_fail(e, "hipSetDevice");
+    if (int rc = noalloc_setup(device)) return rc;
     auto* c = new htg_ctx();
     c->device = device;
     *out = c;
@@ -2288,6 +2369,49 @@ int htg_query_batch_ws(htg_map* map, htg_query_ws* ws, const int32_t* rc, const 
 
 int htg_query_all_pairs_ws(htg_map* map, htg_query_ws* ws, const int32_t* rc, int32_t k, uint16_t* out) {
     return htg_query_batch_ws(map, ws, rc, &k, 1, out);
+}
+
+int htg_query_batch_noalloc(htg_map* map, const int32_t* rc, const int32_t* ks, int32_t nq, uint16_t* out) {
+    HTG_GUARD_BEGIN
+    if (int c = check_batch(rc, ks, nq, out)) return c;
+    if (!map) return fail(HTG_E_INVALID, "null map");
+    if (int u = usable(map)) return u;
+    const int dev = map->ctx->device;
+    if (dev < 0 || dev >= kMaxDevices || !g_noalloc[dev].ptr)
+        return fail(HTG_E_STATE, "no-allocation scratch not set up for this device");
+    NoAllocSlot& sl = g_noalloc[dev];
+    std::lock_guard<std::mutex> dl(sl.mu);    // calls on one device share its array
+    std::lock_guard<std::mutex> ml(map->qmu);  // and the map's stream with its own queries
+    hipError_t e = hipSetDevice(dev);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    QueryPlan& P = sl.plan;
+    BatchLayout& B = sl.lay;
+    if (int r = map->plan_batch(rc, ks, nq, P, B, false, nullptr)) return r;
+    if (B.total_out == 0) return HTG_OK;
+    const size_t o_plan = 0, o_q = o_plan + align256(B.pk.size), o_zp = o_q + align256(B.q_bytes()),
+                 o_out = o_zp + align256(B.zp_bytes()), need = o_out + align256(B.out_bytes());
+    if (need > kNoAllocScratchBytes)
+        return fail(HTG_E_RANGE, "batch needs " + std::to_string(need) + " bytes of device scratch; "
+                                 "htg_query_batch_noalloc holds " + std::to_string(kNoAllocScratchBytes));
+    if (sl.host_plan.size() < B.pk.size) sl.host_plan.resize(B.pk.size);  // host staging, grow-only
+    B.pk.write(sl.host_plan.data());
+    if (map->fault_hit(HTG_FAULT_QUERY)) return fail(HTG_E_HIP, "injected fault (query launch)");
+    char* base = static_cast<char*>(sl.ptr);
+    const hipStream_t st = map->stream;  // in order behind the map's updates
+    if ((e = hipMemcpyAsync(base + o_plan, sl.host_plan.data(), B.pk.size, hipMemcpyHostToDevice, st)) != hipSuccess)
+        return hip_fail(e, "upload plan");
+    // Nothing here needs to start at zero: the front kernel writes every
+    // result entry and every INF fill of the X / Z regions, the projection
+    // writes every partial the fold reads.
+    if (int r = map->enqueue_batch(P, B, base + o_plan, reinterpret_cast<uint16_t*>(base + o_q),
+                                   reinterpret_cast<uint16_t*>(base + o_zp), reinterpret_cast<uint16_t*>(base + o_out),
+                                   nullptr, false, st))
+        return r;
+    if ((e = hipMemcpyAsync(out, base + o_out, size_t(B.total_out) * 2, hipMemcpyDeviceToHost, st)) != hipSuccess ||
+        (e = hipStreamSynchronize(st)) != hipSuccess)
+        return hip_fail(e, "download result");
+    return HTG_OK;
+    HTG_GUARD_END
 }
 
 int htg_inject_fault(htg_map* map, int32_t site, int32_t countdown) {

@@ -75,6 +75,10 @@ struct Packer {
         for (const Part& p : parts)
             if (p.bytes) std::memcpy(dst + p.off, p.src, p.bytes);
     }
+    void clear() {
+        parts.clear();
+        size = 0;
+    }
 };
 
 struct LevelGeo {
@@ -166,6 +170,23 @@ struct QueryPlan {
     }
 };
 
+// Where the parts of one planned query batch go: offsets of each list in the
+// packed plan, the X / Z regions of Q, and the sizes of the device regions
+// (Q, row-chunk partials Zp, result).
+struct BatchLayout {
+    long long total_vp = 0, total_out = 0;
+    int kmax = 0, ns = 0, fill_blocks = 0;
+    size_t nsl = 1, qsz = 0, maxpart = 0;
+    std::vector<size_t> xoff, zoffq, o_items;  // X of levels 0..H-1, Z of levels 1..H; per-level items
+    size_t o_qk = 0, o_qout = 0, o_qvp = 0, o_vprc = 0, o_vpvalid = 0, o_src = 0, o_si = 0, o_tiles = 0,
+           o_glist = 0, o_zgeo = 0, o_fills = 0;
+    Packer pk;
+    size_t q_bytes() const { return (qsz > 8 ? qsz : 8) * 2; }
+    size_t zp_bytes() const { return nsl * (maxpart > 8 ? maxpart : 8) * 2; }
+    // the meet's 16-bit atomics address whole 32-bit words
+    size_t out_bytes() const { return size_t(total_out) * 2 + 4; }
+};
+
 // Per-stream query workspace (grow-only, like HTripQueryScratch,
 // include/htrip/minplus_oracle.hpp:189-238).
 struct QueryWS {
@@ -174,6 +195,7 @@ struct QueryWS {
     hipEvent_t done = nullptr;  // recorded after the batch's last operation
     long long total_out = 0;
     QueryPlan plan;
+    BatchLayout lay;
     HostBuf h_plan, h_out, h_wstat;
     // Q: X of levels 0..H-1 then Z of levels 1..H, each region 8-aligned
     DevBuf d_plan, Q, Zp, d_out, d_wstat;  // d_wstat: k_q_z work counters (bytes, updates)
@@ -335,6 +357,12 @@ struct Map {
     // Asynchronous halves of query(): plan + launch on ws.stream (results
     // copied to ws.h_out when to_host), then wait for ws.done.
     int query_launch(const int32_t* rc, const int32_t* ks, int nq, QueryWS& ws, bool to_host);
+    // The two halves of a launch: host planning (P and B are reused and keep
+    // their capacity), and the kernels of a planned batch on given buffers.
+    int plan_batch(const int32_t* rc, const int32_t* ks, int nq, QueryPlan& P, BatchLayout& B, bool counting,
+                   double* host_sub) const;
+    int enqueue_batch(const QueryPlan& P, const BatchLayout& B, const char* d_plan, uint16_t* Q, uint16_t* Zp,
+                      uint16_t* d_out, unsigned long long* wstat, bool counted, hipStream_t st);
     int query_finish(QueryWS& ws, uint16_t* out_host);
     // Grow ws ahead of time for batches of up to max_vps viewpoints and
     // max_out result entries, so that serving does not allocate (a device

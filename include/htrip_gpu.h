@@ -188,6 +188,20 @@ int htg_query_ws_bytes(const htg_query_ws* ws, uint64_t* bytes);
 int htg_query_all_pairs_ws(htg_map* map, htg_query_ws* ws, const int32_t* rc, int32_t k, uint16_t* out);
 int htg_query_batch_ws(htg_map* map, htg_query_ws* ws, const int32_t* rc, const int32_t* ks, int32_t nq,
                        uint16_t* out);
+/* htg_query_batch that allocates no memory: no hipMalloc / hipFree, no pinned
+ * or device buffer of any kind.  No counterpart in the reference; it is for
+ * callers that cannot allocate (tight serving loops, code that must not
+ * allocate or synchronise on a free).  Its device scratch (plan, lift and
+ * projection buffers, result staging) is one fixed array of 64 MiB per GPU,
+ * held from the library's first use on that GPU until the process ends;
+ * a batch that needs more returns HTG_E_RANGE with the size it needs (on the
+ * 1024^2, b = 32 serving map a single query of up to k ~ 1500 fits).  rc and out
+ * are host memory as in htg_query_batch; the call runs on the map's stream
+ * after its updates and returns when out is written.  Calls on one GPU share
+ * its array: they are serialised by a per-GPU lock (unlocked, they would
+ * race), and the array keeps the last call's contents, which no call reads
+ * before writing. */
+int htg_query_batch_noalloc(htg_map* map, const int32_t* rc, const int32_t* ks, int32_t nq, uint16_t* out);
 
 /* ---- fault injection (test hook) ----------------------------------------
  * The countdown-th time (1 = the next time) the map reaches `site`, the call
