@@ -1666,12 +1666,11 @@ int Map::plan_batch(const int32_t* rcs, const int32_t* ks, int nq, QueryPlan& P,
                             tt[6] = goff, tt[7] = q;
                         }
                     if (counting) {  // the meet's work figures, for kernel timing only
-                        const double zl = L.zlen[size_t(q2)];
-                        for (int i0 = 0; i0 < n; i0 += 8)
-                            for (int jj = j0; jj < je; jj += 8) {
-                                P.m_updates += double(std::min(8, n - i0)) * std::min(8, je - jj) * zl;
-                                P.m_bytes += double(std::min(8, n - i0) + std::min(8, je - jj)) * zl * 2;
-                            }
+                        // algorithmic: the group's n rows of the q2 seam block read
+                        // once (q2's own rows are among them), n x |q2| results
+                        const double zl = L.zlen[size_t(q2)], nj = double(je - j0);
+                        P.m_updates += double(n) * nj * zl;
+                        P.m_bytes += (double(n) * zl + double(n) * nj) * 2;
                     }
                     j0 = je;
                 }

@@ -130,10 +130,14 @@ def sample_free(cells: np.ndarray, k: int, seed: int) -> np.ndarray:
 def disk_observed(truth: np.ndarray, centers, radius: int) -> np.ndarray:
     """Observed layer: ground truth inside the disks, Unknown outside."""
     rows, cols = truth.shape
-    rr, cc = np.mgrid[0:rows, 0:cols]
     inside = np.zeros_like(truth, dtype=bool)
-    for (r, c) in centers:
-        inside |= (rr - r) ** 2 + (cc - c) ** 2 <= radius * radius
+    for (r, c) in centers:  # each disk only touches its bounding box
+        r0, r1 = max(0, int(r) - radius), min(rows, int(r) + radius + 1)
+        c0, c1 = max(0, int(c) - radius), min(cols, int(c) + radius + 1)
+        if r0 >= r1 or c0 >= c1:
+            continue
+        rr, cc = np.mgrid[r0:r1, c0:c1]
+        inside[r0:r1, c0:c1] |= (rr - r) ** 2 + (cc - c) ** 2 <= radius * radius
     return np.where(inside, truth, UNKNOWN).astype(np.uint8)
 
 
@@ -159,6 +163,29 @@ def frontier_order(truth: np.ndarray, observed: np.ndarray, limit: int | None = 
             break
     o = np.array(order, dtype=np.int64)
     return np.stack([o // cols, o % cols], axis=1).astype(np.int32) if len(o) else np.zeros((0, 2), np.int32)
+
+
+def frontier_levels(truth: np.ndarray, observed: np.ndarray) -> np.ndarray:
+    """Every free-but-unobserved cell in BFS level order from the observed free
+    set (cells of one level in ascending row-major index), int32 (n, 2).
+    The same level sets as ``frontier_order`` without the queue's order inside
+    a level; vectorised per level, for whole-map reveals."""
+    rows, cols = truth.shape
+    free = (truth == FREE).ravel()
+    seen = (observed == FREE).ravel().copy()
+    front = np.flatnonzero(seen)
+    parts = []
+    while len(front):
+        r, c = front // cols, front % cols
+        nb = np.concatenate([front[r > 0] - cols, front[r < rows - 1] + cols, front[c > 0] - 1,
+                             front[c < cols - 1] + 1])
+        nb = np.unique(nb[free[nb] & ~seen[nb]])
+        seen[nb] = True
+        if len(nb):
+            parts.append(nb)
+        front = nb
+    o = np.concatenate(parts) if parts else np.zeros(0, np.int64)
+    return np.stack([o // cols, o % cols], axis=1).astype(np.int32)
 
 
 def reveal_scenario(truth: np.ndarray, seed: int, radius: int = 64):

@@ -319,3 +319,161 @@ def merge_traces(traces) -> Trace:
                    np.ascontiguousarray(np.concatenate([tr.rc for tr in traces]), dtype=np.int32),
                    out_off[order].astype(np.int64), np.zeros(max(out_base, 1), np.uint16))
     return merged
+
+
+class Exploration:
+    """A fleet of robots exploring a ground truth from their start disks to
+    the end: each robot reveals cells in its own BFS level order from its
+    disk (grids.frontier_levels), skipping cells another robot revealed
+    first.  Vectorised for long request streams; RobotReveal is the
+    per-cell original used by the tests."""
+
+    def __init__(self, truth: np.ndarray, n_robots: int, radius: int, seed: int, _orders=None):
+        self.truth = truth
+        self.cols = truth.shape[1]
+        self.args = (n_robots, radius, seed)
+        starts = grids.sample_free(truth, n_robots, seed)
+        self.observed = grids.disk_observed(truth, [tuple(p) for p in starts], radius)
+        if _orders is None:
+            _orders = []
+            for p in starts:
+                o = grids.frontier_levels(truth, grids.disk_observed(truth, [tuple(p)], radius))
+                _orders.append(o[:, 0].astype(np.int64) * self.cols + o[:, 1])
+        self.orders = _orders
+        self.ptr = [0] * n_robots
+        self.obs_flat = (self.observed == grids.FREE).ravel().copy()
+        n_free = int((truth == grids.FREE).sum())
+        self.free_list = np.empty(n_free, dtype=np.int64)
+        init = np.flatnonzero(self.obs_flat)
+        self.free_list[:len(init)] = init
+        self.n_free = len(init)
+        self.total_free = n_free
+
+    def fresh(self) -> "Exploration":
+        """The same fleet back at its start disks (the robot orders are shared)."""
+        return Exploration(self.truth, *self.args, _orders=self.orders)
+
+    def advance_to(self, frac: float, batch: int = 143) -> None:
+        """Reveal in robot round robin until `frac` of the free cells are observed."""
+        r = 0
+        while self.n_free < frac * self.total_free:
+            if len(self.reveal(r % len(self.orders), batch)) == 0:
+                break
+            r += 1
+
+    def observed_now(self) -> np.ndarray:
+        """The observed grid at this point of the exploration."""
+        obs = np.where(self.obs_flat.reshape(self.truth.shape), self.truth, grids.UNKNOWN)
+        return obs.astype(np.uint8)
+
+    def reveal(self, robot: int, m: int) -> np.ndarray:
+        """Up to m newly revealed cells (flat indices) from robot `robot`, then
+        from the next robots that still have cells; empty when all are done."""
+        R = len(self.orders)
+        got = []
+        need = m
+        for t in range(R):
+            r = (robot + t) % R
+            order, p = self.orders[r], self.ptr[r]
+            while need > 0 and p < len(order):
+                win = order[p:p + 4 * need + 16]
+                new = win[~self.obs_flat[win]]
+                if len(new) > need:
+                    cut = int(np.searchsorted(np.cumsum(~self.obs_flat[win]), need))  # last window cell taken
+                    win = win[:cut + 1]
+                    new = win[~self.obs_flat[win]]
+                p += len(win)
+                self.obs_flat[new] = True
+                got.append(new)
+                need -= len(new)
+            self.ptr[r] = p
+            if need == 0:
+                break
+        cells = np.concatenate(got) if got else np.zeros(0, np.int64)
+        self.free_list[self.n_free:self.n_free + len(cells)] = cells
+        self.n_free += len(cells)
+        return cells
+
+
+def exploration_trace(sessions, explorations, n: int, rate: float, seed: int, update_frac: float = 0.1,
+                      k_lo: int = 8, k_hi: int = 256, upd_lo: int = 16, upd_hi: int = 512, ring: int = 2048,
+                      check_every: int = 1500):
+    """Poisson stream of n requests spread uniformly over the sessions, each
+    session exploring its own map (Exploration): with probability update_frac
+    a request is an update batch of m ~ logU[upd_lo, upd_hi] cells from a
+    random robot, else a queryAllPairs with k ~ logU[k_lo, k_hi] viewpoints
+    drawn uniformly from the cells its session has revealed by then.  Once a
+    session's map is fully revealed its update requests become queries.
+
+    Results of most queries share a ring of `ring` result regions (the server
+    still computes and delivers every matrix); every `check_every`-th query
+    and the queries of the last ring window get regions of their own, listed
+    in meta["checks"], so they can be compared afterwards."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n))
+    sess = rng.integers(0, len(sessions), n)
+    upd = rng.random(n) < update_frac
+    lk = np.log(k_lo) + rng.random(n) * (np.log(k_hi) - np.log(k_lo))
+    ks = np.rint(np.exp(lk)).astype(np.int64)
+    lu = np.log(upd_lo) + rng.random(n) * (np.log(upd_hi) - np.log(upd_lo))
+    ms = np.rint(np.exp(lu)).astype(np.int64)
+    robot = rng.integers(0, 1 << 30, n)
+    typ = np.zeros(n, np.int32)
+    cnt = np.zeros(n, np.int32)
+    payload = [None] * n
+    q_free = np.zeros(n, np.int64)  # revealed cells of the session when the query arrives
+    upd_cells = [[] for _ in sessions]
+    for i in range(n):
+        ex = explorations[sess[i]]
+        if upd[i]:
+            cells = ex.reveal(int(robot[i] % len(ex.orders)), int(ms[i]))
+            if len(cells):
+                typ[i] = 1
+                cnt[i] = len(cells)
+                payload[i] = cells
+                upd_cells[sess[i]].append(cells)
+                continue
+        cnt[i] = int(ks[i])
+        q_free[i] = ex.n_free
+    # viewpoints: uniform over the session's revealed cells at arrival time
+    qi = np.flatnonzero(typ == 0)
+    for s_idx, ex in enumerate(explorations):
+        mine = qi[sess[qi] == s_idx]
+        if len(mine) == 0:
+            continue
+        kk = cnt[mine].astype(np.int64)
+        lim = np.repeat(q_free[mine], kk)
+        pick = ex.free_list[(rng.random(int(kk.sum())) * lim).astype(np.int64)]
+        parts = np.split(pick, np.cumsum(kk)[:-1])
+        for j, i in enumerate(mine):
+            payload[i] = parts[j]
+    cols = explorations[0].cols
+    rc_off = np.zeros(n, np.int64)
+    rc_off[1:] = np.cumsum(cnt.astype(np.int64))[:-1]
+    flat = np.concatenate([p for p in payload]) if n else np.zeros(0, np.int64)
+    rc = np.stack([flat // cols, flat % cols], axis=1).astype(np.int32).ravel()
+    # result regions: ring for most queries, own regions for the checked ones
+    checks = [int(i) for j, i in enumerate(qi) if j % check_every == 0]
+    last = [int(i) for i in qi[-ring:]]
+    own = sorted(set(checks) | set(last))
+    kmax2 = k_hi * k_hi
+    out_off = np.full(n, -1, np.int64)
+    ring_q = qi[~np.isin(qi, own)]
+    out_off[ring_q] = (np.arange(len(ring_q)) % ring) * kmax2
+    base = ring * kmax2
+    for i in own:
+        out_off[i] = base
+        base += int(cnt[i]) * int(cnt[i])
+    tr = Trace(t, np.asarray(sessions, np.int32)[sess], typ, cnt, rc_off, np.ascontiguousarray(rc), out_off,
+               np.zeros(max(base, 1), np.uint16))
+    tr.meta.update({"checks": own, "session_index": sess, "upd_cells": upd_cells,
+                    "explored": [ex.n_free / ex.total_free for ex in explorations]})
+    return tr
+
+
+def state_at(init: np.ndarray, upd_cells, v: int) -> np.ndarray:
+    """The observed grid after the first v update batches (flat cell lists)."""
+    st = init.copy().ravel()
+    for c in upd_cells[:v]:
+        st[c] = grids.FREE
+    return st.reshape(init.shape)
