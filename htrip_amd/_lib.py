@@ -66,7 +66,7 @@ class KernelStats(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("pivot_block", C.c_int32), ("zq_tile", C.c_int32), ("use_graph", C.c_int32),
-                ("update_pivots", C.c_int32)]
+                ("update_pivots", C.c_int32), ("update_rebuild", C.c_int32)]
 
 
 _lib = None

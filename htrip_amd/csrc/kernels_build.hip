@@ -20,6 +20,12 @@ __device__ __forceinline__ uint32_t relax2(uint32_t acc, uint32_t a, uint32_t b)
     us2 x = __builtin_bit_cast(us2, a), y = __builtin_bit_cast(us2, b), z = __builtin_bit_cast(us2, acc);
     return __builtin_bit_cast(uint32_t, __builtin_elementwise_min(z, __builtin_elementwise_add_sat(x, y)));
 }
+__device__ __forceinline__ uint32_t add2(uint32_t a, uint32_t b) {
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_add_sat(__builtin_bit_cast(us2, a), __builtin_bit_cast(us2, b)));
+}
+__device__ __forceinline__ uint32_t min2(uint32_t a, uint32_t b) {
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_bit_cast(us2, a), __builtin_bit_cast(us2, b)));
+}
 __device__ __forceinline__ uint32_t dup16(uint32_t v) { return (v & 0xffffu) | (v << 16); }
 __device__ __forceinline__ uint16_t add1(uint16_t a, uint16_t b) {
     uint32_t s = uint32_t(a) + b;
@@ -865,6 +871,20 @@ __global__ __launch_bounds__(256) void k_closure_update(int nV, int ld, const in
         }
     }
     __syncthreads();
+    // Minima of the block's pivot rows at the tile's rows (dup'd) and column
+    // pairs: every candidate R[k][i] (+) R[k][j] is >= rmin[i] (+) cmin[j].
+    __shared__ uint32_t rmin[128], cmin[64];
+    {
+        uint32_t m = kInf2;
+        if (tid < 128) {
+            for (int k = 0; k < p; ++k) m = min2(m, reinterpret_cast<const uint32_t*>(&Rr4[k][0])[tid]);
+            rmin[tid] = m;
+        } else if (tid < 192) {
+            for (int k = 0; k < p; ++k) m = min2(m, reinterpret_cast<const uint32_t*>(&Rc4[k][0])[tid - 128]);
+            cmin[tid - 128] = m;
+        }
+    }
+    __syncthreads();
     const int tx = tid & 15, ty = tid >> 4;
     const int col = co[tx], row0 = ro[ty];
     const bool ok = col >= 0 && row0 >= 0;
@@ -880,19 +900,35 @@ __global__ __launch_bounds__(256) void k_closure_update(int nV, int ld, const in
         acc[r][2] = v.z;
         acc[r][3] = v.w;
     }
-#pragma unroll 2
-    for (int k = 0; k < p; ++k) {
-        const uint4 a0 = Rr4[k][ty * 2], a1 = Rr4[k][ty * 2 + 1];
-        const uint4 c = Rc4[k][tx];
-        const uint32_t av[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+    // Exact pruning: an entry with rmin[i] (+) cmin[j] >= D[i][j] cannot
+    // decrease in this block.  A wavefront none of whose entries can skips
+    // the pivot loop; a thread none of whose entries can skips its stores.
+    bool need = false;
 #pragma unroll
-        for (int r = 0; r < 8; ++r) {
-            acc[r][0] = relax2(acc[r][0], av[r], c.x);
-            acc[r][1] = relax2(acc[r][1], av[r], c.y);
-            acc[r][2] = relax2(acc[r][2], av[r], c.z);
-            acc[r][3] = relax2(acc[r][3], av[r], c.w);
+    for (int r = 0; r < 8; ++r) {
+        const uint32_t rm = rmin[ty * 8 + r];
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            const uint32_t lb = add2(rm, cmin[tx * 4 + h]);
+            need |= min2(lb, acc[r][h]) != acc[r][h];
         }
     }
+    if (__ballot(need) != 0) {
+#pragma unroll 2
+        for (int k = 0; k < p; ++k) {
+            const uint4 a0 = Rr4[k][ty * 2], a1 = Rr4[k][ty * 2 + 1];
+            const uint4 c = Rc4[k][tx];
+            const uint32_t av[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                acc[r][0] = relax2(acc[r][0], av[r], c.x);
+                acc[r][1] = relax2(acc[r][1], av[r], c.y);
+                acc[r][2] = relax2(acc[r][2], av[r], c.z);
+                acc[r][3] = relax2(acc[r][3], av[r], c.w);
+            }
+        }
+    }
+    if (!need) return;
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
         const int i = row0 + r;

@@ -652,6 +652,81 @@ int Map::block_cells(const int32_t* rcs, int n) {
     return rebuild(closed, leaves, anc);
 }
 
+// Estimated device time (us) of the blocked closure of the listed nodes over
+// cnt[i] pivots each: blocks of P pivots, each block at least the latency of
+// its two launches, else bound by the packed-u16 VALU rate (P x live^2 updates
+// per node and block).
+static constexpr double kUpdatesPerUs = 23.6e6;  // ~60 % of the 39.3 T/s packed min-plus peak (measured)
+static constexpr double kBlockUs = 55.0;         // a pivot block's two launches at small sizes (measured)
+double Map::closure_us(int lvl, const std::vector<int>& nodes, const int* cnt) const {
+    const LevelGeo& L = lv[size_t(lvl) - 1];
+    const int P = tune.pivot_block;
+    int maxc = 0;
+    for (size_t i = 0; i < nodes.size(); ++i) maxc = std::max(maxc, cnt[i]);
+    double t = 0;
+    for (int b0 = 0; b0 < maxc; b0 += P) {
+        double upd = 0;
+        for (size_t i = 0; i < nodes.size(); ++i) {
+            const double p = std::max(0, std::min(P, cnt[i] - b0));
+            const double live = 8.0 * L.h_live8_cnt[nodes[i]];
+            upd += p * live * live;
+        }
+        t += std::max(kBlockUs, upd / kUpdatesPerUs);
+    }
+    return t;
+}
+
+// The update of a level after its counts are known (cover C and uncovered
+// endpoints B per node, hc = C then B) against a rebuild of the same nodes.
+bool Map::rebuild_is_cheaper(int lvl, const std::vector<int>& nodes, const std::vector<int>& hc, int na) const {
+    if (tune.update_rebuild < 0) return false;
+    if (tune.update_rebuild > 0) return true;
+    const LevelGeo& L = lv[size_t(lvl) - 1];
+    double cover = 0;
+    bool any_b = false;
+    for (int i = 0; i < na; ++i) {
+        const double c = hc[size_t(i)], b = hc[size_t(na + i)], live = 8.0 * L.h_live8_cnt[nodes[size_t(i)]];
+        if (c <= 0 || b <= 0) continue;
+        cover += c * b * live + c * c * b;
+        any_b = true;
+    }
+    const double upd = closure_us(lvl, nodes, hc.data()) + cover / kUpdatesPerUs + (any_b ? 2 * kBlockUs : 0);
+    const std::vector<int> pb = build_pivot_counts(lvl, &nodes);
+    double assemble = 0;
+    for (size_t i = 0; i < nodes.size(); ++i) assemble += double(L.d.nV) * L.d.ld * 2 / 4.0e6;  // ~4 TB/s
+    const double reb = closure_us(lvl, nodes, pb.data()) + assemble + 3 * kBlockUs / 2;
+    if (debug_level())
+        std::fprintf(stderr, "[htg] update lvl %d: update ~%.0f us, rebuild ~%.0f us\n", lvl, upd, reb);
+    return reb < upd;
+}
+
+// Rebuild the listed nodes of one level from their children with the build
+// kernels (assemble, seam edges, closure over the seam endpoints, S_U);
+// changed[i] (nullable) is set when listed node i's S_U decreased.
+int Map::rebuild_level(int lvl, const int* dnodes, const std::vector<int>& hn, int* changed) {
+    LevelGeo& L = lv[size_t(lvl) - 1];
+    const int na = int(hn.size());
+    if (na == 0) return HTG_OK;
+    int rc;
+    if ((rc = ensure_build_ws(lvl))) return rc;
+    launch_fill_u8(static_cast<uint8_t*>(ws_flags.p), size_t(na) * L.d.nV, 0, stream);
+    size_t cstride;
+    const uint16_t* csum = child_summaries(lvl, cstride);
+    launch_node_assemble(lvl, g, L.d.pc, L.d.nV, L.d.ld, L.d.nt, csum, cstride, dnodes, na,
+                         static_cast<uint16_t*>(L.DU.p), L.node_stride, stream);
+    launch_seam_relax(L.d.nV, L.d.ld, L.d.s, L.d.nt, static_cast<const int2*>(L.d_seams.p), int(L.seams.size()),
+                      static_cast<const short2*>(L.d_vu_rc.p), static_cast<const uint8_t*>(d_pass.p), S, dnodes, na,
+                      static_cast<uint16_t*>(L.DU.p), L.node_stride, static_cast<uint8_t*>(ws_flags.p), stream);
+    launch_compact_flags(L.d.nV, dnodes, na, static_cast<uint8_t*>(ws_flags.p), static_cast<int*>(ws_piv.p), L.d.nV,
+                         static_cast<int*>(ws_counts.p), stream);
+    if ((rc = run_closure(lvl, dnodes, hn, build_pivot_counts(lvl, &hn)))) return rc;
+    launch_project_su(L.d.nV, L.d.ld, L.d.ne, static_cast<const int*>(L.d_ext_idx.p), dnodes, na,
+                      static_cast<const uint16_t*>(L.DU.p), L.node_stride, static_cast<uint16_t*>(L.SU.p), changed,
+                      stream);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? HTG_OK : hip_fail(e, "rebuild launch");
+}
+
 // Recompute the listed leaves (BFS from the passability) and rebuild the
 // listed nodes of every level from their children with the build kernels
 // (assemble, seam edges, closure over the seam endpoints, S_U); closed cells
@@ -682,26 +757,10 @@ int Map::rebuild(const std::vector<int2>& closed, const std::vector<int>& leaves
                                       lkt, kBackgroundWG);
     prof_ref().close(lpi, lla);
     for (int lvl = 1; lvl <= H; ++lvl) {
-        LevelGeo& L = lv[size_t(lvl) - 1];
-        const std::vector<int>& hn = anc[size_t(lvl)];
-        const int na = int(hn.size());
-        if (na == 0) continue;
-        const int* nodes = reinterpret_cast<const int*>(up + o_anc[size_t(lvl)]);
-        if ((rc = ensure_build_ws(lvl))) return rc;
-        launch_fill_u8(static_cast<uint8_t*>(ws_flags.p), size_t(na) * L.d.nV, 0, stream);
-        size_t cstride;
-        const uint16_t* csum = child_summaries(lvl, cstride);
-        launch_node_assemble(lvl, g, L.d.pc, L.d.nV, L.d.ld, L.d.nt, csum, cstride, nodes, na,
-                             static_cast<uint16_t*>(L.DU.p), L.node_stride, stream);
-        launch_seam_relax(L.d.nV, L.d.ld, L.d.s, L.d.nt, static_cast<const int2*>(L.d_seams.p), int(L.seams.size()),
-                          static_cast<const short2*>(L.d_vu_rc.p), static_cast<const uint8_t*>(d_pass.p), S, nodes, na,
-                          static_cast<uint16_t*>(L.DU.p), L.node_stride, static_cast<uint8_t*>(ws_flags.p), stream);
-        launch_compact_flags(L.d.nV, nodes, na, static_cast<uint8_t*>(ws_flags.p), static_cast<int*>(ws_piv.p), L.d.nV,
-                             static_cast<int*>(ws_counts.p), stream);
-        if ((rc = run_closure(lvl, nodes, hn, build_pivot_counts(lvl, &hn)))) return rc;
-        launch_project_su(L.d.nV, L.d.ld, L.d.ne, static_cast<const int*>(L.d_ext_idx.p), nodes, na,
-                          static_cast<const uint16_t*>(L.DU.p), L.node_stride, static_cast<uint16_t*>(L.SU.p), nullptr,
-                          stream);
+        if (anc[size_t(lvl)].empty()) continue;
+        if ((rc = rebuild_level(lvl, reinterpret_cast<const int*>(up + o_anc[size_t(lvl)]), anc[size_t(lvl)],
+                                nullptr)))
+            return rc;
     }
     if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "rebuild launch");
     if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "rebuild sync");
@@ -1183,6 +1242,20 @@ int Map::open_cells(const int32_t* rcs, int n) {
         if (debug_level())
             std::fprintf(stderr, "[htg] update lvl %d: %d nodes (nV %d), pivots sum %lld max %d, B sum %lld max %d\n",
                          lvl, na, L.d.nV, sum, mx, bsum, mxb);
+        // A level whose cover makes the update dearer than rebuilding its
+        // candidate nodes from their children (the build's closure over the
+        // passable seam endpoints) is rebuilt instead.  After any reveal every
+        // table equals a fresh build of the revealed grid, so both paths give
+        // the same tables, S_U and changed flags.
+        if (sum > 0 && cover && rebuild_is_cheaper(lvl, cl, hc, na)) {
+            if ((rc = rebuild_level(lvl, dnodes, cl, chg_here))) return rc;
+            this->dirty.nodes[size_t(lvl) - 1] = cl;  // every candidate's table was rewritten
+            rebuilt_levels++;
+            below_any = true;
+            chg_below = chg_here;
+            if (lvl == top_first && fault_hit(HTG_FAULT_UPDATE)) return fail(HTG_E_HIP, "injected fault (update)");
+            continue;
+        }
         if (bsum > 0) {  // fold the uncovered endpoints into the cover rows
             double by_r = 0, upd_r = 0, by_p = 0, upd_p = 0;
             if (prof_ref().on) {
@@ -2645,6 +2718,7 @@ int htg_set_tuning(htg_map* map, const htg_tuning* t) {
         return fail(HTG_E_INVALID, "pivot_block must be 16, 32 or 64");
     if (t->zq_tile < 1 || t->zq_tile > 16) return fail(HTG_E_INVALID, "zq_tile must be 1..16");
     if (t->update_pivots != 0 && t->update_pivots != 1) return fail(HTG_E_INVALID, "update_pivots must be 0 or 1");
+    if (t->update_rebuild < -1 || t->update_rebuild > 1) return fail(HTG_E_INVALID, "update_rebuild must be -1, 0 or 1");
     map->tune = *t;
     return HTG_OK;
 }

@@ -283,7 +283,7 @@ struct Map {
     uint64_t epoch = 0;
     htg_batch_stats stats{0, 0, 0};
     uint64_t budget = 0, device_bytes = 0;
-    htg_tuning tune{64, 4, 0, 0};
+    htg_tuning tune{64, 4, 0, 0, 0};
     QueryWS qws;
     // Held by the calls that use the map's own workspaces (qws, lift_*), so
     // that concurrent queries on one map serialise instead of sharing them.
@@ -331,6 +331,15 @@ struct Map {
     int build();
     int open_cells(const int32_t* rc, int n);
     int block_cells(const int32_t* rc, int n);
+    // One level of a rebuild: assemble, seam edges, closure over the seam
+    // endpoints, S_U (changed[i] set when listed node i's S_U decreased).
+    int rebuild_level(int lvl, const int* dnodes, const std::vector<int>& nodes, int* changed);
+    // Update planning: estimated closure time of the listed nodes, and whether
+    // rebuilding a level's candidates beats updating them (hc: cover counts,
+    // then uncovered-endpoint counts).
+    double closure_us(int lvl, const std::vector<int>& nodes, const int* cnt) const;
+    bool rebuild_is_cheaper(int lvl, const std::vector<int>& nodes, const std::vector<int>& hc, int na) const;
+    long long rebuilt_levels = 0;  // update levels done by a rebuild (printed at HTG_DEBUG >= 2)
     int rebuild(const std::vector<int2>& closed, const std::vector<int>& leaves,
                 const std::vector<std::vector<int>>& anc);
     void reset_versions();

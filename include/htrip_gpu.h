@@ -344,6 +344,10 @@ typedef struct htg_tuning {
                               decreased entries (new ports + endpoints of old-old decreases) after a two-hop
                               pre-relaxation through the uncovered endpoints; 1 every row with a decreased
                               entry (the reference's pivot set, src/internal_node.cpp:332-413).  Both are exact. */
+    int32_t update_rebuild; /* with the cover (update_pivots 0): a level of an incremental update is instead
+                               rebuilt from its children (the build's closure over the passable seam
+                               endpoints) when 0 (default) an estimate says that is cheaper, 1 always, -1
+                               never.  Every table after a reveal equals a fresh build, so all are exact. */
 } htg_tuning;
 int htg_get_tuning(const htg_map* map, htg_tuning* out);
 int htg_set_tuning(htg_map* map, const htg_tuning* t);

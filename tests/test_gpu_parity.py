@@ -105,9 +105,11 @@ def test_incremental_reveal(gpu_ctx, kind, n, b, g, seed, radius, batch, steps):
     ("street", 64, 8, 45, 2, 9, 10),     # small batches: mostly single-block covers
 ])
 def test_update_pivot_sets_agree(gpu_ctx, kind, n, b, seed, robots, batch, steps):
-    """Incremental reveal closed over the vertex cover (default) and over
-    every decreased row (the reference's pivot set): identical tables after
-    every step, both equal to the oracle at the end."""
+    """Incremental reveal closed over the vertex cover (default: each level
+    updated or rebuilt by the cost estimate), over the cover with every level
+    always updated, with every level always rebuilt from its children, and
+    over every decreased row (the reference's pivot set): identical tables and
+    stats after every step, all equal to the oracle at the end."""
     print(f"pivot sets {kind} n={n} b={b} seed={seed} batch={robots}x{batch}", flush=True)
     truth = make(kind, n, seed)
     rv = serving.RobotReveal(truth, robots, 16, seed + 1, limit=100000)
@@ -119,18 +121,28 @@ def test_update_pivot_sets_agree(gpu_ctx, kind, n, b, seed, robots, batch, steps
     R = GpuMap(obs, b, 2)
     R.set_tuning(update_pivots=1)
     R.build()
+    U = GpuMap(obs, b, 2)
+    U.set_tuning(update_rebuild=-1)
+    U.build()
+    W = GpuMap(obs, b, 2)
+    W.set_tuning(update_rebuild=1)
+    W.build()
     assert A.tuning().update_pivots == 0 and R.tuning().update_pivots == 1
+    assert (A.tuning().update_rebuild, U.tuning().update_rebuild, W.tuning().update_rebuild) == (0, -1, 1)
     for st in range(steps):
         cells = np.vstack([rv.next_batch(r, batch) for r in range(robots)]).astype(np.int32)
         O.open_cells(cells)
-        A.open_cells(cells)
-        R.open_cells(cells)
-        assert A.last_batch_stats() == R.last_batch_stats() == O.last_batch_stats(), st
+        for M in (A, R, U, W):
+            M.open_cells(cells)
+        assert A.last_batch_stats() == R.last_batch_stats() == U.last_batch_stats() == W.last_batch_stats() == \
+            O.last_batch_stats(), st
         for lvl in range(1, O.H + 1):
             for i in range(O.num_nodes(lvl)):
-                np.testing.assert_array_equal(A.node_DU(lvl, i), R.node_DU(lvl, i), err_msg=f"step {st} D_U {lvl},{i}")
-    assert_tables_equal(A, O)
-    assert_tables_equal(R, O)
+                ref = A.node_DU(lvl, i)
+                for M in (R, U, W):
+                    np.testing.assert_array_equal(M.node_DU(lvl, i), ref, err_msg=f"step {st} D_U {lvl},{i}")
+    for M in (A, R, U, W):
+        assert_tables_equal(M, O)
     vps = grids.sample_free(O.cells, 32, seed)
     np.testing.assert_array_equal(A.query(vps), grid_bfs(O.cells, vps))
 
