@@ -26,7 +26,8 @@ Modes
   build -- 512^2 maze (width 1), b=16: batched build (graph-captured,
       autotuned) + queryAllPairs k=128.
   reveal -- 512^2 street grid, b=8: 500 steps of 256 reveals, each followed by
-      queryAllPairs k=32, checked against the reference step by step.
+      queryAllPairs k=32; every step's matrix is checked against BFS, and
+      the reference replays the steps within its CPU budget.
   multi -- 8 resident sessions of mixed shapes, mixed traffic (k <= 512,
       reveals <= 1024 cells), BFS spot checks.
 
@@ -366,9 +367,7 @@ def run_served(args):
     n_free = int((truth == grids.FREE).sum())
     uf = args.update_frac
     mode = serving.SNAPSHOT if args.consistency == "snapshot" else serving.STRICT
-    srv = serving.Server(n_slots=args.slots, max_batch_queries=args.batch_queries,
-                         max_batch_viewpoints=args.batch_viewpoints, consistency=mode,
-                         update_cu_percent=args.update_cu, update_tick_us=args.update_tick_us)
+    srv = None
     fleets = []  # one robot fleet per session, seeds 5, 12, 19, ... (orders computed once)
 
     def fleet(j):
@@ -409,9 +408,14 @@ def run_served(args):
         rmix, resmix = plain_rate_of(gm, qmix, args.warmup)
         plain[name] = {"explored": round(float((cells == 1).sum()) / n_free, 3), "k64_matrices_per_s": r64,
                        "kmix_matrices_per_s": rmix}
-        plain_maps[name] = (gm, cells, q64[args.warmup:], res64, qmix[args.warmup:], resmix)
+        plain_maps[name] = (cells, qmix[args.warmup:], resmix)
+        gm.close()  # its stream would take one of the process's hardware queues from the server
         log(f"plain per-call at {name} ({100 * plain[name]['explored']:.0f}% explored): k=64 {r64:.0f}/s, "
             f"k~logU[8,256] {rmix:.0f}/s")
+    srv = serving.Server(n_slots=args.slots, max_batch_queries=args.batch_queries,
+                         max_batch_viewpoints=args.batch_viewpoints, consistency=mode,
+                         update_cu_percent=args.update_cu, update_tick_us=args.update_tick_us,
+                         update_streams=args.update_streams)
     # plain figure over the exploration: harmonic mean of the three states
     plain_mix = 3.0 / sum(1.0 / plain[n]["kmix_matrices_per_s"] for n in plain)
     plain_64 = 3.0 / sum(1.0 / plain[n]["k64_matrices_per_s"] for n in plain)
@@ -463,7 +467,7 @@ def run_served(args):
 
     # ---- per exploration state: served (short trials, one session) against plain
     by_state = {}
-    for name in ("start", "half", "full"):
+    for name in (() if args.quick else ("start", "half", "full")):
         best = None
         for frac in (1.3, 1.1, 0.95, 0.8, 0.65, 0.5, 0.35, 0.25):
             ex = fleet(0).fresh()
@@ -486,8 +490,15 @@ def run_served(args):
         log(f"state {name}: served {best['matrices_per_s']:.0f}/s (p99 {best['p99_ms']:.3f} ms) vs plain k-mix "
             f"{plain[name]['kmix_matrices_per_s']:.0f}/s: {by_state[name]['served_over_plain_kmix']:.1f}x")
 
+    server_stats = srv.stats()
+    srv.close()
+    if args.quick:
+        return {"metric": "served (quick exploration: headline trials only)", "value": s["matrices_per_s"],
+                "p99_ms": s["p99_ms"], "update_p99_ms": s["update_p99_ms"], "rate_trials": [t[2] for t in trials]}
     # ---- kernel figures: fixed served-shaped batches on the start-state map, every dispatch timed
-    gm0, cells0, q64_0, res64_0, qmix_0, resmix_0 = plain_maps["start"]
+    cells0, qmix_0, resmix_0 = plain_maps["start"]
+    gm0 = _lib.GpuMap(cells0, B, G)
+    gm0.build()
     kst, kphase_rate = kernel_phase(gm0, serving.RobotReveal(truth, ROBOTS, RADIUS, seed=5, limit=40000),
                                     KERNEL_BATCH, args.kernel_batches)
     roof, valu, kernels = roofline_of(kst)
@@ -511,7 +522,7 @@ def run_served(args):
                    "mode": "served", "consistency": args.consistency, "window": int(s["queries"] + s["updates"]) //
                    args.steps, "slots": args.slots, "max_batch_queries": args.batch_queries,
                    "update_cu_percent": args.update_cu, "update_tick_us": args.update_tick_us, "update_frac": uf,
-                   "update_lag_ms": args.update_lag_ms},
+                   "update_lag_ms": args.update_lag_ms, "update_streams": args.update_streams or 2},
         "p50_ms": s["p50_ms"], "p99_ms": s["p99_ms"], "max_ms": s["max_ms"],
         "update_p50_ms": s["update_p50_ms"], "update_p99_ms": s["update_p99_ms"],
         "arrival_rate_per_s": s["rate"], "timed_arrival_span_s": s["timed_arrival_span_s"],
@@ -521,7 +532,7 @@ def run_served(args):
         "served_over_plain_note": "plain = the per-call path on the same k mix, harmonic mean over the start, half "
                                   "and fully revealed states; _k64 compares with plain k = 64 queries",
         "by_state": by_state, "served_checks": n_checked,
-        "server_stats": srv.stats(),
+        "server_stats": server_stats,
         "rate_trials": [t[2] for t in trials],
         "roofline": roof, "valu": valu, "cpu_baseline": cpu, "kernels": kernels,
         "kernel_phase": {"batch_queries": KERNEL_BATCH, "batches": args.kernel_batches,
@@ -961,9 +972,19 @@ def run_reveal(args):
     _, _, kernels = kernel_roofline(gk)
     gk.close()
     cpu = None
+    checked = 0
     if not args.no_cpu:
         from oracle import oracle as orc
 
+        # every step's matrix against BFS on that step's grid
+        state = obs.copy()
+        for st in range(steps):
+            c = batches[st]
+            state[c[:, 0], c[:, 1]] = 1
+            if not np.array_equal(orc.grid_bfs(state, views[st][0], orc.bfs_threads()), views[st][1]):
+                raise SystemExit(f"reveal step {st}: GPU matrix differs from BFS")
+            checked += 1
+        log(f"exactness: all {checked} step matrices equal BFS")
         if orc.available("reference"):
             R = orc.Oracle("reference", obs, 8, 2)
             assert R.build()
@@ -989,7 +1010,7 @@ def run_reveal(args):
                                    "reveals per step + queryAllPairs k=32", "mode": "reveal",
                        "update_pivots": args.update_pivots},
             "step_p50_ms": float(np.percentile(lat, 50)), "step_p99_ms": float(np.percentile(lat, 99)),
-            "kernels": kernels, "cpu_baseline": cpu}
+            "steps_checked_against_bfs": checked, "kernels": kernels, "cpu_baseline": cpu}
 
 
 def run_multi(args):
@@ -1009,7 +1030,8 @@ def run_multi(args):
             truths.append(grids.maze_grid(n, seed, 1 if kind == "maze1" else 16))
     need = sum(_lib.footprint(b, 2, n, n) for _, n, b, _ in maps) * 2
     srv = serving.Server(hbm_budget=int(need * 1.05), n_slots=args.slots, max_batch_queries=args.batch_queries,
-                         max_batch_viewpoints=args.batch_viewpoints, consistency=serving.SNAPSHOT)
+                         max_batch_viewpoints=args.batch_viewpoints, consistency=serving.SNAPSHOT,
+                         update_streams=args.update_streams)
     log(f"8 sessions, HBM budget {need * 1.05 / 2**30:.1f} GiB (two copies each)")
 
     def fresh():
@@ -1105,6 +1127,8 @@ def main():
     ap.add_argument("--update-cu", type=int, default=0, help="percent of CUs for update kernels (0 = all)")
     ap.add_argument("--update-frac", type=float, default=0.1, help="share of update requests in the stream")
     ap.add_argument("--update-tick-us", type=int, default=0, help="snapshot mode: minimum spacing of update batches")
+    ap.add_argument("--update-streams", type=int, default=0,
+                    help="served: update streams (and snapshot update threads) the sessions are dealt to; 0 = 2")
     ap.add_argument("--update-pivots", default="cover", choices=sorted(PIVOT_SETS),
                     help="update and reveal modes: close over a vertex cover of the decreased entries, or over "
                          "every decreased row (the reference's pivot set)")
@@ -1128,6 +1152,8 @@ def main():
                              0.5, 0.4, 0.3, 0.22])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--quick", action="store_true", help="served: headline trials only (no per-state table, no "
+                                                         "kernel phase, no CPU leg)")
     args = ap.parse_args()
     if args.steps is None:
         args.steps = 200 if args.mode == "served" else 40

@@ -43,7 +43,8 @@ bool launch_leaf_apsp(const uint8_t* pass, int S, int b, int nl, const int* leaf
 // level's S_U.
 void launch_node_assemble(int lvl, int g, int pc, int nV, int ld, int nt, const uint16_t* child_sum,
                           size_t child_stride, const int* node_list, int n_list, uint16_t* DU, size_t node_stride,
-                          hipStream_t st);
+                          hipStream_t st, const int* keep_cnt = nullptr,
+                          const int* keep_list = nullptr, int ng8 = 0);
 
 // Seam edges: for each (u, v) pair of the level's seam template (relative
 // cell coordinates in vu_rc) whose cells are both passable, D_U[u][v] and

@@ -163,15 +163,33 @@ bool launch_leaf_apsp(const uint8_t* pass, int S, int b, int nl, const int* leaf
 // D_U assembly (src/internal_node.cpp:98-145): identity + child summaries on
 // the diagonal blocks.  A child summary with no finite off-diagonal entry is
 // the identity, so copying every child unconditionally gives the same D_U.
+// An octet is in a node's sorted keep list (binary search).
+__device__ __forceinline__ bool in_octets(const int* __restrict__ list, int n, int o) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (list[mid] < o) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < n && list[lo] == o;
+}
+
 __global__ __launch_bounds__(256) void k_node_assemble(int g, int pc, int nV, int ld, int nt, int child_nt,
                                                        const uint16_t* __restrict__ child_sum, size_t child_stride,
                                                        const int* __restrict__ node_list,
-                                                       uint16_t* __restrict__ DU, size_t node_stride) {
+                                                       uint16_t* __restrict__ DU, size_t node_stride,
+                                                       const int* __restrict__ keep_cnt,
+                                                       const int* __restrict__ keep_list, int ng8) {
     const int groups_per_row = ld / 8;
     const long long gid = (long long)blockIdx.x * 256 + threadIdx.x;
     if (gid >= (long long)nV * groups_per_row) return;
     const int v = int(gid / groups_per_row), u0 = int(gid % groups_per_row) * 8;
     const int node = node_list[blockIdx.y];
+    if (keep_cnt) {  // entries whose row and column octets are both kept stay as they are
+        const int* kl = keep_list + size_t(node) * ng8;
+        const int kn = keep_cnt[node];
+        if (in_octets(kl, kn, v / 8) && in_octets(kl, kn, u0 / 8)) return;
+    }
     const int tx = node % nt, ty = node / nt;
     const int qv = v / pc, pv = v % pc;
     const int cidx = (ty * g + qv / g) * child_nt + (tx * g + qv % g);
@@ -192,12 +210,12 @@ __global__ __launch_bounds__(256) void k_node_assemble(int g, int pc, int nV, in
 
 void launch_node_assemble(int lvl, int g, int pc, int nV, int ld, int nt, const uint16_t* child_sum,
                           size_t child_stride, const int* node_list, int n_list, uint16_t* DU, size_t node_stride,
-                          hipStream_t st) {
+                          hipStream_t st, const int* keep_cnt, const int* keep_list, int ng8) {
     (void)lvl;
     const long long total = (long long)nV * (ld / 8);
     dim3 grid(unsigned((total + 255) / 256), unsigned(n_list));
     hipLaunchKernelGGL(k_node_assemble, grid, dim3(256), 0, st, g, pc, nV, ld, nt, nt * g, child_sum, child_stride,
-                       node_list, DU, node_stride);
+                       node_list, DU, node_stride, keep_cnt, keep_list, ng8);
 }
 
 // ---------------------------------------------------------------------------
@@ -217,13 +235,11 @@ __global__ __launch_bounds__(256) void k_seam_relax(int nV, int ld, int s, int n
     uint16_t* D = DU + size_t(node) * node_stride;
     uint16_t& uv = D[size_t(e.x) * ld + e.y];
     uint16_t& vu = D[size_t(e.y) * ld + e.x];
-    if (uv > 1 || vu > 1) {
-        uv = uv > 1 ? 1 : uv;
-        vu = vu > 1 ? 1 : vu;
-        if (flags) {
-            flags[size_t(blockIdx.y) * nV + e.x] = 1;
-            flags[size_t(blockIdx.y) * nV + e.y] = 1;
-        }
+    if (uv > 1) uv = 1;
+    if (vu > 1) vu = 1;
+    if (flags) {  // every passable seam edge's endpoints are pivots
+        flags[size_t(blockIdx.y) * nV + e.x] = 1;
+        flags[size_t(blockIdx.y) * nV + e.y] = 1;
     }
 }
 

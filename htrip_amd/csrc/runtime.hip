@@ -307,16 +307,19 @@ static void make_level(LevelGeo& L, int lvl, int b, int g, int S, int H) {
 }
 
 int Map::init_live() {
-    // per level: cnt (nn), list (nn * ngrp), cnt8 (nn), list8 (nn * ng8) ints, then map (nn * ngrp) bytes
+    // per level: cnt (nn), list (nn * ngrp), cnt8 (nn), list8 (nn * ng8) ints, map (nn * ngrp) bytes, then at
+    // the root its full octet lists: cnt (nn), list (nn * ng8) ints
     auto a16 = [](size_t x) { return (x + 15) / 16 * 16; };
-    std::vector<size_t> off(lv.size() * 5);
+    std::vector<size_t> off(lv.size() * 7);
     size_t total = 0;
     for (size_t l = 0; l < lv.size(); ++l) {
-        const LevelGeo& L = lv[l];
+        LevelGeo& L = lv[l];
+        L.seam_only = l + 1 == lv.size();
         const size_t nn = size_t(L.nn);
-        const size_t sz[5] = {nn * 4, nn * size_t(L.ngrp) * 4, nn * 4, nn * size_t(L.ng8) * 4, nn * size_t(L.ngrp)};
-        for (int k = 0; k < 5; ++k) {
-            off[l * 5 + k] = total;
+        const size_t sz[7] = {nn * 4, nn * size_t(L.ngrp) * 4, nn * 4, nn * size_t(L.ng8) * 4, nn * size_t(L.ngrp),
+                              L.seam_only ? nn * 4 : 0, L.seam_only ? nn * size_t(L.ng8) * 4 : 0};
+        for (int k = 0; k < 7; ++k) {
+            off[l * 7 + k] = total;
             total += a16(std::max<size_t>(sz[k], 16));
         }
     }
@@ -327,16 +330,21 @@ int Map::init_live() {
     char* d = static_cast<char*>(live_d.p);
     for (size_t l = 0; l < lv.size(); ++l) {
         LevelGeo& L = lv[l];
-        L.h_live_cnt = reinterpret_cast<int*>(h + off[l * 5 + 0]);
-        L.h_live_list = reinterpret_cast<int*>(h + off[l * 5 + 1]);
-        L.h_live8_cnt = reinterpret_cast<int*>(h + off[l * 5 + 2]);
-        L.h_live8_list = reinterpret_cast<int*>(h + off[l * 5 + 3]);
-        L.h_live_map = reinterpret_cast<uint8_t*>(h + off[l * 5 + 4]);
-        L.d_live_cnt = reinterpret_cast<const int*>(d + off[l * 5 + 0]);
-        L.d_live_list = reinterpret_cast<const int*>(d + off[l * 5 + 1]);
-        L.d_live8_cnt = reinterpret_cast<const int*>(d + off[l * 5 + 2]);
-        L.d_live8_list = reinterpret_cast<const int*>(d + off[l * 5 + 3]);
-        L.d_live_map = reinterpret_cast<const uint8_t*>(d + off[l * 5 + 4]);
+        L.h_live_cnt = reinterpret_cast<int*>(h + off[l * 7 + 0]);
+        L.h_live_list = reinterpret_cast<int*>(h + off[l * 7 + 1]);
+        L.h_live8_cnt = reinterpret_cast<int*>(h + off[l * 7 + 2]);
+        L.h_live8_list = reinterpret_cast<int*>(h + off[l * 7 + 3]);
+        L.h_live_map = reinterpret_cast<uint8_t*>(h + off[l * 7 + 4]);
+        L.d_live_cnt = reinterpret_cast<const int*>(d + off[l * 7 + 0]);
+        L.d_live_list = reinterpret_cast<const int*>(d + off[l * 7 + 1]);
+        L.d_live8_cnt = reinterpret_cast<const int*>(d + off[l * 7 + 2]);
+        L.d_live8_list = reinterpret_cast<const int*>(d + off[l * 7 + 3]);
+        L.d_live_map = reinterpret_cast<const uint8_t*>(d + off[l * 7 + 4]);
+        const int a = L.seam_only ? 5 : 2, b = L.seam_only ? 6 : 3;
+        L.h_all8_cnt = reinterpret_cast<int*>(h + off[l * 7 + a]);
+        L.h_all8_list = reinterpret_cast<int*>(h + off[l * 7 + b]);
+        L.d_all8_cnt = reinterpret_cast<const int*>(d + off[l * 7 + a]);
+        L.d_all8_list = reinterpret_cast<const int*>(d + off[l * 7 + b]);
     }
     hipError_t e;
     if (!live_evt && (e = hipEventCreateWithFlags(&live_evt, hipEventDisableTiming)) != hipSuccess)
@@ -409,13 +417,19 @@ int Map::refresh_live(const std::vector<int2>* cells) {
                         ns_live += live_port(q * L.d.pc + L.seam_list[i]);
                     L.seam_live[size_t(W) * g * g + q] = ns_live;
                 }
-                int c8 = 0;
+                int c8 = 0, a8 = 0;
                 for (int o = 0; o < L.ng8; ++o) {
-                    bool live = false;
-                    for (int v = 8 * o; v < 8 * o + 8 && v < L.d.nV && !live; ++v) live = live_port(v);
-                    if (live) L.h_live8_list[size_t(W) * L.ng8 + c8++] = o;
+                    bool live = false, seam = false;
+                    for (int v = 8 * o; v < 8 * o + 8 && v < L.d.nV && !seam; ++v)
+                        if (live_port(v)) {
+                            live = true;
+                            seam = L.seam_flag[size_t(v)] != 0;
+                        }
+                    if (L.seam_only ? seam : live) L.h_live8_list[size_t(W) * L.ng8 + c8++] = o;
+                    if (L.seam_only && live) L.h_all8_list[size_t(W) * L.ng8 + a8++] = o;
                 }
                 L.h_live8_cnt[W] = c8;
+                if (L.seam_only) L.h_all8_cnt[W] = a8;
             }
             a = 0;
             z = L.nn - 1;
@@ -435,9 +449,17 @@ int Map::refresh_live(const std::vector<int2>* cells) {
                     L.h_live_cnt[W] = insert_sorted(L.h_live_list + size_t(W) * ng, L.h_live_cnt[W], gi);
                 }
                 if (L.seam_flag[size_t(v)]) L.seam_live[size_t(W) * g * g + q]++;
-                int* l8 = L.h_live8_list + size_t(W) * L.ng8;
-                const int n8 = L.h_live8_cnt[W], o = v / 8;
-                if (!std::binary_search(l8, l8 + n8, o)) L.h_live8_cnt[W] = insert_sorted(l8, n8, o);
+                const int o = v / 8;
+                if (!L.seam_only || L.seam_flag[size_t(v)]) {
+                    int* l8 = L.h_live8_list + size_t(W) * L.ng8;
+                    const int n8 = L.h_live8_cnt[W];
+                    if (!std::binary_search(l8, l8 + n8, o)) L.h_live8_cnt[W] = insert_sorted(l8, n8, o);
+                }
+                if (L.seam_only) {
+                    int* a8 = L.h_all8_list + size_t(W) * L.ng8;
+                    const int n8 = L.h_all8_cnt[W];
+                    if (!std::binary_search(a8, a8 + n8, o)) L.h_all8_cnt[W] = insert_sorted(a8, n8, o);
+                }
             }
         }
         if (z < a) continue;
@@ -446,6 +468,10 @@ int Map::refresh_live(const std::vector<int2>* cells) {
         touch(L.h_live_list + a0 * ng, (z0 - a0) * ng * 4);
         touch(L.h_live8_cnt + a0, (z0 - a0) * 4);
         touch(L.h_live8_list + a0 * L.ng8, (z0 - a0) * L.ng8 * 4);
+        if (L.seam_only) {
+            touch(L.h_all8_cnt + a0, (z0 - a0) * 4);
+            touch(L.h_all8_list + a0 * L.ng8, (z0 - a0) * L.ng8 * 4);
+        }
         touch(L.h_live_map + a0 * ng, (z0 - a0) * ng);
     }
     if (hi <= lo) return HTG_OK;
@@ -536,13 +562,17 @@ int Map::ensure_build_ws(int lvl) {
 
 // Closure over each listed node's pivot list (already compacted into ws_piv
 // with counts in ws_counts; host copy of counts in hcounts).
-int Map::run_closure(int lvl, const int* nodes, const std::vector<int>& hnodes, const std::vector<int>& hcounts) {
+int Map::run_closure(int lvl, const int* nodes, const std::vector<int>& hnodes, const std::vector<int>& hcounts,
+                     bool full) {
     LevelGeo& L = lv[lvl - 1];
+    const int* h8 = full ? L.h_all8_cnt : L.h_live8_cnt;
+    const int* d8c = full ? L.d_all8_cnt : L.d_live8_cnt;
+    const int* d8l = full ? L.d_all8_list : L.d_live8_list;
     const int n_list = int(hnodes.size());
     int maxc = 0, max8 = 0;
     for (int i = 0; i < n_list; ++i) {
         maxc = std::max(maxc, hcounts[i]);
-        if (hcounts[i] > 0) max8 = std::max(max8, L.h_live8_cnt[hnodes[i]]);
+        if (hcounts[i] > 0) max8 = std::max(max8, h8[hnodes[i]]);
     }
     const int P = tune.pivot_block;
     const int blocks = (maxc + P - 1) / P;
@@ -552,7 +582,7 @@ int Map::run_closure(int lvl, const int* nodes, const std::vector<int>& hnodes, 
             for (int i = 0; i < n_list; ++i) {
                 const double p = std::max(0, std::min(P, hcounts[i] - t * P));
                 if (p <= 0) continue;
-                const double nV = 8.0 * L.h_live8_cnt[hnodes[i]];  // live rows / columns
+                const double nV = 8.0 * h8[hnodes[i]];  // live rows / columns
                 upd_p += p * p * nV + p * p * p;
                 by_p += (p * nV + p * nV) * 2;  // pivot rows in, R out
                 upd_u += p * nV * nV;
@@ -563,13 +593,13 @@ int Map::run_closure(int lvl, const int* nodes, const std::vector<int>& hnodes, 
         int pi = prof_open(HTG_K_CLOSURE_PIVOTS, by_p, upd_p, kt);
         bool la = launch_closure_pivots(P, L.d.nV, L.d.ld, nodes, n_list, static_cast<const int*>(ws_piv.p), L.d.nV,
                                         static_cast<const int*>(ws_counts.p), t, static_cast<const uint16_t*>(L.DU.p),
-                                        L.node_stride, L.d_live8_cnt, L.d_live8_list, L.ng8, max8,
-                                        static_cast<uint16_t*>(ws_R.p), stream, kt);
+                                        L.node_stride, d8c, d8l, L.ng8, max8, static_cast<uint16_t*>(ws_R.p), stream,
+                                        kt);
         prof_ref().close(pi, la);
         pi = prof_open(HTG_K_CLOSURE_UPDATE, by_u, upd_u, kt);
         la = launch_closure_update(P, L.d.nV, L.d.ld, nodes, n_list, static_cast<const int*>(ws_counts.p), t,
-                                   static_cast<uint16_t*>(L.DU.p), L.node_stride, L.d_live8_cnt, L.d_live8_list,
-                                   L.ng8, max8, static_cast<const uint16_t*>(ws_R.p), stream, kt);
+                                   static_cast<uint16_t*>(L.DU.p), L.node_stride, d8c, d8l, L.ng8, max8,
+                                   static_cast<const uint16_t*>(ws_R.p), stream, kt);
         prof_ref().close(pi, la);
     }
     hipError_t e = hipGetLastError();
@@ -707,6 +737,7 @@ int Map::rebuild_level(int lvl, const int* dnodes, const std::vector<int>& hn, i
     LevelGeo& L = lv[size_t(lvl) - 1];
     const int na = int(hn.size());
     if (na == 0) return HTG_OK;
+    if (L.seam_only) root_full = false;
     int rc;
     if ((rc = ensure_build_ws(lvl))) return rc;
     launch_fill_u8(static_cast<uint8_t*>(ws_flags.p), size_t(na) * L.d.nV, 0, stream);
@@ -725,6 +756,40 @@ int Map::rebuild_level(int lvl, const int* dnodes, const std::vector<int>& hn, i
                       stream);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? HTG_OK : hip_fail(e, "rebuild launch");
+}
+
+// The root's full table from the maintained one: entries with a row or a
+// column outside the maintained octets are assembled afresh from the
+// children, then one closure over the seam endpoints runs over every live
+// octet.  The maintained block is exact already: the closure only rewrites
+// its entries with the values they hold, so queries reading it from other
+// streams meanwhile see no change.  S_U follows from the full table.
+int Map::materialize_root() {
+    if (H == 0 || root_full || !built || broken) return HTG_OK;
+    LevelGeo& L = lv[size_t(H) - 1];
+    int rc;
+    if ((rc = ensure_build_ws(H))) return rc;
+    const int* nodes = static_cast<const int*>(L.d_all.p);
+    const std::vector<int> root(size_t(L.nn), 0);  // one node
+    launch_fill_u8(static_cast<uint8_t*>(ws_flags.p), size_t(L.nn) * L.d.nV, 0, stream);
+    size_t cstride;
+    const uint16_t* csum = child_summaries(H, cstride);
+    launch_node_assemble(H, g, L.d.pc, L.d.nV, L.d.ld, L.d.nt, csum, cstride, nodes, L.nn,
+                         static_cast<uint16_t*>(L.DU.p), L.node_stride, stream, L.d_live8_cnt, L.d_live8_list, L.ng8);
+    launch_seam_relax(L.d.nV, L.d.ld, L.d.s, L.d.nt, static_cast<const int2*>(L.d_seams.p), int(L.seams.size()),
+                      static_cast<const short2*>(L.d_vu_rc.p), static_cast<const uint8_t*>(d_pass.p), S, nodes, L.nn,
+                      static_cast<uint16_t*>(L.DU.p), L.node_stride, static_cast<uint8_t*>(ws_flags.p), stream);
+    launch_compact_flags(L.d.nV, nodes, L.nn, static_cast<uint8_t*>(ws_flags.p), static_cast<int*>(ws_piv.p), L.d.nV,
+                         static_cast<int*>(ws_counts.p), stream);
+    if ((rc = run_closure(H, nodes, root, build_pivot_counts(H, &root), true))) return rc;
+    launch_project_su(L.d.nV, L.d.ld, L.d.ne, static_cast<const int*>(L.d_ext_idx.p), nodes, L.nn,
+                      static_cast<const uint16_t*>(L.DU.p), L.node_stride, static_cast<uint16_t*>(L.SU.p), nullptr,
+                      stream);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return hip_fail(e, "root closure");
+    root_full = true;
+    return HTG_OK;
 }
 
 // Recompute the listed leaves (BFS from the passability) and rebuild the
@@ -880,6 +945,7 @@ int Map::build() {
     if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "build sync");
     prof_ref().harvest();
     built = true;
+    root_full = false;  // the build closes the root over its maintained octets only
     broken = false;
     epoch++;
     reset_versions();
@@ -1198,6 +1264,7 @@ int Map::open_cells(const int32_t* rcs, int n) {
         }
         top = lvl;
         if (top_first == 0) top_first = lvl;
+        if (L.seam_only) root_full = false;
         if ((rc = ensure_build_ws(lvl))) return rc;
         const int* dnodes = reinterpret_cast<const int*>(up + o_cand[size_t(lvl)]);
         int max8 = 0;
@@ -1419,6 +1486,7 @@ int Map::copy_all_from(const Map& src) {
     stats = src.stats;
     epoch = src.epoch;
     built = src.built;
+    root_full = src.root_full;
     broken = false;
     reset_versions();
     return HTG_OK;
@@ -1442,6 +1510,7 @@ int Map::query(const int32_t* rcs, const int32_t* ks, int nq, uint16_t* out_host
 int Map::lift_point(int r, int c, uint16_t* x, long long x_cap, int32_t* lens, int32_t* parent, int32_t* quad,
                     int32_t* offset) {
     if (r < 0 || c < 0 || r >= S || c >= S) return fail(HTG_E_RANGE, "point outside the hierarchy");
+    if (int mr = materialize_root()) return mr;  // the root's exterior rows
     std::vector<long long> off(size_t(H) + 2, 0);
     lens[0] = 4 * b - 4;
     for (int l = 1; l <= H; ++l) lens[l] = lv[size_t(l) - 1].d.ne;
@@ -2164,6 +2233,7 @@ int map_clone_on(const htg_map* src, uint64_t hbm_budget, hipStream_t shared, ht
     m->pass = src->pass;
     m->tune = src->tune;
     m->built = src->built;
+    m->root_full = src->root_full;
     m->stats = src->stats;
     m->epoch = src->epoch;
     m->gen = src->gen;
@@ -2214,6 +2284,8 @@ int lift_root(htg_map* src, int32_t rows, int32_t cols, const uint8_t* cells, ht
     hipError_t e = hipSetDevice(src->ctx->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     if ((e = hipStreamSynchronize(src->stream)) != hipSuccess) return hip_fail(e, "sync");
+    // the old root becomes an inner node: every entry of its table counts
+    if (int mr = src->materialize_root()) return mr;
     std::vector<uint8_t> merged(size_t(rows) * cols);
     for (int r = 0; r < rows; ++r)
         for (int c = 0; c < cols; ++c) {
@@ -2567,6 +2639,16 @@ int htg_sync(htg_map* map) {
     return HTG_OK;
 }
 
+// Table reads at the root see its full table (under the map's query lock,
+// as htg_lift_point does).
+static int materialize_locked(htg_map* map, int lvl) {
+    if (lvl != map->H || map->root_full || !map->built || map->broken) return HTG_OK;
+    std::lock_guard<std::mutex> lk(map->qmu);
+    const hipError_t e = hipSetDevice(map->ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    return map->materialize_root();
+}
+
 static int read_rows(htg_map* map, const void* src, size_t rows, size_t cols, size_t ld, uint16_t* out) {
     hipError_t e = hipSetDevice(map->ctx->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
@@ -2606,6 +2688,7 @@ int htg_read_node_DU(htg_map* map, int32_t lvl, int32_t idx, uint16_t* out) {
     if (!map || !out) return fail(HTG_E_INVALID, "null argument");
     if (lvl < 1 || lvl > map->H || idx < 0 || idx >= map->lv[lvl - 1].nn) return fail(HTG_E_RANGE, "node index");
     const LevelGeo& L = map->lv[lvl - 1];
+    if (int rc = materialize_locked(map, lvl)) return rc;
     return read_rows(map, static_cast<const uint16_t*>(L.DU.p) + size_t(idx) * L.node_stride, L.d.nV, L.d.nV, L.d.ld,
                      out);
 }
@@ -2615,6 +2698,7 @@ int htg_read_node_SU(htg_map* map, int32_t lvl, int32_t idx, uint16_t* out) {
     if (lvl < 1 || lvl > map->H || idx < 0 || idx >= map->lv[lvl - 1].nn) return fail(HTG_E_RANGE, "node index");
     const LevelGeo& L = map->lv[lvl - 1];
     const size_t ne = size_t(L.d.ne);
+    if (int rc = materialize_locked(map, lvl)) return rc;
     return read_rows(map, static_cast<const uint16_t*>(L.SU.p) + size_t(idx) * ne * ne, ne, ne, ne, out);
 }
 

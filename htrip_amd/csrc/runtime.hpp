@@ -114,6 +114,15 @@ struct LevelGeo {
     uint8_t* h_live_map = nullptr;
     const int *d_live_cnt = nullptr, *d_live_list = nullptr, *d_live8_cnt = nullptr, *d_live8_list = nullptr;
     const uint8_t* d_live_map = nullptr;
+    // At the root the queries read only the seam x seam block of D_U, and a
+    // closure whose pivots are seam ports keeps that block exact by itself:
+    // the root's live octets (h_live8_*, used by every build and update
+    // kernel) count passable seam ports only, and its full octet lists
+    // (h_all8_*, every passable port) serve the on-demand closure of the
+    // whole table.  Below the root both name the same arrays.
+    bool seam_only = false;
+    int *h_all8_cnt = nullptr, *h_all8_list = nullptr;
+    const int *d_all8_cnt = nullptr, *d_all8_list = nullptr;
     std::vector<int> seam_live;  // [W * g^2 + q]: passable seam ports of child q (rows a projection may use)
     DevBuf d_vu_rc, d_ext_idx, d_ext_child, d_ext_cport, d_seams, d_seam_list, d_seam_off, d_zcols, d_zsrc, d_zoff,
         d_zlen, d_ext_zpos, d_ext_col, d_colpos, d_colext, d_vu_ext, d_all;  // d_all: 0..nn-1
@@ -389,7 +398,16 @@ struct Map {
     // byte range with one asynchronous copy on the map stream.
     int refresh_live(const std::vector<int2>* cells);
     // nodes: the listed nodes (host copy of ws_nodes)
-    int run_closure(int lvl, const int* dnodes, const std::vector<int>& nodes, const std::vector<int>& hcounts);
+    // full: close over every live octet (the root's full lists) instead of
+    // the maintained ones.
+    int run_closure(int lvl, const int* dnodes, const std::vector<int>& nodes, const std::vector<int>& hcounts,
+                    bool full = false);
+    // The root's entries outside its seam x seam block (outer-boundary
+    // ports) are only kept exact on demand: root_full says they are, and
+    // materialize_root() rebuilds the root over its full octet lists when
+    // they are not (liftPoint, table reads, liftRoot).
+    bool root_full = false;
+    int materialize_root();
     const uint16_t* child_summaries(int lvl, size_t& stride) const;
     // Make this map equal to src after src applied one update with dirty set
     // ds (this map equal to src before it): copies only the rewritten tables.

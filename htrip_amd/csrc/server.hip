@@ -115,10 +115,10 @@ struct htg_server {
     int next_session = 1;
     uint64_t used = 0;
     std::vector<std::unique_ptr<Batch>> pool;
-    hipStream_t ustream = nullptr;  // builds and updates of every session
+    std::vector<hipStream_t> ustreams;  // builds and updates; session i runs on ustreams[i % size]
     std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
     uint64_t next_ticket = 1;
-    std::thread updater;
+    std::vector<std::thread> updaters;  // snapshot mode: one per update stream
     int rr = 0;  // session where the next slot starts looking (round robin)
     bool stop = false;
     htg_server_stats stats{};
@@ -360,6 +360,7 @@ int htg_server_create(htg_ctx* ctx, const htg_server_config* cfg, htg_server** o
         if (!ctx || !cfg || !out) return fail(HTG_E_INVALID, "null argument");
         if (cfg->n_slots < 1 || cfg->n_slots > 64 || cfg->max_batch_queries < 1 || cfg->max_batch_viewpoints < 1 ||
             cfg->update_cu_percent < 0 || cfg->update_cu_percent > 100 || cfg->update_tick_us < 0 ||
+            cfg->update_streams < 0 || cfg->update_streams > 8 ||
             (cfg->consistency != HTG_CONSISTENCY_STRICT && cfg->consistency != HTG_CONSISTENCY_SNAPSHOT))
             return fail(HTG_E_INVALID, "bad server configuration");
         hipError_t e = hipSetDevice(ctx->device);
@@ -367,13 +368,20 @@ int htg_server_create(htg_ctx* ctx, const htg_server_config* cfg, htg_server** o
         std::unique_ptr<htg_server> s(new htg_server());
         s->ctx = ctx;
         s->cfg = *cfg;
-        // One update stream and n_slots query streams: a process has few
+        // A few update streams and n_slots query streams: a process has few
         // hardware queues (GPU_MAX_HW_QUEUES, 4 by default), and streams
         // beyond that share queues, so that a query would wait behind update
-        // kernels.  With update_cu_percent the update stream runs on that
-        // share of the CUs only, and queries always find free CUs.
-        if ((e = make_update_stream(ctx->device, cfg->update_cu_percent, &s->ustream)) != hipSuccess)
-            return hip_fail(e, "update stream");
+        // kernels.  Sessions are dealt to the update streams, so updates of
+        // sessions on different streams overlap on the device.  With
+        // update_cu_percent the update streams run on that share of the CUs
+        // only, and queries always find free CUs.
+        const int nus = cfg->update_streams > 0 ? cfg->update_streams : 2;
+        for (int i = 0; i < nus; ++i) {
+            hipStream_t us = nullptr;
+            if ((e = make_update_stream(ctx->device, cfg->update_cu_percent, &us)) != hipSuccess)
+                return hip_fail(e, "update stream");
+            s->ustreams.push_back(us);
+        }
         for (int i = 0; i < cfg->n_slots; ++i) {
             s->pool.emplace_back(new Batch());
             int rc = s->pool.back()->ws.init();
@@ -384,7 +392,8 @@ int htg_server_create(htg_ctx* ctx, const htg_server_config* cfg, htg_server** o
             Batch* bb = bp.get();
             bb->th = std::thread([raw, bb] { raw->slot_loop(bb); });
         }
-        if (raw->snapshot()) s->updater = std::thread([raw] { raw->run_updates(); });
+        if (raw->snapshot())
+            for (size_t i = 0; i < s->ustreams.size(); ++i) s->updaters.emplace_back([raw] { raw->run_updates(); });
         *out = s.release();
         return HTG_OK;
     } catch (...) {
@@ -402,12 +411,13 @@ int htg_server_destroy(htg_server* s) {
     s->cv_upd.notify_all();
     for (auto& bp : s->pool)
         if (bp->th.joinable()) bp->th.join();
-    if (s->updater.joinable()) s->updater.join();
+    for (auto& t : s->updaters)
+        if (t.joinable()) t.join();
     for (auto& kv : s->sessions)
         for (htg_map* m : kv.second.maps)
             if (m) htg_map_destroy(m);
     s->pool.clear();
-    if (s->ustream) (void)hipStreamDestroy(s->ustream);
+    for (hipStream_t us : s->ustreams) (void)hipStreamDestroy(us);
     delete s;
     return HTG_OK;
 }
@@ -428,10 +438,15 @@ int htg_server_add_map(htg_server* s, const htg_config* cfg, int32_t rows, int32
         s->used += need;  // reserve before building
     }
     htg_map* m[2] = {nullptr, nullptr};
-    rc = map_create_on(s->ctx, cfg, rows, cols, cells, 0, s->ustream, &m[0]);
+    hipStream_t us;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        us = s->ustreams[size_t(s->next_session) % s->ustreams.size()];  // the next session id's stream
+    }
+    rc = map_create_on(s->ctx, cfg, rows, cols, cells, 0, us, &m[0]);
     if (rc == HTG_OK) rc = htg_build(m[0]);
     if (rc == HTG_OK && copies == 2) {
-        rc = map_clone_on(m[0], 0, s->ustream, &m[1]);
+        rc = map_clone_on(m[0], 0, us, &m[1]);
         if (rc == HTG_OK) m[1]->profp = m[0]->profp;  // one profiler per session
     }
     // size every slot's workspace for this map's largest batch up front

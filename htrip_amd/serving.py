@@ -26,7 +26,7 @@ STRICT, SNAPSHOT = 0, 1
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("n_slots", C.c_int32), ("max_batch_queries", C.c_int32),
                 ("max_batch_viewpoints", C.c_int32), ("consistency", C.c_int32), ("update_cu_percent", C.c_int32),
-                ("update_tick_us", C.c_int32)]
+                ("update_tick_us", C.c_int32), ("update_streams", C.c_int32)]
 
 
 class ServerStats(C.Structure):
@@ -76,13 +76,13 @@ class Server:
 
     def __init__(self, hbm_budget: int = 0, n_slots: int = 4, max_batch_queries: int = 64,
                  max_batch_viewpoints: int = 8192, consistency: int = STRICT, update_cu_percent: int = 0,
-                 update_tick_us: int = 0, ctx=None):
+                 update_tick_us: int = 0, update_streams: int = 0, ctx=None):
         self.ctx = ctx or _lib.default_context()
         self.lib = self.ctx.lib
         _bind(self.lib)
         self.h = C.c_void_p()
         cfg = ServerConfig(hbm_budget, n_slots, max_batch_queries, max_batch_viewpoints, consistency,
-                           update_cu_percent, update_tick_us)
+                           update_cu_percent, update_tick_us, update_streams)
         check(self.lib.htg_server_create(self.ctx.h, C.byref(cfg), C.byref(self.h)))
         self._keep = {}
 

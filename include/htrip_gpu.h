@@ -268,9 +268,9 @@ int htg_profile_get(htg_map* map, int kernel_class, htg_kernel_stats* out);
  * Up to n_slots query batches are in flight at once on their own streams,
  * and requests that arrive meanwhile join the next batch as soon as a slot
  * frees.  Each stream needs a hardware queue of its own (the process's
- * GPU_MAX_HW_QUEUES >= n_slots + 1 plus any other streams; HIP's default is
- * 4): streams that share a queue run in order behind each other.  No
- * counterpart in the reference's C++ API. */
+ * GPU_MAX_HW_QUEUES >= n_slots + update_streams plus any other streams;
+ * HIP's default is 4): streams that share a queue run in order behind each
+ * other.  No counterpart in the reference's C++ API. */
 enum {
     HTG_CONSISTENCY_STRICT = 0,  /* a query sees exactly the updates submitted before it */
     HTG_CONSISTENCY_SNAPSHOT = 1 /* double-buffered: a query sees the newest applied prefix of updates */
@@ -284,6 +284,9 @@ typedef struct htg_server_config {
     int32_t update_cu_percent;    /* builds/updates run on this share of the CUs (1..99); 0 or 100 = all */
     int32_t update_tick_us;       /* snapshot mode: a session's update batches start at least this far apart,
                                      so more reveals merge into one closure (0 = as soon as possible) */
+    int32_t update_streams;       /* builds and updates run on this many streams (sessions are dealt to them
+                                     round robin, and in snapshot mode as many update threads serve them), so
+                                     updates of different sessions overlap on the device; 0 = 2, at most 8 */
 } htg_server_config;
 typedef struct htg_server_stats {
     uint64_t query_batches, queries, update_batches, updates, hbm_used;
