@@ -89,10 +89,11 @@ void launch_compact_flags_live(int nV, const int* node_list, int n_list, const i
 // (counts as left by launch_compact_flags_live), then the C x C second hop.
 bool launch_cover_relax(int nV, int ld, const int* node_list, int n_list, const int* counts, const int* cpiv,
                         const int* bpiv, int piv_stride, const int* live8_cnt, const int* live8_list, int ng8,
-                        int max_live8, int max_c, uint16_t* DU, size_t node_stride, hipStream_t st, KTimer kt = {});
-bool launch_cover_pairs(int nV, int ld, const int* node_list, int n_list, const int* counts, const int* cpiv,
-                        const int* bpiv, int piv_stride, int max_c, uint16_t* DU, size_t node_stride, hipStream_t st,
+                        int max_live8, int max_c, int max_b, uint16_t* DU, size_t node_stride, hipStream_t st,
                         KTimer kt = {});
+bool launch_cover_pairs(int nV, int ld, const int* node_list, int n_list, const int* counts, const int* cpiv,
+                        const int* bpiv, int piv_stride, int max_c, int max_b, uint16_t* DU, size_t node_stride,
+                        hipStream_t st, KTimer kt = {});
 void launch_project_su_live(int nV, int ld, int ne, const int* vu_ext, const int* node_list, const int* counts,
                             int n_list, const int* live8_cnt, const int* live8_list, int ng8, int max_live8,
                             const uint16_t* DU, size_t node_stride, uint16_t* SU, int* changed, hipStream_t st);

@@ -480,7 +480,9 @@ void launch_compact_flags_live(int nV, const int* node_list, int n_list, const i
 // B columns and writes C columns.
 //
 // Both hops are one rectangular min-plus product, tiled like the closure
-// update: a workgroup owns 16*RPT C rows x 16 column octets (thread = RPT rows
+// update (with B split over grid z when the tiles alone are too few to fill
+// the chip: each part then folds its minima into D with compare-and-swap
+// minima, which lose no smaller value whatever the order): a workgroup owns 16*RPT C rows x 16 column octets (thread = RPT rows
 // x 8 columns, two u16 lanes per v_pk op) and streams the inner set B through
 // LDS in chunks of 32: dup'd A[b][row] = D[row][b] and X[b][octet] (hop 1:
 // the B row's live octet; hop 2: D[c2][b] for 8 consecutive cover entries
@@ -490,6 +492,25 @@ void launch_compact_flags_live(int nV, const int* node_list, int n_list, const i
 // The host picks RPT so that the few large nodes of the upper levels still
 // fill the chip.  Grid: x = row tile * column tiles + column tile, y = slot.
 constexpr int kCoverOct = 16, kCoverK = 32;
+
+// D[w] = min(D[w], v) on a packed pair / one u16 of a 32-bit word, atomically
+// against other workgroups doing the same: values only decrease, so the
+// compare-and-swap loop ends.
+__device__ __forceinline__ void atomic_min_pair(uint32_t* w, uint32_t v) {
+    uint32_t old = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (;;) {
+        const uint32_t nv = min2(old, v);
+        if (nv == old) return;
+        const uint32_t prev = atomicCAS(w, old, nv);
+        if (prev == old) return;
+        old = prev;
+    }
+}
+__device__ __forceinline__ void atomic_min_u16(uint16_t* a, uint32_t v) {
+    const uintptr_t u = reinterpret_cast<uintptr_t>(a);
+    uint32_t* w = reinterpret_cast<uint32_t*>(u & ~uintptr_t(3));
+    atomic_min_pair(w, (u & 2) ? (v << 16) | 0xffffu : v | 0xffff0000u);
+}
 
 template <int RPT> struct LdsRow;
 template <> struct LdsRow<1> {
@@ -509,7 +530,7 @@ template <bool HOP2, int RPT>
 __global__ __launch_bounds__(256) void k_cover_product(int nV, int ld, const int* __restrict__ node_list, int n_list,
                                                        const int* __restrict__ counts, const int* __restrict__ cpiv,
                                                        const int* __restrict__ bpiv, int piv_stride, int col_tiles,
-                                                       const int* __restrict__ live8_cnt,
+                                                       int ksplit, const int* __restrict__ live8_cnt,
                                                        const int* __restrict__ live8_list, int ng8,
                                                        uint16_t* __restrict__ DU, size_t node_stride) {
     constexpr int R = 16 * RPT;               // rows per tile
@@ -522,7 +543,8 @@ __global__ __launch_bounds__(256) void k_cover_product(int nV, int ld, const int
     const int slot = blockIdx.y;
     const int nc = counts[slot], nb = counts[n_list + slot];
     const int I = blockIdx.x / col_tiles, J = blockIdx.x % col_tiles;
-    if (nb == 0 || I * R >= nc) return;
+    const int kb = int((long long)nb * blockIdx.z / ksplit), ke = int((long long)nb * (blockIdx.z + 1) / ksplit);
+    if (kb >= ke || I * R >= nc) return;  // also nb == 0
     const int node = node_list[slot];
     const int n_cols = HOP2 ? nc : 8 * live8_cnt[node];  // columns this node has
     if (J * kCoverOct * 8 >= n_cols) return;
@@ -564,12 +586,12 @@ __global__ __launch_bounds__(256) void k_cover_product(int nV, int ld, const int
 #pragma unroll
         for (int i = 0; i < SA; ++i) {
             const int k = k0 + tid / R + (256 / R) * i;
-            ia[i] = k < nb ? bl[k] : -1;
+            ia[i] = k < ke ? bl[k] : -1;
         }
 #pragma unroll
         for (int i = 0; i < SX; ++i) {
             const int k = k0 + (tid >> 4) + 16 * i;
-            ix[i] = k < nb ? bl[k] : -1;
+            ix[i] = k < ke ? bl[k] : -1;
         }
     };
     auto gather = [&](const int* ia, const int* ix, uint32_t* va, uint4* vx) {
@@ -598,10 +620,10 @@ __global__ __launch_bounds__(256) void k_cover_product(int nV, int ld, const int
     int ia[SA], ix[SX];
     uint32_t va[SA];
     uint4 vx[SX];
-    load_idx(0, ia, ix);
+    load_idx(kb, ia, ix);
     gather(ia, ix, va, vx);
-    load_idx(kCoverK, ia, ix);
-    for (int k0 = 0; k0 < nb; k0 += kCoverK) {
+    load_idx(kb + kCoverK, ia, ix);
+    for (int k0 = kb; k0 < ke; k0 += kCoverK) {
         __syncthreads();  // the previous chunk's arithmetic is done
 #pragma unroll
         for (int i = 0; i < SA; ++i)  // row a_r sits in thread group a_r / RPT, lane a_r % RPT
@@ -609,12 +631,12 @@ __global__ __launch_bounds__(256) void k_cover_product(int nV, int ld, const int
 #pragma unroll
         for (int i = 0; i < SX; ++i) X4[(tid >> 4) + 16 * i][x_o] = vx[i];
         __syncthreads();
-        if (k0 + kCoverK < nb) {  // next chunk in flight during this one's arithmetic
+        if (k0 + kCoverK < ke) {  // next chunk in flight during this one's arithmetic
             gather(ia, ix, va, vx);
             load_idx(k0 + 2 * kCoverK, ia, ix);
         }
         if (!busy) continue;
-        const int m = min(kCoverK, nb - k0);
+        const int m = min(kCoverK, ke - k0);
 #pragma unroll 4
         for (int k = 0; k < m; ++k) {
             uint32_t av[RPT];
@@ -628,6 +650,25 @@ __global__ __launch_bounds__(256) void k_cover_product(int nV, int ld, const int
                 acc[r][3] = relax2(acc[r][3], av[r], x.w);
             }
         }
+    }
+    if (ksplit > 1) {  // other parts of B write the same entries
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            const int row = rowv[ty * RPT + r];
+            if (row < 0) continue;
+            uint16_t* Dr = D + size_t(row) * ld;
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                const int c0 = colv[tx * 8 + 2 * h], c1 = colv[tx * 8 + 2 * h + 1];
+                if (c0 >= 0 && c1 == c0 + 1 && (c0 & 1) == 0) {
+                    atomic_min_pair(reinterpret_cast<uint32_t*>(Dr + c0), acc[r][h]);
+                    continue;
+                }
+                if (c0 >= 0) atomic_min_u16(Dr + c0, acc[r][h] & 0xffffu);
+                if (c1 >= 0) atomic_min_u16(Dr + c1, acc[r][h] >> 16);
+            }
+        }
+        return;
     }
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
@@ -656,15 +697,27 @@ static int cover_rpt(int max_c, int col_tiles, int n_list) {
     return 1;
 }
 
+// Parts of B (grid z) for a grid of `blocks` tiles: enough for 512
+// workgroups, each part at least two chunks of B.
+static int cover_ksplit(long long blocks, int max_b) {
+    const long long want = (512 + blocks - 1) / blocks;
+    const long long most = std::max(1, max_b / (2 * kCoverK));
+    return int(std::max(1ll, std::min({want, most, 64ll})));
+}
+
 #define HTG_COVER_LAUNCH(HOP2, RPT, CT, L8C, L8L, NG8)                                                             \
-    HTG_LAUNCH_T((k_cover_product<HOP2, RPT>),                                                           \
-                          dim3(unsigned(((max_c + 16 * (RPT)-1) / (16 * (RPT))) * (CT)), unsigned(n_list), 1u),   \
-                          dim3(256), 0, st, kt, nV, ld, node_list, n_list, counts, cpiv, bpiv,         \
-                          piv_stride, CT, L8C, L8L, NG8, DU, node_stride)
+    do {                                                                                                           \
+        const long long tiles = (long long)((max_c + 16 * (RPT)-1) / (16 * (RPT))) * (CT);                          \
+        const int ks = cover_ksplit(tiles * n_list, max_b);                                                        \
+        HTG_LAUNCH_T((k_cover_product<HOP2, RPT>), dim3(unsigned(tiles), unsigned(n_list), unsigned(ks)),          \
+                     dim3(256), 0, st, kt, nV, ld, node_list, n_list, counts, cpiv, bpiv, piv_stride, CT, ks, L8C, \
+                     L8L, NG8, DU, node_stride);                                                                   \
+    } while (0)
 
 bool launch_cover_relax(int nV, int ld, const int* node_list, int n_list, const int* counts, const int* cpiv,
                         const int* bpiv, int piv_stride, const int* live8_cnt, const int* live8_list, int ng8,
-                        int max_live8, int max_c, uint16_t* DU, size_t node_stride, hipStream_t st, KTimer kt) {
+                        int max_live8, int max_c, int max_b, uint16_t* DU, size_t node_stride, hipStream_t st,
+                        KTimer kt) {
     if (n_list == 0 || max_live8 == 0 || max_c == 0) return false;
     const int ct = (max_live8 + kCoverOct - 1) / kCoverOct;
     const int rpt = cover_rpt(max_c, ct, n_list);
@@ -675,8 +728,8 @@ bool launch_cover_relax(int nV, int ld, const int* node_list, int n_list, const 
 }
 
 bool launch_cover_pairs(int nV, int ld, const int* node_list, int n_list, const int* counts, const int* cpiv,
-                        const int* bpiv, int piv_stride, int max_c, uint16_t* DU, size_t node_stride, hipStream_t st,
-                        KTimer kt) {
+                        const int* bpiv, int piv_stride, int max_c, int max_b, uint16_t* DU, size_t node_stride,
+                        hipStream_t st, KTimer kt) {
     if (n_list == 0 || max_c == 0) return false;
     const int ct = (max_c + kCoverOct * 8 - 1) / (kCoverOct * 8);
     const int rpt = cover_rpt(max_c, ct, n_list);

@@ -1340,12 +1340,13 @@ int Map::open_cells(const int32_t* rcs, int n) {
             int pi = prof_open(HTG_K_COVER_RELAX, by_r, upd_r, kt);
             bool la = launch_cover_relax(L.d.nV, L.d.ld, dnodes, na, cnt, static_cast<const int*>(ws_piv.p),
                                          static_cast<const int*>(ws_bpiv.p), L.d.nV, L.d_live8_cnt, L.d_live8_list,
-                                         L.ng8, max8, mx, static_cast<uint16_t*>(L.DU.p), L.node_stride, stream, kt);
+                                         L.ng8, max8, mx, mxb, static_cast<uint16_t*>(L.DU.p), L.node_stride, stream,
+                                         kt);
             prof_ref().close(pi, la);
             pi = prof_open(HTG_K_COVER_PAIRS, by_p, upd_p, kt);
             la = launch_cover_pairs(L.d.nV, L.d.ld, dnodes, na, cnt, static_cast<const int*>(ws_piv.p),
-                                    static_cast<const int*>(ws_bpiv.p), L.d.nV, mx, static_cast<uint16_t*>(L.DU.p),
-                                    L.node_stride, stream, kt);
+                                    static_cast<const int*>(ws_bpiv.p), L.d.nV, mx, mxb,
+                                    static_cast<uint16_t*>(L.DU.p), L.node_stride, stream, kt);
             prof_ref().close(pi, la);
         }
         hc.resize(size_t(na));
