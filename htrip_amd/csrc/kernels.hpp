@@ -34,9 +34,24 @@ struct KTimer {
 // dispatched at once and query kernels on other streams find free slots.
 constexpr int kBackgroundWG = 512;
 
-// max_wg > 0 caps the grid (work items are strided over it).
+// max_wg > 0 caps the grid (work items are strided over it); chg_idx
+// (nullable) maps list position i to its changed[] index.
 bool launch_leaf_apsp(const uint8_t* pass, int S, int b, int nl, const int* leaf_list, int n_list, uint16_t* leafD,
-                      uint16_t* leafS, int* changed, hipStream_t st, KTimer kt = {}, int max_wg = 0);
+                      uint16_t* leafS, int* changed, hipStream_t st, KTimer kt = {}, int max_wg = 0,
+                      const int* chg_idx = nullptr);
+// Incremental reveal of a leaf with few new cells (b >= 8, b*b % 8 == 0):
+// BFS rows of the new cells (items: leaf, first pivot, count <=
+// leaf_sources_per_wg(b); pcell: pivot cell ids within the leaf) into the
+// leaf's D and into P[pivot][*], then the rank-k relaxation of the leaf's D
+// and S through them (leaves: leaf, first pivot, count, changed index).
+// Leaves with more new cells than kLeafRelaxMax are recomputed whole.
+constexpr int kLeafRelaxMax = 64;
+int leaf_sources_per_wg(int b);
+bool launch_leaf_pivot_rows(const uint8_t* pass, int S, int b, int nl, const int4* items, int n_items,
+                            const int* pcell, uint16_t* leafD, uint16_t* P, uint16_t* leafS, int* changed,
+                            hipStream_t st, KTimer kt = {}, int max_wg = 0);
+bool launch_leaf_relax(int b, const int4* leaves, int n_leaves, const uint16_t* P, uint16_t* leafD, uint16_t* leafS,
+                       int* changed, hipStream_t st, KTimer kt = {});
 
 // D_U of each listed node := block-diagonal child summaries, INF elsewhere.
 // Child summaries come from leafS (lvl 1, stride (4b-4)^2) or the previous

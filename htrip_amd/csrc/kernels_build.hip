@@ -50,31 +50,20 @@ __device__ __forceinline__ uint64_t lane_next(uint64_t v) {
 }
 
 // ---------------------------------------------------------------------------
-// Leaf all-pairs BFS (src/leaf_tile.cpp:56-153 + projection :198-207).
-// One wavefront runs 64/LANES BFS sources at once; lane r of a source group
-// owns tile row r as a bitmask and a BFS step is a shift/or of neighbour rows
-// (bit-parallel frontier expansion).  Distances land in LDS and are streamed
-// out as whole D rows (coalesced); port sources also write their S row.
+// Leaf BFS core (src/leaf_tile.cpp:56-153): one wavefront runs 64/LANES BFS
+// sources at once; lane r of a source group owns tile row r as a bitmask and
+// a BFS step is a shift/or of neighbour rows (bit-parallel frontier
+// expansion).  Distances from the workgroup's SWG sources land in LDS
+// (dist[slot][cell], INF where unreached); the caller streams them out.
 template <int LANES, typename RowT>
-__global__ __launch_bounds__(256) void k_leaf_apsp(const uint8_t* __restrict__ pass, int S, int b, int nl,
-                                                   const int* __restrict__ leaf_list, int chunks, int n_items,
-                                                   uint16_t* __restrict__ leafD, uint16_t* __restrict__ leafS,
-                                                   int* __restrict__ changed) {
+__device__ __forceinline__ void leaf_bfs(const uint8_t* __restrict__ pass, int S, int b, int R0, int C0, bool valid,
+                                         int src, uint16_t* dist) {
     constexpr int SPW = 64 / LANES;
     constexpr int SWG = 4 * SPW;
-    extern __shared__ uint16_t dist[];  // [SWG][b*b]
-    // work item = (leaf, chunk of SWG sources); a capped grid strides over them
-    for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
-    if (item != int(blockIdx.x)) __syncthreads();  // the previous item's readers of dist are done
-    const int li = item / chunks, chunk = item % chunks;
-    const int leaf = leaf_list ? leaf_list[li] : li;
     const int n = b * b;
-    const int R0 = (leaf / nl) * b, C0 = (leaf % nl) * b;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int grp = lane / LANES, r = lane % LANES;
     const int slot = wave * SPW + grp;
-    const int src = chunk * SWG + slot;
-
     uint32_t* d32 = reinterpret_cast<uint32_t*>(dist);
     for (int i = tid; i < SWG * n / 2; i += 256) d32[i] = kInf2;
     RowT prow = 0;
@@ -83,8 +72,6 @@ __global__ __launch_bounds__(256) void k_leaf_apsp(const uint8_t* __restrict__ p
         for (int c = 0; c < b; ++c) prow |= RowT(p[c] != 0) << c;
     }
     __syncthreads();
-
-    const bool valid = src < n;
     const int sr = valid ? src / b : 0, sc = valid ? src % b : 0;
     const RowT srow = __shfl(prow, sr, LANES);
     RowT front = (valid && r == sr && ((srow >> sc) & 1)) ? (RowT(1) << sc) : RowT(0);
@@ -108,6 +95,31 @@ __global__ __launch_bounds__(256) void k_leaf_apsp(const uint8_t* __restrict__ p
         }
     }
     __syncthreads();
+}
+
+// Leaf all-pairs BFS (src/leaf_tile.cpp:56-153 + projection :198-207): work
+// item (leaf, chunk of SWG consecutive sources); whole D rows are streamed
+// out (coalesced) and port sources also write their S row.  changed[li]
+// (li: position in leaf_list, or chg_idx[li]) is set when S decreased.
+template <int LANES, typename RowT>
+__global__ __launch_bounds__(256) void k_leaf_apsp(const uint8_t* __restrict__ pass, int S, int b, int nl,
+                                                   const int* __restrict__ leaf_list, int chunks, int n_items,
+                                                   uint16_t* __restrict__ leafD, uint16_t* __restrict__ leafS,
+                                                   int* __restrict__ changed, const int* __restrict__ chg_idx) {
+    constexpr int SPW = 64 / LANES;
+    constexpr int SWG = 4 * SPW;
+    extern __shared__ uint16_t dist[];  // [SWG][b*b]
+    // work item = (leaf, chunk of SWG sources); a capped grid strides over them
+    for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
+    if (item != int(blockIdx.x)) __syncthreads();  // the previous item's readers of dist are done
+    const int li = item / chunks, chunk = item % chunks;
+    const int leaf = leaf_list ? leaf_list[li] : li;
+    const int n = b * b;
+    const int R0 = (leaf / nl) * b, C0 = (leaf % nl) * b;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int slot = wave * SPW + lane / LANES;
+    const int src = chunk * SWG + slot;
+    leaf_bfs<LANES, RowT>(pass, S, b, R0, C0, src < n, src, dist);
 
     const int first = chunk * SWG;
     const int nvalid = min(SWG, n - first);
@@ -121,6 +133,7 @@ __global__ __launch_bounds__(256) void k_leaf_apsp(const uint8_t* __restrict__ p
     }
     const int pl = 4 * b - 4;
     uint16_t* gS = leafS + size_t(leaf) * pl * pl;
+    int* chg = changed ? changed + (chg_idx ? chg_idx[li] : li) : nullptr;
     for (int s = 0; s < nvalid; ++s) {
         const int cell = first + s;
         const int pi = port_index(cell / b, cell % b, b);
@@ -130,15 +143,166 @@ __global__ __launch_bounds__(256) void k_leaf_apsp(const uint8_t* __restrict__ p
             port_rc(pj, b, pr, pc);
             const uint16_t v = dist[s * n + pr * b + pc];
             uint16_t* dst = gS + size_t(pi) * pl + pj;
-            if (changed && v < *dst) changed[li] = 1;
+            if (chg && v < *dst) *chg = 1;
             *dst = v;
         }
     }
     }
 }
 
+// Rows of a leaf's newly opened cells (incremental reveal of a leaf): work
+// item (leaf, first pivot, count <= SWG, changed index); BFS from each pivot
+// cell p over the leaf's current passability gives D'[p][*], written to the
+// leaf's D row p and to P[pivot] for the rank-k relaxation; a pivot that is
+// a port also writes its S row (the relaxation leaves row p as it is).
+template <int LANES, typename RowT>
+__global__ __launch_bounds__(256) void k_leaf_pivot_rows(const uint8_t* __restrict__ pass, int S, int b, int nl,
+                                                         const int4* __restrict__ items, int n_items,
+                                                         const int* __restrict__ pcell, uint16_t* __restrict__ leafD,
+                                                         uint16_t* __restrict__ P, uint16_t* __restrict__ leafS,
+                                                         int* __restrict__ changed) {
+    constexpr int SPW = 64 / LANES;
+    extern __shared__ uint16_t dist[];  // [SWG][b*b]
+    for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
+        if (item != int(blockIdx.x)) __syncthreads();
+        const int4 it = items[item];
+        const int leaf = it.x, off = it.y, cnt = it.z;
+        const int n = b * b;
+        const int R0 = (leaf / nl) * b, C0 = (leaf % nl) * b;
+        const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+        const int slot = wave * SPW + lane / LANES;
+        const bool valid = slot < cnt;
+        const int src = valid ? pcell[off + slot] : 0;
+        leaf_bfs<LANES, RowT>(pass, S, b, R0, C0, valid, src, dist);
+        const int n8 = n / 8;  // n % 8 == 0
+        const uint4* s4 = reinterpret_cast<const uint4*>(dist);
+        for (int i = tid; i < cnt * n8; i += 256) {
+            const int sl = i / n8, q = i % n8;
+            const uint4 v = s4[i];
+            reinterpret_cast<uint4*>(leafD + size_t(leaf) * n * n + size_t(pcell[off + sl]) * n)[q] = v;
+            reinterpret_cast<uint4*>(P + size_t(off + sl) * n)[q] = v;
+        }
+        const int pl = 4 * b - 4;
+        uint16_t* gS = leafS + size_t(leaf) * pl * pl;
+        for (int sl = 0; sl < cnt; ++sl) {
+            const int cell = pcell[off + sl];
+            const int pi = port_index(cell / b, cell % b, b);
+            if (pi < 0) continue;
+            for (int pj = tid; pj < pl; pj += 256) {
+                int pr, pc;
+                port_rc(pj, b, pr, pc);
+                const uint16_t v = dist[sl * n + pr * b + pc];
+                uint16_t* dst = gS + size_t(pi) * pl + pj;
+                if (changed && v < *dst) changed[it.w] = 1;
+                *dst = v;
+            }
+        }
+    }
+}
+
+// Rank-k relaxation of a leaf through its new cells:
+//   D'[i][j] = min(D[i][j], min_p P[p][i] + P[p][j]),  P[p] = D'[p][*]
+// exact, since a new shortest path passes through at least one new cell and
+// D'[i][p] = D'[p][i].  Work item: leaf (grid y: leaf, first pivot, count,
+// changed index) x tile of 64 rows x 128 columns (grid x); thread = 4 rows x
+// 8 columns in packed u16 pairs; pivots stream through LDS in chunks of 32
+// (A = dup16(P[p][row]), X = P[p][column octet]).  Port x port entries also
+// go to S, setting changed[] where S decreased.
+constexpr int kLeafRelaxK = 32;
+__global__ __launch_bounds__(256) void k_leaf_relax(int b, const int4* __restrict__ leaves, int col_tiles,
+                                                    const uint16_t* __restrict__ P, uint16_t* __restrict__ leafD,
+                                                    uint16_t* __restrict__ leafS, int* __restrict__ changed) {
+    __shared__ uint4 A4[kLeafRelaxK][16];  // [pivot][row group]: 4 rows' dup16 values
+    __shared__ uint4 X4[kLeafRelaxK][16];  // [pivot][column octet]
+    const int4 lf = leaves[blockIdx.y];
+    const int leaf = lf.x, off = lf.y, k = lf.z;
+    const int n = b * b;
+    const int I = blockIdx.x / col_tiles, J = blockIdx.x % col_tiles;
+    const int i0 = I * 64, j0 = J * 128;
+    if (i0 >= n || j0 >= n) return;
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    uint16_t* D = leafD + size_t(leaf) * n * n;
+    const int col = j0 + 8 * tx;
+    const bool cok = col < n;  // n % 8 == 0: whole octets
+    uint32_t acc[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int row = i0 + ty * 4 + r;
+        uint4 v = make_uint4(kInf2, kInf2, kInf2, kInf2);
+        if (cok && row < n) v = *reinterpret_cast<const uint4*>(D + size_t(row) * n + col);
+        acc[r][0] = v.x, acc[r][1] = v.y, acc[r][2] = v.z, acc[r][3] = v.w;
+    }
+    uint32_t old[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int h = 0; h < 4; ++h) old[r][h] = acc[r][h];
+    const uint16_t* Pl = P + size_t(off) * n;
+    for (int k0 = 0; k0 < k; k0 += kLeafRelaxK) {
+        const int m = min(kLeafRelaxK, k - k0);
+        __syncthreads();  // the previous chunk's readers are done
+        // A: 32 pivots x 64 rows (u16 -> dup16), 8 per thread; X: 32 x 16 octets, 2 per thread
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const int idx = tid + 256 * e;  // (pivot, row group of 4)
+            const int kk = idx >> 4, rg = idx & 15;
+            uint4 a = make_uint4(kInf2, kInf2, kInf2, kInf2);
+            const int row = i0 + rg * 4;
+            if (kk < m && row < n) {
+                const uint2 pv = *reinterpret_cast<const uint2*>(Pl + size_t(k0 + kk) * n + row);
+                a = make_uint4(dup16(pv.x), dup16(pv.x >> 16), dup16(pv.y), dup16(pv.y >> 16));
+            }
+            A4[kk][rg] = a;
+            const int oc = idx & 15;
+            uint4 x = make_uint4(kInf2, kInf2, kInf2, kInf2);
+            if (kk < m && j0 + 8 * oc < n) x = *reinterpret_cast<const uint4*>(Pl + size_t(k0 + kk) * n + j0 + 8 * oc);
+            X4[kk][oc] = x;
+        }
+        __syncthreads();
+        for (int kk = 0; kk < m; ++kk) {
+            const uint4 a = A4[kk][ty];
+            const uint4 x = X4[kk][tx];
+            const uint32_t av[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                acc[r][0] = relax2(acc[r][0], av[r], x.x);
+                acc[r][1] = relax2(acc[r][1], av[r], x.y);
+                acc[r][2] = relax2(acc[r][2], av[r], x.z);
+                acc[r][3] = relax2(acc[r][3], av[r], x.w);
+            }
+        }
+    }
+    if (!cok) return;
+    const int pl = 4 * b - 4;
+    uint16_t* Sl = leafS + size_t(leaf) * pl * pl;
+    bool dec = false;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int row = i0 + ty * 4 + r;
+        if (row >= n) continue;
+        bool same = true;
+#pragma unroll
+        for (int h = 0; h < 4; ++h) same &= acc[r][h] == old[r][h];
+        if (same) continue;
+        *reinterpret_cast<uint4*>(D + size_t(row) * n + col) = make_uint4(acc[r][0], acc[r][1], acc[r][2], acc[r][3]);
+        const int pi = port_index(row / b, row % b, b);
+        if (pi < 0) continue;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const uint32_t v = (acc[r][q >> 1] >> (16 * (q & 1))) & 0xffffu;
+            const uint32_t o = (old[r][q >> 1] >> (16 * (q & 1))) & 0xffffu;
+            if (v == o) continue;
+            const int pj = port_index((col + q) / b, (col + q) % b, b);
+            if (pj < 0) continue;
+            Sl[size_t(pi) * pl + pj] = uint16_t(v);
+            dec = true;
+        }
+    }
+    if (dec && changed) changed[lf.w] = 1;
+}
+
 bool launch_leaf_apsp(const uint8_t* pass, int S, int b, int nl, const int* leaf_list, int n_list, uint16_t* leafD,
-                      uint16_t* leafS, int* changed, hipStream_t st, KTimer kt, int max_wg) {
+                      uint16_t* leafS, int* changed, hipStream_t st, KTimer kt, int max_wg, const int* chg_idx) {
     if (n_list == 0) return false;
     const int n = b * b;
     auto go = [&](auto kern, int lanes) {
@@ -148,7 +312,7 @@ bool launch_leaf_apsp(const uint8_t* pass, int S, int b, int nl, const int* leaf
         const int grid = max_wg > 0 ? std::min(items, max_wg) : items;
         const uint32_t lds = uint32_t(size_t(swg) * n * sizeof(uint16_t));
         HTG_LAUNCH_T(kern, dim3(unsigned(grid)), dim3(256), lds, st, kt, pass, S, b, nl, leaf_list, chunks, items,
-                     leafD, leafS, changed);
+                     leafD, leafS, changed, chg_idx);
     };
     if (b <= 2) go(k_leaf_apsp<2, uint32_t>, 2);
     else if (b <= 4) go(k_leaf_apsp<4, uint32_t>, 4);
@@ -156,6 +320,40 @@ bool launch_leaf_apsp(const uint8_t* pass, int S, int b, int nl, const int* leaf
     else if (b <= 16) go(k_leaf_apsp<16, uint32_t>, 16);
     else if (b <= 32) go(k_leaf_apsp<32, uint32_t>, 32);
     else go(k_leaf_apsp<64, uint64_t>, 64);
+    return true;
+}
+
+int leaf_sources_per_wg(int b) {
+    int lanes = 2;
+    while (lanes < b) lanes *= 2;
+    return 4 * (64 / lanes);
+}
+
+bool launch_leaf_pivot_rows(const uint8_t* pass, int S, int b, int nl, const int4* items, int n_items,
+                            const int* pcell, uint16_t* leafD, uint16_t* P, uint16_t* leafS, int* changed,
+                            hipStream_t st, KTimer kt, int max_wg) {
+    if (n_items == 0) return false;
+    const int n = b * b;
+    const int grid = max_wg > 0 ? std::min(n_items, max_wg) : n_items;
+    auto go = [&](auto kern) {
+        const uint32_t lds = uint32_t(size_t(leaf_sources_per_wg(b)) * n * sizeof(uint16_t));
+        HTG_LAUNCH_T(kern, dim3(unsigned(grid)), dim3(256), lds, st, kt, pass, S, b, nl, items, n_items, pcell, leafD,
+                     P, leafS, changed);
+    };
+    if (b <= 8) go(k_leaf_pivot_rows<8, uint32_t>);
+    else if (b <= 16) go(k_leaf_pivot_rows<16, uint32_t>);
+    else if (b <= 32) go(k_leaf_pivot_rows<32, uint32_t>);
+    else go(k_leaf_pivot_rows<64, uint64_t>);
+    return true;
+}
+
+bool launch_leaf_relax(int b, const int4* leaves, int n_leaves, const uint16_t* P, uint16_t* leafD, uint16_t* leafS,
+                       int* changed, hipStream_t st, KTimer kt) {
+    if (n_leaves == 0) return false;
+    const int n = b * b;
+    const int rt = (n + 63) / 64, ct = (n + 127) / 128;
+    HTG_LAUNCH_T(k_leaf_relax, dim3(unsigned(rt * ct), unsigned(n_leaves)), dim3(256), 0, st, kt, b, leaves, ct, P,
+                 leafD, leafS, changed);
     return true;
 }
 

@@ -1205,10 +1205,42 @@ int Map::open_cells(const int32_t* rcs, int n) {
             }
         }
     }
+    // 5. dirty leaves: a leaf with at most kLeafRelaxMax new cells is revealed
+    // incrementally (BFS rows of its new cells, then the rank-k relaxation of
+    // its table through them), one with more is recomputed whole; a leaf
+    // whose cells were all passable already keeps its tables
+    const int nsq = b * b;
+    const bool relax_ok = b >= 8 && nsq % 8 == 0;
+    std::vector<int> full_leaves, full_idx, pcell;
+    std::vector<int4> rleaves, pitems;
+    {
+        std::vector<std::vector<int>> per(static_cast<size_t>(nd));
+        for (const int2& p : newly) {
+            const int leaf = (p.x / b) * nl + p.y / b;
+            const size_t di = size_t(std::lower_bound(dirty.begin(), dirty.end(), leaf) - dirty.begin());
+            per[di].push_back((p.x % b) * b + p.y % b);
+        }
+        const int swg = leaf_sources_per_wg(b);
+        for (int i = 0; i < nd; ++i) {
+            const int k = int(per[size_t(i)].size());
+            if (k == 0) continue;
+            if (!relax_ok || k > kLeafRelaxMax) {
+                full_leaves.push_back(dirty[size_t(i)]);
+                full_idx.push_back(i);
+                continue;
+            }
+            const int off = int(pcell.size());
+            rleaves.push_back(make_int4(dirty[size_t(i)], off, k, i));
+            for (int c0 = 0; c0 < k; c0 += swg) pitems.push_back(make_int4(dirty[size_t(i)], off + c0, std::min(swg, k - c0), i));
+            pcell.insert(pcell.end(), per[size_t(i)].begin(), per[size_t(i)].end());
+        }
+    }
+    if (!pcell.empty() && (rc = ws_leafP.ensure(pcell.size() * size_t(nsq) * 2))) return rc;
     const auto u2 = clk::now();
     // one packed upload; changed flags: leaves, then each level's candidates
     Packer pk;
-    const size_t o_cells = pk.add(newly), o_leaves = pk.add(dirty);
+    const size_t o_cells = pk.add(newly), o_full = pk.add(full_leaves), o_fidx = pk.add(full_idx),
+                 o_rl = pk.add(rleaves), o_pit = pk.add(pitems), o_pcell = pk.add(pcell);
     std::vector<size_t> o_cand(size_t(H) + 1), o_cslot(size_t(H) + 1), o_edges(size_t(H) + 1), o_chg(size_t(H) + 1),
         o_newp(size_t(H) + 1);
     size_t nchg = size_t(nd);
@@ -1233,12 +1265,32 @@ int Map::open_cells(const int32_t* rcs, int n) {
         launch_set_pass(static_cast<uint8_t*>(d_pass.p), S, reinterpret_cast<const int2*>(up + o_cells),
                         int(newly.size()), stream);
     const double leaf_bytes = (double(b) * b * b * b + double(4 * b - 4) * (4 * b - 4)) * 2 + double(b) * b;
-    KTimer lkt;
-    const int lpi = prof_open(HTG_K_LEAF_APSP, leaf_bytes * nd, 0, lkt);
-    const bool lla = launch_leaf_apsp(static_cast<const uint8_t*>(d_pass.p), S, b, nl,
-                                      reinterpret_cast<const int*>(up + o_leaves), nd, static_cast<uint16_t*>(leafD.p),
-                                      static_cast<uint16_t*>(leafS.p), chg, stream, lkt, kBackgroundWG);
-    prof_ref().close(lpi, lla);
+    {
+        KTimer lkt;
+        int lpi = prof_open(HTG_K_LEAF_APSP, leaf_bytes * double(full_leaves.size()), 0, lkt);
+        bool lla = launch_leaf_apsp(static_cast<const uint8_t*>(d_pass.p), S, b, nl,
+                                    reinterpret_cast<const int*>(up + o_full), int(full_leaves.size()),
+                                    static_cast<uint16_t*>(leafD.p), static_cast<uint16_t*>(leafS.p), chg, stream, lkt,
+                                    kBackgroundWG, reinterpret_cast<const int*>(up + o_fidx));
+        prof_ref().close(lpi, lla);
+        // pivot rows: D row + P row per new cell; relaxation: D read and
+        // written once, P rows of the tile's rows and columns per pivot
+        const double np = double(pcell.size()), nsqd = double(nsq);
+        lpi = prof_open(HTG_K_LEAF_APSP, np * nsqd * 4 + double(b) * b * double(pitems.size()), 0, lkt);
+        lla = launch_leaf_pivot_rows(static_cast<const uint8_t*>(d_pass.p), S, b, nl,
+                                     reinterpret_cast<const int4*>(up + o_pit), int(pitems.size()),
+                                     reinterpret_cast<const int*>(up + o_pcell), static_cast<uint16_t*>(leafD.p),
+                                     static_cast<uint16_t*>(ws_leafP.p), static_cast<uint16_t*>(leafS.p), chg, stream,
+                                     lkt, kBackgroundWG);
+        prof_ref().close(lpi, lla);
+        double upd = 0;
+        for (const int4& r : rleaves) upd += double(r.z) * nsqd * nsqd;
+        lpi = prof_open(HTG_K_LEAF_APSP, double(rleaves.size()) * nsqd * nsqd * 4 + np * nsqd * 2 * 3, upd, lkt);
+        lla = launch_leaf_relax(b, reinterpret_cast<const int4*>(up + o_rl), int(rleaves.size()),
+                                static_cast<const uint16_t*>(ws_leafP.p), static_cast<uint16_t*>(leafD.p),
+                                static_cast<uint16_t*>(leafS.p), chg, stream, lkt);
+        prof_ref().close(lpi, lla);
+    }
     stats.n_touched_leaves = nd;
     stats.n_pivots = piv;
     if (H == 0) {

@@ -335,6 +335,7 @@ struct Map {
     DevBuf ws_nodes, ws_flags, ws_piv, ws_bpiv, ws_counts, ws_R, ws_changed, ws_mask, ws_edges, ws_cells, ws_leaves,
         ws_leaf_changed;
     DevBuf ws_upd, ws_chg;  // update: packed per-level lists; changed flags (leaves, then every level)
+    DevBuf ws_leafP;        // update: BFS rows of the newly opened cells of incrementally revealed leaves
     HostBuf ws_upd_h, ws_chg_h, ws_counts_h;  // pinned staging of the lists, changed flags, pivot counts
 
     int build();
