@@ -1148,8 +1148,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=200)
     ap.add_argument("--consistency", default="snapshot", choices=["snapshot", "strict"])
     ap.add_argument("--rate-fracs", type=float, nargs="+",
-                    default=[2.2, 2.0, 1.82, 1.66, 1.52, 1.4, 1.29, 1.19, 1.1, 1.02, 0.94, 0.86, 0.78, 0.7, 0.6,
-                             0.5, 0.4, 0.3, 0.22])
+                    default=[2.2, 2.0, 1.82, 1.66, 1.52, 1.4, 1.29, 1.19, 1.1, 1.02, 0.94, 0.86, 0.78, 0.72, 0.66,
+                             0.61, 0.56, 0.52, 0.48, 0.44, 0.41, 0.38, 0.35, 0.32, 0.29, 0.26, 0.23])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="served: headline trials only (no per-state table, no "

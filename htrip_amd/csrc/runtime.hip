@@ -497,6 +497,13 @@ static int upload(DevBuf& dst, const std::vector<T>& src, hipStream_t st) {
     return HTG_OK;
 }
 
+// Reserve of the incremental leaf reveal: BFS rows of the new cells (up to
+// 16 MiB) and the leaf-phase list staging.
+static uint64_t leaf_ws_bytes(int b, int S) {
+    const uint64_t nl = uint64_t(S / b), n = uint64_t(b) * b;
+    return std::min<uint64_t>(16ull << 20, nl * nl * kLeafRelaxMax * n * 2) + kUpdateListReserve / 4;
+}
+
 static uint64_t tables_bytes(int b, int g, int S, int H) {
     const uint64_t nl = uint64_t(S / b), n = uint64_t(b) * b, pl = 4ull * b - 4;
     uint64_t t = uint64_t(S) * S + nl * nl * (n * n + pl * pl) * 2;
@@ -517,7 +524,7 @@ static uint64_t build_ws_bytes(int b, int g, int S, int H, int P) {
         const uint64_t w = nn * (uint64_t(d.nV) * (3 + 4 + 4) + 24 + uint64_t(P) * d.ld * 2) + nn * 4 * 4;
         m = std::max(m, w);
     }
-    return m + kUpdateListReserve;
+    return m + kUpdateListReserve + leaf_ws_bytes(b, S);
 }
 
 static void geometry(int b, int g, int rows, int cols, int& S, int& H, int min_S = 0) {
@@ -536,7 +543,10 @@ int Map::reserve_update_ws() {
     int rc;
     size_t nn_max = 0;
     for (const LevelGeo& L : lv) nn_max = std::max(nn_max, size_t(L.nn));
+    const uint64_t leafP = leaf_ws_bytes(b, S) - kUpdateListReserve / 4;
     if ((rc = ws_upd.ensure(kUpdateListReserve)) || (rc = ws_upd_h.ensure(kUpdateListReserve)) ||
+        (rc = ws_leafl.ensure(kUpdateListReserve / 4)) || (rc = ws_leafl_h.ensure(kUpdateListReserve / 4)) ||
+        (rc = ws_leafP.ensure(size_t(leafP))) ||
         (rc = ws_chg.ensure(nchg * 4 + 16)) || (rc = ws_chg_h.ensure(nchg * 4 + 16)) ||
         (rc = ws_counts_h.ensure(2 * nn_max * 4 + 16)))
         return rc;
@@ -1075,6 +1085,85 @@ int Map::open_cells(const int32_t* rcs, int n) {
         }
     }
     if (!newly.empty() && (rc = refresh_live(&newly))) return rc;
+    const int nd = int(dirty.size());
+    // 5. dirty leaves: a leaf with at most kLeafRelaxMax new cells is revealed
+    // incrementally (BFS rows of its new cells, then the rank-k relaxation of
+    // its table through them), one with more is recomputed whole; a leaf
+    // whose cells were all passable already keeps its tables
+    const int nsq = b * b;
+    const bool relax_ok = b >= 8 && nsq % 8 == 0;
+    std::vector<int> full_leaves, full_idx, pcell;
+    std::vector<int4> rleaves, pitems;
+    {
+        std::vector<std::vector<int>> per(static_cast<size_t>(nd));
+        for (const int2& p : newly) {
+            const int leaf = (p.x / b) * nl + p.y / b;
+            const size_t di = size_t(std::lower_bound(dirty.begin(), dirty.end(), leaf) - dirty.begin());
+            per[di].push_back((p.x % b) * b + p.y % b);
+        }
+        const int swg = leaf_sources_per_wg(b);
+        for (int i = 0; i < nd; ++i) {
+            const int k = int(per[size_t(i)].size());
+            if (k == 0) continue;
+            if (!relax_ok || k > kLeafRelaxMax) {
+                full_leaves.push_back(dirty[size_t(i)]);
+                full_idx.push_back(i);
+                continue;
+            }
+            const int off = int(pcell.size());
+            rleaves.push_back(make_int4(dirty[size_t(i)], off, k, i));
+            for (int c0 = 0; c0 < k; c0 += swg) pitems.push_back(make_int4(dirty[size_t(i)], off + c0, std::min(swg, k - c0), i));
+            pcell.insert(pcell.end(), per[size_t(i)].begin(), per[size_t(i)].end());
+        }
+    }
+    if (!pcell.empty() && (rc = ws_leafP.ensure(pcell.size() * size_t(nsq) * 2))) return rc;
+    // The leaf phase goes first, with lists of its own: its kernels run while
+    // the host plans the levels.  Changed flags: leaves, then each level's
+    // candidates (at most every node of the level).
+    size_t nchg_max = size_t(nd);
+    for (const LevelGeo& L : lv) nchg_max += size_t(L.nn);
+    Packer lpk;
+    const size_t o_cells = lpk.add(newly), o_full = lpk.add(full_leaves), o_fidx = lpk.add(full_idx),
+                 o_rl = lpk.add(rleaves), o_pit = lpk.add(pitems), o_pcell = lpk.add(pcell);
+    if ((rc = ws_leafl_h.ensure(lpk.size)) || (rc = ws_leafl.ensure(lpk.size)) ||
+        (rc = ws_chg.ensure(nchg_max * 4 + 16)) || (rc = ws_chg_h.ensure(nchg_max * 4 + 16)))
+        return rc;
+    lpk.write(static_cast<char*>(ws_leafl_h.p));
+    if ((e = hipMemcpyAsync(ws_leafl.p, ws_leafl_h.p, lpk.size, hipMemcpyHostToDevice, stream)) != hipSuccess ||
+        (e = hipMemsetAsync(ws_chg.p, 0, nchg_max * 4, stream)) != hipSuccess)
+        return hip_fail(e, "upload leaf lists");
+    const char* lup = static_cast<const char*>(ws_leafl.p);
+    int* chg = static_cast<int*>(ws_chg.p);
+    if (!newly.empty())
+        launch_set_pass(static_cast<uint8_t*>(d_pass.p), S, reinterpret_cast<const int2*>(lup + o_cells),
+                        int(newly.size()), stream);
+    const double leaf_bytes = (double(b) * b * b * b + double(4 * b - 4) * (4 * b - 4)) * 2 + double(b) * b;
+    {
+        KTimer lkt;
+        int lpi = prof_open(HTG_K_LEAF_APSP, leaf_bytes * double(full_leaves.size()), 0, lkt);
+        bool lla = launch_leaf_apsp(static_cast<const uint8_t*>(d_pass.p), S, b, nl,
+                                    reinterpret_cast<const int*>(lup + o_full), int(full_leaves.size()),
+                                    static_cast<uint16_t*>(leafD.p), static_cast<uint16_t*>(leafS.p), chg, stream, lkt,
+                                    kBackgroundWG, reinterpret_cast<const int*>(lup + o_fidx));
+        prof_ref().close(lpi, lla);
+        // pivot rows: D row + P row per new cell; relaxation: D read and
+        // written once, P rows of the tile's rows and columns per pivot
+        const double np = double(pcell.size()), nsqd = double(nsq);
+        lpi = prof_open(HTG_K_LEAF_APSP, np * nsqd * 4 + double(b) * b * double(pitems.size()), 0, lkt);
+        lla = launch_leaf_pivot_rows(static_cast<const uint8_t*>(d_pass.p), S, b, nl,
+                                     reinterpret_cast<const int4*>(lup + o_pit), int(pitems.size()),
+                                     reinterpret_cast<const int*>(lup + o_pcell), static_cast<uint16_t*>(leafD.p),
+                                     static_cast<uint16_t*>(ws_leafP.p), static_cast<uint16_t*>(leafS.p), chg, stream,
+                                     lkt, kBackgroundWG);
+        prof_ref().close(lpi, lla);
+        double upd = 0;
+        for (const int4& r : rleaves) upd += double(r.z) * nsqd * nsqd;
+        lpi = prof_open(HTG_K_LEAF_APSP, double(rleaves.size()) * nsqd * nsqd * 4 + np * nsqd * 2 * 3, upd, lkt);
+        lla = launch_leaf_relax(b, reinterpret_cast<const int4*>(lup + o_rl), int(rleaves.size()),
+                                static_cast<const uint16_t*>(ws_leafP.p), static_cast<uint16_t*>(leafD.p),
+                                static_cast<uint16_t*>(leafS.p), chg, stream, lkt);
+        prof_ref().close(lpi, lla);
+    }
     const auto u1 = clk::now();
     // 2. leaf pivot count, as the reference's leaf update would collect it:
     // the endpoints of intra-leaf edges with an endpoint opened by this batch
@@ -1145,7 +1234,6 @@ int Map::open_cells(const int32_t* rcs, int n) {
     // reference's routing (a parent is dirty when a child summary changed,
     // src/minplus_oracle.cpp:650-722) are resolved on the device from the
     // changed flags of the level below, so a level needs one host sync.
-    const int nd = int(dirty.size());
     this->dirty.leaves = dirty;
     this->dirty.cells = newly;
     const int g2 = g * g;
@@ -1205,42 +1293,10 @@ int Map::open_cells(const int32_t* rcs, int n) {
             }
         }
     }
-    // 5. dirty leaves: a leaf with at most kLeafRelaxMax new cells is revealed
-    // incrementally (BFS rows of its new cells, then the rank-k relaxation of
-    // its table through them), one with more is recomputed whole; a leaf
-    // whose cells were all passable already keeps its tables
-    const int nsq = b * b;
-    const bool relax_ok = b >= 8 && nsq % 8 == 0;
-    std::vector<int> full_leaves, full_idx, pcell;
-    std::vector<int4> rleaves, pitems;
-    {
-        std::vector<std::vector<int>> per(static_cast<size_t>(nd));
-        for (const int2& p : newly) {
-            const int leaf = (p.x / b) * nl + p.y / b;
-            const size_t di = size_t(std::lower_bound(dirty.begin(), dirty.end(), leaf) - dirty.begin());
-            per[di].push_back((p.x % b) * b + p.y % b);
-        }
-        const int swg = leaf_sources_per_wg(b);
-        for (int i = 0; i < nd; ++i) {
-            const int k = int(per[size_t(i)].size());
-            if (k == 0) continue;
-            if (!relax_ok || k > kLeafRelaxMax) {
-                full_leaves.push_back(dirty[size_t(i)]);
-                full_idx.push_back(i);
-                continue;
-            }
-            const int off = int(pcell.size());
-            rleaves.push_back(make_int4(dirty[size_t(i)], off, k, i));
-            for (int c0 = 0; c0 < k; c0 += swg) pitems.push_back(make_int4(dirty[size_t(i)], off + c0, std::min(swg, k - c0), i));
-            pcell.insert(pcell.end(), per[size_t(i)].begin(), per[size_t(i)].end());
-        }
-    }
-    if (!pcell.empty() && (rc = ws_leafP.ensure(pcell.size() * size_t(nsq) * 2))) return rc;
     const auto u2 = clk::now();
-    // one packed upload; changed flags: leaves, then each level's candidates
+    // one packed upload of the level lists; changed flags: leaves, then each
+    // level's candidates
     Packer pk;
-    const size_t o_cells = pk.add(newly), o_full = pk.add(full_leaves), o_fidx = pk.add(full_idx),
-                 o_rl = pk.add(rleaves), o_pit = pk.add(pitems), o_pcell = pk.add(pcell);
     std::vector<size_t> o_cand(size_t(H) + 1), o_cslot(size_t(H) + 1), o_edges(size_t(H) + 1), o_chg(size_t(H) + 1),
         o_newp(size_t(H) + 1);
     size_t nchg = size_t(nd);
@@ -1252,45 +1308,11 @@ int Map::open_cells(const int32_t* rcs, int n) {
         o_chg[size_t(lvl)] = nchg;
         nchg += cand[size_t(lvl)].size();
     }
-    if ((rc = ws_upd_h.ensure(pk.size)) || (rc = ws_upd.ensure(pk.size)) || (rc = ws_chg.ensure(nchg * 4 + 16)) ||
-        (rc = ws_chg_h.ensure(nchg * 4 + 16)))
-        return rc;
+    if ((rc = ws_upd_h.ensure(pk.size)) || (rc = ws_upd.ensure(pk.size))) return rc;
     pk.write(static_cast<char*>(ws_upd_h.p));
-    if ((e = hipMemcpyAsync(ws_upd.p, ws_upd_h.p, pk.size, hipMemcpyHostToDevice, stream)) != hipSuccess ||
-        (e = hipMemsetAsync(ws_chg.p, 0, nchg * 4, stream)) != hipSuccess)
+    if ((e = hipMemcpyAsync(ws_upd.p, ws_upd_h.p, pk.size, hipMemcpyHostToDevice, stream)) != hipSuccess)
         return hip_fail(e, "upload update lists");
     const char* up = static_cast<const char*>(ws_upd.p);
-    int* chg = static_cast<int*>(ws_chg.p);
-    if (!newly.empty())
-        launch_set_pass(static_cast<uint8_t*>(d_pass.p), S, reinterpret_cast<const int2*>(up + o_cells),
-                        int(newly.size()), stream);
-    const double leaf_bytes = (double(b) * b * b * b + double(4 * b - 4) * (4 * b - 4)) * 2 + double(b) * b;
-    {
-        KTimer lkt;
-        int lpi = prof_open(HTG_K_LEAF_APSP, leaf_bytes * double(full_leaves.size()), 0, lkt);
-        bool lla = launch_leaf_apsp(static_cast<const uint8_t*>(d_pass.p), S, b, nl,
-                                    reinterpret_cast<const int*>(up + o_full), int(full_leaves.size()),
-                                    static_cast<uint16_t*>(leafD.p), static_cast<uint16_t*>(leafS.p), chg, stream, lkt,
-                                    kBackgroundWG, reinterpret_cast<const int*>(up + o_fidx));
-        prof_ref().close(lpi, lla);
-        // pivot rows: D row + P row per new cell; relaxation: D read and
-        // written once, P rows of the tile's rows and columns per pivot
-        const double np = double(pcell.size()), nsqd = double(nsq);
-        lpi = prof_open(HTG_K_LEAF_APSP, np * nsqd * 4 + double(b) * b * double(pitems.size()), 0, lkt);
-        lla = launch_leaf_pivot_rows(static_cast<const uint8_t*>(d_pass.p), S, b, nl,
-                                     reinterpret_cast<const int4*>(up + o_pit), int(pitems.size()),
-                                     reinterpret_cast<const int*>(up + o_pcell), static_cast<uint16_t*>(leafD.p),
-                                     static_cast<uint16_t*>(ws_leafP.p), static_cast<uint16_t*>(leafS.p), chg, stream,
-                                     lkt, kBackgroundWG);
-        prof_ref().close(lpi, lla);
-        double upd = 0;
-        for (const int4& r : rleaves) upd += double(r.z) * nsqd * nsqd;
-        lpi = prof_open(HTG_K_LEAF_APSP, double(rleaves.size()) * nsqd * nsqd * 4 + np * nsqd * 2 * 3, upd, lkt);
-        lla = launch_leaf_relax(b, reinterpret_cast<const int4*>(up + o_rl), int(rleaves.size()),
-                                static_cast<const uint16_t*>(ws_leafP.p), static_cast<uint16_t*>(leafD.p),
-                                static_cast<uint16_t*>(leafS.p), chg, stream, lkt);
-        prof_ref().close(lpi, lla);
-    }
     stats.n_touched_leaves = nd;
     stats.n_pivots = piv;
     if (H == 0) {

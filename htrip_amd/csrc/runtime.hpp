@@ -336,6 +336,8 @@ struct Map {
         ws_leaf_changed;
     DevBuf ws_upd, ws_chg;  // update: packed per-level lists; changed flags (leaves, then every level)
     DevBuf ws_leafP;        // update: BFS rows of the newly opened cells of incrementally revealed leaves
+    DevBuf ws_leafl;        // update: packed leaf-phase lists (new cells, leaf lists, pivot items)
+    HostBuf ws_leafl_h;
     HostBuf ws_upd_h, ws_chg_h, ws_counts_h;  // pinned staging of the lists, changed flags, pivot counts
 
     int build();
