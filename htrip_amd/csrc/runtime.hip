@@ -358,9 +358,13 @@ Map::~Map() {
     if (debug_level() >= 2 && upd_count > 0)
         std::fprintf(stderr,
                      "[htg] update host phases per call (%lld calls): cells+live %.0f us, plan %.0f us, levels "
-                     "host %.0f us, levels device wait %.0f us, stats %.0f us; catch-up %.0f us per call (%lld)\n",
+                     "host %.0f us (rebuild decisions %.0f us), levels device wait %.0f us, stats %.0f us; leaf phase "
+                     "%.0f us of the first; plan: pivots %.0f, seams %.0f, candidates %.0f, new ports %.0f us; "
+                     "catch-up %.0f us per call (%lld)\n",
                      upd_count, 1e6 * upd_time[0] / upd_count, 1e6 * upd_time[1] / upd_count,
-                     1e6 * upd_time[2] / upd_count, 1e6 * upd_time[3] / upd_count, 1e6 * upd_time[4] / upd_count,
+                     1e6 * upd_time[2] / upd_count, 1e6 * upd_time[6] / upd_count, 1e6 * upd_time[3] / upd_count,
+                     1e6 * upd_time[4] / upd_count, 1e6 * upd_time[7] / upd_count, 1e6 * upd_time[8] / upd_count,
+                     1e6 * upd_time[9] / upd_count, 1e6 * upd_time[10] / upd_count, 1e6 * upd_time[11] / upd_count,
                      catchup_count ? 1e6 * upd_time[5] / catchup_count : 0.0, catchup_count);
     if (build_graph) (void)hipGraphExecDestroy(build_graph);
     if (live_evt) {
@@ -1066,14 +1070,28 @@ int Map::open_cells(const int32_t* rcs, int n) {
     std::vector<int2> cells;
     cells.reserve(size_t(n));
     dirty.reserve(size_t(n));
+    // leaf_idx: per leaf, its position in dirty + 1 (0 = not dirty); all zero
+    // between updates
+    if (leaf_idx.size() != size_t(nl) * nl) leaf_idx.assign(size_t(nl) * nl, 0);
     for (int i = 0; i < n; ++i) {
         const int r = rcs[2 * i], c = rcs[2 * i + 1];
         if (r < 0 || c < 0 || r >= S || c >= S) continue;
         cells.push_back(make_int2(r, c));
-        dirty.push_back((r / b) * nl + c / b);
+        const int leaf = (r / b) * nl + c / b;
+        if (!leaf_idx[size_t(leaf)]) {
+            leaf_idx[size_t(leaf)] = 1;
+            dirty.push_back(leaf);
+        }
     }
     std::sort(dirty.begin(), dirty.end());
-    dirty.erase(std::unique(dirty.begin(), dirty.end()), dirty.end());
+    for (size_t i = 0; i < dirty.size(); ++i) leaf_idx[size_t(dirty[i])] = int(i) + 1;
+    struct ClearLeafIdx {  // every return path leaves leaf_idx all zero
+        std::vector<int>& idx;
+        const std::vector<int>& d;
+        ~ClearLeafIdx() {
+            for (int leaf : d) idx[size_t(leaf)] = 0;
+        }
+    } clear_leaf_idx{leaf_idx, dirty};
     if (dirty.empty()) return HTG_OK;
     for (const int2& p : cells) {
         if (p.x >= rows || p.y >= cols) continue;  // padding cells stay impassable
@@ -1084,7 +1102,19 @@ int Map::open_cells(const int32_t* rcs, int n) {
             newly.push_back(p);
         }
     }
-    if (!newly.empty() && (rc = refresh_live(&newly))) return rc;
+    // Only cells on a leaf's border can be ports at some level or end a seam
+    // edge (every tile above the leaves is a union of leaves).
+    auto on_leaf_border = [&](const int2& p) {
+        const int r = p.x % b, c = p.y % b;
+        return r == 0 || c == 0 || r == b - 1 || c == b - 1;
+    };
+    std::vector<int2> bnew, bcells;
+    for (const int2& p : newly)
+        if (on_leaf_border(p)) bnew.push_back(p);
+    for (const int2& p : cells)
+        if (on_leaf_border(p)) bcells.push_back(p);
+    if (!bnew.empty() && (rc = refresh_live(&bnew))) return rc;
+    const auto u0b = clk::now();
     const int nd = int(dirty.size());
     // 5. dirty leaves: a leaf with at most kLeafRelaxMax new cells is revealed
     // incrementally (BFS rows of its new cells, then the rank-k relaxation of
@@ -1095,15 +1125,19 @@ int Map::open_cells(const int32_t* rcs, int n) {
     std::vector<int> full_leaves, full_idx, pcell;
     std::vector<int4> rleaves, pitems;
     {
-        std::vector<std::vector<int>> per(static_cast<size_t>(nd));
-        for (const int2& p : newly) {
-            const int leaf = (p.x / b) * nl + p.y / b;
-            const size_t di = size_t(std::lower_bound(dirty.begin(), dirty.end(), leaf) - dirty.begin());
-            per[di].push_back((p.x % b) * b + p.y % b);
+        // new cells bucketed by dirty leaf (counting sort), in the order seen
+        std::vector<int> start(size_t(nd) + 1, 0), cell_of(newly.size());
+        for (const int2& p : newly) start[size_t(leaf_idx[size_t((p.x / b) * nl + p.y / b)])]++;
+        for (int i = 0; i < nd; ++i) start[size_t(i) + 1] += start[size_t(i)];
+        {
+            std::vector<int> at(start.begin(), start.end() - 1);
+            for (const int2& p : newly)
+                cell_of[size_t(at[size_t(leaf_idx[size_t((p.x / b) * nl + p.y / b)] - 1)]++)] =
+                    (p.x % b) * b + p.y % b;
         }
         const int swg = leaf_sources_per_wg(b);
         for (int i = 0; i < nd; ++i) {
-            const int k = int(per[size_t(i)].size());
+            const int k = start[size_t(i) + 1] - start[size_t(i)];
             if (k == 0) continue;
             if (!relax_ok || k > kLeafRelaxMax) {
                 full_leaves.push_back(dirty[size_t(i)]);
@@ -1112,8 +1146,9 @@ int Map::open_cells(const int32_t* rcs, int n) {
             }
             const int off = int(pcell.size());
             rleaves.push_back(make_int4(dirty[size_t(i)], off, k, i));
-            for (int c0 = 0; c0 < k; c0 += swg) pitems.push_back(make_int4(dirty[size_t(i)], off + c0, std::min(swg, k - c0), i));
-            pcell.insert(pcell.end(), per[size_t(i)].begin(), per[size_t(i)].end());
+            for (int c0 = 0; c0 < k; c0 += swg)
+                pitems.push_back(make_int4(dirty[size_t(i)], off + c0, std::min(swg, k - c0), i));
+            pcell.insert(pcell.end(), cell_of.begin() + start[size_t(i)], cell_of.begin() + start[size_t(i) + 1]);
         }
     }
     if (!pcell.empty() && (rc = ws_leafP.ensure(pcell.size() * size_t(nsq) * 2))) return rc;
@@ -1190,6 +1225,7 @@ int Map::open_cells(const int32_t* rcs, int n) {
         piv = int(marked.size());
         for (size_t key : marked) cell_mark[key] = 0;
     }
+    const auto p1 = clk::now();
     // 3. seam edges by LCA level and node (src/minplus_oracle.cpp:563-598),
     // as (level, node, u, v) sorted and unique
     struct SeamEdge {
@@ -1200,7 +1236,7 @@ int Map::open_cells(const int32_t* rcs, int n) {
         bool operator==(const SeamEdge& o) const { return lvl == o.lvl && node == o.node && u == o.u && v == o.v; }
     };
     std::vector<SeamEdge> seams;
-    for (const int2& p : cells) {
+    for (const int2& p : bcells) {
         if (!passable(p.x, p.y)) continue;
         for (int k = 0; k < 4; ++k) {
             const int nr = p.x + dr[k], nc = p.y + dc[k];
@@ -1229,6 +1265,7 @@ int Map::open_cells(const int32_t* rcs, int n) {
     }
     std::sort(seams.begin(), seams.end());
     seams.erase(std::unique(seams.begin(), seams.end()), seams.end());
+    const auto p2 = clk::now();
     // 4. candidate nodes per level: every ancestor of a dirty leaf (plus, to
     // be safe, every node with new seam edges).  The child masks of the
     // reference's routing (a parent is dirty when a child summary changed,
@@ -1273,12 +1310,13 @@ int Map::open_cells(const int32_t* rcs, int n) {
             se = se_end;
         }
     }
+    const auto p3 = clk::now();
     // new ports per level as (candidate slot, V_U index): the cover of the
     // decreased entries is built around them
     const bool cover = tune.update_pivots == 0;
     std::vector<std::vector<int2>> newp(size_t(H) + 1);
     if (cover) {
-        for (const int2& p : newly) {
+        for (const int2& p : bnew) {
             for (int lvl = 1; lvl <= H; ++lvl) {
                 const LevelGeo& L = lv[lvl - 1];
                 const int ty = p.x / L.d.s, tx = p.y / L.d.s;
@@ -1388,7 +1426,10 @@ int Map::open_cells(const int32_t* rcs, int n) {
         // passable seam endpoints) is rebuilt instead.  After any reveal every
         // table equals a fresh build of the revealed grid, so both paths give
         // the same tables, S_U and changed flags.
-        if (sum > 0 && cover && rebuild_is_cheaper(lvl, cl, hc, na)) {
+        const auto r0 = clk::now();
+        const bool reb = sum > 0 && cover && rebuild_is_cheaper(lvl, cl, hc, na);
+        upd_time[6] += std::chrono::duration<double>(clk::now() - r0).count();
+        if (reb) {
             if ((rc = rebuild_level(lvl, dnodes, cl, chg_here))) return rc;
             this->dirty.nodes[size_t(lvl) - 1] = cl;  // every candidate's table was rewritten
             rebuilt_levels++;
@@ -1467,7 +1508,12 @@ int Map::open_cells(const int32_t* rcs, int n) {
     epoch++;
     const auto u4 = clk::now();
     auto dt = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); };
-    upd_time[0] += dt(u0, u1);  // cell bookkeeping and live groups
+    upd_time[0] += dt(u0, u1);  // cell bookkeeping and live groups, leaf phase
+    upd_time[7] += dt(u0b, u1);  // leaf phase
+    upd_time[8] += dt(u1, p1);   // leaf pivot count
+    upd_time[9] += dt(p1, p2);   // seam edges
+    upd_time[10] += dt(p2, p3);  // candidates
+    upd_time[11] += dt(p3, u2);  // new ports
     upd_time[1] += dt(u1, u2);  // planning: leaf pivots, seams, candidates
     upd_time[2] += dt(u2, u3) - sync_s;  // level loop, host side
     upd_time[3] += sync_s;               // level loop, waiting for the device

@@ -392,7 +392,8 @@ struct Map {
     int ensure_build_ws(int lvl);
     int reserve_update_ws();
     std::vector<uint8_t> cell_mark;            // per grid cell, all zero between updates (deduplication)
-    double upd_time[6] = {0, 0, 0, 0, 0, 0};  // update phases (printed at HTG_DEBUG >= 2)
+    std::vector<int> leaf_idx;                 // per leaf, all zero between updates (dirty-leaf positions)
+    double upd_time[12] = {};  // update phases (printed at HTG_DEBUG >= 2)
     long long upd_count = 0, catchup_count = 0;
     // Lay out the live block of every level (once, at creation).
     int init_live();
