@@ -1019,9 +1019,19 @@ __global__ __launch_bounds__(256) void k_closure_pivots(int nV, int ld, const in
     uint32_t pre[RPW];
 #pragma unroll
     for (int i = 0; i < RPW; ++i) pre[i] = *reinterpret_cast<const uint32_t*>(D + size_t(kr[wave + 4 * i]) * ld + j0);
-    for (int i = tid; i < P * P; i += 256) {
-        const int a = i / P, c = i % P;
-        C[a][c] = (a < p && c < p) ? D[size_t(kr[a]) * ld + kr[c]] : (a == c ? 0 : kInf);
+    {  // the pivot block: every gather in flight before the first LDS store
+        constexpr int GPT = P * P / 256;
+        uint16_t gv[GPT];
+#pragma unroll
+        for (int e = 0; e < GPT; ++e) {
+            const int i = tid + 256 * e, a = i / P, c = i % P;
+            gv[e] = (a < p && c < p) ? D[size_t(kr[a]) * ld + kr[c]] : uint16_t(a == c ? 0 : kInf);
+        }
+#pragma unroll
+        for (int e = 0; e < GPT; ++e) {
+            const int i = tid + 256 * e;
+            C[i / P][i % P] = gv[e];
+        }
     }
     __syncthreads();
     // Floyd-Warshall over the block's pivots.  Thread t keeps EPT entries of
@@ -1116,19 +1126,29 @@ __global__ __launch_bounds__(256) void k_closure_update(int nV, int ld, const in
     else if (tid < 32) co[tid - 16] = J * 16 + tid - 16 < nl8 ? 8 * l8[J * 16 + tid - 16] : -1;
     __syncthreads();
     const uint16_t* R = Rbuf + size_t(slot) * P * ld;
-    // stage with 16-byte loads: item i = (pivot k, octet slot o, side)
-    for (int i = tid; i < P * 32; i += 256) {
+    // stage with 16-byte loads: item i = (pivot k, octet slot o, side); every
+    // load in flight before the first LDS store
+    constexpr int SPT = P * 32 / 256;
+    uint4 sv[SPT];
+#pragma unroll
+    for (int e = 0; e < SPT; ++e) {
+        const int i = tid + 256 * e;
         const int k = i >> 5, o = (i >> 1) & 15, col_side = i & 1;
         const int base = col_side ? co[o] : ro[o];
-        uint4 v = make_uint4(kInf2, kInf2, kInf2, kInf2);
-        if (k < p && base >= 0) {
-            v = *reinterpret_cast<const uint4*>(R + size_t(k) * ld + base);
-            if (base + 8 > nV) {  // octet straddles nV: entries past it are padding
-                uint16_t* e = reinterpret_cast<uint16_t*>(&v);
+        sv[e] = make_uint4(kInf2, kInf2, kInf2, kInf2);
+        if (k < p && base >= 0) sv[e] = *reinterpret_cast<const uint4*>(R + size_t(k) * ld + base);
+    }
 #pragma unroll
-                for (int j = 0; j < 8; ++j)
-                    if (base + j >= nV) e[j] = kInf;
-            }
+    for (int e = 0; e < SPT; ++e) {
+        const int i = tid + 256 * e;
+        const int k = i >> 5, o = (i >> 1) & 15, col_side = i & 1;
+        const int base = col_side ? co[o] : ro[o];
+        uint4 v = sv[e];
+        if (base >= 0 && base + 8 > nV) {  // octet straddles nV: entries past it are padding
+            uint16_t* ev = reinterpret_cast<uint16_t*>(&v);
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (base + j >= nV) ev[j] = kInf;
         }
         if (col_side) {
             Rc4[k][o] = v;
