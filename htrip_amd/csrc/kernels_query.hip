@@ -441,8 +441,9 @@ bool launch_q_fold(int n, int nz, int ldz, const int* src, int ldp, int ksplit, 
 // reference's X_i-projection (+) X_j meet: the minimum over levels is exact.
 // One launch serves every level: Z of each level stays in its own region.
 // One 256-thread workgroup computes an 8 x 8 tile of (i, j) pairs of one
-// (query, W, q2) group, 4 lanes per pair on interleaved columns, streaming
-// the seam block through LDS as packed pairs.
+// (query, W, q2) group, 2 x 2 pairs per thread with 16 lanes per 2 x 2
+// block on interleaved columns, streaming the seam block through LDS as
+// packed pairs.
 // Tiles are 8 ints (group, q2, i0, j0, j_end, level); zgeo holds per level
 // (Z offset lo, hi, ldz, 0, zoff[g2], zlen[g2]).  A pair meets at several
 // levels, so the result is merged with a 16-bit atomic minimum.
@@ -465,7 +466,7 @@ __global__ __launch_bounds__(256) void k_q_meet(const int* __restrict__ tiles, c
                                                 const int* __restrict__ q_k, const long long* __restrict__ q_out,
                                                 const uint16_t* __restrict__ Zall, uint16_t* __restrict__ out) {
     constexpr int CH = 256;      // packed pairs per round (512 columns)
-    constexpr int LDR = CH + 4;  // row stride: the 8 B rows a wavefront reads fall in distinct banks
+    constexpr int LDR = CH + 8;  // row stride: rows two apart fall 16 banks apart
     __shared__ uint32_t A[8][LDR];
     __shared__ uint32_t Bm[8][LDR];
     // tile: group size, target child q2, first rows i0 / j0, j end, level,
@@ -479,10 +480,15 @@ __global__ __launch_bounds__(256) void k_q_meet(const int* __restrict__ tiles, c
     const int* lst = glist + t1.z;
     const int* zg = zgeo + size_t(lvl) * (4 + 2 * g2);
     const int tid = threadIdx.x;
-    const int sub = tid & 3, pair = tid >> 2, pi = pair >> 3, pj = pair & 7;
+    // 2 x 2 pairs per thread: block (bi, bj) = rows 2bi, 2bi+1 of the tile's
+    // i side against rows 2bj, 2bj+1 of its j side, 16 lanes per block on
+    // interleaved columns (four LDS reads feed four packed relaxations)
+    const int l16 = tid & 15, blk = tid >> 4, bi = blk >> 2, bj = blk & 3;
     const int ni = min(8, n - i0), nj = min(8, jend - j0);
-    // the result's address, resolved now (its dependent loads overlap the row loads)
-    const bool writer = sub == 0 && pi < ni && pj < nj;
+    // the result's address, resolved now (its dependent loads overlap the row
+    // loads): lane l < 4 of a block writes pair (2bi + l / 2, 2bj + l % 2)
+    const int pi = 2 * bi + (l16 >> 1), pj = 2 * bj + (l16 & 1);
+    const bool writer = l16 < 4 && pi < ni && pj < nj;
     int si = 0, sj = 0;
     size_t oaddr = 0;
     if (writer) {
@@ -495,14 +501,14 @@ __global__ __launch_bounds__(256) void k_q_meet(const int* __restrict__ tiles, c
     const int off = zg[4 + q2], len2 = zg[4 + g2 + q2] / 2;  // block length is even
     // staging: thread (row, l16) loads columns l16, l16 + 16, ... of one of
     // the 16 rows (8 of A, then 8 of B)
-    const int row = tid >> 4, l16 = tid & 15;
+    const int row = tid >> 4;
     const bool is_a = row < 8;
     const int rr = row & 7;
     const int nrow = is_a ? ni : nj;
     const uint32_t* zr =
         rr < nrow ? reinterpret_cast<const uint32_t*>(Z + size_t(lst[(is_a ? i0 : j0) + rr]) * ldz + off) : nullptr;
     uint32_t* dst = is_a ? A[rr] : Bm[rr];
-    uint32_t acc0 = kInf2, acc1 = kInf2;
+    uint32_t a00 = kInf2, a01 = kInf2, a10 = kInf2, a11 = kInf2;
     for (int c0 = 0; c0 < len2; c0 += CH) {
         const int cn = min(CH, len2 - c0);
         __syncthreads();
@@ -512,18 +518,29 @@ __global__ __launch_bounds__(256) void k_q_meet(const int* __restrict__ tiles, c
             dst[cc] = (zr && cc < cn) ? zr[c0 + cc] : kInf2;
         }
         __syncthreads();
-        // 4 lanes per pair take interleaved columns; two accumulators
-        int cc = sub;
-        for (; cc + 4 < cn; cc += 8) {
-            acc0 = qrelax2(acc0, A[pi][cc], Bm[pj][cc]);
-            acc1 = qrelax2(acc1, A[pi][cc + 4], Bm[pj][cc + 4]);
+        const uint32_t* A0 = A[2 * bi];
+        const uint32_t* A1 = A[2 * bi + 1];
+        const uint32_t* B0 = Bm[2 * bj];
+        const uint32_t* B1 = Bm[2 * bj + 1];
+#pragma unroll 2
+        for (int cc = l16; cc < cn; cc += 16) {
+            const uint32_t x0 = A0[cc], x1 = A1[cc], y0 = B0[cc], y1 = B1[cc];
+            a00 = qrelax2(a00, x0, y0);
+            a01 = qrelax2(a01, x0, y1);
+            a10 = qrelax2(a10, x1, y0);
+            a11 = qrelax2(a11, x1, y1);
         }
-        if (cc < cn) acc0 = qrelax2(acc0, A[pi][cc], Bm[pj][cc]);
     }
-    uint32_t acc = qmin2(acc0, acc1);
-    acc = qmin2(acc, uint32_t(__shfl_xor(int(acc), 1)));
-    acc = qmin2(acc, uint32_t(__shfl_xor(int(acc), 2)));
+    // minimum over the block's 16 lanes (xor 1..8 stays inside the group)
+#pragma unroll
+    for (int m = 1; m < 16; m <<= 1) {
+        a00 = qmin2(a00, uint32_t(__shfl_xor(int(a00), m)));
+        a01 = qmin2(a01, uint32_t(__shfl_xor(int(a01), m)));
+        a10 = qmin2(a10, uint32_t(__shfl_xor(int(a10), m)));
+        a11 = qmin2(a11, uint32_t(__shfl_xor(int(a11), m)));
+    }
     if (!writer || si == sj) return;
+    const uint32_t acc = l16 == 0 ? a00 : l16 == 1 ? a01 : l16 == 2 ? a10 : a11;
     const uint16_t best = min(uint16_t(acc & 0xffffu), uint16_t(acc >> 16));
     if (best >= kInf) return;
     atomic_min_u16(out + oaddr, best);
