@@ -150,42 +150,91 @@ void launch_fill_u8(uint8_t* p, size_t n, uint8_t v, hipStream_t st);
 void launch_set_pass(uint8_t* pass, int S, const int2* cells, int n, hipStream_t st, uint8_t v = 1);
 
 // ---- query -----------------------------------------------------------------
-// One launch: out matrices (INF, 0 on the diagonal of valid viewpoints, leaf
-// D for same-leaf pairs; vp_* arrays are per viewpoint, all queries packed)
-// and X0[slot][p] = leaf D row of the slot's cell at leaf port p; clears
-// wstat (nullable).
-// INF fills: n_fill ranges (start lo, start hi, n8, first block) of n8 units
-// of 8 u16 from fill_base, fill_blocks blocks in all.  Returns false without
-// launching when n_fill > 64 (one ballot finds a block's range).
-bool launch_q_front(int nq, const int* q_k, const long long* q_out, const int* q_vp, const int2* vp_rc,
-                    const uint8_t* vp_valid, int kmax, int n_slots, const int2* slot_rc, int b, int nl,
-                    const uint16_t* leafD, uint16_t* out, uint16_t* X0, int n_fill, const int4* fills,
-                    int fill_blocks, uint16_t* fill_base, unsigned long long* wstat, hipStream_t st);
-// Projection partials: Zp[chunk][slot][col] = min over the chunk's seam rows a
-// of X[slot][a] (+) D_W[q*pc + a][cols(col)] for work items (W, q | first-of-
-// run flag << 16, first slot, count); ksplit row chunks per item; only the
-// live column groups of W (live_list, live_cnt), col_tiles = max over the
-// items of ceil(live_cnt / 64).  With ksplit == 1 the results go straight to
-// Z (position colpos[col]) and the lifted Xn (exterior port colext[col]).
 // Work counters of the query projection: kWorkSlots (bytes, updates) pairs,
 // summed on the host.
 constexpr int kWorkSlots = 64;
-// items: two int4 per work item, (node, child | first flag << 16, first slot,
-// slots) and (live column groups of the node, seam range lo, hi, first seam
-// port of the child).
-bool launch_q_z(int M, int ld, int pc, const uint16_t* DU, size_t node_stride, const int* seam_list,
-                const int* cols, int ngrp, const int* live_list, int col_tiles, int ldp, const int4* items,
-                int n_items, int ksplit, const uint16_t* X, uint16_t* Zp,
-                size_t part_stride, const int* colpos, const int* colext, const int* ext_child,
-                const int* ext_cport, uint16_t* Z, int ldz, uint16_t* Xn, int ne, unsigned long long* wstat,
-                hipStream_t st, KTimer kt = {});
-// Fold: Z[slot][pos] = min_c Zp[c][slot][src[pos]] (INF where src[pos] < 0 or
-// the column's group is not live in the slot's node) and, for ne > 0, the
-// lift Xnext[slot][e] = min(folded column ext_col[e], Xprev[slot][own port of e]).
-bool launch_q_fold(int n, int nz, int ldz, const int* src, int ldp, int ksplit, const uint16_t* Zp,
-                   size_t part_stride, const int2* slot_rc, int side, int nt, int ngrp, const uint8_t* live_map,
-                   uint16_t* Z, int ne, const int* ext_col, const int* ext_child, const int* ext_cport, int pc, int cs,
-                   int g, const uint16_t* Xprev, uint16_t* Xnext, hipStream_t st, KTimer kt = {});
+
+// Argument blocks of the query's work units.  The per-phase kernels take
+// them by value; the persistent launch reads them from the batch's plan in
+// device memory.
+//
+// Front: out matrices (INF, 0 on the diagonal of valid viewpoints, leaf D
+// for same-leaf pairs; q_* per query, vp_* per viewpoint, all queries
+// packed), X0[slot][p] = leaf D row of the slot's cell at leaf port p, and
+// INF fills of n_fill ranges (start lo, start hi, n8, first unit) of n8
+// units of 8 u16 from fill_base (n_fill <= 64: one ballot finds a unit's
+// range); unit 0 clears wstat (nullable).  Units: nq * ib output blocks,
+// leaf_blocks, fill_blocks.
+struct QFrontArgs {
+    const int* q_k;
+    const long long* q_out;
+    const int* q_vp;
+    const int2* vp_rc;
+    const uint8_t* vp_valid;
+    const int2* slot_rc;
+    const uint16_t* leafD;
+    uint16_t* out;
+    uint16_t* X0;
+    const int4* fills;
+    uint16_t* fill_base;
+    unsigned long long* wstat;
+    int nq, ib, leaf_blocks, n_slots, b, nl, n_fill, fill_blocks;
+};
+// One level's projection: Zp[chunk][slot][col] = min over the chunk's seam
+// rows a of X[slot][a] (+) D_W[q*pc + a][cols(col)] for work items (two
+// int4 each: (W, q | first-of-run flag << 16, first slot, count) and (live
+// column groups of W, seam range lo, hi, first seam port of q)); ksplit row
+// chunks per item, col_tiles tiles of 64 live groups (the largest item's
+// count).  With ksplit == 1 the results go straight to Z (position
+// colpos[col]) and the lifted Xn (exterior port colext[col]; ne = 0 at the
+// root).  Units: n_items x col_tiles x ksplit, items fastest.
+// Its fold (ksplit > 1): Z[slot][pos] = min_c Zp[c][slot][zsrc[pos]] (INF
+// where zsrc[pos] < 0 or the column's group is not live in the slot's node)
+// and Xn[slot][e] = min(folded column ext_col[e], X[slot][own port of e]).
+// Units: ceil(n_slots * (nz + ne) / 256).
+struct QZArgs {
+    const uint16_t* DU;
+    const int* seam_list;
+    const int* cols;
+    const int* live_list;
+    const int4* items;
+    const uint16_t* X;
+    uint16_t* Zp;
+    const int* colpos;
+    const int* colext;
+    const int* ext_child;
+    const int* ext_cport;
+    uint16_t* Z;
+    uint16_t* Xn;
+    unsigned long long* wstat;
+    const int* zsrc;
+    const int2* slot_rc;
+    const uint8_t* live_map;
+    const int* ext_col;
+    size_t node_stride, part_stride;
+    int ld, pc, ngrp, ldp, n_items, col_tiles, ksplit, ldz, ne, M;
+    int n_slots, nz, side, nt, cs, g;
+};
+// Meet tiles (8 ints each: group size, q2, i0, j0, j_end, level, the group's
+// offset in glist, its query): out[i][j] = min(out, min over the q2 seam
+// block of Z_level[i] (+) Z_level[j]); zgeo per level: Z offset lo, hi, ldz,
+// 0, zoff[g2], zlen[g2].  One unit per tile.
+struct QMeetArgs {
+    const int* tiles;
+    const int* glist;
+    const int* zgeo;
+    const int* slot_i;
+    const int* q_k;
+    const long long* q_out;
+    const uint16_t* Z;
+    uint16_t* out;
+    int n_tiles, g2;
+};
+
+bool launch_q_front(const QFrontArgs& a, hipStream_t st);
+bool launch_q_z(const QZArgs& a, hipStream_t st, KTimer kt = {});
+bool launch_q_fold(const QZArgs& a, hipStream_t st, KTimer kt = {});
+bool launch_q_meet(const QMeetArgs& a, hipStream_t st, KTimer kt = {});
 // liftPoint: X0[i] = row[cell of leaf port i] (row: the point's row of its leaf D).
 void launch_lift_leaf(const uint16_t* row, int b, uint16_t* X0, hipStream_t st);
 // liftPoint, one level: Xo[e] = min_p X[p] (+) D[row0 + p][column of exterior
@@ -195,11 +244,5 @@ constexpr int kLiftRowsPerChunk = 64;
 void launch_lift_level(const uint16_t* D, int ld, int row0, int pc, int rows_per_chunk, int nchunks,
                        const uint16_t* X, uint16_t* part, const int* ext_child, const int* ext_cport, int ne,
                        uint16_t* Xo, hipStream_t st);
-// Meet tiles of every level (8 ints each: group size, q2, i0, j0, j_end,
-// level, the group's offset in glist, its query):
-// out[i][j] = min(out, min over the q2 seam block of Z_level[i] (+) Z_level[j]).
-bool launch_q_meet(const int* tiles, int n_tiles, const int* glist, const int* zgeo, int g2, const int* slot_i,
-                   const int* q_k, const long long* q_out, const uint16_t* Z, uint16_t* out, hipStream_t st,
-                   KTimer kt = {});
 
 }  // namespace htg

@@ -59,144 +59,142 @@ __device__ __forceinline__ unsigned xcd_logical(unsigned hw, unsigned n) {
     return (k / G) * (X * G) + x * G + k % G;
 }
 
-// Front of the query, one 1-D launch with three block ranges:
-//  [0, nq * ib): output init (src/minplus_oracle.cpp:940-965, same-leaf pairs
-//     :1027-1048) of query blk / ib: INF, 0 on the diagonal of valid
-//     viewpoints, leaf D for same-leaf pairs;
-//  next leaf_blocks: level-0 lift, X0[slot][p] = leaf D row of the slot's cell
-//     at leaf port p (src/minplus_oracle.cpp:1000-1019);
+// Global-memory pointers.  The units read their pointers from argument
+// blocks, which in the persistent launch live in device memory; typed as
+// global, every access is a global_* instruction with a scalar base instead
+// of a flat one with a 64-bit vector address.  Vector loads and stores use
+// the compiler's own vector types (HIP's vector classes do not bind to a
+// global-qualified reference).
+#define HTG_G(T) __attribute__((address_space(1))) T*
+template <typename T>
+__device__ __forceinline__ HTG_G(T) gp(T* p) {
+    return (HTG_G(T))p;
+}
+// The same memory viewed as elements of type V.
+template <typename V, typename T>
+__device__ __forceinline__ HTG_G(V) gpv(T* p) {
+    return (HTG_G(V))p;
+}
+typedef int i32x2 __attribute__((ext_vector_type(2)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+// Front (src/minplus_oracle.cpp:940-965, same-leaf pairs :1027-1048, level-0
+// lift :1000-1019), unit blk of three ranges:
+//  [0, nq * ib): output init of query blk / ib: INF, 0 on the diagonal of
+//     valid viewpoints, leaf D for same-leaf pairs;
+//  next leaf_blocks: level-0 lift, X0[slot][p] = leaf D row of the slot's
+//     cell at leaf port p;
 //  then INF fills of the buffers whose live-only writers leave gaps: fill
-//     range r = (start lo, start hi, n8, first block), n8 units of 8 u16.
-// Block 0 also clears the k_q_z work counters.
-__global__ __launch_bounds__(256) void k_q_front(int nq, int ib, int leaf_blocks, const int* __restrict__ q_k,
-                                                 const long long* __restrict__ q_out, const int* __restrict__ q_vp,
-                                                 const int2* __restrict__ vp_rc, const uint8_t* __restrict__ vp_valid,
-                                                 int n_slots, const int2* __restrict__ slot_rc, int b, int nl,
-                                                 const uint16_t* __restrict__ leafD, uint16_t* __restrict__ out,
-                                                 uint16_t* __restrict__ X0, int n_fill, const int4* __restrict__ fills,
-                                                 uint16_t* __restrict__ fill_base,
-                                                 unsigned long long* __restrict__ wstat) {
-    const int n = b * b;
-    const int blk = int(blockIdx.x);
-    if (wstat && blk == 0 && threadIdx.x < 2 * kWorkSlots) wstat[threadIdx.x] = 0;
-    if (blk < nq * ib) {
-        const int Q = blk / ib;
-        const int k = q_k[Q];
-        const long long idx = (long long)(blk % ib) * 256 + threadIdx.x;
+//     range r = (start lo, start hi, n8, first unit), n8 units of 8 u16.
+// Unit 0 also clears the projection's work counters.
+__device__ __forceinline__ void front_unit(const QFrontArgs& a, int blk) {
+    const int b = a.b, n = b * b;
+    const auto leafD = gp(a.leafD);
+    if (a.wstat && blk == 0 && threadIdx.x < 2 * kWorkSlots) gp(a.wstat)[threadIdx.x] = 0;
+    if (blk < a.nq * a.ib) {
+        const int Q = blk / a.ib;
+        const int k = gp(a.q_k)[Q];
+        const long long idx = (long long)(blk % a.ib) * 256 + threadIdx.x;
         if (idx >= (long long)k * k) return;
         const int i = int(idx / k), j = int(idx % k);
-        const int vi = q_vp[Q] + i, vj = q_vp[Q] + j;
+        const int vp0 = gp(a.q_vp)[Q];
+        const int vi = vp0 + i, vj = vp0 + j;
+        const auto valid = gp(a.vp_valid);
+        const auto vp_rc = gpv<const i32x2>(a.vp_rc);
         uint16_t val = kInf;
-        if (vp_valid[vi] && vp_valid[vj]) {
+        if (valid[vi] && valid[vj]) {
             if (i == j) {
                 val = 0;
             } else {
-                const int2 a = vp_rc[vi], c = vp_rc[vj];
-                if (a.x / b == c.x / b && a.y / b == c.y / b) {
-                    const size_t leaf = size_t(a.x / b) * nl + a.y / b;
-                    val = leafD[leaf * n * n + size_t((a.x % b) * b + a.y % b) * n + (c.x % b) * b + c.y % b];
+                const i32x2 p = vp_rc[vi], c = vp_rc[vj];
+                if (p.x / b == c.x / b && p.y / b == c.y / b) {
+                    const size_t leaf = size_t(p.x / b) * a.nl + p.y / b;
+                    val = leafD[leaf * n * n + size_t((p.x % b) * b + p.y % b) * n + (c.x % b) * b + c.y % b];
                 }
             }
         }
-        out[q_out[Q] + idx] = val;
+        gp(a.out)[gp(a.q_out)[Q] + idx] = val;
         return;
     }
-    const int lb = blk - nq * ib;
-    if (lb < leaf_blocks) {
+    const int lb = blk - a.nq * a.ib;
+    if (lb < a.leaf_blocks) {
         const int pl = 4 * b - 4;
         const long long gid = (long long)lb * 256 + threadIdx.x;
-        if (gid >= (long long)n_slots * pl) return;
+        if (gid >= (long long)a.n_slots * pl) return;
         const int s = int(gid / pl), p = int(gid % pl);
-        const int2 rc = slot_rc[s];
+        const i32x2 rc = gpv<const i32x2>(a.slot_rc)[s];
         int pr, pcc;
         port_rc(p, b, pr, pcc);
-        const size_t leaf = size_t(rc.x / b) * nl + rc.y / b;
-        X0[gid] = leafD[leaf * n * n + size_t((rc.x % b) * b + rc.y % b) * n + pr * b + pcc];
+        const size_t leaf = size_t(rc.x / b) * a.nl + rc.y / b;
+        gp(a.X0)[gid] = leafD[leaf * n * n + size_t((rc.x % b) * b + rc.y % b) * n + pr * b + pcc];
         return;
     }
-    // the fill range of this block: ranges are in order of their first block,
+    // the fill range of this unit: ranges are in order of their first unit,
     // so r counts the ranges after the first that start at or before fb (one
     // load per lane and a ballot instead of a walk of dependent loads)
-    const int fb = lb - leaf_blocks;
+    const auto fills = gpv<const i32x4>(a.fills);
+    const int fb = lb - a.leaf_blocks;
     const int lane = int(threadIdx.x & 63u);
-    const bool starts_before = lane >= 1 && lane < n_fill && fills[lane].w <= fb;
+    const bool starts_before = lane >= 1 && lane < a.n_fill && fills[lane].w <= fb;
     const int r = __popcll(__ballot(starts_before));
-    const int4 f = fills[r];
+    const i32x4 f = fills[r];
     const long long u = (long long)(fb - f.w) * 256 + threadIdx.x;
     if (u >= f.z) return;
     const size_t start = (size_t(unsigned(f.y)) << 32) | unsigned(f.x);
-    reinterpret_cast<uint4*>(fill_base + start)[u] = make_uint4(kInf2, kInf2, kInf2, kInf2);
+    gpv<u32x4>(gp(a.fill_base) + start)[u] = u32x4{kInf2, kInf2, kInf2, kInf2};
 }
 
-bool launch_q_front(int nq, const int* q_k, const long long* q_out, const int* q_vp, const int2* vp_rc,
-                    const uint8_t* vp_valid, int kmax, int n_slots, const int2* slot_rc, int b, int nl,
-                    const uint16_t* leafD, uint16_t* out, uint16_t* X0, int n_fill, const int4* fills,
-                    int fill_blocks, uint16_t* fill_base, unsigned long long* wstat, hipStream_t st) {
-    // the block's fill range is found with one 64-lane ballot over the ranges
-    if (n_fill > 64 || n_fill < 0) return false;
-    if (nq == 0) return true;
-    const int ib = int(((long long)kmax * kmax + 255) / 256);
-    const int leaf_blocks = int(((long long)n_slots * (4 * b - 4) + 255) / 256);
-    const long long blocks = (long long)nq * ib + leaf_blocks + fill_blocks;
-    if (blocks == 0) return true;
-    hipLaunchKernelGGL(k_q_front, dim3(unsigned(blocks)), dim3(256), 0, st, nq, ib, leaf_blocks, q_k, q_out, q_vp,
-                       vp_rc, vp_valid, n_slots, slot_rc, b, nl, leafD, out, X0, n_fill, fills, fill_base, wstat);
-    return true;
-}
-
-// Projection Z (see file comment), split over the seam rows: work item
-// (W, q, first slot, count), column tile and row chunk (the grid's x, y, z in
-// an XCD-aware order);
-// four wavefronts share the columns and take interleaved rounds of rows.
-// Only live column groups are computed: live_list[W] holds the groups of 4
-// projection columns with at least one passable port (the others are INF and
-// k_q_zred writes them so), 64 groups = 256 columns per workgroup.  Each
-// wavefront keeps M x 4 packed accumulators and reads one 8-byte slice of a
-// D_W row per seam row.  Seam rows whose X is INF for all M viewpoints add
-// nothing and are dropped: each round stages 64 rows, compacts the finite
-// ones (ballot + prefix count) into LDS with their X values duplicated into
-// both u16 halves, and pads the count to PF rows with INF.  Partial minima go
-// to Zp[chunk] and k_q_fold folds the chunks; with one chunk (ksplit == 1)
-// the kernel folds in registers instead: column c goes to Z position
-// colpos[c] and, below the root, lifts to exterior port colext[c] of W,
+// Projection Z (see file comment), split over the seam rows: unit (work item
+// bx, column tile by, row chunk bz).  Four wavefronts share the columns and
+// take interleaved rounds of rows.  Only live column groups are computed:
+// live_list[W] holds the groups of 4 projection columns with at least one
+// passable port (the others are INF, written so by the front's fills or the
+// fold), 64 groups = 256 columns per unit.  Each wavefront keeps M x 4
+// packed accumulators and reads one 8-byte slice of a D_W row per seam row.
+// Seam rows whose X is INF for all M viewpoints add nothing and are dropped:
+// each round stages 64 rows, compacts the finite ones (ballot + prefix
+// count) into LDS with their X values duplicated into both u16 halves, and
+// pads the count to PF rows with INF.  Partial minima go to Zp[chunk] and
+// the fold combines the chunks; with one chunk (ksplit == 1) the unit folds
+// in registers instead: column c goes to Z position colpos[c] and, below the
+// root, lifts to exterior port colext[c] of W,
 // Xn[slot][e] = min(Z, X[slot][own port of e] when e is a port of q)
 // (src/minplus_oracle.cpp:1113-1154).  wstat (nullable) accumulates the
 // work done: [0] bytes (D slices of distinct (W, q) items, X reads, Zp
 // writes, each counted once), [1] min-plus updates on finite rows.
+constexpr int kZRows = 64;  // seam rows staged per wavefront and round
+constexpr int kZWaves = 4;  // wavefronts per workgroup: same columns, interleaved row rounds
+template <int M>
+struct ZLds {
+    uint32_t xs[kZWaves][kZRows][M];  // per wavefront, row-major: one row's M values contiguous
+    unsigned roff[kZWaves][kZRows];
+    u32x2 red[kZWaves - 1][M][64];  // accumulators of wavefronts 1..3 for the final minimum
+    unsigned long long wsum[kZWaves][2];
+};
+
 template <int M, int PF>
-__global__ __launch_bounds__(256) void k_q_z(
-    int ld, int pc, const uint16_t* __restrict__ DU, size_t node_stride, const int* __restrict__ seam_list,
-    const int* __restrict__ cols, int ngrp, const int* __restrict__ live_list, int ldp,
-    const int4* __restrict__ items, int ksplit,
-    const uint16_t* __restrict__ X, uint16_t* __restrict__ Zp, size_t part_stride,
-    const int* __restrict__ colpos, const int* __restrict__ colext, const int* __restrict__ ext_child,
-    const int* __restrict__ ext_cport, uint16_t* __restrict__ Z, int ldz, uint16_t* __restrict__ Xn, int ne,
-    unsigned long long* __restrict__ wstat) {
-    constexpr int ACH = 64;  // seam rows staged per wavefront and round (PF: D rows in flight per lane)
-    constexpr int NW = 4;    // wavefronts per workgroup: same columns, interleaved row rounds
+__device__ __forceinline__ void z_unit(const QZArgs& a, int bx, int by, int bz, ZLds<M>& sh) {
     // Padding rows carry INF in xs: min(acc, INF (+) d) leaves acc unchanged,
     // so the loop runs branch-free over whole PF blocks.
-    __shared__ __align__(16) uint32_t xs[NW][ACH][M];  // per wavefront, row-major: one row's M values contiguous
-    __shared__ unsigned roff[NW][ACH];
-    __shared__ uint2 red[NW - 1][M][64];  // accumulators of wavefronts 1..NW-1 for the final minimum
-    // logical (item, column tile, row chunk): items of one (W, q) read the same
-    // D rows, so runs of consecutive items go to one XCD
-    const unsigned lin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-    const unsigned L = xcd_logical<kXcdRun>(lin, gridDim.x * gridDim.y * gridDim.z);
-    const int bx = int(L % gridDim.x), by = int((L / gridDim.x) % gridDim.y), bz = int(L / (gridDim.x * gridDim.y));
-    const int4 it = items[2 * bx], it2 = items[2 * bx + 1];
+    constexpr int ACH = kZRows, NW = kZWaves;
+    const int ld = a.ld, pc = a.pc, ksplit = a.ksplit;
+    const auto items = gpv<const i32x4>(a.items);
+    const i32x4 it = items[2 * bx], it2 = items[2 * bx + 1];
     const int W = it.x, q = it.y & 0xffff, s0 = it.z, m = it.w;
     const bool first_item = (it.y >> 16) & 1;  // first item of its (W, q) run: counts the D bytes once
     const int tid = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int nlive = it2.x;  // live column groups of W
     const int gi = by * 64 + tid;
-    if (by * 64 >= nlive) return;  // whole workgroup past W's live groups
+    if (by * 64 >= nlive) return;  // whole unit past W's live groups
     const bool active = gi < nlive;
-    const int c0 = active ? live_list[size_t(W) * ngrp + gi] * 4 : 0;
+    const int c0 = active ? gp(a.live_list)[size_t(W) * a.ngrp + gi] * 4 : 0;
     // Projection columns come in aligned groups of 4 consecutive V_U ports,
     // so one 8-byte load per seam row feeds the lane's 4 columns.
-    const int base = cols[c0];
-    const uint16_t* D = DU + size_t(W) * node_stride + base;
+    const int base = gp(a.cols)[c0];
+    const auto D = gp(a.DU) + size_t(W) * a.node_stride + base;
+    const auto X = gp(a.X);
     // The epilogue's metadata (Z positions, exterior ports, the slot's own
     // X at those ports), loaded now by the wavefront that writes the result:
     // its dependent loads overlap the row loads instead of following them.
@@ -207,20 +205,20 @@ __global__ __launch_bounds__(256) void k_q_z(
 #pragma unroll
         for (int k = 0; k < 4; ++k) ownx[v][k] = kInf;
     if (ksplit == 1 && w == 0 && active) {
-        const int4 cp = *reinterpret_cast<const int4*>(colpos + c0);
-        const int4 ce = *reinterpret_cast<const int4*>(colext + c0);
+        const i32x4 cp = *gpv<const i32x4>(gp(a.colpos) + c0);
+        const i32x4 ce = *gpv<const i32x4>(gp(a.colext) + c0);
         pos[0] = cp.x, pos[1] = cp.y, pos[2] = cp.z, pos[3] = cp.w;
         ext[0] = ce.x, ext[1] = ce.y, ext[2] = ce.z, ext[3] = ce.w;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const int own = (ext[k] >= 0 && ext_child[ext[k]] == q) ? ext_cport[ext[k]] : -1;
+            const int own = (ext[k] >= 0 && gp(a.ext_child)[ext[k]] == q) ? gp(a.ext_cport)[ext[k]] : -1;
             if (own < 0) continue;
 #pragma unroll
             for (int v = 0; v < M; ++v)
                 if (v < m) ownx[v][k] = X[size_t(s0 + v) * pc + own];
         }
     }
-    const int* seam = seam_list + it2.y;  // child q's seam ports
+    const auto seam = gp(a.seam_list) + it2.y;  // child q's seam ports
     const int ns = it2.z - it2.y;
     const int a_lo = int((long long)ns * bz / ksplit), a_hi = int((long long)ns * (bz + 1) / ksplit);
     const unsigned pad_off = unsigned(q * pc + it2.w) * unsigned(ld);
@@ -248,38 +246,39 @@ __global__ __launch_bounds__(256) void k_q_z(
         const unsigned long long bal = __ballot(fin);
         const int nk = __popcll(bal);
         const int nkp = (nk + PF - 1) / PF * PF;
-        const int pos = __popcll(bal & ((1ull << tid) - 1ull));
+        const int pos_r = __popcll(bal & ((1ull << tid) - 1ull));
         __syncthreads();  // previous round's readers are done with xs / roff
         if (fin) {
-            roff[w][pos] = unsigned(q * pc + sr) * unsigned(ld);
+            sh.roff[w][pos_r] = unsigned(q * pc + sr) * unsigned(ld);
 #pragma unroll
-            for (int v = 0; v < M; ++v) xs[w][pos][v] = qdup(xv[v]);
+            for (int v = 0; v < M; ++v) sh.xs[w][pos_r][v] = qdup(xv[v]);
         }
         if (tid >= nk && tid < nkp) {
-            roff[w][tid] = pad_off;
+            sh.roff[w][tid] = pad_off;
 #pragma unroll
-            for (int v = 0; v < M; ++v) xs[w][tid][v] = kInf2;
+            for (int v = 0; v < M; ++v) sh.xs[w][tid][v] = kInf2;
         }
         __syncthreads();
         rows_done += nk;
         if (!active) continue;
         // D rows double-buffered: the next PF rows are in flight while this
         // block's PF rows are folded in
-        uint2 dv[PF], dn[PF];
+        u32x2 dv[PF], dn[PF];
         if (nkp > 0) {
 #pragma unroll
-            for (int u = 0; u < PF; ++u) dn[u] = *reinterpret_cast<const uint2*>(D + roff[w][u]);
+            for (int u = 0; u < PF; ++u) dn[u] = *gpv<const u32x2>(D + sh.roff[w][u]);
         }
-        for (int a = 0; a < nkp; a += PF) {
+        for (int k0 = 0; k0 < nkp; k0 += PF) {
 #pragma unroll
             for (int u = 0; u < PF; ++u) dv[u] = dn[u];
-            if (a + PF < nkp) {
+            if (k0 + PF < nkp) {
 #pragma unroll
-                for (int u = 0; u < PF; ++u) dn[u] = *reinterpret_cast<const uint2*>(D + roff[w][a + PF + u]);
+                for (int u = 0; u < PF; ++u)
+                    dn[u] = *gpv<const u32x2>(D + sh.roff[w][k0 + PF + u]);
             }
 #pragma unroll
             for (int u = 0; u < PF; ++u) {
-                const uint32_t* xr = xs[w][a + u];
+                const auto xr = sh.xs[w][k0 + u];
                 // all saturating adds first, then the mins: independent
                 // instructions back to back instead of add->min chains
                 uint32_t t0[M], t1[M];
@@ -299,8 +298,7 @@ __global__ __launch_bounds__(256) void k_q_z(
     // work counters: the wavefronts' figures meet in LDS and wavefront 0 adds
     // them to one of kWorkSlots counter pairs (one same-address atomic per
     // wavefront would serialise in the L2 and double the kernel's time)
-    __shared__ unsigned long long wsum[NW][2];
-    if (wstat && tid == 0) {
+    if (a.wstat && tid == 0) {
         const int nact = min(64, nlive - by * 64);
         const unsigned long long cols4 = 4ull * nact;
         unsigned long long bytes = first_item ? (unsigned long long)rows_done * cols4 * 2 : 0ull;
@@ -308,35 +306,40 @@ __global__ __launch_bounds__(256) void k_q_z(
             if (by == 0) bytes += (unsigned long long)m * (a_hi - a_lo) * 2;
             bytes += (unsigned long long)m * cols4 * 2;
         }
-        wsum[w][0] = bytes;
-        wsum[w][1] = (unsigned long long)rows_done * cols4 * m;
+        sh.wsum[w][0] = bytes;
+        sh.wsum[w][1] = (unsigned long long)rows_done * cols4 * m;
     }
     // minimum over the wavefronts, in wavefront 0
     if (w > 0) {
 #pragma unroll
-        for (int v = 0; v < M; ++v) red[w - 1][v][tid] = make_uint2(acc[v][0], acc[v][1]);
+        for (int v = 0; v < M; ++v) sh.red[w - 1][v][tid] = u32x2{acc[v][0], acc[v][1]};
     }
     __syncthreads();
     if (w > 0) return;
-    if (wstat && tid == 0) {
+    if (a.wstat && tid == 0) {
         const unsigned slot = (unsigned(bx) * 131u + unsigned(by) * 17u + unsigned(bz) * 7u) % unsigned(kWorkSlots);
-        atomicAdd(&wstat[2 * slot], wsum[0][0] + wsum[1][0] + wsum[2][0] + wsum[3][0]);
-        atomicAdd(&wstat[2 * slot + 1], wsum[0][1] + wsum[1][1] + wsum[2][1] + wsum[3][1]);
+        const auto ws = gp(a.wstat);
+        unsigned long long by_sum = 0, up_sum = 0;
+#pragma unroll
+        for (int j = 0; j < NW; ++j) by_sum += sh.wsum[j][0], up_sum += sh.wsum[j][1];
+        __hip_atomic_fetch_add(ws + 2 * slot, by_sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(ws + 2 * slot + 1, up_sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 #pragma unroll
     for (int j = 0; j < NW - 1; ++j)
 #pragma unroll
         for (int v = 0; v < M; ++v) {
-            const uint2 o = red[j][v][tid];
+            const u32x2 o = sh.red[j][v][tid];
             acc[v][0] = qmin2(acc[v][0], o.x);
             acc[v][1] = qmin2(acc[v][1], o.y);
         }
     if (!active) return;
     if (ksplit > 1) {
-        uint16_t* P = Zp + size_t(bz) * part_stride;
+        const auto P = gp(a.Zp) + size_t(bz) * a.part_stride;
 #pragma unroll
         for (int v = 0; v < M; ++v)
-            if (v < m) *reinterpret_cast<uint2*>(P + size_t(s0 + v) * ldp + c0) = make_uint2(acc[v][0], acc[v][1]);
+            if (v < m)
+                *gpv<u32x2>(P + size_t(s0 + v) * a.ldp + c0) = u32x2{acc[v][0], acc[v][1]};
         return;
     }
 #pragma unroll
@@ -347,138 +350,91 @@ __global__ __launch_bounds__(256) void k_q_z(
                                  uint16_t(acc[v][1] & 0xffffu), uint16_t(acc[v][1] >> 16)};
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            if (pos[k] >= 0) Z[sl * ldz + pos[k]] = val[k];
-            if (ext[k] >= 0) Xn[sl * ne + ext[k]] = ownx[v][k] < val[k] ? ownx[v][k] : val[k];
+            if (pos[k] >= 0) gp(a.Z)[sl * a.ldz + pos[k]] = val[k];
+            if (ext[k] >= 0) gp(a.Xn)[sl * a.ne + ext[k]] = ownx[v][k] < val[k] ? ownx[v][k] : val[k];
         }
     }
 }
 
-bool launch_q_z(int M, int ld, int pc, const uint16_t* DU, size_t node_stride, const int* seam_list,
-                const int* cols, int ngrp, const int* live_list, int col_tiles, int ldp, const int4* items,
-                int n_items, int ksplit, const uint16_t* X, uint16_t* Zp,
-                size_t part_stride, const int* colpos, const int* colext, const int* ext_child,
-                const int* ext_cport, uint16_t* Z, int ldz, uint16_t* Xn, int ne, unsigned long long* wstat,
-                hipStream_t st, KTimer kt) {
-    if (n_items == 0 || col_tiles == 0) return false;
-    const dim3 grid{unsigned(n_items), unsigned(col_tiles), unsigned(ksplit)};
-#define HTG_QZ(MM)                                                                                                  \
-    HTG_LAUNCH_T((k_q_z<MM, 8>), grid, dim3(256), 0, st, kt, ld, pc, DU, node_stride, seam_list, cols, ngrp,        \
-                 live_list, ldp, items, ksplit, X, Zp, part_stride, colpos, colext, ext_child, ext_cport, Z, ldz,   \
-                 Xn, ne, wstat)
-    if (M <= 1) HTG_QZ(1);
-    else if (M <= 2) HTG_QZ(2);
-    else if (M <= 4) HTG_QZ(4);
-    else if (M <= 8) HTG_QZ(8);
-    else HTG_QZ(16);
-#undef HTG_QZ
-    return true;
-}
-
-// Fold of the row chunks, one launch with two index ranges per slot:
-//  pos < nz: Z[slot][pos] = min_c Zp[c][slot][src[pos]], the layout of
+// Fold of the row chunks, unit blk of two index ranges per slot:
+//  pos < nz: Z[slot][pos] = min_c Zp[c][slot][zsrc[pos]], the layout of
 //     contiguous per-child seam blocks then the other exterior ports; INF
-//     where src[pos] < 0 (block padding) or where the column's group is not
-//     live in the slot's node (no passable port: k_q_z skipped it);
+//     where zsrc[pos] < 0 (block padding) or where the column's group is not
+//     live in the slot's node (no passable port: the projection skipped it);
 //  pos = nz + e (below the root): lift to W's exterior port e
-//     (src/minplus_oracle.cpp:1113-1154), Xnext[slot][e] = min(folded column
+//     (src/minplus_oracle.cpp:1113-1154), Xn[slot][e] = min(folded column
 //     ext_col[e], X[slot][own port of e] when e is a port of the slot's child).
-__device__ __forceinline__ uint16_t fold_col(int col, int s, int W, int ngrp, const uint8_t* __restrict__ live_map,
-                                             const uint16_t* __restrict__ Zp, int ldp, int ksplit,
-                                             size_t part_stride) {
+__device__ __forceinline__ uint16_t fold_col(const QZArgs& a, int col, int s, int W) {
     uint16_t v = kInf;
-    if (col >= 0 && live_map[size_t(W) * ngrp + col / 4]) {
-        const uint16_t* p = Zp + size_t(s) * ldp + col;
-        for (int c = 0; c < ksplit; ++c) {
-            const uint16_t x = p[size_t(c) * part_stride];
+    if (col >= 0 && gp(a.live_map)[size_t(W) * a.ngrp + col / 4]) {
+        const auto p = gp(a.Zp) + size_t(s) * a.ldp + col;
+        for (int c = 0; c < a.ksplit; ++c) {
+            const uint16_t x = p[size_t(c) * a.part_stride];
             v = x < v ? x : v;
         }
     }
     return v;
 }
 
-__global__ __launch_bounds__(256) void k_q_fold(int n, int nz, int ldz, const int* __restrict__ src, int ldp,
-                                                int ksplit, const uint16_t* __restrict__ Zp, size_t part_stride,
-                                                const int2* __restrict__ slot_rc, int side, int nt, int ngrp,
-                                                const uint8_t* __restrict__ live_map, uint16_t* __restrict__ Z, int ne,
-                                                const int* __restrict__ ext_col, const int* __restrict__ ext_child,
-                                                const int* __restrict__ ext_cport, int pc, int cs, int g,
-                                                const uint16_t* __restrict__ Xprev, uint16_t* __restrict__ Xnext) {
-    const int per = nz + ne;
-    const long long gid = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (gid >= (long long)n * per) return;
+__device__ __forceinline__ void fold_unit(const QZArgs& a, int blk) {
+    const int per = a.nz + a.ne;
+    const long long gid = (long long)blk * 256 + threadIdx.x;
+    if (gid >= (long long)a.n_slots * per) return;
     const int s = int(gid / per), pos = int(gid % per);
-    const int2 rc = slot_rc[s];
-    const int W = (rc.x / side) * nt + rc.y / side;
-    if (pos < nz) {
-        Z[size_t(s) * ldz + pos] = fold_col(src[pos], s, W, ngrp, live_map, Zp, ldp, ksplit, part_stride);
+    const i32x2 rc = gpv<const i32x2>(a.slot_rc)[s];
+    const int W = (rc.x / a.side) * a.nt + rc.y / a.side;
+    if (pos < a.nz) {
+        gp(a.Z)[size_t(s) * a.ldz + pos] = fold_col(a, gp(a.zsrc)[pos], s, W);
         return;
     }
-    const int e = pos - nz;
-    uint16_t v = fold_col(ext_col[e], s, W, ngrp, live_map, Zp, ldp, ksplit, part_stride);
-    const int q = ((rc.x / cs) % g) * g + (rc.y / cs) % g;
-    if (ext_child[e] == q) {
-        const uint16_t d = Xprev[size_t(s) * pc + ext_cport[e]];
+    const int e = pos - a.nz;
+    uint16_t v = fold_col(a, gp(a.ext_col)[e], s, W);
+    const int q = ((rc.x / a.cs) % a.g) * a.g + (rc.y / a.cs) % a.g;
+    if (gp(a.ext_child)[e] == q) {
+        const uint16_t d = gp(a.X)[size_t(s) * a.pc + gp(a.ext_cport)[e]];
         v = d < v ? d : v;
     }
-    Xnext[size_t(s) * ne + e] = v;
-}
-
-bool launch_q_fold(int n, int nz, int ldz, const int* src, int ldp, int ksplit, const uint16_t* Zp,
-                   size_t part_stride, const int2* slot_rc, int side, int nt, int ngrp, const uint8_t* live_map,
-                   uint16_t* Z, int ne, const int* ext_col, const int* ext_child, const int* ext_cport, int pc, int cs,
-                   int g, const uint16_t* Xprev, uint16_t* Xnext, hipStream_t st, KTimer kt) {
-    if (n == 0) return false;
-    const long long total = (long long)n * (nz + ne);
-    HTG_LAUNCH_T(k_q_fold, dim3(unsigned((total + 255) / 256)), dim3(256), 0, st, kt, n, nz, ldz,
-                          src, ldp, ksplit, Zp, part_stride, slot_rc, side, nt, ngrp, live_map, Z, ne, ext_col,
-                          ext_child, ext_cport, pc, cs, g, Xprev, Xnext);
-    return true;
+    gp(a.Xn)[size_t(s) * a.ne + e] = v;
 }
 
 // Meet: m_ij = min_{b in seam(q_j)} Z_i[b] (+) Z_j[b].  Every candidate is the
 // length of a real walk (i -> b -> j inside W), and Z_j[b] <= X_j[b] on q_j's
 // seam ports (D_W[b][b] = 0), so the candidate is never above the
 // reference's X_i-projection (+) X_j meet: the minimum over levels is exact.
-// One launch serves every level: Z of each level stays in its own region.
-// One 256-thread workgroup computes an 8 x 8 tile of (i, j) pairs of one
-// (query, W, q2) group, 2 x 2 pairs per thread with 16 lanes per 2 x 2
-// block on interleaved columns, streaming the seam block through LDS as
-// packed pairs.
-// Tiles are 8 ints (group, q2, i0, j0, j_end, level); zgeo holds per level
-// (Z offset lo, hi, ldz, 0, zoff[g2], zlen[g2]).  A pair meets at several
-// levels, so the result is merged with a 16-bit atomic minimum.
-__device__ __forceinline__ void atomic_min_u16(uint16_t* o, uint16_t v) {
+// Z of each level stays in its own region.  One 256-thread unit computes an
+// 8 x 8 tile of (i, j) pairs of one (query, W, q2) group, 2 x 2 pairs per
+// thread with 16 lanes per 2 x 2 block on interleaved columns, streaming the
+// seam block through LDS as packed pairs.  A pair meets at several levels,
+// so the result is merged with a 16-bit atomic minimum.
+__device__ __forceinline__ void atomic_min_u16(HTG_G(uint16_t) o, uint16_t v) {
     const uintptr_t a = reinterpret_cast<uintptr_t>(o);
-    unsigned* w = reinterpret_cast<unsigned*>(a & ~uintptr_t(3));
+    HTG_G(unsigned) const w = (HTG_G(unsigned))(a & ~uintptr_t(3));
     const int sh = (a & 2) ? 16 : 0;
     unsigned old = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (;;) {
         if (((old >> sh) & 0xffffu) <= v) return;
         const unsigned nw = (old & ~(0xffffu << sh)) | (unsigned(v) << sh);
-        const unsigned prev = atomicCAS(w, old, nw);
-        if (prev == old) return;
-        old = prev;
+        if (__hip_atomic_compare_exchange_strong(w, &old, nw, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT))
+            return;
     }
 }
 
-__global__ __launch_bounds__(256) void k_q_meet(const int* __restrict__ tiles, const int* __restrict__ glist,
-                                                const int* __restrict__ zgeo, int g2, const int* __restrict__ slot_i,
-                                                const int* __restrict__ q_k, const long long* __restrict__ q_out,
-                                                const uint16_t* __restrict__ Zall, uint16_t* __restrict__ out) {
-    constexpr int CH = 256;      // packed pairs per round (512 columns)
-    constexpr int LDR = CH + 8;  // row stride: rows two apart fall 16 banks apart
-    __shared__ uint32_t A[8][LDR];
-    __shared__ uint32_t Bm[8][LDR];
+constexpr int kMeetCh = 256;            // packed pairs per round (512 columns)
+constexpr int kMeetLd = kMeetCh + 8;    // row stride: rows two apart fall 16 banks apart
+struct MeetLds {
+    uint32_t A[8][kMeetLd];
+    uint32_t B[8][kMeetLd];
+};
+
+__device__ __forceinline__ void meet_unit(const QMeetArgs& a, unsigned bt, MeetLds& sh) {
     // tile: group size, target child q2, first rows i0 / j0, j end, level,
     // the group's offset in glist, its query
-    // logical tile: tiles of one group and target child share Z rows, so runs
-    // of consecutive tiles go to one XCD
-    const unsigned bt = xcd_logical<kXcdRun>(blockIdx.x, gridDim.x);
-    const int4 t0 = *reinterpret_cast<const int4*>(tiles + size_t(bt) * 8);
-    const int4 t1 = *reinterpret_cast<const int4*>(tiles + size_t(bt) * 8 + 4);
+    const auto tl = gpv<const i32x4>(gp(a.tiles) + size_t(bt) * 8);
+    const i32x4 t0 = tl[0], t1 = tl[1];
     const int n = t0.x, q2 = t0.y, i0 = t0.z, j0 = t0.w, jend = t1.x, lvl = t1.y, Q = t1.w;
-    const int* lst = glist + t1.z;
-    const int* zg = zgeo + size_t(lvl) * (4 + 2 * g2);
+    const auto lst = gp(a.glist) + t1.z;
+    const auto zg = gp(a.zgeo) + size_t(lvl) * (4 + 2 * a.g2);
     const int tid = threadIdx.x;
     // 2 x 2 pairs per thread: block (bi, bj) = rows 2bi, 2bi+1 of the tile's
     // i side against rows 2bj, 2bj+1 of its j side, 16 lanes per block on
@@ -494,34 +450,36 @@ __global__ __launch_bounds__(256) void k_q_meet(const int* __restrict__ tiles, c
     if (writer) {
         si = lst[i0 + pi];
         sj = lst[j0 + pj];
-        oaddr = size_t(q_out[Q]) + size_t(slot_i[si]) * q_k[Q] + slot_i[sj];
+        const auto slot_i = gp(a.slot_i);
+        oaddr = size_t(gp(a.q_out)[Q]) + size_t(slot_i[si]) * gp(a.q_k)[Q] + slot_i[sj];
     }
-    const uint16_t* Z = Zall + ((size_t(unsigned(zg[1])) << 32) | unsigned(zg[0]));
+    const auto Z = gp(a.Z) + ((size_t(unsigned(zg[1])) << 32) | unsigned(zg[0]));
     const int ldz = zg[2];
-    const int off = zg[4 + q2], len2 = zg[4 + g2 + q2] / 2;  // block length is even
+    const int off = zg[4 + q2], len2 = zg[4 + a.g2 + q2] / 2;  // block length is even
     // staging: thread (row, l16) loads columns l16, l16 + 16, ... of one of
     // the 16 rows (8 of A, then 8 of B)
     const int row = tid >> 4;
     const bool is_a = row < 8;
     const int rr = row & 7;
     const int nrow = is_a ? ni : nj;
-    const uint32_t* zr =
-        rr < nrow ? reinterpret_cast<const uint32_t*>(Z + size_t(lst[(is_a ? i0 : j0) + rr]) * ldz + off) : nullptr;
-    uint32_t* dst = is_a ? A[rr] : Bm[rr];
+    HTG_G(const uint32_t) const zr =
+        rr < nrow ? gpv<const uint32_t>(Z + size_t(lst[(is_a ? i0 : j0) + rr]) * ldz + off)
+                  : nullptr;
+    const auto dst = is_a ? sh.A[rr] : sh.B[rr];
     uint32_t a00 = kInf2, a01 = kInf2, a10 = kInf2, a11 = kInf2;
-    for (int c0 = 0; c0 < len2; c0 += CH) {
-        const int cn = min(CH, len2 - c0);
+    for (int c0 = 0; c0 < len2; c0 += kMeetCh) {
+        const int cn = min(kMeetCh, len2 - c0);
         __syncthreads();
 #pragma unroll
-        for (int u = 0; u < CH / 16; ++u) {
+        for (int u = 0; u < kMeetCh / 16; ++u) {
             const int cc = l16 + 16 * u;
             dst[cc] = (zr && cc < cn) ? zr[c0 + cc] : kInf2;
         }
         __syncthreads();
-        const uint32_t* A0 = A[2 * bi];
-        const uint32_t* A1 = A[2 * bi + 1];
-        const uint32_t* B0 = Bm[2 * bj];
-        const uint32_t* B1 = Bm[2 * bj + 1];
+        const auto A0 = sh.A[2 * bi];
+        const auto A1 = sh.A[2 * bi + 1];
+        const auto B0 = sh.B[2 * bj];
+        const auto B1 = sh.B[2 * bj + 1];
 #pragma unroll 2
         for (int cc = l16; cc < cn; cc += 16) {
             const uint32_t x0 = A0[cc], x1 = A1[cc], y0 = B0[cc], y1 = B1[cc];
@@ -543,15 +501,61 @@ __global__ __launch_bounds__(256) void k_q_meet(const int* __restrict__ tiles, c
     const uint32_t acc = l16 == 0 ? a00 : l16 == 1 ? a01 : l16 == 2 ? a10 : a11;
     const uint16_t best = min(uint16_t(acc & 0xffffu), uint16_t(acc >> 16));
     if (best >= kInf) return;
-    atomic_min_u16(out + oaddr, best);
+    atomic_min_u16(gp(a.out) + oaddr, best);
 }
 
-bool launch_q_meet(const int* tiles, int n_tiles, const int* glist, const int* zgeo, int g2, const int* slot_i,
-                   const int* q_k, const long long* q_out, const uint16_t* Z, uint16_t* out, hipStream_t st,
-                   KTimer kt) {
-    if (n_tiles == 0) return false;
-    HTG_LAUNCH_T(k_q_meet, dim3(unsigned(n_tiles)), dim3(256), 0, st, kt, tiles, glist, zgeo, g2, slot_i, q_k,
-                 q_out, Z, out);
+// ---- one launch per phase ---------------------------------------------------
+__global__ __launch_bounds__(256) void k_q_front(QFrontArgs a) { front_unit(a, int(blockIdx.x)); }
+
+// Units in an XCD-aware order: items of one (W, q) read the same D rows, so
+// runs of consecutive items go to one XCD.
+template <int M, int PF>
+__global__ __launch_bounds__(256) void k_q_z(QZArgs a) {
+    __shared__ ZLds<M> sh;
+    const unsigned lin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    const unsigned L = xcd_logical<kXcdRun>(lin, gridDim.x * gridDim.y * gridDim.z);
+    z_unit<M, PF>(a, int(L % gridDim.x), int((L / gridDim.x) % gridDim.y), int(L / (gridDim.x * gridDim.y)), sh);
+}
+
+__global__ __launch_bounds__(256) void k_q_fold(QZArgs a) { fold_unit(a, int(blockIdx.x)); }
+
+// Tiles of one group and target child share Z rows: runs of consecutive
+// tiles go to one XCD.
+__global__ __launch_bounds__(256) void k_q_meet(QMeetArgs a) {
+    __shared__ MeetLds sh;
+    meet_unit(a, xcd_logical<kXcdRun>(blockIdx.x, gridDim.x), sh);
+}
+
+bool launch_q_front(const QFrontArgs& a, hipStream_t st) {
+    // the unit's fill range is found with one 64-lane ballot over the ranges
+    if (a.n_fill > 64 || a.n_fill < 0) return false;
+    const long long blocks = (long long)a.nq * a.ib + a.leaf_blocks + a.fill_blocks;
+    if (blocks == 0) return true;
+    hipLaunchKernelGGL(k_q_front, dim3(unsigned(blocks)), dim3(256), 0, st, a);
+    return true;
+}
+
+bool launch_q_z(const QZArgs& a, hipStream_t st, KTimer kt) {
+    if (a.n_items == 0 || a.col_tiles == 0) return false;
+    const dim3 grid{unsigned(a.n_items), unsigned(a.col_tiles), unsigned(a.ksplit)};
+    if (a.M <= 1) HTG_LAUNCH_T((k_q_z<1, 8>), grid, dim3(256), 0, st, kt, a);
+    else if (a.M <= 2) HTG_LAUNCH_T((k_q_z<2, 8>), grid, dim3(256), 0, st, kt, a);
+    else if (a.M <= 4) HTG_LAUNCH_T((k_q_z<4, 8>), grid, dim3(256), 0, st, kt, a);
+    else if (a.M <= 8) HTG_LAUNCH_T((k_q_z<8, 8>), grid, dim3(256), 0, st, kt, a);
+    else HTG_LAUNCH_T((k_q_z<16, 8>), grid, dim3(256), 0, st, kt, a);
+    return true;
+}
+
+bool launch_q_fold(const QZArgs& a, hipStream_t st, KTimer kt) {
+    const long long total = (long long)a.n_slots * (a.nz + a.ne);
+    if (total == 0) return false;
+    HTG_LAUNCH_T(k_q_fold, dim3(unsigned((total + 255) / 256)), dim3(256), 0, st, kt, a);
+    return true;
+}
+
+bool launch_q_meet(const QMeetArgs& a, hipStream_t st, KTimer kt) {
+    if (a.n_tiles == 0) return false;
+    HTG_LAUNCH_T(k_q_meet, dim3(unsigned(a.n_tiles)), dim3(256), 0, st, kt, a);
     return true;
 }
 

@@ -1972,57 +1972,116 @@ int Map::plan_batch(const int32_t* rcs, const int32_t* ks, int nq, QueryPlan& P,
     return HTG_OK;
 }
 
-// The launches of one planned batch on stream st: front (output init,
-// level-0 lift, INF fills), per level the projection (and the fold of its
-// row chunks), then the meet of every level.  dp holds B.pk's packed plan;
-// Qb, Zp and dout have B's sizes.
-int Map::enqueue_batch(const QueryPlan& P, const BatchLayout& B, const char* dp, uint16_t* Qb, uint16_t* Zp,
-                       uint16_t* dout, unsigned long long* wstat, bool counted, hipStream_t st) {
+// Device addresses of a planned batch into the kernels' argument blocks.
+void Map::bind_batch(QueryPlan& P, const BatchLayout& B, const char* dp, uint16_t* Qb, uint16_t* Zp,
+                     uint16_t* dout, unsigned long long* wstat) const {
     auto I = [&](size_t o) { return reinterpret_cast<const int*>(dp + o); };
-    const uint16_t* dleafD = static_cast<const uint16_t*>(leafD.p);
     const int nq = int(P.q_k.size());
-    const int g2 = g * g;
-    const int2* srcs = reinterpret_cast<const int2*>(dp + B.o_src);
-    if (!launch_q_front(nq, I(B.o_qk), reinterpret_cast<const long long*>(dp + B.o_qout), I(B.o_qvp),
-                        reinterpret_cast<const int2*>(dp + B.o_vprc), reinterpret_cast<const uint8_t*>(dp + B.o_vpvalid),
-                        B.kmax, H > 0 ? B.ns : 0, srcs, b, nl, dleafD, dout, Qb + B.xoff[0], int(P.fills.size()),
-                        reinterpret_cast<const int4*>(dp + B.o_fills), B.fill_blocks, Qb, wstat, st))
-        return fail(HTG_E_INVALID, "query front: more than 64 fill ranges");
+    const int ns = H > 0 ? B.ns : 0;
+    QFrontArgs& F = P.front;
+    F = QFrontArgs{};
+    F.q_k = I(B.o_qk);
+    F.q_out = reinterpret_cast<const long long*>(dp + B.o_qout);
+    F.q_vp = I(B.o_qvp);
+    F.vp_rc = reinterpret_cast<const int2*>(dp + B.o_vprc);
+    F.vp_valid = reinterpret_cast<const uint8_t*>(dp + B.o_vpvalid);
+    F.slot_rc = reinterpret_cast<const int2*>(dp + B.o_src);
+    F.leafD = static_cast<const uint16_t*>(leafD.p);
+    F.out = dout;
+    F.X0 = Qb + B.xoff[0];
+    F.fills = reinterpret_cast<const int4*>(dp + B.o_fills);
+    F.fill_base = Qb;
+    F.wstat = wstat;
+    F.nq = nq;
+    F.ib = int(((long long)B.kmax * B.kmax + 255) / 256);
+    F.leaf_blocks = int(((long long)ns * (4 * b - 4) + 255) / 256);
+    F.n_slots = ns;
+    F.b = b;
+    F.nl = nl;
+    F.n_fill = int(P.fills.size());
+    F.fill_blocks = B.fill_blocks;
+    P.zargs.resize(size_t(H));
+    for (int l = 1; l <= H; ++l) {
+        const LevelGeo& L = lv[size_t(l) - 1];
+        const LevelPlan& LP = P.levels[size_t(l) - 1];
+        QZArgs& Z = P.zargs[size_t(l) - 1];
+        Z = QZArgs{};
+        Z.DU = static_cast<const uint16_t*>(L.DU.p);
+        Z.seam_list = static_cast<const int*>(L.d_seam_list.p);
+        Z.cols = static_cast<const int*>(L.d_zcols.p);
+        Z.live_list = L.d_live_list;
+        Z.items = reinterpret_cast<const int4*>(dp + B.o_items[size_t(l) - 1]);
+        Z.X = Qb + B.xoff[size_t(l) - 1];                // X of this level (stride pc)
+        Z.Zp = Zp;
+        Z.colpos = static_cast<const int*>(L.d_colpos.p);
+        Z.colext = static_cast<const int*>(L.d_colext.p);
+        Z.ext_child = static_cast<const int*>(L.d_ext_child.p);
+        Z.ext_cport = static_cast<const int*>(L.d_ext_cport.p);
+        Z.Z = Qb + B.zoffq[size_t(l)];
+        Z.Xn = l < H ? Qb + B.xoff[size_t(l)] : nullptr;  // lifted X (stride ne)
+        Z.wstat = wstat;
+        Z.zsrc = static_cast<const int*>(L.d_zsrc.p);
+        Z.slot_rc = F.slot_rc;
+        Z.live_map = L.d_live_map;
+        Z.ext_col = static_cast<const int*>(L.d_ext_col.p);
+        Z.node_stride = L.node_stride;
+        Z.part_stride = B.nsl * L.ldp;
+        Z.ld = L.d.ld;
+        Z.pc = L.d.pc;
+        Z.ngrp = L.ngrp;
+        Z.ldp = L.ldp;
+        Z.n_items = int(LP.items.size() / 2);
+        Z.col_tiles = LP.col_tiles;
+        Z.ksplit = LP.ksplit;
+        Z.ldz = L.ldz;
+        Z.ne = l < H ? L.d.ne : 0;  // no lift above the root
+        Z.M = LP.M;
+        Z.n_slots = ns;
+        Z.nz = L.nz;
+        Z.side = L.d.s;
+        Z.nt = L.d.nt;
+        Z.cs = L.d.cs;
+        Z.g = g;
+    }
+    QMeetArgs& Mt = P.meet;
+    Mt = QMeetArgs{};
+    Mt.tiles = I(B.o_tiles);
+    Mt.glist = I(B.o_glist);
+    Mt.zgeo = I(B.o_zgeo);
+    Mt.slot_i = I(B.o_si);
+    Mt.q_k = F.q_k;
+    Mt.q_out = F.q_out;
+    Mt.Z = Qb;
+    Mt.out = dout;
+    Mt.n_tiles = ns > 0 ? int(P.mtiles.size() / 8) : 0;
+    Mt.g2 = g * g;
+}
+
+// The launches of one bound batch: front (output init, level-0 lift, INF
+// fills), per level the projection (and the fold of its row chunks), then
+// the meet of every level.
+int Map::enqueue_batch(const QueryPlan& P, const BatchLayout& B, bool counted, hipStream_t st) {
+    if (!launch_q_front(P.front, st)) return fail(HTG_E_INVALID, "query front: more than 64 fill ranges");
     for (int l = 1; l <= H && B.ns > 0; ++l) {
         const LevelGeo& L = lv[l - 1];
-        const LevelPlan& LP = P.levels[l - 1];
-        const uint16_t* Xl = Qb + B.xoff[size_t(l) - 1];            // X of this level (stride pc)
-        uint16_t* Xn = l < H ? Qb + B.xoff[size_t(l)] : nullptr;  // lifted X (stride ne)
-        uint16_t* Zl = Qb + B.zoffq[size_t(l)];
-        const int ne = l < H ? L.d.ne : 0;  // no lift above the root
+        const QZArgs& Z = P.zargs[size_t(l) - 1];
         // bytes and updates of the projection are counted on the device
         KTimer kt;
         const int pi = counted ? prof_ref().open(HTG_K_Q_PROJECT, 0.0, 0.0, kt, st) : -1;
-        const size_t part_stride = B.nsl * L.ldp;
-        const bool la = launch_q_z(LP.M, L.d.ld, L.d.pc, static_cast<const uint16_t*>(L.DU.p), L.node_stride,
-                   static_cast<const int*>(L.d_seam_list.p), static_cast<const int*>(L.d_zcols.p), L.ngrp,
-                   L.d_live_list, LP.col_tiles, L.ldp,
-                   reinterpret_cast<const int4*>(dp + B.o_items[size_t(l) - 1]), int(LP.items.size() / 2), LP.ksplit, Xl,
-                   Zp, part_stride, static_cast<const int*>(L.d_colpos.p),
-                   static_cast<const int*>(L.d_colext.p), static_cast<const int*>(L.d_ext_child.p),
-                   static_cast<const int*>(L.d_ext_cport.p), Zl, L.ldz, Xn, ne, wstat, st, kt);
+        const bool la = launch_q_z(Z, st, kt);
         prof_ref().close(pi, la);
-        if (LP.ksplit > 1) {
-            const int pf =
-                counted ? prof_ref().open(HTG_K_Q_LIFT, double(B.ns) * (L.nz + 2.0 * ne + L.d.pc) * 2, 0, kt, st) : -1;
-            const bool lf = launch_q_fold(B.ns, L.nz, L.ldz, static_cast<const int*>(L.d_zsrc.p), L.ldp, LP.ksplit,
-                          Zp, part_stride, srcs, L.d.s, L.d.nt, L.ngrp,
-                          L.d_live_map, Zl, ne, static_cast<const int*>(L.d_ext_col.p),
-                          static_cast<const int*>(L.d_ext_child.p), static_cast<const int*>(L.d_ext_cport.p), L.d.pc,
-                          L.d.cs, g, Xl, Xn, st, kt);
+        if (Z.ksplit > 1) {
+            const int pf = counted ? prof_ref().open(HTG_K_Q_LIFT, double(B.ns) * (L.nz + 2.0 * Z.ne + L.d.pc) * 2,
+                                                     0, kt, st)
+                                   : -1;
+            const bool lf = launch_q_fold(Z, st, kt);
             prof_ref().close(pf, lf);
         }
     }
-    if (B.ns > 0 && !P.mtiles.empty()) {
+    if (B.ns > 0 && P.meet.n_tiles > 0) {
         KTimer kt;
         const int pm = counted ? prof_ref().open(HTG_K_Q_MEET, P.m_bytes, P.m_updates, kt, st) : -1;
-        const bool lm = launch_q_meet(I(B.o_tiles), int(P.mtiles.size() / 8), I(B.o_glist), I(B.o_zgeo), g2, I(B.o_si), I(B.o_qk),
-                                      reinterpret_cast<const long long*>(dp + B.o_qout), Qb, dout, st, kt);
+        const bool lm = launch_q_meet(P.meet, st, kt);
         prof_ref().close(pm, lm);
     }
     hipError_t e;
@@ -2045,35 +2104,34 @@ int Map::query_launch(const int32_t* rcs, const int32_t* ks, int nq, QueryWS& ws
         return e0 == hipSuccess ? HTG_OK : hip_fail(e0, "record");
     }
     const auto th1 = std::chrono::steady_clock::now();
-    // ---- upload (one packed staging buffer)
+    // ---- buffers, device addresses, upload (one packed staging buffer)
     const Packer& pk = B.pk;
-    if ((rc = ws.h_plan.ensure(pk.size))) return rc;
-    pk.write(static_cast<char*>(ws.h_plan.p));
-    if ((rc = ws.d_plan.ensure(pk.size)) || (rc = ws.Q.ensure(B.q_bytes())) || (rc = ws.Zp.ensure(B.zp_bytes())) ||
-        (rc = ws.d_out.ensure(B.out_bytes())) || (rc = ws.h_out.ensure(size_t(B.total_out) * 2)))
+    const long long total_out = B.total_out;
+    if ((rc = ws.h_plan.ensure(pk.size)) || (rc = ws.d_plan.ensure(pk.size)) || (rc = ws.Q.ensure(B.q_bytes())) ||
+        (rc = ws.Zp.ensure(B.zp_bytes())) || (rc = ws.d_out.ensure(B.out_bytes())) ||
+        (rc = ws.h_out.ensure((size_t(total_out) * 2 + 15) / 16 * 16)))
         return rc;
-    if (fault_hit(HTG_FAULT_QUERY)) return fail(HTG_E_HIP, "injected fault (query launch)");
-    hipStream_t st = ws.stream;
-    static const bool kernel_copy = [] {
-        const char* e = std::getenv("HTG_KERNEL_COPY");
-        return !e || std::atoi(e) != 0;
-    }();
-    if (kernel_copy) {
-        launch_copy_bytes(ws.h_plan.p, ws.d_plan.p, pk.size, st);
-    } else if ((e = hipMemcpyAsync(ws.d_plan.p, ws.h_plan.p, pk.size, hipMemcpyHostToDevice, st)) != hipSuccess) {
-        return hip_fail(e, "upload plan");
-    }
-    const auto th2 = std::chrono::steady_clock::now();
     unsigned long long* wstat = nullptr;
     if (ws.counted) {
         if ((rc = ws.d_wstat.ensure(kWorkSlots * 16)) || (rc = ws.h_wstat.ensure(kWorkSlots * 16))) return rc;
         wstat = static_cast<unsigned long long*>(ws.d_wstat.p);
     }
-    if ((rc = enqueue_batch(P, B, static_cast<const char*>(ws.d_plan.p), static_cast<uint16_t*>(ws.Q.p),
-                            static_cast<uint16_t*>(ws.Zp.p), static_cast<uint16_t*>(ws.d_out.p), wstat, ws.counted,
-                            st)))
-        return rc;
-    const long long total_out = B.total_out;
+    hipStream_t st = ws.stream;
+    bind_batch(P, B, static_cast<const char*>(ws.d_plan.p), static_cast<uint16_t*>(ws.Q.p),
+               static_cast<uint16_t*>(ws.Zp.p), static_cast<uint16_t*>(ws.d_out.p), wstat);
+    pk.write(static_cast<char*>(ws.h_plan.p));
+    if (fault_hit(HTG_FAULT_QUERY)) return fail(HTG_E_HIP, "injected fault (query launch)");
+    static const bool kernel_copy = [] {
+        const char* e = std::getenv("HTG_KERNEL_COPY");
+        return !e || std::atoi(e) != 0;
+    }();
+    const auto th2 = std::chrono::steady_clock::now();
+    if (kernel_copy) {
+        launch_copy_bytes(ws.h_plan.p, ws.d_plan.p, pk.size, st);
+    } else if ((e = hipMemcpyAsync(ws.d_plan.p, ws.h_plan.p, pk.size, hipMemcpyHostToDevice, st)) != hipSuccess) {
+        return hip_fail(e, "upload plan");
+    }
+    if ((rc = enqueue_batch(P, B, ws.counted, st))) return rc;
     if (to_host) {
         if (kernel_copy)
             launch_copy_bytes(ws.d_out.p, ws.h_out.p, size_t(total_out) * 2, st);
@@ -2658,20 +2716,19 @@ int htg_query_batch_noalloc(htg_map* map, const int32_t* rc, const int32_t* ks, 
     if (need > kNoAllocScratchBytes)
         return fail(HTG_E_RANGE, "batch needs " + std::to_string(need) + " bytes of device scratch; "
                                  "htg_query_batch_noalloc holds " + std::to_string(kNoAllocScratchBytes));
+    char* base = static_cast<char*>(sl.ptr);
+    map->bind_batch(P, B, base + o_plan, reinterpret_cast<uint16_t*>(base + o_q),
+                    reinterpret_cast<uint16_t*>(base + o_zp), reinterpret_cast<uint16_t*>(base + o_out), nullptr);
     if (sl.host_plan.size() < B.pk.size) sl.host_plan.resize(B.pk.size);  // host staging, grow-only
     B.pk.write(sl.host_plan.data());
     if (map->fault_hit(HTG_FAULT_QUERY)) return fail(HTG_E_HIP, "injected fault (query launch)");
-    char* base = static_cast<char*>(sl.ptr);
     const hipStream_t st = map->stream;  // in order behind the map's updates
     if ((e = hipMemcpyAsync(base + o_plan, sl.host_plan.data(), B.pk.size, hipMemcpyHostToDevice, st)) != hipSuccess)
         return hip_fail(e, "upload plan");
     // Nothing here needs to start at zero: the front kernel writes every
     // result entry and every INF fill of the X / Z regions, the projection
     // writes every partial the fold reads.
-    if (int r = map->enqueue_batch(P, B, base + o_plan, reinterpret_cast<uint16_t*>(base + o_q),
-                                   reinterpret_cast<uint16_t*>(base + o_zp), reinterpret_cast<uint16_t*>(base + o_out),
-                                   nullptr, false, st))
-        return r;
+    if (int r = map->enqueue_batch(P, B, false, st)) return r;
     if ((e = hipMemcpyAsync(out, base + o_out, size_t(B.total_out) * 2, hipMemcpyDeviceToHost, st)) != hipSuccess ||
         (e = hipStreamSynchronize(st)) != hipSuccess)
         return hip_fail(e, "download result");

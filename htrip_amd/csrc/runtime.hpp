@@ -157,6 +157,10 @@ struct QueryPlan {
     std::vector<int> glist;    // slot indices
     std::vector<int> zgeo;     // per level: Z offset lo, hi, ldz, 0, zoff[g2], zlen[g2]
     std::vector<int4> fills;   // INF fills of Q: (start lo, start hi, n8, first block)
+    // the batch's kernel arguments (device addresses), from bind_batch
+    QFrontArgs front{};
+    std::vector<QZArgs> zargs;  // per level
+    QMeetArgs meet{};
     double m_bytes = 0, m_updates = 0;
     void clear() {
         q_k.clear();
@@ -193,7 +197,7 @@ struct BatchLayout {
     size_t q_bytes() const { return (qsz > 8 ? qsz : 8) * 2; }
     size_t zp_bytes() const { return nsl * (maxpart > 8 ? maxpart : 8) * 2; }
     // the meet's 16-bit atomics address whole 32-bit words
-    size_t out_bytes() const { return size_t(total_out) * 2 + 4; }
+    size_t out_bytes() const { return (size_t(total_out) * 2 + 4 + 15) / 16 * 16; }
 };
 
 // Per-stream query workspace (grow-only, like HTripQueryScratch,
@@ -382,8 +386,13 @@ struct Map {
     // their capacity), and the kernels of a planned batch on given buffers.
     int plan_batch(const int32_t* rc, const int32_t* ks, int nq, QueryPlan& P, BatchLayout& B, bool counting,
                    double* host_sub) const;
-    int enqueue_batch(const QueryPlan& P, const BatchLayout& B, const char* d_plan, uint16_t* Q, uint16_t* Zp,
-                      uint16_t* d_out, unsigned long long* wstat, bool counted, hipStream_t st);
+    // Device addresses of a planned batch (plan at d_plan, X / Z regions Q,
+    // row-chunk partials Zp, result d_out, counters wstat) into the kernels'
+    // argument blocks in P.
+    void bind_batch(QueryPlan& P, const BatchLayout& B, const char* d_plan, uint16_t* Q, uint16_t* Zp,
+                    uint16_t* d_out, unsigned long long* wstat) const;
+    // The kernels of a bound batch on stream st (plan already on the device).
+    int enqueue_batch(const QueryPlan& P, const BatchLayout& B, bool counted, hipStream_t st);
     int query_finish(QueryWS& ws, uint16_t* out_host);
     // Grow ws ahead of time for batches of up to max_vps viewpoints and
     // max_out result entries, so that serving does not allocate (a device

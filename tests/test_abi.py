@@ -84,23 +84,33 @@ def test_query_argument_checks_need_no_device():
     assert lib.htg_server_inject_fault(None, 1, 1, 1) == _lib.HTG_E_INVALID
 
 
+class _FrontArgs(C.Structure):
+    """QFrontArgs (htrip_amd/csrc/kernels.hpp): 12 pointers, then 8 ints."""
+    _fields_ = [(f"p{i}", C.c_void_p) for i in range(12)] + [
+        (n, C.c_int) for n in ("nq", "ib", "leaf_blocks", "n_slots", "b", "nl", "n_fill", "fill_blocks")]
+
+
+def _launcher(name):
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True).stdout
+    sym = [ln.split()[-1] for ln in out.splitlines() if name in ln]
+    assert len(sym) == 1
+    return getattr(C.CDLL(_lib.LIB_PATH), sym[0])
+
+
 def test_query_front_refuses_more_than_64_fill_ranges():
-    """The front kernel finds a block's fill range with one 64-lane ballot, so
-    its launcher refuses more ranges (returns false, launches nothing) rather
+    """The front finds a unit's fill range with one 64-lane ballot, so its
+    launcher refuses more ranges (returns false, launches nothing) rather
     than leaving gaps uninitialised.  Called with no device: the guard comes
     before any HIP call."""
-    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True).stdout
-    sym = [ln.split()[-1] for ln in out.splitlines() if "launch_q_front" in ln]
-    assert len(sym) == 1
-    lib = C.CDLL(_lib.LIB_PATH)
-    f = getattr(lib, sym[0])
+    f = _launcher("launch_q_front")
     f.restype = C.c_bool
-    vp, ci = C.c_void_p, C.c_int
-    # (nq, q_k, q_out, q_vp, vp_rc, vp_valid, kmax, n_slots, slot_rc, b, nl, leafD, out, X0, n_fill, fills,
-    #  fill_blocks, fill_base, wstat, stream)
-    f.argtypes = [ci, vp, vp, vp, vp, vp, ci, ci, vp, ci, ci, vp, vp, vp, ci, vp, ci, vp, vp, vp]
-    args = lambda n_fill, nq: [nq, None, None, None, None, None, 4, 0, None, 8, 1, None, None, None, n_fill, None,  # noqa
-                               0, None, None, None]
-    assert f(*args(65, 1)) is False
-    assert f(*args(-1, 1)) is False
-    assert f(*args(64, 0)) is True  # in range, nothing to launch
+    f.argtypes = [C.POINTER(_FrontArgs), C.c_void_p]
+
+    def args(n_fill, nq):
+        a = _FrontArgs()
+        a.nq, a.ib, a.b, a.nl, a.n_fill = nq, 1, 8, 1, n_fill
+        return C.byref(a)
+
+    assert f(args(65, 1), None) is False
+    assert f(args(-1, 1), None) is False
+    assert f(args(64, 0), None) is True  # in range, nothing to launch
