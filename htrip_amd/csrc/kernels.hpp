@@ -215,10 +215,11 @@ struct QZArgs {
     int ld, pc, ngrp, ldp, n_items, col_tiles, ksplit, ldz, ne, M;
     int n_slots, nz, side, nt, cs, g;
 };
-// Meet tiles (8 ints each: group size, q2, i0, j0, j_end, level, the group's
-// offset in glist, its query): out[i][j] = min(out, min over the q2 seam
-// block of Z_level[i] (+) Z_level[j]); zgeo per level: Z offset lo, hi, ldz,
-// 0, zoff[g2], zlen[g2].  One unit per tile.
+// Meet tiles (8 ints each: group size, q2, i0, j0, j_end, level | (16 x 16
+// tile) << 16, the group's offset in glist, its query): out[i][j] = min(out,
+// min over the q2 seam block of Z_level[i] (+) Z_level[j]); zgeo per level:
+// Z offset lo, hi, ldz, 0, zoff[g2], zlen[g2].  One unit per tile; a tile
+// covers 8 x 8 pairs, or 16 x 16 when its flag is set.
 struct QMeetArgs {
     const int* tiles;
     const int* glist;

@@ -358,7 +358,7 @@ __device__ __forceinline__ void z_unit(const QZArgs& a, int bx, int by, int bz, 
 
 // Fold of the row chunks, unit blk of two index ranges per slot:
 //  pos < nz: Z[slot][pos] = min_c Zp[c][slot][zsrc[pos]], the layout of
-//     contiguous per-child seam blocks then the other exterior ports; INF
+//     contiguous per-child seam blocks; INF
 //     where zsrc[pos] < 0 (block padding) or where the column's group is not
 //     live in the slot's node (no passable port: the projection skipped it);
 //  pos = nz + e (below the root): lift to W's exterior port e
@@ -401,11 +401,13 @@ __device__ __forceinline__ void fold_unit(const QZArgs& a, int blk) {
 // length of a real walk (i -> b -> j inside W), and Z_j[b] <= X_j[b] on q_j's
 // seam ports (D_W[b][b] = 0), so the candidate is never above the
 // reference's X_i-projection (+) X_j meet: the minimum over levels is exact.
-// Z of each level stays in its own region.  One 256-thread unit computes an
-// 8 x 8 tile of (i, j) pairs of one (query, W, q2) group, 2 x 2 pairs per
-// thread with 16 lanes per 2 x 2 block on interleaved columns, streaming the
-// seam block through LDS as packed pairs.  A pair meets at several levels,
-// so the result is merged with a 16-bit atomic minimum.
+// Z of each level stays in its own region.  One 256-thread unit computes a
+// tile of (i, j) pairs of one (query, W, q2) group, streaming the seam block
+// through LDS as packed pairs: 16 blocks of PB x PB pairs, 16 lanes per block
+// on interleaved columns, so that 2 PB LDS reads feed PB^2 packed
+// relaxations.  Groups of at least 16 viewpoints on both sides take 16 x 16
+// tiles (PB = 4), smaller ones 8 x 8 (PB = 2).  A pair meets at several
+// levels, so the result is merged with a 16-bit atomic minimum.
 __device__ __forceinline__ void atomic_min_u16(HTG_G(uint16_t) o, uint16_t v) {
     const uintptr_t a = reinterpret_cast<uintptr_t>(o);
     HTG_G(unsigned) const w = (HTG_G(unsigned))(a & ~uintptr_t(3));
@@ -420,31 +422,37 @@ __device__ __forceinline__ void atomic_min_u16(HTG_G(uint16_t) o, uint16_t v) {
     }
 }
 
-constexpr int kMeetCh = 256;            // packed pairs per round (512 columns)
-constexpr int kMeetLd = kMeetCh + 8;    // row stride: rows two apart fall 16 banks apart
+// Rows per tile side 4 PB; packed pairs per staging round CH (the LDS holds
+// 8 PB rows of CH + 16 / PB words: rows PB apart fall 16 banks apart).
+template <int PB>
+struct MeetGeo {
+    static constexpr int R = 4 * PB;
+    static constexpr int CH = 512 / PB;
+    static constexpr int LD = CH + 16 / PB;
+};
 struct MeetLds {
-    uint32_t A[8][kMeetLd];
-    uint32_t B[8][kMeetLd];
+    uint32_t w[2 * MeetGeo<2>::R * MeetGeo<2>::LD > 2 * MeetGeo<4>::R * MeetGeo<4>::LD
+                   ? 2 * MeetGeo<2>::R * MeetGeo<2>::LD
+                   : 2 * MeetGeo<4>::R * MeetGeo<4>::LD];
 };
 
-__device__ __forceinline__ void meet_unit(const QMeetArgs& a, unsigned bt, MeetLds& sh) {
-    // tile: group size, target child q2, first rows i0 / j0, j end, level,
-    // the group's offset in glist, its query
-    const auto tl = gpv<const i32x4>(gp(a.tiles) + size_t(bt) * 8);
-    const i32x4 t0 = tl[0], t1 = tl[1];
-    const int n = t0.x, q2 = t0.y, i0 = t0.z, j0 = t0.w, jend = t1.x, lvl = t1.y, Q = t1.w;
-    const auto lst = gp(a.glist) + t1.z;
+template <int PB>
+__device__ __forceinline__ void meet_tile(const QMeetArgs& a, int n, int q2, int i0, int j0, int jend, int lvl,
+                                          int goff, int Q, MeetLds& sh) {
+    constexpr int R = MeetGeo<PB>::R, CH = MeetGeo<PB>::CH, LD = MeetGeo<PB>::LD;
+    uint32_t* const A = sh.w;
+    uint32_t* const Bm = sh.w + R * LD;
+    const auto lst = gp(a.glist) + goff;
     const auto zg = gp(a.zgeo) + size_t(lvl) * (4 + 2 * a.g2);
     const int tid = threadIdx.x;
-    // 2 x 2 pairs per thread: block (bi, bj) = rows 2bi, 2bi+1 of the tile's
-    // i side against rows 2bj, 2bj+1 of its j side, 16 lanes per block on
-    // interleaved columns (four LDS reads feed four packed relaxations)
+    // block (bi, bj) = rows PB bi .. of the tile's i side against rows PB bj
+    // .. of its j side; lane l16 of the block takes columns l16, l16 + 16, ...
     const int l16 = tid & 15, blk = tid >> 4, bi = blk >> 2, bj = blk & 3;
-    const int ni = min(8, n - i0), nj = min(8, jend - j0);
+    const int ni = min(R, n - i0), nj = min(R, jend - j0);
     // the result's address, resolved now (its dependent loads overlap the row
-    // loads): lane l < 4 of a block writes pair (2bi + l / 2, 2bj + l % 2)
-    const int pi = 2 * bi + (l16 >> 1), pj = 2 * bj + (l16 & 1);
-    const bool writer = l16 < 4 && pi < ni && pj < nj;
+    // loads): lane l < PB^2 of a block writes pair (PB bi + l / PB, PB bj + l % PB)
+    const int pi = PB * bi + l16 / PB, pj = PB * bj + l16 % PB;
+    const bool writer = l16 < PB * PB && pi < ni && pj < nj;
     int si = 0, sj = 0;
     size_t oaddr = 0;
     if (writer) {
@@ -456,52 +464,91 @@ __device__ __forceinline__ void meet_unit(const QMeetArgs& a, unsigned bt, MeetL
     const auto Z = gp(a.Z) + ((size_t(unsigned(zg[1])) << 32) | unsigned(zg[0]));
     const int ldz = zg[2];
     const int off = zg[4 + q2], len2 = zg[4 + a.g2 + q2] / 2;  // block length is even
-    // staging: thread (row, l16) loads columns l16, l16 + 16, ... of one of
-    // the 16 rows (8 of A, then 8 of B)
-    const int row = tid >> 4;
-    const bool is_a = row < 8;
-    const int rr = row & 7;
-    const int nrow = is_a ? ni : nj;
-    HTG_G(const uint32_t) const zr =
-        rr < nrow ? gpv<const uint32_t>(Z + size_t(lst[(is_a ? i0 : j0) + rr]) * ldz + off)
-                  : nullptr;
-    const auto dst = is_a ? sh.A[rr] : sh.B[rr];
-    uint32_t a00 = kInf2, a01 = kInf2, a10 = kInf2, a11 = kInf2;
-    for (int c0 = 0; c0 < len2; c0 += kMeetCh) {
-        const int cn = min(kMeetCh, len2 - c0);
+    // staging: in pass p, thread (row, l16) loads columns l16, l16 + 16, ...
+    // of staging row 16 p + row (rows 0 .. R-1 are the i side, then the j side)
+    constexpr int PASSES = 2 * R / 16;
+    HTG_G(const uint32_t) zr[PASSES];
+    uint32_t* dst[PASSES];
+#pragma unroll
+    for (int p = 0; p < PASSES; ++p) {
+        const int row = 16 * p + (tid >> 4);
+        const bool is_a = row < R;
+        const int rr = is_a ? row : row - R;
+        zr[p] = rr < (is_a ? ni : nj)
+                    ? gpv<const uint32_t>(Z + size_t(lst[(is_a ? i0 : j0) + rr]) * ldz + off)
+                    : nullptr;
+        dst[p] = (is_a ? A : Bm) + rr * LD;
+    }
+    uint32_t acc[PB][PB];
+#pragma unroll
+    for (int x = 0; x < PB; ++x)
+#pragma unroll
+        for (int y = 0; y < PB; ++y) acc[x][y] = kInf2;
+    // the staging values of the next round are loaded into registers while
+    // this round's relaxations run
+    uint32_t nxt[PASSES][CH / 16];
+    auto load_round = [&](int c0) {
+        const int cn = min(CH, len2 - c0);
+#pragma unroll
+        for (int p = 0; p < PASSES; ++p)
+#pragma unroll
+            for (int u = 0; u < CH / 16; ++u) {
+                const int cc = l16 + 16 * u;
+                nxt[p][u] = (zr[p] && cc < cn) ? zr[p][c0 + cc] : kInf2;
+            }
+    };
+    if (len2 > 0) load_round(0);
+    for (int c0 = 0; c0 < len2; c0 += CH) {
+        const int cn = min(CH, len2 - c0);
         __syncthreads();
 #pragma unroll
-        for (int u = 0; u < kMeetCh / 16; ++u) {
-            const int cc = l16 + 16 * u;
-            dst[cc] = (zr && cc < cn) ? zr[c0 + cc] : kInf2;
-        }
+        for (int p = 0; p < PASSES; ++p)
+#pragma unroll
+            for (int u = 0; u < CH / 16; ++u) dst[p][l16 + 16 * u] = nxt[p][u];
         __syncthreads();
-        const auto A0 = sh.A[2 * bi];
-        const auto A1 = sh.A[2 * bi + 1];
-        const auto B0 = sh.B[2 * bj];
-        const auto B1 = sh.B[2 * bj + 1];
+        if (c0 + CH < len2) load_round(c0 + CH);
 #pragma unroll 2
         for (int cc = l16; cc < cn; cc += 16) {
-            const uint32_t x0 = A0[cc], x1 = A1[cc], y0 = B0[cc], y1 = B1[cc];
-            a00 = qrelax2(a00, x0, y0);
-            a01 = qrelax2(a01, x0, y1);
-            a10 = qrelax2(a10, x1, y0);
-            a11 = qrelax2(a11, x1, y1);
+            uint32_t x[PB], y[PB];
+#pragma unroll
+            for (int k = 0; k < PB; ++k) {
+                x[k] = A[(PB * bi + k) * LD + cc];
+                y[k] = Bm[(PB * bj + k) * LD + cc];
+            }
+#pragma unroll
+            for (int u = 0; u < PB; ++u)
+#pragma unroll
+                for (int v = 0; v < PB; ++v) acc[u][v] = qrelax2(acc[u][v], x[u], y[v]);
         }
     }
     // minimum over the block's 16 lanes (xor 1..8 stays inside the group)
 #pragma unroll
-    for (int m = 1; m < 16; m <<= 1) {
-        a00 = qmin2(a00, uint32_t(__shfl_xor(int(a00), m)));
-        a01 = qmin2(a01, uint32_t(__shfl_xor(int(a01), m)));
-        a10 = qmin2(a10, uint32_t(__shfl_xor(int(a10), m)));
-        a11 = qmin2(a11, uint32_t(__shfl_xor(int(a11), m)));
-    }
+    for (int m = 1; m < 16; m <<= 1)
+#pragma unroll
+        for (int u = 0; u < PB; ++u)
+#pragma unroll
+            for (int v = 0; v < PB; ++v) acc[u][v] = qmin2(acc[u][v], uint32_t(__shfl_xor(int(acc[u][v]), m)));
     if (!writer || si == sj) return;
-    const uint32_t acc = l16 == 0 ? a00 : l16 == 1 ? a01 : l16 == 2 ? a10 : a11;
-    const uint16_t best = min(uint16_t(acc & 0xffffu), uint16_t(acc >> 16));
+    uint32_t mine = kInf2;
+#pragma unroll
+    for (int u = 0; u < PB; ++u)
+#pragma unroll
+        for (int v = 0; v < PB; ++v)
+            if (l16 == u * PB + v) mine = acc[u][v];
+    const uint16_t best = min(uint16_t(mine & 0xffffu), uint16_t(mine >> 16));
     if (best >= kInf) return;
     atomic_min_u16(gp(a.out) + oaddr, best);
+}
+
+// Tile bt: group size, target child q2, first rows i0 / j0, j end, level (bit
+// 16: a 16 x 16 tile), the group's offset in glist, its query.
+__device__ __forceinline__ void meet_unit(const QMeetArgs& a, unsigned bt, MeetLds& sh) {
+    const auto tl = gpv<const i32x4>(gp(a.tiles) + size_t(bt) * 8);
+    const i32x4 t0 = tl[0], t1 = tl[1];
+    if ((t1.y >> 16) & 1)
+        meet_tile<4>(a, t0.x, t0.y, t0.z, t0.w, t1.x, t1.y & 0xffff, t1.z, t1.w, sh);
+    else
+        meet_tile<2>(a, t0.x, t0.y, t0.z, t0.w, t1.x, t1.y & 0xffff, t1.z, t1.w, sh);
 }
 
 // ---- one launch per phase ---------------------------------------------------

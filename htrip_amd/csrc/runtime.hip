@@ -271,22 +271,13 @@ static void make_level(LevelGeo& L, int lvl, int b, int g, int S, int H) {
         if (ns & 1) L.zsrc.push_back(-1);
         L.zlen[q] = int(L.zsrc.size()) - L.zoff[q];
     }
-    L.ext_zpos.assign(d.ne, 0);
-    if (!L.gather) {
-        for (int e = 0; e < d.ne; ++e) {
-            const int v = L.ext_idx[e];
-            if (zpos_of[v] < 0) {
-                zpos_of[v] = int(L.zsrc.size());
-                L.zsrc.push_back(natcol[v]);
-            }
-            L.ext_zpos[e] = zpos_of[v];
-        }
-    }
+    // Z holds the seam blocks only: the meet reads nothing else, and the
+    // lift to the exterior ports goes to X of the next level directly.
     L.nz = int(L.zsrc.size());
     L.ldz = round_up(std::max(L.nz, 8), 8);
     L.ext_col.assign(d.ne, -1);
     if (!L.gather)
-        for (int e = 0; e < d.ne; ++e) L.ext_col[e] = L.zsrc[size_t(L.ext_zpos[e])];
+        for (int e = 0; e < d.ne; ++e) L.ext_col[e] = natcol[size_t(L.ext_idx[e])];
     L.vu_ext.assign(d.nV, -1);
     for (int e = 0; e < d.ne; ++e) L.vu_ext[size_t(L.ext_idx[e])] = e;
     L.colpos.assign(L.ncols, -1);
@@ -1920,12 +1911,15 @@ int Map::plan_batch(const int32_t* rcs, const int32_t* ks, int nq, QueryPlan& P,
                     const int q2 = qdig(by_q[size_t(a0 + j0)]);
                     int je = j0 + 1;
                     while (je < n && qdig(by_q[size_t(a0 + je)]) == q2) ++je;
-                    for (int i0 = 0; i0 < n; i0 += 8)
-                        for (int jj = j0; jj < je; jj += 8) {
+                    // 16 x 16 tiles when both sides have 16 viewpoints, else 8 x 8
+                    const bool big = n >= 16 && je - j0 >= 16;
+                    const int T = big ? 16 : 8;
+                    for (int i0 = 0; i0 < n; i0 += T)
+                        for (int jj = j0; jj < je; jj += T) {
                             const size_t o = P.mtiles.size();
                             P.mtiles.resize(o + 8);
                             int* tt = P.mtiles.data() + o;
-                            tt[0] = n, tt[1] = q2, tt[2] = i0, tt[3] = jj, tt[4] = je, tt[5] = l - 1;
+                            tt[0] = n, tt[1] = q2, tt[2] = i0, tt[3] = jj, tt[4] = je, tt[5] = (l - 1) | (big << 16);
                             tt[6] = goff, tt[7] = q;
                         }
                     if (counting) {  // the meet's work figures, for kernel timing only
@@ -2382,7 +2376,7 @@ int map_create_on(htg_ctx* ctx, const htg_config* cfg, int32_t rows, int32_t col
             (rc = upload(L.d_seams, L.seams, st)) || (rc = upload(L.d_seam_list, L.seam_list, st)) ||
             (rc = upload(L.d_seam_off, L.seam_off, st)) || (rc = upload(L.d_zcols, L.zcols, st)) ||
             (rc = upload(L.d_zsrc, L.zsrc, st)) || (rc = upload(L.d_zoff, L.zoff, st)) || (rc = upload(L.d_zlen, L.zlen, st)) ||
-            (rc = upload(L.d_ext_zpos, L.ext_zpos, st)) || (rc = upload(L.d_ext_col, L.ext_col, st)) ||
+            (rc = upload(L.d_ext_col, L.ext_col, st)) ||
             (rc = upload(L.d_colpos, L.colpos, st)) || (rc = upload(L.d_colext, L.colext, st)) ||
             (rc = upload(L.d_vu_ext, L.vu_ext, st)))
             return rc;

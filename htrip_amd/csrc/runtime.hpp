@@ -92,9 +92,9 @@ struct LevelGeo {
     std::vector<int2> seams;
     std::vector<int> seam_list, seam_off;  // per-child seam ports
     // Query projection: natural columns (V_U indices) the projection computes,
-    // and the Z layout: per-child seam blocks (even-padded) then the other
-    // exterior ports.  zsrc[pos] = natural column of Z position pos or -1.
-    std::vector<int> zcols, zsrc, zoff, zlen, ext_zpos, ext_col;  // ext_col[e]: projection column of port e
+    // and the Z layout: per-child seam blocks (even-padded).  zsrc[pos] =
+    // projection column of Z position pos or -1.
+    std::vector<int> zcols, zsrc, zoff, zlen, ext_col;  // ext_col[e]: projection column of port e
     // per projection column: Z position (or -1) and exterior port index below
     // the root (or -1), for projections that fold in registers
     std::vector<int> colpos, colext;
@@ -125,7 +125,7 @@ struct LevelGeo {
     const int *d_all8_cnt = nullptr, *d_all8_list = nullptr;
     std::vector<int> seam_live;  // [W * g^2 + q]: passable seam ports of child q (rows a projection may use)
     DevBuf d_vu_rc, d_ext_idx, d_ext_child, d_ext_cport, d_seams, d_seam_list, d_seam_off, d_zcols, d_zsrc, d_zoff,
-        d_zlen, d_ext_zpos, d_ext_col, d_colpos, d_colext, d_vu_ext, d_all;  // d_all: 0..nn-1
+        d_zlen, d_ext_col, d_colpos, d_colext, d_vu_ext, d_all;  // d_all: 0..nn-1
     DevBuf DU, SU;
 };
 
@@ -153,7 +153,7 @@ struct QueryPlan {
     std::vector<int> slot_q, slot_i;
     std::vector<LevelPlan> levels;
     // meet, every level in one launch
-    std::vector<int> mtiles;   // 8 ints each: group size, q2, i0, j0, j_end, level, glist offset, query
+    std::vector<int> mtiles;   // 8 ints each: group size, q2, i0, j0, j_end, level | 16x16 << 16, glist offset, query
     std::vector<int> glist;    // slot indices
     std::vector<int> zgeo;     // per level: Z offset lo, hi, ldz, 0, zoff[g2], zlen[g2]
     std::vector<int4> fills;   // INF fills of Q: (start lo, start hi, n8, first block)
