@@ -64,14 +64,37 @@ __device__ __forceinline__ void leaf_bfs(const uint8_t* __restrict__ pass, int S
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int grp = lane / LANES, r = lane % LANES;
     const int slot = wave * SPW + grp;
-    uint32_t* d32 = reinterpret_cast<uint32_t*>(dist);
-    for (int i = tid; i < SWG * n / 2; i += 256) d32[i] = kInf2;
-    RowT prow = 0;
-    if (r < b) {
-        const uint8_t* p = pass + size_t(R0 + r) * S + C0;
-        for (int c = 0; c < b; ++c) prow |= RowT(p[c] != 0) << c;
+    if ((SWG * n) % 8 == 0) {
+        uint4* d128 = reinterpret_cast<uint4*>(dist);
+        for (int i = tid; i < SWG * n / 8; i += 256) d128[i] = make_uint4(kInf2, kInf2, kInf2, kInf2);
+    } else {
+        uint32_t* d32 = reinterpret_cast<uint32_t*>(dist);
+        for (int i = tid; i < SWG * n / 2; i += 256) d32[i] = kInf2;
+    }
+    // The tile's passability rows as bit masks, each row read once per
+    // workgroup with word loads (every source group of the workgroup shares
+    // the tile): row loads one byte per lane touched a cache line per lane
+    // and per byte, and bounded the whole kernel.
+    __shared__ RowT rowmask[64];
+    if (tid < b) {
+        const uint8_t* p = pass + size_t(R0 + tid) * S + C0;
+        RowT m = 0;
+        if ((b & 3) == 0) {  // 4-aligned: C0 and S are multiples of b
+            const uint32_t* p4 = reinterpret_cast<const uint32_t*>(p);
+            for (int w = 0; w < b / 4; ++w) {
+                const uint32_t x = p4[w];
+                // 0x80 in each nonzero byte, then the four flags gathered
+                // into bits 28..31 by one multiply
+                const uint32_t nz = (((x & 0x7f7f7f7fu) + 0x7f7f7f7fu) | x) & 0x80808080u;
+                m |= RowT(((nz >> 7) * 0x10204080u) >> 28) << (4 * w);
+            }
+        } else {
+            for (int c = 0; c < b; ++c) m |= RowT(p[c] != 0) << c;
+        }
+        rowmask[tid] = m;
     }
     __syncthreads();
+    const RowT prow = r < b ? rowmask[r] : RowT(0);
     const int sr = valid ? src / b : 0, sc = valid ? src % b : 0;
     const RowT srow = __shfl(prow, sr, LANES);
     RowT front = (valid && r == sr && ((srow >> sc) & 1)) ? (RowT(1) << sc) : RowT(0);
@@ -108,7 +131,7 @@ __global__ __launch_bounds__(256) void k_leaf_apsp(const uint8_t* __restrict__ p
                                                    int* __restrict__ changed, const int* __restrict__ chg_idx) {
     constexpr int SPW = 64 / LANES;
     constexpr int SWG = 4 * SPW;
-    extern __shared__ uint16_t dist[];  // [SWG][b*b]
+    extern __shared__ __align__(16) uint16_t dist[];  // [SWG][b*b]
     // work item = (leaf, chunk of SWG sources); a capped grid strides over them
     for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
     if (item != int(blockIdx.x)) __syncthreads();  // the previous item's readers of dist are done
@@ -162,7 +185,7 @@ __global__ __launch_bounds__(256) void k_leaf_pivot_rows(const uint8_t* __restri
                                                          uint16_t* __restrict__ P, uint16_t* __restrict__ leafS,
                                                          int* __restrict__ changed) {
     constexpr int SPW = 64 / LANES;
-    extern __shared__ uint16_t dist[];  // [SWG][b*b]
+    extern __shared__ __align__(16) uint16_t dist[];  // [SWG][b*b]
     for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
         if (item != int(blockIdx.x)) __syncthreads();
         const int4 it = items[item];
