@@ -41,9 +41,18 @@ int hip_fail(hipError_t e, const char* what) {
 
 // Buffers that have to grow double: freeing device or pinned memory waits
 // for the whole device, which stalls every other stream's work.
+static int debug_level();
+
+// Growing a buffer frees the old one, and freeing device or pinned memory
+// waits for the whole device; HTG_DEBUG >= 3 reports every growth.
+static void note_growth(const char* what, size_t from, size_t to) {
+    if (debug_level() >= 3) std::fprintf(stderr, "[htg] %s grows %zu -> %zu bytes\n", what, from, to);
+}
+
 int DevBuf::ensure(size_t n) {
     if (n <= bytes) return HTG_OK;
     if (bytes > 0) n = std::max(n, 2 * bytes);
+    note_growth("device buffer", bytes, n);
     release();
     if (n == 0) return HTG_OK;
     hipError_t e = hipMalloc(&p, n);
@@ -64,6 +73,7 @@ void DevBuf::release() {
 int HostBuf::ensure(size_t n) {
     if (n <= bytes) return HTG_OK;
     if (bytes > 0) n = std::max(n, 2 * bytes);
+    note_growth("pinned buffer", bytes, n);
     release();
     if (n == 0) return HTG_OK;
     hipError_t e = hipHostMalloc(&p, n, hipHostMallocDefault);
