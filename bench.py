@@ -415,7 +415,7 @@ def run_served(args):
     srv = serving.Server(n_slots=args.slots, max_batch_queries=args.batch_queries,
                          max_batch_viewpoints=args.batch_viewpoints, consistency=mode,
                          update_cu_percent=args.update_cu, update_tick_us=args.update_tick_us,
-                         update_streams=args.update_streams)
+                         update_streams=args.update_streams or 1)
     # plain figure over the exploration: harmonic mean of the three states
     plain_mix = 3.0 / sum(1.0 / plain[n]["kmix_matrices_per_s"] for n in plain)
     plain_64 = 3.0 / sum(1.0 / plain[n]["k64_matrices_per_s"] for n in plain)
@@ -522,7 +522,7 @@ def run_served(args):
                    "mode": "served", "consistency": args.consistency, "window": int(s["queries"] + s["updates"]) //
                    args.steps, "slots": args.slots, "max_batch_queries": args.batch_queries,
                    "update_cu_percent": args.update_cu, "update_tick_us": args.update_tick_us, "update_frac": uf,
-                   "update_lag_ms": args.update_lag_ms, "update_streams": args.update_streams or 2},
+                   "update_lag_ms": args.update_lag_ms, "update_streams": args.update_streams or 1},
         "p50_ms": s["p50_ms"], "p99_ms": s["p99_ms"], "max_ms": s["max_ms"],
         "update_p50_ms": s["update_p50_ms"], "update_p99_ms": s["update_p99_ms"],
         "arrival_rate_per_s": s["rate"], "timed_arrival_span_s": s["timed_arrival_span_s"],
@@ -1128,7 +1128,9 @@ def main():
     ap.add_argument("--update-frac", type=float, default=0.1, help="share of update requests in the stream")
     ap.add_argument("--update-tick-us", type=int, default=0, help="snapshot mode: minimum spacing of update batches")
     ap.add_argument("--update-streams", type=int, default=0,
-                    help="served: update streams (and snapshot update threads) the sessions are dealt to; 0 = 2")
+                    help="update streams (and snapshot update threads) the sessions are dealt to; 0 = the mode's "
+                         "default: 1 for served (one update at a time merges more reveals per closure and keeps "
+                         "heavy update kernels of different sessions from coinciding), 2 for multi")
     ap.add_argument("--update-pivots", default="cover", choices=sorted(PIVOT_SETS),
                     help="update and reveal modes: close over a vertex cover of the decreased entries, or over "
                          "every decreased row (the reference's pivot set)")
