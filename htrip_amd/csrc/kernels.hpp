@@ -126,6 +126,12 @@ bool launch_closure_pivots(int P, int nV, int ld, const int* node_list, int n_li
 bool launch_closure_update(int P, int nV, int ld, const int* node_list, int n_list, const int* counts, int t,
                            uint16_t* DU, size_t node_stride, const int* live8_cnt, const int* live8_list, int ng8,
                            int max_live8, const uint16_t* Rbuf, hipStream_t st, KTimer kt = {});
+// Phase 2 of block t (from Rcur) and phase 1 of block t + 1 (into Rnext, a
+// different buffer) in one launch.
+bool launch_closure_step(int P, int nV, int ld, const int* node_list, int n_list, const int* pivots, int piv_stride,
+                         const int* counts, int t, uint16_t* DU, size_t node_stride, const int* live8_cnt,
+                         const int* live8_list, int ng8, int max_live8, const uint16_t* Rcur, uint16_t* Rnext,
+                         hipStream_t st, KTimer kt = {});
 
 // S_U := D_U[ext][ext]; changed[i] = 1 if listed node i's S_U decreased.
 void launch_project_su(int nV, int ld, int ne, const int* ext_idx, const int* node_list, int n_list,

@@ -1004,36 +1004,46 @@ void launch_compact_flags(int nV, const int* node_list, int n_list, uint8_t* fla
 //   R[k][j] = min_{k'} C*[k][k'] (+) D[piv_k'][j]
 // are the pivot rows after FW over the block (src/internal_node.cpp:199-231
 // runs the same pivots one at a time).
+//
+// The phase can run in the same launch as phase 2 of the previous block
+// (Rprev set): it then reads the pivot rows it needs as they are after that
+// block, D'[k][j] = min(D[k][j], min_{k'} Rprev[k'][k] (+) Rprev[k'][j]),
+// whether or not the concurrent phase-2 workgroups have stored those rows yet
+// (both give the same minimum).  Pivot ports are passable, so their columns
+// of Rprev lie in live octets, which the previous block wrote.
 template <int P>
-__global__ __launch_bounds__(256) void k_closure_pivots(int nV, int ld, const int* __restrict__ node_list,
-                                                        const int* __restrict__ pivots, int piv_stride,
-                                                        const int* __restrict__ counts, int t,
-                                                        const uint16_t* __restrict__ DU, size_t node_stride,
-                                                        const int* __restrict__ live8_cnt,
-                                                        const int* __restrict__ live8_list, int ng8,
-                                                        uint16_t* __restrict__ Rbuf) {
+struct PivotLds {
+    int kr[P];
+    alignas(16) uint16_t C[P][P];
+    alignas(16) uint32_t CdT[P][P];  // CdT[k2][k] = dup(C*[k][k2])
+    uint32_t Dt[P][64];              // pivot rows at this block's 128 columns (2 per lane)
+    uint16_t Rk[P][P];               // Rprev[k'][pivot a] of the previous block
+};
+
+template <int P>
+__device__ __forceinline__ void pivots_unit(PivotLds<P>& sh, int nV, int ld, const int* __restrict__ node_list,
+                                            const int* __restrict__ pivots, int piv_stride,
+                                            const int* __restrict__ counts, int t, const uint16_t* __restrict__ DU,
+                                            size_t node_stride, const int* __restrict__ live8_cnt,
+                                            const int* __restrict__ live8_list, int ng8, uint16_t* __restrict__ Rbuf,
+                                            const uint16_t* __restrict__ Rprev, int bx, int slot) {
     // Workgroup = 16 live octets (128 columns) of one node; wave w computes
     // output pivot rows [w*P/4, (w+1)*P/4) for the lane's 2 columns.  Columns
     // of dead octets (no passable port) stay INF in every pivot row and are
     // skipped.
     constexpr int KW = P / 4;
-    const int slot = blockIdx.y;
     const int p = min(P, counts[slot] - t * P);
     if (p <= 0) return;
     const int node = node_list[slot];
     const int nl8 = live8_cnt[node];
-    if (int(blockIdx.x) * 16 >= nl8) return;
-    __shared__ int kr[P];
-    __shared__ __align__(16) uint16_t C[P][P];
-    __shared__ __align__(16) uint32_t CdT[P][P];  // CdT[k2][k] = dup(C*[k][k2])
+    if (bx * 16 >= nl8) return;
     const int tid = threadIdx.x;
     const uint16_t* D = DU + size_t(node) * node_stride;
     const int* K = pivots + size_t(slot) * piv_stride + t * P;
-    __shared__ uint32_t Dt[P][64];  // pivot rows at this block's 128 columns (2 per lane)
-    if (tid < P) kr[tid] = tid < p ? K[tid] : 0;
+    if (tid < P) sh.kr[tid] = tid < p ? K[tid] : 0;
     __syncthreads();
     const int wave = tid >> 6, lane = tid & 63;
-    const int oct = blockIdx.x * 16 + (lane >> 2);
+    const int oct = bx * 16 + (lane >> 2);
     const bool col_ok = oct < nl8;
     const int j0 = col_ok ? 8 * live8_list[size_t(node) * ng8 + oct] + (lane & 3) * 2 : 0;
     // Issue the pivot-row loads first (wave w takes rows w, w + 4, ...); they
@@ -1041,20 +1051,64 @@ __global__ __launch_bounds__(256) void k_closure_pivots(int nV, int ld, const in
     constexpr int RPW = P / 4;
     uint32_t pre[RPW];
 #pragma unroll
-    for (int i = 0; i < RPW; ++i) pre[i] = *reinterpret_cast<const uint32_t*>(D + size_t(kr[wave + 4 * i]) * ld + j0);
+    for (int i = 0; i < RPW; ++i)
+        pre[i] = *reinterpret_cast<const uint32_t*>(D + size_t(sh.kr[wave + 4 * i]) * ld + j0);
+    const uint16_t* Rp = Rprev ? Rprev + size_t(slot) * P * ld : nullptr;
+    const int pp = Rprev ? min(P, counts[slot] - (t - 1) * P) : 0;  // rows of the previous block
+    constexpr int GPT = P * P / 256;
     {  // the pivot block: every gather in flight before the first LDS store
-        constexpr int GPT = P * P / 256;
         uint16_t gv[GPT];
 #pragma unroll
         for (int e = 0; e < GPT; ++e) {
             const int i = tid + 256 * e, a = i / P, c = i % P;
-            gv[e] = (a < p && c < p) ? D[size_t(kr[a]) * ld + kr[c]] : uint16_t(a == c ? 0 : kInf);
+            gv[e] = (a < p && c < p) ? D[size_t(sh.kr[a]) * ld + sh.kr[c]] : uint16_t(a == c ? 0 : kInf);
         }
 #pragma unroll
         for (int e = 0; e < GPT; ++e) {
             const int i = tid + 256 * e;
-            C[i / P][i % P] = gv[e];
+            sh.C[i / P][i % P] = gv[e];
         }
+    }
+    if (Rp) {
+        {  // the previous block's pivot rows at this block's pivot columns
+            uint16_t rv[GPT];
+#pragma unroll
+            for (int e = 0; e < GPT; ++e) {
+                const int i = tid + 256 * e, a = i / P, c = i % P;
+                rv[e] = (a < pp && c < p) ? Rp[size_t(a) * ld + sh.kr[c]] : uint16_t(kInf);
+            }
+#pragma unroll
+            for (int e = 0; e < GPT; ++e) {
+                const int i = tid + 256 * e;
+                sh.Rk[i / P][i % P] = rv[e];
+            }
+        }
+        __syncthreads();
+        // the lane's pivot-row pairs: pre[i] = min(pre[i], Rk[k'][row] (+) Rprev[k'][j0 pair])
+#pragma unroll 2
+        for (int k = 0; k < pp; ++k) {
+            const uint32_t r2 = *reinterpret_cast<const uint32_t*>(Rp + size_t(k) * ld + j0);
+#pragma unroll
+            for (int i = 0; i < RPW; ++i) pre[i] = relax2(pre[i], dup16(sh.Rk[k][wave + 4 * i]), r2);
+        }
+        // C[a][c] = min(C[a][c], min_k' Rk[k'][a] (+) Rk[k'][c]) for the
+        // thread's entries (fixed c, a = a0 + (256 / P) e)
+        const int c = tid % P, a0 = tid / P;
+        uint16_t cv[GPT];
+#pragma unroll
+        for (int e = 0; e < GPT; ++e) cv[e] = sh.C[a0 + (256 / P) * e][c];
+#pragma unroll 1
+        for (int k = 0; k < pp; ++k) {
+            const uint16_t rc = sh.Rk[k][c];
+#pragma unroll
+            for (int e = 0; e < GPT; ++e) {
+                const uint16_t v = add1(sh.Rk[k][a0 + (256 / P) * e], rc);
+                cv[e] = v < cv[e] ? v : cv[e];
+            }
+        }
+        __syncthreads();  // every thread has read C
+#pragma unroll
+        for (int e = 0; e < GPT; ++e) sh.C[a0 + (256 / P) * e][c] = cv[e];
     }
     __syncthreads();
     // Floyd-Warshall over the block's pivots.  Thread t keeps EPT entries of
@@ -1068,34 +1122,34 @@ __global__ __launch_bounds__(256) void k_closure_pivots(int nV, int ld, const in
     if constexpr (EPT >= 2) {
         constexpr int EP2 = EPT / 2;
         uint32_t cr[EP2];
-        const uint32_t* Crow = reinterpret_cast<const uint32_t*>(&C[fa][fc0]);
+        const uint32_t* Crow = reinterpret_cast<const uint32_t*>(&sh.C[fa][fc0]);
 #pragma unroll
         for (int e = 0; e < EP2; ++e) cr[e] = Crow[e];
         for (int k = 0; k < p; ++k) {
-            const uint32_t aik = dup16(C[fa][k]);
-            const uint32_t* rk = reinterpret_cast<const uint32_t*>(&C[k][fc0]);
+            const uint32_t aik = dup16(sh.C[fa][k]);
+            const uint32_t* rk = reinterpret_cast<const uint32_t*>(&sh.C[k][fc0]);
 #pragma unroll
             for (int e = 0; e < EP2; ++e) cr[e] = relax2(cr[e], aik, rk[e]);
-            uint32_t* Cw = reinterpret_cast<uint32_t*>(&C[fa][fc0]);
+            uint32_t* Cw = reinterpret_cast<uint32_t*>(&sh.C[fa][fc0]);
 #pragma unroll
             for (int e = 0; e < EP2; ++e) Cw[e] = cr[e];
             __syncthreads();
         }
     } else {
-        uint16_t cr = C[fa][fc0];
+        uint16_t cr = sh.C[fa][fc0];
         for (int k = 0; k < p; ++k) {
-            const uint16_t v = add1(C[fa][k], C[k][fc0]);
+            const uint16_t v = add1(sh.C[fa][k], sh.C[k][fc0]);
             cr = v < cr ? v : cr;
-            C[fa][fc0] = cr;
+            sh.C[fa][fc0] = cr;
             __syncthreads();
         }
     }
     for (int i = tid; i < P * P; i += 256) {
         const int k2 = i / P, k = i % P;
-        CdT[k2][k] = dup16(C[k][k2]);
+        sh.CdT[k2][k] = dup16(sh.C[k][k2]);
     }
 #pragma unroll
-    for (int i = 0; i < RPW; ++i) Dt[wave + 4 * i][lane] = pre[i];
+    for (int i = 0; i < RPW; ++i) sh.Dt[wave + 4 * i][lane] = pre[i];
     __syncthreads();
     if (!col_ok) return;
     const int kbase = wave * KW;
@@ -1103,9 +1157,9 @@ __global__ __launch_bounds__(256) void k_closure_pivots(int nV, int ld, const in
 #pragma unroll
     for (int k = 0; k < KW; ++k) acc[k] = kInf2;
     for (int k2 = 0; k2 < p; ++k2) {
-        const uint32_t dv = Dt[k2][lane];
+        const uint32_t dv = sh.Dt[k2][lane];
         // CdT rows k2 >= p hold INF off the diagonal; acc[k >= p] is never stored
-        const uint4* c4 = reinterpret_cast<const uint4*>(&CdT[k2][kbase]);
+        const uint4* c4 = reinterpret_cast<const uint4*>(&sh.CdT[k2][kbase]);
 #pragma unroll
         for (int k = 0; k < KW; k += 4) {
             const uint4 cv = c4[k / 4];
@@ -1127,26 +1181,29 @@ __global__ __launch_bounds__(256) void k_closure_pivots(int nV, int ld, const in
 // entries), 8 x 8 elements per thread held as 32 packed registers; R's row
 // slices for the tile sit in LDS (rows pre-duplicated into both u16 halves).
 template <int P>
-__global__ __launch_bounds__(256) void k_closure_update(int nV, int ld, const int* __restrict__ node_list,
-                                                        const int* __restrict__ counts, int t, int tiles,
-                                                        uint16_t* __restrict__ DU, size_t node_stride,
-                                                        const int* __restrict__ live8_cnt,
-                                                        const int* __restrict__ live8_list, int ng8,
-                                                        const uint16_t* __restrict__ Rbuf) {
-    const int slot = blockIdx.y;
+struct UpdateLds {
+    uint4 Rr4[P][32];  // dup'd R[k][rows of the tile's 16 row octets]
+    uint4 Rc4[P][16];  // R[k][columns of the tile's 16 column octets]
+    int ro[16], co[16];  // first port of each row / column octet, -1 past the live ones
+    uint32_t rmin[128], cmin[64];
+};
+
+template <int P>
+__device__ __forceinline__ void update_unit(UpdateLds<P>& sh, int nV, int ld, const int* __restrict__ node_list,
+                                            const int* __restrict__ counts, int t, int tiles,
+                                            uint16_t* __restrict__ DU, size_t node_stride,
+                                            const int* __restrict__ live8_cnt, const int* __restrict__ live8_list,
+                                            int ng8, const uint16_t* __restrict__ Rbuf, int bx, int slot) {
     const int p = min(P, counts[slot] - t * P);
     if (p <= 0) return;
     const int node = node_list[slot];
     const int nl8 = live8_cnt[node];
-    const int I = blockIdx.x / tiles, J = blockIdx.x % tiles;
+    const int I = bx / tiles, J = bx % tiles;
     if (I * 16 >= nl8 || J * 16 >= nl8) return;  // tile past this node's live octets
-    __shared__ uint4 Rr4[P][32];  // dup'd R[k][rows of the tile's 16 row octets]
-    __shared__ uint4 Rc4[P][16];  // R[k][columns of the tile's 16 column octets]
-    __shared__ int ro[16], co[16];  // first port of each row / column octet, -1 past the live ones
     const int tid = threadIdx.x;
     const int* l8 = live8_list + size_t(node) * ng8;
-    if (tid < 16) ro[tid] = I * 16 + tid < nl8 ? 8 * l8[I * 16 + tid] : -1;
-    else if (tid < 32) co[tid - 16] = J * 16 + tid - 16 < nl8 ? 8 * l8[J * 16 + tid - 16] : -1;
+    if (tid < 16) sh.ro[tid] = I * 16 + tid < nl8 ? 8 * l8[I * 16 + tid] : -1;
+    else if (tid < 32) sh.co[tid - 16] = J * 16 + tid - 16 < nl8 ? 8 * l8[J * 16 + tid - 16] : -1;
     __syncthreads();
     const uint16_t* R = Rbuf + size_t(slot) * P * ld;
     // stage with 16-byte loads: item i = (pivot k, octet slot o, side); every
@@ -1157,7 +1214,7 @@ __global__ __launch_bounds__(256) void k_closure_update(int nV, int ld, const in
     for (int e = 0; e < SPT; ++e) {
         const int i = tid + 256 * e;
         const int k = i >> 5, o = (i >> 1) & 15, col_side = i & 1;
-        const int base = col_side ? co[o] : ro[o];
+        const int base = col_side ? sh.co[o] : sh.ro[o];
         sv[e] = make_uint4(kInf2, kInf2, kInf2, kInf2);
         if (k < p && base >= 0) sv[e] = *reinterpret_cast<const uint4*>(R + size_t(k) * ld + base);
     }
@@ -1165,7 +1222,7 @@ __global__ __launch_bounds__(256) void k_closure_update(int nV, int ld, const in
     for (int e = 0; e < SPT; ++e) {
         const int i = tid + 256 * e;
         const int k = i >> 5, o = (i >> 1) & 15, col_side = i & 1;
-        const int base = col_side ? co[o] : ro[o];
+        const int base = col_side ? sh.co[o] : sh.ro[o];
         uint4 v = sv[e];
         if (base >= 0 && base + 8 > nV) {  // octet straddles nV: entries past it are padding
             uint16_t* ev = reinterpret_cast<uint16_t*>(&v);
@@ -1174,29 +1231,28 @@ __global__ __launch_bounds__(256) void k_closure_update(int nV, int ld, const in
                 if (base + j >= nV) ev[j] = kInf;
         }
         if (col_side) {
-            Rc4[k][o] = v;
+            sh.Rc4[k][o] = v;
         } else {
-            Rr4[k][2 * o] = make_uint4(dup16(v.x), dup16(v.x >> 16), dup16(v.y), dup16(v.y >> 16));
-            Rr4[k][2 * o + 1] = make_uint4(dup16(v.z), dup16(v.z >> 16), dup16(v.w), dup16(v.w >> 16));
+            sh.Rr4[k][2 * o] = make_uint4(dup16(v.x), dup16(v.x >> 16), dup16(v.y), dup16(v.y >> 16));
+            sh.Rr4[k][2 * o + 1] = make_uint4(dup16(v.z), dup16(v.z >> 16), dup16(v.w), dup16(v.w >> 16));
         }
     }
     __syncthreads();
     // Minima of the block's pivot rows at the tile's rows (dup'd) and column
     // pairs: every candidate R[k][i] (+) R[k][j] is >= rmin[i] (+) cmin[j].
-    __shared__ uint32_t rmin[128], cmin[64];
     {
         uint32_t m = kInf2;
         if (tid < 128) {
-            for (int k = 0; k < p; ++k) m = min2(m, reinterpret_cast<const uint32_t*>(&Rr4[k][0])[tid]);
-            rmin[tid] = m;
+            for (int k = 0; k < p; ++k) m = min2(m, reinterpret_cast<const uint32_t*>(&sh.Rr4[k][0])[tid]);
+            sh.rmin[tid] = m;
         } else if (tid < 192) {
-            for (int k = 0; k < p; ++k) m = min2(m, reinterpret_cast<const uint32_t*>(&Rc4[k][0])[tid - 128]);
-            cmin[tid - 128] = m;
+            for (int k = 0; k < p; ++k) m = min2(m, reinterpret_cast<const uint32_t*>(&sh.Rc4[k][0])[tid - 128]);
+            sh.cmin[tid - 128] = m;
         }
     }
     __syncthreads();
     const int tx = tid & 15, ty = tid >> 4;
-    const int col = co[tx], row0 = ro[ty];
+    const int col = sh.co[tx], row0 = sh.ro[ty];
     const bool ok = col >= 0 && row0 >= 0;
     uint16_t* D = DU + size_t(node) * node_stride;
     uint32_t acc[8][4];
@@ -1216,18 +1272,18 @@ __global__ __launch_bounds__(256) void k_closure_update(int nV, int ld, const in
     bool need = false;
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
-        const uint32_t rm = rmin[ty * 8 + r];
+        const uint32_t rm = sh.rmin[ty * 8 + r];
 #pragma unroll
         for (int h = 0; h < 4; ++h) {
-            const uint32_t lb = add2(rm, cmin[tx * 4 + h]);
+            const uint32_t lb = add2(rm, sh.cmin[tx * 4 + h]);
             need |= min2(lb, acc[r][h]) != acc[r][h];
         }
     }
     if (__ballot(need) != 0) {
 #pragma unroll 2
         for (int k = 0; k < p; ++k) {
-            const uint4 a0 = Rr4[k][ty * 2], a1 = Rr4[k][ty * 2 + 1];
-            const uint4 c = Rc4[k][tx];
+            const uint4 a0 = sh.Rr4[k][ty * 2], a1 = sh.Rr4[k][ty * 2 + 1];
+            const uint4 c = sh.Rc4[k][tx];
             const uint32_t av[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
 #pragma unroll
             for (int r = 0; r < 8; ++r) {
@@ -1244,6 +1300,59 @@ __global__ __launch_bounds__(256) void k_closure_update(int nV, int ld, const in
         const int i = row0 + r;
         if (ok && i < nV)
             *reinterpret_cast<uint4*>(D + size_t(i) * ld + col) = make_uint4(acc[r][0], acc[r][1], acc[r][2], acc[r][3]);
+    }
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_closure_pivots(int nV, int ld, const int* __restrict__ node_list,
+                                                        const int* __restrict__ pivots, int piv_stride,
+                                                        const int* __restrict__ counts, int t,
+                                                        const uint16_t* __restrict__ DU, size_t node_stride,
+                                                        const int* __restrict__ live8_cnt,
+                                                        const int* __restrict__ live8_list, int ng8,
+                                                        uint16_t* __restrict__ Rbuf) {
+    __shared__ __align__(16) unsigned char lds[sizeof(PivotLds<P>)];
+    pivots_unit<P>(*reinterpret_cast<PivotLds<P>*>(lds), nV, ld, node_list, pivots, piv_stride, counts, t, DU,
+                   node_stride, live8_cnt, live8_list, ng8, Rbuf, nullptr, int(blockIdx.x), int(blockIdx.y));
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_closure_update(int nV, int ld, const int* __restrict__ node_list,
+                                                        const int* __restrict__ counts, int t, int tiles,
+                                                        uint16_t* __restrict__ DU, size_t node_stride,
+                                                        const int* __restrict__ live8_cnt,
+                                                        const int* __restrict__ live8_list, int ng8,
+                                                        const uint16_t* __restrict__ Rbuf) {
+    __shared__ __align__(16) unsigned char lds[sizeof(UpdateLds<P>)];
+    update_unit<P>(*reinterpret_cast<UpdateLds<P>*>(lds), nV, ld, node_list, counts, t, tiles, DU, node_stride,
+                   live8_cnt, live8_list, ng8, Rbuf, int(blockIdx.x), int(blockIdx.y));
+}
+
+// Phase 2 of block t and phase 1 of block t + 1 in one launch (a 1-D grid):
+// workgroups [0, n_list * tiles) are the pivot column blocks of block t + 1
+// of every listed node (reading Rcur for their pivot rows, writing Rnext),
+// the rest the update tiles of block t (reading Rcur).  The pivot blocks of
+// all nodes come first: their Floyd-Warshall chains are the longest paths of
+// the launch, so they start with the first workgroups.
+template <int P>
+__global__ __launch_bounds__(256) void k_closure_step(int nV, int ld, const int* __restrict__ node_list, int n_list,
+                                                      const int* __restrict__ pivots, int piv_stride,
+                                                      const int* __restrict__ counts, int t, int tiles,
+                                                      uint16_t* __restrict__ DU, size_t node_stride,
+                                                      const int* __restrict__ live8_cnt,
+                                                      const int* __restrict__ live8_list, int ng8,
+                                                      const uint16_t* __restrict__ Rcur,
+                                                      uint16_t* __restrict__ Rnext) {
+    constexpr size_t kLds = sizeof(PivotLds<P>) > sizeof(UpdateLds<P>) ? sizeof(PivotLds<P>) : sizeof(UpdateLds<P>);
+    __shared__ __align__(16) unsigned char lds[kLds];
+    const int lin = int(blockIdx.x), npiv = n_list * tiles;
+    if (lin < npiv) {
+        pivots_unit<P>(*reinterpret_cast<PivotLds<P>*>(lds), nV, ld, node_list, pivots, piv_stride, counts, t + 1,
+                       DU, node_stride, live8_cnt, live8_list, ng8, Rnext, Rcur, lin % tiles, lin / tiles);
+    } else {
+        const int u = lin - npiv, nt2 = tiles * tiles;
+        update_unit<P>(*reinterpret_cast<UpdateLds<P>*>(lds), nV, ld, node_list, counts, t, tiles, DU, node_stride,
+                       live8_cnt, live8_list, ng8, Rcur, u % nt2, u / nt2);
     }
 }
 
@@ -1275,6 +1384,25 @@ bool launch_closure_update(int P, int nV, int ld, const int* node_list, int n_li
     else if (P == 32) HTG_CU(32);
     else HTG_CU(64);
 #undef HTG_CU
+    return true;
+}
+
+bool launch_closure_step(int P, int nV, int ld, const int* node_list, int n_list, const int* pivots, int piv_stride,
+                         const int* counts, int t, uint16_t* DU, size_t node_stride, const int* live8_cnt,
+                         const int* live8_list, int ng8, int max_live8, const uint16_t* Rcur, uint16_t* Rnext,
+                         hipStream_t st, KTimer kt) {
+    if (n_list == 0 || max_live8 == 0 || Rcur == Rnext) return false;
+    const int tiles = (max_live8 + 15) / 16;
+    const long long blocks = (long long)n_list * (tiles * tiles + tiles);
+    if (blocks > 0x7fffffffll) return false;
+    const dim3 grid{unsigned(blocks), 1u, 1u};
+#define HTG_CS(PP)                                                                                              \
+    HTG_LAUNCH_T(k_closure_step<PP>, grid, dim3(256), 0, st, kt, nV, ld, node_list, n_list, pivots, piv_stride, \
+                 counts, t, tiles, DU, node_stride, live8_cnt, live8_list, ng8, Rcur, Rnext)
+    if (P == 16) HTG_CS(16);
+    else if (P == 32) HTG_CS(32);
+    else HTG_CS(64);
+#undef HTG_CS
     return true;
 }
 

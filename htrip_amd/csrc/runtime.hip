@@ -569,7 +569,8 @@ int Map::ensure_build_ws(int lvl) {
     if ((rc = ws_piv.ensure(nn * L.d.nV * 4))) return rc;
     if ((rc = ws_bpiv.ensure(nn * L.d.nV * 4))) return rc;
     if ((rc = ws_counts.ensure(2 * nn * 4 + 16))) return rc;
-    if ((rc = ws_R.ensure(nn * size_t(tune.pivot_block) * L.d.ld * 2))) return rc;
+    // two pivot-row buffers: a block's update runs beside the next block's pivots
+    if ((rc = ws_R.ensure(2 * nn * size_t(tune.pivot_block) * L.d.ld * 2))) return rc;
     if ((rc = ws_changed.ensure(nn * 4 + 16))) return rc;
     if ((rc = ws_mask.ensure(nn * 4 + 16))) return rc;
     return HTG_OK;
@@ -591,30 +592,52 @@ int Map::run_closure(int lvl, const int* nodes, const std::vector<int>& hnodes, 
     }
     const int P = tune.pivot_block;
     const int blocks = (maxc + P - 1) / P;
-    for (int t = 0; t < blocks; ++t) {
-        double upd_p = 0, upd_u = 0, by_p = 0, by_u = 0;
-        if (prof_ref().on) {
-            for (int i = 0; i < n_list; ++i) {
-                const double p = std::max(0, std::min(P, hcounts[i] - t * P));
-                if (p <= 0) continue;
-                const double nV = 8.0 * h8[hnodes[i]];  // live rows / columns
-                upd_p += p * p * nV + p * p * p;
-                by_p += (p * nV + p * nV) * 2;  // pivot rows in, R out
-                upd_u += p * nV * nV;
-                by_u += (2 * nV * nV + 2 * p * nV) * 2;  // D_U tile in + out, R in
-            }
+    // Pivot blocks of up to 32: block 0's pivots, then per block t its update
+    // together with block t + 1's pivots, then the last block's update
+    // (blocks + 1 launches; the pivot rows alternate between the two halves
+    // of ws_R).  Blocks of 64 run the pivots and the update of each block as
+    // two launches: the next block's pivot rows then cost each pivot
+    // workgroup a 64 x 64 x 64 correction that outweighs the saved launch.
+    const bool fused = P <= 32;
+    uint16_t* Rb[2] = {static_cast<uint16_t*>(ws_R.p), static_cast<uint16_t*>(ws_R.p) + size_t(L.nn) * P * L.d.ld};
+    auto figures = [&](int t, double& upd_p, double& upd_u, double& by_p, double& by_u) {
+        upd_p = upd_u = by_p = by_u = 0;
+        if (!prof_ref().on || t >= blocks) return;
+        for (int i = 0; i < n_list; ++i) {
+            const double p = std::max(0, std::min(P, hcounts[i] - t * P));
+            if (p <= 0) continue;
+            const double nV = 8.0 * h8[hnodes[i]];  // live rows / columns
+            upd_p += p * p * nV + p * p * p;
+            by_p += (p * nV + p * nV) * 2;  // pivot rows in, R out
+            upd_u += p * nV * nV;
+            by_u += (2 * nV * nV + 2 * p * nV) * 2;  // D_U tile in + out, R in
         }
+    };
+    for (int t = 0; t < blocks; ++t) {
+        double upd_p, upd_u, by_p, by_u, up2 = 0, uu2 = 0, bp2 = 0, bu2 = 0;
+        figures(t, upd_p, upd_u, by_p, by_u);
+        uint16_t* R = fused ? Rb[t & 1] : Rb[0];
+        if (t == 0 || !fused) {
+            KTimer kt;
+            const int pi = prof_open(HTG_K_CLOSURE_PIVOTS, by_p, upd_p, kt);
+            const bool la = launch_closure_pivots(P, L.d.nV, L.d.ld, nodes, n_list, static_cast<const int*>(ws_piv.p),
+                                                  L.d.nV, static_cast<const int*>(ws_counts.p), t,
+                                                  static_cast<const uint16_t*>(L.DU.p), L.node_stride, d8c, d8l,
+                                                  L.ng8, max8, R, stream, kt);
+            prof_ref().close(pi, la);
+        }
+        // the update of block t (fused: with block t + 1's pivots, counted with it)
+        const bool step = fused && t + 1 < blocks;
+        if (step) figures(t + 1, up2, uu2, bp2, bu2);
         KTimer kt;
-        int pi = prof_open(HTG_K_CLOSURE_PIVOTS, by_p, upd_p, kt);
-        bool la = launch_closure_pivots(P, L.d.nV, L.d.ld, nodes, n_list, static_cast<const int*>(ws_piv.p), L.d.nV,
-                                        static_cast<const int*>(ws_counts.p), t, static_cast<const uint16_t*>(L.DU.p),
-                                        L.node_stride, d8c, d8l, L.ng8, max8, static_cast<uint16_t*>(ws_R.p), stream,
-                                        kt);
-        prof_ref().close(pi, la);
-        pi = prof_open(HTG_K_CLOSURE_UPDATE, by_u, upd_u, kt);
-        la = launch_closure_update(P, L.d.nV, L.d.ld, nodes, n_list, static_cast<const int*>(ws_counts.p), t,
-                                   static_cast<uint16_t*>(L.DU.p), L.node_stride, d8c, d8l, L.ng8, max8,
-                                   static_cast<const uint16_t*>(ws_R.p), stream, kt);
+        const int pi = prof_open(HTG_K_CLOSURE_UPDATE, by_u + bp2, upd_u + up2, kt);
+        const bool la =
+            step ? launch_closure_step(P, L.d.nV, L.d.ld, nodes, n_list, static_cast<const int*>(ws_piv.p), L.d.nV,
+                                       static_cast<const int*>(ws_counts.p), t, static_cast<uint16_t*>(L.DU.p),
+                                       L.node_stride, d8c, d8l, L.ng8, max8, R, Rb[(t + 1) & 1], stream, kt)
+                 : launch_closure_update(P, L.d.nV, L.d.ld, nodes, n_list, static_cast<const int*>(ws_counts.p), t,
+                                         static_cast<uint16_t*>(L.DU.p), L.node_stride, d8c, d8l, L.ng8, max8, R,
+                                         stream, kt);
         prof_ref().close(pi, la);
     }
     hipError_t e = hipGetLastError();

@@ -296,7 +296,7 @@ struct Map {
     uint64_t epoch = 0;
     htg_batch_stats stats{0, 0, 0};
     uint64_t budget = 0, device_bytes = 0;
-    htg_tuning tune{64, 4, 0, 0, 0};
+    htg_tuning tune{32, 4, 0, 0, 0};
     QueryWS qws;
     // Held by the calls that use the map's own workspaces (qws, lift_*), so
     // that concurrent queries on one map serialise instead of sharing them.

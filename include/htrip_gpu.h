@@ -340,7 +340,8 @@ int htg_server_replay(htg_server* s, int32_t n, const double* t_arrival, const i
 
 /* ---- build tuning (the project's autotuner picks these) --------------- */
 typedef struct htg_tuning {
-    int32_t pivot_block;   /* pivots per closure block: 16, 32 or 64 */
+    int32_t pivot_block;   /* pivots per closure block: 16, 32 (default) or 64; blocks of up to 32 run each
+                              block's update in the same launch as the next block's pivot rows */
     int32_t zq_tile;       /* viewpoints per query-projection work item: 1..16 */
     int32_t use_graph;     /* run the build as one captured hipGraph (replayed while its launch shape holds) */
     int32_t update_pivots; /* incremental reveal closes each node over: 0 (default) a vertex cover of the
