@@ -278,16 +278,26 @@ void htg_server::slot_loop(Batch* b) {
 }
 
 // Snapshot mode: apply one session's pending updates to its back copy and
-// swap.  Called with lk held; returns false when no session had work.
+// swap.  Called with lk held; returns false when no session had work.  The
+// session whose oldest pending update has waited longest goes first (a busy
+// session would otherwise refill its queue during its own update and keep
+// the others waiting).
 bool htg_server::update_one(std::unique_lock<std::mutex>& lk) {
+    const auto now = Clock::now();
+    Session* pick = nullptr;
+    Clock::time_point oldest{};
     for (auto& kv : sessions) {
         Session& ss = kv.second;
         if (ss.upd_pending.empty() || ss.updating) continue;
-        if (!stop && cfg.update_tick_us > 0 &&
-            Clock::now() - ss.last_update < std::chrono::microseconds(cfg.update_tick_us))
+        if (!stop && cfg.update_tick_us > 0 && now - ss.last_update < std::chrono::microseconds(cfg.update_tick_us))
             continue;  // coalesce: the next tick takes everything pending then
+        if (ss.inflight[1 - ss.front] > 0) continue;  // queries still read the back copy
+        const Clock::time_point t = ss.upd_pending.front()->t_submit;
+        if (!pick || t < oldest) pick = &ss, oldest = t;
+    }
+    if (pick) {
+        Session& ss = *pick;
         const int back = 1 - ss.front;
-        if (ss.inflight[back] > 0) continue;  // queries still read the back copy
         std::vector<Request*> run(ss.upd_pending.begin(), ss.upd_pending.end());
         ss.upd_pending.clear();
         ss.updating++;
