@@ -1576,11 +1576,11 @@ int Map::catch_up_from(const Map& src, const DirtySet& ds) {
     std::vector<size_t> o_nodes(ds.nodes.size());
     for (size_t l = 0; l < ds.nodes.size(); ++l) o_nodes[l] = pk.add(ds.nodes[l]);
     int rc;
-    if ((rc = ws_upd_h.ensure(pk.size)) || (rc = ws_upd.ensure(pk.size))) return rc;
-    pk.write(static_cast<char*>(ws_upd_h.p));
-    if ((e = hipMemcpyAsync(ws_upd.p, ws_upd_h.p, pk.size, hipMemcpyHostToDevice, stream)) != hipSuccess)
+    if ((rc = ws_cu_h.ensure(pk.size)) || (rc = ws_cu.ensure(pk.size))) return rc;
+    pk.write(static_cast<char*>(ws_cu_h.p));
+    if ((e = hipMemcpyAsync(ws_cu.p, ws_cu_h.p, pk.size, hipMemcpyHostToDevice, stream)) != hipSuccess)
         return hip_fail(e, "catch-up lists");
-    const char* up = static_cast<const char*>(ws_upd.p);
+    const char* up = static_cast<const char*>(ws_cu.p);
     const int* dleaves = reinterpret_cast<const int*>(up + o_leaves);
     const int nlv = int(ds.leaves.size());
     launch_copy_items(dleaves, nlv, n * n, static_cast<const uint16_t*>(src.leafD.p), static_cast<uint16_t*>(leafD.p),
@@ -1598,8 +1598,11 @@ int Map::catch_up_from(const Map& src, const DirtySet& ds) {
                               static_cast<uint16_t*>(L.DU.p), static_cast<const uint16_t*>(Ls.SU.p),
                               static_cast<uint16_t*>(L.SU.p), stream);
     }
+    // No host wait: the update that the catch-up prepares runs behind the
+    // copies on the map's stream, the map is not queried before that update
+    // has finished (its own final wait), and the next catch-up into ws_cu_h
+    // comes after that wait too.
     if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "catch-up copy");
-    if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "catch-up sync");
     stats = src.stats;
     epoch = src.epoch;
     stamp(ds.leaves, ds.nodes, 0);

@@ -343,6 +343,10 @@ struct Map {
     DevBuf ws_leafl;        // update: packed leaf-phase lists (new cells, leaf lists, pivot items)
     HostBuf ws_leafl_h;
     HostBuf ws_upd_h, ws_chg_h, ws_counts_h;  // pinned staging of the lists, changed flags, pivot counts
+    // snapshot catch-up lists: staging of their own, since the update that
+    // follows refills ws_upd_h while the catch-up's upload may still read it
+    DevBuf ws_cu;
+    HostBuf ws_cu_h;
 
     int build();
     int open_cells(const int32_t* rc, int n);
