@@ -54,7 +54,7 @@ def build_dropin_test(out: str) -> str:
     repo = os.path.dirname(HERE)
     src = os.path.join(repo, "tests", "cpp", "test_dropin.cpp")
     cmd = [
-        "g++", "-std=c++20", "-O2", "-o", out, src,
+        "g++", "-std=c++20", "-O2", "-pthread", "-o", out, src,
         "-I", os.path.join(HERE, "include"), "-I", os.path.join(repo, "include"),
         "-L", HERE, "-lhtrip_gpu", f"-Wl,-rpath,{HERE}",
     ]

@@ -2,13 +2,16 @@
 // compiled against htrip_amd/include/htrip/*.hpp and linked with
 // libhtrip_gpu.so.  Mirrors suites of the reference's apps/test_exactness.cpp
 // (known answers 1 and 37, blocked viewpoint INF, batch openings, copy
-// semantics, incremental refresh, reopened viewpoint, liftPoint, liftRoot) and checks
-// every matrix against a BFS written here.
+// semantics, incremental refresh, reopened viewpoint, liftPoint, liftRoot),
+// plus const queries from several threads each with its own scratch, as the
+// reference's simulator issues them (src/robot_simulator.cpp:1541-1546), and
+// checks every matrix against a BFS written here.
 // Exit status 0 = all passed.
 #include <cstdio>
 #include <cstdlib>
 #include <queue>
 #include <random>
+#include <thread>
 #include <vector>
 
 #include "htrip/grid_map.hpp"
@@ -296,6 +299,45 @@ int main() {
         o.liftRoot(g128);
         const dist_t d = o.queryDistance({0, 0}, {0, 100}, g128);
         expect(d == bfs_all_pairs(g128, {{0, 0}, {0, 100}})[1] && d == 100, "suite 16.4 liftRoot after a batch, d = 100");
+    }
+    {  // const queries from 4 threads, each with its own HTripQueryScratch, on one oracle
+        GridMap g(96, 96, CellState::Free);
+        std::mt19937 rng(11);
+        for (int i = 0; i < 900; ++i) g.blockCorridor({int(rng() % 96), int(rng() % 96)});
+        HierarchyConfig cfg;
+        cfg.tile_size_b = 16;
+        MinPlusOracle o(cfg);
+        o.build(g);
+        constexpr int kThreads = 4, kQueries = 12;
+        std::vector<std::vector<std::vector<Point>>> sets(kThreads);
+        std::vector<std::vector<std::vector<dist_t>>> want(kThreads);
+        for (int t = 0; t < kThreads; ++t)
+            for (int q = 0; q < kQueries; ++q) {
+                std::vector<Point> vps;
+                const size_t k = 8 + size_t(rng() % 40);
+                while (vps.size() < k) {
+                    const Point p{int(rng() % 96), int(rng() % 96)};
+                    if (g.isPassable(p)) vps.push_back(p);
+                }
+                want[size_t(t)].push_back(bfs_all_pairs(g, vps));
+                sets[size_t(t)].push_back(std::move(vps));
+            }
+        const MinPlusOracle& co = o;
+        std::vector<int> good(kThreads, 0);
+        std::vector<std::thread> th;
+        for (int t = 0; t < kThreads; ++t)
+            th.emplace_back([&, t] {
+                MinPlusOracle::HTripQueryScratch scratch;
+                std::vector<dist_t> out;
+                for (int q = 0; q < kQueries; ++q) {
+                    co.queryAllPairs(sets[size_t(t)][size_t(q)], g, out, scratch);
+                    good[size_t(t)] += out == want[size_t(t)][size_t(q)];
+                }
+            });
+        for (auto& x : th) x.join();
+        bool ok = true;
+        for (int t = 0; t < kThreads; ++t) ok = ok && good[size_t(t)] == kQueries;
+        expect(ok, "4 threads, own scratch each, 48 matrices == BFS");
     }
     std::printf("%s (%d failure%s)\n", failures ? "FAILED" : "ALL PASSED", failures, failures == 1 ? "" : "s");
     return failures ? 1 : 0;
