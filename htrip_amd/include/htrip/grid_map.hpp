@@ -1,9 +1,15 @@
 #pragma once
 // Drop-in replacement of the reference's include/htrip/grid_map.hpp for the
 // parts the distance oracle reads: dimensions, the observed layer, passability
-// and edge weights (grid_map.hpp:31-334), plus the MovingAI loader
-// (src/grid_map.cpp:16-76).  Header-only.
+// and edge weights (grid_map.hpp:31-334), the MovingAI loader
+// (src/grid_map.cpp:16-76) and the two synthetic world producers the
+// reference's test and benchmark programs build their grids with
+// (src/grid_map.cpp:78-200).  Header-only.
+#include <algorithm>
+#include <cstdint>
 #include <fstream>
+#include <queue>
+#include <random>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -106,6 +112,100 @@ public:
                 ground_truth[size_t(r) * w + c] =
                     (ch == '.' || ch == 'G' || ch == 'S') ? CellState::Free : CellState::Obstacle;
             }
+        return true;
+    }
+
+    // src/grid_map.cpp:78-141.  Rooms of room_size (at least 4) tile the grid;
+    // a room's interior is its cells one step inside its square, so walls
+    // between rooms are two cells thick.  Every room then gets a doorway
+    // through the wall to its east neighbour and one to its south neighbour,
+    // at a random offset inside the room, from one mt19937_64 stream drawn in
+    // room order (row-major; east before south), so that a seed gives the
+    // reference's world cell for cell.  observed starts equal to the truth.
+    void generateProceduralRooms(int r, int c, int room_size = 16, uint64_t seed = 42) {
+        const int rs = std::max(room_size, 4);
+        rows = r;
+        cols = c;
+        const size_t n = size_t(rows) * size_t(cols);
+        ground_truth.assign(n, CellState::Obstacle);
+        observed.assign(n, CellState::Unknown);
+        weights.assign(n, 1);
+        auto free_cell = [&](int y, int x) {
+            if (y < rows && x < cols) ground_truth[size_t(index(y, x))] = CellState::Free;
+        };
+        const int ny = rows / rs, nx = cols / rs;
+        for (int ry = 0; ry < ny; ++ry)
+            for (int rx = 0; rx < nx; ++rx) {
+                const int y1 = std::min((ry + 1) * rs - 1, rows - 1), x1 = std::min((rx + 1) * rs - 1, cols - 1);
+                for (int y = ry * rs + 1; y < y1; ++y)
+                    for (int x = rx * rs + 1; x < x1; ++x) free_cell(y, x);
+            }
+        std::mt19937_64 rng(seed);
+        auto offset = [&] { return 1 + int(rng() % uint64_t(rs - 2)); };
+        for (int ry = 0; ry < ny; ++ry)
+            for (int rx = 0; rx < nx; ++rx) {
+                if (rx + 1 < nx) {  // east: both wall layers at one row
+                    const int y = ry * rs + offset();
+                    if (y < rows) free_cell(y, (rx + 1) * rs - 1), free_cell(y, (rx + 1) * rs);
+                }
+                if (ry + 1 < ny) {  // south: both wall layers at one column
+                    const int x = rx * rs + offset();
+                    if (x < cols) free_cell((ry + 1) * rs - 1, x), free_cell((ry + 1) * rs, x);
+                }
+            }
+        observed = ground_truth;
+    }
+
+    // src/grid_map.cpp:143-200.  A partly explored snapshot: everything
+    // Unknown, then a 4-neighbour BFS (north, south, west, east) from
+    // start_point (or, when that is not free in the truth, the first free cell
+    // in row-major order) over the truth until max(10, free * coverage) free
+    // cells are revealed.  A reached obstacle is revealed as an obstacle.  A
+    // reached free cell is skipped (left Unknown, not expanded) with
+    // probability island_prob once more than 50 cells are revealed; the
+    // uniform draw happens for every reached free cell while island_prob > 0,
+    // as in the reference, so a seed gives the same snapshot.
+    bool initExplorationSnapshot(Point start_point, double target_coverage = 0.45, double island_prob = 0.1,
+                                 uint64_t seed = 12345) {
+        auto truth_free = [&](int i) { return ground_truth[size_t(i)] == CellState::Free; };
+        if (!inBounds(start_point) || !truth_free(index(start_point))) {
+            const auto it = std::find(ground_truth.begin(), ground_truth.end(), CellState::Free);
+            if (it == ground_truth.end()) return false;
+            start_point = point(int(it - ground_truth.begin()));
+        }
+        const size_t n = size_t(rows) * size_t(cols);
+        observed.assign(n, CellState::Unknown);
+        std::mt19937_64 rng(seed);
+        std::uniform_real_distribution<double> u01(0.0, 1.0);
+        const int total = int(std::count(ground_truth.begin(), ground_truth.end(), CellState::Free));
+        const int budget = std::max(10, int(total * target_coverage));
+        std::vector<uint8_t> seen(n, 0);
+        std::queue<Point> frontier;
+        frontier.push(start_point);
+        seen[size_t(index(start_point))] = 1;
+        observed[size_t(index(start_point))] = CellState::Free;
+        int revealed = 1;
+        const Point step[4] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}};
+        while (!frontier.empty() && revealed < budget) {
+            const Point at = frontier.front();
+            frontier.pop();
+            for (const Point& d : step) {
+                const Point nb{at.r + d.r, at.c + d.c};
+                if (!inBounds(nb)) continue;
+                const size_t i = size_t(index(nb));
+                if (seen[i]) continue;
+                seen[i] = 1;
+                if (ground_truth[i] == CellState::Obstacle) {
+                    observed[i] = CellState::Obstacle;
+                    continue;
+                }
+                if (ground_truth[i] != CellState::Free) continue;
+                if (island_prob > 0.0 && u01(rng) < island_prob && revealed > 50) continue;
+                observed[i] = CellState::Free;
+                frontier.push(nb);
+                if (++revealed >= budget) break;
+            }
+        }
         return true;
     }
 };

@@ -91,6 +91,8 @@ def _lib(kind: str):
         api["time_queries"] = fn("time_queries", C.c_double, vp, _i32p, _i32p, ci, ci, C.c_void_p)
         api["time_lifts"] = fn("time_lifts", C.c_double, vp, _i32p, ci)
         api["load_movingai"] = fn("load_movingai", ci, C.c_char_p, _i32p, _i32p, _u8p, C.c_int64)
+        api["world"] = fn("world", ci, ci, ci, ci, C.c_uint64, ci, ci, C.c_double, C.c_double, C.c_uint64, _u8p,
+                          _u8p)
         api["lift_root"] = fn("lift_root", None, vp, ci, ci, _u8p)
         api["frontiers"] = fn("frontiers", ci, ci, ci, _u8p, ci, ci, _i32p, ci, _i32p, ci, _u8p, _u8p)
         api["inc_create"] = fn("inc_create", vp)
@@ -111,6 +113,17 @@ def reference_load_movingai(path: str):
     out = np.empty(int(rows[0]) * int(cols[0]), dtype=np.uint8)
     api["load_movingai"](path.encode(), rows, cols, out, out.size)
     return out.reshape(int(rows[0]), int(cols[0]))
+
+
+def reference_world(rows: int, cols: int, room: int, seed: int, start=(0, 0), coverage: float = 0.0,
+                    island: float = 0.1, seed2: int = 12345):
+    """The compiled reference's GridMap::generateProceduralRooms and, when
+    coverage > 0, initExplorationSnapshot: (ok, truth codes, observed codes)."""
+    api = _lib("reference")
+    truth = np.empty(rows * cols, dtype=np.uint8)
+    obs = np.empty(rows * cols, dtype=np.uint8)
+    ok = api["world"](rows, cols, room, seed, start[0], start[1], coverage, island, seed2, truth, obs)
+    return bool(ok), truth.reshape(rows, cols), obs.reshape(rows, cols)
 
 
 def reference_frontiers(obs: np.ndarray, min_size: int = 2, max_size: int = 16):

@@ -245,6 +245,19 @@ int ref_load_movingai(const char* path, int32_t* rows, int32_t* cols, uint8_t* o
         for (size_t i = 0; i < g.ground_truth.size(); ++i) out[i] = uint8_t(g.ground_truth[i]);
     return 1;
 }
+// GridMap::generateProceduralRooms(rows, cols, room, seed), then, when
+// coverage > 0, initExplorationSnapshot({sr, sc}, coverage, island, seed2):
+// the ground truth and observed codes (rows * cols each) and the snapshot's
+// return value (1 when it was not run).
+int ref_world(int rows, int cols, int room, uint64_t seed, int sr, int sc, double coverage, double island,
+              uint64_t seed2, uint8_t* truth, uint8_t* obs) {
+    GridMap g;
+    g.generateProceduralRooms(rows, cols, room, seed);
+    const int ok = coverage > 0 ? int(g.initExplorationSnapshot(Point{sr, sc}, coverage, island, seed2)) : 1;
+    for (size_t i = 0; i < g.ground_truth.size(); ++i) truth[i] = uint8_t(g.ground_truth[i]);
+    for (size_t i = 0; i < g.observed.size(); ++i) obs[i] = uint8_t(g.observed[i]);
+    return ok;
+}
 // MinPlusOracle::liftPoint: X levels back to back (lens[l] entries each),
 // par/quad/off H+1 entries, active[l] in the reference's order padded with -1
 // to lens[l].

@@ -2,11 +2,14 @@
 // compiled against htrip_amd/include/htrip/*.hpp and linked with
 // libhtrip_gpu.so.  Mirrors suites of the reference's apps/test_exactness.cpp
 // (known answers 1 and 37, blocked viewpoint INF, batch openings, copy
-// semantics, incremental refresh, reopened viewpoint, liftPoint, liftRoot),
+// semantics, incremental refresh, reopened viewpoint, liftPoint, liftRoot,
+// a non-square workspace, changing viewpoint lists, queryDistance against the
+// matrix, on the reference's own procedural worlds),
 // plus const queries from several threads each with its own scratch, as the
 // reference's simulator issues them (src/robot_simulator.cpp:1541-1546), and
 // checks every matrix against a BFS written here.
 // Exit status 0 = all passed.
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <queue>
@@ -299,6 +302,80 @@ int main() {
         o.liftRoot(g128);
         const dist_t d = o.queryDistance({0, 0}, {0, 100}, g128);
         expect(d == bfs_all_pairs(g128, {{0, 0}, {0, 100}})[1] && d == 100, "suite 16.4 liftRoot after a batch, d = 100");
+    }
+    {  // Suite 11: non-square workspace, 20 x 44 rooms (b = 8 pads S to 64), openings (:798-841)
+        GridMap g;
+        g.generateProceduralRooms(20, 44, 8, 453);
+        g.initExplorationSnapshot({2, 2}, 0.65, 0.10, 54);
+        std::vector<Point> vps;
+        for (int i = 0; i < 20 * 44 && vps.size() < 10; ++i)
+            if (g.observed[size_t(i)] == CellState::Free) vps.push_back(g.point(i));
+        HierarchyConfig cfg;
+        cfg.tile_size_b = 8;
+        MinPlusOracle o(cfg);
+        o.build(g);
+        MinPlusOracle::HTripQueryScratch scratch;
+        MinPlusOracle::HTripIncrementalState state;
+        std::vector<dist_t> inc;
+        o.queryAllPairsIncremental(vps, g, inc, scratch, state);
+        bool ok = o.S == 64 && matches_bfs(o, g, vps) && inc == bfs_all_pairs(g, vps);
+        std::mt19937_64 rng(55);
+        int opened = 0;
+        for (int step = 0; step < 400 && opened < 15; ++step) {
+            const Point p{int(rng() % 20), int(rng() % 44)};
+            if (g.ground_truth[size_t(g.index(p))] != CellState::Free || g.getObserved(p) == CellState::Free) continue;
+            g.openDoorway(p);
+            o.onCellOpened(p, g);
+            ++opened;
+        }
+        o.queryAllPairsIncremental(vps, g, inc, scratch, state);
+        ok = ok && opened == 15 && matches_bfs(o, g, vps) && inc == bfs_all_pairs(g, vps);
+        expect(ok, "suite 11 20x44 rooms, 15 openings == BFS");
+    }
+    {  // Suite 14: the viewpoint list changes (k and order) between incremental calls (:945-985)
+        GridMap g;
+        g.generateProceduralRooms(32, 32, 16, 856);
+        g.initExplorationSnapshot({1, 1}, 0.70, 0.10, 58);
+        std::vector<Point> cells;
+        for (int i = 0; i < 32 * 32; ++i)
+            if (g.observed[size_t(i)] == CellState::Free) cells.push_back(g.point(i));
+        std::mt19937_64 rng(59);
+        std::shuffle(cells.begin(), cells.end(), rng);
+        const std::vector<Point> a(cells.begin(), cells.begin() + 5);
+        std::vector<Point> bl(cells.begin(), cells.begin() + 8);
+        std::reverse(bl.begin(), bl.end());
+        std::vector<Point> ar(a.rbegin(), a.rend());
+        HierarchyConfig cfg;
+        cfg.tile_size_b = 16;
+        MinPlusOracle o(cfg);
+        o.build(g);
+        MinPlusOracle::HTripQueryScratch scratch;
+        MinPlusOracle::HTripIncrementalState state;
+        bool ok = true;
+        const std::vector<Point>* lists[] = {&a, &bl, &a, &ar};
+        for (const auto* l : lists) {
+            std::vector<dist_t> inc;
+            o.queryAllPairsIncremental(*l, g, inc, scratch, state);
+            ok = ok && inc == bfs_all_pairs(g, *l) && inc == o.queryAllPairs(*l, g);
+        }
+        expect(ok, "suite 14 lists A, B, A, A reversed: incremental == BFS");
+    }
+    {  // Suite 15: queryDistance equals the matrix for every pair (:987-1024)
+        GridMap g;
+        g.generateProceduralRooms(40, 40, 16, 957);
+        g.initExplorationSnapshot({2, 2}, 0.68, 0.10, 60);
+        std::vector<Point> vps;
+        for (int i = 0; i < 40 * 40 && vps.size() < 10; ++i)
+            if (g.observed[size_t(i)] == CellState::Free) vps.push_back(g.point(i));
+        HierarchyConfig cfg;
+        cfg.tile_size_b = 8;
+        MinPlusOracle o(cfg);
+        o.build(g);
+        const auto m = o.queryAllPairs(vps, g);
+        bool ok = m == bfs_all_pairs(g, vps);
+        for (size_t i = 0; i < vps.size(); ++i)
+            for (size_t j = 0; j < vps.size(); ++j) ok = ok && o.queryDistance(vps[i], vps[j], g) == m[i * vps.size() + j];
+        expect(ok, "suite 15 queryDistance == matrix for 100 pairs");
     }
     {  // const queries from 4 threads, each with its own HTripQueryScratch, on one oracle
         GridMap g(96, 96, CellState::Free);
