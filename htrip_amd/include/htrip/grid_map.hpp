@@ -2,11 +2,12 @@
 // Drop-in replacement of the reference's include/htrip/grid_map.hpp for the
 // parts the distance oracle reads: dimensions, the observed layer, passability
 // and edge weights (grid_map.hpp:31-334), the MovingAI loader
-// (src/grid_map.cpp:16-76) and the two synthetic world producers the
-// reference's test and benchmark programs build their grids with
-// (src/grid_map.cpp:78-200).  Header-only.
+// (src/grid_map.cpp:16-76), cell weights and clearance, and the two synthetic
+// world producers the reference's test and benchmark programs build their
+// grids with (src/grid_map.cpp:78-200).  Header-only.
 #include <algorithm>
 #include <cstdint>
+#include <cstdlib>
 #include <fstream>
 #include <queue>
 #include <random>
@@ -56,6 +57,30 @@ public:
         return weights[size_t(index(r, c))];
     }
     [[nodiscard]] dist_t getCellWeight(Point p) const noexcept { return getCellWeight(p.r, p.c); }
+    // grid_map.hpp:297-313: the weight array is created (all 1) on first use.
+    void setCellWeight(int r, int c, dist_t w) noexcept {
+        if (!inBounds(r, c)) return;
+        const size_t n = size_t(rows) * size_t(cols);
+        if (weights.size() < n) weights.assign(n, 1);
+        weights[size_t(index(r, c))] = w;
+    }
+    void setCellWeight(Point p, dist_t w) noexcept { setCellWeight(p.r, p.c, w); }
+    // grid_map.hpp:174-212: clearance tier of an observed cell.  0 when it is
+    // not passable; 1 when one of its 8 neighbours is an obstacle or off the
+    // grid; 2 when a cell at Chebyshev distance 2 is; 3 otherwise.  Unknown
+    // cells do not count as obstacles.
+    [[nodiscard]] uint8_t getClearance(int r, int c) const noexcept {
+        if (!isPassable(r, c)) return 0;
+        auto walled = [&](int y, int x) {
+            return !inBounds(y, x) || observed[size_t(index(y, x))] == CellState::Obstacle;
+        };
+        for (int ring = 1; ring <= 2; ++ring)
+            for (int dy = -ring; dy <= ring; ++dy)
+                for (int dx = -ring; dx <= ring; ++dx)
+                    if (std::max(std::abs(dy), std::abs(dx)) == ring && walled(r + dy, c + dx)) return uint8_t(ring);
+        return 3;
+    }
+    [[nodiscard]] uint8_t getClearance(Point p) const noexcept { return getClearance(p.r, p.c); }
     [[nodiscard]] dist_t edgeWeight(Point u, Point v) const noexcept {
         const dist_t wu = getCellWeight(u), wv = getCellWeight(v);
         if (wu >= INF || wv >= INF) return INF;

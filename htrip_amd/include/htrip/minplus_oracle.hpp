@@ -2,7 +2,8 @@
 // Drop-in replacement of the reference's include/htrip/minplus_oracle.hpp:
 // htrip::MinPlusOracle (alias HTripOracle) with the same hot-path signatures
 // (build, onCellOpened, onBatchCellsOpened, onCellBlocked, liftRoot,
-// queryAllPairs with HTripQueryScratch, queryAllPairsIncremental,
+// onBatchLeavesChanged, onCellWeightChanged / onTileWeightsChanged for unit
+// weights, queryAllPairs with HTripQueryScratch, queryAllPairsIncremental,
 // queryDistance, liftPoint) and
 // the public metadata callers read
 // (config, S, H, num_leaf_tiles_x/y, leaves.size(), depth(), findLcaLevel,
@@ -16,6 +17,8 @@
 //    besides build()'s bool).
 //  * hybrid_k_threshold is accepted and ignored: the GPU query is exact for
 //    every k, as the reference's BFS fallback is (minplus_oracle.cpp:951-958).
+//  * Cell weights other than 1 are refused (std::invalid_argument) by build,
+//    liftRoot and the weight-change calls: the index is unit-weight.
 #include <algorithm>
 #include <cstring>
 #include <memory>
@@ -40,6 +43,14 @@ inline htg_ctx* device_context() {
 }
 inline void check(int rc) {
     if (rc != HTG_OK) throw std::runtime_error(std::string("htrip: ") + htg_last_error());
+}
+// The GPU index computes unit-weight distances: a grid whose passable cells
+// carry another weight is refused rather than answered wrongly.
+inline void require_unit_weights(const GridMap& g) {
+    const size_t n = std::min(g.weights.size(), g.observed.size());
+    for (size_t i = 0; i < n; ++i)
+        if (g.observed[i] != CellState::Obstacle && g.weights[i] != 1)
+            throw std::invalid_argument("htrip: the GPU index supports cell weight 1 only");
 }
 }  // namespace detail
 
@@ -167,6 +178,7 @@ public:
     // src/minplus_oracle.cpp:48-137
     bool build(const GridMap& grid) {
         if (!config.isValid() || grid.rows <= 0 || grid.cols <= 0) return false;
+        detail::require_unit_weights(grid);
         std::vector<uint8_t> cells(grid.observed.size());
         for (size_t i = 0; i < cells.size(); ++i) cells[i] = uint8_t(grid.observed[i]);
         htg_config c{config.tile_size_b, config.group_w, config.group_h, config.hybrid_k_threshold};
@@ -187,6 +199,7 @@ public:
     // the device's; the rest is computed from grid.observed.
     void liftRoot(const GridMap& grid) {
         if (!map_) throw std::runtime_error("htrip: liftRoot before build");
+        detail::require_unit_weights(grid);
         std::vector<uint8_t> cells(grid.observed.size());
         for (size_t i = 0; i < cells.size(); ++i) cells[i] = uint8_t(grid.observed[i]);
         htg_map* m = map_.get();
@@ -227,6 +240,49 @@ public:
         if (!map_ || grid.getObserved(p) == CellState::Obstacle) return;
         detail::check(htg_open_cell(map_.get(), p.r, p.c));
         ++update_counter;
+    }
+
+    // src/minplus_oracle.cpp:304-362.  The GPU index computes unit-weight
+    // distances only: setting a cell's weight to 1 changes no table, any other
+    // weight is refused (std::invalid_argument, the grid left unchanged)
+    // rather than answered wrongly.
+    void onCellWeightChanged(Point p, dist_t new_weight, GridMap& grid) {
+        if (!grid.inBounds(p) || grid.getCellWeight(p) == new_weight) return;
+        if (new_weight != 1) throw std::invalid_argument("htrip: the GPU index supports cell weight 1 only");
+        grid.setCellWeight(p, new_weight);
+    }
+    // src/minplus_oracle.cpp:364-425, with the same rule for the weights.
+    void onTileWeightsChanged(int tx, int ty, const std::vector<std::pair<Point, dist_t>>& cell_weights,
+                              GridMap& grid) {
+        if (tx < 0 || tx >= num_leaf_tiles_x || ty < 0 || ty >= num_leaf_tiles_y) return;
+        for (const auto& pw : cell_weights)
+            if (grid.inBounds(pw.first) && pw.second != 1)
+                throw std::invalid_argument("htrip: the GPU index supports cell weight 1 only");
+        for (const auto& pw : cell_weights) grid.setCellWeight(pw.first, pw.second);
+    }
+    // src/minplus_oracle.cpp:507-527: the listed leaves' cells may have
+    // changed in grid.  Cells that became impassable are blocked first (their
+    // leaves and ancestors are rebuilt), then cells that became passable are
+    // opened (the incremental reveal); both are exact for any change, where
+    // the reference's upward propagation assumes decreases only.
+    void onBatchLeavesChanged(const std::vector<int>& leaf_indices, const GridMap& grid) {
+        if (!map_ || leaf_indices.empty()) return;
+        const int b = config.tile_size_b, nleaves = num_leaf_tiles_x * num_leaf_tiles_y;
+        std::vector<int32_t> open_rc, block_rc;
+        for (int li : leaf_indices) {
+            if (li < 0 || li >= nleaves) continue;
+            const int r0 = (li / num_leaf_tiles_x) * b, c0 = (li % num_leaf_tiles_x) * b;
+            for (int r = r0; r < std::min(r0 + b, grid.rows); ++r)
+                for (int c = c0; c < std::min(c0 + b, grid.cols); ++c) {
+                    auto& to = grid.isPassable(r, c) ? open_rc : block_rc;
+                    to.push_back(r);
+                    to.push_back(c);
+                }
+        }
+        if (!block_rc.empty()) detail::check(htg_block_cells(map_.get(), block_rc.data(), int(block_rc.size() / 2)));
+        if (!open_rc.empty()) detail::check(htg_open_cells(map_.get(), open_rc.data(), int(open_rc.size() / 2)));
+        ++update_counter;
+        epoch++;
     }
 
     // src/minplus_oracle.cpp:247-302.  The cell must already be blocked in

@@ -1,8 +1,8 @@
 // CPU helper for tests/test_maps.py: the drop-in htrip::GridMap's
 // generateProceduralRooms(rows, cols, room, seed) and, when coverage > 0,
 // initExplorationSnapshot({sr, sc}, coverage, island, seed2).  Writes
-// "ok rows cols\n", then the ground-truth codes and the observed codes as raw
-// bytes to stdout.
+// "ok rows cols\n", then the ground-truth codes, the observed codes and the
+// observed grid's getClearance tiers as raw bytes to stdout.
 #include <cstdio>
 #include <cstdlib>
 
@@ -20,5 +20,7 @@ int main(int argc, char** argv) {
     std::printf("%d %d %d\n", ok ? 1 : 0, g.rows, g.cols);
     for (auto v : g.ground_truth) std::putchar(int(v));
     for (auto v : g.observed) std::putchar(int(v));
+    for (int r = 0; r < g.rows; ++r)
+        for (int c = 0; c < g.cols; ++c) std::putchar(int(g.getClearance(r, c)));
     return 0;
 }

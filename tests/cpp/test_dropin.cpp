@@ -377,6 +377,30 @@ int main() {
             for (size_t j = 0; j < vps.size(); ++j) ok = ok && o.queryDistance(vps[i], vps[j], g) == m[i * vps.size() + j];
         expect(ok, "suite 15 queryDistance == matrix for 100 pairs");
     }
+    {  // onBatchLeavesChanged: cells of two leaves opened and blocked in the grid (:507-527);
+       // unit weights accepted, other weights refused with the grid unchanged
+        GridMap g;
+        g.generateProceduralRooms(64, 64, 16, 77);
+        HierarchyConfig cfg;
+        cfg.tile_size_b = 16;
+        MinPlusOracle o(cfg);
+        o.build(g);
+        std::vector<Point> vps = {{3, 3}, {20, 20}, {40, 8}, {60, 60}, {17, 40}, {33, 33}};
+        bool ok = matches_bfs(o, g, vps);
+        for (int r = 0; r < 16; ++r) g.openDoorway({r, 15}), g.openDoorway({r, 16});  // a wide passage, leaves 0 and 1
+        for (int c = 18; c < 30; ++c) g.blockCorridor({20, c});                         // a wall across leaf 5
+        o.onBatchLeavesChanged({0, 1, 5}, g);
+        ok = ok && matches_bfs(o, g, vps);
+        o.onCellWeightChanged({3, 3}, 1, g);
+        bool refused = false;
+        try {
+            o.onCellWeightChanged({3, 4}, 5, g);
+        } catch (const std::invalid_argument&) {
+            refused = true;
+        }
+        ok = ok && refused && g.getCellWeight({3, 4}) == 1 && matches_bfs(o, g, vps);
+        expect(ok, "onBatchLeavesChanged open + block == BFS; weight 5 refused");
+    }
     {  // const queries from 4 threads, each with its own HTripQueryScratch, on one oracle
         GridMap g(96, 96, CellState::Free);
         std::mt19937 rng(11);

@@ -7,7 +7,8 @@ grids.  The drop-in's output is checked against the compiled reference
 definitions fix: room interiors free, every room reachable through its
 doorways, a snapshot that reveals a connected set of truth-free cells around
 its start, no more than its budget, and obstacles only next to revealed
-cells."""
+cells.  GridMap::getClearance (grid_map.hpp:174-212) is checked on the same
+grids against its definition and the reference."""
 import os
 import subprocess
 from collections import deque
@@ -45,7 +46,8 @@ def gen(tmp_path_factory):
         head, _, body = out.partition(b"\n")
         ok, r, c = (int(v) for v in head.split())
         cells = np.frombuffer(body, dtype=np.uint8)
-        return bool(ok), cells[:r * c].reshape(r, c), cells[r * c:].reshape(r, c)
+        return bool(ok), cells[:r * c].reshape(r, c), cells[r * c:2 * r * c].reshape(r, c), \
+            cells[2 * r * c:].reshape(r, c)
 
     return run
 
@@ -73,7 +75,7 @@ def components(mask):
 @pytest.mark.parametrize("case", CASES, ids=[f"{c[0]}x{c[1]}-room{c[2]}-cov{c[5]}" for c in CASES])
 def test_world_matches_the_reference_and_its_definition(case, gen):
     rows, cols, room, seed, start, coverage, island, seed2 = case
-    ok, truth, obs = gen(*case)
+    ok, truth, obs, clr = gen(*case)
     rs = max(room, 4)
     ny, nx = rows // rs, cols // rs
     # every room interior is free, and the rooms form one component through the doorways
@@ -107,8 +109,18 @@ def test_world_matches_the_reference_and_its_definition(case, gen):
                 assert seen.sum() == (lab == lab[sr, sc]).sum()
     else:
         assert ok and (obs == truth).all()
+    # getClearance: 0 off the passable cells, else the Chebyshev ring (1 or 2)
+    # of the nearest obstacle or grid edge, 3 beyond
+    pad = np.pad(obs == OBST, 2, constant_values=True)
+    for r, c in np.argwhere(np.ones_like(obs, bool)):
+        want = 0
+        if obs[r, c] == FREE:
+            win = pad[r:r + 5, c:c + 5]
+            want = 1 if win[1:4, 1:4].any() else 2 if win.any() else 3
+        assert clr[r, c] == want
     if orc.available("reference"):
         rok, rtruth, robs = orc.reference_world(rows, cols, room, seed, start, coverage, island, seed2)
         assert rok == ok
         np.testing.assert_array_equal(truth, rtruth)
         np.testing.assert_array_equal(obs, robs)
+        np.testing.assert_array_equal(clr, orc.reference_frontiers(obs)[0])
