@@ -1,7 +1,9 @@
 """Frontier viewpoints (htrip_amd.frontier) against the reference's
 FrontierManager and GridMap::getClearance (src/frontier.cpp,
-include/htrip/grid_map.hpp:174-203), plus hand-worked cases."""
+include/htrip/grid_map.hpp:174-203), plus hand-worked cases; the C++
+drop-in's FrontierManager and BfsOracle against both."""
 import os
+import subprocess
 
 import numpy as np
 import pytest
@@ -10,7 +12,8 @@ from htrip_amd import frontier, grids
 from oracle import oracle as orc
 
 U, F, X = 0, 1, 2
-MAPDIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "maps")
+HERE = os.path.dirname(os.path.abspath(__file__))
+MAPDIR = os.path.join(HERE, "golden", "maps")
 
 
 def test_clearance_by_hand():
@@ -73,3 +76,55 @@ def test_viewpoints_for_queries():
     vps = frontier.frontier_viewpoints(obs, robots, 20)
     assert len(vps) >= 3 and (vps[:2] == robots).all()
     assert (obs[vps[:, 0], vps[:, 1]] == F).all()
+
+
+@pytest.fixture(scope="module")
+def cpp_frontier_bfs(tmp_path_factory):
+    """The C++ drop-in FrontierManager and BfsOracle (htrip_amd/include/htrip/
+    frontier.hpp, bfs_oracle.hpp) through tests/cpp/frontier_bfs.cpp."""
+    d = tmp_path_factory.mktemp("frontier_cpp")
+    exe = str(d / "frontier_bfs")
+    subprocess.run(["g++", "-std=c++20", "-O2", "-I", os.path.join(HERE, "..", "htrip_amd", "include"),
+                    os.path.join(HERE, "cpp", "frontier_bfs.cpp"), "-o", exe], check=True)
+
+    def run(obs, vps, min_size, max_size):
+        R, Cn = obs.shape
+        path = str(d / "in.bin")
+        with open(path, "wb") as f:
+            f.write(f"{R} {Cn} {len(vps)} {min_size} {max_size}\n".encode())
+            f.write(np.ascontiguousarray(obs, dtype=np.uint8).tobytes())
+            f.write(" ".join(f"{r} {c}" for r, c in vps).encode())
+        out = subprocess.run([exe, path], capture_output=True, check=True).stdout
+        front = np.frombuffer(out[:R * Cn], dtype=np.uint8).reshape(R, Cn).astype(bool)
+        lines = out[R * Cn:].decode().splitlines()
+        n = int(lines[0])
+        clusters = []
+        for ln in lines[1:1 + n]:
+            v = [int(x) for x in ln.split()]
+            clusters.append((v[0], (v[1], v[2]), [(v[4 + 2 * j], v[5 + 2 * j]) for j in range(v[3])]))
+        m = np.array([int(x) for x in lines[1 + n].split()], dtype=np.int64).reshape(len(vps), len(vps))
+        return front, clusters, m
+
+    return run
+
+
+@pytest.mark.parametrize("name,obs", list(observed_cases()), ids=[n for n, _ in observed_cases()])
+@pytest.mark.parametrize("min_size,max_size", [(2, 16), (1, 0), (3, 5)])
+def test_cpp_dropin_frontiers_and_bfs(name, obs, min_size, max_size, cpp_frontier_bfs):
+    """The C++ drop-in gives the Python restatement's frontier cells and
+    clusters (ids, representatives, cell order) and, when oracle/_ref is built,
+    the reference's; its BfsOracle gives the oracle's BFS matrix, with
+    impassable, off-grid and repeated viewpoints."""
+    rng = np.random.default_rng(len(name) * 7 + min_size)
+    R, Cn = obs.shape
+    vps = [tuple(int(v) for v in p) for p in np.argwhere(obs == F)[rng.integers(0, int((obs == F).sum()), 10)]]
+    vps += [(0, 0), (R - 1, Cn - 1), vps[0], (int(np.argwhere(obs == 2)[0][0]), int(np.argwhere(obs == 2)[0][1]))]
+    front, clusters, m = cpp_frontier_bfs(obs, vps, min_size, max_size)
+    np.testing.assert_array_equal(front, frontier.frontier_cells(obs))
+    got = frontier.detect_and_cluster(obs, min_size, max_size)
+    assert clusters == [(c.id, c.representative, c.cells) for c in got]
+    np.testing.assert_array_equal(m, orc.grid_bfs(obs, np.array(vps, dtype=np.int32)))
+    if orc.available("reference"):
+        _, rfront, ref = orc.reference_frontiers(obs, min_size, max_size)
+        np.testing.assert_array_equal(front, rfront)
+        assert clusters == ref

@@ -433,6 +433,25 @@ def test_cpp_dropin_program(gpu_ctx):
     assert res.returncode == 0, res.stdout + res.stderr
 
 
+def test_reference_exactness_program_on_the_dropin(gpu_ctx):
+    """The reference's own exactness program (apps/test_exactness.cpp: suites 1
+    to 16 over its procedural worlds, its frontier viewpoints and its BfsOracle),
+    unchanged, compiled against the drop-in headers and linked with the GPU
+    library (oracle/_ref/test_exactness_dropin, built by oracle/Makefile where
+    the reference sources are present)."""
+    import os
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(__file__), "..", "oracle", "_ref", "test_exactness_dropin")
+    if not os.path.exists(exe):
+        pytest.skip("oracle/_ref/test_exactness_dropin not built (reference sources absent at build time)")
+    print("reference exactness program on the drop-in", flush=True)
+    res = subprocess.run([exe], capture_output=True, text=True, timeout=280)
+    print(res.stdout[-3000:])
+    assert res.returncode == 0, res.stdout[-3000:] + res.stderr[-2000:]
+    assert "ALL EXACTNESS VERIFICATION SUITES PASSED" in res.stdout
+
+
 @pytest.mark.parametrize("kind,n,b,g,seed", [("maze", 128, 16, 2, 31), ("street", 96, 8, 2, 32),
                                               ("street", 64, 8, 4, 33)])
 def test_graph_build_and_tuning_keep_tables(gpu_ctx, kind, n, b, g, seed):
