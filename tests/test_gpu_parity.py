@@ -452,6 +452,25 @@ def test_reference_exactness_program_on_the_dropin(gpu_ctx):
     assert "ALL EXACTNESS VERIFICATION SUITES PASSED" in res.stdout
 
 
+@pytest.mark.parametrize("args", [[], ["--seeds", "4", "--size", "128", "--tile", "8", "--steps", "16"],
+                                  ["--seeds", "3", "--size", "64", "--tile", "8", "--group", "4"]])
+def test_reference_property_program_on_the_dropin(gpu_ctx, args):
+    """The reference's property-based differential program
+    (apps/test_property_based.cpp: incremental updates against rebuilds and
+    BFS over seeded worlds), unchanged, against the drop-in and the GPU library."""
+    import os
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(__file__), "..", "oracle", "_ref", "test_property_based_dropin")
+    if not os.path.exists(exe):
+        pytest.skip("oracle/_ref/test_property_based_dropin not built (reference sources absent at build time)")
+    print(f"reference property program on the drop-in {args}", flush=True)
+    res = subprocess.run([exe] + args, capture_output=True, text=True, timeout=280)
+    print(res.stdout[-2000:])
+    assert res.returncode == 0, res.stdout[-3000:] + res.stderr[-2000:]
+    assert "ALL PROPERTY-BASED CHECKS PASSED" in res.stdout
+
+
 @pytest.mark.parametrize("kind,n,b,g,seed", [("maze", 128, 16, 2, 31), ("street", 96, 8, 2, 32),
                                               ("street", 64, 8, 4, 33)])
 def test_graph_build_and_tuning_keep_tables(gpu_ctx, kind, n, b, g, seed):
