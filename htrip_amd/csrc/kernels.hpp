@@ -6,6 +6,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <vector>
 
 namespace htg {
 
@@ -101,14 +102,19 @@ void launch_compact_flags_live(int nV, const int* node_list, int n_list, const i
                                int ng8, uint8_t* flags, size_t plane, int* pivots, int* bpiv, int piv_stride,
                                int* counts, hipStream_t st);
 // Pre-relaxation of the cover rows C through the uncovered endpoints B
-// (counts as left by launch_compact_flags_live), then the C x C second hop.
+// (counts as left by launch_compact_flags_live), then the C x C second hop,
+// over work units planned on the host: plan_cover_units appends, for listed
+// nodes with cover counts c[i], uncovered-endpoint counts b[i] and live octet
+// counts l8[i], the units (slot, row tile | column tile << 16, B begin, B end)
+// of one hop and returns the rows per thread (1, 2 or 4) to launch them with.
+int plan_cover_units(bool hop2, int n_list, const int* c, const int* b, const int* l8, std::vector<int4>& units);
 bool launch_cover_relax(int nV, int ld, const int* node_list, int n_list, const int* counts, const int* cpiv,
-                        const int* bpiv, int piv_stride, const int* live8_cnt, const int* live8_list, int ng8,
-                        int max_live8, int max_c, int max_b, uint16_t* DU, size_t node_stride, hipStream_t st,
-                        KTimer kt = {});
-bool launch_cover_pairs(int nV, int ld, const int* node_list, int n_list, const int* counts, const int* cpiv,
-                        const int* bpiv, int piv_stride, int max_c, int max_b, uint16_t* DU, size_t node_stride,
+                        const int* bpiv, int piv_stride, const int4* units, int n_units, int rpt,
+                        const int* live8_cnt, const int* live8_list, int ng8, uint16_t* DU, size_t node_stride,
                         hipStream_t st, KTimer kt = {});
+bool launch_cover_pairs(int nV, int ld, const int* node_list, int n_list, const int* counts, const int* cpiv,
+                        const int* bpiv, int piv_stride, const int4* units, int n_units, int rpt, uint16_t* DU,
+                        size_t node_stride, hipStream_t st, KTimer kt = {});
 void launch_project_su_live(int nV, int ld, int ne, const int* vu_ext, const int* node_list, const int* counts,
                             int n_list, const int* live8_cnt, const int* live8_list, int ng8, int max_live8,
                             const uint16_t* DU, size_t node_stride, uint16_t* SU, int* changed, hipStream_t st);

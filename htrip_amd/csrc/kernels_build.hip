@@ -701,36 +701,50 @@ void launch_compact_flags_live(int nV, const int* node_list, int n_list, const i
 // B columns and writes C columns.
 //
 // Both hops are one rectangular min-plus product, tiled like the closure
-// update (with B split over grid z when the tiles alone are too few to fill
-// the chip: each part then folds its minima into D with compare-and-swap
-// minima, which lose no smaller value whatever the order): a workgroup owns 16*RPT C rows x 16 column octets (thread = RPT rows
-// x 8 columns, two u16 lanes per v_pk op) and streams the inner set B through
-// LDS in chunks of 32: dup'd A[b][row] = D[row][b] and X[b][octet] (hop 1:
-// the B row's live octet; hop 2: D[c2][b] for 8 consecutive cover entries
-// c2).  Staging is software-pipelined: the gathers of chunk i+1 are issued
-// before chunk i is computed, with their B indices loaded one chunk earlier
-// still.  Waves whose rows are all past the cover size skip the arithmetic.
-// The host picks RPT so that the few large nodes of the upper levels still
-// fill the chip.  Grid: x = row tile * column tiles + column tile, y = slot.
+// update: a work unit owns 16*RPT C rows x 16 column octets of one node
+// (thread = RPT rows x 8 columns, two u16 lanes per v_pk op) and a range of
+// the inner set B, which it streams through LDS in chunks of 32: dup'd
+// A[b][row] = D[row][b] and X[b][octet] (hop 1: the B row's live octet;
+// hop 2: D[c2][b] for 8 consecutive cover entries c2).  Staging is
+// software-pipelined: the gathers of chunk i+1 are issued before chunk i is
+// computed, with their B indices loaded one chunk earlier still.  Waves whose
+// rows are all past the cover size skip the arithmetic.  The host lists the
+// units from the level's C and B counts, so no workgroup is spent on a tile
+// past a node's cover, and it splits a node's B into parts of a few chunks:
+// a unit's duration is a chain of dependent chunk stages, so one node with a
+// long B would otherwise hold the launch for all of them.  Units over a part
+// of B start from INF and fold their minima into D with compare-and-swap
+// minima, which lose no smaller value whatever the order.
 constexpr int kCoverOct = 16, kCoverK = 32;
 
-// D[w] = min(D[w], v) on a packed pair / one u16 of a 32-bit word, atomically
-// against other workgroups doing the same: values only decrease, so the
-// compare-and-swap loop ends.
-__device__ __forceinline__ void atomic_min_pair(uint32_t* w, uint32_t v) {
-    uint32_t old = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// D[w] = min(D[w], v) for N packed words at once (null address: no word),
+// atomically against other units doing the same: every word is read once,
+// all compare-and-swaps of a round are issued before any result is examined,
+// and only words that another unit lowered in between go round again.  Values
+// only decrease, so the rounds end.
+template <int N>
+__device__ __forceinline__ void fold_min_words(uint32_t* (&wa)[N], const uint32_t (&wv)[N]) {
+    uint32_t old[N], nv[N], prev[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) old[i] = wa[i] ? __hip_atomic_load(wa[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
     for (;;) {
-        const uint32_t nv = min2(old, v);
-        if (nv == old) return;
-        const uint32_t prev = atomicCAS(w, old, nv);
-        if (prev == old) return;
-        old = prev;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            nv[i] = min2(old[i], wv[i]);
+            if (nv[i] == old[i]) wa[i] = nullptr;  // nothing to lower
+        }
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+            if (wa[i]) prev[i] = atomicCAS(wa[i], old[i], nv[i]);
+        bool again = false;
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+            if (wa[i]) {
+                if (prev[i] == old[i]) wa[i] = nullptr;
+                else old[i] = prev[i], again = true;
+            }
+        if (!again) return;
     }
-}
-__device__ __forceinline__ void atomic_min_u16(uint16_t* a, uint32_t v) {
-    const uintptr_t u = reinterpret_cast<uintptr_t>(a);
-    uint32_t* w = reinterpret_cast<uint32_t*>(u & ~uintptr_t(3));
-    atomic_min_pair(w, (u & 2) ? (v << 16) | 0xffffu : v | 0xffff0000u);
 }
 
 template <int RPT> struct LdsRow;
@@ -750,8 +764,8 @@ template <> struct LdsRow<4> {
 template <bool HOP2, int RPT>
 __global__ __launch_bounds__(256) void k_cover_product(int nV, int ld, const int* __restrict__ node_list, int n_list,
                                                        const int* __restrict__ counts, const int* __restrict__ cpiv,
-                                                       const int* __restrict__ bpiv, int piv_stride, int col_tiles,
-                                                       int ksplit, const int* __restrict__ live8_cnt,
+                                                       const int* __restrict__ bpiv, int piv_stride,
+                                                       const int4* __restrict__ units, const int* __restrict__ live8_cnt,
                                                        const int* __restrict__ live8_list, int ng8,
                                                        uint16_t* __restrict__ DU, size_t node_stride) {
     constexpr int R = 16 * RPT;               // rows per tile
@@ -761,11 +775,11 @@ __global__ __launch_bounds__(256) void k_cover_product(int nV, int ld, const int
     __shared__ RowT A4[kCoverK][16];  // dup16(D[row][b]): the RPT rows of a thread
     __shared__ uint4 X4[kCoverK][kCoverOct];
     __shared__ int rowv[R], colv[kCoverOct * 8];
-    const int slot = blockIdx.y;
+    const int4 un = units[blockIdx.x];  // slot, row tile | column tile << 16, B range
+    const int slot = un.x, I = un.y & 0xffff, J = un.y >> 16, kb = un.z, ke = un.w;
     const int nc = counts[slot], nb = counts[n_list + slot];
-    const int I = blockIdx.x / col_tiles, J = blockIdx.x % col_tiles;
-    const int kb = int((long long)nb * blockIdx.z / ksplit), ke = int((long long)nb * (blockIdx.z + 1) / ksplit);
-    if (kb >= ke || I * R >= nc) return;  // also nb == 0
+    if (kb >= ke || ke > nb || I * R >= nc) return;
+    const bool part = kb > 0 || ke < nb;  // other units fold the rest of B into the same entries
     const int node = node_list[slot];
     const int n_cols = HOP2 ? nc : 8 * live8_cnt[node];  // columns this node has
     if (J * kCoverOct * 8 >= n_cols) return;
@@ -798,8 +812,8 @@ __global__ __launch_bounds__(256) void k_cover_product(int nV, int ld, const int
 #pragma unroll
         for (int h = 0; h < 4; ++h) {
             const int c0 = colv[tx * 8 + 2 * h], c1 = colv[tx * 8 + 2 * h + 1];
-            const uint32_t lo = row >= 0 && c0 >= 0 ? D[size_t(row) * ld + c0] : kInf;
-            const uint32_t hi = row >= 0 && c1 >= 0 ? D[size_t(row) * ld + c1] : kInf;
+            const uint32_t lo = !part && row >= 0 && c0 >= 0 ? D[size_t(row) * ld + c0] : kInf;
+            const uint32_t hi = !part && row >= 0 && c1 >= 0 ? D[size_t(row) * ld + c1] : kInf;
             acc[r][h] = lo | (hi << 16);
         }
     }
@@ -872,22 +886,36 @@ __global__ __launch_bounds__(256) void k_cover_product(int nV, int ld, const int
             }
         }
     }
-    if (ksplit > 1) {  // other parts of B write the same entries
+    if (part) {  // one batch of up to 8 words per row
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
             const int row = rowv[ty * RPT + r];
             if (row < 0) continue;
             uint16_t* Dr = D + size_t(row) * ld;
+            uint32_t* wa[8];
+            uint32_t wv[8];
 #pragma unroll
             for (int h = 0; h < 4; ++h) {
                 const int c0 = colv[tx * 8 + 2 * h], c1 = colv[tx * 8 + 2 * h + 1];
-                if (c0 >= 0 && c1 == c0 + 1 && (c0 & 1) == 0) {
-                    atomic_min_pair(reinterpret_cast<uint32_t*>(Dr + c0), acc[r][h]);
+                wa[2 * h] = wa[2 * h + 1] = nullptr;
+                wv[2 * h] = wv[2 * h + 1] = 0xffffffffu;
+                if (c0 >= 0 && c1 == c0 + 1 && (c0 & 1) == 0) {  // an aligned pair: one word
+                    wa[2 * h] = reinterpret_cast<uint32_t*>(Dr + c0);
+                    wv[2 * h] = acc[r][h];
                     continue;
                 }
-                if (c0 >= 0) atomic_min_u16(Dr + c0, acc[r][h] & 0xffffu);
-                if (c1 >= 0) atomic_min_u16(Dr + c1, acc[r][h] >> 16);
+                // single entries: the other half of the word is min'd with 0xffff
+                const uint32_t v2[2] = {acc[r][h] & 0xffffu, acc[r][h] >> 16};
+                const int c2[2] = {c0, c1};
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    if (c2[e] < 0) continue;
+                    const uintptr_t u = reinterpret_cast<uintptr_t>(Dr + c2[e]);
+                    wa[2 * h + e] = reinterpret_cast<uint32_t*>(u & ~uintptr_t(3));
+                    wv[2 * h + e] = (u & 2) ? (v2[e] << 16) | 0xffffu : v2[e] | 0xffff0000u;
+                }
             }
+            fold_min_words(wa, wv);
         }
         return;
     }
@@ -908,56 +936,74 @@ __global__ __launch_bounds__(256) void k_cover_product(int nV, int ld, const int
     }
 }
 
-// Rows per thread: the tallest tile that still gives 512 or more workgroups
-// (update work that leaves room for query batches on other streams).
-static int cover_rpt(int max_c, int col_tiles, int n_list) {
-    for (int rpt = 4; rpt > 1; rpt /= 2) {
-        const long long blocks = (long long)((max_c + 16 * rpt - 1) / (16 * rpt)) * col_tiles * n_list;
-        if (blocks >= 512) return rpt;
+// Shortest B part, and the units a level's work is spread over: a launch
+// lasts as long as its longest unit, while every unit beyond the few hundred
+// that fill the chip holds compute units that query batches on other streams
+// are waiting for.
+constexpr int kCoverPart = 2 * kCoverK;
+constexpr long long kCoverUnits = 512;
+
+int plan_cover_units(bool hop2, int n_list, const int* c, const int* b, const int* l8, std::vector<int4>& units) {
+    auto row_tiles = [&](int i, int rpt) { return (c[i] + 16 * rpt - 1) / (16 * rpt); };
+    auto col_tiles = [&](int i) {
+        return hop2 ? (c[i] + kCoverOct * 8 - 1) / (kCoverOct * 8) : (l8[i] + kCoverOct - 1) / kCoverOct;
+    };
+    auto tiles = [&](int i, int rpt) -> long long {
+        return c[i] > 0 && b[i] > 0 ? (long long)row_tiles(i, rpt) * col_tiles(i) : 0;
+    };
+    // rows per thread: the tallest tile that still gives 512 or more tiles
+    int rpt = 1;
+    for (int r = 4; r > 1; r /= 2) {
+        long long t = 0;
+        for (int i = 0; i < n_list; ++i) t += tiles(i, r);
+        if (t >= 512) {
+            rpt = r;
+            break;
+        }
     }
-    return 1;
+    // B parts: a node's B is split into parts of about the level's work per
+    // unit at kCoverUnits units (B entries x tiles), at least kCoverPart long,
+    // so a long B no longer sets the launch's duration while short ones stay
+    // whole; that makes at most kCoverUnits units more than there are tiles
+    long long work = 0;
+    for (int i = 0; i < n_list; ++i) work += tiles(i, rpt) * b[i];
+    const long long len = std::max<long long>(kCoverPart, (work + kCoverUnits - 1) / kCoverUnits);
+    auto parts = [&](int i) { return int(std::min<long long>(64, (b[i] + len - 1) / len)); };
+    for (int i = 0; i < n_list; ++i) {
+        if (tiles(i, rpt) == 0) continue;
+        const int ks = parts(i), rt = row_tiles(i, rpt), ct = col_tiles(i);
+        for (int z = 0; z < ks; ++z) {
+            const int kb = int((long long)b[i] * z / ks), ke = int((long long)b[i] * (z + 1) / ks);
+            for (int I = 0; I < rt; ++I)
+                for (int J = 0; J < ct; ++J) units.push_back(make_int4(i, I | (J << 16), kb, ke));
+        }
+    }
+    return rpt;
 }
 
-// Parts of B (grid z) for a grid of `blocks` tiles: enough for 512
-// workgroups, each part at least two chunks of B.
-static int cover_ksplit(long long blocks, int max_b) {
-    const long long want = (512 + blocks - 1) / blocks;
-    const long long most = std::max(1, max_b / (2 * kCoverK));
-    return int(std::max(1ll, std::min({want, most, 64ll})));
-}
-
-#define HTG_COVER_LAUNCH(HOP2, RPT, CT, L8C, L8L, NG8)                                                             \
-    do {                                                                                                           \
-        const long long tiles = (long long)((max_c + 16 * (RPT)-1) / (16 * (RPT))) * (CT);                          \
-        const int ks = cover_ksplit(tiles * n_list, max_b);                                                        \
-        HTG_LAUNCH_T((k_cover_product<HOP2, RPT>), dim3(unsigned(tiles), unsigned(n_list), unsigned(ks)),          \
-                     dim3(256), 0, st, kt, nV, ld, node_list, n_list, counts, cpiv, bpiv, piv_stride, CT, ks, L8C, \
-                     L8L, NG8, DU, node_stride);                                                                   \
-    } while (0)
+#define HTG_COVER_LAUNCH(HOP2, RPT, L8C, L8L, NG8)                                                                 \
+    HTG_LAUNCH_T((k_cover_product<HOP2, RPT>), dim3(unsigned(n_units)), dim3(256), 0, st, kt, nV, ld, node_list,    \
+                 n_list, counts, cpiv, bpiv, piv_stride, units, L8C, L8L, NG8, DU, node_stride)
 
 bool launch_cover_relax(int nV, int ld, const int* node_list, int n_list, const int* counts, const int* cpiv,
-                        const int* bpiv, int piv_stride, const int* live8_cnt, const int* live8_list, int ng8,
-                        int max_live8, int max_c, int max_b, uint16_t* DU, size_t node_stride, hipStream_t st,
-                        KTimer kt) {
-    if (n_list == 0 || max_live8 == 0 || max_c == 0) return false;
-    const int ct = (max_live8 + kCoverOct - 1) / kCoverOct;
-    const int rpt = cover_rpt(max_c, ct, n_list);
-    if (rpt == 4) HTG_COVER_LAUNCH(false, 4, ct, live8_cnt, live8_list, ng8);
-    else if (rpt == 2) HTG_COVER_LAUNCH(false, 2, ct, live8_cnt, live8_list, ng8);
-    else HTG_COVER_LAUNCH(false, 1, ct, live8_cnt, live8_list, ng8);
+                        const int* bpiv, int piv_stride, const int4* units, int n_units, int rpt,
+                        const int* live8_cnt, const int* live8_list, int ng8, uint16_t* DU, size_t node_stride,
+                        hipStream_t st, KTimer kt) {
+    if (n_units <= 0) return false;
+    if (rpt == 4) HTG_COVER_LAUNCH(false, 4, live8_cnt, live8_list, ng8);
+    else if (rpt == 2) HTG_COVER_LAUNCH(false, 2, live8_cnt, live8_list, ng8);
+    else HTG_COVER_LAUNCH(false, 1, live8_cnt, live8_list, ng8);
     return true;
 }
 
 bool launch_cover_pairs(int nV, int ld, const int* node_list, int n_list, const int* counts, const int* cpiv,
-                        const int* bpiv, int piv_stride, int max_c, int max_b, uint16_t* DU, size_t node_stride,
-                        hipStream_t st, KTimer kt) {
-    if (n_list == 0 || max_c == 0) return false;
-    const int ct = (max_c + kCoverOct * 8 - 1) / (kCoverOct * 8);
-    const int rpt = cover_rpt(max_c, ct, n_list);
+                        const int* bpiv, int piv_stride, const int4* units, int n_units, int rpt, uint16_t* DU,
+                        size_t node_stride, hipStream_t st, KTimer kt) {
+    if (n_units <= 0) return false;
     const int* nul = nullptr;
-    if (rpt == 4) HTG_COVER_LAUNCH(true, 4, ct, nul, nul, 0);
-    else if (rpt == 2) HTG_COVER_LAUNCH(true, 2, ct, nul, nul, 0);
-    else HTG_COVER_LAUNCH(true, 1, ct, nul, nul, 0);
+    if (rpt == 4) HTG_COVER_LAUNCH(true, 4, nul, nul, 0);
+    else if (rpt == 2) HTG_COVER_LAUNCH(true, 2, nul, nul, 0);
+    else HTG_COVER_LAUNCH(true, 1, nul, nul, 0);
     return true;
 }
 #undef HTG_COVER_LAUNCH

@@ -553,7 +553,8 @@ int Map::reserve_update_ws() {
         (rc = ws_leafl.ensure(kUpdateListReserve / 4)) || (rc = ws_leafl_h.ensure(kUpdateListReserve / 4)) ||
         (rc = ws_leafP.ensure(size_t(leafP))) ||
         (rc = ws_chg.ensure(nchg * 4 + 16)) || (rc = ws_chg_h.ensure(nchg * 4 + 16)) ||
-        (rc = ws_counts_h.ensure(2 * nn_max * 4 + 16)))
+        (rc = ws_counts_h.ensure(2 * nn_max * 4 + 16)) || (rc = ws_cov.ensure(kCoverUnitReserve)) ||
+        (rc = ws_cov_h.ensure(kCoverUnitReserve)))
         return rc;
     for (int lvl = 1; lvl <= H; ++lvl)
         if ((rc = ensure_build_ws(lvl))) return rc;
@@ -1475,16 +1476,31 @@ int Map::open_cells(const int32_t* rcs, int n) {
                     by_p += (2 * c * bb + c * c) * 2;
                 }
             }
+            // work units of both hops from the counts just read, in one upload
+            // (the level's sync above has drained the previous level's upload)
+            std::vector<int> l8(static_cast<size_t>(na));
+            for (int i = 0; i < na; ++i) l8[size_t(i)] = L.h_live8_cnt[cl[size_t(i)]];
+            cov_units.clear();
+            const int rpt1 = plan_cover_units(false, na, hc.data(), hc.data() + na, l8.data(), cov_units);
+            const int n1 = int(cov_units.size());
+            const int rpt2 = plan_cover_units(true, na, hc.data(), hc.data() + na, l8.data(), cov_units);
+            const int n2 = int(cov_units.size()) - n1;
+            const size_t ub = cov_units.size() * sizeof(int4);
+            if ((rc = ws_cov_h.ensure(ub)) || (rc = ws_cov.ensure(ub))) return rc;
+            std::memcpy(ws_cov_h.p, cov_units.data(), ub);
+            if ((e = hipMemcpyAsync(ws_cov.p, ws_cov_h.p, ub, hipMemcpyHostToDevice, stream)) != hipSuccess)
+                return hip_fail(e, "upload cover units");
+            const int4* units = static_cast<const int4*>(ws_cov.p);
             KTimer kt;
             int pi = prof_open(HTG_K_COVER_RELAX, by_r, upd_r, kt);
             bool la = launch_cover_relax(L.d.nV, L.d.ld, dnodes, na, cnt, static_cast<const int*>(ws_piv.p),
-                                         static_cast<const int*>(ws_bpiv.p), L.d.nV, L.d_live8_cnt, L.d_live8_list,
-                                         L.ng8, max8, mx, mxb, static_cast<uint16_t*>(L.DU.p), L.node_stride, stream,
-                                         kt);
+                                         static_cast<const int*>(ws_bpiv.p), L.d.nV, units, n1, rpt1,
+                                         L.d_live8_cnt, L.d_live8_list, L.ng8, static_cast<uint16_t*>(L.DU.p),
+                                         L.node_stride, stream, kt);
             prof_ref().close(pi, la);
             pi = prof_open(HTG_K_COVER_PAIRS, by_p, upd_p, kt);
             la = launch_cover_pairs(L.d.nV, L.d.ld, dnodes, na, cnt, static_cast<const int*>(ws_piv.p),
-                                    static_cast<const int*>(ws_bpiv.p), L.d.nV, mx, mxb,
+                                    static_cast<const int*>(ws_bpiv.p), L.d.nV, units + n1, n2, rpt2,
                                     static_cast<uint16_t*>(L.DU.p), L.node_stride, stream, kt);
             prof_ref().close(pi, la);
         }

@@ -39,6 +39,8 @@ struct DevBuf {
 
 // Packed update lists reserved per map (grown, doubling, past it).
 constexpr size_t kUpdateListReserve = size_t(1) << 20;
+// Cover-product work units reserved per map (16 bytes each; a level past it grows the buffers).
+constexpr size_t kCoverUnitReserve = size_t(1) << 17;
 
 // Grow-only pinned host buffer.
 struct HostBuf {
@@ -343,6 +345,9 @@ struct Map {
     DevBuf ws_leafl;        // update: packed leaf-phase lists (new cells, leaf lists, pivot items)
     HostBuf ws_leafl_h;
     HostBuf ws_upd_h, ws_chg_h, ws_counts_h;  // pinned staging of the lists, changed flags, pivot counts
+    DevBuf ws_cov;  // update: work units of a level's cover products
+    HostBuf ws_cov_h;
+    std::vector<int4> cov_units;
     // snapshot catch-up lists: staging of their own, since the update that
     // follows refills ws_upd_h while the catch-up's upload may still read it
     DevBuf ws_cu;
