@@ -747,7 +747,8 @@ double Map::closure_us(int lvl, const std::vector<int>& nodes, const int* cnt) c
 
 // The update of a level after its counts are known (cover C and uncovered
 // endpoints B per node, hc = C then B) against a rebuild of the same nodes.
-bool Map::rebuild_is_cheaper(int lvl, const std::vector<int>& nodes, const std::vector<int>& hc, int na) const {
+bool Map::rebuild_is_cheaper(int lvl, const std::vector<int>& nodes, const std::vector<int>& hc, int na,
+                             const std::vector<int>& pb) const {
     if (tune.update_rebuild < 0) return false;
     if (tune.update_rebuild > 0) return true;
     const LevelGeo& L = lv[size_t(lvl) - 1];
@@ -760,7 +761,6 @@ bool Map::rebuild_is_cheaper(int lvl, const std::vector<int>& nodes, const std::
         any_b = true;
     }
     const double upd = closure_us(lvl, nodes, hc.data()) + cover / kUpdatesPerUs + (any_b ? 2 * kBlockUs : 0);
-    const std::vector<int> pb = build_pivot_counts(lvl, &nodes);
     double assemble = 0;
     for (size_t i = 0; i < nodes.size(); ++i) assemble += double(L.d.nV) * L.d.ld * 2 / 4.0e6;  // ~4 TB/s
     const double reb = closure_us(lvl, nodes, pb.data()) + assemble + 3 * kBlockUs / 2;
@@ -1430,6 +1430,12 @@ int Map::open_cells(const int32_t* rcs, int n) {
         if ((rc = ws_counts_h.ensure(hc.size() * 4))) return rc;  // pinned: a pageable copy would block
         if ((e = hipMemcpyAsync(ws_counts_h.p, cnt, hc.size() * 4, hipMemcpyDeviceToHost, stream)) != hipSuccess)
             return hip_fail(e, "download counts");
+        // the rebuild estimate's pivot counts (passable seam endpoints) depend
+        // on host state only: counted while the level's kernels run
+        const auto r0 = clk::now();
+        std::vector<int> pb;
+        if (cover && tune.update_rebuild == 0) pb = build_pivot_counts(lvl, &cl);
+        upd_time[6] += std::chrono::duration<double>(clk::now() - r0).count();
         if ((e = sync()) != hipSuccess) return hip_fail(e, "update sync");
         std::memcpy(hc.data(), ws_counts_h.p, hc.size() * 4);
         long long sum = 0, bsum = 0;
@@ -1451,9 +1457,9 @@ int Map::open_cells(const int32_t* rcs, int n) {
         // passable seam endpoints) is rebuilt instead.  After any reveal every
         // table equals a fresh build of the revealed grid, so both paths give
         // the same tables, S_U and changed flags.
-        const auto r0 = clk::now();
-        const bool reb = sum > 0 && cover && rebuild_is_cheaper(lvl, cl, hc, na);
-        upd_time[6] += std::chrono::duration<double>(clk::now() - r0).count();
+        const auto r1 = clk::now();
+        const bool reb = sum > 0 && cover && rebuild_is_cheaper(lvl, cl, hc, na, pb);
+        upd_time[6] += std::chrono::duration<double>(clk::now() - r1).count();
         if (reb) {
             if ((rc = rebuild_level(lvl, dnodes, cl, chg_here))) return rc;
             this->dirty.nodes[size_t(lvl) - 1] = cl;  // every candidate's table was rewritten

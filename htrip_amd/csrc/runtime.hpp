@@ -361,9 +361,10 @@ struct Map {
     int rebuild_level(int lvl, const int* dnodes, const std::vector<int>& nodes, int* changed);
     // Update planning: estimated closure time of the listed nodes, and whether
     // rebuilding a level's candidates beats updating them (hc: cover counts,
-    // then uncovered-endpoint counts).
+    // then uncovered-endpoint counts; pb: build_pivot_counts of the same nodes).
     double closure_us(int lvl, const std::vector<int>& nodes, const int* cnt) const;
-    bool rebuild_is_cheaper(int lvl, const std::vector<int>& nodes, const std::vector<int>& hc, int na) const;
+    bool rebuild_is_cheaper(int lvl, const std::vector<int>& nodes, const std::vector<int>& hc, int na,
+                            const std::vector<int>& pb) const;
     long long rebuilt_levels = 0;  // update levels done by a rebuild (printed at HTG_DEBUG >= 2)
     int rebuild(const std::vector<int2>& closed, const std::vector<int>& leaves,
                 const std::vector<std::vector<int>>& anc);
